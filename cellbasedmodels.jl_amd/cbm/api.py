@@ -1,0 +1,452 @@
+"""ABM / Community / evolve_ over the HIP engine.
+
+Mirrors the reference's user-facing seams:
+  platform      src/platforms.jl:1-88          -> HIP()
+  neighbors     src/neighbors.jl (CBMNeighbors) -> CBMNeighbors.*
+  integrators   src/integrators.jl              -> CBMIntegrators.*
+  models        src/models/softSpheres.jl, docs/src/Aggregation.md:62-123 -> CBMModels.*
+  ABM           src/AgentStructure/agentStructure.jl:131-304
+  Community     src/CommunityStructure/communityStructure.jl:71-187
+  loadToPlatform!/bringFromPlatform!  communityStructure.jl:606-711
+  step!/evolve!  src/CommunityStructure/step.jl:96-155
+Errors from the library raise CBMError, as the reference raises `error()`.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+
+
+# ------------------------------------------------------------------ platform
+class HIP:
+    """The MI355X platform (the reference's CPU()/GPU() seam, src/platforms.jl:9-65)."""
+
+    def __repr__(self):
+        return "HIP()"
+
+
+def version():
+    return A.lib().cbm_version().decode()
+
+
+# ----------------------------------------------------------------- neighbors
+class CBMNeighbors:
+    """Neighbor algorithms of src/neighbors.jl with the reference's constructor keywords."""
+
+    class Full:
+        kind = A.FULL
+
+        def _fill(self, m):
+            pass
+
+    class CellLinked:
+        kind = A.CELL_LINKED
+
+        def __init__(self, cellEdge):
+            self.cellEdge = float(cellEdge)
+
+        def _fill(self, m):
+            m.cell_edge = self.cellEdge
+
+    class VerletDisplacement:
+        """List radius `skin` (the reference's meaning, src/neighbors.jl:385); the list is
+        rebuilt when an agent moved `rebuildDisplacement` (default skin/2, :403)."""
+        kind = A.VERLET_DISPLACEMENT
+
+        def __init__(self, skin, nMaxNeighbors, rebuildDisplacement=None):
+            self.skin = float(skin)
+            self.nMaxNeighbors = int(nMaxNeighbors)
+            self.rebuildDisplacement = rebuildDisplacement
+
+        def _fill(self, m):
+            m.skin = self.skin
+            m.max_neighbors = self.nMaxNeighbors
+            m.rebuild_disp = float(self.rebuildDisplacement) if self.rebuildDisplacement else 0.0
+
+    class VerletTime:
+        kind = A.VERLET_TIME
+
+        def __init__(self, skin, dtNeighborRecompute, nMaxNeighbors):
+            self.skin = float(skin)
+            self.dtNeighborRecompute = float(dtNeighborRecompute)
+            self.nMaxNeighbors = int(nMaxNeighbors)
+
+        def _fill(self, m):
+            m.skin = self.skin
+            m.dt_recompute = self.dtNeighborRecompute
+            m.max_neighbors = self.nMaxNeighbors
+
+    class CLVD:
+        kind = A.CLVD
+
+        def __init__(self, skin, nMaxNeighbors, cellEdge, rebuildDisplacement=None):
+            self.skin = float(skin)
+            self.nMaxNeighbors = int(nMaxNeighbors)
+            self.cellEdge = float(cellEdge)
+            self.rebuildDisplacement = rebuildDisplacement
+
+        def _fill(self, m):
+            m.skin = self.skin
+            m.max_neighbors = self.nMaxNeighbors
+            m.cell_edge = self.cellEdge
+            m.rebuild_disp = float(self.rebuildDisplacement) if self.rebuildDisplacement else 0.0
+
+
+# --------------------------------------------------------------- integrators
+class CBMIntegrators:
+    """src/integrators.jl:465 (Euler), :516 (Heun), :583 (RungeKutta4), :683 (EM)."""
+
+    class Euler:
+        kind = A.EULER
+
+    class Heun:
+        kind = A.HEUN
+
+    class RungeKutta4:
+        kind = A.RK4
+
+    class EM:
+        kind = A.EM
+
+
+# -------------------------------------------------------------------- models
+class _Law:
+    def __init__(self, law, p, walls=0.0):
+        self.law = law
+        self.p = list(p) + [0.0] * (8 - len(p))
+
+
+class CBMModels:
+    """Built-in agentODE interaction bodies (pair force s(d)*(x_i-x_j)/d)."""
+
+    @staticmethod
+    def noInteraction():
+        return _Law(A.LAW_NONE, [])
+
+    @staticmethod
+    def linearRepulsion(k, contact):
+        """F = k (contact - d) u for d < contact (softSpheres idiom, src/models/softSpheres.jl:19-36)."""
+        return _Law(A.LAW_LINEAR, [k, contact])
+
+    @staticmethod
+    def repulsionAdhesion(k, r, eps, cutoffFactor=2.4):
+        """Repulsion k(2r-d) for d<2r and a continuous adhesion well of depth eps
+        out to cutoffFactor*r (the Aggregation shape of docs/src/Aggregation.md:92-98
+        made continuous at both breakpoints)."""
+        return _Law(A.LAW_REP_ADH, [k, r, eps, cutoffFactor])
+
+    @staticmethod
+    def aggregation(rRep, fRep, rAtr, fAtr, walls=True):
+        """The Aggregation model of docs/src/Aggregation.md:62-123 (with its wall push)."""
+        return _Law(A.LAW_AGGREGATION, [rRep, fRep, rAtr, fAtr, 1.0 if walls else 0.0])
+
+
+class CBMRules:
+    """Built-in agentRule bodies (see enum cbm_rule in include/cbm_hip.h)."""
+
+    @staticmethod
+    def growthDivision(g_mean=0.01, g_spread=0.1, split=0.5, ratio=2.0):
+        return (A.RULE_GROWTH_DIVISION, [g_mean, g_spread, split, ratio])
+
+    @staticmethod
+    def parity():
+        return (A.RULE_PARITY, [0.0, 0.0, 0.0, 0.0])
+
+    @staticmethod
+    def death(rate):
+        return (A.RULE_DEATH, [rate, 0.0, 0.0, 0.0])
+
+
+# ----------------------------------------------------------------------- ABM
+class ABM:
+    """Model description: ABM(dims; agentODE, agentSDE, agentRule, neighborsAlg, agentAlg, platform).
+
+    agentODE    a CBMModels law (the interaction body); `drift` adds dt(x) = -drift*x
+    agentSDE    diagonal noise amplitude g (EM only): dt(x) = g in the SDE part
+    lagged      dt(x) reads the previous step's committed force accumulator, the
+                reference's read-old/write-new idiom (agentStructure.jl:352-453)
+    periodic    periodic box with minimum-image distances
+    """
+
+    def __init__(self, dims, agentODE=None, agentSDE=0.0, agentRule=None, drift=0.0,
+                 neighborsAlg=None, agentAlg=None, platform=None, periodic=False,
+                 lagged=False, seed=0, medium=None):
+        if dims not in (1, 2, 3):
+            raise ValueError("dims must be 1, 2 or 3")
+        self.dims = dims
+        self.agentODE = agentODE if agentODE is not None else CBMModels.noInteraction()
+        self.agentSDE = float(agentSDE)
+        self.agentRule = agentRule
+        self.drift = float(drift)
+        self.neighbors = neighborsAlg if neighborsAlg is not None else CBMNeighbors.Full()
+        self.agentAlg = agentAlg if agentAlg is not None else CBMIntegrators.Euler()
+        self.platform = platform if platform is not None else HIP()
+        if not isinstance(self.platform, HIP):
+            raise ValueError("this package implements the HIP() platform")
+        self.periodic = bool(periodic)
+        self.lagged = bool(lagged)
+        self.seed = int(seed)
+        self.medium = medium
+
+    def model_desc(self, dt):
+        m = A.ModelDesc()
+        m.dims = self.dims
+        m.integrator = self.agentAlg.kind
+        m.neighbors = self.neighbors.kind
+        self.neighbors._fill(m)
+        m.law = self.agentODE.law
+        m.periodic = 1 if self.periodic else 0
+        m.lagged = 1 if self.lagged else 0
+        m.dt = float(dt)
+        m.drift = self.drift
+        m.noise = self.agentSDE
+        m.seed = self.seed
+        if self.agentRule is not None:
+            m.rule = self.agentRule[0]
+            for q, v in enumerate(self.agentRule[1]):
+                m.rule_p[q] = v
+        if self.medium is not None:
+            for a, n in enumerate(self.medium["NMedium"]):
+                m.medium_n[a] = int(n)
+            m.medium_D = float(self.medium.get("D", 0.0))
+            m.medium_consume = float(self.medium.get("consume", 0.0))
+            m.medium_c0 = float(self.medium.get("c0", 0.0))
+        return m
+
+
+def sim_desc(abm, N, n_max, simBox, law_p=None):
+    s = A.SimDesc()
+    s.n = int(N)
+    s.n_max = int(max(n_max, N, 1))
+    box = np.asarray(simBox, dtype=np.float64).reshape(-1, 2)
+    for a in range(abm.dims):
+        s.box_lo[a] = box[a, 0]
+        s.box_hi[a] = box[a, 1]
+    p = law_p if law_p is not None else abm.agentODE.p
+    for q, v in enumerate(p[:8]):
+        s.p[q] = float(v)
+    return s
+
+
+_FIELDS = {"x": A.F_X, "y": A.F_Y, "z": A.F_Z, "id": A.F_ID, "fx": A.F_FX, "fy": A.F_FY,
+           "fz": A.F_FZ, "vol": A.F_VOL, "growth": A.F_GROWTH, "vbirth": A.F_VBIRTH, "lf": A.F_LF}
+
+
+class Context:
+    """Owns one cbm_ctx (a set of simulations on the current device)."""
+
+    def __init__(self, model, sims):
+        self.lib = A.lib()
+        arr = (A.SimDesc * len(sims))(*sims)
+        h = C.c_void_p()
+        rc = self.lib.cbm_create(C.byref(h), C.byref(model), len(sims), arr)
+        self.h = h
+        if rc != A.OK:
+            msg = self.lib.cbm_last_error(h).decode() if h else ""
+            if h:
+                self.lib.cbm_destroy(h)
+                self.h = None
+            raise A.CBMError(rc, msg)
+        self.model = model
+        self.sims = list(sims)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cbm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        A.check(rc, self.h)
+
+    def upload(self, field, values, sim=0):
+        isint = field in A.INT_FIELDS
+        v = np.ascontiguousarray(values, dtype=np.int64 if isint else np.float64)
+        self._chk(self.lib.cbm_upload(self.h, field, v.ctypes.data, v.size, sim))
+
+    def download(self, field, n, sim=0):
+        isint = field in A.INT_FIELDS
+        v = np.empty(int(n), dtype=np.int64 if isint else np.float64)
+        self._chk(self.lib.cbm_download(self.h, field, v.ctypes.data, v.size, sim))
+        return v
+
+    def set_n(self, n, sim=0):
+        self._chk(self.lib.cbm_set_n(self.h, sim, int(n)))
+
+    def neighbors(self):
+        self._chk(self.lib.cbm_neighbors(self.h))
+
+    def step(self, n=1):
+        self._chk(self.lib.cbm_step(self.h, int(n)))
+
+    def step_graph(self, n, replays):
+        self._chk(self.lib.cbm_step_graph(self.h, int(n), int(replays)))
+
+    def sync(self):
+        self._chk(self.lib.cbm_sync(self.h))
+
+    def n(self, sim=0):
+        v = C.c_int64()
+        self._chk(self.lib.cbm_get_n(self.h, sim, C.byref(v)))
+        return v.value
+
+    def id_max(self, sim=0):
+        v = C.c_int64()
+        self._chk(self.lib.cbm_get_id_max(self.h, sim, C.byref(v)))
+        return v.value
+
+    def time(self):
+        t = C.c_double()
+        s = C.c_int64()
+        self._chk(self.lib.cbm_get_t(self.h, C.byref(t), C.byref(s)))
+        return t.value, s.value
+
+    def rebuilds(self):
+        v = C.c_int64()
+        self._chk(self.lib.cbm_get_rebuilds(self.h, C.byref(v)))
+        return v.value
+
+    def neighbor_lists(self, radius, cap, sim=0):
+        n = self.n(sim)
+        counts = np.zeros(max(n, 1), dtype=np.int32)
+        lists = np.full(max(n, 1) * cap, -1, dtype=np.int64)
+        self._chk(self.lib.cbm_get_neighbor_lists(self.h, sim, float(radius), counts.ctypes.data,
+                                                  lists.ctypes.data, int(cap)))
+        return counts[:n], lists[: n * cap].reshape(n, cap)
+
+    def count_neighbors(self, radius, sim=0):
+        n = self.n(sim)
+        out = np.zeros(max(n, 1), dtype=np.int64)
+        self._chk(self.lib.cbm_count_neighbors(self.h, sim, float(radius), out.ctypes.data))
+        return out[:n]
+
+    def last_timing(self):
+        tot, force, nl = C.c_double(), C.c_double(), C.c_int64()
+        self._chk(self.lib.cbm_last_timing(self.h, C.byref(tot), C.byref(force), C.byref(nl)))
+        return tot.value, force.value, nl.value
+
+    def profile(self, nsteps):
+        """Per-kernel average device times (ms) over nsteps steps, see cbm_profile."""
+        ms = np.zeros(10)
+        nl = np.zeros(10, dtype=np.int64)
+        self._chk(self.lib.cbm_profile(self.h, int(nsteps), ms.ctypes.data, nl.ctypes.data, 10))
+        return ms, nl
+
+    def set_block(self, block):
+        self._chk(self.lib.cbm_set_block(self.h, int(block)))
+
+
+# ----------------------------------------------------------------- Community
+class Community:
+    """State of one simulation: Community(abm; N, dt, simBox, ...) of the reference.
+
+    Agent parameters are numpy arrays in the reference's slot order (``com.x``,
+    ``com.id``, ...); they live on the device between loadToPlatform_ and
+    bringFromPlatform_, exactly as in the reference.
+    """
+
+    def __init__(self, abm, N, dt, simBox, lawParameters=None, **params):
+        self.abm = abm
+        self.N = int(N)
+        self.dt = float(dt)
+        self.t = 0.0
+        self.simBox = np.asarray(simBox, dtype=np.float64).reshape(-1, 2)[: abm.dims]
+        self.law_p = lawParameters
+        self.params = {}
+        for k in ("x", "y", "z")[: abm.dims]:
+            self.params[k] = np.zeros(self.N)
+        self.params["id"] = np.arange(1, self.N + 1, dtype=np.int64)
+        if abm.lagged:
+            for k in ("fx", "fy", "fz")[: abm.dims]:
+                self.params[k] = np.zeros(self.N)
+        for k, v in params.items():
+            self[k] = v
+        self.ctx = None
+        self.pastTimes = []
+
+    def __getitem__(self, k):
+        return self.params[k]
+
+    def __setitem__(self, k, v):
+        if k not in _FIELDS:
+            raise KeyError(f"{k} is not an agent parameter of this model")
+        dt = np.int64 if _FIELDS[k] in A.INT_FIELDS else np.float64
+        arr = np.broadcast_to(np.asarray(v, dtype=dt), (self.N,)).copy()
+        self.params[k] = arr
+
+    def __getattr__(self, k):
+        p = self.__dict__.get("params")
+        if p is not None and k in p:
+            return p[k]
+        raise AttributeError(k)
+
+    def __setattr__(self, k, v):
+        if k in _FIELDS and "params" in self.__dict__:
+            self[k] = v
+        else:
+            object.__setattr__(self, k, v)
+
+
+def loadToPlatform_(com, preallocateAgents=0):
+    """communityStructure.jl:606-655: allocate on the device (capacity N + preallocateAgents),
+    upload every agent parameter and compute the neighbor structure."""
+    abm = com.abm
+    model = abm.model_desc(com.dt)
+    sim = sim_desc(abm, com.N, com.N + int(preallocateAgents), com.simBox.reshape(-1).tolist(), com.law_p)
+    com.ctx = Context(model, [sim])
+    for k, arr in com.params.items():
+        com.ctx.upload(_FIELDS[k], arr)
+    com.ctx.neighbors()
+    return com
+
+
+def bringFromPlatform_(com):
+    """communityStructure.jl:663-711: download every agent parameter (slot order, 1:N)."""
+    ctx = com.ctx
+    n = ctx.n()
+    object.__setattr__(com, "N", n)
+    com.t, _ = ctx.time()
+    for k in list(com.params):
+        com.params[k] = ctx.download(_FIELDS[k], n)
+    return com
+
+
+def computeNeighbors_(com):
+    """src/neighbors.jl:18."""
+    com.ctx.neighbors()
+
+
+def step_(com, n=1):
+    """step! (src/CommunityStructure/step.jl:96-107), n times, device-resident."""
+    com.ctx.step(n)
+    com.t, _ = com.ctx.time()
+
+
+def evolve_(com, steps, saveEach=None, preallocateAgents=0):
+    """evolve! (step.jl:124-155): loads if needed, runs `steps` steps, snapshots every
+    saveEach steps into com.pastTimes (the reference's saveRAM!)."""
+    if com.ctx is None:
+        loadToPlatform_(com, preallocateAgents)
+    if not saveEach:
+        step_(com, steps)
+    else:
+        done = 0
+        while done < steps:
+            k = min(saveEach, steps - done)
+            step_(com, k)
+            done += k
+            bringFromPlatform_(com)
+            com.pastTimes.append({"t": com.t, **{k2: v.copy() for k2, v in com.params.items()}})
+    bringFromPlatform_(com)
+    return com
+
+
+def getParameter(com, syms):
+    """communityStructure.jl getParameter over the saved snapshots."""
+    return {s: [snap[s] for snap in com.pastTimes] for s in syms}

@@ -1,0 +1,14 @@
+#!/bin/sh
+# Builds libcbm_hip.so for gfx950 in-tree (next to the Python package).
+set -e
+HERE="$(cd "$(dirname "$0")" && pwd)"
+OUT="$HERE/../cbm/lib"
+mkdir -p "$OUT"
+HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
+FLAGS="--offload-arch=gfx950 -O3 -std=c++17 -fPIC -ffp-contract=off -fno-fast-math -Xclang -target-feature -Xclang -packed-fp32-ops -Wall -Wno-unused-function -Wno-unused-variable"
+$HIPCC $FLAGS -c "$HERE/cbm_kernels.hip" -o "$OUT/cbm_kernels.o"
+$HIPCC $FLAGS -c "$HERE/cbm_api.hip" -o "$OUT/cbm_api.o"
+$HIPCC --offload-arch=gfx950 -shared -fPIC -o "$OUT/libcbm_hip.so.tmp" "$OUT/cbm_kernels.o" "$OUT/cbm_api.o"
+mv "$OUT/libcbm_hip.so.tmp" "$OUT/libcbm_hip.so"
+rm -f "$OUT/cbm_kernels.o" "$OUT/cbm_api.o"
+echo "built $OUT/libcbm_hip.so"

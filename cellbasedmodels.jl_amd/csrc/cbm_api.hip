@@ -1,0 +1,871 @@
+// cbm_api.hip -- host side of libcbm_hip.so: the C ABI declared in
+// include/cbm_hip.h, context/buffer management and the step orchestration
+// that replaces step!/update!/computeNeighbors! of the reference
+// (src/CommunityStructure/step.jl:96-107, update.jl:177-321,
+// src/neighbors.jl:18).
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include "cbm_internal.h"
+#include "cbm_device.h"
+
+
+using namespace cbm;
+
+#define CBM_VERSION "cbm_hip 0.1 (gfx950, FP64)"
+
+// ------------------------------------------------------------------ context
+struct Column {
+  void* buf[2] = {nullptr, nullptr};
+  int cidx = 0;
+  int bytes = 8;
+  void* cur() const { return buf[cidx]; }
+  void* alt() const { return buf[1 - cidx]; }
+};
+
+enum ColId {
+  C_X = 0, C_Y, C_Z, C_VOL,            // DE state group (flip together)
+  C_ID, C_SLOT, C_FX, C_FY, C_FZ, C_GR, C_VB, C_LF,
+  C_NCOLS
+};
+
+struct cbm_ctx {
+  cbm_model_desc m;
+  std::vector<cbm_sim_desc> sims;
+  int S = 1;
+  int dims = 1, nvar = 1;
+  bool growth = false, lagged = false, verlet = false, cl = false, full = false, vtime = false;
+  int nb_kind = NB_FULL;
+  int block = 256;
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  // geometry
+  std::vector<int32_t> h_block_sim, h_block_off;
+  int nblocks = 0;
+  int32_t* d_block_sim = nullptr;
+  int32_t* d_block_off = nullptr;
+  std::vector<SimDev> h_sims;
+  SimDev* d_sims = nullptr;
+  SimState* d_st = nullptr;
+  int64_t C = 0;      // rows
+  int64_t NB = 0;     // buckets
+  double list_radius = 0.0, thr = 0.0;
+
+  // per-agent columns
+  Column col[C_NCOLS];
+  bool has_col[C_NCOLS] = {};
+  double* Fnew[3] = {nullptr, nullptr, nullptr};
+  double* gr_new = nullptr;     // growth rule write-new buffers
+  double* vb_new = nullptr;
+  double* h[2][MAX_STATE] = {};
+  double* kacc[MAX_STATE] = {};
+  double* xref[3] = {nullptr, nullptr, nullptr};
+  uint8_t* dead = nullptr;
+
+  // neighbor engine
+  int32_t *key = nullptr, *count = nullptr, *cursor = nullptr, *start = nullptr, *tmp = nullptr, *cell_row = nullptr;
+  unsigned long long* scan_status = nullptr;
+  unsigned int* scan_ticket = nullptr;    // [0] ticket, [1] done, [2] epoch
+  int32_t* list = nullptr;
+  int32_t* list_n = nullptr;
+  int cap = 0;
+  int32_t* flags = nullptr;               // [0..1] rebuild flags by step parity, [2] list overflow, [3] capacity
+  int64_t* counters = nullptr;            // [0] rebuilds, [1] device step
+  bool structure_valid = false;
+
+  // staging for transfers
+  void* d_staging = nullptr;
+  int64_t staging_bytes = 0;
+
+  // time
+  int64_t step = 0;
+  double t = 0.0;
+  double t_next = 0.0;
+
+  // timing
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool profiling = false;
+  std::vector<hipEvent_t> prof_ev;       // pairs
+  std::vector<int> prof_cat;
+  size_t prof_used = 0;
+  double last_ms = 0.0;
+  int64_t last_launches = 0;
+
+  std::string err;
+};
+
+static int fail(cbm_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                        \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess)                                                        \
+      return fail((ctx), CBM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+static hipError_t dalloc(T** p, int64_t n) {
+  *p = nullptr;
+  if (n <= 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)n);
+  if (e == hipSuccess) e = hipMemset(*p, 0, sizeof(T) * (size_t)n);
+  return e;
+}
+
+static RowGeom geom(cbm_ctx* c) {
+  RowGeom g;
+  g.block_sim = c->d_block_sim;
+  g.block_off = c->d_block_off;
+  g.sims = c->d_sims;
+  g.st = c->d_st;
+  return g;
+}
+
+static void prof_begin(cbm_ctx* c, int cat) {
+  if (!c->profiling || c->prof_used + 2 > c->prof_ev.size()) return;
+  c->prof_cat[c->prof_used / 2] = cat;
+  (void)hipEventRecord(c->prof_ev[c->prof_used], c->stream);
+}
+static void prof_end(cbm_ctx* c) {
+  if (!c->profiling || c->prof_used + 2 > c->prof_ev.size()) return;
+  (void)hipEventRecord(c->prof_ev[c->prof_used + 1], c->stream);
+  c->prof_used += 2;
+}
+
+// ----------------------------------------------------------------- create
+static int validate(cbm_ctx* c) {
+  const cbm_model_desc& m = c->m;
+  if (m.dims < 1 || m.dims > 3) return fail(c, CBM_ERR_CONFIG, "dims must be 1, 2 or 3");
+  if (m.integrator < CBM_EULER || m.integrator > CBM_EM) return fail(c, CBM_ERR_CONFIG, "unknown integrator");
+  if (m.neighbors < CBM_FULL || m.neighbors > CBM_CLVD) return fail(c, CBM_ERR_CONFIG, "unknown neighbor algorithm");
+  if (m.law < CBM_LAW_NONE || m.law > CBM_LAW_AGGREGATION) return fail(c, CBM_ERR_CONFIG, "unknown or uncompiled interaction law");
+  if (!(m.dt > 0)) return fail(c, CBM_ERR_CONFIG, "dt must be positive");
+  if (m.neighbors == CBM_CELL_LINKED && !(m.cell_edge > 0)) return fail(c, CBM_ERR_CONFIG, "CellLinked needs cellEdge > 0");
+  bool verlet = m.neighbors == CBM_VERLET_DISPLACEMENT || m.neighbors == CBM_VERLET_TIME || m.neighbors == CBM_CLVD;
+  if (verlet && !(m.skin > 0)) return fail(c, CBM_ERR_CONFIG, "Verlet algorithms need skin > 0");
+  if (verlet && (m.max_neighbors < 1 || m.max_neighbors > 4096)) return fail(c, CBM_ERR_CONFIG, "nMaxNeighbors must be in [1, 4096]");
+  if (m.neighbors == CBM_VERLET_TIME && !(m.dt_recompute > 0)) return fail(c, CBM_ERR_CONFIG, "VerletTime needs dtNeighborRecompute > 0");
+  if (m.rule != CBM_RULE_NONE) return fail(c, CBM_ERR_CONFIG, "agent rules are not available in this build");
+  if (m.medium_n[0] > 0) return fail(c, CBM_ERR_CONFIG, "medium is not available in this build");
+  for (int s = 0; s < c->S; ++s) {
+    const cbm_sim_desc& sd = c->sims[s];
+    if (sd.n < 0 || sd.n_max < sd.n || sd.n_max < 1) return fail(c, CBM_ERR_CONFIG, "need 0 <= N <= n_max and n_max >= 1");
+    for (int a = 0; a < m.dims; ++a)
+      if (!(sd.box_hi[a] > sd.box_lo[a])) return fail(c, CBM_ERR_CONFIG, "simBox must have hi > lo on every axis");
+  }
+  return CBM_OK;
+}
+
+static int setup_geometry(cbm_ctx* c) {
+  const cbm_model_desc& m = c->m;
+  c->h_sims.resize(c->S);
+  int64_t row = 0, cell = 0;
+  double edge = m.cell_edge;
+  if (c->verlet) edge = std::max(m.cell_edge, m.skin);
+  for (int s = 0; s < c->S; ++s) {
+    const cbm_sim_desc& in = c->sims[s];
+    SimDev& sd = c->h_sims[s];
+    std::memset(&sd, 0, sizeof(sd));
+    for (int a = 0; a < 3; ++a) {
+      sd.lo[a] = in.box_lo[a]; sd.hi[a] = in.box_hi[a];
+      sd.L[a] = in.box_hi[a] - in.box_lo[a];
+      sd.gn[a] = 1; sd.gedge[a] = 1.0;
+    }
+    for (int q = 0; q < 8; ++q) sd.p[q] = in.p[q];
+    if (!c->full) {
+      for (int a = 0; a < m.dims; ++a) {
+        double L = sd.L[a];
+        if (m.periodic) {
+          int64_t n = (int64_t)std::floor(L / edge);
+          if (n < 3) return fail(c, CBM_ERR_CONFIG, "periodic cell grid needs >= 3 cells per axis (box too small for the cell edge)");
+          sd.gn[a] = (int32_t)n;
+          sd.gedge[a] = L / (double)n;
+        } else {
+          double n = std::ceil(L / edge + 2.0);   // src/neighbors.jl:604
+          if (n > 2.0e9) return fail(c, CBM_ERR_CONFIG, "cell grid too large");
+          sd.gn[a] = (int32_t)n;
+          sd.gedge[a] = edge;
+        }
+      }
+    }
+    int64_t nc = (int64_t)sd.gn[0] * sd.gn[1] * sd.gn[2];
+    if (nc > (int64_t)1 << 30) return fail(c, CBM_ERR_CONFIG, "cell grid too large; raise cellEdge");
+    sd.row_off = row;
+    sd.n_max = in.n_max;
+    sd.cell_off = cell;
+    sd.ncells = nc;
+    row += in.n_max;
+    cell += nc + 2;
+  }
+  if (row > ((int64_t)1 << 31) - 1) return fail(c, CBM_ERR_CONFIG, "total capacity exceeds 2^31 agents");
+  if (cell > ((int64_t)1 << 31) - 1) return fail(c, CBM_ERR_CONFIG, "total cell count exceeds 2^31");
+  c->C = row;
+  c->NB = cell;
+  return CBM_OK;
+}
+
+static int setup_blocks(cbm_ctx* c) {
+  c->h_block_sim.clear();
+  c->h_block_off.assign(c->S, 0);
+  for (int s = 0; s < c->S; ++s) {
+    c->h_block_off[s] = (int32_t)c->h_block_sim.size();
+    int64_t nb = (c->sims[s].n_max + c->block - 1) / c->block;
+    for (int64_t b = 0; b < nb; ++b) c->h_block_sim.push_back(s);
+  }
+  c->nblocks = (int)c->h_block_sim.size();
+  if (c->d_block_sim) { (void)hipFree(c->d_block_sim); c->d_block_sim = nullptr; }
+  if (c->d_block_off) { (void)hipFree(c->d_block_off); c->d_block_off = nullptr; }
+  HIPCHK(c, dalloc(&c->d_block_sim, c->nblocks));
+  HIPCHK(c, dalloc(&c->d_block_off, c->S));
+  HIPCHK(c, hipMemcpy(c->d_block_sim, c->h_block_sim.data(), sizeof(int32_t) * c->nblocks, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_block_off, c->h_block_off.data(), sizeof(int32_t) * c->S, hipMemcpyHostToDevice));
+  return CBM_OK;
+}
+
+static int alloc_column(cbm_ctx* c, int k, int bytes) {
+  c->has_col[k] = true;
+  c->col[k].bytes = bytes;
+  for (int b = 0; b < 2; ++b) {
+    if (bytes == 8) { double* p; HIPCHK(c, dalloc(&p, c->C)); c->col[k].buf[b] = p; }
+    else { int32_t* p; HIPCHK(c, dalloc(&p, c->C)); c->col[k].buf[b] = p; }
+  }
+  return CBM_OK;
+}
+
+static void destroy_ctx(cbm_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  auto F = [](void* p) { if (p) (void)hipFree(p); };
+  for (int k = 0; k < C_NCOLS; ++k) { F(c->col[k].buf[0]); F(c->col[k].buf[1]); }
+  for (int a = 0; a < 3; ++a) { F(c->Fnew[a]); F(c->xref[a]); }
+  for (int v = 0; v < MAX_STATE; ++v) { F(c->h[0][v]); F(c->h[1][v]); F(c->kacc[v]); }
+  F(c->gr_new); F(c->vb_new); F(c->dead);
+  F(c->key); F(c->count); F(c->cursor); F(c->start); F(c->tmp); F(c->cell_row);
+  F(c->scan_status); F(c->scan_ticket); F(c->list); F(c->list_n); F(c->flags); F(c->counters);
+  F(c->d_staging); F(c->d_block_sim); F(c->d_block_off); F(c->d_sims); F(c->d_st);
+  for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static int init_ctx(cbm_ctx* c) {
+  const cbm_model_desc& m = c->m;
+  int rc = validate(c);
+  if (rc) return rc;
+  c->dims = m.dims;
+  c->growth = false;
+  c->nvar = c->dims + (c->growth ? 1 : 0);
+  c->lagged = m.lagged != 0;
+  c->full = m.neighbors == CBM_FULL;
+  c->cl = m.neighbors == CBM_CELL_LINKED;
+  c->verlet = !c->full && !c->cl;
+  c->vtime = m.neighbors == CBM_VERLET_TIME;
+  c->nb_kind = c->full ? NB_FULL : (c->cl ? NB_CELL : NB_LIST);
+  c->list_radius = m.skin;
+  c->thr = m.rebuild_disp > 0 ? m.rebuild_disp : m.skin / 2;   // src/neighbors.jl:403
+  HIPCHK(c, hipGetDevice(&c->device));
+  HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(c, hipEventCreate(&c->ev0));
+  HIPCHK(c, hipEventCreate(&c->ev1));
+  if ((rc = setup_geometry(c))) return rc;
+  if ((rc = setup_blocks(c))) return rc;
+  HIPCHK(c, dalloc(&c->d_sims, c->S));
+  HIPCHK(c, hipMemcpy(c->d_sims, c->h_sims.data(), sizeof(SimDev) * c->S, hipMemcpyHostToDevice));
+  std::vector<SimState> st(c->S);
+  for (int s = 0; s < c->S; ++s) {
+    std::memset(&st[s], 0, sizeof(SimState));
+    st[s].N = c->sims[s].n; st[s].n_tot = c->sims[s].n; st[s].id_max = c->sims[s].n;
+  }
+  HIPCHK(c, dalloc(&c->d_st, c->S));
+  HIPCHK(c, hipMemcpy(c->d_st, st.data(), sizeof(SimState) * c->S, hipMemcpyHostToDevice));
+
+  // columns
+  for (int a = 0; a < c->dims; ++a) if ((rc = alloc_column(c, C_X + a, 8))) return rc;
+  if ((rc = alloc_column(c, C_ID, 8))) return rc;
+  if ((rc = alloc_column(c, C_SLOT, 4))) return rc;
+  if (c->lagged) {
+    for (int a = 0; a < c->dims; ++a) { if ((rc = alloc_column(c, C_FX + a, 8))) return rc; HIPCHK(c, dalloc(&c->Fnew[a], c->C)); }
+  }
+  // slot = local row, id = local row + 1 (id = 1:N, BASEPARAMETERS constants.jl:21)
+  {
+    std::vector<int32_t> slot(c->C);
+    std::vector<int64_t> id(c->C);
+    for (int s = 0; s < c->S; ++s)
+      for (int64_t r = 0; r < c->sims[s].n_max; ++r) {
+        slot[c->h_sims[s].row_off + r] = (int32_t)r;
+        id[c->h_sims[s].row_off + r] = r + 1;
+      }
+    for (int b = 0; b < 2; ++b) {
+      HIPCHK(c, hipMemcpy(c->col[C_SLOT].buf[b], slot.data(), sizeof(int32_t) * c->C, hipMemcpyHostToDevice));
+      HIPCHK(c, hipMemcpy(c->col[C_ID].buf[b], id.data(), sizeof(int64_t) * c->C, hipMemcpyHostToDevice));
+    }
+  }
+  // stage buffers
+  int nst = (m.integrator == CBM_HEUN) ? 1 : (m.integrator == CBM_RK4 ? 2 : 0);
+  for (int b = 0; b < nst; ++b)
+    for (int v = 0; v < c->nvar; ++v) HIPCHK(c, dalloc(&c->h[b][v], c->C));
+  if (m.integrator == CBM_RK4)
+    for (int v = 0; v < c->nvar; ++v) HIPCHK(c, dalloc(&c->kacc[v], c->C));
+  // neighbor engine
+  HIPCHK(c, dalloc(&c->key, c->C));
+  HIPCHK(c, dalloc(&c->tmp, c->C));
+  HIPCHK(c, dalloc(&c->cell_row, c->C));
+  HIPCHK(c, dalloc(&c->count, c->NB));
+  HIPCHK(c, dalloc(&c->cursor, c->NB));
+  HIPCHK(c, dalloc(&c->start, c->NB + 1));
+  HIPCHK(c, dalloc(&c->scan_status, scan_blocks(c->NB) + 1));
+  HIPCHK(c, dalloc(&c->scan_ticket, 4));
+  {
+    unsigned int ep = 1;
+    HIPCHK(c, hipMemcpy(c->scan_ticket + 2, &ep, sizeof(ep), hipMemcpyHostToDevice));
+  }
+  HIPCHK(c, dalloc(&c->flags, 8));
+  HIPCHK(c, dalloc(&c->counters, 4));
+  if (c->verlet) {
+    c->cap = m.max_neighbors;
+    for (int a = 0; a < c->dims; ++a) HIPCHK(c, dalloc(&c->xref[a], c->C));
+    HIPCHK(c, dalloc(&c->list, (int64_t)c->cap * c->C));
+    HIPCHK(c, dalloc(&c->list_n, c->C));
+  }
+  int64_t maxn = 1;
+  for (int s = 0; s < c->S; ++s) maxn = std::max(maxn, c->sims[s].n_max);
+  c->staging_bytes = maxn * 8;
+  HIPCHK(c, hipMalloc(&c->d_staging, (size_t)c->staging_bytes));
+  HIPCHK(c, hipDeviceSynchronize());
+  return CBM_OK;
+}
+
+// ------------------------------------------------------------ step pieces
+static PermCols perm_cols(cbm_ctx* c) {
+  PermCols pc;
+  std::memset(&pc, 0, sizeof(pc));
+  for (int k = 0; k < C_NCOLS; ++k) {
+    if (!c->has_col[k]) continue;
+    pc.cur[pc.n] = c->col[k].cur();
+    pc.alt[pc.n] = c->col[k].alt();
+    pc.bytes[pc.n] = c->col[k].bytes;
+    pc.n++;
+  }
+  return pc;
+}
+
+static RebuildArgs rebuild_args(cbm_ctx* c, const int32_t* flag, int32_t* flag_clear, bool tick) {
+  RebuildArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.g = geom(c);
+  a.flag = flag;
+  a.flag_clear = flag_clear;
+  a.rebuilds = c->counters;
+  a.step_inc = tick ? c->counters + 1 : nullptr;
+  for (int d = 0; d < c->dims; ++d) a.pos[d] = (const double*)c->col[C_X + d].cur();
+  a.dead = nullptr;
+  a.slot = (const int32_t*)c->col[C_SLOT].cur();
+  a.key = c->key; a.count = c->count; a.cursor = c->cursor; a.start = c->start; a.tmp = c->tmp;
+  a.cell_row = c->cell_row;
+  a.dims = c->dims; a.periodic = c->m.periodic; a.full = c->full ? 1 : 0;
+  a.geometric = c->verlet ? 1 : 0;
+  return a;
+}
+
+// Sort agents into buckets and permute every column.  `flag` NULL runs
+// unconditionally; otherwise every kernel exits early unless *flag != 0.
+// copy_back: Verlet mode (build lists, copy the sorted rows back into the
+// current buffers); otherwise the alternate buffers become current.
+static int rebuild_chain(cbm_ctx* c, const int32_t* flag, int32_t* flag_clear, bool tick, bool copy_back) {
+  RebuildArgs a = rebuild_args(c, flag, flag_clear, tick);
+  HIPCHK(c, launch_keys(c->nblocks, c->block, c->stream, a));
+  ScanArgs s;
+  std::memset(&s, 0, sizeof(s));
+  s.flag = flag; s.count = c->count; s.cursor = c->cursor; s.start = c->start; s.n = c->NB;
+  s.status = c->scan_status; s.ticket = c->scan_ticket; s.done = c->scan_ticket + 1;
+  s.epoch = c->scan_ticket + 2; s.nblocks = scan_blocks(c->NB);
+  HIPCHK(c, launch_scan(c->stream, s));
+  HIPCHK(c, launch_scatter(c->nblocks, c->block, c->stream, a));
+  PermuteArgs pa;
+  pa.rb = a;
+  pa.cols = perm_cols(c);
+  HIPCHK(c, launch_permute(c->nblocks, c->block, c->stream, pa));
+  if (copy_back) {
+    VerletArgs va;
+    std::memset(&va, 0, sizeof(va));
+    va.rb = a;
+    va.cols = pa.cols;
+    for (int d = 0; d < c->dims; ++d) { va.apos[d] = (const double*)c->col[C_X + d].alt(); va.xref[d] = c->xref[d]; }
+    va.aslot = (const int32_t*)c->col[C_SLOT].alt();
+    va.list = c->list; va.list_n = c->list_n; va.ldl = c->C; va.cap = c->cap;
+    va.overflow = c->flags + 2;
+    va.rl = c->list_radius;
+    HIPCHK(c, launch_verlet(c->dims, c->nblocks, c->block, c->stream, va));
+  } else {
+    for (int k = 0; k < C_NCOLS; ++k) if (c->has_col[k]) c->col[k].cidx ^= 1;
+  }
+  return CBM_OK;
+}
+
+// Forced (re)build before step c->step: computeNeighbors! of loadToPlatform!
+static int force_rebuild(cbm_ctx* c) {
+  if (c->verlet) {
+    int32_t* f = c->flags + ((c->step + 1) & 1);
+    HIPCHK(c, launch_set_flag(c->stream, f, 1));
+    int rc = rebuild_chain(c, f, c->flags + (c->step & 1), false, true);
+    if (rc) return rc;
+    if (c->vtime) c->t_next = c->t + c->m.dt_recompute;
+  } else {
+    int rc = rebuild_chain(c, nullptr, nullptr, false, false);
+    if (rc) return rc;
+  }
+  c->structure_valid = true;
+  return CBM_OK;
+}
+
+static StageArgs stage_base(cbm_ctx* c) {
+  StageArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.g = geom(c);
+  if (c->lagged) for (int d = 0; d < c->dims; ++d) { a.F[d] = (const double*)c->col[C_FX + d].cur(); a.Fnew[d] = c->Fnew[d]; }
+  for (int d = 0; d < c->dims; ++d) a.xref[d] = c->xref[d];
+  a.id = (const int64_t*)c->col[C_ID].cur();
+  a.list = c->list; a.list_n = c->list_n; a.ldl = c->C;
+  a.cell_start = c->start; a.cell_row = c->cell_row;
+  a.dt = c->m.dt; a.sqdt = std::sqrt(c->m.dt); a.drift = c->m.drift; a.noise = c->m.noise; a.thr = c->thr;
+  a.seed = c->m.seed;
+  a.step_ptr = c->counters + 1;
+  a.periodic = c->m.periodic;
+  a.lagged = c->lagged ? 1 : 0;
+  a.walls = (c->m.law == CBM_LAW_AGGREGATION) ? 1 : 0;
+  a.nvar = c->nvar;
+  a.write_fnew = c->lagged ? 1 : 0;
+  return a;
+}
+
+// Optional per-launch event pairs (cbm_profile): category = stage kind, or
+// PROF_CHAIN for the neighbor update of one step.
+constexpr int PROF_CHAIN = 8;
+constexpr int PROF_NCAT = 10;
+
+static void prof_begin(cbm_ctx* c, int cat);
+static void prof_end(cbm_ctx* c);
+
+static int run_stage(cbm_ctx* c, StageArgs& a, int kind) {
+  a.kind = kind;
+  prof_begin(c, kind);
+  HIPCHK(c, launch_stage(c->dims, c->nb_kind, c->m.law, c->nblocks, c->block, c->stream, a));
+  prof_end(c);
+  c->last_launches++;
+  return CBM_OK;
+}
+
+static double* statecol(cbm_ctx* c, int v, bool alt) {
+  int k = (v < c->dims) ? C_X + v : C_VOL;
+  return (double*)(alt ? c->col[k].alt() : c->col[k].cur());
+}
+
+static void flip_state(cbm_ctx* c) {
+  for (int v = 0; v < c->nvar; ++v) { int k = (v < c->dims) ? C_X + v : C_VOL; c->col[k].cidx ^= 1; }
+}
+
+static int neighbor_update(cbm_ctx* c, int64_t s, bool disp);
+
+static int step_once(cbm_ctx* c) {
+  int rc;
+  if (!c->structure_valid && (rc = force_rebuild(c))) return rc;
+  const int64_t s = c->step;
+  StageArgs a = stage_base(c);
+  int32_t* flag_now = c->flags + (s & 1);
+  const bool disp = c->verlet && !c->vtime;
+  auto set_state = [&](double* const* S, double* const* U, double* const* out) {
+    for (int v = 0; v < c->nvar; ++v) { a.S[v] = S[v]; a.U[v] = U[v]; a.out[v] = out[v]; a.kacc[v] = c->kacc[v]; }
+  };
+  double* cur[MAX_STATE] = {};
+  double* alt[MAX_STATE] = {};
+  for (int v = 0; v < c->nvar; ++v) { cur[v] = statecol(c, v, false); alt[v] = statecol(c, v, true); }
+  a.flag = flag_now;
+  switch (c->m.integrator) {
+    case CBM_EULER:
+    case CBM_EM:
+      set_state(cur, cur, alt);
+      a.disp_check = disp ? 1 : 0;
+      if ((rc = run_stage(c, a, c->m.integrator == CBM_EULER ? ST_EULER : ST_EM))) return rc;
+      flip_state(c);
+      break;
+    case CBM_HEUN:
+      set_state(cur, cur, c->h[0]);
+      a.disp_check = 0;
+      if ((rc = run_stage(c, a, ST_HEUN1))) return rc;
+      set_state(c->h[0], cur, cur);
+      a.disp_check = disp ? 1 : 0;
+      if ((rc = run_stage(c, a, ST_HEUN2))) return rc;
+      break;
+    default:  // RK4
+      a.disp_check = 0;
+      set_state(cur, cur, c->h[0]);
+      if ((rc = run_stage(c, a, ST_RK4_1))) return rc;
+      set_state(c->h[0], cur, c->h[1]);
+      if ((rc = run_stage(c, a, ST_RK4_2))) return rc;
+      set_state(c->h[1], cur, c->h[0]);
+      if ((rc = run_stage(c, a, ST_RK4_3))) return rc;
+      set_state(c->h[0], cur, cur);
+      a.disp_check = disp ? 1 : 0;
+      if ((rc = run_stage(c, a, ST_RK4_4))) return rc;
+      break;
+  }
+  // update!: commit the write-new force accumulator (read-old/write-new)
+  if (c->lagged)
+    for (int d = 0; d < c->dims; ++d) {
+      Column& col = c->col[C_FX + d];
+      void* committed = col.buf[col.cidx];
+      col.buf[col.cidx] = c->Fnew[d];
+      c->Fnew[d] = (double*)committed;
+    }
+  c->t = c->t + c->m.dt;
+  c->step++;
+  // computeNeighbors!
+  prof_begin(c, PROF_CHAIN);
+  rc = neighbor_update(c, s, disp);
+  prof_end(c);
+  return rc;
+}
+
+static int neighbor_update(cbm_ctx* c, int64_t s, bool disp) {
+  int rc;
+  int32_t* flag_now = c->flags + (s & 1);
+  if (c->cl) {
+    if ((rc = rebuild_chain(c, nullptr, nullptr, true, false))) return rc;
+  } else if (disp) {
+    if ((rc = rebuild_chain(c, flag_now, c->flags + ((s + 1) & 1), true, true))) return rc;
+  } else if (c->vtime && c->t_next <= c->t) {   // src/neighbors.jl:255
+    int32_t* f = c->flags + (s & 1);
+    HIPCHK(c, launch_set_flag(c->stream, f, 1));
+    if ((rc = rebuild_chain(c, f, c->flags + ((s + 1) & 1), true, true))) return rc;
+    c->t_next = c->t + c->m.dt_recompute;
+  } else {
+    HIPCHK(c, launch_tick(c->stream, c->counters + 1));
+  }
+  return CBM_OK;
+}
+
+static int check_flags(cbm_ctx* c) {
+  int32_t f[8];
+  HIPCHK(c, hipMemcpyAsync(f, c->flags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (f[2]) return fail(c, CBM_ERR_LIST_OVERFLOW, "Verlet list overflow: raise nMaxNeighbors");
+  if (f[3]) return fail(c, CBM_ERR_CAPACITY, "agent capacity exceeded: raise preallocation (n_max)");
+  return CBM_OK;
+}
+
+// ======================================================================= ABI
+extern "C" {
+
+const char* cbm_version(void) { return CBM_VERSION; }
+
+int cbm_create(cbm_ctx** out, const cbm_model_desc* model, int32_t n_sims, const cbm_sim_desc* sims) {
+  if (!out || !model || !sims || n_sims < 1) return CBM_ERR_ARG;
+  *out = nullptr;
+  cbm_ctx* c = new cbm_ctx();
+  c->m = *model;
+  c->S = n_sims;
+  c->sims.assign(sims, sims + n_sims);
+  int rc = init_ctx(c);
+  *out = c;   // returned even on failure so cbm_last_error can explain
+  return rc;
+}
+
+void cbm_destroy(cbm_ctx* c) { destroy_ctx(c); }
+
+const char* cbm_last_error(const cbm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int cbm_compile(cbm_ctx* c, const char* body) {
+  (void)body;
+  return fail(c, CBM_ERR_COMPILE, "user interaction bodies are not available in this build");
+}
+
+static int field_col(cbm_ctx* c, int32_t field, int& k) {
+  switch (field) {
+    case CBM_F_X: case CBM_F_Y: case CBM_F_Z:
+      if (field >= c->dims) return CBM_ERR_ARG;
+      k = C_X + field; return CBM_OK;
+    case CBM_F_ID: k = C_ID; return CBM_OK;
+    case CBM_F_FX: case CBM_F_FY: case CBM_F_FZ:
+      if (!c->lagged || field - CBM_F_FX >= c->dims) return CBM_ERR_ARG;
+      k = C_FX + (field - CBM_F_FX); return CBM_OK;
+    default: return CBM_ERR_ARG;
+  }
+}
+
+static int xfer(cbm_ctx* c, int32_t field, void* host, int64_t n, int32_t sim, bool up) {
+  if (!c) return CBM_ERR_ARG;
+  if (sim < 0 || sim >= c->S) return fail(c, CBM_ERR_ARG, "simulation index out of range");
+  int k;
+  if (field_col(c, field, k)) return fail(c, CBM_ERR_ARG, "field not present in this model");
+  if (n < 0 || n > c->sims[sim].n_max) return fail(c, CBM_ERR_ARG, "n exceeds capacity");
+  if (n == 0) return CBM_OK;
+  const int bytes = c->col[k].bytes;
+  // rows of this simulation only: a one-simulation geometry view
+  XferArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.g = geom(c);
+  a.slot = (const int32_t*)c->col[C_SLOT].cur();
+  a.field = c->col[k].cur();
+  a.staging = c->d_staging;
+  a.bytes = bytes;
+  a.to_device = up ? 1 : 0;
+  a.n = n;
+  a.sim = sim;
+  if (up) HIPCHK(c, hipMemcpyAsync(c->d_staging, host, (size_t)n * bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, launch_xfer(c->nblocks, c->block, c->stream, a));
+  if (!up) {
+    HIPCHK(c, hipMemcpyAsync(host, c->d_staging, (size_t)n * bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  } else {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (k <= C_Z) c->structure_valid = false;
+  }
+  return CBM_OK;
+}
+
+int cbm_upload(cbm_ctx* c, int32_t field, const void* host, int64_t n, int32_t sim) {
+  return xfer(c, field, const_cast<void*>(host), n, sim, true);
+}
+
+int cbm_download(cbm_ctx* c, int32_t field, void* host, int64_t n, int32_t sim) {
+  int rc = xfer(c, field, host, n, sim, false);
+  if (rc) return rc;
+  return check_flags(c);
+}
+
+int cbm_set_n(cbm_ctx* c, int32_t sim, int64_t n) {
+  if (!c || sim < 0 || sim >= c->S) return CBM_ERR_ARG;
+  if (n < 0 || n > c->sims[sim].n_max) return fail(c, CBM_ERR_CAPACITY, "N exceeds capacity");
+  SimState st;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(&st, c->d_st + sim, sizeof(st), hipMemcpyDeviceToHost));
+  st.N = n; st.n_tot = n;
+  if (st.id_max < n) st.id_max = n;
+  HIPCHK(c, hipMemcpy(c->d_st + sim, &st, sizeof(st), hipMemcpyHostToDevice));
+  c->structure_valid = false;
+  return CBM_OK;
+}
+
+int cbm_upload_medium(cbm_ctx* c, const double*, int64_t) { return fail(c, CBM_ERR_CONFIG, "model has no medium"); }
+int cbm_download_medium(cbm_ctx* c, double*, int64_t) { return fail(c, CBM_ERR_CONFIG, "model has no medium"); }
+
+int cbm_neighbors(cbm_ctx* c) {
+  if (!c) return CBM_ERR_ARG;
+  c->structure_valid = false;
+  int rc = force_rebuild(c);
+  if (rc) return rc;
+  return CBM_OK;
+}
+
+int cbm_step(cbm_ctx* c, int64_t nsteps) {
+  if (!c || nsteps < 0) return CBM_ERR_ARG;
+  c->last_launches = 0;
+  if (!c->structure_valid) { int rc = force_rebuild(c); if (rc) return rc; }
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  for (int64_t k = 0; k < nsteps; ++k) {
+    int rc = step_once(c);
+    if (rc) return rc;
+  }
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  return CBM_OK;
+}
+
+int cbm_step_graph(cbm_ctx* c, int64_t nsteps, int64_t replays) {
+  if (!c || nsteps <= 0 || replays < 0) return CBM_ERR_ARG;
+  if (c->vtime) return fail(c, CBM_ERR_CONFIG, "graph capture needs a device-side rebuild decision (not VerletTime)");
+  if (nsteps % 2) return fail(c, CBM_ERR_ARG, "graph step count must be even (buffer parity)");
+  if (!c->structure_valid) { int rc = force_rebuild(c); if (rc) return rc; }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int cidx0[C_NCOLS];
+  for (int k = 0; k < C_NCOLS; ++k) cidx0[k] = c->col[k].cidx;
+  void* fnew0[3] = {c->Fnew[0], c->Fnew[1], c->Fnew[2]};
+  int64_t step0 = c->step;
+  double t0 = c->t;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = CBM_OK;
+  for (int64_t k = 0; k < nsteps && rc == CBM_OK; ++k) rc = step_once(c);
+  hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+  if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+  if (ce != hipSuccess) return fail(c, CBM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+  bool same = true;
+  for (int k = 0; k < C_NCOLS; ++k) same = same && (c->col[k].cidx == cidx0[k]);
+  for (int d = 0; d < 3; ++d) same = same && (c->Fnew[d] == fnew0[d]);
+  if (!same) { (void)hipGraphDestroy(graph); return fail(c, CBM_ERR_CONFIG, "step buffers are not periodic over the captured steps"); }
+  // capture advanced the host counters once; the replays do the real work
+  c->step = step0; c->t = t0;
+  hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (ie != hipSuccess) { (void)hipGraphDestroy(graph); return fail(c, CBM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie)); }
+  c->last_launches = 0;
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  for (int64_t r = 0; r < replays; ++r) {
+    hipError_t le = hipGraphLaunch(exec, c->stream);
+    if (le != hipSuccess) { (void)hipGraphExecDestroy(exec); (void)hipGraphDestroy(graph); return fail(c, CBM_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(le)); }
+    for (int64_t k = 0; k < nsteps; ++k) { c->t = c->t + c->m.dt; c->step++; }
+  }
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  return CBM_OK;
+}
+
+int cbm_sync(cbm_ctx* c) {
+  if (!c) return CBM_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_flags(c);
+}
+
+int cbm_get_n(cbm_ctx* c, int32_t sim, int64_t* n) {
+  if (!c || !n || sim < 0 || sim >= c->S) return CBM_ERR_ARG;
+  SimState st;
+  HIPCHK(c, hipMemcpyAsync(&st, c->d_st + sim, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *n = st.N;
+  return CBM_OK;
+}
+
+int cbm_get_id_max(cbm_ctx* c, int32_t sim, int64_t* id_max) {
+  if (!c || !id_max || sim < 0 || sim >= c->S) return CBM_ERR_ARG;
+  SimState st;
+  HIPCHK(c, hipMemcpyAsync(&st, c->d_st + sim, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *id_max = st.id_max;
+  return CBM_OK;
+}
+
+int cbm_get_t(cbm_ctx* c, double* t, int64_t* step) {
+  if (!c) return CBM_ERR_ARG;
+  if (t) *t = c->t;
+  if (step) *step = c->step;
+  return CBM_OK;
+}
+
+int cbm_get_rebuilds(cbm_ctx* c, int64_t* rebuilds) {
+  if (!c || !rebuilds) return CBM_ERR_ARG;
+  int64_t v[4];
+  HIPCHK(c, hipMemcpyAsync(v, c->counters, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *rebuilds = v[0];
+  return CBM_OK;
+}
+
+static int export_sets(cbm_ctx* c, int32_t sim, double radius, int32_t* counts, int64_t* lists, int64_t cap, int64_t* counts64) {
+  if (!c || sim < 0 || sim >= c->S) return CBM_ERR_ARG;
+  if (!c->structure_valid) { int rc = force_rebuild(c); if (rc) return rc; }
+  int64_t N;
+  int rc = cbm_get_n(c, sim, &N);
+  if (rc) return rc;
+  int32_t* d_counts = nullptr; int64_t* d_lists = nullptr; int64_t* d_c64 = nullptr; int32_t* d_ovf = nullptr;
+  int64_t nmax = c->sims[sim].n_max;
+  HIPCHK(c, dalloc(&d_ovf, 1));
+  if (counts) HIPCHK(c, dalloc(&d_counts, nmax));
+  if (lists) HIPCHK(c, dalloc(&d_lists, nmax * std::max<int64_t>(cap, 1)));
+  if (counts64) HIPCHK(c, dalloc(&d_c64, nmax));
+  ExportArgs e;
+  std::memset(&e, 0, sizeof(e));
+  e.s = stage_base(c);
+  for (int d = 0; d < c->dims; ++d) e.s.S[d] = (const double*)c->col[C_X + d].cur();
+  e.slot = (const int32_t*)c->col[C_SLOT].cur();
+  e.counts = d_counts; e.lists = d_lists; e.counts64 = d_c64; e.cap = cap; e.radius = radius;
+  e.take_all = (c->verlet && (lists || radius <= 0.0)) ? 1 : 0;
+  e.overflow = d_ovf;
+  e.only_sim = sim;
+  int ovf = 0;
+  hipError_t le = launch_export(c->dims, c->nb_kind, c->nblocks, c->block, c->stream, e);
+  if (le == hipSuccess) le = hipStreamSynchronize(c->stream);
+  if (le == hipSuccess && counts) le = hipMemcpy(counts, d_counts, sizeof(int32_t) * N, hipMemcpyDeviceToHost);
+  if (le == hipSuccess && lists) le = hipMemcpy(lists, d_lists, sizeof(int64_t) * N * cap, hipMemcpyDeviceToHost);
+  if (le == hipSuccess && counts64) le = hipMemcpy(counts64, d_c64, sizeof(int64_t) * N, hipMemcpyDeviceToHost);
+  if (le == hipSuccess) le = hipMemcpy(&ovf, d_ovf, sizeof(int), hipMemcpyDeviceToHost);
+  (void)hipFree(d_ovf); if (d_counts) (void)hipFree(d_counts); if (d_lists) (void)hipFree(d_lists); if (d_c64) (void)hipFree(d_c64);
+  if (le != hipSuccess) return fail(c, CBM_ERR_HIP, std::string("neighbor export: ") + hipGetErrorString(le));
+  if (ovf) return fail(c, CBM_ERR_LIST_OVERFLOW, "neighbor export capacity too small");
+  return CBM_OK;
+}
+
+int cbm_get_neighbor_lists(cbm_ctx* c, int32_t sim, double radius, int32_t* counts, int64_t* lists, int64_t cap) {
+  if (!counts || !lists || cap < 1) return CBM_ERR_ARG;
+  return export_sets(c, sim, radius, counts, lists, cap, nullptr);
+}
+
+int cbm_count_neighbors(cbm_ctx* c, int32_t sim, double radius, int64_t* out) {
+  if (!out) return CBM_ERR_ARG;
+  return export_sets(c, sim, radius, nullptr, nullptr, 0, out);
+}
+
+int cbm_adjoint(cbm_ctx* c, const cbm_loss_desc*, double*, double*) {
+  return fail(c, CBM_ERR_CONFIG, "adjoint is not available in this build");
+}
+
+int cbm_set_block(cbm_ctx* c, int32_t block) {
+  if (!c || (block != 64 && block != 128 && block != 256)) return CBM_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->block = block;
+  return setup_blocks(c);
+}
+
+int cbm_autotune(cbm_ctx* c, const char*, int32_t* block) {
+  if (!c) return CBM_ERR_ARG;
+  if (block) *block = c->block;
+  return CBM_OK;
+}
+
+int cbm_profile(cbm_ctx* c, int64_t nsteps, double* avg_ms, int64_t* launches, int32_t ncat) {
+  if (!c || nsteps < 1 || !avg_ms || !launches || ncat < PROF_NCAT) return CBM_ERR_ARG;
+  if (!c->structure_valid) { int rc = force_rebuild(c); if (rc) return rc; }
+  size_t need = (size_t)nsteps * 12 * 2;
+  while (c->prof_ev.size() < need) {
+    hipEvent_t e;
+    HIPCHK(c, hipEventCreate(&e));
+    c->prof_ev.push_back(e);
+  }
+  c->prof_cat.assign(c->prof_ev.size() / 2 + 1, 0);
+  c->prof_used = 0;
+  c->profiling = true;
+  int rc = CBM_OK;
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  for (int64_t k = 0; k < nsteps && rc == CBM_OK; ++k) rc = step_once(c);
+  c->profiling = false;
+  if (rc) return rc;
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int q = 0; q < ncat; ++q) { avg_ms[q] = 0.0; launches[q] = 0; }
+  for (size_t p = 0; p + 1 < c->prof_used; p += 2) {
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->prof_ev[p], c->prof_ev[p + 1]));
+    int cat = c->prof_cat[p / 2];
+    avg_ms[cat] += ms;
+    launches[cat] += 1;
+  }
+  for (int q = 0; q < PROF_NCAT - 1; ++q) if (launches[q]) avg_ms[q] /= (double)launches[q];
+  float tot = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&tot, c->ev0, c->ev1));
+  avg_ms[PROF_NCAT - 1] = tot / (double)nsteps;
+  launches[PROF_NCAT - 1] = nsteps;
+  return check_flags(c);
+}
+
+int cbm_last_timing(cbm_ctx* c, double* ms_total, double* ms_force, int64_t* launches) {
+  if (!c) return CBM_ERR_ARG;
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  if (ms_total) *ms_total = ms;
+  if (ms_force) *ms_force = ms;
+  if (launches) *launches = c->last_launches;
+  return CBM_OK;
+}
+
+}  // extern "C"

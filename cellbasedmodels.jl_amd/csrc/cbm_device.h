@@ -1,0 +1,102 @@
+// cbm_device.h -- device-side building blocks of the agent step kernels
+// (gfx950, FP64).  Self-contained so that the same text can be compiled by
+// hipcc into libcbm_hip.so and by hipRTC around a user interaction body.
+//
+// Arithmetic is written in the exact evaluation order of the reference
+// expressions it restates; the library is compiled with -ffp-contract=off so
+// no multiply-add is fused and results are bit-identical to an IEEE host
+// evaluation of the same expressions.
+#ifndef CBM_DEVICE_H
+#define CBM_DEVICE_H
+
+#ifndef CBM_HIPRTC
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#endif
+
+namespace cbm {
+
+// ---------------------------------------------------------------- Philox
+// Philox4x32-10 (Salmon et al., SC'11): counter-based, so every agent draws
+// its own noise from (seed, step, id, sim, purpose) with no RNG state.
+struct u4 { uint32_t x, y, z, w; };
+
+__device__ __forceinline__ u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
+    uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+    u4 o;
+    o.x = hi1 ^ c.y ^ k0; o.y = lo1;
+    o.z = hi0 ^ c.w ^ k1; o.w = lo0;
+    c = o;
+  }
+  return c;
+}
+
+__device__ __forceinline__ u4 draw(uint64_t seed, int64_t step, int64_t id, int32_t sim, uint32_t purpose) {
+  u4 c; c.x = (uint32_t)step; c.y = (uint32_t)id; c.z = (uint32_t)sim; c.w = purpose;
+  return philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// 53-bit uniform in [0,1).
+__device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// Box-Muller transform (Box & Muller, 1958) of one Philox block.
+__device__ __forceinline__ void normal2(u4 r, double& z0, double& z1) {
+  double u1 = 1.0 - u53(r.x, r.y);
+  double u2 = u53(r.z, r.w);
+  double rad = sqrt(-2.0 * log(u1));
+  double th = 6.283185307179586 * u2;
+  double s, c;
+  sincos(th, &s, &c);
+  z0 = rad * c;
+  z1 = rad * s;
+}
+
+// floor of the exact real quotient a/b for a >= 0, b > 0: the reference's
+// Int(a ÷ b) (src/neighbors.jl:636, src/metrics.jl:19).  The fused
+// multiply-add gives the exact residual a - k*b, which corrects the rounded
+// quotient by one where a/b rounded across an integer.
+__device__ __forceinline__ int64_t floor_div(double a, double b) {
+  double k = floor(a / b);
+  double r = fma(-k, b, a);
+  if (r < 0.0) k -= 1.0; else if (r >= b) k += 1.0;
+  return (int64_t)k;
+}
+
+// ------------------------------------------------------------- pair laws
+// Pair magnitude s(d); the force on i is s * (x_i - x_j)/d.  Returns false
+// for pairs that do not interact (including coincident agents, d == 0).
+enum { LAW_NONE = 0, LAW_LINEAR = 1, LAW_REP_ADH = 2, LAW_AGGREGATION = 3, LAW_CUSTOM = 4 };
+
+template <int LAW>
+__device__ __forceinline__ bool pair_s(const double* __restrict__ p, double d, double& s) {
+  if (!(d > 0.0)) return false;
+  if (LAW == LAW_LINEAR) {
+    if (d < p[1]) { s = p[0] * (p[1] - d); return true; }
+    return false;
+  } else if (LAW == LAW_REP_ADH) {
+    double c = 2.0 * p[1];
+    double dc = p[3] * p[1];
+    if (d < c) { s = p[0] * (c - d); return true; }
+    if (d < dc) {
+      double w = dc - c;
+      s = 0.0 - p[2] * 4.0 * (d - c) * (dc - d) / (w * w);
+      return true;
+    }
+    return false;
+  } else if (LAW == LAW_AGGREGATION) {
+    if (d < p[0]) { s = p[1] * (p[0] - d); return true; }
+    if (d < p[2]) { s = 0.0 - p[3] * (p[2] - d); return true; }
+    return false;
+  }
+  return false;
+}
+
+}  // namespace cbm
+
+#endif  // CBM_DEVICE_H

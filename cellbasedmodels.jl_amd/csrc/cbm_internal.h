@@ -1,0 +1,192 @@
+// cbm_internal.h -- structures shared by the step kernels and the host side
+// of libcbm_hip.so.  Not part of the C ABI.
+#ifndef CBM_INTERNAL_H
+#define CBM_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/cbm_hip.h"
+
+namespace cbm {
+
+constexpr int MAX_COLS = 16;      // permutable per-agent columns
+constexpr int MAX_STATE = 4;      // DE variables: up to 3 positions + vol
+
+// Per-simulation constants (device array, one entry per simulation).
+// Row space: simulation s owns rows [row_off, row_off + n_max).
+// Cell space: simulation s owns global buckets [cell_off, cell_off + ncells + 2):
+// its grid cells, then a bucket for removed agents, then a "gap" bucket
+// sized so that a global exclusive scan of bucket counts lands every bucket
+// start on its global row index.
+struct SimDev {
+  double lo[3], hi[3], L[3];
+  double p[8];
+  double gedge[3];
+  int32_t gn[3];
+  int32_t pad0;
+  int64_t row_off, n_max;
+  int64_t cell_off, ncells;
+};
+
+// Per-simulation mutable scalars (device array).
+struct SimState {
+  int64_t N;        // com.N (agents committed)
+  int64_t n_tot;    // rows populated this step (N + NAdd_ before compaction)
+  int64_t n_add;    // NAdd_
+  int64_t n_remove; // NRemove_
+  int64_t id_max;   // idMax_
+  int64_t pad[3];
+};
+
+// Launch geometry shared by row-parallel kernels: block b serves
+// simulation block_sim[b], rows (b - block_off[sim]) * blockDim + tid.
+struct RowGeom {
+  const int32_t* block_sim;
+  const int32_t* block_off;
+  const SimDev* sims;
+  SimState* st;
+};
+
+// Integrator stage kinds (one kernel launch each).  See src/integrators.jl.
+enum StageKind {
+  ST_EULER = 0,     // u' = u + du*dt                        (:496-498)
+  ST_EM = 1,        // u' = (u + f*dt) + (g*xi)*sqrt(dt)      (:724-727)
+  ST_HEUN1 = 2,     // h = u + du*dt                          (:551-553)
+  ST_HEUN2 = 3,     // u' = (h + (u + du*dt))/2               (:564-566)
+  ST_RK4_1 = 4,     // k = k1;        h = u + k1*dt/2         (:620-624)
+  ST_RK4_2 = 5,     // k = k + 2*k2;  h = u + k2*dt/2
+  ST_RK4_3 = 6,     // k = k + 2*k3;  h = u + k3*dt
+  ST_RK4_4 = 7      // u' = u + ((k + k4)*dt)/6                (:659-661)
+};
+
+enum NeighborKind { NB_FULL = 0, NB_CELL = 1, NB_LIST = 2 };
+
+struct StageArgs {
+  RowGeom g;
+  const double* S[MAX_STATE];   // stage state read by the force (neighbors + self)
+  const double* U[MAX_STATE];   // step-start state u
+  double* out[MAX_STATE];       // stage output (h, or u')
+  double* kacc[MAX_STATE];      // RK4 accumulator
+  const double* F[3];           // committed force accumulator (lagged mode)
+  double* Fnew[3];              // force accumulator written by this stage
+  const double* xref[3];        // positions at the last Verlet build
+  const double* growth;         // per-agent growth rate (vol variable)
+  const int64_t* id;
+  const int32_t* list;          // ELL Verlet list, entry q of row i at q*ldl + i
+  const int32_t* list_n;
+  int64_t ldl;
+  const int32_t* cell_start;    // bucket -> first global row
+  const int32_t* cell_row;      // row -> global bucket (cell-linked)
+  int32_t* flag;                // rebuild flag written by the final stage
+  double dt, sqdt, drift, noise, thr;
+  uint64_t seed;
+  const int64_t* step_ptr;     // device step counter (Philox counter word)
+  int32_t kind, periodic, lagged, walls, nvar, disp_check, write_fnew, pad;
+};
+
+// Per-agent columns moved by the cell sort permutation.
+struct PermCols {
+  void* cur[MAX_COLS];
+  void* alt[MAX_COLS];
+  int32_t bytes[MAX_COLS];
+  int32_t n;
+};
+
+
+// ------------------------------------------------------ kernel arguments
+struct RebuildArgs {
+  RowGeom g;
+  const int32_t* flag;        // NULL: unconditional
+  int32_t* flag_clear;        // flag of the next step, cleared by k_keys
+  int64_t* rebuilds;          // rebuild counter
+  int64_t* step_inc;          // device step counter advanced by k_keys (may be NULL)
+  const double* pos[3];       // current positions
+  const uint8_t* dead;        // per-row removal mark (may be NULL)
+  const int32_t* slot;        // per-row logical slot (current)
+  int32_t* key;               // per-row bucket
+  int32_t* count;             // per-bucket count
+  int32_t* cursor;            // per-bucket scatter cursor
+  int32_t* start;             // per-bucket first row (exclusive scan), nb+1 entries
+  int32_t* tmp;               // bucket-ordered source rows
+  int32_t* cell_row;          // destination row -> bucket
+  int32_t dims, periodic, full, geometric;
+};
+
+struct ScanArgs {
+  const int32_t* flag;
+  int32_t* count;
+  int32_t* cursor;
+  int32_t* start;
+  int64_t n;
+  unsigned long long* status;
+  unsigned int* ticket;
+  unsigned int* done;
+  unsigned int* epoch;        // device launch epoch (>= 1)
+  int32_t nblocks;
+};
+
+struct PermuteArgs {
+  RebuildArgs rb;
+  PermCols cols;
+};
+
+struct VerletArgs {
+  RebuildArgs rb;
+  PermCols cols;
+  const double* apos[3];      // alternate (sorted) positions
+  const int32_t* aslot;       // alternate (sorted) slots
+  double* xref[3];
+  int32_t* list;
+  int32_t* list_n;
+  int64_t ldl;
+  int32_t cap;
+  int32_t* overflow;
+  double rl;                  // list radius
+};
+
+struct XferArgs {
+  RowGeom g;
+  const int32_t* slot;
+  void* field;          // device column
+  void* staging;        // logical-order staging (per-sim n values, sim offset applied)
+  int32_t bytes;
+  int32_t to_device;
+  int64_t n;            // values in staging
+  int32_t sim;          // simulation whose rows are transferred
+  int32_t pad;
+};
+
+struct ExportArgs {
+  StageArgs s;
+  const int32_t* slot;
+  int32_t* counts;      // per logical slot
+  int64_t* lists;       // per logical slot * cap
+  int64_t* counts64;
+  int64_t cap;
+  double radius;
+  int32_t take_all;     // Verlet: export the list itself
+  int32_t* overflow;
+  int32_t only_sim;
+  int32_t pad;
+};
+
+// ------------------------------------------------------------- launchers
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+inline int scan_blocks(int64_t n) { return (int)((n + SCAN_TILE - 1) / SCAN_TILE); }
+
+hipError_t launch_stage(int dims, int nb, int law, int nblocks, int block, hipStream_t s, const StageArgs& a);
+hipError_t launch_keys(int nblocks, int block, hipStream_t s, const RebuildArgs& a);
+hipError_t launch_scan(hipStream_t s, const ScanArgs& a);
+hipError_t launch_scatter(int nblocks, int block, hipStream_t s, const RebuildArgs& a);
+hipError_t launch_permute(int nblocks, int block, hipStream_t s, const PermuteArgs& a);
+hipError_t launch_verlet(int dims, int nblocks, int block, hipStream_t s, const VerletArgs& a);
+hipError_t launch_set_flag(hipStream_t s, int32_t* flag, int32_t v);
+hipError_t launch_tick(hipStream_t s, int64_t* counter);
+hipError_t launch_xfer(int nblocks, int block, hipStream_t s, const XferArgs& a);
+hipError_t launch_export(int dims, int nb, int nblocks, int block, hipStream_t s, const ExportArgs& e);
+
+}  // namespace cbm
+
+#endif

@@ -1,0 +1,619 @@
+// cbm_kernels.hip -- gfx950 kernels of the per-step agent update.
+//
+//   k_stage<D,NB,LAW>  neighbor gather + pair law + one integrator stage
+//                      (the generated agentODE kernel of
+//                      src/AgentStructure/functionDE.jl:92-108 fused with the
+//                      CBMIntegrators broadcasts, src/integrators.jl:465-734)
+//   k_keys             cell key + bucket histogram   (assignCells, src/neighbors.jl:763-821)
+//   k_scan             single-pass exclusive scan    (cumsum, src/neighbors.jl:934-950)
+//   k_scatter          bucket scatter                (sortAgentsInCells, src/neighbors.jl:837-899)
+//   k_permute          deterministic in-cell order + column permutation
+//   k_verlet           Verlet list build from cells (src/neighbors.jl:92-165, 1071-1183)
+//
+// Agents are stored sorted by (cell, logical slot); every neighbor gather
+// therefore reads a compact, mostly L2-resident window of rows, and the
+// candidate order (cells in cellPosNeigh order, ascending slot within a
+// cell, Verlet lists ascending by slot) equals the reference CPU loop order.
+#include "cbm_internal.h"
+#include "cbm_device.h"
+
+namespace cbm {
+
+// ------------------------------------------------------------ row geometry
+struct RowIdx { int sim; int64_t local; int64_t row; };
+
+__device__ __forceinline__ RowIdx row_index(const RowGeom& g) {
+  RowIdx r;
+  const int b = blockIdx.x;
+  r.sim = g.block_sim[b];
+  r.local = (int64_t)(b - g.block_off[r.sim]) * blockDim.x + threadIdx.x;
+  r.row = g.sims[r.sim].row_off + r.local;
+  return r;
+}
+
+__device__ __forceinline__ double min_image(double d, double L) {
+  if (d > 0.5 * L) d -= L; else if (d < -0.5 * L) d += L;
+  return d;
+}
+
+// Axis cell of a position: cellPos (src/neighbors.jl:636-638) on the padded
+// grid, or the periodic grid of n cells of edge L/n.
+// geometric: bin by position then clamp (Verlet list builds, which must keep
+// every pair within the list radius in adjacent cells even for agents just
+// outside the box, where the reference clamping would separate them).
+__device__ __forceinline__ int32_t axis_cell(const SimDev& sd, int a, double x, int periodic, int geometric) {
+  if (geometric) {
+    if (x < sd.lo[a]) return 0;
+    int64_t c = floor_div(x - sd.lo[a], sd.gedge[a]) + (periodic ? 0 : 1);
+    return (int32_t)(c > sd.gn[a] - 1 ? sd.gn[a] - 1 : c);
+  }
+  if (periodic) {
+    int64_t c = floor_div(x - sd.lo[a], sd.gedge[a]);
+    if (c < 0) c = 0;
+    if (c > sd.gn[a] - 1) c = sd.gn[a] - 1;
+    return (int32_t)c;
+  }
+  if (x > sd.hi[a]) return sd.gn[a] - 1;
+  if (x < sd.lo[a]) return 0;
+  return (int32_t)(floor_div(x - sd.lo[a], sd.gedge[a]) + 1);
+}
+
+// k-th of the 3^D neighbor cells (cellPosNeigh order, x fastest) as a local
+// cell index, or -1 outside a non-periodic grid.
+template <int D>
+__device__ __forceinline__ int32_t neigh_cell(const SimDev& sd, int32_t cx, int32_t cy, int32_t cz, int k, int periodic) {
+  int32_t q[3] = {cx, cy, cz};
+  const int off[3] = {k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1};
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    int32_t v = q[a] + off[a];
+    if (periodic) { v = v < 0 ? v + sd.gn[a] : (v >= sd.gn[a] ? v - sd.gn[a] : v); }
+    else if (v < 0 || v >= sd.gn[a]) return -1;
+    q[a] = v;
+  }
+  return q[0] + sd.gn[0] * (q[1] + sd.gn[1] * q[2]);
+}
+
+__host__ __device__ constexpr int pow3(int d) { return d == 1 ? 3 : (d == 2 ? 9 : 27); }
+
+// Walk the candidate neighbors of row i in reference loop order.
+template <int D, int NB, class Fn>
+__device__ __forceinline__ void for_candidates(const StageArgs& a, const SimDev& sd, int64_t N, int64_t i, Fn&& fn) {
+  if (NB == NB_LIST) {
+    const int32_t cnt = a.list_n[i];
+    for (int32_t q = 0; q < cnt; ++q) fn((int64_t)a.list[(int64_t)q * a.ldl + i]);
+  } else if (NB == NB_FULL) {
+    const int64_t j0 = sd.row_off, j1 = sd.row_off + N;
+    for (int64_t j = j0; j < j1; ++j) if (j != i) fn(j);
+  } else {
+    const int32_t c = a.cell_row[i] - (int32_t)sd.cell_off;
+    const int32_t cx = c % sd.gn[0];
+    const int32_t cy = (c / sd.gn[0]) % sd.gn[1];
+    const int32_t cz = c / (sd.gn[0] * sd.gn[1]);
+    for (int k = 0; k < pow3(D); ++k) {
+      const int32_t nc = neigh_cell<D>(sd, cx, cy, cz, k, a.periodic);
+      if (nc < 0) continue;
+      const int64_t gc = sd.cell_off + nc;
+      const int32_t j0 = a.cell_start[gc], j1 = a.cell_start[gc + 1];
+      for (int32_t j = j0; j < j1; ++j) if (j != i) fn((int64_t)j);
+    }
+  }
+}
+
+// ----------------------------------------------------------- stage kernel
+template <int D, int NB, int LAW>
+__global__ __launch_bounds__(256) void k_stage(StageArgs a) {
+  const RowIdx ri = row_index(a.g);
+  const SimDev& sd = a.g.sims[ri.sim];
+  const int64_t N = a.g.st[ri.sim].N;
+  if (ri.local >= N) return;
+  const int64_t i = ri.row;
+
+  double xi[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < D; ++c) xi[c] = a.S[c][i];
+  double acc[3] = {0.0, 0.0, 0.0};
+
+  if (LAW != LAW_NONE) {
+    double p[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) p[q] = sd.p[q];
+    const int periodic = a.periodic;
+    for_candidates<D, NB>(a, sd, N, i, [&](int64_t j) {
+      double dx[3];
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        dx[c] = xi[c] - a.S[c][j];
+        if (periodic) dx[c] = min_image(dx[c], sd.L[c]);
+      }
+      double s2 = dx[0] * dx[0];
+      if (D > 1) s2 = s2 + dx[1] * dx[1];
+      if (D > 2) s2 = s2 + dx[2] * dx[2];
+      const double d = sqrt(s2);
+      double s;
+      if (pair_s<LAW>(p, d, s)) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) acc[c] = acc[c] + s * (dx[c] / d);
+      }
+    });
+    if (LAW == LAW_AGGREGATION && a.walls) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        const double x = xi[c];
+        if (x < sd.lo[c] + p[0] / 2) acc[c] = acc[c] + p[1];
+        else if (x > sd.hi[c] - p[0] / 2) acc[c] = acc[c] - p[1];
+      }
+    }
+  }
+
+  // derivative of each DE variable
+  double du[MAX_STATE];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    double v = a.lagged ? a.F[c][i] : acc[c];
+    if (a.drift != 0.0) v = v - a.drift * xi[c];
+    du[c] = v;
+    if (a.write_fnew) a.Fnew[c][i] = acc[c];
+  }
+  const bool grow = a.nvar > D;
+  if (grow) du[D] = a.growth[i] * a.S[D][i];
+
+  const double dt = a.dt;
+  const int nv = grow ? D + 1 : D;
+  double res[MAX_STATE];
+  switch (a.kind) {
+    case ST_EULER:
+      for (int c = 0; c < nv; ++c) res[c] = a.U[c][i] + du[c] * dt;
+      break;
+    case ST_EM: {
+      double z[4] = {0.0, 0.0, 0.0, 0.0};
+      const int64_t id = a.id[i];
+      const int64_t step = *a.step_ptr;
+      normal2(draw(a.seed, step, id, ri.sim, 0x100u), z[0], z[1]);
+      if (D > 2) normal2(draw(a.seed, step, id, ri.sim, 0x101u), z[2], z[3]);
+      for (int c = 0; c < D; ++c) res[c] = (a.U[c][i] + du[c] * dt) + a.noise * z[c] * a.sqdt;
+      if (grow) res[D] = a.U[D][i] + du[D] * dt;
+      break;
+    }
+    case ST_HEUN1:
+      for (int c = 0; c < nv; ++c) res[c] = a.U[c][i] + du[c] * dt;
+      break;
+    case ST_HEUN2:
+      for (int c = 0; c < nv; ++c) res[c] = (a.S[c][i] + (a.U[c][i] + du[c] * dt)) / 2;
+      break;
+    case ST_RK4_1:
+      for (int c = 0; c < nv; ++c) { a.kacc[c][i] = du[c]; res[c] = a.U[c][i] + du[c] * dt / 2; }
+      break;
+    case ST_RK4_2:
+      for (int c = 0; c < nv; ++c) { a.kacc[c][i] = a.kacc[c][i] + 2 * du[c]; res[c] = a.U[c][i] + du[c] * dt / 2; }
+      break;
+    case ST_RK4_3:
+      for (int c = 0; c < nv; ++c) { a.kacc[c][i] = a.kacc[c][i] + 2 * du[c]; res[c] = a.U[c][i] + du[c] * dt; }
+      break;
+    default:  // ST_RK4_4
+      for (int c = 0; c < nv; ++c) res[c] = a.U[c][i] + (a.kacc[c][i] + du[c]) * dt / 6;
+      break;
+  }
+
+  const bool final_stage = (a.kind == ST_EULER || a.kind == ST_EM || a.kind == ST_HEUN2 || a.kind == ST_RK4_4);
+  if (final_stage && a.periodic) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      double x = res[c];
+      if (x >= sd.lo[c] + sd.L[c]) x = x - sd.L[c];
+      else if (x < sd.lo[c]) x = x + sd.L[c];
+      res[c] = x;
+    }
+  }
+  for (int c = 0; c < nv; ++c) a.out[c][i] = res[c];
+
+  if (final_stage && a.disp_check) {
+    double dx[3];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      dx[c] = res[c] - a.xref[c][i];
+      if (a.periodic) dx[c] = min_image(dx[c], sd.L[c]);
+    }
+    double s2 = dx[0] * dx[0];
+    if (D > 1) s2 = s2 + dx[1] * dx[1];
+    if (D > 2) s2 = s2 + dx[2] * dx[2];
+    if (sqrt(s2) >= a.thr) *a.flag = 1;   // src/neighbors.jl:403 (>= skin/2)
+  }
+}
+
+// --------------------------------------------------------- rebuild chain
+
+__global__ __launch_bounds__(256) void k_keys(RebuildArgs a) {
+  if (a.step_inc && blockIdx.x == 0 && threadIdx.x == 0) *a.step_inc += 1;
+  if (a.flag && *a.flag == 0) return;
+  const RowIdx ri = row_index(a.g);
+  const SimDev& sd = a.g.sims[ri.sim];
+  const SimState& st = a.g.st[ri.sim];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (a.flag_clear) *a.flag_clear = 0;
+    if (a.rebuilds) *a.rebuilds += 1;
+  }
+  if (ri.local == 0) a.count[sd.cell_off + sd.ncells + 1] = (int32_t)(sd.n_max - st.n_tot);
+  if (ri.local >= st.n_tot) return;
+  const int64_t r = ri.row;
+  int32_t c;
+  if (a.dead && a.dead[r]) c = (int32_t)sd.ncells;
+  else if (a.full) c = 0;
+  else {
+    int32_t q[3] = {0, 0, 0};
+    for (int d = 0; d < a.dims; ++d) q[d] = axis_cell(sd, d, a.pos[d][r], a.periodic, a.geometric);
+    c = q[0] + sd.gn[0] * (q[1] + sd.gn[1] * q[2]);
+  }
+  c += (int32_t)sd.cell_off;
+  a.key[r] = c;
+  atomicAdd(&a.count[c], 1);
+}
+
+// Single-pass exclusive scan with decoupled look-back (Merrill & Garland,
+// "Single-pass Parallel Prefix Scan with Decoupled Look-back", 2016).  Tiles
+// are ordered by an atomic ticket, so a tile only ever waits on tiles that
+// are already running; status words carry a launch epoch kept in device
+// memory (so graph replays stay distinct) and never need clearing.  The scan also zeroes count[] and cursor[] for the next
+// rebuild.
+
+__device__ __forceinline__ unsigned long long scan_word(unsigned int epoch, unsigned int flag, unsigned int value) {
+  return ((unsigned long long)(epoch & 0x3FFFFFFFu) << 34) | ((unsigned long long)flag << 32) | value;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
+  if (a.flag && *a.flag == 0) return;
+  __shared__ unsigned int s_tile;
+  __shared__ int32_t s_wave[SCAN_THREADS / 64];
+  __shared__ int32_t s_prefix;
+  const int tid = threadIdx.x;
+  __shared__ unsigned int s_epoch;
+  if (tid == 0) {
+    s_tile = atomicAdd(a.ticket, 1u);
+    s_epoch = __hip_atomic_load(a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  const int64_t tile = s_tile;
+  const int64_t base = tile * SCAN_TILE + (int64_t)tid * SCAN_ITEMS;
+  int32_t v[SCAN_ITEMS];
+  int32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int64_t idx = base + k;
+    v[k] = idx < a.n ? a.count[idx] : 0;
+    sum += v[k];
+  }
+  // wavefront inclusive scan of per-thread sums
+  const int lane = tid & 63, wave = tid >> 6;
+  int32_t incl = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) s_wave[wave] = incl;
+  __syncthreads();
+  int32_t wave_off = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < SCAN_THREADS / 64; ++w) {
+    const int32_t t = s_wave[w];
+    if (w < wave) wave_off += t;
+    total += t;
+  }
+  if (tid == 0) {
+    const unsigned int ep = s_epoch;
+    int32_t prefix = 0;
+    if (tile == 0) {
+      __hip_atomic_store(&a.status[0], scan_word(ep, 2u, (unsigned int)total), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(&a.status[tile], scan_word(ep, 1u, (unsigned int)total), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      int64_t p = tile - 1;
+      while (p >= 0) {
+        const unsigned long long w = __hip_atomic_load(&a.status[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned int wep = (unsigned int)(w >> 34);
+        const unsigned int wfl = (unsigned int)(w >> 32) & 3u;
+        if (wep != (ep & 0x3FFFFFFFu) || wfl == 0u) continue;   // predecessor running: wait
+        prefix += (int32_t)(unsigned int)w;
+        if (wfl == 2u) break;
+        --p;
+      }
+      __hip_atomic_store(&a.status[tile], scan_word(ep, 2u, (unsigned int)(prefix + total)), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_prefix = prefix;
+  }
+  __syncthreads();
+  int32_t run = s_prefix + wave_off + (incl - sum);
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int64_t idx = base + k;
+    if (idx < a.n) {
+      a.start[idx] = run;
+      a.count[idx] = 0;
+      a.cursor[idx] = 0;
+      run += v[k];
+      if (idx == a.n - 1) a.start[a.n] = run;
+    }
+  }
+  if (tid == 0) {
+    const unsigned int fin = atomicAdd(a.done, 1u);
+    if (fin == (unsigned int)a.nblocks - 1u) {
+      // every tile has read the epoch and published; advance it for the next launch
+      const unsigned int nx = (s_epoch + 1u) & 0x3FFFFFFFu;
+      __hip_atomic_store(a.epoch, nx == 0u ? 1u : nx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(a.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(a.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(RebuildArgs a) {
+  if (a.flag && *a.flag == 0) return;
+  const RowIdx ri = row_index(a.g);
+  const SimDev& sd = a.g.sims[ri.sim];
+  if (ri.local >= a.g.st[ri.sim].n_tot) return;
+  const int64_t r = ri.row;
+  const int32_t c = a.key[r];
+  int32_t pos;
+  if (a.full && c == (int32_t)sd.cell_off) pos = a.start[c] + a.slot[r];
+  else pos = a.start[c] + atomicAdd(&a.cursor[c], 1);
+  a.tmp[pos] = (int32_t)r;
+}
+
+
+// Each source row finds its rank by logical slot inside its bucket and
+// moves every column to that row of the alternate buffers.
+__global__ __launch_bounds__(256) void k_permute(PermuteArgs pa) {
+  const RebuildArgs& a = pa.rb;
+  if (a.flag && *a.flag == 0) return;
+  const RowIdx ri = row_index(a.g);
+  const SimDev& sd = a.g.sims[ri.sim];
+  if (ri.local >= a.g.st[ri.sim].n_tot) return;
+  const int64_t r = ri.row;
+  const int32_t c = a.key[r];
+  if (c == (int32_t)(sd.cell_off + sd.ncells)) return;   // removed agent
+  const int32_t s0 = a.start[c];
+  int32_t dst;
+  const int32_t my = a.slot[r];
+  if (a.full) dst = s0 + my;
+  else {
+    const int32_t s1 = a.start[c + 1];
+    int32_t rank = 0;
+    for (int32_t q = s0; q < s1; ++q) rank += (a.slot[a.tmp[q]] < my) ? 1 : 0;
+    dst = s0 + rank;
+  }
+  for (int k = 0; k < pa.cols.n; ++k) {
+    if (pa.cols.bytes[k] == 8)
+      ((uint64_t*)pa.cols.alt[k])[dst] = ((const uint64_t*)pa.cols.cur[k])[r];
+    else
+      ((uint32_t*)pa.cols.alt[k])[dst] = ((const uint32_t*)pa.cols.cur[k])[r];
+  }
+  a.cell_row[dst] = c;
+}
+
+// Verlet list build on the freshly sorted rows (alternate buffers), then
+// copy-back of every column to the current buffers and snapshot of the
+// reference positions.  Entries are global rows ordered by ascending slot,
+// the order of the reference brute-force build (src/neighbors.jl:114-126).
+
+template <int D>
+__global__ __launch_bounds__(256) void k_verlet(VerletArgs va) {
+  const RebuildArgs& a = va.rb;
+  if (a.flag && *a.flag == 0) return;
+  const RowIdx ri = row_index(a.g);
+  const SimDev& sd = a.g.sims[ri.sim];
+  const int64_t N = a.g.st[ri.sim].N;
+  if (ri.local >= N) return;
+  const int64_t i = ri.row;
+  double xi[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < D; ++c) xi[c] = va.apos[c][i];
+  const int32_t c = a.cell_row[i] - (int32_t)sd.cell_off;
+  const int32_t cx = c % sd.gn[0];
+  const int32_t cy = (c / sd.gn[0]) % sd.gn[1];
+  const int32_t cz = c / (sd.gn[0] * sd.gn[1]);
+  int32_t cnt = 0;
+  for (int k = 0; k < pow3(D); ++k) {
+    const int32_t nc = neigh_cell<D>(sd, cx, cy, cz, k, a.periodic);
+    if (nc < 0) continue;
+    const int64_t gc = sd.cell_off + nc;
+    const int32_t j0 = a.start[gc], j1 = a.start[gc + 1];
+    for (int32_t j = j0; j < j1; ++j) {
+      if (j == i) continue;
+      double dx[3];
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        dx[q] = xi[q] - va.apos[q][j];
+        if (a.periodic) dx[q] = min_image(dx[q], sd.L[q]);
+      }
+      double s2 = dx[0] * dx[0];
+      if (D > 1) s2 = s2 + dx[1] * dx[1];
+      if (D > 2) s2 = s2 + dx[2] * dx[2];
+      if (!(sqrt(s2) < va.rl)) continue;
+      if (cnt < va.cap) {
+        // insertion by ascending logical slot
+        const int32_t sj = va.aslot[j];
+        int32_t pos = cnt;
+        while (pos > 0) {
+          const int32_t prev = va.list[(int64_t)(pos - 1) * va.ldl + i];
+          if (va.aslot[prev] < sj) break;
+          va.list[(int64_t)pos * va.ldl + i] = prev;
+          --pos;
+        }
+        va.list[(int64_t)pos * va.ldl + i] = j;
+      } else {
+        *va.overflow = 1;
+      }
+      ++cnt;
+    }
+  }
+  va.list_n[i] = cnt < va.cap ? cnt : va.cap;
+  for (int k = 0; k < va.cols.n; ++k) {
+    if (va.cols.bytes[k] == 8)
+      ((uint64_t*)va.cols.cur[k])[i] = ((const uint64_t*)va.cols.alt[k])[i];
+    else
+      ((uint32_t*)va.cols.cur[k])[i] = ((const uint32_t*)va.cols.alt[k])[i];
+  }
+#pragma unroll
+  for (int q = 0; q < D; ++q) va.xref[q][i] = xi[q];
+}
+
+// --------------------------------------------------------------- utilities
+__global__ void k_set_flag(int32_t* flag, int32_t v) { if (threadIdx.x == 0 && blockIdx.x == 0) *flag = v; }
+__global__ void k_tick(int64_t* counter) { if (threadIdx.x == 0 && blockIdx.x == 0) *counter += 1; }
+
+
+// Upload scatters logical-order values into rows; download gathers them back.
+__global__ __launch_bounds__(256) void k_xfer(XferArgs a) {
+  const RowIdx ri = row_index(a.g);
+  if (ri.sim != a.sim) return;
+  const int64_t r = ri.row;
+  const int64_t s = a.slot[r];
+  if (ri.local >= a.g.st[ri.sim].N) return;   // committed agents only
+  if (s >= a.n) return;
+  if (a.bytes == 8) {
+    if (a.to_device) ((uint64_t*)a.field)[r] = ((const uint64_t*)a.staging)[s];
+    else ((uint64_t*)a.staging)[s] = ((const uint64_t*)a.field)[r];
+  } else {
+    if (a.to_device) ((uint32_t*)a.field)[r] = ((const uint32_t*)a.staging)[s];
+    else ((uint32_t*)a.staging)[s] = ((const uint32_t*)a.field)[r];
+  }
+}
+
+// Neighbor-set export (mode 0: lists in slots, mode 1: counts only) for the
+// sorted-set parity checks and the neighbor-count agentRule of the
+// reference KAT (test/testCommunity.jl:361-415).
+
+template <int D, int NB>
+__global__ __launch_bounds__(256) void k_export(ExportArgs e) {
+  const StageArgs& a = e.s;
+  const RowIdx ri = row_index(a.g);
+  if (ri.sim != e.only_sim) return;
+  const SimDev& sd = a.g.sims[ri.sim];
+  const int64_t N = a.g.st[ri.sim].N;
+  if (ri.local >= N) return;
+  const int64_t i = ri.row;
+  const int64_t si = e.slot[i];
+  double xi[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < D; ++c) xi[c] = a.S[c][i];
+  int64_t k = 0;
+  for_candidates<D, NB>(a, sd, N, i, [&](int64_t j) {
+    bool take = e.take_all != 0;
+    if (!take) {
+      double dx[3];
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        dx[c] = xi[c] - a.S[c][j];
+        if (a.periodic) dx[c] = min_image(dx[c], sd.L[c]);
+      }
+      double s2 = dx[0] * dx[0];
+      if (D > 1) s2 = s2 + dx[1] * dx[1];
+      if (D > 2) s2 = s2 + dx[2] * dx[2];
+      take = sqrt(s2) < e.radius;
+    }
+    if (take) {
+      if (e.lists) {
+        if (k < e.cap) e.lists[si * e.cap + k] = e.slot[j];
+        else *e.overflow = 1;
+      }
+      ++k;
+    }
+  });
+  if (e.counts) e.counts[si] = (int32_t)k;
+  if (e.counts64) e.counts64[si] = k;
+}
+
+// ------------------------------------------------------------- launchers
+template <int D, int NB>
+static hipError_t launch_stage_law(int law, dim3 grid, dim3 block, hipStream_t s, const StageArgs& a) {
+  switch (law) {
+    case LAW_NONE: hipLaunchKernelGGL((k_stage<D, NB, LAW_NONE>), grid, block, 0, s, a); break;
+    case LAW_LINEAR: hipLaunchKernelGGL((k_stage<D, NB, LAW_LINEAR>), grid, block, 0, s, a); break;
+    case LAW_REP_ADH: hipLaunchKernelGGL((k_stage<D, NB, LAW_REP_ADH>), grid, block, 0, s, a); break;
+    case LAW_AGGREGATION: hipLaunchKernelGGL((k_stage<D, NB, LAW_AGGREGATION>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int D>
+static hipError_t launch_stage_nb(int nb, int law, dim3 grid, dim3 block, hipStream_t s, const StageArgs& a) {
+  switch (nb) {
+    case NB_FULL: return launch_stage_law<D, NB_FULL>(law, grid, block, s, a);
+    case NB_CELL: return launch_stage_law<D, NB_CELL>(law, grid, block, s, a);
+    default: return launch_stage_law<D, NB_LIST>(law, grid, block, s, a);
+  }
+}
+
+hipError_t launch_stage(int dims, int nb, int law, int nblocks, int block, hipStream_t s, const StageArgs& a) {
+  dim3 grid(nblocks), blk(block);
+  switch (dims) {
+    case 1: return launch_stage_nb<1>(nb, law, grid, blk, s, a);
+    case 2: return launch_stage_nb<2>(nb, law, grid, blk, s, a);
+    default: return launch_stage_nb<3>(nb, law, grid, blk, s, a);
+  }
+}
+
+hipError_t launch_keys(int nblocks, int block, hipStream_t s, const RebuildArgs& a) {
+  hipLaunchKernelGGL(k_keys, dim3(nblocks), dim3(block), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan(hipStream_t s, const ScanArgs& a) {
+  hipLaunchKernelGGL(k_scan, dim3(a.nblocks), dim3(SCAN_THREADS), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter(int nblocks, int block, hipStream_t s, const RebuildArgs& a) {
+  hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(block), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_permute(int nblocks, int block, hipStream_t s, const PermuteArgs& a) {
+  hipLaunchKernelGGL(k_permute, dim3(nblocks), dim3(block), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_verlet(int dims, int nblocks, int block, hipStream_t s, const VerletArgs& a) {
+  switch (dims) {
+    case 1: hipLaunchKernelGGL(k_verlet<1>, dim3(nblocks), dim3(block), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_verlet<2>, dim3(nblocks), dim3(block), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_verlet<3>, dim3(nblocks), dim3(block), 0, s, a); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_set_flag(hipStream_t s, int32_t* flag, int32_t v) {
+  hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, flag, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_tick(hipStream_t s, int64_t* counter) {
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s, counter);
+  return hipGetLastError();
+}
+
+hipError_t launch_xfer(int nblocks, int block, hipStream_t s, const XferArgs& a) {
+  hipLaunchKernelGGL(k_xfer, dim3(nblocks), dim3(block), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int D>
+static hipError_t launch_export_nb(int nb, dim3 grid, dim3 block, hipStream_t s, const ExportArgs& e) {
+  switch (nb) {
+    case NB_FULL: hipLaunchKernelGGL((k_export<D, NB_FULL>), grid, block, 0, s, e); break;
+    case NB_CELL: hipLaunchKernelGGL((k_export<D, NB_CELL>), grid, block, 0, s, e); break;
+    default: hipLaunchKernelGGL((k_export<D, NB_LIST>), grid, block, 0, s, e); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_export(int dims, int nb, int nblocks, int block, hipStream_t s, const ExportArgs& e) {
+  dim3 grid(nblocks), blk(block);
+  switch (dims) {
+    case 1: return launch_export_nb<1>(nb, grid, blk, s, e);
+    case 2: return launch_export_nb<2>(nb, grid, blk, s, e);
+    default: return launch_export_nb<3>(nb, grid, blk, s, e);
+  }
+}
+
+}  // namespace cbm

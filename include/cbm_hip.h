@@ -1,0 +1,258 @@
+/*
+ * cbm_hip.h -- C ABI of libcbm_hip.so, the MI355X (gfx950) per-step agent
+ * engine behind the CellBasedModels.jl `ABM` / `Community` / `evolve!` API.
+ *
+ * Everything here is plain C: integers, doubles and pointers.  No HIP or
+ * torch type crosses the boundary.  A host binding (the Julia `ccall`
+ * stubs, or the Python ctypes mirror in cellbasedmodels.jl_amd/cbm) calls
+ * these entry points; the library owns every device buffer, its HIP stream
+ * and its compiled code objects.
+ *
+ * Conventions
+ *  - Return value 0 (CBM_OK) on success, a negative CBM_ERR_* on failure; the
+ *    message is available from cbm_last_error(ctx).  The reference raises a
+ *    Julia `error()` in the same situations
+ *    (src/CommunityStructure/communityStructure.jl:122-124,718-722).
+ *  - Agent data crosses the boundary in the reference's *logical slot order*
+ *    (slot s = Julia index s+1 of `com.x`, `com.id`, ...).  Internally the
+ *    library keeps agents sorted by grid cell; that permutation never leaks.
+ *  - Calls on one context are serialised on its stream and are asynchronous
+ *    except cbm_download*, cbm_get_*, cbm_sync and cbm_adjoint, which block.
+ *  - FP64 state, int64 ids, int32 indices inside the library.
+ */
+#ifndef CBM_HIP_H
+#define CBM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum cbm_status {
+  CBM_OK = 0,
+  CBM_ERR_CONFIG = -1,        /* invalid model / community description      */
+  CBM_ERR_CAPACITY = -2,      /* more agents than n_max (reference D12 drops them silently) */
+  CBM_ERR_LIST_OVERFLOW = -3, /* a Verlet list needed more than max_neighbors entries */
+  CBM_ERR_COMPILE = -4,       /* hipRTC rejected a user interaction body    */
+  CBM_ERR_HIP = -5,           /* HIP runtime error                          */
+  CBM_ERR_ARG = -6,           /* bad argument (field id, size, sim index)   */
+  CBM_ERR_NONFINITE = -7      /* NaN/Inf found in agent state                */
+};
+
+/* ---------------------------------------------------------- integrators */
+/* src/integrators.jl:465 (Euler), :516 (Heun), :583 (RungeKutta4), :683 (EM) */
+enum cbm_integrator { CBM_EULER = 0, CBM_HEUN = 1, CBM_RK4 = 2, CBM_EM = 3 };
+
+/* ------------------------------------------------------ neighbor engines */
+/* src/neighbors.jl:27 (Full), :196 (VerletTime), :310 (VerletDisplacement),
+ * :587 (CellLinked), :997 (CLVD) */
+enum cbm_neighbors {
+  CBM_FULL = 0,
+  CBM_CELL_LINKED = 1,
+  CBM_VERLET_DISPLACEMENT = 2,
+  CBM_VERLET_TIME = 3,
+  CBM_CLVD = 4
+};
+
+/* ------------------------------------------------------ interaction laws */
+/* Built-in agentODE interaction bodies (the soft-sphere idioms of
+ * src/models/softSpheres.jl and docs/src/Aggregation.md:62-123).  Each is a
+ * pair force F_ij = s(d) * (x_i - x_j)/d summed over neighbors.
+ *   CBM_LAW_LINEAR       p = {k, c}:            s = k (c - d)           for d < c
+ *   CBM_LAW_REP_ADH      p = {k, r_eq, eps, f}: c = 2 r_eq, dc = f r_eq
+ *                          s = k (c - d)                              d < c
+ *                          s = -eps 4 (d - c)(dc - d)/((dc - c)(dc - c))  c <= d < dc
+ *   CBM_LAW_AGGREGATION  p = {rRep, fRep, rAtr, fAtr, walls}:
+ *                          s = fRep (rRep - d)  d < rRep;  s = -fAtr (rAtr - d)  d < rAtr
+ *                          plus the box-wall push fRep within rRep/2 of a wall
+ *   CBM_LAW_CUSTOM       user body compiled with hipRTC (cbm_compile)          */
+enum cbm_law {
+  CBM_LAW_NONE = 0,
+  CBM_LAW_LINEAR = 1,
+  CBM_LAW_REP_ADH = 2,
+  CBM_LAW_AGGREGATION = 3,
+  CBM_LAW_CUSTOM = 4
+};
+
+/* ------------------------------------------------------------ agent rules */
+/* Built-in agentRule bodies (src/AgentStructure/functionRule.jl:9-111).
+ *   CBM_RULE_GROWTH_DIVISION rule_p = {g_mean, g_spread, split, ratio}:
+ *     vol grows by dvol/dt = growth*vol (integrated with the agent DE);
+ *     when vol >= ratio*vbirth the agent divides: parent and one new agent
+ *     are displaced -/+ split along a uniform random axis, each gets vol/2,
+ *     vbirth = vol/2 and a fresh growth ~ U[g_mean(1-g_spread), g_mean(1+g_spread)].
+ *   CBM_RULE_PARITY  the reference add/remove known-answer rule
+ *     (test/testCommunity.jl:241-316): x += 1; odd id -> @removeAgent();
+ *     even id -> @addAgent(lf = id).
+ *   CBM_RULE_DEATH   rule_p = {rate}: each agent is removed with probability
+ *     rate*dt per step (Philox uniform keyed by seed, step, id).            */
+enum cbm_rule {
+  CBM_RULE_NONE = 0,
+  CBM_RULE_GROWTH_DIVISION = 1,
+  CBM_RULE_PARITY = 2,
+  CBM_RULE_DEATH = 3
+};
+
+/* ------------------------------------------------------ per-agent fields */
+/* Field ids for cbm_upload / cbm_download.  Doubles unless marked int64. */
+enum cbm_field {
+  CBM_F_X = 0, CBM_F_Y = 1, CBM_F_Z = 2,       /* positions (DE variables)    */
+  CBM_F_ID = 3,                                /* int64, 1-based like `id`    */
+  CBM_F_FX = 4, CBM_F_FY = 5, CBM_F_FZ = 6,    /* committed force accumulator (lagged mode) */
+  CBM_F_VOL = 7, CBM_F_GROWTH = 8, CBM_F_VBIRTH = 9,
+  CBM_F_LF = 10,                               /* int64 payload (`lf` of the add/remove KAT) */
+  CBM_F_COUNT = 11
+};
+
+/* --------------------------------------------------- model description */
+/* What `ABM(...)` + the neighbor/integrator constructors fix for a model.
+ * Replaces ABM(dims; agentODE, agentSDE, agentRule, neighborsAlg, agentAlg)
+ * (src/AgentStructure/agentStructure.jl:131-304). */
+typedef struct cbm_model_desc {
+  int32_t dims;          /* 1, 2 or 3                                        */
+  int32_t integrator;    /* enum cbm_integrator                              */
+  int32_t neighbors;     /* enum cbm_neighbors                               */
+  int32_t law;           /* enum cbm_law                                     */
+  int32_t rule;          /* enum cbm_rule                                    */
+  int32_t periodic;      /* 1: periodic box, minimum-image distances         */
+  int32_t lagged;        /* 1: dt(x) reads the previous step's committed force
+                            accumulator (reference read-old/write-new idiom,
+                            src/AgentStructure/agentStructure.jl:352-453)     */
+  int32_t max_neighbors; /* Verlet list capacity (nMaxNeighbors)             */
+  double dt;
+  double cell_edge;      /* CellLinked / CLVD cellEdge                       */
+  double skin;           /* Verlet list radius (the reference's `skin`)      */
+  double rebuild_disp;   /* displacement that triggers a Verlet rebuild;
+                            <= 0 selects the reference rule skin/2
+                            (src/neighbors.jl:403)                          */
+  double dt_recompute;   /* VerletTime dtNeighborRecompute                   */
+  double drift;          /* agentODE linear term: dt(x) += -drift*x          */
+  double noise;          /* agentSDE diagonal term g (EM only)               */
+  double rule_p[4];      /* rule parameters (see enum cbm_rule)              */
+  uint64_t seed;         /* Philox4x32-10 key for all in-kernel randomness   */
+  /* medium (explicit FTCS diffusion of one field on a cell-centred grid) */
+  int32_t medium_n[3];   /* NMedium; all zero = no medium                    */
+  int32_t medium_pad;
+  double medium_D;       /* diffusion coefficient                            */
+  double medium_consume; /* uptake rate per agent in a node: dc/dt -= rate*c*n */
+  double medium_c0;      /* initial concentration                            */
+} cbm_model_desc;
+
+/* Per-simulation description.  One context may hold many independent
+ * simulations (the batched sweep); each has its own agents, box and law
+ * parameters.  Replaces Community(abm; N, simBox, ...)
+ * (src/CommunityStructure/communityStructure.jl:108-187). */
+typedef struct cbm_sim_desc {
+  int64_t n;             /* initial agent count N                            */
+  int64_t n_max;         /* capacity (N + preallocateAgents)                 */
+  double box_lo[3];      /* simBox[:,1]                                      */
+  double box_hi[3];      /* simBox[:,2]                                      */
+  double p[8];           /* law parameters (see enum cbm_law)                */
+} cbm_sim_desc;
+
+typedef struct cbm_ctx cbm_ctx;
+
+/* ------------------------------------------------------------ lifecycle */
+/* ABM(...) + Community(...) + loadToPlatform! allocation
+ * (src/CommunityStructure/communityStructure.jl:468-655). */
+int cbm_create(cbm_ctx** out, const cbm_model_desc* model, int32_t n_sims,
+               const cbm_sim_desc* sims);
+void cbm_destroy(cbm_ctx* ctx);
+const char* cbm_last_error(const cbm_ctx* ctx);
+/* Library version string and device name (for records). */
+const char* cbm_version(void);
+
+/* compileABM! / Main.eval (src/AgentStructure/functionDE.jl:110): compile a
+ * user interaction body (HIP C++, see cbm_skeleton.h for the contract) with
+ * hipRTC and use it instead of the built-in law (model law CBM_LAW_CUSTOM). */
+int cbm_compile(cbm_ctx* ctx, const char* pair_body_hip);
+
+/* ------------------------------------------------------- data transfer */
+/* loadToPlatform! / bringFromPlatform! (communityStructure.jl:606-711).
+ * `n` values in logical slot order for simulation `sim`.  Uploading CBM_F_X
+ * of simulation `sim` does not change N; N is set at creation or with
+ * cbm_set_n.  Int64 fields take/return int64_t, the others double. */
+int cbm_upload(cbm_ctx* ctx, int32_t field, const void* host, int64_t n, int32_t sim);
+int cbm_download(cbm_ctx* ctx, int32_t field, void* host, int64_t n, int32_t sim);
+int cbm_set_n(cbm_ctx* ctx, int32_t sim, int64_t n);
+/* Medium field download/upload (row-major x fastest), sim 0 only. */
+int cbm_upload_medium(cbm_ctx* ctx, const double* host, int64_t n);
+int cbm_download_medium(cbm_ctx* ctx, double* host, int64_t n);
+
+/* -------------------------------------------------------------- stepping */
+/* computeNeighbors! (src/neighbors.jl:18): (re)build the neighbor
+ * structures from the current positions.  Called once after upload. */
+int cbm_neighbors(cbm_ctx* ctx);
+/* step! x nsteps (src/CommunityStructure/step.jl:96-107): rule -> DE stages
+ * -> medium -> commit/compaction -> neighbor update, all device resident. */
+int cbm_step(cbm_ctx* ctx, int64_t nsteps);
+/* Record nsteps of the step loop into a HIP graph and replay it `replays`
+ * times (the evolve! loop without per-step host work). */
+int cbm_step_graph(cbm_ctx* ctx, int64_t nsteps, int64_t replays);
+int cbm_sync(cbm_ctx* ctx);
+
+/* ------------------------------------------------------------- queries */
+int cbm_get_n(cbm_ctx* ctx, int32_t sim, int64_t* n);        /* com.N      */
+int cbm_get_t(cbm_ctx* ctx, double* t, int64_t* step);      /* com.t      */
+int cbm_get_id_max(cbm_ctx* ctx, int32_t sim, int64_t* id_max); /* idMax_ */
+/* Number of Verlet rebuilds (or CL binnings) performed so far. */
+int cbm_get_rebuilds(cbm_ctx* ctx, int64_t* rebuilds);
+/* Neighbor lists of simulation `sim` in logical slots: counts[n] and
+ * lists[n*cap] (row i holds the slots of agent i's list, unordered).  For
+ * CellLinked/Full the "list" is the candidate set within `radius` of the
+ * current structure.  Returns CBM_ERR_LIST_OVERFLOW if cap is too small. */
+int cbm_get_neighbor_lists(cbm_ctx* ctx, int32_t sim, double radius,
+                           int32_t* counts, int64_t* lists, int64_t cap);
+/* The neighbor-count agentRule of the reference KAT
+ * (test/testCommunity.jl:361-415): out[i] = #{j in structure : d_ij < radius}.
+ * For Verlet models, radius <= 0 returns the list lengths instead. */
+int cbm_count_neighbors(cbm_ctx* ctx, int32_t sim, double radius, int64_t* out);
+
+/* --------------------------------------------------------------- adjoint */
+/* Gradient of a trajectory loss through the overdamped-Euler mechanics step
+ * (no reference counterpart: the reference has no AD).  The loss is
+ *   L = sum_{m} mean_i |x_i(t_m) - x*_i(m)|^2      (mean over agents and
+ * save points is taken as 1/(n_saves*N) sum ...), for the REP_ADH law with
+ * parameters (k, r_eq, eps) of every simulation.  target holds n_saves
+ * snapshots of dims*N doubles (SoA per snapshot: x[N], y[N], z[N]) per sim,
+ * concatenated over sims.  Checkpoints every `checkpoint_every` steps are
+ * kept on device and the steps between them recomputed in the backward pass.
+ * grad[3*s + {0,1,2}] = dL_s/d(k, r_eq, eps); loss[s] = L_s. */
+typedef struct cbm_loss_desc {
+  int64_t n_steps;
+  int64_t checkpoint_every;
+  int32_t n_saves;
+  int32_t pad;
+  const int64_t* save_steps;   /* n_saves step indices in (0, n_steps]     */
+  const double* target;        /* see above                                */
+} cbm_loss_desc;
+int cbm_adjoint(cbm_ctx* ctx, const cbm_loss_desc* loss_desc, double* grad,
+                double* loss);
+
+/* -------------------------------------------------------------- autotune */
+/* Pick block size (and, for Verlet models, list radius margin) by timing a
+ * few steps; the choice is cached per (model, device) in `cache_path` (may
+ * be NULL).  Writes the chosen block size to *block. */
+int cbm_autotune(cbm_ctx* ctx, const char* cache_path, int32_t* block);
+int cbm_set_block(cbm_ctx* ctx, int32_t block);
+
+/* ------------------------------------------------------------ profiling */
+/* Run nsteps steps with HIP events around every launch on the context
+ * stream (no reference counterpart: the measurement hook of the benchmark).
+ * avg_ms[k] / launches[k] for k = 0..7: the integrator stage kernels
+ * (0 Euler, 1 EM, 2 Heun stage 1, 3 Heun stage 2, 4-7 RK4 stages 1-4);
+ * k = 8: the neighbor update of one step (total, launches = steps);
+ * k = 9: whole step.  ncat must be >= 10. */
+int cbm_profile(cbm_ctx* ctx, int64_t nsteps, double* avg_ms, int64_t* launches, int32_t ncat);
+/* Average device time (ms) of the last cbm_step call's force kernels, from
+ * HIP events on the context stream; *launches = force-kernel launches timed. */
+int cbm_last_timing(cbm_ctx* ctx, double* ms_total, double* ms_force,
+                    int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CBM_HIP_H */

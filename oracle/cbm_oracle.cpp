@@ -1,0 +1,872 @@
+// cbm_oracle.cpp -- CPU restatement of the CellBasedModels.jl per-step agent
+// update.  TEST INFRASTRUCTURE ONLY: this is the parity checker for the HIP
+// library (tests/, __graft_entry__.smoke and bench.py's cpu_baseline leg);
+// no product code path may load or call it.
+//
+// Everything is kept in the reference's logical slot order and follows the
+// reference's serial CPU algorithms:
+//   cell binning / counting sort      src/neighbors.jl:587-979
+//   cellPos / cellPosNeigh            src/neighbors.jl:636-708
+//   Full / Verlet list build          src/neighbors.jl:27-57, 92-165
+//   Verlet displacement trigger       src/neighbors.jl:367-570
+//   VerletTime trigger                src/neighbors.jl:240-281
+//   euclidean                         src/metrics.jl:34-36
+//   cellInMesh                        src/metrics.jl:18-20
+//   Euler/Heun/RK4/EM                 src/integrators.jl:442-734
+//   read-old / write-new (lag)        src/AgentStructure/agentStructure.jl:352-453
+//   @addAgent / @removeAgent          src/AgentStructure/functionRule.jl:9-111
+//   update! compaction + commit       src/CommunityStructure/update.jl:15-303
+//   step! order                       src/CommunityStructure/step.jl:96-107
+//   @∂2 medium stencil                src/AgentStructure/macros.jl:203-257
+// with the spec decisions the GPU path shares (exact floor quotients,
+// periodic minimum image, canonical ascending compaction order, Philox noise,
+// zero-flux ghost-mirror medium, d>0 guard on pair laws).
+//
+// Compiled with -ffp-contract=off: every expression below is evaluated in
+// the written order with IEEE double rounding.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <cstdio>
+#include <vector>
+#include <algorithm>
+#include <string>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "../include/cbm_hip.h"
+
+namespace {
+
+// ------------------------------------------------------------------ Philox
+// Philox4x32-10, Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3" (SC'11).  Multipliers and Weyl increments from the paper.
+struct U4 { uint32_t v[4]; };
+static inline void mulhilo(uint32_t a, uint32_t b, uint32_t& hi, uint32_t& lo) {
+  uint64_t p = (uint64_t)a * (uint64_t)b;
+  hi = (uint32_t)(p >> 32); lo = (uint32_t)p;
+}
+static U4 philox(U4 c, uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; ++r) {
+    if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    uint32_t hi0, lo0, hi1, lo1;
+    mulhilo(0xD2511F53u, c.v[0], hi0, lo0);
+    mulhilo(0xCD9E8D57u, c.v[2], hi1, lo1);
+    U4 o;
+    o.v[0] = hi1 ^ c.v[1] ^ k0; o.v[1] = lo1;
+    o.v[2] = hi0 ^ c.v[3] ^ k1; o.v[3] = lo0;
+    c = o;
+  }
+  return c;
+}
+// 53-bit uniform in [0,1) from two 32-bit words.
+static inline double u53(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+static U4 draw(uint64_t seed, int64_t step, int64_t id, int32_t sim, uint32_t purpose) {
+  U4 c; c.v[0] = (uint32_t)step; c.v[1] = (uint32_t)id; c.v[2] = (uint32_t)sim; c.v[3] = purpose;
+  return philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+// Box-Muller (Box & Muller 1958) on one Philox block -> two N(0,1).
+static void normal2(const U4& r, double& z0, double& z1) {
+  double u1 = 1.0 - u53(r.v[0], r.v[1]);
+  double u2 = u53(r.v[2], r.v[3]);
+  double rad = std::sqrt(-2.0 * std::log(u1));
+  double th = 6.283185307179586 * u2;
+  z0 = rad * std::cos(th);
+  z1 = rad * std::sin(th);
+}
+
+// floor(a/b) of the exact real quotient, a >= 0, b > 0 (the reference's
+// Int(a ÷ b) of src/neighbors.jl:636 and src/metrics.jl:19).
+static inline int64_t floor_div(double a, double b) {
+  double k = std::floor(a / b);
+  double r = std::fma(-k, b, a);
+  if (r < 0.0) k -= 1.0; else if (r >= b) k += 1.0;
+  return (int64_t)k;
+}
+
+// ------------------------------------------------------------ dual numbers
+// Forward-mode dual number with 3 tangents (k, r_eq, eps), used only for the
+// adjoint cross-check (the reference Manifest pins ForwardDiff 0.10.36,
+// which is what this stands in for).
+struct D3 {
+  double v, g[3];
+  D3() : v(0) { g[0] = g[1] = g[2] = 0; }
+  D3(double x) : v(x) { g[0] = g[1] = g[2] = 0; }
+};
+static inline D3 operator+(const D3& a, const D3& b) { D3 r; r.v = a.v + b.v; for (int i = 0; i < 3; ++i) r.g[i] = a.g[i] + b.g[i]; return r; }
+static inline D3 operator-(const D3& a, const D3& b) { D3 r; r.v = a.v - b.v; for (int i = 0; i < 3; ++i) r.g[i] = a.g[i] - b.g[i]; return r; }
+static inline D3 operator*(const D3& a, const D3& b) { D3 r; r.v = a.v * b.v; for (int i = 0; i < 3; ++i) r.g[i] = a.g[i] * b.v + a.v * b.g[i]; return r; }
+static inline D3 operator/(const D3& a, const D3& b) { D3 r; r.v = a.v / b.v; for (int i = 0; i < 3; ++i) r.g[i] = (a.g[i] * b.v - a.v * b.g[i]) / (b.v * b.v); return r; }
+static inline D3 sqrt(const D3& a) { D3 r; r.v = std::sqrt(a.v); for (int i = 0; i < 3; ++i) r.g[i] = a.g[i] / (2.0 * r.v); return r; }
+static inline double val(const D3& a) { return a.v; }
+static inline double val(double a) { return a; }
+using std::sqrt;
+
+// ------------------------------------------------------------------- model
+struct Grid {
+  int dims = 1;
+  bool periodic = false;
+  int64_t n[3] = {1, 1, 1};
+  double edge[3] = {1, 1, 1};
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  int64_t total() const { return n[0] * n[1] * n[2]; }
+  // cellPos, src/neighbors.jl:636-638 (padded, clamped) or the periodic grid.
+  bool geometric = false;   // Verlet builds: bin by position, then clamp
+  int64_t axis_cell(int a, double x) const {
+    if (geometric) {
+      if (x < lo[a]) return 0;
+      int64_t c = floor_div(x - lo[a], edge[a]) + (periodic ? 0 : 1);
+      return c > n[a] - 1 ? n[a] - 1 : c;
+    }
+    if (periodic) {
+      int64_t c = floor_div(x - lo[a], edge[a]);
+      if (c < 0) c = 0;
+      if (c > n[a] - 1) c = n[a] - 1;
+      return c;
+    }
+    if (x > hi[a]) return n[a] - 1;
+    if (x < lo[a]) return 0;
+    return floor_div(x - lo[a], edge[a]) + 1;
+  }
+};
+
+struct Orc {
+  cbm_model_desc m;
+  cbm_sim_desc s;
+  int dims = 1;
+  int64_t N = 0, nMax = 0, idMax = 0, step = 0;
+  double t = 0;
+  bool growth = false;
+  std::vector<double> X[3], F[3], Fnew[3];
+  std::vector<double> vol, gr, vb, grNew, vbNew;
+  std::vector<int64_t> id, lf;
+  std::vector<char> alive;
+  // neighbor structures
+  Grid g;
+  std::vector<int64_t> cellOf, cellStart, cellCount, cellAgents;
+  std::vector<int32_t> nbN;
+  std::vector<int64_t> nbList;
+  std::vector<double> Xref[3];
+  double tNext = 0;
+  int64_t rebuilds = 0;
+  bool structure_valid = false;
+  // medium
+  int64_t mn[3] = {1, 1, 1};
+  double mdx[3] = {1, 1, 1};
+  std::vector<double> c, cnew;
+  std::vector<int32_t> nodeCount;
+  int err = 0;
+  std::string msg;
+
+  bool is_verlet() const {
+    return m.neighbors == CBM_VERLET_DISPLACEMENT || m.neighbors == CBM_VERLET_TIME || m.neighbors == CBM_CLVD;
+  }
+  double rebuild_thr() const { return m.rebuild_disp > 0 ? m.rebuild_disp : m.skin / 2; }
+
+  // Minimum-image component (periodic boxes only).
+  inline double mi(int a, double d) const {
+    if (!m.periodic) return d;
+    double L = s.box_hi[a] - s.box_lo[a];
+    if (d > 0.5 * L) d -= L; else if (d < -0.5 * L) d += L;
+    return d;
+  }
+  template <class T> inline T mi(int a, T d) const {
+    if (!m.periodic) return d;
+    double L = s.box_hi[a] - s.box_lo[a];
+    if (val(d) > 0.5 * L) d = d - T(L); else if (val(d) < -0.5 * L) d = d + T(L);
+    return d;
+  }
+};
+
+static void set_grid(Orc& o, Grid& g, double edge) {
+  g.dims = o.dims;
+  g.periodic = o.m.periodic != 0;
+  for (int a = 0; a < 3; ++a) { g.n[a] = 1; g.edge[a] = 1; g.lo[a] = 0; g.hi[a] = 0; }
+  for (int a = 0; a < o.dims; ++a) {
+    g.lo[a] = o.s.box_lo[a]; g.hi[a] = o.s.box_hi[a];
+    double L = g.hi[a] - g.lo[a];
+    if (g.periodic) {
+      g.n[a] = (int64_t)std::floor(L / edge);
+      if (g.n[a] < 1) g.n[a] = 1;
+      g.edge[a] = L / (double)g.n[a];
+    } else {
+      // nCells_ = ceil(L/edge + 2), src/neighbors.jl:604
+      g.n[a] = (int64_t)std::ceil(L / edge + 2.0);
+      g.edge[a] = edge;
+    }
+  }
+}
+
+static int64_t cell_of(const Orc& o, const Grid& g, int64_t i) {
+  int64_t c[3] = {0, 0, 0};
+  for (int a = 0; a < o.dims; ++a) c[a] = g.axis_cell(a, o.X[a][i]);
+  return c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
+}
+
+// neighborsCellLinked{d}CPU!: zero counts, assignCells (serial), cumsum,
+// sortAgentsInCells (serial, ascending agent index within a cell).
+static void bin(Orc& o, const Grid& g) {
+  int64_t nc = g.total();
+  o.cellCount.assign(nc, 0);
+  o.cellStart.assign(nc + 1, 0);
+  o.cellOf.assign(o.N, 0);
+  o.cellAgents.assign(o.N, 0);
+  for (int64_t i = 0; i < o.N; ++i) { o.cellOf[i] = cell_of(o, g, i); o.cellCount[o.cellOf[i]]++; }
+  for (int64_t k = 0; k < nc; ++k) o.cellStart[k + 1] = o.cellStart[k] + o.cellCount[k];
+  std::vector<int64_t> cur(o.cellStart.begin(), o.cellStart.end() - 1);
+  for (int64_t i = 0; i < o.N; ++i) o.cellAgents[cur[o.cellOf[i]]++] = i;
+}
+
+// cellPosNeigh, src/neighbors.jl:658-708 (0-based, x fastest; the correct
+// nX stride in 2-D -- the reference's py*nY of :683 only differs when nX!=nY)
+// or the periodic wrap.  Returns -1 outside a non-periodic grid.
+static int64_t neigh_cell(const Grid& g, int64_t cell, int k) {
+  int64_t p[3];
+  p[0] = cell % g.n[0];
+  p[1] = (cell / g.n[0]) % g.n[1];
+  p[2] = cell / (g.n[0] * g.n[1]);
+  int off[3] = {k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1};
+  for (int a = 0; a < 3; ++a) {
+    if (a >= g.dims) { off[a] = 0; continue; }
+    int64_t q = p[a] + off[a];
+    if (g.periodic) { q = (q + g.n[a]) % g.n[a]; }
+    else if (q < 0 || q >= g.n[a]) return -1;
+    p[a] = q;
+  }
+  return p[0] + g.n[0] * (p[1] + g.n[1] * p[2]);
+}
+
+static int n_neigh_cells(const Orc& o) { int r = 1; for (int a = 0; a < o.dims; ++a) r *= 3; return r; }
+
+// Enumerate the candidate neighbors of agent i in the order the reference
+// loops visit them (neighborsLoop, src/neighbors.jl:65-76, 715-747, 37).
+template <class Fn>
+static void for_candidates(const Orc& o, int64_t i, Fn fn) {
+  if (o.m.neighbors == CBM_FULL) {
+    for (int64_t j = 0; j < o.N; ++j) if (j != i) fn(j);
+  } else if (o.m.neighbors == CBM_CELL_LINKED) {
+    int nk = n_neigh_cells(o);
+    for (int k = 0; k < nk; ++k) {
+      int64_t c = neigh_cell(o.g, o.cellOf[i], k);
+      if (c < 0) continue;
+      for (int64_t q = o.cellStart[c]; q < o.cellStart[c + 1]; ++q) {
+        int64_t j = o.cellAgents[q];
+        if (j != i) fn(j);
+      }
+    }
+  } else {
+    int32_t cnt = o.nbN[i];
+    for (int32_t q = 0; q < cnt; ++q) fn(o.nbList[(int64_t)q * o.nMax + i]);
+  }
+}
+
+template <class T>
+static inline T dist(const Orc& o, const T* dx) {
+  // euclidean, src/metrics.jl:34-36: sqrt((x1-x2)^2+(y1-y2)^2+(z1-z2)^2)
+  T s = dx[0] * dx[0];
+  if (o.dims > 1) s = s + dx[1] * dx[1];
+  if (o.dims > 2) s = s + dx[2] * dx[2];
+  return sqrt(s);
+}
+
+// Pair magnitude s(d) of the built-in laws; returns false when the pair does
+// not interact.  p = law parameters (possibly duals).
+template <class T>
+static inline bool law_s(int law, const T* p, const T& d, T& s) {
+  if (!(val(d) > 0.0)) return false;
+  if (law == CBM_LAW_LINEAR) {
+    if (val(d) < val(p[1])) { s = p[0] * (p[1] - d); return true; }
+    return false;
+  }
+  if (law == CBM_LAW_REP_ADH) {
+    T c = T(2.0) * p[1];
+    T dc = p[3] * p[1];
+    if (val(d) < val(c)) { s = p[0] * (c - d); return true; }
+    if (val(d) < val(dc)) {
+      T w = dc - c;
+      s = T(0.0) - p[2] * T(4.0) * (d - c) * (dc - d) / (w * w);
+      return true;
+    }
+    return false;
+  }
+  if (law == CBM_LAW_AGGREGATION) {
+    if (val(d) < val(p[0])) { s = p[1] * (p[0] - d); return true; }
+    if (val(d) < val(p[2])) { s = T(0.0) - p[3] * (p[2] - d); return true; }
+    return false;
+  }
+  return false;
+}
+
+// The agentODE body evaluated on stage state S for agent i: returns the
+// derivative du (positions) and the new force accumulator acc.
+static void deriv(const Orc& o, int64_t i, const std::vector<double>* S, double* du, double* acc) {
+  double a3[3] = {0, 0, 0};
+  const double* p = o.s.p;
+  int law = o.m.law;
+  if (law != CBM_LAW_NONE) {
+    for_candidates(o, i, [&](int64_t j) {
+      double dx[3] = {0, 0, 0};
+      for (int a = 0; a < o.dims; ++a) dx[a] = o.mi(a, S[a][i] - S[a][j]);
+      double d = dist(o, dx);
+      double sv;
+      if (law_s(law, p, d, sv))
+        for (int a = 0; a < o.dims; ++a) a3[a] = a3[a] + sv * (dx[a] / d);
+    });
+    if (law == CBM_LAW_AGGREGATION && p[4] != 0.0) {
+      // wall push of docs/src/Aggregation.md:100-112
+      for (int a = 0; a < o.dims; ++a) {
+        double x = S[a][i];
+        if (x < o.s.box_lo[a] + p[0] / 2) a3[a] = a3[a] + p[1];
+        else if (x > o.s.box_hi[a] - p[0] / 2) a3[a] = a3[a] - p[1];
+      }
+    }
+  }
+  for (int a = 0; a < o.dims; ++a) {
+    acc[a] = a3[a];
+    double v = o.m.lagged ? o.F[a][i] : a3[a];
+    if (o.m.drift != 0.0) v = v - o.m.drift * S[a][i];
+    du[a] = v;
+  }
+}
+
+// ---------------------------------------------------------------- Verlet
+static void build_verlet(Orc& o) {
+  int cap = o.m.max_neighbors;
+  o.nbN.assign(o.nMax, 0);
+  o.nbList.assign((int64_t)cap * o.nMax, -1);
+  double rl = o.m.skin;
+  // Candidate enumeration through a scratch grid of edge >= skin; lists are
+  // then ordered by ascending slot, identical to the reference brute force
+  // `for i2_ in 1:N` (src/neighbors.jl:114-126).
+  // The reference's cellPos clamps every out-of-box agent into the outermost
+  // padding cell, which can separate an agent just outside the box from a
+  // neighbor two cells away; the list build therefore bins geometrically
+  // (then clamps), which keeps every pair within rl in adjacent cells.
+  Grid g; set_grid(o, g, std::max(rl, 1e-300));
+  g.geometric = true;
+  bool brute = (o.N < 64);
+  for (int a = 0; a < o.dims; ++a) if (g.n[a] < 3) brute = true;
+  std::vector<int64_t> cOf, cSt, cAg;
+  if (!brute) {
+    int64_t nc = g.total();
+    std::vector<int64_t> cnt(nc, 0);
+    cOf.assign(o.N, 0); cSt.assign(nc + 1, 0); cAg.assign(o.N, 0);
+    for (int64_t i = 0; i < o.N; ++i) { cOf[i] = cell_of(o, g, i); cnt[cOf[i]]++; }
+    for (int64_t k = 0; k < nc; ++k) cSt[k + 1] = cSt[k] + cnt[k];
+    std::vector<int64_t> cur(cSt.begin(), cSt.end() - 1);
+    for (int64_t i = 0; i < o.N; ++i) cAg[cur[cOf[i]]++] = i;
+  }
+  int nk = n_neigh_cells(o);
+  int overflow = 0;
+#pragma omp parallel for schedule(static) reduction(| : overflow)
+  for (int64_t i = 0; i < o.N; ++i) {
+    std::vector<int64_t> cand;
+    auto test = [&](int64_t j) {
+      if (j == i) return;
+      double dx[3] = {0, 0, 0};
+      for (int a = 0; a < o.dims; ++a) dx[a] = o.mi(a, o.X[a][i] - o.X[a][j]);
+      if (dist(o, dx) < rl) cand.push_back(j);
+    };
+    if (brute) { for (int64_t j = 0; j < o.N; ++j) test(j); }
+    else {
+      for (int k = 0; k < nk; ++k) {
+        int64_t c = neigh_cell(g, cOf[i], k);
+        if (c < 0) continue;
+        for (int64_t q = cSt[c]; q < cSt[c + 1]; ++q) test(cAg[q]);
+      }
+      std::sort(cand.begin(), cand.end());
+      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    }
+    if ((int64_t)cand.size() > cap) { overflow = 1; cand.resize(cap); }
+    o.nbN[i] = (int32_t)cand.size();
+    for (size_t q = 0; q < cand.size(); ++q) o.nbList[(int64_t)q * o.nMax + i] = cand[q];
+  }
+  if (overflow) { o.err = CBM_ERR_LIST_OVERFLOW; o.msg = "Verlet list overflow"; }
+  for (int a = 0; a < o.dims; ++a)
+    for (int64_t i = 0; i < o.N; ++i) o.Xref[a][i] = o.X[a][i];
+  o.rebuilds++;
+}
+
+static void compute_neighbors(Orc& o, bool force) {
+  if (o.m.neighbors == CBM_FULL) { o.structure_valid = true; return; }
+  if (o.m.neighbors == CBM_CELL_LINKED) { bin(o, o.g); o.rebuilds++; o.structure_valid = true; return; }
+  bool rebuild = force || !o.structure_valid;
+  if (!rebuild) {
+    if (o.m.neighbors == CBM_VERLET_TIME) {
+      rebuild = (o.tNext <= o.t);   // src/neighbors.jl:255
+    } else {
+      double thr = o.rebuild_thr();
+      for (int64_t i = 0; i < o.N && !rebuild; ++i) {
+        double dx[3] = {0, 0, 0};
+        for (int a = 0; a < o.dims; ++a) dx[a] = o.mi(a, o.X[a][i] - o.Xref[a][i]);
+        if (dist(o, dx) >= thr) rebuild = true;   // src/neighbors.jl:403
+      }
+    }
+  }
+  if (rebuild) {
+    build_verlet(o);
+    if (o.m.neighbors == CBM_VERLET_TIME) o.tNext = o.t + o.m.dt_recompute;
+  }
+  o.structure_valid = true;
+}
+
+// --------------------------------------------------------------- medium
+static void medium_step(Orc& o, int64_t nAgents) {
+  int64_t nn = o.mn[0] * o.mn[1] * o.mn[2];
+  o.nodeCount.assign(nn, 0);
+  for (int64_t i = 0; i < nAgents; ++i) {
+    int64_t q[3] = {0, 0, 0};
+    for (int a = 0; a < o.dims; ++a) {
+      double x = o.X[a][i];
+      // cellInMesh, src/metrics.jl:18-20 (0-based; x == xMax clamped)
+      int64_t v;
+      if (x > o.s.box_hi[a]) v = o.mn[a] - 1;
+      else if (x < o.s.box_lo[a]) v = 0;
+      else v = floor_div(x - o.s.box_lo[a], o.mdx[a]);
+      if (v > o.mn[a] - 1) v = o.mn[a] - 1;
+      q[a] = v;
+    }
+    o.nodeCount[q[0] + o.mn[0] * (q[1] + o.mn[1] * q[2])]++;
+  }
+  o.cnew.resize(nn);
+  const double D = o.m.medium_D, dt = o.m.dt, u = o.m.medium_consume;
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < nn; ++k) {
+    int64_t p[3] = {k % o.mn[0], (k / o.mn[0]) % o.mn[1], k / (o.mn[0] * o.mn[1])};
+    double cc = o.c[k];
+    double lap = 0.0;
+    int64_t stride[3] = {1, o.mn[0], o.mn[0] * o.mn[1]};
+    for (int a = 0; a < o.dims; ++a) {
+      // zero-flux ghost mirror: c[-1] = c[0], c[n] = c[n-1]
+      double cp = (p[a] + 1 < o.mn[a]) ? o.c[k + stride[a]] : cc;
+      double cm = (p[a] > 0) ? o.c[k - stride[a]] : cc;
+      double l = ((cp - 2 * cc) + cm) / (o.mdx[a] * o.mdx[a]);   // @∂2, macros.jl:203-257
+      lap = (a == 0) ? l : lap + l;
+    }
+    double f = D * lap - (u * cc) * (double)o.nodeCount[k];
+    o.cnew[k] = cc + dt * f;
+  }
+  o.c.swap(o.cnew);
+}
+
+// ------------------------------------------------------------------ rules
+struct Births { std::vector<int64_t> parent; };
+
+// The built-in agentRule bodies, executed serially in ascending slot order
+// (the single-thread order that makes @removeAgent's hole list ascending).
+static void apply_rule(Orc& o, std::vector<int64_t>& holes, int64_t& nAdd) {
+  holes.clear(); nAdd = 0;
+  if (o.m.rule == CBM_RULE_NONE) return;
+  int64_t N = o.N;
+  for (int64_t i = 0; i < N; ++i) {
+    if (o.m.rule == CBM_RULE_PARITY) {
+      double xold = o.X[0][i];
+      o.X[0][i] = xold + 1.0;
+      if (o.id[i] % 2 == 1) {
+        holes.push_back(i); o.alive[i] = 0;
+      } else if (o.id[i] % 2 == 0) {
+        int64_t nw = N + nAdd;
+        int64_t newId = o.idMax + 1 + nAdd;
+        if (nw >= o.nMax) { o.err = CBM_ERR_CAPACITY; o.msg = "agent capacity exceeded"; continue; }
+        nAdd++;
+        for (int a = 0; a < o.dims; ++a) { o.X[a][nw] = o.X[a][i]; o.F[a][nw] = o.F[a][i]; o.Fnew[a][nw] = o.F[a][i]; }
+        o.X[0][nw] = xold;
+        o.id[nw] = newId; o.lf[nw] = o.id[i]; o.alive[nw] = 1;
+        if (o.growth) { o.vol[nw] = o.vol[i]; o.grNew[nw] = o.gr[i]; o.vbNew[nw] = o.vb[i]; o.gr[nw] = o.gr[i]; o.vb[nw] = o.vb[i]; }
+      }
+    } else if (o.m.rule == CBM_RULE_DEATH) {
+      U4 r = draw(o.m.seed, o.step, o.id[i], 0, 0x400u);
+      if (u53(r.v[0], r.v[1]) < o.m.rule_p[0] * o.m.dt) { holes.push_back(i); o.alive[i] = 0; }
+    } else if (o.m.rule == CBM_RULE_GROWTH_DIVISION) {
+      if (!(o.vol[i] >= o.m.rule_p[3] * o.vb[i])) continue;
+      int64_t nw = N + nAdd;
+      if (nw >= o.nMax) { o.err = CBM_ERR_CAPACITY; o.msg = "agent capacity exceeded"; continue; }
+      int64_t newId = o.idMax + 1 + nAdd;
+      nAdd++;
+      U4 ra = draw(o.m.seed, o.step, o.id[i], 0, 0x200u);
+      U4 rg = draw(o.m.seed, o.step, o.id[i], 0, 0x300u);
+      double ax[3] = {0, 0, 0};
+      if (o.dims == 1) ax[0] = (u53(ra.v[0], ra.v[1]) < 0.5) ? -1.0 : 1.0;
+      else if (o.dims == 2) {
+        double th = 6.283185307179586 * u53(ra.v[0], ra.v[1]);
+        ax[0] = std::cos(th); ax[1] = std::sin(th);
+      } else {
+        double z = 2.0 * u53(ra.v[0], ra.v[1]) - 1.0;
+        double ph = 6.283185307179586 * u53(ra.v[2], ra.v[3]);
+        double rr = std::sqrt(1.0 - z * z);
+        ax[0] = rr * std::cos(ph); ax[1] = rr * std::sin(ph); ax[2] = z;
+      }
+      double split = o.m.rule_p[2];
+      double half = o.vol[i] / 2.0;
+      double gm = o.m.rule_p[0], gs = o.m.rule_p[1];
+      double g1 = gm * (1.0 + gs * (2.0 * u53(rg.v[0], rg.v[1]) - 1.0));
+      double g2 = gm * (1.0 + gs * (2.0 * u53(rg.v[2], rg.v[3]) - 1.0));
+      for (int a = 0; a < o.dims; ++a) {
+        double x = o.X[a][i];
+        o.X[a][nw] = x + split * ax[a];
+        o.X[a][i] = x - split * ax[a];
+        o.F[a][nw] = o.F[a][i]; o.Fnew[a][nw] = o.F[a][i];
+      }
+      o.vol[i] = half; o.vol[nw] = half;
+      o.vbNew[i] = half; o.vbNew[nw] = half; o.vb[nw] = half;
+      o.grNew[i] = g1; o.grNew[nw] = g2; o.gr[nw] = g2;
+      o.id[nw] = newId; o.lf[nw] = o.lf[i]; o.alive[nw] = 1;
+    }
+  }
+}
+
+// update!: canonical compaction (src/CommunityStructure/update.jl:15-119),
+// ascending holes, survivors scanned from the tail.
+static void compact(Orc& o, const std::vector<int64_t>& holes, int64_t nAdd) {
+  int64_t R = (int64_t)holes.size();
+  int64_t top = o.N + nAdd;
+  if (R > 0) {
+    std::vector<int64_t> repos;
+    int64_t last = top - 1;
+    while ((int64_t)repos.size() < R && last >= 0) { if (o.alive[last]) repos.push_back(last); last--; }
+    for (int64_t m = 0; m < (int64_t)repos.size(); ++m) {
+      int64_t dst = holes[m], src = repos[m];
+      if (!(dst < src)) continue;
+      for (int a = 0; a < o.dims; ++a) { o.X[a][dst] = o.X[a][src]; o.F[a][dst] = o.F[a][src]; o.Fnew[a][dst] = o.Fnew[a][src]; }
+      o.id[dst] = o.id[src]; o.lf[dst] = o.lf[src];
+      if (o.growth) { o.vol[dst] = o.vol[src]; o.gr[dst] = o.gr[src]; o.vb[dst] = o.vb[src]; o.grNew[dst] = o.grNew[src]; o.vbNew[dst] = o.vbNew[src]; }
+    }
+  }
+  o.N = top - R;
+  o.idMax += nAdd;
+  for (int64_t i = 0; i < o.N; ++i) o.alive[i] = 1;
+}
+
+// ---------------------------------------------------------------- stepping
+static void de_step(Orc& o, int64_t nA) {
+  const int d = o.dims;
+  const double dt = o.m.dt;
+  const int nv = d + (o.growth ? 1 : 0);     // DE variables: positions (+ vol)
+  std::vector<double> U[4];
+  for (int a = 0; a < d; ++a) U[a] = o.X[a];
+  if (o.growth) U[d] = o.vol;
+  auto eval = [&](const std::vector<double>* S, std::vector<double>* K) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nA; ++i) {
+      double du[3], acc[3];
+      deriv(o, i, S, du, acc);
+      for (int a = 0; a < d; ++a) { K[a][i] = du[a]; o.Fnew[a][i] = acc[a]; }
+      if (o.growth) K[d][i] = o.gr[i] * S[d][i];
+    }
+  };
+  std::vector<double> K1[4], K2[4], K3[4], K4[4], H[4];
+  for (int v = 0; v < nv; ++v) { K1[v].assign(nA, 0); H[v] = U[v]; }
+  int integ = o.m.integrator;
+  if (integ == CBM_EULER) {
+    eval(U, K1);
+    for (int v = 0; v < nv; ++v) for (int64_t i = 0; i < nA; ++i) U[v][i] = U[v][i] + K1[v][i] * dt;
+  } else if (integ == CBM_HEUN) {
+    eval(U, K1);
+    for (int v = 0; v < nv; ++v) for (int64_t i = 0; i < nA; ++i) H[v][i] = U[v][i] + K1[v][i] * dt;
+    for (int v = 0; v < nv; ++v) K2[v].assign(nA, 0);
+    eval(H, K2);
+    for (int v = 0; v < nv; ++v) for (int64_t i = 0; i < nA; ++i) U[v][i] = (H[v][i] + (U[v][i] + K2[v][i] * dt)) / 2;
+  } else if (integ == CBM_RK4) {
+    for (int v = 0; v < nv; ++v) { K2[v].assign(nA, 0); K3[v].assign(nA, 0); K4[v].assign(nA, 0); }
+    eval(U, K1);
+    for (int v = 0; v < nv; ++v) for (int64_t i = 0; i < nA; ++i) H[v][i] = U[v][i] + K1[v][i] * dt / 2;
+    eval(H, K2);
+    for (int v = 0; v < nv; ++v) for (int64_t i = 0; i < nA; ++i) H[v][i] = U[v][i] + K2[v][i] * dt / 2;
+    eval(H, K3);
+    for (int v = 0; v < nv; ++v) for (int64_t i = 0; i < nA; ++i) H[v][i] = U[v][i] + K3[v][i] * dt;
+    eval(H, K4);
+    for (int v = 0; v < nv; ++v)
+      for (int64_t i = 0; i < nA; ++i)
+        U[v][i] = U[v][i] + (K1[v][i] + 2 * K2[v][i] + 2 * K3[v][i] + K4[v][i]) * dt / 6;
+  } else {  // EM
+    eval(U, K1);
+    double sq = std::sqrt(dt);
+    double gsig = o.m.noise;
+    for (int64_t i = 0; i < nA; ++i) {
+      double z[4];
+      U4 r0 = draw(o.m.seed, o.step, o.id[i], 0, 0x100u);
+      normal2(r0, z[0], z[1]);
+      if (d > 2) { U4 r1 = draw(o.m.seed, o.step, o.id[i], 0, 0x101u); normal2(r1, z[2], z[3]); }
+      for (int a = 0; a < d; ++a) U[a][i] = (U[a][i] + K1[a][i] * dt) + gsig * z[a] * sq;
+      if (o.growth) U[d][i] = U[d][i] + K1[d][i] * dt;
+    }
+  }
+  for (int a = 0; a < d; ++a) std::copy(U[a].begin(), U[a].begin() + nA, o.X[a].begin());
+  if (o.growth) std::copy(U[d].begin(), U[d].begin() + nA, o.vol.begin());
+}
+
+static void wrap_positions(Orc& o) {
+  if (!o.m.periodic) return;
+  for (int a = 0; a < o.dims; ++a) {
+    double lo = o.s.box_lo[a], L = o.s.box_hi[a] - o.s.box_lo[a];
+    for (int64_t i = 0; i < o.N; ++i) {
+      double x = o.X[a][i];
+      if (x >= lo + L) x = x - L; else if (x < lo) x = x + L;
+      o.X[a][i] = x;
+    }
+  }
+}
+
+static void one_step(Orc& o) {
+  std::vector<int64_t> holes; int64_t nAdd = 0;
+  apply_rule(o, holes, nAdd);                 // agentStepRule!
+  int64_t nA = o.N;                            // DE loop runs over 1:N
+  de_step(o, nA);                              // agentStepDE!
+  if (o.m.medium_n[0] > 0) medium_step(o, nA); // mediumStepDE!
+  // update!: commit the write-new buffers, then compact
+  for (int a = 0; a < o.dims; ++a) for (int64_t i = 0; i < nA; ++i) o.F[a][i] = o.Fnew[a][i];
+  if (o.growth) for (int64_t i = 0; i < o.N + nAdd; ++i) { o.gr[i] = o.grNew[i]; o.vb[i] = o.vbNew[i]; }
+  bool changed = (nAdd > 0) || !holes.empty();
+  compact(o, holes, nAdd);
+  o.t = o.t + o.m.dt;
+  o.step++;
+  wrap_positions(o);
+  compute_neighbors(o, changed);               // computeNeighbors!
+}
+
+static int field_is_int(int f) { return f == CBM_F_ID || f == CBM_F_LF; }
+
+// -------------------------------------------------------------- adjoint
+// Overdamped Euler trajectory loss for the REP_ADH law (model-scope
+// parameters theta = (k, r_eq, eps), p[3] = dc factor), evaluated in type T.
+// The neighbor set of each step is the Verlet list of the oracle, rebuilt
+// with the usual trigger on the value part of the state.
+template <class T>
+static T traj_loss(Orc& o, const T* theta, int64_t n_steps, int32_t n_saves,
+                   const int64_t* save_steps, const double* target) {
+  const int d = o.dims;
+  const int64_t N = o.N;
+  std::vector<T> X[3], K[3];
+  for (int a = 0; a < d; ++a) { X[a].resize(N); K[a].resize(N); for (int64_t i = 0; i < N; ++i) X[a][i] = T(o.X[a][i]); }
+  T P[4] = {theta[0], theta[1], theta[2], T(o.s.p[3])};
+  T loss(0.0);
+  int32_t si = 0;
+  compute_neighbors(o, true);
+  for (int64_t st = 1; st <= n_steps; ++st) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+      T acc[3] = {T(0.0), T(0.0), T(0.0)};
+      for (int32_t q = 0; q < o.nbN[i]; ++q) {
+        int64_t j = o.nbList[(int64_t)q * o.nMax + i];
+        T dx[3] = {T(0.0), T(0.0), T(0.0)};
+        for (int a = 0; a < d; ++a) dx[a] = o.mi(a, X[a][i] - X[a][j]);
+        T dd = dist(o, dx);
+        T sv;
+        if (law_s(CBM_LAW_REP_ADH, P, dd, sv))
+          for (int a = 0; a < d; ++a) acc[a] = acc[a] + sv * (dx[a] / dd);
+      }
+      for (int a = 0; a < d; ++a) K[a][i] = acc[a];
+    }
+    for (int a = 0; a < d; ++a) for (int64_t i = 0; i < N; ++i) X[a][i] = X[a][i] + K[a][i] * T(o.m.dt);
+    // keep the value part in the oracle state for the rebuild trigger
+    for (int a = 0; a < d; ++a) for (int64_t i = 0; i < N; ++i) o.X[a][i] = val(X[a][i]);
+    o.t = o.t + o.m.dt; o.step++;
+    compute_neighbors(o, false);
+    if (si < n_saves && save_steps[si] == st) {
+      const double* tg = target + (int64_t)si * d * N;
+      T part(0.0);
+      for (int a = 0; a < d; ++a)
+        for (int64_t i = 0; i < N; ++i) { T e = X[a][i] - T(tg[(int64_t)a * N + i]); part = part + e * e; }
+      loss = loss + part;
+      si++;
+    }
+  }
+  return loss / T((double)n_saves * (double)N);
+}
+
+}  // namespace
+
+// ================================================================== C API
+extern "C" {
+
+void* orc_create(const cbm_model_desc* m, const cbm_sim_desc* s) {
+  Orc* o = new Orc();
+  o->m = *m; o->s = *s;
+  o->dims = m->dims;
+  o->N = s->n; o->nMax = std::max<int64_t>(s->n_max, s->n);
+  o->idMax = s->n;
+  o->growth = (m->rule == CBM_RULE_GROWTH_DIVISION);
+  for (int a = 0; a < 3; ++a) {
+    o->X[a].assign(o->nMax, 0); o->F[a].assign(o->nMax, 0); o->Fnew[a].assign(o->nMax, 0); o->Xref[a].assign(o->nMax, 0);
+  }
+  o->id.resize(o->nMax); o->lf.assign(o->nMax, 0); o->alive.assign(o->nMax, 1);
+  for (int64_t i = 0; i < o->nMax; ++i) o->id[i] = i + 1;
+  o->vol.assign(o->nMax, 1.0); o->gr.assign(o->nMax, 0.0); o->vb.assign(o->nMax, 1.0);
+  o->grNew.assign(o->nMax, 0.0); o->vbNew.assign(o->nMax, 1.0);
+  double edge = m->cell_edge;
+  if (m->neighbors == CBM_CELL_LINKED) {
+    set_grid(*o, o->g, edge);
+    if (o->g.periodic)
+      for (int a = 0; a < o->dims; ++a)
+        if (o->g.n[a] < 3) { o->err = CBM_ERR_CONFIG; o->msg = "periodic cell grid needs >= 3 cells per axis"; }
+  }
+  if (m->medium_n[0] > 0) {
+    for (int a = 0; a < 3; ++a) { o->mn[a] = 1; o->mdx[a] = 1; }
+    for (int a = 0; a < o->dims; ++a) {
+      o->mn[a] = m->medium_n[a];
+      o->mdx[a] = (s->box_hi[a] - s->box_lo[a]) / (double)m->medium_n[a];   // communityStructure.jl:274
+    }
+    o->c.assign(o->mn[0] * o->mn[1] * o->mn[2], m->medium_c0);
+  }
+  return o;
+}
+
+void orc_destroy(void* h) { delete (Orc*)h; }
+
+int orc_set(void* h, int field, const void* data, int64_t n) {
+  Orc& o = *(Orc*)h;
+  if (n > o.nMax) return CBM_ERR_ARG;
+  for (int64_t i = 0; i < n; ++i) {
+    double v = field_is_int(field) ? 0.0 : ((const double*)data)[i];
+    int64_t iv = field_is_int(field) ? ((const int64_t*)data)[i] : 0;
+    switch (field) {
+      case CBM_F_X: case CBM_F_Y: case CBM_F_Z: o.X[field][i] = v; break;
+      case CBM_F_FX: case CBM_F_FY: case CBM_F_FZ: o.F[field - CBM_F_FX][i] = v; o.Fnew[field - CBM_F_FX][i] = v; break;
+      case CBM_F_VOL: o.vol[i] = v; break;
+      case CBM_F_GROWTH: o.gr[i] = v; o.grNew[i] = v; break;
+      case CBM_F_VBIRTH: o.vb[i] = v; o.vbNew[i] = v; break;
+      case CBM_F_ID: o.id[i] = iv; break;
+      case CBM_F_LF: o.lf[i] = iv; break;
+      default: return CBM_ERR_ARG;
+    }
+  }
+  o.structure_valid = false;
+  return 0;
+}
+
+int orc_get(void* h, int field, void* out, int64_t n) {
+  Orc& o = *(Orc*)h;
+  if (n > o.nMax) return CBM_ERR_ARG;
+  for (int64_t i = 0; i < n; ++i) {
+    double* dv = (double*)out; int64_t* iv = (int64_t*)out;
+    switch (field) {
+      case CBM_F_X: case CBM_F_Y: case CBM_F_Z: dv[i] = o.X[field][i]; break;
+      case CBM_F_FX: case CBM_F_FY: case CBM_F_FZ: dv[i] = o.F[field - CBM_F_FX][i]; break;
+      case CBM_F_VOL: dv[i] = o.vol[i]; break;
+      case CBM_F_GROWTH: dv[i] = o.gr[i]; break;
+      case CBM_F_VBIRTH: dv[i] = o.vb[i]; break;
+      case CBM_F_ID: iv[i] = o.id[i]; break;
+      case CBM_F_LF: iv[i] = o.lf[i]; break;
+      default: return CBM_ERR_ARG;
+    }
+  }
+  return 0;
+}
+
+int64_t orc_n(void* h) { return ((Orc*)h)->N; }
+int64_t orc_id_max(void* h) { return ((Orc*)h)->idMax; }
+int64_t orc_rebuilds(void* h) { return ((Orc*)h)->rebuilds; }
+double orc_t(void* h) { return ((Orc*)h)->t; }
+int orc_error(void* h) { return ((Orc*)h)->err; }
+
+// loadToPlatform! -> first computeNeighbors!
+int orc_init(void* h) {
+  Orc& o = *(Orc*)h;
+  compute_neighbors(o, true);
+  return o.err;
+}
+
+int orc_step(void* h, int64_t nsteps) {
+  Orc& o = *(Orc*)h;
+  if (!o.structure_valid) compute_neighbors(o, true);
+  for (int64_t k = 0; k < nsteps; ++k) one_step(o);
+  return o.err;
+}
+
+// Neighbor set of agent i (slots), for sorted-set comparisons.  Verlet: the
+// list; CellLinked / Full: the candidates within `radius`.
+int64_t orc_neighbors_of(void* h, int64_t i, double radius, int64_t* out, int64_t cap) {
+  Orc& o = *(Orc*)h;
+  int64_t k = 0;
+  if (o.is_verlet()) {
+    for (int32_t q = 0; q < o.nbN[i]; ++q) { if (k < cap) out[k] = o.nbList[(int64_t)q * o.nMax + i]; k++; }
+    return k;
+  }
+  for_candidates(o, i, [&](int64_t j) {
+    double dx[3] = {0, 0, 0};
+    for (int a = 0; a < o.dims; ++a) dx[a] = o.mi(a, o.X[a][i] - o.X[a][j]);
+    if (dist(o, dx) < radius) { if (k < cap) out[k] = j; k++; }
+  });
+  return k;
+}
+
+// agentRule of the neighbor KAT (test/testCommunity.jl:375-383).
+int orc_count_neighbors(void* h, double radius, int64_t* out) {
+  Orc& o = *(Orc*)h;
+  for (int64_t i = 0; i < o.N; ++i) {
+    int64_t c = 0;
+    for_candidates(o, i, [&](int64_t j) {
+      double dx[3] = {0, 0, 0};
+      for (int a = 0; a < o.dims; ++a) dx[a] = o.mi(a, o.X[a][i] - o.X[a][j]);
+      if (dist(o, dx) < radius) c++;
+    });
+    out[i] = c;
+  }
+  return 0;
+}
+
+int orc_get_medium(void* h, double* out, int64_t n) {
+  Orc& o = *(Orc*)h;
+  if (n > (int64_t)o.c.size()) return CBM_ERR_ARG;
+  std::memcpy(out, o.c.data(), sizeof(double) * n);
+  return 0;
+}
+int orc_set_medium(void* h, const double* in, int64_t n) {
+  Orc& o = *(Orc*)h;
+  if (n > (int64_t)o.c.size()) return CBM_ERR_ARG;
+  std::memcpy(o.c.data(), in, sizeof(double) * n);
+  return 0;
+}
+
+// Philox known-answer access for tests: out[4] = philox4x32-10(ctr, key).
+void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+  U4 c; for (int i = 0; i < 4; ++i) c.v[i] = ctr[i];
+  U4 r = philox(c, key[0], key[1]);
+  for (int i = 0; i < 4; ++i) out[i] = r.v[i];
+}
+void orc_normals(uint64_t seed, int64_t step, int64_t id, int32_t sim, uint32_t purpose, double* z2) {
+  U4 r = draw(seed, step, id, sim, purpose);
+  normal2(r, z2[0], z2[1]);
+}
+int64_t orc_floor_div(double a, double b) { return floor_div(a, b); }
+
+int orc_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+// loss with double arithmetic; theta = (k, r_eq, eps).  Restores positions.
+double orc_traj_loss(void* h, const double* theta, int64_t n_steps, int32_t n_saves,
+                     const int64_t* save_steps, const double* target) {
+  Orc& o = *(Orc*)h;
+  std::vector<double> keep[3];
+  for (int a = 0; a < o.dims; ++a) keep[a] = o.X[a];
+  double t0 = o.t; int64_t s0 = o.step;
+  double r = traj_loss<double>(o, theta, n_steps, n_saves, save_steps, target);
+  for (int a = 0; a < o.dims; ++a) o.X[a] = keep[a];
+  o.t = t0; o.step = s0; o.structure_valid = false;
+  return r;
+}
+
+// loss and forward-mode gradient (dual numbers).  Restores positions.
+double orc_traj_grad_dual(void* h, const double* theta, int64_t n_steps, int32_t n_saves,
+                          const int64_t* save_steps, const double* target, double* grad) {
+  Orc& o = *(Orc*)h;
+  std::vector<double> keep[3];
+  for (int a = 0; a < o.dims; ++a) keep[a] = o.X[a];
+  double t0 = o.t; int64_t s0 = o.step;
+  D3 th[3];
+  for (int q = 0; q < 3; ++q) { th[q] = D3(theta[q]); th[q].g[q] = 1.0; }
+  D3 r = traj_loss<D3>(o, th, n_steps, n_saves, save_steps, target);
+  for (int q = 0; q < 3; ++q) grad[q] = r.g[q];
+  for (int a = 0; a < o.dims; ++a) o.X[a] = keep[a];
+  o.t = t0; o.step = s0; o.structure_valid = false;
+  return r.v;
+}
+
+}  // extern "C"

@@ -1,0 +1,172 @@
+"""ctypes wrapper of liboracle.so -- the CPU restatement of the reference
+semantics.  TEST INFRASTRUCTURE ONLY: imported by tests/, by
+__graft_entry__.smoke() and by bench.py's cpu_baseline leg, always as the
+checker, never as the product path.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        h = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        sig = {
+            "orc_create": (P, [P, P]),
+            "orc_destroy": (None, [P]),
+            "orc_set": (C.c_int, [P, C.c_int, P, C.c_int64]),
+            "orc_get": (C.c_int, [P, C.c_int, P, C.c_int64]),
+            "orc_n": (C.c_int64, [P]),
+            "orc_id_max": (C.c_int64, [P]),
+            "orc_rebuilds": (C.c_int64, [P]),
+            "orc_t": (C.c_double, [P]),
+            "orc_error": (C.c_int, [P]),
+            "orc_init": (C.c_int, [P]),
+            "orc_step": (C.c_int, [P, C.c_int64]),
+            "orc_neighbors_of": (C.c_int64, [P, C.c_int64, C.c_double, P, C.c_int64]),
+            "orc_count_neighbors": (C.c_int, [P, C.c_double, P]),
+            "orc_get_medium": (C.c_int, [P, P, C.c_int64]),
+            "orc_set_medium": (C.c_int, [P, P, C.c_int64]),
+            "orc_philox": (None, [P, P, P]),
+            "orc_normals": (None, [C.c_uint64, C.c_int64, C.c_int64, C.c_int32, C.c_uint32, P]),
+            "orc_floor_div": (C.c_int64, [C.c_double, C.c_double]),
+            "orc_threads": (C.c_int, []),
+            "orc_traj_loss": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P]),
+            "orc_traj_grad_dual": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P, P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = h
+    return _lib
+
+
+INT_FIELDS = (3, 10)
+
+
+class Oracle:
+    """One simulation in logical slot order (the reference's CPU platform)."""
+
+    def __init__(self, model, sim):
+        self.lib = lib()
+        self.model = model
+        self.sim = sim
+        self.h = self.lib.orc_create(C.addressof(model), C.addressof(sim))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def set(self, field, values):
+        v = np.ascontiguousarray(values, dtype=np.int64 if field in INT_FIELDS else np.float64)
+        rc = self.lib.orc_set(self.h, field, v.ctypes.data, v.size)
+        assert rc == 0, rc
+
+    def get(self, field, n=None):
+        n = self.n if n is None else n
+        v = np.empty(n, dtype=np.int64 if field in INT_FIELDS else np.float64)
+        rc = self.lib.orc_get(self.h, field, v.ctypes.data, n)
+        assert rc == 0, rc
+        return v
+
+    @property
+    def n(self):
+        return self.lib.orc_n(self.h)
+
+    @property
+    def id_max(self):
+        return self.lib.orc_id_max(self.h)
+
+    @property
+    def t(self):
+        return self.lib.orc_t(self.h)
+
+    @property
+    def rebuilds(self):
+        return self.lib.orc_rebuilds(self.h)
+
+    def error(self):
+        return self.lib.orc_error(self.h)
+
+    def init(self):
+        return self.lib.orc_init(self.h)
+
+    def step(self, k=1):
+        return self.lib.orc_step(self.h, int(k))
+
+    def neighbors_of(self, i, radius, cap=4096):
+        out = np.empty(cap, dtype=np.int64)
+        k = self.lib.orc_neighbors_of(self.h, int(i), float(radius), out.ctypes.data, cap)
+        return out[: min(k, cap)]
+
+    def neighbor_sets(self, radius, cap=4096):
+        return [sorted(self.neighbors_of(i, radius, cap).tolist()) for i in range(self.n)]
+
+    def count_neighbors(self, radius):
+        out = np.empty(max(self.n, 1), dtype=np.int64)
+        self.lib.orc_count_neighbors(self.h, float(radius), out.ctypes.data)
+        return out[: self.n]
+
+    def medium(self, n):
+        v = np.empty(n)
+        assert self.lib.orc_get_medium(self.h, v.ctypes.data, n) == 0
+        return v
+
+    def set_medium(self, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        assert self.lib.orc_set_medium(self.h, v.ctypes.data, v.size) == 0
+
+    def traj_loss(self, theta, n_steps, save_steps, target):
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        ss = np.ascontiguousarray(save_steps, dtype=np.int64)
+        tg = np.ascontiguousarray(target, dtype=np.float64)
+        return self.lib.orc_traj_loss(self.h, th.ctypes.data, int(n_steps), ss.size, ss.ctypes.data, tg.ctypes.data)
+
+    def traj_grad_dual(self, theta, n_steps, save_steps, target):
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        ss = np.ascontiguousarray(save_steps, dtype=np.int64)
+        tg = np.ascontiguousarray(target, dtype=np.float64)
+        g = np.zeros(3)
+        loss = self.lib.orc_traj_grad_dual(self.h, th.ctypes.data, int(n_steps), ss.size, ss.ctypes.data,
+                                           tg.ctypes.data, g.ctypes.data)
+        return loss, g
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox(c.ctypes.data, k.ctypes.data, o.ctypes.data)
+    return o
+
+
+def normals(seed, step, id_, sim, purpose):
+    z = np.zeros(2)
+    lib().orc_normals(seed, step, id_, sim, purpose, z.ctypes.data)
+    return z
+
+
+def floor_div(a, b):
+    return lib().orc_floor_div(float(a), float(b))
+
+
+def threads():
+    return lib().orc_threads()
