@@ -54,6 +54,8 @@ struct cbm_ctx {
   SimState* d_st = nullptr;
   int64_t C = 0;      // rows
   int64_t NB = 0;     // buckets
+  int64_t NR = 0;     // rank-table entries (sum of cells)
+  std::vector<int32_t> h_crank;
   double list_radius = 0.0, thr = 0.0;
 
   // per-agent columns
@@ -68,6 +70,8 @@ struct cbm_ctx {
   uint8_t* dead = nullptr;
 
   // neighbor engine
+  int32_t* crank = nullptr;               // per-sim cell -> Morton rank tables
+  int32_t* lin = nullptr;
   int32_t *key = nullptr, *count = nullptr, *cursor = nullptr, *start = nullptr, *tmp = nullptr, *cell_row = nullptr;
   unsigned long long* scan_status = nullptr;
   unsigned int* scan_ticket = nullptr;    // [0] ticket, [1] done, [2] epoch
@@ -164,6 +168,30 @@ static int validate(cbm_ctx* c) {
   return CBM_OK;
 }
 
+// Bucket order of the cells: rank of each linear cell (x fastest) along the
+// Morton (Z-order) curve, so that rows sorted by bucket are spatially compact.
+static uint64_t spread_bits(uint64_t v, int dims) {
+  uint64_t r = 0;
+  for (int b = 0; b < 21; ++b) r |= ((v >> b) & 1ull) << (b * dims);
+  return r;
+}
+
+static void morton_ranks(const int32_t* gn, int dims, std::vector<int32_t>& out) {
+  const int64_t nc = (int64_t)gn[0] * gn[1] * gn[2];
+  std::vector<std::pair<uint64_t, int32_t>> key((size_t)nc);
+  for (int64_t l = 0; l < nc; ++l) {
+    uint64_t cx = (uint64_t)(l % gn[0]), cy = (uint64_t)((l / gn[0]) % gn[1]), cz = (uint64_t)(l / ((int64_t)gn[0] * gn[1]));
+    uint64_t code = spread_bits(cx, dims);
+    if (dims > 1) code |= spread_bits(cy, dims) << 1;
+    if (dims > 2) code |= spread_bits(cz, dims) << 2;
+    key[(size_t)l] = {code, (int32_t)l};
+  }
+  std::sort(key.begin(), key.end());
+  const size_t base = out.size();
+  out.resize(base + (size_t)nc);
+  for (int64_t r = 0; r < nc; ++r) out[base + (size_t)key[(size_t)r].second] = (int32_t)r;
+}
+
 static int setup_geometry(cbm_ctx* c) {
   const cbm_model_desc& m = c->m;
   c->h_sims.resize(c->S);
@@ -202,6 +230,8 @@ static int setup_geometry(cbm_ctx* c) {
     sd.n_max = in.n_max;
     sd.cell_off = cell;
     sd.ncells = nc;
+    sd.rank_off = (int64_t)c->h_crank.size();
+    morton_ranks(sd.gn, m.dims, c->h_crank);
     row += in.n_max;
     cell += nc + 2;
   }
@@ -209,6 +239,7 @@ static int setup_geometry(cbm_ctx* c) {
   if (cell > ((int64_t)1 << 31) - 1) return fail(c, CBM_ERR_CONFIG, "total cell count exceeds 2^31");
   c->C = row;
   c->NB = cell;
+  c->NR = (int64_t)c->h_crank.size();
   return CBM_OK;
 }
 
@@ -248,7 +279,7 @@ static void destroy_ctx(cbm_ctx* c) {
   for (int a = 0; a < 3; ++a) { F(c->Fnew[a]); F(c->xref[a]); }
   for (int v = 0; v < MAX_STATE; ++v) { F(c->h[0][v]); F(c->h[1][v]); F(c->kacc[v]); }
   F(c->gr_new); F(c->vb_new); F(c->dead);
-  F(c->key); F(c->count); F(c->cursor); F(c->start); F(c->tmp); F(c->cell_row);
+  F(c->key); F(c->lin); F(c->crank); F(c->count); F(c->cursor); F(c->start); F(c->tmp); F(c->cell_row);
   F(c->scan_status); F(c->scan_ticket); F(c->list); F(c->list_n); F(c->flags); F(c->counters);
   F(c->d_staging); F(c->d_block_sim); F(c->d_block_off); F(c->d_sims); F(c->d_st);
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
@@ -318,6 +349,11 @@ static int init_ctx(cbm_ctx* c) {
     for (int v = 0; v < c->nvar; ++v) HIPCHK(c, dalloc(&c->kacc[v], c->C));
   // neighbor engine
   HIPCHK(c, dalloc(&c->key, c->C));
+  HIPCHK(c, dalloc(&c->lin, c->C));
+  HIPCHK(c, dalloc(&c->crank, c->NR));
+  HIPCHK(c, hipMemcpy(c->crank, c->h_crank.data(), sizeof(int32_t) * c->NR, hipMemcpyHostToDevice));
+  c->h_crank.clear();
+  c->h_crank.shrink_to_fit();
   HIPCHK(c, dalloc(&c->tmp, c->C));
   HIPCHK(c, dalloc(&c->cell_row, c->C));
   HIPCHK(c, dalloc(&c->count, c->NB));
@@ -370,7 +406,7 @@ static RebuildArgs rebuild_args(cbm_ctx* c, const int32_t* flag, int32_t* flag_c
   for (int d = 0; d < c->dims; ++d) a.pos[d] = (const double*)c->col[C_X + d].cur();
   a.dead = nullptr;
   a.slot = (const int32_t*)c->col[C_SLOT].cur();
-  a.key = c->key; a.count = c->count; a.cursor = c->cursor; a.start = c->start; a.tmp = c->tmp;
+  a.key = c->key; a.lin = c->lin; a.crank = c->crank; a.count = c->count; a.cursor = c->cursor; a.start = c->start; a.tmp = c->tmp;
   a.cell_row = c->cell_row;
   a.dims = c->dims; a.periodic = c->m.periodic; a.full = c->full ? 1 : 0;
   a.geometric = c->verlet ? 1 : 0;
@@ -436,7 +472,7 @@ static StageArgs stage_base(cbm_ctx* c) {
   for (int d = 0; d < c->dims; ++d) a.xref[d] = c->xref[d];
   a.id = (const int64_t*)c->col[C_ID].cur();
   a.list = c->list; a.list_n = c->list_n; a.ldl = c->C;
-  a.cell_start = c->start; a.cell_row = c->cell_row;
+  a.cell_start = c->start; a.cell_row = c->cell_row; a.crank = c->crank;
   a.dt = c->m.dt; a.sqdt = std::sqrt(c->m.dt); a.drift = c->m.drift; a.noise = c->m.noise; a.thr = c->thr;
   a.seed = c->m.seed;
   a.step_ptr = c->counters + 1;
@@ -647,11 +683,12 @@ int cbm_set_n(cbm_ctx* c, int32_t sim, int64_t n) {
   if (!c || sim < 0 || sim >= c->S) return CBM_ERR_ARG;
   if (n < 0 || n > c->sims[sim].n_max) return fail(c, CBM_ERR_CAPACITY, "N exceeds capacity");
   SimState st;
+  HIPCHK(c, hipMemcpyAsync(&st, c->d_st + sim, sizeof(st), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  HIPCHK(c, hipMemcpy(&st, c->d_st + sim, sizeof(st), hipMemcpyDeviceToHost));
   st.N = n; st.n_tot = n;
   if (st.id_max < n) st.id_max = n;
-  HIPCHK(c, hipMemcpy(c->d_st + sim, &st, sizeof(st), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpyAsync(c->d_st + sim, &st, sizeof(st), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   c->structure_valid = false;
   return CBM_OK;
 }
@@ -767,12 +804,16 @@ static int export_sets(cbm_ctx* c, int32_t sim, double radius, int32_t* counts, 
   int64_t N;
   int rc = cbm_get_n(c, sim, &N);
   if (rc) return rc;
-  int32_t* d_counts = nullptr; int64_t* d_lists = nullptr; int64_t* d_c64 = nullptr; int32_t* d_ovf = nullptr;
+  // Scratch for the export; every entry read back below is written by the
+  // kernel, and the overflow word is cleared on the context stream (the
+  // stream does not synchronise with the null stream).
+  int32_t* d_counts = nullptr; int64_t* d_lists = nullptr; int64_t* d_c64 = nullptr;
+  int32_t* d_ovf = c->flags + 4;
   int64_t nmax = c->sims[sim].n_max;
-  HIPCHK(c, dalloc(&d_ovf, 1));
-  if (counts) HIPCHK(c, dalloc(&d_counts, nmax));
-  if (lists) HIPCHK(c, dalloc(&d_lists, nmax * std::max<int64_t>(cap, 1)));
-  if (counts64) HIPCHK(c, dalloc(&d_c64, nmax));
+  if (counts) HIPCHK(c, hipMalloc((void**)&d_counts, sizeof(int32_t) * nmax));
+  if (lists) HIPCHK(c, hipMalloc((void**)&d_lists, sizeof(int64_t) * nmax * std::max<int64_t>(cap, 1)));
+  if (counts64) HIPCHK(c, hipMalloc((void**)&d_c64, sizeof(int64_t) * nmax));
+  HIPCHK(c, launch_set_flag(c->stream, d_ovf, 0));
   ExportArgs e;
   std::memset(&e, 0, sizeof(e));
   e.s = stage_base(c);
@@ -785,11 +826,12 @@ static int export_sets(cbm_ctx* c, int32_t sim, double radius, int32_t* counts, 
   int ovf = 0;
   hipError_t le = launch_export(c->dims, c->nb_kind, c->nblocks, c->block, c->stream, e);
   if (le == hipSuccess) le = hipStreamSynchronize(c->stream);
-  if (le == hipSuccess && counts) le = hipMemcpy(counts, d_counts, sizeof(int32_t) * N, hipMemcpyDeviceToHost);
-  if (le == hipSuccess && lists) le = hipMemcpy(lists, d_lists, sizeof(int64_t) * N * cap, hipMemcpyDeviceToHost);
-  if (le == hipSuccess && counts64) le = hipMemcpy(counts64, d_c64, sizeof(int64_t) * N, hipMemcpyDeviceToHost);
-  if (le == hipSuccess) le = hipMemcpy(&ovf, d_ovf, sizeof(int), hipMemcpyDeviceToHost);
-  (void)hipFree(d_ovf); if (d_counts) (void)hipFree(d_counts); if (d_lists) (void)hipFree(d_lists); if (d_c64) (void)hipFree(d_c64);
+  if (le == hipSuccess && counts) le = hipMemcpyAsync(counts, d_counts, sizeof(int32_t) * N, hipMemcpyDeviceToHost, c->stream);
+  if (le == hipSuccess && lists) le = hipMemcpyAsync(lists, d_lists, sizeof(int64_t) * N * cap, hipMemcpyDeviceToHost, c->stream);
+  if (le == hipSuccess && counts64) le = hipMemcpyAsync(counts64, d_c64, sizeof(int64_t) * N, hipMemcpyDeviceToHost, c->stream);
+  if (le == hipSuccess) le = hipMemcpyAsync(&ovf, d_ovf, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (le == hipSuccess) le = hipStreamSynchronize(c->stream);
+  if (d_counts) (void)hipFree(d_counts); if (d_lists) (void)hipFree(d_lists); if (d_c64) (void)hipFree(d_c64);
   if (le != hipSuccess) return fail(c, CBM_ERR_HIP, std::string("neighbor export: ") + hipGetErrorString(le));
   if (ovf) return fail(c, CBM_ERR_LIST_OVERFLOW, "neighbor export capacity too small");
   return CBM_OK;

@@ -69,33 +69,64 @@ __device__ __forceinline__ int64_t floor_div(double a, double b) {
 }
 
 // ------------------------------------------------------------- pair laws
-// Pair magnitude s(d); the force on i is s * (x_i - x_j)/d.  Returns false
-// for pairs that do not interact (including coincident agents, d == 0).
+// Pair magnitude s(d) of the built-in agentODE bodies with their constants
+// hoisted out of the pair loop.  operator() returns whether the pair
+// interacts (coincident agents, d == 0, never do) and always sets s, so the
+// caller can update its accumulators with selects instead of branches.
 enum { LAW_NONE = 0, LAW_LINEAR = 1, LAW_REP_ADH = 2, LAW_AGGREGATION = 3, LAW_CUSTOM = 4 };
 
-template <int LAW>
-__device__ __forceinline__ bool pair_s(const double* __restrict__ p, double d, double& s) {
-  if (!(d > 0.0)) return false;
-  if (LAW == LAW_LINEAR) {
-    if (d < p[1]) { s = p[0] * (p[1] - d); return true; }
-    return false;
-  } else if (LAW == LAW_REP_ADH) {
-    double c = 2.0 * p[1];
-    double dc = p[3] * p[1];
-    if (d < c) { s = p[0] * (c - d); return true; }
-    if (d < dc) {
-      double w = dc - c;
-      s = 0.0 - p[2] * 4.0 * (d - c) * (dc - d) / (w * w);
-      return true;
-    }
-    return false;
-  } else if (LAW == LAW_AGGREGATION) {
-    if (d < p[0]) { s = p[1] * (p[0] - d); return true; }
-    if (d < p[2]) { s = 0.0 - p[3] * (p[2] - d); return true; }
-    return false;
+template <int LAW> struct PairLaw;
+
+// s = k (c - d) for d < c
+template <> struct PairLaw<LAW_LINEAR> {
+  static constexpr bool per_axis_division = false;
+  double k, c;
+  __device__ __forceinline__ explicit PairLaw(const double* p) : k(p[0]), c(p[1]) {}
+  __device__ __forceinline__ bool operator()(double d, double& s) const {
+    s = k * (c - d);
+    return d > 0.0 && d < c;
   }
-  return false;
-}
+};
+
+// c = 2 r_eq, dc = f r_eq, a4 = 4 eps / (dc - c)^2:
+//   s = k (c - d)                      d < c
+//   s = 0 - (a4 (d - c)) (dc - d)      c <= d < dc   (well of depth eps)
+template <> struct PairLaw<LAW_REP_ADH> {
+  static constexpr bool per_axis_division = false;
+  double k, c, dc, a4;
+  __device__ __forceinline__ explicit PairLaw(const double* p) : k(p[0]) {
+    c = 2.0 * p[1];
+    dc = p[3] * p[1];
+    const double w = dc - c;
+    a4 = (4.0 * p[2]) / (w * w);
+  }
+  __device__ __forceinline__ bool operator()(double d, double& s) const {
+    const double rep = k * (c - d);
+    const double adh = 0.0 - a4 * (d - c) * (dc - d);
+    s = d < c ? rep : adh;
+    return d > 0.0 && d < dc;
+  }
+};
+
+// docs/src/Aggregation.md:88-98:
+//   s = fRep (rRep - d)  d < rRep;  s = -fAtr (rAtr - d)  rRep <= d < rAtr
+template <> struct PairLaw<LAW_AGGREGATION> {
+  static constexpr bool per_axis_division = true;   // dirx = (x - x[i2])/d
+  double rRep, fRep, rAtr, fAtr;
+  __device__ __forceinline__ explicit PairLaw(const double* p) : rRep(p[0]), fRep(p[1]), rAtr(p[2]), fAtr(p[3]) {}
+  __device__ __forceinline__ bool operator()(double d, double& s) const {
+    const double rep = fRep * (rRep - d);
+    const double atr = 0.0 - fAtr * (rAtr - d);
+    s = d < rRep ? rep : atr;
+    return d > 0.0 && (d < rRep || d < rAtr);
+  }
+};
+
+template <> struct PairLaw<LAW_NONE> {
+  static constexpr bool per_axis_division = false;
+  __device__ __forceinline__ explicit PairLaw(const double*) {}
+  __device__ __forceinline__ bool operator()(double, double& s) const { s = 0.0; return false; }
+};
 
 }  // namespace cbm
 

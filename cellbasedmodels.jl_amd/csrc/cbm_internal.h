@@ -26,6 +26,8 @@ struct SimDev {
   int32_t pad0;
   int64_t row_off, n_max;
   int64_t cell_off, ncells;
+  int64_t rank_off;     // offset of this simulation's cell -> Morton-rank table
+  int64_t pad1;
 };
 
 // Per-simulation mutable scalars (device array).
@@ -76,7 +78,8 @@ struct StageArgs {
   const int32_t* list_n;
   int64_t ldl;
   const int32_t* cell_start;    // bucket -> first global row
-  const int32_t* cell_row;      // row -> global bucket (cell-linked)
+  const int32_t* cell_row;      // row -> linear local cell (cell-linked)
+  const int32_t* crank;         // linear local cell -> bucket rank (Morton order)
   int32_t* flag;                // rebuild flag written by the final stage
   double dt, sqdt, drift, noise, thr;
   uint64_t seed;
@@ -104,11 +107,13 @@ struct RebuildArgs {
   const uint8_t* dead;        // per-row removal mark (may be NULL)
   const int32_t* slot;        // per-row logical slot (current)
   int32_t* key;               // per-row bucket
+  int32_t* lin;               // per-row linear local cell
+  const int32_t* crank;       // linear local cell -> bucket rank (Morton order)
   int32_t* count;             // per-bucket count
   int32_t* cursor;            // per-bucket scatter cursor
   int32_t* start;             // per-bucket first row (exclusive scan), nb+1 entries
   int32_t* tmp;               // bucket-ordered source rows
-  int32_t* cell_row;          // destination row -> bucket
+  int32_t* cell_row;          // destination row -> linear local cell
   int32_t dims, periodic, full, geometric;
 };
 

@@ -17,14 +17,27 @@
 #include "cbm_internal.h"
 #include "cbm_device.h"
 
+#ifndef CBM_LIST_CHUNK
+#define CBM_LIST_CHUNK 2
+#endif
+
 namespace cbm {
 
 // ------------------------------------------------------------ row geometry
 struct RowIdx { int sim; int64_t local; int64_t row; };
 
+// Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
+// (own L2) serves one contiguous range of rows, i.e. one compact region of
+// the Morton-ordered agents, and neighbor gathers stay in that L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7;
+  const int x = b & 7, idx = b >> 3;
+  return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
+}
+
 __device__ __forceinline__ RowIdx row_index(const RowGeom& g) {
   RowIdx r;
-  const int b = blockIdx.x;
+  const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   r.sim = g.block_sim[b];
   r.local = (int64_t)(b - g.block_off[r.sim]) * blockDim.x + threadIdx.x;
   r.row = g.sims[r.sim].row_off + r.local;
@@ -86,14 +99,15 @@ __device__ __forceinline__ void for_candidates(const StageArgs& a, const SimDev&
     const int64_t j0 = sd.row_off, j1 = sd.row_off + N;
     for (int64_t j = j0; j < j1; ++j) if (j != i) fn(j);
   } else {
-    const int32_t c = a.cell_row[i] - (int32_t)sd.cell_off;
+    const int32_t c = a.cell_row[i];
     const int32_t cx = c % sd.gn[0];
     const int32_t cy = (c / sd.gn[0]) % sd.gn[1];
     const int32_t cz = c / (sd.gn[0] * sd.gn[1]);
+    const int32_t* __restrict__ rank = a.crank + sd.rank_off;
     for (int k = 0; k < pow3(D); ++k) {
       const int32_t nc = neigh_cell<D>(sd, cx, cy, cz, k, a.periodic);
       if (nc < 0) continue;
-      const int64_t gc = sd.cell_off + nc;
+      const int64_t gc = sd.cell_off + rank[nc];
       const int32_t j0 = a.cell_start[gc], j1 = a.cell_start[gc + 1];
       for (int32_t j = j0; j < j1; ++j) if (j != i) fn((int64_t)j);
     }
@@ -101,7 +115,12 @@ __device__ __forceinline__ void for_candidates(const StageArgs& a, const SimDev&
 }
 
 // ----------------------------------------------------------- stage kernel
-template <int D, int NB, int LAW>
+// Minimum image (periodic boxes): the same comparisons as the oracle, as selects.
+__device__ __forceinline__ double wrap_dx(double d, double L, double hL) {
+  return d > hL ? d - L : (d < -hL ? d + L : d);
+}
+
+template <int D, int NB, int LAW, int PER, int NOISE>
 __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   const RowIdx ri = row_index(a.g);
   const SimDev& sd = a.g.sims[ri.sim];
@@ -115,33 +134,73 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   double acc[3] = {0.0, 0.0, 0.0};
 
   if (LAW != LAW_NONE) {
-    double p[8];
+    const PairLaw<LAW> law(sd.p);
+    double L[3] = {0.0, 0.0, 0.0}, hL[3] = {0.0, 0.0, 0.0};
+    if (PER) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) p[q] = sd.p[q];
-    const int periodic = a.periodic;
-    for_candidates<D, NB>(a, sd, N, i, [&](int64_t j) {
+      for (int c = 0; c < D; ++c) { L[c] = sd.L[c]; hL[c] = 0.5 * sd.L[c]; }
+    }
+    // one neighbor at position xj (the agentODE interaction body)
+    auto body = [&](const double* xj, bool live) {
       double dx[3];
 #pragma unroll
       for (int c = 0; c < D; ++c) {
-        dx[c] = xi[c] - a.S[c][j];
-        if (periodic) dx[c] = min_image(dx[c], sd.L[c]);
+        dx[c] = xi[c] - xj[c];
+        if (PER) dx[c] = wrap_dx(dx[c], L[c], hL[c]);
       }
       double s2 = dx[0] * dx[0];
       if (D > 1) s2 = s2 + dx[1] * dx[1];
       if (D > 2) s2 = s2 + dx[2] * dx[2];
       const double d = sqrt(s2);
       double s;
-      if (pair_s<LAW>(p, d, s)) {
+      const bool on = law(d, s) && live;
+      if (PairLaw<LAW>::per_axis_division) {
 #pragma unroll
-        for (int c = 0; c < D; ++c) acc[c] = acc[c] + s * (dx[c] / d);
+        for (int c = 0; c < D; ++c) { const double v = acc[c] + s * (dx[c] / d); acc[c] = on ? v : acc[c]; }
+      } else {
+        const double f = s / d;               // built-in laws: F = (s/d) (x_i - x_j)
+#pragma unroll
+        for (int c = 0; c < D; ++c) { const double v = acc[c] + f * dx[c]; acc[c] = on ? v : acc[c]; }
       }
-    });
+    };
+    if (NB == NB_LIST) {
+      // Verlet list in chunks of LCH gathers issued together (memory-level
+      // parallelism per lane), branch-free: slots past the count re-read the
+      // last entry and are masked out of the accumulation.
+      constexpr int LCH = CBM_LIST_CHUNK;
+      const int32_t cnt = a.list_n[i];
+      const int32_t* __restrict__ lst = a.list + i;
+      const int64_t ldl = a.ldl;
+      for (int32_t q0 = 0; q0 < cnt; q0 += LCH) {
+        int32_t jj[LCH];
+#pragma unroll
+        for (int u = 0; u < LCH; ++u) {
+          const int32_t q = (q0 + u < cnt) ? q0 + u : cnt - 1;
+          jj[u] = lst[(int64_t)q * ldl];
+        }
+        double px[LCH][3];
+#pragma unroll
+        for (int u = 0; u < LCH; ++u)
+#pragma unroll
+          for (int c = 0; c < D; ++c) px[u][c] = a.S[c][jj[u]];
+#pragma unroll
+        for (int u = 0; u < LCH; ++u) body(px[u], q0 + u < cnt);
+      }
+    } else {
+      for_candidates<D, NB>(a, sd, N, i, [&](int64_t j) {
+        double xj[3];
+#pragma unroll
+        for (int c = 0; c < D; ++c) xj[c] = a.S[c][j];
+        body(xj, true);
+      });
+    }
     if (LAW == LAW_AGGREGATION && a.walls) {
+      const double rRep = sd.p[0], fRep = sd.p[1];
 #pragma unroll
       for (int c = 0; c < D; ++c) {
         const double x = xi[c];
-        if (x < sd.lo[c] + p[0] / 2) acc[c] = acc[c] + p[1];
-        else if (x > sd.hi[c] - p[0] / 2) acc[c] = acc[c] - p[1];
+        if (x < sd.lo[c] + rRep / 2) acc[c] = acc[c] + fRep;
+        else if (x > sd.hi[c] - rRep / 2) acc[c] = acc[c] - fRep;
       }
     }
   }
@@ -161,41 +220,42 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   const double dt = a.dt;
   const int nv = grow ? D + 1 : D;
   double res[MAX_STATE];
-  switch (a.kind) {
-    case ST_EULER:
-      for (int c = 0; c < nv; ++c) res[c] = a.U[c][i] + du[c] * dt;
-      break;
-    case ST_EM: {
-      double z[4] = {0.0, 0.0, 0.0, 0.0};
-      const int64_t id = a.id[i];
-      const int64_t step = *a.step_ptr;
-      normal2(draw(a.seed, step, id, ri.sim, 0x100u), z[0], z[1]);
-      if (D > 2) normal2(draw(a.seed, step, id, ri.sim, 0x101u), z[2], z[3]);
-      for (int c = 0; c < D; ++c) res[c] = (a.U[c][i] + du[c] * dt) + a.noise * z[c] * a.sqdt;
-      if (grow) res[D] = a.U[D][i] + du[D] * dt;
-      break;
+  if (NOISE) {                                // EM: (u + f dt) + (g xi) sqrt(dt)
+    double z[4] = {0.0, 0.0, 0.0, 0.0};
+    const int64_t id = a.id[i];
+    const int64_t step = *a.step_ptr;
+    normal2(draw(a.seed, step, id, ri.sim, 0x100u), z[0], z[1]);
+    if (D > 2) normal2(draw(a.seed, step, id, ri.sim, 0x101u), z[2], z[3]);
+#pragma unroll
+    for (int c = 0; c < D; ++c) res[c] = (a.U[c][i] + du[c] * dt) + a.noise * z[c] * a.sqdt;
+    if (grow) res[D] = a.U[D][i] + du[D] * dt;
+  } else {
+    switch (a.kind) {
+      case ST_EULER:
+        for (int c = 0; c < nv; ++c) res[c] = a.U[c][i] + du[c] * dt;
+        break;
+      case ST_HEUN1:
+        for (int c = 0; c < nv; ++c) res[c] = a.U[c][i] + du[c] * dt;
+        break;
+      case ST_HEUN2:
+        for (int c = 0; c < nv; ++c) res[c] = (a.S[c][i] + (a.U[c][i] + du[c] * dt)) / 2;
+        break;
+      case ST_RK4_1:
+        for (int c = 0; c < nv; ++c) { a.kacc[c][i] = du[c]; res[c] = a.U[c][i] + du[c] * dt / 2; }
+        break;
+      case ST_RK4_2:
+        for (int c = 0; c < nv; ++c) { a.kacc[c][i] = a.kacc[c][i] + 2 * du[c]; res[c] = a.U[c][i] + du[c] * dt / 2; }
+        break;
+      case ST_RK4_3:
+        for (int c = 0; c < nv; ++c) { a.kacc[c][i] = a.kacc[c][i] + 2 * du[c]; res[c] = a.U[c][i] + du[c] * dt; }
+        break;
+      default:  // ST_RK4_4
+        for (int c = 0; c < nv; ++c) res[c] = a.U[c][i] + (a.kacc[c][i] + du[c]) * dt / 6;
+        break;
     }
-    case ST_HEUN1:
-      for (int c = 0; c < nv; ++c) res[c] = a.U[c][i] + du[c] * dt;
-      break;
-    case ST_HEUN2:
-      for (int c = 0; c < nv; ++c) res[c] = (a.S[c][i] + (a.U[c][i] + du[c] * dt)) / 2;
-      break;
-    case ST_RK4_1:
-      for (int c = 0; c < nv; ++c) { a.kacc[c][i] = du[c]; res[c] = a.U[c][i] + du[c] * dt / 2; }
-      break;
-    case ST_RK4_2:
-      for (int c = 0; c < nv; ++c) { a.kacc[c][i] = a.kacc[c][i] + 2 * du[c]; res[c] = a.U[c][i] + du[c] * dt / 2; }
-      break;
-    case ST_RK4_3:
-      for (int c = 0; c < nv; ++c) { a.kacc[c][i] = a.kacc[c][i] + 2 * du[c]; res[c] = a.U[c][i] + du[c] * dt; }
-      break;
-    default:  // ST_RK4_4
-      for (int c = 0; c < nv; ++c) res[c] = a.U[c][i] + (a.kacc[c][i] + du[c]) * dt / 6;
-      break;
   }
 
-  const bool final_stage = (a.kind == ST_EULER || a.kind == ST_EM || a.kind == ST_HEUN2 || a.kind == ST_RK4_4);
+  const bool final_stage = NOISE || (a.kind == ST_EULER || a.kind == ST_HEUN2 || a.kind == ST_RK4_4);
   if (final_stage && a.periodic) {
 #pragma unroll
     for (int c = 0; c < D; ++c) {
@@ -224,28 +284,32 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
 // --------------------------------------------------------- rebuild chain
 
 __global__ __launch_bounds__(256) void k_keys(RebuildArgs a) {
-  if (a.step_inc && blockIdx.x == 0 && threadIdx.x == 0) *a.step_inc += 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // runs whether or not this step rebuilds: advance the device step and
+    // clear the flag the next step's final stage will set
+    if (a.step_inc) *a.step_inc += 1;
+    if (a.flag_clear) *a.flag_clear = 0;
+  }
   if (a.flag && *a.flag == 0) return;
   const RowIdx ri = row_index(a.g);
   const SimDev& sd = a.g.sims[ri.sim];
   const SimState& st = a.g.st[ri.sim];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (a.flag_clear) *a.flag_clear = 0;
-    if (a.rebuilds) *a.rebuilds += 1;
-  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && a.rebuilds) *a.rebuilds += 1;
   if (ri.local == 0) a.count[sd.cell_off + sd.ncells + 1] = (int32_t)(sd.n_max - st.n_tot);
   if (ri.local >= st.n_tot) return;
   const int64_t r = ri.row;
-  int32_t c;
+  int32_t c, lin = 0;
   if (a.dead && a.dead[r]) c = (int32_t)sd.ncells;
   else if (a.full) c = 0;
   else {
     int32_t q[3] = {0, 0, 0};
     for (int d = 0; d < a.dims; ++d) q[d] = axis_cell(sd, d, a.pos[d][r], a.periodic, a.geometric);
-    c = q[0] + sd.gn[0] * (q[1] + sd.gn[1] * q[2]);
+    lin = q[0] + sd.gn[0] * (q[1] + sd.gn[1] * q[2]);
+    c = a.crank[sd.rank_off + lin];          // bucket = Morton rank of the cell
   }
   c += (int32_t)sd.cell_off;
   a.key[r] = c;
+  a.lin[r] = lin;
   atomicAdd(&a.count[c], 1);
 }
 
@@ -386,7 +450,7 @@ __global__ __launch_bounds__(256) void k_permute(PermuteArgs pa) {
     else
       ((uint32_t*)pa.cols.alt[k])[dst] = ((const uint32_t*)pa.cols.cur[k])[r];
   }
-  a.cell_row[dst] = c;
+  a.cell_row[dst] = a.lin[r];
 }
 
 // Verlet list build on the freshly sorted rows (alternate buffers), then
@@ -395,26 +459,21 @@ __global__ __launch_bounds__(256) void k_permute(PermuteArgs pa) {
 // the order of the reference brute-force build (src/neighbors.jl:114-126).
 
 template <int D>
-__global__ __launch_bounds__(256) void k_verlet(VerletArgs va) {
+__device__ __forceinline__ int32_t verlet_row(const VerletArgs& va, const SimDev& sd, int64_t i) {
   const RebuildArgs& a = va.rb;
-  if (a.flag && *a.flag == 0) return;
-  const RowIdx ri = row_index(a.g);
-  const SimDev& sd = a.g.sims[ri.sim];
-  const int64_t N = a.g.st[ri.sim].N;
-  if (ri.local >= N) return;
-  const int64_t i = ri.row;
   double xi[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int c = 0; c < D; ++c) xi[c] = va.apos[c][i];
-  const int32_t c = a.cell_row[i] - (int32_t)sd.cell_off;
+  const int32_t c = a.cell_row[i];
   const int32_t cx = c % sd.gn[0];
   const int32_t cy = (c / sd.gn[0]) % sd.gn[1];
   const int32_t cz = c / (sd.gn[0] * sd.gn[1]);
+  const int32_t* __restrict__ rank = a.crank + sd.rank_off;
   int32_t cnt = 0;
   for (int k = 0; k < pow3(D); ++k) {
     const int32_t nc = neigh_cell<D>(sd, cx, cy, cz, k, a.periodic);
     if (nc < 0) continue;
-    const int64_t gc = sd.cell_off + nc;
+    const int64_t gc = sd.cell_off + rank[nc];
     const int32_t j0 = a.start[gc], j1 = a.start[gc + 1];
     for (int32_t j = j0; j < j1; ++j) {
       if (j == i) continue;
@@ -445,7 +504,8 @@ __global__ __launch_bounds__(256) void k_verlet(VerletArgs va) {
       ++cnt;
     }
   }
-  va.list_n[i] = cnt < va.cap ? cnt : va.cap;
+  const int32_t kept = cnt < va.cap ? cnt : va.cap;
+  va.list_n[i] = kept;
   for (int k = 0; k < va.cols.n; ++k) {
     if (va.cols.bytes[k] == 8)
       ((uint64_t*)va.cols.cur[k])[i] = ((const uint64_t*)va.cols.alt[k])[i];
@@ -454,7 +514,21 @@ __global__ __launch_bounds__(256) void k_verlet(VerletArgs va) {
   }
 #pragma unroll
   for (int q = 0; q < D; ++q) va.xref[q][i] = xi[q];
+  return kept;
 }
+
+
+template <int D>
+__global__ __launch_bounds__(256) void k_verlet(VerletArgs va) {
+  const RebuildArgs& a = va.rb;
+  if (a.flag && *a.flag == 0) return;
+  const RowIdx ri = row_index(a.g);
+  const SimDev& sd = a.g.sims[ri.sim];
+  const int64_t N = a.g.st[ri.sim].N;
+  if (ri.local >= N) return;
+  (void)verlet_row<D>(va, sd, ri.row);
+}
+
 
 // --------------------------------------------------------------- utilities
 __global__ void k_set_flag(int32_t* flag, int32_t v) { if (threadIdx.x == 0 && blockIdx.x == 0) *flag = v; }
@@ -523,33 +597,43 @@ __global__ __launch_bounds__(256) void k_export(ExportArgs e) {
 }
 
 // ------------------------------------------------------------- launchers
-template <int D, int NB>
+template <int D, int NB, int PER, int NOISE>
 static hipError_t launch_stage_law(int law, dim3 grid, dim3 block, hipStream_t s, const StageArgs& a) {
   switch (law) {
-    case LAW_NONE: hipLaunchKernelGGL((k_stage<D, NB, LAW_NONE>), grid, block, 0, s, a); break;
-    case LAW_LINEAR: hipLaunchKernelGGL((k_stage<D, NB, LAW_LINEAR>), grid, block, 0, s, a); break;
-    case LAW_REP_ADH: hipLaunchKernelGGL((k_stage<D, NB, LAW_REP_ADH>), grid, block, 0, s, a); break;
-    case LAW_AGGREGATION: hipLaunchKernelGGL((k_stage<D, NB, LAW_AGGREGATION>), grid, block, 0, s, a); break;
+    case LAW_LINEAR: hipLaunchKernelGGL((k_stage<D, NB, LAW_LINEAR, PER, NOISE>), grid, block, 0, s, a); break;
+    case LAW_REP_ADH: hipLaunchKernelGGL((k_stage<D, NB, LAW_REP_ADH, PER, NOISE>), grid, block, 0, s, a); break;
+    case LAW_AGGREGATION: hipLaunchKernelGGL((k_stage<D, NB, LAW_AGGREGATION, PER, NOISE>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-template <int D>
+template <int D, int PER, int NOISE>
 static hipError_t launch_stage_nb(int nb, int law, dim3 grid, dim3 block, hipStream_t s, const StageArgs& a) {
   switch (nb) {
-    case NB_FULL: return launch_stage_law<D, NB_FULL>(law, grid, block, s, a);
-    case NB_CELL: return launch_stage_law<D, NB_CELL>(law, grid, block, s, a);
-    default: return launch_stage_law<D, NB_LIST>(law, grid, block, s, a);
+    case NB_FULL: return launch_stage_law<D, NB_FULL, PER, NOISE>(law, grid, block, s, a);
+    case NB_CELL: return launch_stage_law<D, NB_CELL, PER, NOISE>(law, grid, block, s, a);
+    default: return launch_stage_law<D, NB_LIST, PER, NOISE>(law, grid, block, s, a);
   }
+}
+
+template <int D, int NOISE>
+static hipError_t launch_stage_dims(int nb, int law, int periodic, dim3 grid, dim3 block, hipStream_t s, const StageArgs& a) {
+  if (law == LAW_NONE) {      // no interaction body: no neighbor walk at all
+    hipLaunchKernelGGL((k_stage<D, NB_FULL, LAW_NONE, 0, NOISE>), grid, block, 0, s, a);
+    return hipGetLastError();
+  }
+  return periodic ? launch_stage_nb<D, 1, NOISE>(nb, law, grid, block, s, a)
+                  : launch_stage_nb<D, 0, NOISE>(nb, law, grid, block, s, a);
 }
 
 hipError_t launch_stage(int dims, int nb, int law, int nblocks, int block, hipStream_t s, const StageArgs& a) {
   dim3 grid(nblocks), blk(block);
+  const bool noise = a.kind == ST_EM;
   switch (dims) {
-    case 1: return launch_stage_nb<1>(nb, law, grid, blk, s, a);
-    case 2: return launch_stage_nb<2>(nb, law, grid, blk, s, a);
-    default: return launch_stage_nb<3>(nb, law, grid, blk, s, a);
+    case 1: return noise ? launch_stage_dims<1, 1>(nb, law, a.periodic, grid, blk, s, a) : launch_stage_dims<1, 0>(nb, law, a.periodic, grid, blk, s, a);
+    case 2: return noise ? launch_stage_dims<2, 1>(nb, law, a.periodic, grid, blk, s, a) : launch_stage_dims<2, 0>(nb, law, a.periodic, grid, blk, s, a);
+    default: return noise ? launch_stage_dims<3, 1>(nb, law, a.periodic, grid, blk, s, a) : launch_stage_dims<3, 0>(nb, law, a.periodic, grid, blk, s, a);
   }
 }
 

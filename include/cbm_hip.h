@@ -59,7 +59,9 @@ enum cbm_neighbors {
 /* ------------------------------------------------------ interaction laws */
 /* Built-in agentODE interaction bodies (the soft-sphere idioms of
  * src/models/softSpheres.jl and docs/src/Aggregation.md:62-123).  Each is a
- * pair force F_ij = s(d) * (x_i - x_j)/d summed over neighbors.
+ * pair force summed over neighbors: F_ij = (s(d)/d) * (x_i - x_j) for LINEAR
+ * and REP_ADH, and s(d) * ((x_i - x_j)/d) per axis for AGGREGATION (the
+ * reference's dirx = (x - x[i2])/d).
  *   CBM_LAW_LINEAR       p = {k, c}:            s = k (c - d)           for d < c
  *   CBM_LAW_REP_ADH      p = {k, r_eq, eps, f}: c = 2 r_eq, dc = f r_eq
  *                          s = k (c - d)                              d < c

@@ -281,12 +281,14 @@ static inline bool law_s(int law, const T* p, const T& d, T& s) {
     return false;
   }
   if (law == CBM_LAW_REP_ADH) {
+    // c = 2 r_eq, dc = f r_eq, a4 = 4 eps / (dc - c)^2 (well of depth eps)
     T c = T(2.0) * p[1];
     T dc = p[3] * p[1];
     if (val(d) < val(c)) { s = p[0] * (c - d); return true; }
     if (val(d) < val(dc)) {
       T w = dc - c;
-      s = T(0.0) - p[2] * T(4.0) * (d - c) * (dc - d) / (w * w);
+      T a4 = (T(4.0) * p[2]) / (w * w);
+      s = T(0.0) - a4 * (d - c) * (dc - d);
       return true;
     }
     return false;
@@ -311,8 +313,15 @@ static void deriv(const Orc& o, int64_t i, const std::vector<double>* S, double*
       for (int a = 0; a < o.dims; ++a) dx[a] = o.mi(a, S[a][i] - S[a][j]);
       double d = dist(o, dx);
       double sv;
-      if (law_s(law, p, d, sv))
-        for (int a = 0; a < o.dims; ++a) a3[a] = a3[a] + sv * (dx[a] / d);
+      if (law_s(law, p, d, sv)) {
+        if (law == CBM_LAW_AGGREGATION) {
+          // docs/src/Aggregation.md:88-98: dirx = (x - x[i2])/d; fx += s*dirx
+          for (int a = 0; a < o.dims; ++a) a3[a] = a3[a] + sv * (dx[a] / d);
+        } else {
+          const double f = sv / d;          // built-in laws: F = (s/d) (x_i - x_j)
+          for (int a = 0; a < o.dims; ++a) a3[a] = a3[a] + f * dx[a];
+        }
+      }
     });
     if (law == CBM_LAW_AGGREGATION && p[4] != 0.0) {
       // wall push of docs/src/Aggregation.md:100-112
@@ -654,8 +663,10 @@ static T traj_loss(Orc& o, const T* theta, int64_t n_steps, int32_t n_saves,
         for (int a = 0; a < d; ++a) dx[a] = o.mi(a, X[a][i] - X[a][j]);
         T dd = dist(o, dx);
         T sv;
-        if (law_s(CBM_LAW_REP_ADH, P, dd, sv))
-          for (int a = 0; a < d; ++a) acc[a] = acc[a] + sv * (dx[a] / dd);
+        if (law_s(CBM_LAW_REP_ADH, P, dd, sv)) {
+          T f = sv / dd;
+          for (int a = 0; a < d; ++a) acc[a] = acc[a] + f * dx[a];
+        }
       }
       for (int a = 0; a < d; ++a) K[a][i] = acc[a];
     }

@@ -208,8 +208,8 @@ def test_full_and_clvd_parity(gpu_available):
     n = 500
     box = [[0.0, 15.0]] * 2
     pos = uniform_positions(n, box, 13)
-    for nb, kw in [(A.FULL, {}), (A.CLVD, dict(skin=2.6, max_neighbors=32, cell_edge=3.0)),
-                   (A.VERLET_TIME, dict(skin=2.6, max_neighbors=32, dt_recompute=0.05))]:
+    for nb, kw in [(A.FULL, {}), (A.CLVD, dict(skin=2.6, max_neighbors=96, cell_edge=3.0)),
+                   (A.VERLET_TIME, dict(skin=2.6, max_neighbors=96, dt_recompute=0.05))]:
         m = model(2, integrator=A.HEUN, neighbors=nb, law=A.LAW_LINEAR, dt=0.01, **kw)
         s = sim(n, box, [1.0, 2.0])
         o, g = pair(m, s, pos)
