@@ -38,7 +38,7 @@ struct cbm_ctx {
   std::vector<cbm_sim_desc> sims;
   int S = 1;
   int dims = 1, nvar = 1;
-  bool growth = false, lagged = false, verlet = false, cl = false, full = false, vtime = false;
+  bool growth = false, lagged = false, verlet = false, cl = false, full = false, vtime = false, rules = false;
   int nb_kind = NB_FULL;
   int block = 256;
   int device = 0;
@@ -64,6 +64,11 @@ struct cbm_ctx {
   double* Fnew[3] = {nullptr, nullptr, nullptr};
   double* gr_new = nullptr;     // growth rule write-new buffers
   double* vb_new = nullptr;
+  int32_t* n_add_slot = nullptr; // rule decisions by logical slot
+  int32_t* dead_slot = nullptr;
+  int32_t* off_add = nullptr;    // their exclusive prefixes
+  int32_t* off_dead = nullptr;
+  int32_t* holes = nullptr;
   double* h[2][MAX_STATE] = {};
   double* kacc[MAX_STATE] = {};
   double* xref[3] = {nullptr, nullptr, nullptr};
@@ -81,6 +86,13 @@ struct cbm_ctx {
   int32_t* flags = nullptr;               // [0..1] rebuild flags by step parity, [2] list overflow, [3] capacity
   int64_t* counters = nullptr;            // [0] rebuilds, [1] device step
   bool structure_valid = false;
+
+  // medium
+  bool medium = false;
+  double* med_c[2] = {nullptr, nullptr};
+  int med_cur = 0;
+  int32_t* med_count = nullptr;
+  int64_t med_nodes = 0;
 
   // staging for transfers
   void* d_staging = nullptr;
@@ -157,8 +169,14 @@ static int validate(cbm_ctx* c) {
   if (verlet && !(m.skin > 0)) return fail(c, CBM_ERR_CONFIG, "Verlet algorithms need skin > 0");
   if (verlet && (m.max_neighbors < 1 || m.max_neighbors > 4096)) return fail(c, CBM_ERR_CONFIG, "nMaxNeighbors must be in [1, 4096]");
   if (m.neighbors == CBM_VERLET_TIME && !(m.dt_recompute > 0)) return fail(c, CBM_ERR_CONFIG, "VerletTime needs dtNeighborRecompute > 0");
-  if (m.rule != CBM_RULE_NONE) return fail(c, CBM_ERR_CONFIG, "agent rules are not available in this build");
-  if (m.medium_n[0] > 0) return fail(c, CBM_ERR_CONFIG, "medium is not available in this build");
+  if (m.rule < CBM_RULE_NONE || m.rule > CBM_RULE_DEATH) return fail(c, CBM_ERR_CONFIG, "unknown agent rule");
+  if (m.rule != CBM_RULE_NONE && c->S != 1) return fail(c, CBM_ERR_CONFIG, "agent rules (division/death) need a single simulation per context");
+  if (m.medium_n[0] > 0) {
+    if (c->S != 1) return fail(c, CBM_ERR_CONFIG, "a medium needs a single simulation per context");
+    for (int a = 0; a < m.dims; ++a)
+      if (m.medium_n[a] < 1) return fail(c, CBM_ERR_CONFIG, "NMedium must be >= 1 on every axis");
+    if (!(m.medium_D >= 0.0) || !(m.medium_consume >= 0.0)) return fail(c, CBM_ERR_CONFIG, "medium D and uptake must be >= 0");
+  }
   for (int s = 0; s < c->S; ++s) {
     const cbm_sim_desc& sd = c->sims[s];
     if (sd.n < 0 || sd.n_max < sd.n || sd.n_max < 1) return fail(c, CBM_ERR_CONFIG, "need 0 <= N <= n_max and n_max >= 1");
@@ -279,6 +297,8 @@ static void destroy_ctx(cbm_ctx* c) {
   for (int a = 0; a < 3; ++a) { F(c->Fnew[a]); F(c->xref[a]); }
   for (int v = 0; v < MAX_STATE; ++v) { F(c->h[0][v]); F(c->h[1][v]); F(c->kacc[v]); }
   F(c->gr_new); F(c->vb_new); F(c->dead);
+  F(c->n_add_slot); F(c->dead_slot); F(c->off_add); F(c->off_dead); F(c->holes);
+  F(c->med_c[0]); F(c->med_c[1]); F(c->med_count);
   F(c->key); F(c->lin); F(c->crank); F(c->count); F(c->cursor); F(c->start); F(c->tmp); F(c->cell_row);
   F(c->scan_status); F(c->scan_ticket); F(c->list); F(c->list_n); F(c->flags); F(c->counters);
   F(c->d_staging); F(c->d_block_sim); F(c->d_block_off); F(c->d_sims); F(c->d_st);
@@ -294,7 +314,8 @@ static int init_ctx(cbm_ctx* c) {
   int rc = validate(c);
   if (rc) return rc;
   c->dims = m.dims;
-  c->growth = false;
+  c->growth = m.rule == CBM_RULE_GROWTH_DIVISION;
+  c->rules = m.rule != CBM_RULE_NONE;
   c->nvar = c->dims + (c->growth ? 1 : 0);
   c->lagged = m.lagged != 0;
   c->full = m.neighbors == CBM_FULL;
@@ -326,6 +347,33 @@ static int init_ctx(cbm_ctx* c) {
   if ((rc = alloc_column(c, C_SLOT, 4))) return rc;
   if (c->lagged) {
     for (int a = 0; a < c->dims; ++a) { if ((rc = alloc_column(c, C_FX + a, 8))) return rc; HIPCHK(c, dalloc(&c->Fnew[a], c->C)); }
+  }
+  if (c->growth) {
+    if ((rc = alloc_column(c, C_VOL, 8))) return rc;
+    if ((rc = alloc_column(c, C_GR, 8))) return rc;
+    if ((rc = alloc_column(c, C_VB, 8))) return rc;
+    HIPCHK(c, dalloc(&c->gr_new, c->C));
+    HIPCHK(c, dalloc(&c->vb_new, c->C));
+  }
+  if (c->rules) {
+    if ((rc = alloc_column(c, C_LF, 8))) return rc;
+    const int64_t nm = c->sims[0].n_max;
+    HIPCHK(c, dalloc(&c->dead, c->C));
+    HIPCHK(c, dalloc(&c->n_add_slot, nm + 1));
+    HIPCHK(c, dalloc(&c->dead_slot, nm + 1));
+    HIPCHK(c, dalloc(&c->off_add, nm + 1));
+    HIPCHK(c, dalloc(&c->off_dead, nm + 1));
+    HIPCHK(c, dalloc(&c->holes, nm + 1));
+  }
+  if (m.medium_n[0] > 0) {
+    c->medium = true;
+    c->med_nodes = 1;
+    for (int a = 0; a < c->dims; ++a) c->med_nodes *= m.medium_n[a];
+    HIPCHK(c, dalloc(&c->med_c[0], c->med_nodes));
+    HIPCHK(c, dalloc(&c->med_c[1], c->med_nodes));
+    HIPCHK(c, dalloc(&c->med_count, c->med_nodes));
+    std::vector<double> c0((size_t)c->med_nodes, m.medium_c0);
+    HIPCHK(c, hipMemcpy(c->med_c[0], c0.data(), sizeof(double) * c->med_nodes, hipMemcpyHostToDevice));
   }
   // slot = local row, id = local row + 1 (id = 1:N, BASEPARAMETERS constants.jl:21)
   {
@@ -404,7 +452,7 @@ static RebuildArgs rebuild_args(cbm_ctx* c, const int32_t* flag, int32_t* flag_c
   a.rebuilds = c->counters;
   a.step_inc = tick ? c->counters + 1 : nullptr;
   for (int d = 0; d < c->dims; ++d) a.pos[d] = (const double*)c->col[C_X + d].cur();
-  a.dead = nullptr;
+  a.dead = c->dead;
   a.slot = (const int32_t*)c->col[C_SLOT].cur();
   a.key = c->key; a.lin = c->lin; a.crank = c->crank; a.count = c->count; a.cursor = c->cursor; a.start = c->start; a.tmp = c->tmp;
   a.cell_row = c->cell_row;
@@ -441,6 +489,7 @@ static int rebuild_chain(cbm_ctx* c, const int32_t* flag, int32_t* flag_clear, b
     va.list = c->list; va.list_n = c->list_n; va.ldl = c->C; va.cap = c->cap;
     va.overflow = c->flags + 2;
     va.rl = c->list_radius;
+    va.build = c->verlet ? 1 : 0;
     HIPCHK(c, launch_verlet(c->dims, c->nblocks, c->block, c->stream, va));
   } else {
     for (int k = 0; k < C_NCOLS; ++k) if (c->has_col[k]) c->col[k].cidx ^= 1;
@@ -481,6 +530,7 @@ static StageArgs stage_base(cbm_ctx* c) {
   a.walls = (c->m.law == CBM_LAW_AGGREGATION) ? 1 : 0;
   a.nvar = c->nvar;
   a.write_fnew = c->lagged ? 1 : 0;
+  if (c->growth) a.growth = (const double*)c->col[C_GR].cur();
   return a;
 }
 
@@ -512,10 +562,93 @@ static void flip_state(cbm_ctx* c) {
 
 static int neighbor_update(cbm_ctx* c, int64_t s, bool disp);
 
+static RuleArgs rule_args(cbm_ctx* c, int32_t* flag) {
+  RuleArgs r;
+  std::memset(&r, 0, sizeof(r));
+  r.g = geom(c);
+  r.rule = c->m.rule; r.dims = c->dims; r.lagged = c->lagged ? 1 : 0; r.growth = c->growth ? 1 : 0;
+  for (int q = 0; q < 4; ++q) r.rp[q] = c->m.rule_p[q];
+  r.dt = c->m.dt; r.seed = c->m.seed; r.step_ptr = c->counters + 1;
+  for (int d = 0; d < c->dims; ++d) { r.pos[d] = (double*)c->col[C_X + d].cur(); r.pos_alt[d] = (double*)c->col[C_X + d].alt(); }
+  if (c->growth) {
+    r.vol = (double*)c->col[C_VOL].cur();
+    r.vol_alt = (double*)c->col[C_VOL].alt();
+    r.gr = (const double*)c->col[C_GR].cur(); r.gr_new = c->gr_new;
+    r.vb = (const double*)c->col[C_VB].cur(); r.vb_new = c->vb_new;
+  }
+  if (c->lagged) for (int d = 0; d < c->dims; ++d) { r.F[d] = (double*)c->col[C_FX + d].cur(); r.Fnew[d] = c->Fnew[d]; }
+  r.id = (int64_t*)c->col[C_ID].cur();
+  r.lf = (int64_t*)c->col[C_LF].cur();
+  r.slot = (int32_t*)c->col[C_SLOT].cur();
+  r.dead = c->dead;
+  r.n_add_slot = c->n_add_slot; r.dead_slot = c->dead_slot;
+  r.off_add = c->off_add; r.off_dead = c->off_dead;
+  r.holes = c->holes;
+  r.err = c->flags + 3;
+  r.rebuild_flag = flag;
+  return r;
+}
+
+static int slot_scan(cbm_ctx* c, int32_t* in, int32_t* out, int64_t n) {
+  ScanArgs s;
+  std::memset(&s, 0, sizeof(s));
+  s.flag = nullptr; s.count = in; s.cursor = nullptr; s.start = out; s.n = n;
+  s.status = c->scan_status; s.ticket = c->scan_ticket; s.done = c->scan_ticket + 1;
+  s.epoch = c->scan_ticket + 2; s.nblocks = scan_blocks(n);
+  HIPCHK(c, launch_scan(c->stream, s));
+  return CBM_OK;
+}
+
+// agentStepRule!: decisions by slot, their prefixes, then the effects.
+static int rule_phase(cbm_ctx* c, int32_t* flag) {
+  RuleArgs r = rule_args(c, flag);
+  const int64_t nm = c->sims[0].n_max;
+  HIPCHK(c, launch_rule_decide(c->nblocks, c->block, c->stream, r));
+  int rc;
+  if ((rc = slot_scan(c, c->n_add_slot, c->off_add, nm))) return rc;
+  if ((rc = slot_scan(c, c->dead_slot, c->off_dead, nm))) return rc;
+  HIPCHK(c, launch_rule_apply(c->nblocks, c->block, c->stream, r));
+  return CBM_OK;
+}
+
+// update!: canonical compaction of the logical slots and the new counts.
+static int compact_phase(cbm_ctx* c, int32_t* flag) {
+  RuleArgs r = rule_args(c, flag);
+  HIPCHK(c, launch_compact_holes(c->nblocks, c->block, c->stream, r));
+  HIPCHK(c, launch_compact_slots(c->nblocks, c->block, c->stream, r));
+  HIPCHK(c, launch_commit_counts(c->stream, r, c->S));
+  return CBM_OK;
+}
+
+// mediumStepDE!: agents per node, then one FTCS step of the field.
+static int medium_phase(cbm_ctx* c) {
+  MediumArgs md;
+  std::memset(&md, 0, sizeof(md));
+  md.g = geom(c);
+  for (int d = 0; d < c->dims; ++d) md.pos[d] = (const double*)c->col[C_X + d].cur();
+  md.c = c->med_c[c->med_cur];
+  md.cn = c->med_c[1 - c->med_cur];
+  md.count = c->med_count;
+  md.dims = c->dims;
+  for (int a = 0; a < 3; ++a) {
+    md.n[a] = a < c->dims ? c->m.medium_n[a] : 1;
+    md.lo[a] = c->sims[0].box_lo[a];
+    md.hi[a] = c->sims[0].box_hi[a];
+    md.dx[a] = a < c->dims ? (md.hi[a] - md.lo[a]) / (double)md.n[a] : 1.0;   // communityStructure.jl:274
+  }
+  md.D = c->m.medium_D; md.uptake = c->m.medium_consume; md.dt = c->m.dt;
+  md.nodes = c->med_nodes;
+  HIPCHK(c, launch_medium_count(c->nblocks, c->block, c->stream, md));
+  HIPCHK(c, launch_medium_step(c->stream, md));
+  c->med_cur ^= 1;
+  return CBM_OK;
+}
+
 static int step_once(cbm_ctx* c) {
   int rc;
   if (!c->structure_valid && (rc = force_rebuild(c))) return rc;
   const int64_t s = c->step;
+  if (c->rules && (rc = rule_phase(c, c->flags + (s & 1)))) return rc;
   StageArgs a = stage_base(c);
   int32_t* flag_now = c->flags + (s & 1);
   const bool disp = c->verlet && !c->vtime;
@@ -555,14 +688,20 @@ static int step_once(cbm_ctx* c) {
       if ((rc = run_stage(c, a, ST_RK4_4))) return rc;
       break;
   }
-  // update!: commit the write-new force accumulator (read-old/write-new)
+  // update!: commit the write-new buffers (read-old/write-new), then compact
+  auto commit = [](Column& col, double*& fresh) {
+    void* committed = col.buf[col.cidx];
+    col.buf[col.cidx] = fresh;
+    fresh = (double*)committed;
+  };
   if (c->lagged)
-    for (int d = 0; d < c->dims; ++d) {
-      Column& col = c->col[C_FX + d];
-      void* committed = col.buf[col.cidx];
-      col.buf[col.cidx] = c->Fnew[d];
-      c->Fnew[d] = (double*)committed;
-    }
+    for (int d = 0; d < c->dims; ++d) commit(c->col[C_FX + d], c->Fnew[d]);
+  if (c->growth) {
+    commit(c->col[C_GR], c->gr_new);
+    commit(c->col[C_VB], c->vb_new);
+  }
+  if (c->medium && (rc = medium_phase(c))) return rc;   // mediumStepDE!
+  if (c->rules && (rc = compact_phase(c, c->flags + (s & 1)))) return rc;
   c->t = c->t + c->m.dt;
   c->step++;
   // computeNeighbors!
@@ -584,6 +723,8 @@ static int neighbor_update(cbm_ctx* c, int64_t s, bool disp) {
     HIPCHK(c, launch_set_flag(c->stream, f, 1));
     if ((rc = rebuild_chain(c, f, c->flags + ((s + 1) & 1), true, true))) return rc;
     c->t_next = c->t + c->m.dt_recompute;
+  } else if (c->rules) {   // rebuild only when the population changed (device flag)
+    if ((rc = rebuild_chain(c, flag_now, c->flags + ((s + 1) & 1), true, true))) return rc;
   } else {
     HIPCHK(c, launch_tick(c->stream, c->counters + 1));
   }
@@ -631,6 +772,10 @@ static int field_col(cbm_ctx* c, int32_t field, int& k) {
       if (field >= c->dims) return CBM_ERR_ARG;
       k = C_X + field; return CBM_OK;
     case CBM_F_ID: k = C_ID; return CBM_OK;
+    case CBM_F_VOL: if (!c->growth) return CBM_ERR_ARG; k = C_VOL; return CBM_OK;
+    case CBM_F_GROWTH: if (!c->growth) return CBM_ERR_ARG; k = C_GR; return CBM_OK;
+    case CBM_F_VBIRTH: if (!c->growth) return CBM_ERR_ARG; k = C_VB; return CBM_OK;
+    case CBM_F_LF: if (!c->rules) return CBM_ERR_ARG; k = C_LF; return CBM_OK;
     case CBM_F_FX: case CBM_F_FY: case CBM_F_FZ:
       if (!c->lagged || field - CBM_F_FX >= c->dims) return CBM_ERR_ARG;
       k = C_FX + (field - CBM_F_FX); return CBM_OK;
@@ -693,8 +838,23 @@ int cbm_set_n(cbm_ctx* c, int32_t sim, int64_t n) {
   return CBM_OK;
 }
 
-int cbm_upload_medium(cbm_ctx* c, const double*, int64_t) { return fail(c, CBM_ERR_CONFIG, "model has no medium"); }
-int cbm_download_medium(cbm_ctx* c, double*, int64_t) { return fail(c, CBM_ERR_CONFIG, "model has no medium"); }
+int cbm_upload_medium(cbm_ctx* c, const double* host, int64_t n) {
+  if (!c) return CBM_ERR_ARG;
+  if (!c->medium) return fail(c, CBM_ERR_CONFIG, "model has no medium");
+  if (!host || n != c->med_nodes) return fail(c, CBM_ERR_ARG, "medium upload needs prod(NMedium) values");
+  HIPCHK(c, hipMemcpyAsync(c->med_c[c->med_cur], host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CBM_OK;
+}
+
+int cbm_download_medium(cbm_ctx* c, double* host, int64_t n) {
+  if (!c) return CBM_ERR_ARG;
+  if (!c->medium) return fail(c, CBM_ERR_CONFIG, "model has no medium");
+  if (!host || n != c->med_nodes) return fail(c, CBM_ERR_ARG, "medium download needs prod(NMedium) values");
+  HIPCHK(c, hipMemcpyAsync(host, c->med_c[c->med_cur], sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CBM_OK;
+}
 
 int cbm_neighbors(cbm_ctx* c) {
   if (!c) return CBM_ERR_ARG;
@@ -726,6 +886,9 @@ int cbm_step_graph(cbm_ctx* c, int64_t nsteps, int64_t replays) {
   int cidx0[C_NCOLS];
   for (int k = 0; k < C_NCOLS; ++k) cidx0[k] = c->col[k].cidx;
   void* fnew0[3] = {c->Fnew[0], c->Fnew[1], c->Fnew[2]};
+  const int med0 = c->med_cur;
+  void* grn0 = c->gr_new;
+  void* vbn0 = c->vb_new;
   int64_t step0 = c->step;
   double t0 = c->t;
   hipGraph_t graph = nullptr;
@@ -739,6 +902,7 @@ int cbm_step_graph(cbm_ctx* c, int64_t nsteps, int64_t replays) {
   bool same = true;
   for (int k = 0; k < C_NCOLS; ++k) same = same && (c->col[k].cidx == cidx0[k]);
   for (int d = 0; d < 3; ++d) same = same && (c->Fnew[d] == fnew0[d]);
+  same = same && c->med_cur == med0 && (void*)c->gr_new == grn0 && (void*)c->vb_new == vbn0;
   if (!same) { (void)hipGraphDestroy(graph); return fail(c, CBM_ERR_CONFIG, "step buffers are not periodic over the captured steps"); }
   // capture advanced the host counters once; the replays do the real work
   c->step = step0; c->t = t0;

@@ -392,7 +392,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
     if (idx < a.n) {
       a.start[idx] = run;
       a.count[idx] = 0;
-      a.cursor[idx] = 0;
+      if (a.cursor) a.cursor[idx] = 0;
       run += v[k];
       if (idx == a.n - 1) a.start[a.n] = run;
     }
@@ -461,6 +461,15 @@ __global__ __launch_bounds__(256) void k_permute(PermuteArgs pa) {
 template <int D>
 __device__ __forceinline__ int32_t verlet_row(const VerletArgs& va, const SimDev& sd, int64_t i) {
   const RebuildArgs& a = va.rb;
+  if (!va.build) {
+    for (int k = 0; k < va.cols.n; ++k) {
+      if (va.cols.bytes[k] == 8)
+        ((uint64_t*)va.cols.cur[k])[i] = ((const uint64_t*)va.cols.alt[k])[i];
+      else
+        ((uint32_t*)va.cols.cur[k])[i] = ((const uint32_t*)va.cols.alt[k])[i];
+    }
+    return 0;
+  }
   double xi[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int c = 0; c < D; ++c) xi[c] = va.apos[c][i];
@@ -529,6 +538,215 @@ __global__ __launch_bounds__(256) void k_verlet(VerletArgs va) {
   (void)verlet_row<D>(va, sd, ri.row);
 }
 
+
+// ------------------------------------------------------------------ rules
+// Built-in agentRule bodies (src/AgentStructure/functionRule.jl:9-111 for
+// @addAgent/@removeAgent).  Decisions are pure functions of the agent's
+// committed state and Philox(seed, step, id, sim, purpose), so the decide and
+// apply passes agree and the result is independent of thread order.
+struct RuleDecision { int32_t add; int32_t dead; };
+
+__device__ __forceinline__ RuleDecision rule_decide(const RuleArgs& a, int sim, int64_t r, int64_t step) {
+  RuleDecision d{0, 0};
+  const int64_t id = a.id[r];
+  if (a.rule == 2) {                       // parity KAT: odd ids removed, even ids add one
+    if (id % 2 == 1) d.dead = 1; else d.add = 1;
+  } else if (a.rule == 3) {                // death with probability rate*dt
+    const u4 q = draw(a.seed, step, id, sim, 0x400u);
+    d.dead = u53(q.x, q.y) < a.rp[0] * a.dt ? 1 : 0;
+  } else if (a.rule == 1) {                // growth and division
+    d.add = (a.vol[r] >= a.rp[3] * a.vb[r]) ? 1 : 0;
+  }
+  return d;
+}
+
+__global__ __launch_bounds__(256) void k_rule_decide(RuleArgs a) {
+  const RowIdx ri = row_index(a.g);
+  const int64_t N = a.g.st[ri.sim].N;
+  if (ri.local >= N) return;
+  const int64_t r = ri.row;
+  const RuleDecision d = rule_decide(a, ri.sim, r, *a.step_ptr);
+  const int32_t s = a.slot[r];
+  a.n_add_slot[s] = d.add;
+  a.dead_slot[s] = d.dead;
+}
+
+// Re-evaluates the rule and writes its effects: parent updates in place,
+// daughters at rows N + (exclusive prefix of daughters by slot), which is the
+// reference's single-thread append order; new ids idMax + 1 + that prefix.
+__global__ __launch_bounds__(256) void k_rule_apply(RuleArgs a) {
+  const RowIdx ri = row_index(a.g);
+  SimState& st = a.g.st[ri.sim];
+  const SimDev& sd = a.g.sims[ri.sim];
+  const int64_t N = st.N;
+  const int64_t nmax = sd.n_max;
+  const int64_t A = a.off_add[nmax];
+  if (ri.local == 0) {
+    st.n_add = A;
+    st.n_remove = a.off_dead[nmax];
+    st.n_tot = (N + A <= nmax) ? N + A : nmax;
+    if (N + A > nmax) *a.err = 1;          // reference D12 drops the agent silently
+  }
+  if (ri.local >= N) return;
+  const int64_t r = ri.row;
+  const int64_t step = *a.step_ptr;
+  const RuleDecision d = rule_decide(a, ri.sim, r, step);
+  const int32_t s = a.slot[r];
+  a.dead[r] = (uint8_t)d.dead;
+  const int D = a.dims;
+  if (a.rule == 1) {                        // write-new values for every agent
+    a.gr_new[r] = a.gr[r];
+    a.vb_new[r] = a.vb[r];
+  }
+  if (a.rule == 2) {                        // x = x + 1 (test/testCommunity.jl:247)
+    const double xold = a.pos[0][r];
+    a.pos[0][r] = xold + 1.0;
+    if (d.add) {
+      const int64_t o = a.off_add[s];
+      const int64_t nw = N + o;
+      if (nw >= nmax) return;
+      const int64_t dst = sd.row_off + nw;
+      for (int c = 0; c < D; ++c) {
+        const double v = (c == 0) ? xold : a.pos[c][r];
+        a.pos[c][dst] = v;
+        a.pos_alt[c][dst] = v;
+      }
+      if (a.lagged) for (int c = 0; c < D; ++c) { a.F[c][dst] = a.F[c][r]; a.Fnew[c][dst] = a.F[c][r]; }
+      a.id[dst] = st.id_max + 1 + o;
+      a.lf[dst] = a.id[r];
+      a.slot[dst] = (int32_t)nw;
+      a.dead[dst] = 0;
+    }
+  } else if (a.rule == 1 && d.add) {
+    const int64_t o = a.off_add[s];
+    const int64_t nw = N + o;
+    if (nw >= nmax) return;
+    const int64_t dst = sd.row_off + nw;
+    const int64_t id = a.id[r];
+    const u4 ra = draw(a.seed, step, id, ri.sim, 0x200u);
+    const u4 rg = draw(a.seed, step, id, ri.sim, 0x300u);
+    double ax[3] = {0.0, 0.0, 0.0};
+    if (D == 1) ax[0] = (u53(ra.x, ra.y) < 0.5) ? -1.0 : 1.0;
+    else if (D == 2) {
+      const double th = 6.283185307179586 * u53(ra.x, ra.y);
+      ax[0] = cos(th); ax[1] = sin(th);
+    } else {
+      const double z = 2.0 * u53(ra.x, ra.y) - 1.0;
+      const double ph = 6.283185307179586 * u53(ra.z, ra.w);
+      const double rr = sqrt(1.0 - z * z);
+      ax[0] = rr * cos(ph); ax[1] = rr * sin(ph); ax[2] = z;
+    }
+    const double split = a.rp[2];
+    const double half = a.vol[r] / 2.0;
+    const double gm = a.rp[0], gs = a.rp[1];
+    const double g1 = gm * (1.0 + gs * (2.0 * u53(rg.x, rg.y) - 1.0));
+    const double g2 = gm * (1.0 + gs * (2.0 * u53(rg.z, rg.w) - 1.0));
+    for (int c = 0; c < D; ++c) {
+      const double x = a.pos[c][r];
+      const double xn = x + split * ax[c];
+      a.pos[c][dst] = xn;
+      a.pos_alt[c][dst] = xn;
+      a.pos[c][r] = x - split * ax[c];
+    }
+    if (a.lagged) for (int c = 0; c < D; ++c) { a.F[c][dst] = a.F[c][r]; a.Fnew[c][dst] = a.F[c][r]; }
+    a.vol[r] = half; a.vol[dst] = half; a.vol_alt[dst] = half;
+    a.vb_new[r] = half; a.vb_new[dst] = half; ((double*)a.vb)[dst] = half;
+    a.gr_new[r] = g1; a.gr_new[dst] = g2; ((double*)a.gr)[dst] = g2;
+    a.id[dst] = st.id_max + 1 + o;
+    a.lf[dst] = a.lf[r];
+    a.slot[dst] = (int32_t)nw;
+    a.dead[dst] = 0;
+  }
+}
+
+// update! compaction (src/CommunityStructure/update.jl:15-119) in the order
+// of the reference's serial run: holes ascending, survivors taken from the
+// tail; hole m receives survivor m whenever hole < survivor.  With
+// N' = N + A - R this is: alive slots below N' stay, the alive slots at or
+// above N' move, the m-th from the top into the m-th hole below N'.
+__global__ __launch_bounds__(256) void k_compact_holes(RuleArgs a) {
+  const RowIdx ri = row_index(a.g);
+  const SimState& st = a.g.st[ri.sim];
+  if (ri.local >= st.n_tot) return;
+  const int64_t r = ri.row;
+  if (!a.dead[r]) return;
+  const int64_t Np = st.N + st.n_add - st.n_remove;
+  const int32_t s = a.slot[r];
+  if (s < Np) a.holes[a.off_dead[s]] = s;
+}
+
+__global__ __launch_bounds__(256) void k_compact_slots(RuleArgs a) {
+  const RowIdx ri = row_index(a.g);
+  const SimState& st = a.g.st[ri.sim];
+  const int64_t N = st.N, A = st.n_add, R = st.n_remove;
+  const int64_t Np = N + A - R;
+  if (ri.local < st.n_tot) {
+    const int64_t r = ri.row;
+    if (!a.dead[r]) {
+      const int64_t s = a.slot[r];
+      if (s >= Np) {
+        const int64_t m = (N + A - 1 - s) - (R - (int64_t)a.off_dead[s]);
+        a.slot[r] = a.holes[m];
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ medium
+// Agents per medium node: cellInMesh (src/metrics.jl:18-20) per axis, the
+// nearest-lower node of a cell-centred grid, clamped to the grid.  Integer
+// atomics make the count independent of thread order; the reference's
+// per-agent read-modify-write of c is racy.
+__global__ __launch_bounds__(256) void k_medium_count(MediumArgs m) {
+  const RowIdx ri = row_index(m.g);
+  if (ri.sim != 0 || ri.local >= m.g.st[0].N) return;
+  const int64_t r = ri.row;
+  int64_t q[3] = {0, 0, 0};
+  for (int a = 0; a < m.dims; ++a) {
+    const double x = m.pos[a][r];
+    int64_t v;
+    if (x > m.hi[a]) v = m.n[a] - 1;
+    else if (x < m.lo[a]) v = 0;
+    else v = floor_div(x - m.lo[a], m.dx[a]);
+    q[a] = v > m.n[a] - 1 ? m.n[a] - 1 : v;
+  }
+  atomicAdd(&m.count[q[0] + (int64_t)m.n[0] * (q[1] + (int64_t)m.n[1] * q[2])], 1);
+}
+
+// FTCS: c' = c + dt (D sum_a (c[+a] - 2c + c[-a])/dx_a^2 - (uptake c) n_agents),
+// ghost nodes mirroring the boundary node (zero flux, conserves mass).
+__global__ __launch_bounds__(256) void k_medium_step(MediumArgs m) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m.nodes) return;
+  const int64_t nx = m.n[0], ny = m.n[1];
+  const int64_t p[3] = {k % nx, (k / nx) % ny, k / (nx * ny)};
+  const int64_t stride[3] = {1, nx, nx * ny};
+  const double cc = m.c[k];
+  double lap = 0.0;
+  for (int a = 0; a < m.dims; ++a) {
+    const double cp = (p[a] + 1 < m.n[a]) ? m.c[k + stride[a]] : cc;
+    const double cm = (p[a] > 0) ? m.c[k - stride[a]] : cc;
+    const double l = ((cp - 2 * cc) + cm) / (m.dx[a] * m.dx[a]);
+    lap = (a == 0) ? l : lap + l;
+  }
+  const int32_t na = m.count[k];
+  m.count[k] = 0;
+  const double f = m.D * lap - (m.uptake * cc) * (double)na;
+  m.cn[k] = cc + m.dt * f;
+}
+
+// N' = N + NAdd_ - NRemove_ and idMax_ += NAdd_ (update.jl:204-214), after
+// every compaction kernel has read the old counts; flags the neighbor
+// rebuild when the population changed.
+__global__ void k_commit_counts(RuleArgs a, int32_t nsims) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsims) return;
+  SimState& st = a.g.st[s];
+  const int64_t A = st.n_add, R = st.n_remove;
+  st.N = st.N + A - R;
+  st.id_max += A;
+  if (A + R > 0) *a.rebuild_flag = 1;
+}
 
 // --------------------------------------------------------------- utilities
 __global__ void k_set_flag(int32_t* flag, int32_t v) { if (threadIdx.x == 0 && blockIdx.x == 0) *flag = v; }
@@ -663,6 +881,36 @@ hipError_t launch_verlet(int dims, int nblocks, int block, hipStream_t s, const 
     case 2: hipLaunchKernelGGL(k_verlet<2>, dim3(nblocks), dim3(block), 0, s, a); break;
     default: hipLaunchKernelGGL(k_verlet<3>, dim3(nblocks), dim3(block), 0, s, a); break;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_medium_count(int nblocks, int block, hipStream_t s, const MediumArgs& a) {
+  hipLaunchKernelGGL(k_medium_count, dim3(nblocks), dim3(block), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_medium_step(hipStream_t s, const MediumArgs& a) {
+  hipLaunchKernelGGL(k_medium_step, dim3((unsigned)((a.nodes + 255) / 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_rule_decide(int nblocks, int block, hipStream_t s, const RuleArgs& a) {
+  hipLaunchKernelGGL(k_rule_decide, dim3(nblocks), dim3(block), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_rule_apply(int nblocks, int block, hipStream_t s, const RuleArgs& a) {
+  hipLaunchKernelGGL(k_rule_apply, dim3(nblocks), dim3(block), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_compact_holes(int nblocks, int block, hipStream_t s, const RuleArgs& a) {
+  hipLaunchKernelGGL(k_compact_holes, dim3(nblocks), dim3(block), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_compact_slots(int nblocks, int block, hipStream_t s, const RuleArgs& a) {
+  hipLaunchKernelGGL(k_compact_slots, dim3(nblocks), dim3(block), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_commit_counts(hipStream_t s, const RuleArgs& a, int32_t nsims) {
+  hipLaunchKernelGGL(k_commit_counts, dim3((nsims + 63) / 64), dim3(64), 0, s, a, nsims);
   return hipGetLastError();
 }
 
