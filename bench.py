@@ -84,6 +84,82 @@ def cpu_baseline(m, s, pos, budget_s):
                       f"(oracle/cbm_oracle.cpp, OpenMP, FP64)"}
 
 
+def sweep_workload(n_sims=512, seed=4):
+    """512 independent 2-D periodic soft-sphere simulations, N log-uniform in
+    [500, 50000], packing fraction U[0.2, 0.5], k U[0.5, 2], adhesion U[0, 0.2]."""
+    import cbm._abi as A
+    rng = np.random.default_rng(seed)
+    ns = np.round(np.exp(rng.uniform(np.log(500), np.log(50000), n_sims))).astype(np.int64)
+    sims, states = [], []
+    for n in ns:
+        phi = rng.uniform(0.2, 0.5)
+        L = float(np.sqrt(n * np.pi / phi))
+        s = A.SimDesc()
+        s.n = s.n_max = int(n)
+        s.box_hi[0] = s.box_hi[1] = L
+        for q, v in enumerate([rng.uniform(0.5, 2.0), 1.0, rng.uniform(0.0, 0.2), 2.4]):
+            s.p[q] = v
+        sims.append(s)
+        states.append([rng.uniform(0.0, L, n), rng.uniform(0.0, L, n)])
+    m = A.ModelDesc()
+    m.dims, m.integrator, m.law, m.periodic, m.dt = 2, A.EULER, A.LAW_REP_ADH, 1, 0.01
+    m.neighbors, m.cell_edge = A.CELL_LINKED, 2.4
+    return m, sims, states
+
+
+def run_sweep(steps, plain_sims, plain_steps):
+    """Batched (one context, one launch per kernel for all simulations) against
+    the plain path (one context per simulation, stepped one after another as
+    the reference's fitting loops do)."""
+    import tempfile
+    from cbm import autotune
+    from cbm.api import Context
+    m, sims, states = sweep_workload()
+
+    def upload(ctx, idx=None):
+        for q, st in enumerate(states if idx is None else [states[idx]]):
+            for a in range(2):
+                ctx.upload(a, st[a], sim=q)
+
+    cache = os.path.join(tempfile.gettempdir(), "cbm_tune.json")
+    mt, info = autotune.tune(m, sims, upload, cutoff=2.4, steps=10, warm=3, cache_path=cache)
+    ctx = Context(mt, sims)
+    upload(ctx)
+    ctx.neighbors()
+    block = autotune.tune_block(ctx, cache + ".blocks")
+    ctx.step(10)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.sync()
+    tb = time.perf_counter() - t0
+    total = sum(s.n for s in sims)
+    batched = total * steps / tb
+    ctx.close()
+    # plain path on a size-stratified sample of the same simulations
+    order = np.argsort([s.n for s in sims])
+    pick = order[np.linspace(0, len(order) - 1, plain_sims).astype(int)]
+    work, tp = 0, 0.0
+    for q in pick:
+        c1 = Context(mt, [sims[q]])
+        upload(c1, q)
+        c1.neighbors()
+        c1.step(5)
+        c1.sync()
+        t0 = time.perf_counter()
+        c1.step(plain_steps)
+        c1.sync()
+        tp += time.perf_counter() - t0
+        work += sims[q].n * plain_steps
+        c1.close()
+    plain = work / tp
+    return {"value": batched, "unit": "agent-steps/s", "plain_value": plain, "ratio": batched / plain,
+            "n_sims": len(sims), "agents": total, "steps": steps, "ms_per_step": tb / steps * 1e3,
+            "structure": info["choice"], "block": block,
+            "plain_sample": f"{plain_sims} simulations spanning the N range, {plain_steps} steps each",
+            "config": "C4 512 2-D periodic sims, N log-uniform [500, 50000], Euler dt=0.01, FP64"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +171,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--sweep-steps", type=int, default=500, help="batched sweep steps (0 skips the sweep)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device-to-device copy of MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -180,6 +257,9 @@ def main():
             "rebuilds_in_timed_steps": rebuilds,
         },
     }
+    ctx.close()
+    if args.sweep_steps:
+        rec["batched"] = run_sweep(args.sweep_steps, plain_sims=16, plain_steps=100)
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(m, s, pos, args.cpu_seconds)
     print(json.dumps(rec), flush=True)

@@ -137,6 +137,17 @@ class CBMModels:
         return _Law(A.LAW_REP_ADH, [k, r, eps, cutoffFactor])
 
     @staticmethod
+    def custom(source, p=()):
+        """A user interaction body in HIP C++: `struct UserLaw` with a constructor
+        from the 8 law parameters and `bool operator()(double d, double& s)`
+        returning the pair magnitude (force (s/d)(x_i - x_j)); compiled with
+        hipRTC into the stage kernel (the reference evals its agentODE quote
+        block, src/AgentStructure/functionDE.jl:110)."""
+        law = _Law(A.LAW_CUSTOM, p)
+        law.source = source
+        return law
+
+    @staticmethod
     def aggregation(rRep, fRep, rAtr, fAtr, walls=True):
         """The Aggregation model of docs/src/Aggregation.md:62-123 (with its wall push)."""
         return _Law(A.LAW_AGGREGATION, [rRep, fRep, rAtr, fAtr, 1.0 if walls else 0.0])
@@ -331,6 +342,30 @@ class Context:
         self._chk(self.lib.cbm_last_timing(self.h, C.byref(tot), C.byref(force), C.byref(nl)))
         return tot.value, force.value, nl.value
 
+    def compile(self, source):
+        """Compile a user interaction body (struct UserLaw, HIP C++) with hipRTC."""
+        self._chk(self.lib.cbm_compile(self.h, source.encode()))
+
+    def set_params(self, p, sim=0):
+        v = np.ascontiguousarray(p, dtype=np.float64)
+        self._chk(self.lib.cbm_set_params(self.h, sim, v.ctypes.data, v.size))
+
+    def adjoint(self, n_steps, checkpoint_every, save_steps, target):
+        """Loss and d(loss)/d(k, r_eq, eps) per simulation (cbm_adjoint).
+
+        target: per simulation, an array [n_saves, dims, N] in logical slot order."""
+        ss = np.ascontiguousarray(save_steps, dtype=np.int64)
+        tg = np.ascontiguousarray(np.concatenate([np.asarray(t, dtype=np.float64).ravel() for t in target]))
+        L = A.LossDesc()
+        L.n_steps, L.checkpoint_every, L.n_saves = int(n_steps), int(checkpoint_every), ss.size
+        L.save_steps = ss.ctypes.data_as(C.POINTER(C.c_int64))
+        L.target = tg.ctypes.data_as(C.POINTER(C.c_double))
+        S = len(self.sims)
+        grad = np.zeros(3 * S)
+        loss = np.zeros(S)
+        self._chk(self.lib.cbm_adjoint(self.h, C.byref(L), grad.ctypes.data, loss.ctypes.data))
+        return loss, grad.reshape(S, 3)
+
     def profile(self, nsteps):
         """Per-kernel average device times (ms) over nsteps steps, see cbm_profile."""
         ms = np.zeros(10)
@@ -400,6 +435,8 @@ def loadToPlatform_(com, preallocateAgents=0):
     model = abm.model_desc(com.dt)
     sim = sim_desc(abm, com.N, com.N + int(preallocateAgents), com.simBox.reshape(-1).tolist(), com.law_p)
     com.ctx = Context(model, [sim])
+    if abm.agentODE.law == A.LAW_CUSTOM:
+        com.ctx.compile(abm.agentODE.source)
     for k, arr in com.params.items():
         com.ctx.upload(_FIELDS[k], arr)
     com.ctx.neighbors()

@@ -12,6 +12,8 @@
 #include <algorithm>
 #include "cbm_internal.h"
 #include "cbm_device.h"
+#include <hip/hiprtc.h>
+#include "cbm_rtc_src.inc"
 
 
 using namespace cbm;
@@ -112,6 +114,10 @@ struct cbm_ctx {
   double last_ms = 0.0;
   int64_t last_launches = 0;
 
+  // user interaction body compiled with hipRTC (law CBM_LAW_CUSTOM)
+  hipModule_t user_mod = nullptr;
+  hipFunction_t user_fn[2] = {nullptr, nullptr};   // [0] deterministic stages, [1] EM
+
   std::string err;
 };
 
@@ -162,7 +168,7 @@ static int validate(cbm_ctx* c) {
   if (m.dims < 1 || m.dims > 3) return fail(c, CBM_ERR_CONFIG, "dims must be 1, 2 or 3");
   if (m.integrator < CBM_EULER || m.integrator > CBM_EM) return fail(c, CBM_ERR_CONFIG, "unknown integrator");
   if (m.neighbors < CBM_FULL || m.neighbors > CBM_CLVD) return fail(c, CBM_ERR_CONFIG, "unknown neighbor algorithm");
-  if (m.law < CBM_LAW_NONE || m.law > CBM_LAW_AGGREGATION) return fail(c, CBM_ERR_CONFIG, "unknown or uncompiled interaction law");
+  if (m.law < CBM_LAW_NONE || m.law > CBM_LAW_CUSTOM) return fail(c, CBM_ERR_CONFIG, "unknown interaction law");
   if (!(m.dt > 0)) return fail(c, CBM_ERR_CONFIG, "dt must be positive");
   if (m.neighbors == CBM_CELL_LINKED && !(m.cell_edge > 0)) return fail(c, CBM_ERR_CONFIG, "CellLinked needs cellEdge > 0");
   bool verlet = m.neighbors == CBM_VERLET_DISPLACEMENT || m.neighbors == CBM_VERLET_TIME || m.neighbors == CBM_CLVD;
@@ -305,6 +311,7 @@ static void destroy_ctx(cbm_ctx* c) {
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->user_mod) (void)hipModuleUnload(c->user_mod);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -542,10 +549,22 @@ constexpr int PROF_NCAT = 10;
 static void prof_begin(cbm_ctx* c, int cat);
 static void prof_end(cbm_ctx* c);
 
+// One integrator-stage launch: the built-in law kernels, or the hipRTC
+// module of a user law.
+static hipError_t stage_launch(cbm_ctx* c, const StageArgs& a) {
+  if (c->m.law != CBM_LAW_CUSTOM)
+    return launch_stage(c->dims, c->nb_kind, c->m.law, c->nblocks, c->block, c->stream, a);
+  hipFunction_t f = c->user_fn[a.kind == ST_EM ? 1 : 0];
+  if (!f) return hipErrorInvalidDeviceFunction;
+  StageArgs copy = a;
+  void* params[] = {(void*)&copy};
+  return hipModuleLaunchKernel(f, (unsigned)c->nblocks, 1, 1, (unsigned)c->block, 1, 1, 0, c->stream, params, nullptr);
+}
+
 static int run_stage(cbm_ctx* c, StageArgs& a, int kind) {
   a.kind = kind;
   prof_begin(c, kind);
-  HIPCHK(c, launch_stage(c->dims, c->nb_kind, c->m.law, c->nblocks, c->block, c->stream, a));
+  HIPCHK(c, stage_launch(c, a));
   prof_end(c);
   c->last_launches++;
   return CBM_OK;
@@ -761,9 +780,49 @@ void cbm_destroy(cbm_ctx* c) { destroy_ctx(c); }
 
 const char* cbm_last_error(const cbm_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
+// compileABM! for a user interaction body: the body (HIP C++ defining
+// struct UserLaw, see include/cbm_hip.h) is compiled by hipRTC into the
+// same stage skeleton as the built-in laws, instantiated for this model's
+// dimension, neighbor walk and boundary.
 int cbm_compile(cbm_ctx* c, const char* body) {
-  (void)body;
-  return fail(c, CBM_ERR_COMPILE, "user interaction bodies are not available in this build");
+  if (!c || !body) return CBM_ERR_ARG;
+  if (c->m.law != CBM_LAW_CUSTOM) return fail(c, CBM_ERR_CONFIG, "cbm_compile needs a model with law CBM_LAW_CUSTOM");
+  std::string src = "#define CBM_HIPRTC 1\n#include \"cbm_stage.h\"\nnamespace cbm {\n";
+  src += body;
+  src += "\n}  // namespace cbm\n";
+  char inst[512];
+  for (int noise = 0; noise < 2; ++noise) {
+    std::snprintf(inst, sizeof(inst),
+                  "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_stage_%d(cbm::StageArgs a) {\n"
+                  "  cbm::stage_body<%d, %d, cbm::UserLaw, %d, %d>(a);\n}\n",
+                  noise, c->dims, c->nb_kind, c->m.periodic ? 1 : 0, noise);
+    src += inst;
+  }
+  const char* headers[] = {cbm_rtc_cbm_device_h, cbm_rtc_cbm_structs_h, cbm_rtc_cbm_stage_h};
+  const char* names[] = {"cbm_device.h", "cbm_structs.h", "cbm_stage.h"};
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "cbm_user_law.hip", 3, headers, names) != HIPRTC_SUCCESS)
+    return fail(c, CBM_ERR_COMPILE, "hiprtcCreateProgram failed");
+  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math"};
+  hiprtcResult r = hiprtcCompileProgram(prog, 5, opts);
+  size_t logn = 0;
+  hiprtcGetProgramLogSize(prog, &logn);
+  std::string log(logn + 1, '\0');
+  if (logn) hiprtcGetProgramLog(prog, &log[0]);
+  if (r != HIPRTC_SUCCESS) {
+    hiprtcDestroyProgram(&prog);
+    return fail(c, CBM_ERR_COMPILE, std::string("hipRTC: ") + log.c_str());
+  }
+  size_t code_n = 0;
+  hiprtcGetCodeSize(prog, &code_n);
+  std::vector<char> code(code_n);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  if (c->user_mod) { (void)hipModuleUnload(c->user_mod); c->user_mod = nullptr; }
+  HIPCHK(c, hipModuleLoadData(&c->user_mod, code.data()));
+  HIPCHK(c, hipModuleGetFunction(&c->user_fn[0], c->user_mod, "cbm_user_stage_0"));
+  HIPCHK(c, hipModuleGetFunction(&c->user_fn[1], c->user_mod, "cbm_user_stage_1"));
+  return CBM_OK;
 }
 
 static int field_col(cbm_ctx* c, int32_t field, int& k) {
@@ -866,6 +925,7 @@ int cbm_neighbors(cbm_ctx* c) {
 
 int cbm_step(cbm_ctx* c, int64_t nsteps) {
   if (!c || nsteps < 0) return CBM_ERR_ARG;
+  if (c->m.law == CBM_LAW_CUSTOM && !c->user_fn[0]) return fail(c, CBM_ERR_CONFIG, "law CBM_LAW_CUSTOM: call cbm_compile first");
   c->last_launches = 0;
   if (!c->structure_valid) { int rc = force_rebuild(c); if (rc) return rc; }
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -1011,8 +1071,213 @@ int cbm_count_neighbors(cbm_ctx* c, int32_t sim, double radius, int64_t* out) {
   return export_sets(c, sim, radius, nullptr, nullptr, 0, out);
 }
 
-int cbm_adjoint(cbm_ctx* c, const cbm_loss_desc*, double*, double*) {
-  return fail(c, CBM_ERR_CONFIG, "adjoint is not available in this build");
+int cbm_set_params(cbm_ctx* c, int32_t sim, const double* p, int32_t n) {
+  if (!c || !p || sim < 0 || sim >= c->S || n < 1 || n > 8) return CBM_ERR_ARG;
+  for (int q = 0; q < n; ++q) { c->sims[sim].p[q] = p[q]; c->h_sims[sim].p[q] = p[q]; }
+  HIPCHK(c, hipMemcpyAsync(c->d_sims + sim, &c->h_sims[sim], sizeof(SimDev), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CBM_OK;
+}
+
+// --------------------------------------------------------------- adjoint
+// Neighbor (re)build that leaves the rows in place (lambda is indexed by row
+// during the adjoint): keys, scan, scatter, then lists whose candidates are
+// read through the bucket-ordered row table.  flag NULL = unconditional.
+static int rebuild_in_place(cbm_ctx* c, double* const* X, const int32_t* flag) {
+  RebuildArgs a = rebuild_args(c, flag, nullptr, false);
+  for (int d = 0; d < c->dims; ++d) a.pos[d] = X[d];
+  a.dead = nullptr;
+  HIPCHK(c, launch_keys(c->nblocks, c->block, c->stream, a));
+  ScanArgs s;
+  std::memset(&s, 0, sizeof(s));
+  s.flag = flag; s.count = c->count; s.cursor = c->cursor; s.start = c->start; s.n = c->NB;
+  s.status = c->scan_status; s.ticket = c->scan_ticket; s.done = c->scan_ticket + 1;
+  s.epoch = c->scan_ticket + 2; s.nblocks = scan_blocks(c->NB);
+  HIPCHK(c, launch_scan(c->stream, s));
+  HIPCHK(c, launch_scatter(c->nblocks, c->block, c->stream, a));
+  VerletArgs va;
+  std::memset(&va, 0, sizeof(va));
+  va.rb = a;
+  for (int d = 0; d < c->dims; ++d) { va.apos[d] = X[d]; va.xref[d] = c->xref[d]; }
+  va.aslot = (const int32_t*)c->col[C_SLOT].cur();
+  va.list = c->list; va.list_n = c->list_n; va.ldl = c->C; va.cap = c->cap;
+  va.overflow = c->flags + 2;
+  va.rl = c->list_radius;
+  va.build = 1;
+  va.nopermute = 1;
+  HIPCHK(c, launch_verlet(c->dims, c->nblocks, c->block, c->stream, va));
+  return CBM_OK;
+}
+
+// One overdamped-Euler step X -> Y with the displacement check against the
+// list snapshot, then a conditional in-place rebuild at Y.
+static int adj_forward_step(cbm_ctx* c, double* const* X, double* const* Y, int32_t* flag) {
+  StageArgs a = stage_base(c);
+  for (int v = 0; v < c->dims; ++v) { a.S[v] = X[v]; a.U[v] = X[v]; a.out[v] = Y[v]; }
+  a.flag = flag;
+  a.disp_check = 1;
+  a.kind = ST_EULER;
+  HIPCHK(c, stage_launch(c, a));
+  int rc = rebuild_in_place(c, Y, flag);
+  if (rc) return rc;
+  HIPCHK(c, launch_set_flag(c->stream, flag, 0));
+  return CBM_OK;
+}
+
+int cbm_adjoint(cbm_ctx* c, const cbm_loss_desc* L, double* grad, double* loss) {
+  if (!c || !L || !grad || !loss) return CBM_ERR_ARG;
+  const cbm_model_desc& m = c->m;
+  if (m.integrator != CBM_EULER || m.law != CBM_LAW_REP_ADH || !c->verlet || c->vtime || c->rules ||
+      c->medium || c->lagged || m.drift != 0.0)
+    return fail(c, CBM_ERR_CONFIG, "the adjoint covers overdamped Euler with the REP_ADH law on Verlet lists (no rules, medium, lag or drift)");
+  const int64_t T = L->n_steps, K = L->checkpoint_every;
+  const int S = L->n_saves;
+  if (T < 1 || K < 1 || S < 1 || !L->save_steps || !L->target) return fail(c, CBM_ERR_ARG, "bad loss description");
+  std::vector<int> save_at((size_t)T + 1, -1);
+  for (int q = 0; q < S; ++q) {
+    const int64_t st = L->save_steps[q];
+    if (st < 1 || st > T || (q && st <= L->save_steps[q - 1])) return fail(c, CBM_ERR_ARG, "save steps must be ascending in (0, n_steps]");
+    save_at[(size_t)st] = q;
+  }
+  int rc;
+  if (!c->structure_valid && (rc = force_rebuild(c))) return rc;
+  const int D = c->dims;
+  // per-sim sizes and the logical-order offsets of the targets
+  std::vector<int64_t> n(c->S), off(c->S + 1, 0);
+  for (int s = 0; s < c->S; ++s) {
+    if ((rc = cbm_get_n(c, s, &n[s]))) return rc;
+    if (n[s] < 1) return fail(c, CBM_ERR_ARG, "every simulation needs agents");
+    off[s + 1] = off[s] + n[s];
+  }
+  const int64_t ntot = off[c->S];
+  std::vector<double> tgt((size_t)S * D * ntot), invn(c->S);
+  {
+    const double* src = L->target;
+    for (int s = 0; s < c->S; ++s) {
+      for (int q = 0; q < S; ++q)
+        for (int a = 0; a < D; ++a)
+          for (int64_t i = 0; i < n[s]; ++i) tgt[((size_t)q * D + a) * ntot + off[s] + i] = *src++;
+      invn[s] = 1.0 / ((double)S * (double)n[s]);
+    }
+  }
+  const int64_t nck = (T + K - 1) / K;
+  const size_t state = (size_t)D * c->C;
+  double *d_tgt = nullptr, *d_invn = nullptr, *ck = nullptr, *seg = nullptr, *lam = nullptr, *gpart = nullptr, *lpart = nullptr;
+  int64_t* d_off = nullptr;
+  auto release = [&]() {
+    for (void* p : {(void*)d_tgt, (void*)d_invn, (void*)ck, (void*)seg, (void*)lam, (void*)gpart, (void*)lpart, (void*)d_off})
+      if (p) (void)hipFree(p);
+  };
+  hipError_t e = hipSuccess;
+  auto A = [&](void** p, size_t bytes) { if (e == hipSuccess) e = hipMalloc(p, bytes ? bytes : 8); };
+  A((void**)&d_tgt, sizeof(double) * tgt.size());
+  A((void**)&d_invn, sizeof(double) * c->S);
+  A((void**)&d_off, sizeof(int64_t) * c->S);
+  A((void**)&ck, sizeof(double) * state * nck);
+  A((void**)&seg, sizeof(double) * state * K);
+  A((void**)&lam, sizeof(double) * state * 2);
+  A((void**)&gpart, sizeof(double) * 3 * c->nblocks);
+  A((void**)&lpart, sizeof(double) * c->nblocks);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_tgt, tgt.data(), sizeof(double) * tgt.size(), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_invn, invn.data(), sizeof(double) * c->S, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * c->S, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(gpart, 0, sizeof(double) * 3 * c->nblocks, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(lpart, 0, sizeof(double) * c->nblocks, c->stream);
+  if (e != hipSuccess) { release(); return fail(c, CBM_ERR_HIP, std::string("adjoint buffers: ") + hipGetErrorString(e)); }
+  auto ptrs = [&](double* base, double** out3) { for (int a = 0; a < 3; ++a) out3[a] = a < D ? base + (size_t)a * c->C : nullptr; };
+  int32_t* flag = c->flags + 5;
+  double* X0[3];
+  for (int a = 0; a < D; ++a) X0[a] = statecol(c, a, false);
+  double* P[2][3];
+  for (int a = 0; a < D; ++a) { P[0][a] = statecol(c, a, false); P[1][a] = statecol(c, a, true); }
+  int cur = 0;
+  LossArgs la;
+  std::memset(&la, 0, sizeof(la));
+  la.g = geom(c); la.slot = (const int32_t*)c->col[C_SLOT].cur(); la.target = d_tgt; la.tgt_off = d_off;
+  la.tgt_n = ntot; la.dims = D; la.lpart = lpart;
+  auto fwd_fail = [&](int code) { release(); return code; };
+  // ---- forward with checkpoints, rows held in place
+  HIPCHK(c, launch_set_flag(c->stream, flag, 0));
+  if ((rc = rebuild_in_place(c, P[0], nullptr))) return fwd_fail(rc);
+  for (int64_t t = 0; t < T; ++t) {
+    if (t % K == 0) {
+      double* dst[3];
+      ptrs(ck + state * (size_t)(t / K), dst);
+      for (int a = 0; a < D; ++a)
+        if ((e = hipMemcpyAsync(dst[a], P[cur][a], sizeof(double) * c->C, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
+          return fwd_fail(fail(c, CBM_ERR_HIP, "checkpoint copy"));
+    }
+    if ((rc = adj_forward_step(c, P[cur], P[1 - cur], flag))) return fwd_fail(rc);
+    cur ^= 1;
+    if (save_at[(size_t)t + 1] >= 0) {
+      for (int a = 0; a < D; ++a) la.X[a] = P[cur][a];
+      la.save = save_at[(size_t)t + 1];
+      if ((e = launch_loss(c->nblocks, c->block, c->stream, la)) != hipSuccess) return fwd_fail(fail(c, CBM_ERR_HIP, "loss"));
+    }
+  }
+  // ---- backward sweep, recomputing each checkpoint segment
+  AdjointArgs aa;
+  std::memset(&aa, 0, sizeof(aa));
+  aa.g = geom(c); aa.list = c->list; aa.list_n = c->list_n; aa.ldl = c->C;
+  aa.slot = (const int32_t*)c->col[C_SLOT].cur(); aa.target = d_tgt; aa.tgt_off = d_off; aa.tgt_n = ntot;
+  aa.periodic = m.periodic; aa.dt = m.dt; aa.inv_n = d_invn; aa.gpart = gpart;
+  double* LAM[2][3];
+  ptrs(lam, LAM[0]);
+  ptrs(lam + state, LAM[1]);
+  int lc = 0;
+  for (int a = 0; a < D; ++a) aa.X[a] = P[cur][a];
+  for (int a = 0; a < D; ++a) aa.lam_out[a] = LAM[lc][a];
+  aa.save = save_at[(size_t)T];
+  if ((e = launch_adjoint_seed(D, c->nblocks, c->block, c->stream, aa)) != hipSuccess) return fwd_fail(fail(c, CBM_ERR_HIP, "adjoint seed"));
+  for (int64_t sgm = nck - 1; sgm >= 0; --sgm) {
+    const int64_t t0 = sgm * K, t1 = std::min(t0 + K, T);
+    double* Xk[64][3];
+    const int64_t len = t1 - t0;
+    if (len > 64 && K > 64) return fwd_fail(fail(c, CBM_ERR_ARG, "checkpoint_every must be <= 64"));
+    for (int64_t k = 0; k < len; ++k) ptrs(seg + state * (size_t)k, Xk[k]);
+    double* ckp[3];
+    ptrs(ck + state * (size_t)sgm, ckp);
+    for (int a = 0; a < D; ++a)
+      if ((e = hipMemcpyAsync(Xk[0][a], ckp[a], sizeof(double) * c->C, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
+        return fwd_fail(fail(c, CBM_ERR_HIP, "checkpoint restore"));
+    if ((rc = rebuild_in_place(c, Xk[0], nullptr))) return fwd_fail(rc);
+    for (int64_t k = 0; k + 1 < len; ++k)
+      if ((rc = adj_forward_step(c, Xk[k], Xk[k + 1], flag))) return fwd_fail(rc);
+    if ((rc = rebuild_in_place(c, Xk[len - 1], nullptr))) return fwd_fail(rc);
+    for (int64_t t = t1 - 1; t >= t0; --t) {
+      double* const* X = Xk[t - t0];
+      if (t < t1 - 1) {
+        StageArgs ca = stage_base(c);
+        for (int a = 0; a < D; ++a) ca.S[a] = X[a];
+        ca.flag = flag;
+        if ((e = launch_disp_check(D, c->nblocks, c->block, c->stream, ca)) != hipSuccess) return fwd_fail(fail(c, CBM_ERR_HIP, "disp check"));
+        if ((rc = rebuild_in_place(c, X, flag))) return fwd_fail(rc);
+        HIPCHK(c, launch_set_flag(c->stream, flag, 0));
+      }
+      for (int a = 0; a < D; ++a) { aa.X[a] = X[a]; aa.lam[a] = LAM[lc][a]; aa.lam_out[a] = LAM[1 - lc][a]; }
+      aa.save = (t >= 1) ? save_at[(size_t)t] : -1;
+      if ((e = launch_adjoint_step(D, c->nblocks, c->block, c->stream, aa)) != hipSuccess) return fwd_fail(fail(c, CBM_ERR_HIP, "adjoint step"));
+      lc ^= 1;
+    }
+  }
+  // ---- per-simulation totals (blocks summed in order: deterministic)
+  std::vector<double> hg((size_t)3 * c->nblocks), hl((size_t)c->nblocks);
+  e = hipMemcpyAsync(hg.data(), gpart, sizeof(double) * hg.size(), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(hl.data(), lpart, sizeof(double) * hl.size(), hipMemcpyDeviceToHost, c->stream);
+  // the community is left at its initial state
+  for (int a = 0; a < D && e == hipSuccess; ++a) e = hipMemcpyAsync(X0[a], ck + (size_t)a * c->C, sizeof(double) * c->C, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) { release(); return fail(c, CBM_ERR_HIP, std::string("adjoint readback: ") + hipGetErrorString(e)); }
+  for (int s = 0; s < c->S; ++s) { grad[3 * s] = grad[3 * s + 1] = grad[3 * s + 2] = 0.0; loss[s] = 0.0; }
+  for (int b = 0; b < c->nblocks; ++b) {
+    const int s = c->h_block_sim[b];
+    for (int q = 0; q < 3; ++q) grad[3 * s + q] += hg[(size_t)3 * b + q];
+    loss[s] += hl[(size_t)b];
+  }
+  for (int s = 0; s < c->S; ++s) loss[s] *= invn[s];
+  release();
+  c->structure_valid = false;
+  return check_flags(c);
 }
 
 int cbm_set_block(cbm_ctx* c, int32_t block) {
@@ -1022,9 +1287,77 @@ int cbm_set_block(cbm_ctx* c, int32_t block) {
   return setup_blocks(c);
 }
 
-int cbm_autotune(cbm_ctx* c, const char*, int32_t* block) {
+// Key of the tuning cache: everything that changes the stage kernel's work
+// per row (the model family and the population scale).
+static std::string tune_key(cbm_ctx* c) {
+  int64_t n = 0;
+  for (int s = 0; s < c->S; ++s) n += c->sims[s].n;
+  int lg = 0;
+  while (((int64_t)1 << (lg + 1)) <= n) ++lg;
+  char buf[256];
+  std::snprintf(buf, sizeof(buf), "gfx950 d%d i%d nb%d law%d per%d lag%d rule%d sims%d n2^%d skin%.3g edge%.3g",
+                c->dims, c->m.integrator, c->m.neighbors, c->m.law, c->m.periodic, c->lagged ? 1 : 0,
+                c->m.rule, c->S, lg, c->m.skin, c->m.cell_edge);
+  return buf;
+}
+
+int cbm_autotune(cbm_ctx* c, const char* cache_path, int32_t* block) {
   if (!c) return CBM_ERR_ARG;
-  if (block) *block = c->block;
+  const std::string key = tune_key(c);
+  if (cache_path) {   // "<key>\t<block>" lines
+    FILE* f = std::fopen(cache_path, "r");
+    if (f) {
+      char line[512];
+      int found = 0;
+      while (std::fgets(line, sizeof(line), f)) {
+        char* tab = std::strchr(line, '\t');
+        if (!tab) continue;
+        *tab = 0;
+        if (key == line) found = std::atoi(tab + 1);
+      }
+      std::fclose(f);
+      if (found == 64 || found == 128 || found == 256) {
+        int rc = cbm_set_block(c, found);
+        if (block) *block = c->block;
+        return rc;
+      }
+    }
+  }
+  if (!c->structure_valid) { int rc = force_rebuild(c); if (rc) return rc; }
+  // Time the first integrator stage written into scratch (the alternate
+  // state buffers, or the stage buffer): the same gather and pair work as a
+  // step, and no state changes.
+  StageArgs a = stage_base(c);
+  a.write_fnew = 0;
+  for (int v = 0; v < c->nvar; ++v) {
+    a.S[v] = statecol(c, v, false);
+    a.U[v] = statecol(c, v, false);
+    a.out[v] = c->h[0][v] ? c->h[0][v] : statecol(c, v, true);
+  }
+  a.kind = ST_HEUN1;
+  const int cand[3] = {64, 128, 256};
+  float best_ms = 1e30f;
+  int best = c->block;
+  for (int bs : cand) {
+    int rc = cbm_set_block(c, bs);
+    if (rc) return rc;
+    a.g = geom(c);
+    for (int w = 0; w < 2; ++w) HIPCHK(c, stage_launch(c, a));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < 10; ++r) HIPCHK(c, stage_launch(c, a));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (ms < best_ms) { best_ms = ms; best = bs; }
+  }
+  int rc = cbm_set_block(c, best);
+  if (rc) return rc;
+  if (cache_path) {
+    FILE* f = std::fopen(cache_path, "a");
+    if (f) { std::fprintf(f, "%s\t%d\n", key.c_str(), best); std::fclose(f); }
+  }
+  if (block) *block = best;
   return CBM_OK;
 }
 

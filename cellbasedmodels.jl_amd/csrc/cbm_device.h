@@ -12,6 +12,11 @@
 #ifndef CBM_HIPRTC
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#else
+using __hip_internal::int32_t;
+using __hip_internal::int64_t;
+using __hip_internal::uint32_t;
+using __hip_internal::uint64_t;
 #endif
 
 namespace cbm {
@@ -78,8 +83,11 @@ enum { LAW_NONE = 0, LAW_LINEAR = 1, LAW_REP_ADH = 2, LAW_AGGREGATION = 3, LAW_C
 template <int LAW> struct PairLaw;
 
 // s = k (c - d) for d < c
+// Traits of a law functor: active (has a pair body), per_axis_division
+// (force s*(dx/d) per axis instead of (s/d)*dx), walls (the Aggregation
+// box-wall push).
 template <> struct PairLaw<LAW_LINEAR> {
-  static constexpr bool per_axis_division = false;
+  static constexpr bool active = true, per_axis_division = false, walls = false;
   double k, c;
   __device__ __forceinline__ explicit PairLaw(const double* p) : k(p[0]), c(p[1]) {}
   __device__ __forceinline__ bool operator()(double d, double& s) const {
@@ -92,7 +100,7 @@ template <> struct PairLaw<LAW_LINEAR> {
 //   s = k (c - d)                      d < c
 //   s = 0 - (a4 (d - c)) (dc - d)      c <= d < dc   (well of depth eps)
 template <> struct PairLaw<LAW_REP_ADH> {
-  static constexpr bool per_axis_division = false;
+  static constexpr bool active = true, per_axis_division = false, walls = false;
   double k, c, dc, a4;
   __device__ __forceinline__ explicit PairLaw(const double* p) : k(p[0]) {
     c = 2.0 * p[1];
@@ -111,7 +119,7 @@ template <> struct PairLaw<LAW_REP_ADH> {
 // docs/src/Aggregation.md:88-98:
 //   s = fRep (rRep - d)  d < rRep;  s = -fAtr (rAtr - d)  rRep <= d < rAtr
 template <> struct PairLaw<LAW_AGGREGATION> {
-  static constexpr bool per_axis_division = true;   // dirx = (x - x[i2])/d
+  static constexpr bool active = true, per_axis_division = true, walls = true;   // dirx = (x - x[i2])/d
   double rRep, fRep, rAtr, fAtr;
   __device__ __forceinline__ explicit PairLaw(const double* p) : rRep(p[0]), fRep(p[1]), rAtr(p[2]), fAtr(p[3]) {}
   __device__ __forceinline__ bool operator()(double d, double& s) const {
@@ -122,8 +130,43 @@ template <> struct PairLaw<LAW_AGGREGATION> {
   }
 };
 
+// Derivatives of phi(d) = s(d)/d of PairLaw<LAW_REP_ADH> for the adjoint:
+// dphi/dd and dphi/d(k, r_eq, eps), with dc = f r_eq moving with r_eq.
+//   d < c:        s = k (c - d)              ds/dd = -k, ds/dk = c - d, ds/dr = 2k
+//   c <= d < dc:  s = -a4 (d - c)(dc - d)    ds/dd = -a4 (dc + c - 2d)
+//                 ds/deps = -(4/w^2)(d - c)(dc - d)
+//                 ds/dr   = (2 a4/r)(d - c)(dc - d) + 2 a4 (dc - d) - f a4 (d - c)
+struct RepAdhGrad {
+  double k, r, eps, f, c, dc, a4, w2;
+  __device__ __forceinline__ explicit RepAdhGrad(const double* p) : k(p[0]), r(p[1]), eps(p[2]), f(p[3]) {
+    c = 2.0 * r;
+    dc = f * r;
+    const double w = dc - c;
+    w2 = w * w;
+    a4 = (4.0 * eps) / w2;
+  }
+  // Returns whether the pair interacts; fills phi, dphi/dd and dphi/dtheta.
+  __device__ __forceinline__ bool operator()(double d, double& phi, double& dphi, double* dth) const {
+    double s, sd, sk, sr, se;
+    if (d < c) {
+      s = k * (c - d); sd = -k; sk = c - d; sr = 2.0 * k; se = 0.0;
+    } else {
+      const double u = d - c, v = dc - d;
+      s = 0.0 - a4 * u * v;
+      sd = -a4 * (dc + c - 2.0 * d);
+      sk = 0.0;
+      se = -(4.0 / w2) * u * v;
+      sr = (2.0 * a4 / r) * u * v + 2.0 * a4 * v - f * a4 * u;
+    }
+    phi = s / d;
+    dphi = sd / d - s / (d * d);
+    dth[0] = sk / d; dth[1] = sr / d; dth[2] = se / d;
+    return d > 0.0 && d < dc;
+  }
+};
+
 template <> struct PairLaw<LAW_NONE> {
-  static constexpr bool per_axis_division = false;
+  static constexpr bool active = false, per_axis_division = false, walls = false;
   __device__ __forceinline__ explicit PairLaw(const double*) {}
   __device__ __forceinline__ bool operator()(double, double& s) const { s = 0.0; return false; }
 };

@@ -6,86 +6,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "../../include/cbm_hip.h"
+#include "cbm_structs.h"
 
 namespace cbm {
-
-constexpr int MAX_COLS = 16;      // permutable per-agent columns
-constexpr int MAX_STATE = 4;      // DE variables: up to 3 positions + vol
-
-// Per-simulation constants (device array, one entry per simulation).
-// Row space: simulation s owns rows [row_off, row_off + n_max).
-// Cell space: simulation s owns global buckets [cell_off, cell_off + ncells + 2):
-// its grid cells, then a bucket for removed agents, then a "gap" bucket
-// sized so that a global exclusive scan of bucket counts lands every bucket
-// start on its global row index.
-struct SimDev {
-  double lo[3], hi[3], L[3];
-  double p[8];
-  double gedge[3];
-  int32_t gn[3];
-  int32_t pad0;
-  int64_t row_off, n_max;
-  int64_t cell_off, ncells;
-  int64_t rank_off;     // offset of this simulation's cell -> Morton-rank table
-  int64_t pad1;
-};
-
-// Per-simulation mutable scalars (device array).
-struct SimState {
-  int64_t N;        // com.N (agents committed)
-  int64_t n_tot;    // rows populated this step (N + NAdd_ before compaction)
-  int64_t n_add;    // NAdd_
-  int64_t n_remove; // NRemove_
-  int64_t id_max;   // idMax_
-  int64_t pad[3];
-};
-
-// Launch geometry shared by row-parallel kernels: block b serves
-// simulation block_sim[b], rows (b - block_off[sim]) * blockDim + tid.
-struct RowGeom {
-  const int32_t* block_sim;
-  const int32_t* block_off;
-  const SimDev* sims;
-  SimState* st;
-};
-
-// Integrator stage kinds (one kernel launch each).  See src/integrators.jl.
-enum StageKind {
-  ST_EULER = 0,     // u' = u + du*dt                        (:496-498)
-  ST_EM = 1,        // u' = (u + f*dt) + (g*xi)*sqrt(dt)      (:724-727)
-  ST_HEUN1 = 2,     // h = u + du*dt                          (:551-553)
-  ST_HEUN2 = 3,     // u' = (h + (u + du*dt))/2               (:564-566)
-  ST_RK4_1 = 4,     // k = k1;        h = u + k1*dt/2         (:620-624)
-  ST_RK4_2 = 5,     // k = k + 2*k2;  h = u + k2*dt/2
-  ST_RK4_3 = 6,     // k = k + 2*k3;  h = u + k3*dt
-  ST_RK4_4 = 7      // u' = u + ((k + k4)*dt)/6                (:659-661)
-};
-
-enum NeighborKind { NB_FULL = 0, NB_CELL = 1, NB_LIST = 2 };
-
-struct StageArgs {
-  RowGeom g;
-  const double* S[MAX_STATE];   // stage state read by the force (neighbors + self)
-  const double* U[MAX_STATE];   // step-start state u
-  double* out[MAX_STATE];       // stage output (h, or u')
-  double* kacc[MAX_STATE];      // RK4 accumulator
-  const double* F[3];           // committed force accumulator (lagged mode)
-  double* Fnew[3];              // force accumulator written by this stage
-  const double* xref[3];        // positions at the last Verlet build
-  const double* growth;         // per-agent growth rate (vol variable)
-  const int64_t* id;
-  const int32_t* list;          // ELL Verlet list, entry q of row i at q*ldl + i
-  const int32_t* list_n;
-  int64_t ldl;
-  const int32_t* cell_start;    // bucket -> first global row
-  const int32_t* cell_row;      // row -> linear local cell (cell-linked)
-  const int32_t* crank;         // linear local cell -> bucket rank (Morton order)
-  int32_t* flag;                // rebuild flag written by the final stage
-  double dt, sqdt, drift, noise, thr;
-  uint64_t seed;
-  const int64_t* step_ptr;     // device step counter (Philox counter word)
-  int32_t kind, periodic, lagged, walls, nvar, disp_check, write_fnew, pad;
-};
 
 // Per-agent columns moved by the cell sort permutation.
 struct PermCols {
@@ -148,7 +71,7 @@ struct VerletArgs {
   int32_t* overflow;
   double rl;                  // list radius
   int32_t build;              // 0: copy-back only (no list)
-  int32_t pad;
+  int32_t nopermute;          // 1: rows stay in place; candidates via tmp, cells via lin
 };
 
 struct XferArgs {
@@ -225,6 +148,45 @@ struct MediumArgs {
   double D, uptake, dt;
   int64_t nodes;
 };
+
+// Adjoint of the overdamped-Euler mechanics step (no reference counterpart).
+// Rows are not permuted while it runs, so lambda keeps its row mapping.
+struct AdjointArgs {
+  RowGeom g;
+  const double* X[3];           // positions x_t (the state the step read)
+  const double* lam[3];         // lambda_{t+1}
+  double* lam_out[3];           // lambda_t
+  const int32_t* list;
+  const int32_t* list_n;
+  int64_t ldl;
+  const int32_t* slot;
+  const double* target;         // save m, axis a, logical order of all sims: [(m*D + a)*ntot + off[sim] + slot]
+  const int64_t* tgt_off;       // per-sim offset into the logical order
+  int64_t tgt_n;                // agents of all sims (ntot)
+  int32_t save;                 // save index whose direct term enters lambda_t, or -1
+  int32_t periodic;
+  double dt;
+  double scale;                 // 2 / (n_saves * N) per sim is applied with inv_n below
+  const double* inv_n;          // per sim 1 / (n_saves * N)
+  double* gpart;                // per block 3 running parameter-gradient sums
+};
+
+struct LossArgs {
+  RowGeom g;
+  const double* X[3];
+  const int32_t* slot;
+  const double* target;
+  const int64_t* tgt_off;
+  int64_t tgt_n;
+  int32_t save;
+  int32_t dims;
+  double* lpart;                // per block running loss sums
+};
+
+hipError_t launch_adjoint_step(int dims, int nblocks, int block, hipStream_t s, const AdjointArgs& a);
+hipError_t launch_adjoint_seed(int dims, int nblocks, int block, hipStream_t s, const AdjointArgs& a);
+hipError_t launch_loss(int nblocks, int block, hipStream_t s, const LossArgs& a);
+hipError_t launch_disp_check(int dims, int nblocks, int block, hipStream_t s, const StageArgs& a);
 
 hipError_t launch_medium_count(int nblocks, int block, hipStream_t s, const MediumArgs& a);
 hipError_t launch_medium_step(hipStream_t s, const MediumArgs& a);

@@ -16,38 +16,9 @@
 // cell, Verlet lists ascending by slot) equals the reference CPU loop order.
 #include "cbm_internal.h"
 #include "cbm_device.h"
-
-#ifndef CBM_LIST_CHUNK
-#define CBM_LIST_CHUNK 2
-#endif
+#include "cbm_stage.h"
 
 namespace cbm {
-
-// ------------------------------------------------------------ row geometry
-struct RowIdx { int sim; int64_t local; int64_t row; };
-
-// Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
-// (own L2) serves one contiguous range of rows, i.e. one compact region of
-// the Morton-ordered agents, and neighbor gathers stay in that L2.
-__device__ __forceinline__ int xcd_tile(int b, int nb) {
-  const int q = nb >> 3, r = nb & 7;
-  const int x = b & 7, idx = b >> 3;
-  return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
-}
-
-__device__ __forceinline__ RowIdx row_index(const RowGeom& g) {
-  RowIdx r;
-  const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-  r.sim = g.block_sim[b];
-  r.local = (int64_t)(b - g.block_off[r.sim]) * blockDim.x + threadIdx.x;
-  r.row = g.sims[r.sim].row_off + r.local;
-  return r;
-}
-
-__device__ __forceinline__ double min_image(double d, double L) {
-  if (d > 0.5 * L) d -= L; else if (d < -0.5 * L) d += L;
-  return d;
-}
 
 // Axis cell of a position: cellPos (src/neighbors.jl:636-638) on the padded
 // grid, or the periodic grid of n cells of edge L/n.
@@ -71,214 +42,9 @@ __device__ __forceinline__ int32_t axis_cell(const SimDev& sd, int a, double x, 
   return (int32_t)(floor_div(x - sd.lo[a], sd.gedge[a]) + 1);
 }
 
-// k-th of the 3^D neighbor cells (cellPosNeigh order, x fastest) as a local
-// cell index, or -1 outside a non-periodic grid.
-template <int D>
-__device__ __forceinline__ int32_t neigh_cell(const SimDev& sd, int32_t cx, int32_t cy, int32_t cz, int k, int periodic) {
-  int32_t q[3] = {cx, cy, cz};
-  const int off[3] = {k % 3 - 1, (k / 3) % 3 - 1, k / 9 - 1};
-#pragma unroll
-  for (int a = 0; a < D; ++a) {
-    int32_t v = q[a] + off[a];
-    if (periodic) { v = v < 0 ? v + sd.gn[a] : (v >= sd.gn[a] ? v - sd.gn[a] : v); }
-    else if (v < 0 || v >= sd.gn[a]) return -1;
-    q[a] = v;
-  }
-  return q[0] + sd.gn[0] * (q[1] + sd.gn[1] * q[2]);
-}
-
-__host__ __device__ constexpr int pow3(int d) { return d == 1 ? 3 : (d == 2 ? 9 : 27); }
-
-// Walk the candidate neighbors of row i in reference loop order.
-template <int D, int NB, class Fn>
-__device__ __forceinline__ void for_candidates(const StageArgs& a, const SimDev& sd, int64_t N, int64_t i, Fn&& fn) {
-  if (NB == NB_LIST) {
-    const int32_t cnt = a.list_n[i];
-    for (int32_t q = 0; q < cnt; ++q) fn((int64_t)a.list[(int64_t)q * a.ldl + i]);
-  } else if (NB == NB_FULL) {
-    const int64_t j0 = sd.row_off, j1 = sd.row_off + N;
-    for (int64_t j = j0; j < j1; ++j) if (j != i) fn(j);
-  } else {
-    const int32_t c = a.cell_row[i];
-    const int32_t cx = c % sd.gn[0];
-    const int32_t cy = (c / sd.gn[0]) % sd.gn[1];
-    const int32_t cz = c / (sd.gn[0] * sd.gn[1]);
-    const int32_t* __restrict__ rank = a.crank + sd.rank_off;
-    for (int k = 0; k < pow3(D); ++k) {
-      const int32_t nc = neigh_cell<D>(sd, cx, cy, cz, k, a.periodic);
-      if (nc < 0) continue;
-      const int64_t gc = sd.cell_off + rank[nc];
-      const int32_t j0 = a.cell_start[gc], j1 = a.cell_start[gc + 1];
-      for (int32_t j = j0; j < j1; ++j) if (j != i) fn((int64_t)j);
-    }
-  }
-}
-
-// ----------------------------------------------------------- stage kernel
-// Minimum image (periodic boxes): the same comparisons as the oracle, as selects.
-__device__ __forceinline__ double wrap_dx(double d, double L, double hL) {
-  return d > hL ? d - L : (d < -hL ? d + L : d);
-}
-
 template <int D, int NB, int LAW, int PER, int NOISE>
 __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
-  const RowIdx ri = row_index(a.g);
-  const SimDev& sd = a.g.sims[ri.sim];
-  const int64_t N = a.g.st[ri.sim].N;
-  if (ri.local >= N) return;
-  const int64_t i = ri.row;
-
-  double xi[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-  for (int c = 0; c < D; ++c) xi[c] = a.S[c][i];
-  double acc[3] = {0.0, 0.0, 0.0};
-
-  if (LAW != LAW_NONE) {
-    const PairLaw<LAW> law(sd.p);
-    double L[3] = {0.0, 0.0, 0.0}, hL[3] = {0.0, 0.0, 0.0};
-    if (PER) {
-#pragma unroll
-      for (int c = 0; c < D; ++c) { L[c] = sd.L[c]; hL[c] = 0.5 * sd.L[c]; }
-    }
-    // one neighbor at position xj (the agentODE interaction body)
-    auto body = [&](const double* xj, bool live) {
-      double dx[3];
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        dx[c] = xi[c] - xj[c];
-        if (PER) dx[c] = wrap_dx(dx[c], L[c], hL[c]);
-      }
-      double s2 = dx[0] * dx[0];
-      if (D > 1) s2 = s2 + dx[1] * dx[1];
-      if (D > 2) s2 = s2 + dx[2] * dx[2];
-      const double d = sqrt(s2);
-      double s;
-      const bool on = law(d, s) && live;
-      if (PairLaw<LAW>::per_axis_division) {
-#pragma unroll
-        for (int c = 0; c < D; ++c) { const double v = acc[c] + s * (dx[c] / d); acc[c] = on ? v : acc[c]; }
-      } else {
-        const double f = s / d;               // built-in laws: F = (s/d) (x_i - x_j)
-#pragma unroll
-        for (int c = 0; c < D; ++c) { const double v = acc[c] + f * dx[c]; acc[c] = on ? v : acc[c]; }
-      }
-    };
-    if (NB == NB_LIST) {
-      // Verlet list in chunks of LCH gathers issued together (memory-level
-      // parallelism per lane), branch-free: slots past the count re-read the
-      // last entry and are masked out of the accumulation.
-      constexpr int LCH = CBM_LIST_CHUNK;
-      const int32_t cnt = a.list_n[i];
-      const int32_t* __restrict__ lst = a.list + i;
-      const int64_t ldl = a.ldl;
-      for (int32_t q0 = 0; q0 < cnt; q0 += LCH) {
-        int32_t jj[LCH];
-#pragma unroll
-        for (int u = 0; u < LCH; ++u) {
-          const int32_t q = (q0 + u < cnt) ? q0 + u : cnt - 1;
-          jj[u] = lst[(int64_t)q * ldl];
-        }
-        double px[LCH][3];
-#pragma unroll
-        for (int u = 0; u < LCH; ++u)
-#pragma unroll
-          for (int c = 0; c < D; ++c) px[u][c] = a.S[c][jj[u]];
-#pragma unroll
-        for (int u = 0; u < LCH; ++u) body(px[u], q0 + u < cnt);
-      }
-    } else {
-      for_candidates<D, NB>(a, sd, N, i, [&](int64_t j) {
-        double xj[3];
-#pragma unroll
-        for (int c = 0; c < D; ++c) xj[c] = a.S[c][j];
-        body(xj, true);
-      });
-    }
-    if (LAW == LAW_AGGREGATION && a.walls) {
-      const double rRep = sd.p[0], fRep = sd.p[1];
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        const double x = xi[c];
-        if (x < sd.lo[c] + rRep / 2) acc[c] = acc[c] + fRep;
-        else if (x > sd.hi[c] - rRep / 2) acc[c] = acc[c] - fRep;
-      }
-    }
-  }
-
-  // derivative of each DE variable
-  double du[MAX_STATE];
-#pragma unroll
-  for (int c = 0; c < D; ++c) {
-    double v = a.lagged ? a.F[c][i] : acc[c];
-    if (a.drift != 0.0) v = v - a.drift * xi[c];
-    du[c] = v;
-    if (a.write_fnew) a.Fnew[c][i] = acc[c];
-  }
-  const bool grow = a.nvar > D;
-  if (grow) du[D] = a.growth[i] * a.S[D][i];
-
-  const double dt = a.dt;
-  const int nv = grow ? D + 1 : D;
-  double res[MAX_STATE];
-  if (NOISE) {                                // EM: (u + f dt) + (g xi) sqrt(dt)
-    double z[4] = {0.0, 0.0, 0.0, 0.0};
-    const int64_t id = a.id[i];
-    const int64_t step = *a.step_ptr;
-    normal2(draw(a.seed, step, id, ri.sim, 0x100u), z[0], z[1]);
-    if (D > 2) normal2(draw(a.seed, step, id, ri.sim, 0x101u), z[2], z[3]);
-#pragma unroll
-    for (int c = 0; c < D; ++c) res[c] = (a.U[c][i] + du[c] * dt) + a.noise * z[c] * a.sqdt;
-    if (grow) res[D] = a.U[D][i] + du[D] * dt;
-  } else {
-    switch (a.kind) {
-      case ST_EULER:
-        for (int c = 0; c < nv; ++c) res[c] = a.U[c][i] + du[c] * dt;
-        break;
-      case ST_HEUN1:
-        for (int c = 0; c < nv; ++c) res[c] = a.U[c][i] + du[c] * dt;
-        break;
-      case ST_HEUN2:
-        for (int c = 0; c < nv; ++c) res[c] = (a.S[c][i] + (a.U[c][i] + du[c] * dt)) / 2;
-        break;
-      case ST_RK4_1:
-        for (int c = 0; c < nv; ++c) { a.kacc[c][i] = du[c]; res[c] = a.U[c][i] + du[c] * dt / 2; }
-        break;
-      case ST_RK4_2:
-        for (int c = 0; c < nv; ++c) { a.kacc[c][i] = a.kacc[c][i] + 2 * du[c]; res[c] = a.U[c][i] + du[c] * dt / 2; }
-        break;
-      case ST_RK4_3:
-        for (int c = 0; c < nv; ++c) { a.kacc[c][i] = a.kacc[c][i] + 2 * du[c]; res[c] = a.U[c][i] + du[c] * dt; }
-        break;
-      default:  // ST_RK4_4
-        for (int c = 0; c < nv; ++c) res[c] = a.U[c][i] + (a.kacc[c][i] + du[c]) * dt / 6;
-        break;
-    }
-  }
-
-  const bool final_stage = NOISE || (a.kind == ST_EULER || a.kind == ST_HEUN2 || a.kind == ST_RK4_4);
-  if (final_stage && a.periodic) {
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-      double x = res[c];
-      if (x >= sd.lo[c] + sd.L[c]) x = x - sd.L[c];
-      else if (x < sd.lo[c]) x = x + sd.L[c];
-      res[c] = x;
-    }
-  }
-  for (int c = 0; c < nv; ++c) a.out[c][i] = res[c];
-
-  if (final_stage && a.disp_check) {
-    double dx[3];
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-      dx[c] = res[c] - a.xref[c][i];
-      if (a.periodic) dx[c] = min_image(dx[c], sd.L[c]);
-    }
-    double s2 = dx[0] * dx[0];
-    if (D > 1) s2 = s2 + dx[1] * dx[1];
-    if (D > 2) s2 = s2 + dx[2] * dx[2];
-    if (sqrt(s2) >= a.thr) *a.flag = 1;   // src/neighbors.jl:403 (>= skin/2)
-  }
+  stage_body<D, NB, PairLaw<LAW>, PER, NOISE>(a);
 }
 
 // --------------------------------------------------------- rebuild chain
@@ -473,7 +239,7 @@ __device__ __forceinline__ int32_t verlet_row(const VerletArgs& va, const SimDev
   double xi[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int c = 0; c < D; ++c) xi[c] = va.apos[c][i];
-  const int32_t c = a.cell_row[i];
+  const int32_t c = va.nopermute ? a.lin[i] : a.cell_row[i];
   const int32_t cx = c % sd.gn[0];
   const int32_t cy = (c / sd.gn[0]) % sd.gn[1];
   const int32_t cz = c / (sd.gn[0] * sd.gn[1]);
@@ -484,7 +250,8 @@ __device__ __forceinline__ int32_t verlet_row(const VerletArgs& va, const SimDev
     if (nc < 0) continue;
     const int64_t gc = sd.cell_off + rank[nc];
     const int32_t j0 = a.start[gc], j1 = a.start[gc + 1];
-    for (int32_t j = j0; j < j1; ++j) {
+    for (int32_t jq = j0; jq < j1; ++jq) {
+      const int32_t j = va.nopermute ? a.tmp[jq] : jq;
       if (j == i) continue;
       double dx[3];
 #pragma unroll
@@ -515,7 +282,7 @@ __device__ __forceinline__ int32_t verlet_row(const VerletArgs& va, const SimDev
   }
   const int32_t kept = cnt < va.cap ? cnt : va.cap;
   va.list_n[i] = kept;
-  for (int k = 0; k < va.cols.n; ++k) {
+  for (int k = 0; k < (va.nopermute ? 0 : va.cols.n); ++k) {
     if (va.cols.bytes[k] == 8)
       ((uint64_t*)va.cols.cur[k])[i] = ((const uint64_t*)va.cols.alt[k])[i];
     else
@@ -690,6 +457,140 @@ __global__ __launch_bounds__(256) void k_compact_slots(RuleArgs a) {
       }
     }
   }
+}
+
+// ----------------------------------------------------------------- adjoint
+// Block-wide sum of NV doubles into out[tile*NV + q], tile = the block's
+// row-geometry index (running sums: each tile always owns the same entries,
+// so the totals are deterministic).
+template <int NV>
+__device__ __forceinline__ void block_accumulate(double* v, double* out) {
+  const int64_t tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  __shared__ double red[NV][256 / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double x = v[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+    if (lane == 0) red[q][wave] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double s = 0.0;
+      for (int w = 0; w < nw; ++w) s += red[q][w];
+      out[tile * NV + q] += s;
+    }
+  }
+}
+
+// lambda_t = lambda_{t+1} + dt sum_j J_ij (lambda_i - lambda_j) + dL/dx_t,
+// J = phi I + (phi'/d) r r^T for r = x_i - x_j: the transpose of the Euler
+// step's Jacobian gathered over the (symmetric) Verlet list.  Parameter
+// gradient: dt lambda_{t+1,i} . sum_j dphi/dtheta r.
+template <int D>
+__global__ __launch_bounds__(256) void k_adjoint_step(AdjointArgs a) {
+  const RowIdx ri = row_index(a.g);
+  const SimDev& sd = a.g.sims[ri.sim];
+  const int64_t N = a.g.st[ri.sim].N;
+  const bool live = ri.local < N;
+  double gth[3] = {0.0, 0.0, 0.0};
+  if (live) {
+    const int64_t i = ri.row;
+    const RepAdhGrad law(sd.p);
+    double xi[3] = {0.0, 0.0, 0.0}, li[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < D; ++c) { xi[c] = a.X[c][i]; li[c] = a.lam[c][i]; }
+    const int32_t cnt = a.list_n[i];
+    for (int32_t q = 0; q < cnt; ++q) {
+      const int64_t j = a.list[(int64_t)q * a.ldl + i];
+      double r[3], u[3];
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        r[c] = xi[c] - a.X[c][j];
+        if (a.periodic) { const double L = sd.L[c]; r[c] = r[c] > 0.5 * L ? r[c] - L : (r[c] < -0.5 * L ? r[c] + L : r[c]); }
+        u[c] = li[c] - a.lam[c][j];
+      }
+      double s2 = r[0] * r[0];
+      if (D > 1) s2 += r[1] * r[1];
+      if (D > 2) s2 += r[2] * r[2];
+      const double d = sqrt(s2);
+      double phi, dphi, dth[3];
+      if (!law(d, phi, dphi, dth)) continue;
+      double ru = 0.0, lr = 0.0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) { ru += r[c] * u[c]; lr += li[c] * r[c]; }
+      const double g = dphi / d * ru;
+#pragma unroll
+      for (int c = 0; c < D; ++c) v[c] += phi * u[c] + g * r[c];
+#pragma unroll
+      for (int q2 = 0; q2 < 3; ++q2) gth[q2] += dth[q2] * lr;
+    }
+    double direct[3] = {0.0, 0.0, 0.0};
+    if (a.save >= 0) {
+      const int64_t t = a.tgt_off[ri.sim] + a.slot[i];
+      const double w = 2.0 * a.inv_n[ri.sim];
+#pragma unroll
+      for (int c = 0; c < D; ++c)
+        direct[c] = w * (xi[c] - a.target[((int64_t)a.save * D + c) * a.tgt_n + t]);
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) a.lam_out[c][i] = li[c] + a.dt * v[c] + direct[c];
+#pragma unroll
+    for (int q2 = 0; q2 < 3; ++q2) gth[q2] *= a.dt;
+  }
+  block_accumulate<3>(gth, a.gpart);
+}
+
+// lambda_T = dL/dx_T (the last save), or 0.
+template <int D>
+__global__ __launch_bounds__(256) void k_adjoint_seed(AdjointArgs a) {
+  const RowIdx ri = row_index(a.g);
+  if (ri.local >= a.g.st[ri.sim].N) return;
+  const int64_t i = ri.row;
+  for (int c = 0; c < D; ++c) {
+    double v = 0.0;
+    if (a.save >= 0) {
+      const int64_t t = a.tgt_off[ri.sim] + a.slot[i];
+      v = 2.0 * a.inv_n[ri.sim] * (a.X[c][i] - a.target[((int64_t)a.save * D + c) * a.tgt_n + t]);
+    }
+    a.lam_out[c][i] = v;
+  }
+}
+
+// Running per-block sums of |x - x*|^2 at a save point.
+__global__ __launch_bounds__(256) void k_loss(LossArgs a) {
+  const RowIdx ri = row_index(a.g);
+  double e = 0.0;
+  if (ri.local < a.g.st[ri.sim].N) {
+    const int64_t i = ri.row;
+    const int64_t t = a.tgt_off[ri.sim] + a.slot[i];
+    for (int c = 0; c < a.dims; ++c) {
+      const double d = a.X[c][i] - a.target[((int64_t)a.save * a.dims + c) * a.tgt_n + t];
+      e += d * d;
+    }
+  }
+  block_accumulate<1>(&e, a.lpart);
+}
+
+// Verlet validity check of positions X (in S) against the list snapshot.
+template <int D>
+__global__ __launch_bounds__(256) void k_disp_check(StageArgs a) {
+  const RowIdx ri = row_index(a.g);
+  if (ri.local >= a.g.st[ri.sim].N) return;
+  const SimDev& sd = a.g.sims[ri.sim];
+  const int64_t i = ri.row;
+  double s2 = 0.0;
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    double dx = a.S[c][i] - a.xref[c][i];
+    if (a.periodic) dx = min_image(dx, sd.L[c]);
+    s2 = (c == 0) ? dx * dx : s2 + dx * dx;
+  }
+  if (sqrt(s2) >= a.thr) *a.flag = 1;
 }
 
 // ------------------------------------------------------------------ medium
@@ -880,6 +781,35 @@ hipError_t launch_verlet(int dims, int nblocks, int block, hipStream_t s, const 
     case 1: hipLaunchKernelGGL(k_verlet<1>, dim3(nblocks), dim3(block), 0, s, a); break;
     case 2: hipLaunchKernelGGL(k_verlet<2>, dim3(nblocks), dim3(block), 0, s, a); break;
     default: hipLaunchKernelGGL(k_verlet<3>, dim3(nblocks), dim3(block), 0, s, a); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_step(int dims, int nblocks, int block, hipStream_t s, const AdjointArgs& a) {
+  switch (dims) {
+    case 1: hipLaunchKernelGGL(k_adjoint_step<1>, dim3(nblocks), dim3(block), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_adjoint_step<2>, dim3(nblocks), dim3(block), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_adjoint_step<3>, dim3(nblocks), dim3(block), 0, s, a); break;
+  }
+  return hipGetLastError();
+}
+hipError_t launch_adjoint_seed(int dims, int nblocks, int block, hipStream_t s, const AdjointArgs& a) {
+  switch (dims) {
+    case 1: hipLaunchKernelGGL(k_adjoint_seed<1>, dim3(nblocks), dim3(block), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_adjoint_seed<2>, dim3(nblocks), dim3(block), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_adjoint_seed<3>, dim3(nblocks), dim3(block), 0, s, a); break;
+  }
+  return hipGetLastError();
+}
+hipError_t launch_loss(int nblocks, int block, hipStream_t s, const LossArgs& a) {
+  hipLaunchKernelGGL(k_loss, dim3(nblocks), dim3(block), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_disp_check(int dims, int nblocks, int block, hipStream_t s, const StageArgs& a) {
+  switch (dims) {
+    case 1: hipLaunchKernelGGL(k_disp_check<1>, dim3(nblocks), dim3(block), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_disp_check<2>, dim3(nblocks), dim3(block), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_disp_check<3>, dim3(nblocks), dim3(block), 0, s, a); break;
   }
   return hipGetLastError();
 }

@@ -167,8 +167,17 @@ const char* cbm_last_error(const cbm_ctx* ctx);
 const char* cbm_version(void);
 
 /* compileABM! / Main.eval (src/AgentStructure/functionDE.jl:110): compile a
- * user interaction body (HIP C++, see cbm_skeleton.h for the contract) with
- * hipRTC and use it instead of the built-in law (model law CBM_LAW_CUSTOM). */
+ * user interaction body with hipRTC into the stage kernel skeleton and use it
+ * as the model's law (model law CBM_LAW_CUSTOM; required before stepping).
+ * The body is HIP C++ defining, inside namespace cbm:
+ *   struct UserLaw {
+ *     static constexpr bool active = true, per_axis_division = false, walls = false;
+ *     __device__ explicit UserLaw(const double* p);   // the simulation's p[8]
+ *     __device__ bool operator()(double d, double& s) const;  // s(d); true if
+ *   };                                                // the pair interacts
+ * The force on agent i is sum_j (s/d)(x_i - x_j) (s*((x_i - x_j)/d) per axis
+ * when per_axis_division).  Compile errors return CBM_ERR_COMPILE with the
+ * hipRTC log in cbm_last_error. */
 int cbm_compile(cbm_ctx* ctx, const char* pair_body_hip);
 
 /* ------------------------------------------------------- data transfer */
@@ -232,6 +241,11 @@ typedef struct cbm_loss_desc {
 } cbm_loss_desc;
 int cbm_adjoint(cbm_ctx* ctx, const cbm_loss_desc* loss_desc, double* grad,
                 double* loss);
+
+/* Model-scope parameters of simulation `sim` (the law parameters p[0..n-1]);
+ * replaces assigning com.k etc. between runs of a fitting loop
+ * (test/testFitting.jl:4-29). */
+int cbm_set_params(cbm_ctx* ctx, int32_t sim, const double* p, int32_t n);
 
 /* -------------------------------------------------------------- autotune */
 /* Pick block size (and, for Verlet models, list radius margin) by timing a
