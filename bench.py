@@ -160,6 +160,122 @@ def run_sweep(steps, plain_sims, plain_steps):
             "config": "C4 512 2-D periodic sims, N log-uniform [500, 50000], Euler dt=0.01, FP64"}
 
 
+def adjoint_workload(n=65536, sets=64, steps=200, saves=(50, 100, 150, 200), seed=5):
+    """C5: 3-D overdamped Euler at packing fraction 0.3, dt 0.01; the target is
+    the run from (k, r_eq, eps) = (1.3, 1.1, 0.08); the gradient is taken at
+    `sets` parameter sets around (1, 1, 0.05)."""
+    import cbm._abi as A
+    from cbm.api import Context
+    L = (n * 4 / 3 * np.pi / 0.3) ** (1 / 3)
+    m = A.ModelDesc()
+    m.dims, m.integrator, m.law, m.dt = 3, A.EULER, A.LAW_REP_ADH, 0.01
+    m.neighbors, m.skin, m.max_neighbors, m.rebuild_disp = A.VERLET_DISPLACEMENT, 3.0, 48, 0.15
+    rng = np.random.default_rng(seed)
+    pos = [rng.uniform(0.0, L, n) for _ in range(3)]
+
+    def simd(theta):
+        s = A.SimDesc()
+        s.n = s.n_max = n
+        for a in range(3):
+            s.box_hi[a] = L
+        for q, v in enumerate(list(theta) + [2.4]):
+            s.p[q] = v
+        return s
+
+    tctx = Context(m, [simd((1.3, 1.1, 0.08))])
+    for a in range(3):
+        tctx.upload(a, pos[a])
+    tctx.neighbors()
+    target, done = [], 0
+    for st in saves:
+        tctx.step(st - done)
+        done = st
+        target.append(np.stack([tctx.download(a, n) for a in range(3)]))
+    tctx.close()
+    thetas = [np.array([1.0, 1.0, 0.05]) * rng.uniform(0.9, 1.1, 3) for _ in range(sets)]
+    return m, [simd(t) for t in thetas], pos, np.stack(target), thetas
+
+
+def run_adjoint(steps=200, every=20):
+    from cbm.api import Context
+    saves = (50, 100, 150, 200)
+    m, sims, pos, target, thetas = adjoint_workload(steps=steps, saves=saves)
+
+    def fresh():
+        c = Context(m, sims)
+        for q in range(len(sims)):
+            for a in range(3):
+                c.upload(a, pos[a], sim=q)
+        c.neighbors()
+        c.sync()
+        return c
+
+    cf = fresh()
+    t0 = time.perf_counter()
+    cf.step(steps)
+    cf.sync()
+    t_fwd = time.perf_counter() - t0
+    cf.close()
+    ca = fresh()
+    t0 = time.perf_counter()
+    loss, grad = ca.adjoint(steps, every, saves, [target] * len(sims))
+    t_adj = time.perf_counter() - t0
+    ca.close()
+    agents = sum(s.n for s in sims)
+    return {"adjoint_ms": t_adj * 1e3, "forward_ms": t_fwd * 1e3, "ratio": t_adj / t_fwd,
+            "agents": agents, "sets": len(sims), "steps": steps, "checkpoint_every": every,
+            "forward_agent_steps_per_s": agents * steps / t_fwd,
+            "loss0": float(loss[0]), "grad0": [float(v) for v in grad[0]],
+            "config": "C5 3-D overdamped Euler, N=65536 x 64 parameter sets, 200 steps, dt 0.01, FP64"}
+
+
+def run_colony(target_n=1_000_000, max_steps=6000):
+    """C3: 1000 agents (r = 1, k = 1) at the centre of a 2048 x 2048 medium
+    (spacing 1, zero flux, D = 1, uptake 0.1 c per agent, FTCS dt = 0.2, c0 = 1);
+    volumes grow at 0.01 +-10 % per unit time and divide at twice the birth
+    volume.  Runs until target_n agents."""
+    import cbm._abi as A
+    from cbm.api import Context
+    m = A.ModelDesc()
+    m.dims, m.integrator, m.law, m.dt = 2, A.EULER, A.LAW_LINEAR, 0.2
+    m.neighbors, m.cell_edge = A.CELL_LINKED, 2.0
+    m.rule, m.seed = A.RULE_GROWTH_DIVISION, 3
+    for q, v in enumerate((0.01, 0.1, 0.5, 2.0)):
+        m.rule_p[q] = v
+    m.medium_n[0] = m.medium_n[1] = 2048
+    m.medium_D, m.medium_consume, m.medium_c0 = 1.0, 0.1, 1.0
+    n0 = 1000
+    s = A.SimDesc()
+    s.n, s.n_max = n0, int(target_n * 2.1)
+    s.box_hi[0] = s.box_hi[1] = 2048.0
+    s.p[0], s.p[1] = 1.0, 2.0
+    rng = np.random.default_rng(3)
+    r = np.sqrt(n0 / 0.9) * np.sqrt(rng.uniform(0, 1, n0))
+    th = rng.uniform(0, 2 * np.pi, n0)
+    ctx = Context(m, [s])
+    ctx.upload(A.F_X, 1024.0 + r * np.cos(th))
+    ctx.upload(A.F_Y, 1024.0 + r * np.sin(th))
+    ctx.upload(A.F_VOL, rng.uniform(1.0, 2.0, n0))
+    ctx.upload(A.F_VBIRTH, np.ones(n0))
+    ctx.upload(A.F_GROWTH, rng.uniform(0.009, 0.011, n0))
+    ctx.neighbors()
+    ctx.sync()
+    steps, agent_steps, chunk = 0, 0, 100
+    t0 = time.perf_counter()
+    n = n0
+    while n < target_n and steps < max_steps:
+        ctx.step(chunk)
+        n_new = ctx.n()
+        agent_steps += chunk * (n + n_new) / 2      # N grows smoothly within a chunk
+        n = n_new
+        steps += chunk
+    wall = time.perf_counter() - t0
+    ctx.close()
+    return {"value": agent_steps / wall, "unit": "agent-steps/s (N_t summed over steps)", "steps": steps,
+            "final_n": n, "wall_s": wall, "medium_node_steps_per_s": 2048 * 2048 * steps / wall,
+            "config": "C3 2-D colony 1000 -> 1M agents with a 2048^2 FTCS medium, Euler dt 0.2, CL edge 2, FP64"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +288,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--sweep-steps", type=int, default=500, help="batched sweep steps (0 skips the sweep)")
+    ap.add_argument("--adjoint", type=int, default=1, help="1: also time the C5 adjoint against its forward run")
+    ap.add_argument("--colony", type=int, default=1, help="1: also run the C3 growing colony with its medium")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device-to-device copy of MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -260,6 +378,10 @@ def main():
     ctx.close()
     if args.sweep_steps:
         rec["batched"] = run_sweep(args.sweep_steps, plain_sims=16, plain_steps=100)
+    if args.adjoint:
+        rec["adjoint"] = run_adjoint()
+    if args.colony:
+        rec["colony"] = run_colony()
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(m, s, pos, args.cpu_seconds)
     print(json.dumps(rec), flush=True)
