@@ -80,7 +80,8 @@ struct cbm_ctx {
   int32_t* crank = nullptr;               // per-sim cell -> Morton rank tables
   int32_t* lin = nullptr;
   int32_t *key = nullptr, *count = nullptr, *cursor = nullptr, *start = nullptr, *tmp = nullptr, *cell_row = nullptr;
-  unsigned long long* scan_status = nullptr;
+  unsigned long long* scan_status = nullptr;   // one status word per scan tile
+  int64_t scan_tiles = 0;                      // capacity of scan_status in tiles
   unsigned int* scan_ticket = nullptr;    // [0] ticket, [1] done, [2] epoch
   int32_t* list = nullptr;
   int32_t* list_n = nullptr;
@@ -414,7 +415,13 @@ static int init_ctx(cbm_ctx* c) {
   HIPCHK(c, dalloc(&c->count, c->NB));
   HIPCHK(c, dalloc(&c->cursor, c->NB));
   HIPCHK(c, dalloc(&c->start, c->NB + 1));
-  HIPCHK(c, dalloc(&c->scan_status, scan_blocks(c->NB) + 1));
+  {
+    // the largest scan: the bucket table, or the rule decisions by slot
+    int64_t longest = c->NB;
+    for (int s = 0; s < c->S; ++s) longest = std::max<int64_t>(longest, c->sims[s].n_max + 1);
+    c->scan_tiles = scan_blocks(longest);
+    HIPCHK(c, dalloc(&c->scan_status, c->scan_tiles + 1));
+  }
   HIPCHK(c, dalloc(&c->scan_ticket, 4));
   {
     unsigned int ep = 1;
@@ -472,15 +479,25 @@ static RebuildArgs rebuild_args(cbm_ctx* c, const int32_t* flag, int32_t* flag_c
 // unconditionally; otherwise every kernel exits early unless *flag != 0.
 // copy_back: Verlet mode (build lists, copy the sorted rows back into the
 // current buffers); otherwise the alternate buffers become current.
+// Exclusive scan of n int32 counts (see k_scan); refuses a scan with more
+// tiles than the status array holds.
+static int run_scan(cbm_ctx* c, const int32_t* flag, int32_t* count, int32_t* cursor, int32_t* start, int64_t n) {
+  if (scan_blocks(n) > c->scan_tiles)
+    return fail(c, CBM_ERR_ARG, "scan longer than its status array");
+  ScanArgs s;
+  std::memset(&s, 0, sizeof(s));
+  s.flag = flag; s.count = count; s.cursor = cursor; s.start = start; s.n = n;
+  s.status = c->scan_status; s.ticket = c->scan_ticket; s.done = c->scan_ticket + 1;
+  s.epoch = c->scan_ticket + 2; s.nblocks = scan_blocks(n);
+  HIPCHK(c, launch_scan(c->stream, s));
+  return CBM_OK;
+}
+
 static int rebuild_chain(cbm_ctx* c, const int32_t* flag, int32_t* flag_clear, bool tick, bool copy_back) {
   RebuildArgs a = rebuild_args(c, flag, flag_clear, tick);
   HIPCHK(c, launch_keys(c->nblocks, c->block, c->stream, a));
-  ScanArgs s;
-  std::memset(&s, 0, sizeof(s));
-  s.flag = flag; s.count = c->count; s.cursor = c->cursor; s.start = c->start; s.n = c->NB;
-  s.status = c->scan_status; s.ticket = c->scan_ticket; s.done = c->scan_ticket + 1;
-  s.epoch = c->scan_ticket + 2; s.nblocks = scan_blocks(c->NB);
-  HIPCHK(c, launch_scan(c->stream, s));
+  int rc = run_scan(c, flag, c->count, c->cursor, c->start, c->NB);
+  if (rc) return rc;
   HIPCHK(c, launch_scatter(c->nblocks, c->block, c->stream, a));
   PermuteArgs pa;
   pa.rb = a;
@@ -609,13 +626,7 @@ static RuleArgs rule_args(cbm_ctx* c, int32_t* flag) {
 }
 
 static int slot_scan(cbm_ctx* c, int32_t* in, int32_t* out, int64_t n) {
-  ScanArgs s;
-  std::memset(&s, 0, sizeof(s));
-  s.flag = nullptr; s.count = in; s.cursor = nullptr; s.start = out; s.n = n;
-  s.status = c->scan_status; s.ticket = c->scan_ticket; s.done = c->scan_ticket + 1;
-  s.epoch = c->scan_ticket + 2; s.nblocks = scan_blocks(n);
-  HIPCHK(c, launch_scan(c->stream, s));
-  return CBM_OK;
+  return run_scan(c, nullptr, in, nullptr, out, n);
 }
 
 // agentStepRule!: decisions by slot, their prefixes, then the effects.
@@ -1088,12 +1099,8 @@ static int rebuild_in_place(cbm_ctx* c, double* const* X, const int32_t* flag) {
   for (int d = 0; d < c->dims; ++d) a.pos[d] = X[d];
   a.dead = nullptr;
   HIPCHK(c, launch_keys(c->nblocks, c->block, c->stream, a));
-  ScanArgs s;
-  std::memset(&s, 0, sizeof(s));
-  s.flag = flag; s.count = c->count; s.cursor = c->cursor; s.start = c->start; s.n = c->NB;
-  s.status = c->scan_status; s.ticket = c->scan_ticket; s.done = c->scan_ticket + 1;
-  s.epoch = c->scan_ticket + 2; s.nblocks = scan_blocks(c->NB);
-  HIPCHK(c, launch_scan(c->stream, s));
+  int rc = run_scan(c, flag, c->count, c->cursor, c->start, c->NB);
+  if (rc) return rc;
   HIPCHK(c, launch_scatter(c->nblocks, c->block, c->stream, a));
   VerletArgs va;
   std::memset(&va, 0, sizeof(va));

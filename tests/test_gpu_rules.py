@@ -107,3 +107,22 @@ def test_growth_division_matches_oracle(gpu_available, dims, nb):
         assert np.array_equal(g.download(A.F_VOL, n), o.get(A.F_VOL))
         assert np.array_equal(g.download(A.F_GROWTH, n), o.get(A.F_GROWTH))
     assert o.n > 3 * n0
+
+
+def test_rule_scan_longer_than_bucket_table(gpu_available):
+    """Capacity far above the cell count: the per-slot rule scans need more
+    scan tiles than the bucket scan (regression: the scan status array was
+    sized for the bucket scan only)."""
+    n0, cap = 100, 200_000
+    box = [[0.0, 40.0], [0.0, 40.0]]
+    rng = np.random.default_rng(2)
+    pos = [rng.uniform(15, 25, n0), rng.uniform(15, 25, n0)]
+    m = model(2, integrator=A.EULER, neighbors=A.CELL_LINKED, law=A.LAW_LINEAR, dt=0.05, cell_edge=2.0,
+              rule=A.RULE_GROWTH_DIVISION, rule_p=(0.5, 0.1, 0.5, 2.0), seed=6)
+    fields = {A.F_X: pos[0], A.F_Y: pos[1], A.F_VOL: np.ones(n0), A.F_VBIRTH: np.ones(n0),
+              A.F_GROWTH: rng.uniform(0.45, 0.55, n0)}
+    o, g = pair(m, sim(n0, box, [1.0, 2.0], n_max=cap), fields)
+    o.step(40)
+    g.step(40)
+    same_slots(o, g, 2, 1e-9)
+    assert o.n > n0
