@@ -229,11 +229,12 @@ def run_adjoint(steps=200, every=20):
             "config": "C5 3-D overdamped Euler, N=65536 x 64 parameter sets, 200 steps, dt 0.01, FP64"}
 
 
-def run_colony(target_n=1_000_000, max_steps=6000):
+def run_colony(target_n=500_000, max_steps=6000):
     """C3: 1000 agents (r = 1, k = 1) at the centre of a 2048 x 2048 medium
     (spacing 1, zero flux, D = 1, uptake 0.1 c per agent, FTCS dt = 0.2, c0 = 1);
     volumes grow at 0.01 +-10 % per unit time and divide at twice the birth
-    volume.  Runs until target_n agents."""
+    volume.  Runs until target_n agents (500k keeps the colony, radius ~750,
+    inside the 2048 box)."""
     import cbm._abi as A
     from cbm.api import Context
     m = A.ModelDesc()
@@ -273,7 +274,7 @@ def run_colony(target_n=1_000_000, max_steps=6000):
     ctx.close()
     return {"value": agent_steps / wall, "unit": "agent-steps/s (N_t summed over steps)", "steps": steps,
             "final_n": n, "wall_s": wall, "medium_node_steps_per_s": 2048 * 2048 * steps / wall,
-            "config": "C3 2-D colony 1000 -> 1M agents with a 2048^2 FTCS medium, Euler dt 0.2, CL edge 2, FP64"}
+            "config": f"C3 2-D colony 1000 -> {target_n} agents with a 2048^2 FTCS medium, Euler dt 0.2, CL edge 2, FP64"}
 
 
 def main():
@@ -308,6 +309,7 @@ def main():
         del src, dst
 
     from cbm.api import Context
+    print(f"bench: C2 {args.workload} N={args.n}", file=sys.stderr, flush=True)
     m, s, pos, L = workload(args.workload, args.n)
     ctx = Context(m, [s])
     for a in range(3):
@@ -376,13 +378,20 @@ def main():
         },
     }
     ctx.close()
+    def phase(name):
+        print(f"bench: {name}", file=sys.stderr, flush=True)
+
     if args.sweep_steps:
+        phase("batched sweep (512 sims)")
         rec["batched"] = run_sweep(args.sweep_steps, plain_sims=16, plain_steps=100)
     if args.adjoint:
+        phase("adjoint (64 x 65536 agents, 200 steps)")
         rec["adjoint"] = run_adjoint()
     if args.colony:
+        phase("growing colony with medium")
         rec["colony"] = run_colony()
     if not args.no_cpu:
+        phase("cpu baseline")
         rec["cpu_baseline"] = cpu_baseline(m, s, pos, args.cpu_seconds)
     print(json.dumps(rec), flush=True)
 
