@@ -1247,20 +1247,25 @@ int cbm_adjoint(cbm_ctx* c, const cbm_loss_desc* L, double* grad, double* loss) 
     for (int a = 0; a < D; ++a)
       if ((e = hipMemcpyAsync(Xk[0][a], ckp[a], sizeof(double) * c->C, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
         return fwd_fail(fail(c, CBM_ERR_HIP, "checkpoint restore"));
-    if ((rc = rebuild_in_place(c, Xk[0], nullptr))) return fwd_fail(rc);
+    // The list only has to be valid for the positions it is used with (every
+    // agent within the rebuild displacement of its list snapshot), in either
+    // time direction: check, and rebuild only when that fails.
+    auto ensure_valid = [&](double* const* X) -> int {
+      StageArgs ca = stage_base(c);
+      for (int a = 0; a < D; ++a) ca.S[a] = X[a];
+      ca.flag = flag;
+      if (launch_disp_check(D, c->nblocks, c->block, c->stream, ca) != hipSuccess) return fail(c, CBM_ERR_HIP, "disp check");
+      int r2 = rebuild_in_place(c, X, flag);
+      if (r2) return r2;
+      HIPCHK(c, launch_set_flag(c->stream, flag, 0));
+      return CBM_OK;
+    };
+    if ((rc = ensure_valid(Xk[0]))) return fwd_fail(rc);
     for (int64_t k = 0; k + 1 < len; ++k)
       if ((rc = adj_forward_step(c, Xk[k], Xk[k + 1], flag))) return fwd_fail(rc);
-    if ((rc = rebuild_in_place(c, Xk[len - 1], nullptr))) return fwd_fail(rc);
     for (int64_t t = t1 - 1; t >= t0; --t) {
       double* const* X = Xk[t - t0];
-      if (t < t1 - 1) {
-        StageArgs ca = stage_base(c);
-        for (int a = 0; a < D; ++a) ca.S[a] = X[a];
-        ca.flag = flag;
-        if ((e = launch_disp_check(D, c->nblocks, c->block, c->stream, ca)) != hipSuccess) return fwd_fail(fail(c, CBM_ERR_HIP, "disp check"));
-        if ((rc = rebuild_in_place(c, X, flag))) return fwd_fail(rc);
-        HIPCHK(c, launch_set_flag(c->stream, flag, 0));
-      }
+      if (t < t1 - 1 && (rc = ensure_valid(X))) return fwd_fail(rc);
       for (int a = 0; a < D; ++a) { aa.X[a] = X[a]; aa.lam[a] = LAM[lc][a]; aa.lam_out[a] = LAM[1 - lc][a]; }
       aa.save = (t >= 1) ? save_at[(size_t)t] : -1;
       if ((e = launch_adjoint_step(D, c->nblocks, c->block, c->stream, aa)) != hipSuccess) return fwd_fail(fail(c, CBM_ERR_HIP, "adjoint step"));
