@@ -24,6 +24,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "cellbasedmodels.jl_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the stage kernels, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs scaled by what the same counters read for a device copy of
+# known size; written next to the kernel-trace summaries.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` at N = n, or None when not measured."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = table.get(kernel)
+    if not e or e.get("N") != n:
+        return None
+    return e.get("bytes_per_launch")
 
 
 def workload(kind, n):
@@ -341,6 +358,7 @@ def main():
     dom = max(sb, key=lambda k: ms[k])
     achieved = sb[dom] * s.n / (ms[dom] * 1e-3) / 1e9
     step_bytes = sum(sb.values()) * s.n
+    dom_name = {2: "k_stage Heun stage 1", 3: "k_stage Heun stage 2", 1: "k_stage EM"}[dom]
     rec = {
         "metric": "agent-steps/s @1M 3D",
         "value": value,
@@ -358,12 +376,12 @@ def main():
                    "N": s.n, "box": L, "dt": m.dt, "graph_steps": args.graph},
         "roofline": {
             "bound": "hbm",
-            "kernel": {2: "k_stage Heun stage 1", 3: "k_stage Heun stage 2", 1: "k_stage EM"}[dom],
+            "kernel": dom_name,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom_name, s.n),
             "alg_bytes_per_launch": sb[dom] * s.n,
             "avg_launch_ms": ms[dom],
             "mean_list_len": kbar,

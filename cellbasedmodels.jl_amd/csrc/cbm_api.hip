@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -58,7 +59,7 @@ struct cbm_ctx {
   int64_t NB = 0;     // buckets
   int64_t NR = 0;     // rank-table entries (sum of cells)
   std::vector<int32_t> h_crank;
-  double list_radius = 0.0, thr = 0.0;
+  double list_radius = 0.0, list_r2 = 0.0, thr = 0.0;
 
   // per-agent columns
   Column col[C_NCOLS];
@@ -72,6 +73,7 @@ struct cbm_ctx {
   int32_t* off_dead = nullptr;
   int32_t* holes = nullptr;
   double* h[2][MAX_STATE] = {};
+
   double* kacc[MAX_STATE] = {};
   double* xref[3] = {nullptr, nullptr, nullptr};
   uint8_t* dead = nullptr;
@@ -199,6 +201,23 @@ static uint64_t spread_bits(uint64_t v, int dims) {
   uint64_t r = 0;
   for (int b = 0; b < 21; ++b) r |= ((v >> b) & 1ull) << (b * dims);
   return r;
+}
+
+// Largest s with sqrt(s) < r.  A correctly rounded sqrt is monotone, so the
+// list test sqrt(d2) < r of the reference (src/neighbors.jl:114-126) is
+// exactly d2 <= sq_threshold(r), with no square root per candidate.
+static double sq_threshold(double r) {
+  if (!(r > 0)) return -1.0;
+  double a = r * r;
+  while (a > 0 && !(std::sqrt(a) < r)) a = std::nextafter(a, 0.0);
+  while (std::sqrt(std::nextafter(a, INFINITY)) < r) a = std::nextafter(a, INFINITY);
+  return a;
+}
+
+// Sorted list entries a lane keeps in LDS during the list build.
+static int verlet_lds_r(const cbm_ctx* c) {
+  const int r = VERLET_LDS_BYTES / (8 * c->block);
+  return std::max(1, std::min(r, (int)c->cap));
 }
 
 static void morton_ranks(const int32_t* gn, int dims, std::vector<int32_t>& out) {
@@ -332,6 +351,7 @@ static int init_ctx(cbm_ctx* c) {
   c->vtime = m.neighbors == CBM_VERLET_TIME;
   c->nb_kind = c->full ? NB_FULL : (c->cl ? NB_CELL : NB_LIST);
   c->list_radius = m.skin;
+  c->list_r2 = sq_threshold(m.skin);
   c->thr = m.rebuild_disp > 0 ? m.rebuild_disp : m.skin / 2;   // src/neighbors.jl:403
   HIPCHK(c, hipGetDevice(&c->device));
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -403,6 +423,7 @@ static int init_ctx(cbm_ctx* c) {
     for (int v = 0; v < c->nvar; ++v) HIPCHK(c, dalloc(&c->h[b][v], c->C));
   if (m.integrator == CBM_RK4)
     for (int v = 0; v < c->nvar; ++v) HIPCHK(c, dalloc(&c->kacc[v], c->C));
+
   // neighbor engine
   HIPCHK(c, dalloc(&c->key, c->C));
   HIPCHK(c, dalloc(&c->lin, c->C));
@@ -512,7 +533,8 @@ static int rebuild_chain(cbm_ctx* c, const int32_t* flag, int32_t* flag_clear, b
     va.aslot = (const int32_t*)c->col[C_SLOT].alt();
     va.list = c->list; va.list_n = c->list_n; va.ldl = c->C; va.cap = c->cap;
     va.overflow = c->flags + 2;
-    va.rl = c->list_radius;
+    va.rl2 = c->list_r2;
+    va.lds_r = verlet_lds_r(c);
     va.build = c->verlet ? 1 : 0;
     HIPCHK(c, launch_verlet(c->dims, c->nblocks, c->block, c->stream, va));
   } else {
@@ -546,7 +568,7 @@ static StageArgs stage_base(cbm_ctx* c) {
   a.id = (const int64_t*)c->col[C_ID].cur();
   a.list = c->list; a.list_n = c->list_n; a.ldl = c->C;
   a.cell_start = c->start; a.cell_row = c->cell_row; a.crank = c->crank;
-  a.dt = c->m.dt; a.sqdt = std::sqrt(c->m.dt); a.drift = c->m.drift; a.noise = c->m.noise; a.thr = c->thr;
+  a.dt = c->m.dt; a.sqdt = std::sqrt(c->m.dt); a.drift = c->m.drift; a.noise = c->m.noise; a.thr2 = sq_threshold(c->thr);
   a.seed = c->m.seed;
   a.step_ptr = c->counters + 1;
   a.periodic = c->m.periodic;
@@ -1094,7 +1116,8 @@ int cbm_set_params(cbm_ctx* c, int32_t sim, const double* p, int32_t n) {
 // Neighbor (re)build that leaves the rows in place (lambda is indexed by row
 // during the adjoint): keys, scan, scatter, then lists whose candidates are
 // read through the bucket-ordered row table.  flag NULL = unconditional.
-static int rebuild_in_place(cbm_ctx* c, double* const* X, const int32_t* flag) {
+static int rebuild_in_place(cbm_ctx* c, double* const* X, const int32_t* flag,
+                            int32_t* list = nullptr, int32_t* list_n = nullptr) {
   RebuildArgs a = rebuild_args(c, flag, nullptr, false);
   for (int d = 0; d < c->dims; ++d) a.pos[d] = X[d];
   a.dead = nullptr;
@@ -1107,9 +1130,11 @@ static int rebuild_in_place(cbm_ctx* c, double* const* X, const int32_t* flag) {
   va.rb = a;
   for (int d = 0; d < c->dims; ++d) { va.apos[d] = X[d]; va.xref[d] = c->xref[d]; }
   va.aslot = (const int32_t*)c->col[C_SLOT].cur();
-  va.list = c->list; va.list_n = c->list_n; va.ldl = c->C; va.cap = c->cap;
+  va.list = list ? list : c->list; va.list_n = list_n ? list_n : c->list_n;
+  va.ldl = c->C; va.cap = c->cap;
   va.overflow = c->flags + 2;
-  va.rl = c->list_radius;
+  va.rl2 = c->list_r2;
+  va.lds_r = verlet_lds_r(c);
   va.build = 1;
   va.nopermute = 1;
   HIPCHK(c, launch_verlet(c->dims, c->nblocks, c->block, c->stream, va));
@@ -1130,6 +1155,23 @@ static int adj_forward_step(cbm_ctx* c, double* const* X, double* const* Y, int3
   HIPCHK(c, launch_set_flag(c->stream, flag, 0));
   return CBM_OK;
 }
+
+// One overdamped-Euler step X -> Y on the given lists, optionally with the
+// displacement check into flag.
+static int adj_stage(cbm_ctx* c, double* const* X, double* const* Y, const int32_t* list,
+                     const int32_t* list_n, int32_t* flag) {
+  StageArgs a = stage_base(c);
+  for (int v = 0; v < c->dims; ++v) { a.S[v] = X[v]; a.U[v] = X[v]; a.out[v] = Y[v]; }
+  a.list = list; a.list_n = list_n;
+  a.flag = flag;
+  a.disp_check = flag ? 1 : 0;
+  a.kind = ST_EULER;
+  HIPCHK(c, stage_launch(c, a));
+  return CBM_OK;
+}
+
+// Status of the snapshot forward pass when its memory budget runs out.
+constexpr int ADJ_NO_ROOM = 1;
 
 int cbm_adjoint(cbm_ctx* c, const cbm_loss_desc* L, double* grad, double* loss) {
   if (!c || !L || !grad || !loss) return CBM_ERR_ARG;
@@ -1203,75 +1245,155 @@ int cbm_adjoint(cbm_ctx* c, const cbm_loss_desc* L, double* grad, double* loss) 
   la.g = geom(c); la.slot = (const int32_t*)c->col[C_SLOT].cur(); la.target = d_tgt; la.tgt_off = d_off;
   la.tgt_n = ntot; la.dims = D; la.lpart = lpart;
   auto fwd_fail = [&](int code) { release(); return code; };
-  // ---- forward with checkpoints, rows held in place
-  HIPCHK(c, launch_set_flag(c->stream, flag, 0));
-  if ((rc = rebuild_in_place(c, P[0], nullptr))) return fwd_fail(rc);
-  for (int64_t t = 0; t < T; ++t) {
-    if (t % K == 0) {
-      double* dst[3];
-      ptrs(ck + state * (size_t)(t / K), dst);
-      for (int a = 0; a < D; ++a)
-        if ((e = hipMemcpyAsync(dst[a], P[cur][a], sizeof(double) * c->C, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
-          return fwd_fail(fail(c, CBM_ERR_HIP, "checkpoint copy"));
-    }
-    if ((rc = adj_forward_step(c, P[cur], P[1 - cur], flag))) return fwd_fail(rc);
-    cur ^= 1;
-    if (save_at[(size_t)t + 1] >= 0) {
-      for (int a = 0; a < D; ++a) la.X[a] = P[cur][a];
-      la.save = save_at[(size_t)t + 1];
-      if ((e = launch_loss(c->nblocks, c->block, c->stream, la)) != hipSuccess) return fwd_fail(fail(c, CBM_ERR_HIP, "loss"));
-    }
-  }
-  // ---- backward sweep, recomputing each checkpoint segment
+  // ---- list snapshots.  Every list the forward pass builds is kept (while
+  // half of the free device memory holds them), so the recompute and the
+  // backward sweep run on exactly the forward's lists: no displacement
+  // checks, no rebuilds.  Without room the lists are re-validated, and
+  // rebuilt where needed, along the backward sweep instead.
+  std::vector<int32_t*> snap_l, snap_n;
+  std::vector<int32_t> epoch((size_t)T, 0);
+  auto free_snaps = [&]() {
+    for (int32_t* p : snap_l) (void)hipFree(p);
+    for (int32_t* p : snap_n) (void)hipFree(p);
+    snap_l.clear(); snap_n.clear();
+  };
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+  size_t budget = free_b / 2;
+  if (const char* mb = std::getenv("CBM_ADJ_SNAPSHOT_MB")) budget = (size_t)std::strtoull(mb, nullptr, 10) << 20;
+  const size_t snap_bytes = sizeof(int32_t) * ((size_t)c->cap + 1) * (size_t)c->C;
+  auto new_snap = [&]() -> bool {
+    if ((snap_l.size() + 1) * snap_bytes > budget) return false;
+    int32_t *l = nullptr, *nn = nullptr;
+    if (hipMalloc((void**)&l, sizeof(int32_t) * (size_t)c->cap * (size_t)c->C) != hipSuccess) return false;
+    if (hipMalloc((void**)&nn, sizeof(int32_t) * (size_t)c->C) != hipSuccess) { (void)hipFree(l); return false; }
+    snap_l.push_back(l); snap_n.push_back(nn);
+    return true;
+  };
   AdjointArgs aa;
   std::memset(&aa, 0, sizeof(aa));
-  aa.g = geom(c); aa.list = c->list; aa.list_n = c->list_n; aa.ldl = c->C;
+  aa.g = geom(c); aa.ldl = c->C;
   aa.slot = (const int32_t*)c->col[C_SLOT].cur(); aa.target = d_tgt; aa.tgt_off = d_off; aa.tgt_n = ntot;
   aa.periodic = m.periodic; aa.dt = m.dt; aa.inv_n = d_invn; aa.gpart = gpart;
   double* LAM[2][3];
   ptrs(lam, LAM[0]);
   ptrs(lam + state, LAM[1]);
-  int lc = 0;
-  for (int a = 0; a < D; ++a) aa.X[a] = P[cur][a];
-  for (int a = 0; a < D; ++a) aa.lam_out[a] = LAM[lc][a];
-  aa.save = save_at[(size_t)T];
-  if ((e = launch_adjoint_seed(D, c->nblocks, c->block, c->stream, aa)) != hipSuccess) return fwd_fail(fail(c, CBM_ERR_HIP, "adjoint seed"));
-  for (int64_t sgm = nck - 1; sgm >= 0; --sgm) {
-    const int64_t t0 = sgm * K, t1 = std::min(t0 + K, T);
-    double* Xk[64][3];
-    const int64_t len = t1 - t0;
-    if (len > 64 && K > 64) return fwd_fail(fail(c, CBM_ERR_ARG, "checkpoint_every must be <= 64"));
-    for (int64_t k = 0; k < len; ++k) ptrs(seg + state * (size_t)k, Xk[k]);
-    double* ckp[3];
-    ptrs(ck + state * (size_t)sgm, ckp);
-    for (int a = 0; a < D; ++a)
-      if ((e = hipMemcpyAsync(Xk[0][a], ckp[a], sizeof(double) * c->C, hipMemcpyDeviceToDevice, c->stream)) != hipSuccess)
-        return fwd_fail(fail(c, CBM_ERR_HIP, "checkpoint restore"));
-    // The list only has to be valid for the positions it is used with (every
-    // agent within the rebuild displacement of its list snapshot), in either
-    // time direction: check, and rebuild only when that fails.
-    auto ensure_valid = [&](double* const* X) -> int {
-      StageArgs ca = stage_base(c);
-      for (int a = 0; a < D; ++a) ca.S[a] = X[a];
-      ca.flag = flag;
-      if (launch_disp_check(D, c->nblocks, c->block, c->stream, ca) != hipSuccess) return fail(c, CBM_ERR_HIP, "disp check");
-      int r2 = rebuild_in_place(c, X, flag);
-      if (r2) return r2;
-      HIPCHK(c, launch_set_flag(c->stream, flag, 0));
-      return CBM_OK;
-    };
-    if ((rc = ensure_valid(Xk[0]))) return fwd_fail(rc);
-    for (int64_t k = 0; k + 1 < len; ++k)
-      if ((rc = adj_forward_step(c, Xk[k], Xk[k + 1], flag))) return fwd_fail(rc);
-    for (int64_t t = t1 - 1; t >= t0; --t) {
-      double* const* X = Xk[t - t0];
-      if (t < t1 - 1 && (rc = ensure_valid(X))) return fwd_fail(rc);
-      for (int a = 0; a < D; ++a) { aa.X[a] = X[a]; aa.lam[a] = LAM[lc][a]; aa.lam_out[a] = LAM[1 - lc][a]; }
-      aa.save = (t >= 1) ? save_at[(size_t)t] : -1;
-      if ((e = launch_adjoint_step(D, c->nblocks, c->block, c->stream, aa)) != hipSuccess) return fwd_fail(fail(c, CBM_ERR_HIP, "adjoint step"));
-      lc ^= 1;
+  bool moved = false;   // P[0] no longer holds the initial state (it is in ck[0])
+  auto run = [&](bool snaps) -> int {
+    int rc2;
+    cur = 0;
+    int e = 0;
+    HIPCHK(c, launch_set_flag(c->stream, flag, 0));
+    // ---- forward with checkpoints, rows held in place
+    if (snaps) {
+      if (!new_snap()) return ADJ_NO_ROOM;
+      if ((rc2 = rebuild_in_place(c, P[0], nullptr, snap_l[0], snap_n[0]))) return rc2;
+    } else if ((rc2 = rebuild_in_place(c, P[0], nullptr))) {
+      return rc2;
     }
+    for (int64_t t = 0; t < T; ++t) {
+      if (t % K == 0) {
+        double* dst[3];
+        ptrs(ck + state * (size_t)(t / K), dst);
+        for (int a = 0; a < D; ++a)
+          if (hipMemcpyAsync(dst[a], P[cur][a], sizeof(double) * c->C, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+            return fail(c, CBM_ERR_HIP, "checkpoint copy");
+      }
+      if (snaps) {
+        epoch[(size_t)t] = e;
+        if ((rc2 = adj_stage(c, P[cur], P[1 - cur], snap_l[(size_t)e], snap_n[(size_t)e], flag))) return rc2;
+      } else if ((rc2 = adj_forward_step(c, P[cur], P[1 - cur], flag))) {
+        return rc2;
+      }
+      cur ^= 1;
+      moved = true;
+      if (save_at[(size_t)t + 1] >= 0) {
+        for (int a = 0; a < D; ++a) la.X[a] = P[cur][a];
+        la.save = save_at[(size_t)t + 1];
+        if (launch_loss(c->nblocks, c->block, c->stream, la) != hipSuccess) return fail(c, CBM_ERR_HIP, "loss");
+      }
+      if (snaps && t + 1 < T) {
+        int32_t f = 0;
+        HIPCHK(c, hipMemcpyAsync(&f, flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (f) {
+          if (!new_snap()) return ADJ_NO_ROOM;
+          ++e;
+          HIPCHK(c, launch_set_flag(c->stream, flag, 0));
+          if ((rc2 = rebuild_in_place(c, P[cur], nullptr, snap_l[(size_t)e], snap_n[(size_t)e]))) return rc2;
+        }
+      }
+    }
+    // ---- backward sweep, recomputing each checkpoint segment
+    int lc = 0;
+    aa.list = c->list; aa.list_n = c->list_n;
+    for (int a = 0; a < D; ++a) aa.X[a] = P[cur][a];
+    for (int a = 0; a < D; ++a) aa.lam_out[a] = LAM[lc][a];
+    aa.save = save_at[(size_t)T];
+    if (launch_adjoint_seed(D, c->nblocks, c->block, c->stream, aa) != hipSuccess) return fail(c, CBM_ERR_HIP, "adjoint seed");
+    for (int64_t sgm = nck - 1; sgm >= 0; --sgm) {
+      const int64_t t0 = sgm * K, t1 = std::min(t0 + K, T);
+      double* Xk[64][3];
+      const int64_t len = t1 - t0;
+      for (int64_t k = 0; k < len; ++k) ptrs(seg + state * (size_t)k, Xk[k]);
+      double* ckp[3];
+      ptrs(ck + state * (size_t)sgm, ckp);
+      for (int a = 0; a < D; ++a)
+        if (hipMemcpyAsync(Xk[0][a], ckp[a], sizeof(double) * c->C, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+          return fail(c, CBM_ERR_HIP, "checkpoint restore");
+      // Without snapshots the list only has to be valid for the positions it
+      // is used with (every agent within the rebuild displacement of its
+      // list snapshot), in either time direction: check, and rebuild only
+      // when that fails.
+      auto ensure_valid = [&](double* const* X) -> int {
+        StageArgs ca = stage_base(c);
+        for (int a = 0; a < D; ++a) ca.S[a] = X[a];
+        ca.flag = flag;
+        if (launch_disp_check(D, c->nblocks, c->block, c->stream, ca) != hipSuccess) return fail(c, CBM_ERR_HIP, "disp check");
+        int r3 = rebuild_in_place(c, X, flag);
+        if (r3) return r3;
+        HIPCHK(c, launch_set_flag(c->stream, flag, 0));
+        return CBM_OK;
+      };
+      if (snaps) {
+        for (int64_t k = 0; k + 1 < len; ++k) {
+          const size_t ep = (size_t)epoch[(size_t)(t0 + k)];
+          if ((rc2 = adj_stage(c, Xk[k], Xk[k + 1], snap_l[ep], snap_n[ep], nullptr))) return rc2;
+        }
+      } else {
+        if ((rc2 = ensure_valid(Xk[0]))) return rc2;
+        for (int64_t k = 0; k + 1 < len; ++k)
+          if ((rc2 = adj_forward_step(c, Xk[k], Xk[k + 1], flag))) return rc2;
+      }
+      for (int64_t t = t1 - 1; t >= t0; --t) {
+        double* const* X = Xk[t - t0];
+        if (snaps) {
+          aa.list = snap_l[(size_t)epoch[(size_t)t]]; aa.list_n = snap_n[(size_t)epoch[(size_t)t]];
+        } else if (t < t1 - 1 && (rc2 = ensure_valid(X))) {
+          return rc2;
+        }
+        for (int a = 0; a < D; ++a) { aa.X[a] = X[a]; aa.lam[a] = LAM[lc][a]; aa.lam_out[a] = LAM[1 - lc][a]; }
+        aa.save = (t >= 1) ? save_at[(size_t)t] : -1;
+        if (launch_adjoint_step(D, c->nblocks, c->block, c->stream, aa) != hipSuccess) return fail(c, CBM_ERR_HIP, "adjoint step");
+        lc ^= 1;
+      }
+    }
+    return CBM_OK;
+  };
+  if (K > 64) return fwd_fail(fail(c, CBM_ERR_ARG, "checkpoint_every must be <= 64"));
+  rc = run(true);
+  if (rc == ADJ_NO_ROOM) {
+    // start again without snapshots from the initial state
+    free_snaps();
+    if (moved)
+      for (int a = 0; a < D && e == hipSuccess; ++a)
+        e = hipMemcpyAsync(P[0][a], ck + (size_t)a * c->C, sizeof(double) * c->C, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(lpart, 0, sizeof(double) * c->nblocks, c->stream);
+    if (e != hipSuccess) return fwd_fail(fail(c, CBM_ERR_HIP, "adjoint restart"));
+    rc = run(false);
   }
+  free_snaps();
+  if (rc) return fwd_fail(rc);
   // ---- per-simulation totals (blocks summed in order: deterministic)
   std::vector<double> hg((size_t)3 * c->nblocks), hl((size_t)c->nblocks);
   e = hipMemcpyAsync(hg.data(), gpart, sizeof(double) * hg.size(), hipMemcpyDeviceToHost, c->stream);

@@ -137,30 +137,32 @@ template <> struct PairLaw<LAW_AGGREGATION> {
 //                 ds/deps = -(4/w^2)(d - c)(dc - d)
 //                 ds/dr   = (2 a4/r)(d - c)(dc - d) + 2 a4 (dc - d) - f a4 (d - c)
 struct RepAdhGrad {
-  double k, r, eps, f, c, dc, a4, w2;
-  __device__ __forceinline__ explicit RepAdhGrad(const double* p) : k(p[0]), r(p[1]), eps(p[2]), f(p[3]) {
+  double k, r, f, c, dc, a4, e4, a4r, fa4;
+  __device__ __forceinline__ explicit RepAdhGrad(const double* p) : k(p[0]), r(p[1]), f(p[3]) {
     c = 2.0 * r;
     dc = f * r;
     const double w = dc - c;
-    w2 = w * w;
-    a4 = (4.0 * eps) / w2;
+    const double w2 = w * w;
+    a4 = (4.0 * p[2]) / w2;
+    e4 = 4.0 / w2;
+    a4r = 2.0 * a4 / r;
+    fa4 = f * a4;
   }
-  // Returns whether the pair interacts; fills phi, dphi/dd and dphi/dtheta.
-  __device__ __forceinline__ bool operator()(double d, double& phi, double& dphi, double* dth) const {
-    double s, sd, sk, sr, se;
-    if (d < c) {
-      s = k * (c - d); sd = -k; sk = c - d; sr = 2.0 * k; se = 0.0;
-    } else {
-      const double u = d - c, v = dc - d;
-      s = 0.0 - a4 * u * v;
-      sd = -a4 * (dc + c - 2.0 * d);
-      sk = 0.0;
-      se = -(4.0 / w2) * u * v;
-      sr = (2.0 * a4 / r) * u * v + 2.0 * a4 * v - f * a4 * u;
-    }
-    phi = s / d;
-    dphi = sd / d - s / (d * d);
-    dth[0] = sk / d; dth[1] = sr / d; dth[2] = se / d;
+  // Branch-free: returns whether the pair interacts and fills phi = s/d,
+  // dphi/dd / d and dphi/dtheta, with id = 1/d.
+  __device__ __forceinline__ bool operator()(double d, double id, double& phi, double& dphid,
+                                             double* dth) const {
+    const double u = d - c, v = dc - d;
+    const double uv = u * v;
+    const bool rep = d < c;
+    const double s = rep ? k * (c - d) : 0.0 - a4 * uv;
+    const double sd = rep ? -k : -a4 * (dc + c - 2.0 * d);
+    const double sk = rep ? c - d : 0.0;
+    const double sr = rep ? 2.0 * k : a4r * uv + 2.0 * a4 * v - fa4 * u;
+    const double se = rep ? 0.0 : -e4 * uv;
+    phi = s * id;
+    dphid = (sd - phi) * id * id;
+    dth[0] = sk * id; dth[1] = sr * id; dth[2] = se * id;
     return d > 0.0 && d < dc;
   }
 };

@@ -69,10 +69,15 @@ struct VerletArgs {
   int64_t ldl;
   int32_t cap;
   int32_t* overflow;
-  double rl;                  // list radius
+  double rl2;                 // largest s with sqrt(s) < list radius (exact test on d^2)
   int32_t build;              // 0: copy-back only (no list)
   int32_t nopermute;          // 1: rows stay in place; candidates via tmp, cells via lin
+  int32_t lds_r;              // list entries a lane keeps sorted in LDS before spilling
+  int32_t pad;
 };
+
+// LDS budget of the list build: lds_r * block * 8 bytes.
+constexpr int VERLET_LDS_BYTES = 32768;
 
 struct XferArgs {
   RowGeom g;
@@ -199,7 +204,7 @@ hipError_t launch_commit_counts(hipStream_t s, const RuleArgs& a, int32_t nsims)
 
 // ------------------------------------------------------------- launchers
 constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 inline int scan_blocks(int64_t n) { return (int)((n + SCAN_TILE - 1) / SCAN_TILE); }
 

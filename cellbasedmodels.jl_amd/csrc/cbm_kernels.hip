@@ -129,26 +129,42 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
     if (w < wave) wave_off += t;
     total += t;
   }
-  if (tid == 0) {
-    const unsigned int ep = s_epoch;
+  if (wave == 0) {
+    // Publish this tile's aggregate, then look back 64 tiles at a time: lane
+    // l reads tile p - l; the window ends at the nearest tile that already
+    // holds its inclusive prefix (tiles before 0 count as an inclusive 0).
+    const unsigned int ep = s_epoch & 0x3FFFFFFFu;
     int32_t prefix = 0;
-    if (tile == 0) {
-      __hip_atomic_store(&a.status[0], scan_word(ep, 2u, (unsigned int)total), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      __hip_atomic_store(&a.status[tile], scan_word(ep, 1u, (unsigned int)total), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      int64_t p = tile - 1;
-      while (p >= 0) {
-        const unsigned long long w = __hip_atomic_load(&a.status[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned int wep = (unsigned int)(w >> 34);
-        const unsigned int wfl = (unsigned int)(w >> 32) & 3u;
-        if (wep != (ep & 0x3FFFFFFFu) || wfl == 0u) continue;   // predecessor running: wait
-        prefix += (int32_t)(unsigned int)w;
-        if (wfl == 2u) break;
-        --p;
+    if (lane == 0)
+      __hip_atomic_store(&a.status[tile], scan_word(ep, tile == 0 ? 2u : 1u, (unsigned int)total),
+                         __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    int64_t p = tile - 1;
+    while (p >= 0) {
+      const int64_t q = p - lane;
+      unsigned int fl = 2u, val = 0u;
+      if (q >= 0) {
+        const unsigned long long w = __hip_atomic_load(&a.status[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        fl = ((unsigned int)(w >> 34) == ep) ? ((unsigned int)(w >> 32) & 3u) : 0u;
+        val = (unsigned int)w;
       }
-      __hip_atomic_store(&a.status[tile], scan_word(ep, 2u, (unsigned int)(prefix + total)), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long inc = __ballot(fl == 2u);
+      const int stop = inc ? __ffsll((unsigned long long)inc) - 1 : 63;
+      // every tile up to the stop lane must have published; else wait
+      const unsigned long long unready = __ballot(fl == 0u && lane <= stop);
+      if (unready) { __builtin_amdgcn_s_sleep(1); continue; }
+      int32_t x = lane <= stop ? (int32_t)val : 0;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+      prefix += x;
+      if (inc) break;
+      p -= 64;
     }
-    s_prefix = prefix;
+    if (lane == 0) {
+      if (tile > 0)
+        __hip_atomic_store(&a.status[tile], scan_word(ep, 2u, (unsigned int)(prefix + total)),
+                           __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      s_prefix = prefix;
+    }
   }
   __syncthreads();
   int32_t run = s_prefix + wave_off + (incl - sum);
@@ -224,8 +240,41 @@ __global__ __launch_bounds__(256) void k_permute(PermuteArgs pa) {
 // reference positions.  Entries are global rows ordered by ascending slot,
 // the order of the reference brute-force build (src/neighbors.jl:114-126).
 
+// Accept row j into row i's list.  The first R entries are kept sorted by
+// slot in this lane's LDS buffer (entry q at buf[q * B]); a list longer than
+// R is spilled to the global list once and continued there by insertion.
+__device__ __forceinline__ void verlet_accept(const VerletArgs& va, int64_t i, int2* __restrict__ buf,
+                                              int B, int R, int32_t& cnt, int32_t j, int32_t sj) {
+  if (cnt < R) {
+    int32_t pos = cnt;
+    while (pos > 0) {
+      const int2 prev = buf[(pos - 1) * B];
+      if (prev.x < sj) break;
+      buf[pos * B] = prev;
+      --pos;
+    }
+    buf[pos * B] = make_int2(sj, j);
+  } else {
+    if (cnt == R)
+      for (int q = 0; q < R; ++q) va.list[(int64_t)q * va.ldl + i] = buf[q * B].y;
+    if (cnt < va.cap) {
+      int32_t pos = cnt;
+      while (pos > 0) {
+        const int32_t prev = va.list[(int64_t)(pos - 1) * va.ldl + i];
+        if (va.aslot[prev] < sj) break;
+        va.list[(int64_t)pos * va.ldl + i] = prev;
+        --pos;
+      }
+      va.list[(int64_t)pos * va.ldl + i] = j;
+    } else {
+      *va.overflow = 1;
+    }
+  }
+  ++cnt;
+}
+
 template <int D>
-__device__ __forceinline__ int32_t verlet_row(const VerletArgs& va, const SimDev& sd, int64_t i) {
+__device__ __forceinline__ int32_t verlet_row(const VerletArgs& va, const SimDev& sd, int64_t i, int2* buf, int B) {
   const RebuildArgs& a = va.rb;
   if (!va.build) {
     for (int k = 0; k < va.cols.n; ++k) {
@@ -240,46 +289,91 @@ __device__ __forceinline__ int32_t verlet_row(const VerletArgs& va, const SimDev
 #pragma unroll
   for (int c = 0; c < D; ++c) xi[c] = va.apos[c][i];
   const int32_t c = va.nopermute ? a.lin[i] : a.cell_row[i];
-  const int32_t cx = c % sd.gn[0];
-  const int32_t cy = (c / sd.gn[0]) % sd.gn[1];
-  const int32_t cz = c / (sd.gn[0] * sd.gn[1]);
+  const int32_t cc[3] = {c % sd.gn[0], (c / sd.gn[0]) % sd.gn[1], c / (sd.gn[0] * sd.gn[1])};
   const int32_t* __restrict__ rank = a.crank + sd.rank_off;
+  const int R = va.lds_r;
   int32_t cnt = 0;
-  for (int k = 0; k < pow3(D); ++k) {
-    const int32_t nc = neigh_cell<D>(sd, cx, cy, cz, k, a.periodic);
-    if (nc < 0) continue;
-    const int64_t gc = sd.cell_off + rank[nc];
-    const int32_t j0 = a.start[gc], j1 = a.start[gc + 1];
-    for (int32_t jq = j0; jq < j1; ++jq) {
-      const int32_t j = va.nopermute ? a.tmp[jq] : jq;
-      if (j == i) continue;
-      double dx[3];
+  // Columns of three x-adjacent cells; the candidates of a column are walked
+  // as one flattened range so that lanes of a wavefront stay in step.
+  constexpr int NCOL = D == 1 ? 1 : (D == 2 ? 3 : 9);
 #pragma unroll
-      for (int q = 0; q < D; ++q) {
-        dx[q] = xi[q] - va.apos[q][j];
-        if (a.periodic) dx[q] = min_image(dx[q], sd.L[q]);
+  for (int col = 0; col < NCOL; ++col) {
+    int32_t yz = 0;
+    bool ok = true;
+    if (D > 1) {
+      int32_t q[2] = {cc[1] + col % 3 - 1, cc[2] + col / 3 - 1};
+#pragma unroll
+      for (int e = 0; e < D - 1; ++e) {
+        int32_t v = q[e];
+        if (a.periodic) v = v < 0 ? v + sd.gn[1 + e] : (v >= sd.gn[1 + e] ? v - sd.gn[1 + e] : v);
+        else ok = ok && v >= 0 && v < sd.gn[1 + e];
+        q[e] = v;
       }
-      double s2 = dx[0] * dx[0];
-      if (D > 1) s2 = s2 + dx[1] * dx[1];
-      if (D > 2) s2 = s2 + dx[2] * dx[2];
-      if (!(sqrt(s2) < va.rl)) continue;
-      if (cnt < va.cap) {
-        // insertion by ascending logical slot
-        const int32_t sj = va.aslot[j];
-        int32_t pos = cnt;
-        while (pos > 0) {
-          const int32_t prev = va.list[(int64_t)(pos - 1) * va.ldl + i];
-          if (va.aslot[prev] < sj) break;
-          va.list[(int64_t)pos * va.ldl + i] = prev;
-          --pos;
+      yz = sd.gn[0] * (q[0] + (D > 2 ? sd.gn[1] * q[1] : 0));
+    }
+    if (!ok) continue;
+    int32_t s3[3], n3[3];
+#pragma unroll
+    for (int dxc = 0; dxc < 3; ++dxc) {
+      int32_t v = cc[0] + dxc - 1;
+      bool okx = true;
+      if (a.periodic) v = v < 0 ? v + sd.gn[0] : (v >= sd.gn[0] ? v - sd.gn[0] : v);
+      else okx = v >= 0 && v < sd.gn[0];
+      s3[dxc] = 0; n3[dxc] = 0;
+      if (okx) {
+        const int64_t gc = sd.cell_off + rank[yz + v];
+        s3[dxc] = a.start[gc];
+        n3[dxc] = a.start[gc + 1] - s3[dxc];
+      }
+    }
+    // candidates in chunks of VC whose loads are issued together; slots past
+    // the end load nothing and are skipped
+    constexpr int VC = 4;
+    const int32_t n01 = n3[0] + n3[1], tot = n01 + n3[2];
+    for (int32_t t0 = 0; t0 < tot; t0 += VC) {
+      int32_t jj[VC];
+#pragma unroll
+      for (int u = 0; u < VC; ++u) {
+        const int32_t t = t0 + u;
+        jj[u] = (int32_t)i;
+        if (t < tot) {
+          const int32_t jq = t < n3[0] ? s3[0] + t : (t < n01 ? s3[1] + (t - n3[0]) : s3[2] + (t - n01));
+          jj[u] = va.nopermute ? a.tmp[jq] : jq;
         }
-        va.list[(int64_t)pos * va.ldl + i] = j;
-      } else {
-        *va.overflow = 1;
       }
-      ++cnt;
+      double px[VC][3];
+      int32_t sl[VC];
+#pragma unroll
+      for (int u = 0; u < VC; ++u) {
+#pragma unroll
+        for (int q = 0; q < D; ++q) px[u][q] = xi[q];
+        sl[u] = 0;
+        if (t0 + u < tot) {
+#pragma unroll
+          for (int q = 0; q < D; ++q) px[u][q] = va.apos[q][jj[u]];
+          sl[u] = va.aslot[jj[u]];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < VC; ++u) {
+        if (t0 + u >= tot || jj[u] == i) continue;
+        double dx[3];
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+          dx[q] = xi[q] - px[u][q];
+          if (a.periodic) dx[q] = min_image(dx[q], sd.L[q]);
+        }
+        double s2 = dx[0] * dx[0];
+        if (D > 1) s2 = s2 + dx[1] * dx[1];
+        if (D > 2) s2 = s2 + dx[2] * dx[2];
+        // s2 <= rl2  <=>  sqrt(s2) < rl for a correctly rounded sqrt
+        if (!(s2 <= va.rl2)) continue;
+        verlet_accept(va, i, buf, B, R, cnt, jj[u], sl[u]);
+      }
     }
   }
+  if (cnt <= R)
+    for (int32_t q = 0; q < cnt; ++q) va.list[(int64_t)q * va.ldl + i] = buf[q * B].y;
   const int32_t kept = cnt < va.cap ? cnt : va.cap;
   va.list_n[i] = kept;
   for (int k = 0; k < (va.nopermute ? 0 : va.cols.n); ++k) {
@@ -296,13 +390,14 @@ __device__ __forceinline__ int32_t verlet_row(const VerletArgs& va, const SimDev
 
 template <int D>
 __global__ __launch_bounds__(256) void k_verlet(VerletArgs va) {
+  extern __shared__ int2 vbuf[];
   const RebuildArgs& a = va.rb;
   if (a.flag && *a.flag == 0) return;
   const RowIdx ri = row_index(a.g);
   const SimDev& sd = a.g.sims[ri.sim];
   const int64_t N = a.g.st[ri.sim].N;
   if (ri.local >= N) return;
-  (void)verlet_row<D>(va, sd, ri.row);
+  (void)verlet_row<D>(va, sd, ri.row, vbuf + threadIdx.x, (int)blockDim.x);
 }
 
 
@@ -504,30 +599,58 @@ __global__ __launch_bounds__(256) void k_adjoint_step(AdjointArgs a) {
     double xi[3] = {0.0, 0.0, 0.0}, li[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int c = 0; c < D; ++c) { xi[c] = a.X[c][i]; li[c] = a.lam[c][i]; }
-    const int32_t cnt = a.list_n[i];
-    for (int32_t q = 0; q < cnt; ++q) {
-      const int64_t j = a.list[(int64_t)q * a.ldl + i];
+    double L[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < D; ++c) L[c] = sd.L[c];
+    auto pair = [&](const double* xj, const double* lj, bool in) {
       double r[3], u[3];
 #pragma unroll
       for (int c = 0; c < D; ++c) {
-        r[c] = xi[c] - a.X[c][j];
-        if (a.periodic) { const double L = sd.L[c]; r[c] = r[c] > 0.5 * L ? r[c] - L : (r[c] < -0.5 * L ? r[c] + L : r[c]); }
-        u[c] = li[c] - a.lam[c][j];
+        r[c] = xi[c] - xj[c];
+        if (a.periodic) r[c] = r[c] > 0.5 * L[c] ? r[c] - L[c] : (r[c] < -0.5 * L[c] ? r[c] + L[c] : r[c]);
+        u[c] = li[c] - lj[c];
       }
       double s2 = r[0] * r[0];
       if (D > 1) s2 += r[1] * r[1];
       if (D > 2) s2 += r[2] * r[2];
       const double d = sqrt(s2);
-      double phi, dphi, dth[3];
-      if (!law(d, phi, dphi, dth)) continue;
+      const double id = 1.0 / d;
+      double phi, dphid, dth[3];
+      const bool on = law(d, id, phi, dphid, dth) && in;
       double ru = 0.0, lr = 0.0;
 #pragma unroll
       for (int c = 0; c < D; ++c) { ru += r[c] * u[c]; lr += li[c] * r[c]; }
-      const double g = dphi / d * ru;
+      const double pe = on ? phi : 0.0;
+      const double ge = on ? dphid * ru : 0.0;
 #pragma unroll
-      for (int c = 0; c < D; ++c) v[c] += phi * u[c] + g * r[c];
+      for (int c = 0; c < D; ++c) v[c] += pe * u[c] + ge * r[c];
 #pragma unroll
-      for (int q2 = 0; q2 < 3; ++q2) gth[q2] += dth[q2] * lr;
+      for (int q2 = 0; q2 < 3; ++q2) gth[q2] += (on ? dth[q2] : 0.0) * lr;
+    };
+    // chunks of two list entries, next indices prefetched; entries past the
+    // count take the agent itself without a load (d = 0, never active)
+    const int32_t cnt = a.list_n[i];
+    const int64_t ldl = a.ldl;
+    const int32_t* __restrict__ lp = a.list + i;
+    int32_t jn[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) jn[u] = (u < cnt) ? lp[(int64_t)u * ldl] : (int32_t)i;
+    for (int32_t q0 = 0; q0 < cnt; q0 += 2) {
+      double xj[2][3], lj[2][3];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) { xj[u][c] = xi[c]; lj[u][c] = li[c]; }
+        if (q0 + u < cnt) {
+#pragma unroll
+          for (int c = 0; c < D; ++c) { xj[u][c] = a.X[c][jn[u]]; lj[u][c] = a.lam[c][jn[u]]; }
+        }
+      }
+      lp += 2 * ldl;
+#pragma unroll
+      for (int u = 0; u < 2; ++u) jn[u] = (q0 + 2 + u < cnt) ? lp[(int64_t)u * ldl] : (int32_t)i;
+#pragma unroll
+      for (int u = 0; u < 2; ++u) pair(xj[u], lj[u], q0 + u < cnt);
     }
     double direct[3] = {0.0, 0.0, 0.0};
     if (a.save >= 0) {
@@ -590,7 +713,7 @@ __global__ __launch_bounds__(256) void k_disp_check(StageArgs a) {
     if (a.periodic) dx = min_image(dx, sd.L[c]);
     s2 = (c == 0) ? dx * dx : s2 + dx * dx;
   }
-  if (sqrt(s2) >= a.thr) *a.flag = 1;
+  if (s2 > a.thr2) *a.flag = 1;   // sqrt(s2) >= thr
 }
 
 // ------------------------------------------------------------------ medium
@@ -777,10 +900,11 @@ hipError_t launch_permute(int nblocks, int block, hipStream_t s, const PermuteAr
 }
 
 hipError_t launch_verlet(int dims, int nblocks, int block, hipStream_t s, const VerletArgs& a) {
+  const size_t lds = a.build ? (size_t)a.lds_r * (size_t)block * sizeof(int2) : 0;
   switch (dims) {
-    case 1: hipLaunchKernelGGL(k_verlet<1>, dim3(nblocks), dim3(block), 0, s, a); break;
-    case 2: hipLaunchKernelGGL(k_verlet<2>, dim3(nblocks), dim3(block), 0, s, a); break;
-    default: hipLaunchKernelGGL(k_verlet<3>, dim3(nblocks), dim3(block), 0, s, a); break;
+    case 1: hipLaunchKernelGGL(k_verlet<1>, dim3(nblocks), dim3(block), lds, s, a); break;
+    case 2: hipLaunchKernelGGL(k_verlet<2>, dim3(nblocks), dim3(block), lds, s, a); break;
+    default: hipLaunchKernelGGL(k_verlet<3>, dim3(nblocks), dim3(block), lds, s, a); break;
   }
   return hipGetLastError();
 }

@@ -130,31 +130,37 @@ __device__ __forceinline__ void stage_body(const StageArgs& a) {
 #pragma unroll
         for (int c = 0; c < D; ++c) { const double v = acc[c] + s * (dx[c] / d); acc[c] = on ? v : acc[c]; }
       } else {
-        const double f = s / d;               // built-in laws: F = (s/d) (x_i - x_j)
+        // F = (s/d) (x_i - x_j).  An inactive pair adds 0*dx = +-0, which
+        // leaves acc unchanged: acc starts at +0 and a sum is -0 only when
+        // both terms are, so acc is never -0.
+        const double f = s / d;
+        const double fe = on ? f : 0.0;
 #pragma unroll
-        for (int c = 0; c < D; ++c) { const double v = acc[c] + f * dx[c]; acc[c] = on ? v : acc[c]; }
+        for (int c = 0; c < D; ++c) acc[c] = acc[c] + fe * dx[c];
       }
     };
     if (NB == NB_LIST) {
       // Verlet list in chunks of LCH gathers issued together (memory-level
-      // parallelism per lane), branch-free: slots past the count re-read the
-      // last entry and are masked out of the accumulation.
+      // parallelism per lane), branch-free: slots past the count read the
+      // agent itself (d = 0, never active) and add nothing.  The list
+      // indices of the next chunk are loaded before the current chunk's
+      // positions are used.
       constexpr int LCH = CBM_LIST_CHUNK;
       const int32_t cnt = a.list_n[i];
-      const int32_t* __restrict__ lst = a.list + i;
       const int64_t ldl = a.ldl;
-      for (int32_t q0 = 0; q0 < cnt; q0 += LCH) {
-        int32_t jj[LCH];
+      const int32_t* __restrict__ lp = a.list + i;     // entry q at lp[q * ldl]
+      int32_t jn[LCH];
 #pragma unroll
-        for (int u = 0; u < LCH; ++u) {
-          const int32_t q = (q0 + u < cnt) ? q0 + u : cnt - 1;
-          jj[u] = lst[(int64_t)q * ldl];
-        }
+      for (int u = 0; u < LCH; ++u) jn[u] = (u < cnt) ? lp[(int64_t)u * ldl] : (int32_t)i;
+      for (int32_t q0 = 0; q0 < cnt; q0 += LCH) {
         double px[LCH][3];
 #pragma unroll
         for (int u = 0; u < LCH; ++u)
 #pragma unroll
-          for (int c = 0; c < D; ++c) px[u][c] = a.S[c][jj[u]];
+          for (int c = 0; c < D; ++c) px[u][c] = a.S[c][jn[u]];
+        lp += LCH * ldl;
+#pragma unroll
+        for (int u = 0; u < LCH; ++u) jn[u] = (q0 + LCH + u < cnt) ? lp[(int64_t)u * ldl] : (int32_t)i;
 #pragma unroll
         for (int u = 0; u < LCH; ++u) body(px[u], q0 + u < cnt);
       }
@@ -249,7 +255,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a) {
     double s2 = dx[0] * dx[0];
     if (D > 1) s2 = s2 + dx[1] * dx[1];
     if (D > 2) s2 = s2 + dx[2] * dx[2];
-    if (sqrt(s2) >= a.thr) *a.flag = 1;   // src/neighbors.jl:403 (>= skin/2)
+    if (s2 > a.thr2) *a.flag = 1;   // sqrt(s2) >= thr, src/neighbors.jl:403 (>= skin/2)
   }
 }
 

@@ -88,7 +88,8 @@ struct StageArgs {
   const int32_t* cell_row;      // row -> linear local cell (cell-linked)
   const int32_t* crank;         // linear local cell -> bucket rank (Morton order)
   int32_t* flag;                // rebuild flag written by the final stage
-  double dt, sqdt, drift, noise, thr;
+  double dt, sqdt, drift, noise;
+  double thr2;                  // largest d2 with sqrt(d2) < rebuild displacement
   uint64_t seed;
   const int64_t* step_ptr;     // device step counter (Philox counter word)
   int32_t kind, periodic, lagged, walls, nvar, disp_check, write_fnew, pad;

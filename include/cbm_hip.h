@@ -228,8 +228,12 @@ int cbm_count_neighbors(cbm_ctx* ctx, int32_t sim, double radius, int64_t* out);
  * save points is taken as 1/(n_saves*N) sum ...), for the REP_ADH law with
  * parameters (k, r_eq, eps) of every simulation.  target holds n_saves
  * snapshots of dims*N doubles (SoA per snapshot: x[N], y[N], z[N]) per sim,
- * concatenated over sims.  Checkpoints every `checkpoint_every` steps are
- * kept on device and the steps between them recomputed in the backward pass.
+ * concatenated over sims.  Checkpoints every `checkpoint_every` (<= 64)
+ * steps are kept on device and the steps between them recomputed in the
+ * backward pass.  The Verlet lists of the forward pass are kept too while
+ * they fit in half of the free device memory (or in CBM_ADJ_SNAPSHOT_MB MiB
+ * when that environment variable is set); otherwise the lists are checked
+ * and rebuilt along the backward pass, with identical results.
  * grad[3*s + {0,1,2}] = dL_s/d(k, r_eq, eps); loss[s] = L_s. */
 typedef struct cbm_loss_desc {
   int64_t n_steps;

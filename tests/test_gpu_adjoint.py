@@ -3,6 +3,8 @@ loss equal to the oracle's, gradient d loss / d(k, r_eq, eps) equal to the
 oracle's forward-mode dual-number gradient (the stand-in for ForwardDiff on
 the CPU reference) and to central finite differences of the device loss
 within 1e-6 relative."""
+import os
+
 import numpy as np
 import pytest
 
@@ -97,3 +99,19 @@ def test_batched_adjoint_parameter_sets(gpu_available):
         rl, rg = o.traj_grad_dual(t, steps, saves, tgt.ravel())
         assert abs(loss[q] - rl) <= 1e-12 * abs(rl)
         assert np.allclose(grad[q], rg, rtol=1e-9)
+
+
+def test_adjoint_without_list_snapshots_is_identical(gpu_available):
+    """The backward sweep that re-validates and rebuilds lists gives the same
+    bits as the one on the forward pass's list snapshots (lists are sorted by
+    slot, so the active pairs are summed in the same order)."""
+    m, box, pos, saves, tgt, steps = setup(n=512, steps=30, saves=(15, 30), seed=11)
+    g = device(m, box, pos, [THETA0])
+    loss_s, grad_s = g.adjoint(steps, 8, saves, [tgt])
+    os.environ["CBM_ADJ_SNAPSHOT_MB"] = "0"
+    try:
+        loss_r, grad_r = g.adjoint(steps, 8, saves, [tgt])
+    finally:
+        del os.environ["CBM_ADJ_SNAPSHOT_MB"]
+    assert np.array_equal(loss_s, loss_r)
+    assert np.array_equal(grad_s, grad_r)
