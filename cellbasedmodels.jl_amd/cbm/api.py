@@ -353,13 +353,17 @@ class Context:
     def adjoint(self, n_steps, checkpoint_every, save_steps, target):
         """Loss and d(loss)/d(k, r_eq, eps) per simulation (cbm_adjoint).
 
-        target: per simulation, an array [n_saves, dims, N] in logical slot order."""
+        target: per simulation, an array [n_saves, dims, N] in logical slot order,
+        or None to reuse the targets of the previous call (already on the device)."""
         ss = np.ascontiguousarray(save_steps, dtype=np.int64)
-        tg = np.ascontiguousarray(np.concatenate([np.asarray(t, dtype=np.float64).ravel() for t in target]))
         L = A.LossDesc()
         L.n_steps, L.checkpoint_every, L.n_saves = int(n_steps), int(checkpoint_every), ss.size
         L.save_steps = ss.ctypes.data_as(C.POINTER(C.c_int64))
-        L.target = tg.ctypes.data_as(C.POINTER(C.c_double))
+        if target is None:
+            L.target = None
+        else:
+            tg = np.ascontiguousarray(np.concatenate([np.asarray(t, dtype=np.float64).ravel() for t in target]))
+            L.target = tg.ctypes.data_as(C.POINTER(C.c_double))
         S = len(self.sims)
         grad = np.zeros(3 * S)
         loss = np.zeros(S)

@@ -88,6 +88,13 @@ struct cbm_ctx {
   int32_t* list = nullptr;
   int32_t* list_n = nullptr;
   int cap = 0;
+
+  // adjoint work buffers, kept across calls and grown on demand
+  struct DevBuf { void* p = nullptr; size_t bytes = 0; };
+  DevBuf adj_tgt, adj_tbase, adj_invn, adj_ck, adj_seg, adj_lam, adj_gpart, adj_lpart;
+  int adj_tgt_saves = 0;                  // shape of the targets held in adj_tgt
+  std::vector<int64_t> adj_tgt_n;
+  std::vector<int32_t*> snap_l, snap_n;   // Verlet-list snapshots of the forward pass
   int32_t* flags = nullptr;               // [0..1] rebuild flags by step parity, [2] list overflow, [3] capacity
   int64_t* counters = nullptr;            // [0] rebuilds, [1] device step
   bool structure_valid = false;
@@ -327,6 +334,10 @@ static void destroy_ctx(cbm_ctx* c) {
   F(c->med_c[0]); F(c->med_c[1]); F(c->med_count);
   F(c->key); F(c->lin); F(c->crank); F(c->count); F(c->cursor); F(c->start); F(c->tmp); F(c->cell_row);
   F(c->scan_status); F(c->scan_ticket); F(c->list); F(c->list_n); F(c->flags); F(c->counters);
+  for (cbm_ctx::DevBuf* b : {&c->adj_tgt, &c->adj_tbase, &c->adj_invn, &c->adj_ck, &c->adj_seg, &c->adj_lam,
+                             &c->adj_gpart, &c->adj_lpart}) F(b->p);
+  for (int32_t* p : c->snap_l) F(p);
+  for (int32_t* p : c->snap_n) F(p);
   F(c->d_staging); F(c->d_block_sim); F(c->d_block_off); F(c->d_sims); F(c->d_st);
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1181,7 +1192,7 @@ int cbm_adjoint(cbm_ctx* c, const cbm_loss_desc* L, double* grad, double* loss) 
     return fail(c, CBM_ERR_CONFIG, "the adjoint covers overdamped Euler with the REP_ADH law on Verlet lists (no rules, medium, lag or drift)");
   const int64_t T = L->n_steps, K = L->checkpoint_every;
   const int S = L->n_saves;
-  if (T < 1 || K < 1 || S < 1 || !L->save_steps || !L->target) return fail(c, CBM_ERR_ARG, "bad loss description");
+  if (T < 1 || K < 1 || S < 1 || !L->save_steps) return fail(c, CBM_ERR_ARG, "bad loss description");
   std::vector<int> save_at((size_t)T + 1, -1);
   for (int q = 0; q < S; ++q) {
     const int64_t st = L->save_steps[q];
@@ -1191,49 +1202,55 @@ int cbm_adjoint(cbm_ctx* c, const cbm_loss_desc* L, double* grad, double* loss) 
   int rc;
   if (!c->structure_valid && (rc = force_rebuild(c))) return rc;
   const int D = c->dims;
-  // per-sim sizes and the logical-order offsets of the targets
-  std::vector<int64_t> n(c->S), off(c->S + 1, 0);
+  // per-sim sizes; the targets stay in the caller's layout (per sim, per
+  // save, per axis, slot order) at per-sim bases
+  std::vector<int64_t> n(c->S), base(c->S + 1, 0);
+  std::vector<double> invn(c->S);
   for (int s = 0; s < c->S; ++s) {
     if ((rc = cbm_get_n(c, s, &n[s]))) return rc;
     if (n[s] < 1) return fail(c, CBM_ERR_ARG, "every simulation needs agents");
-    off[s + 1] = off[s] + n[s];
+    base[s + 1] = base[s] + (int64_t)S * D * n[s];
+    invn[s] = 1.0 / ((double)S * (double)n[s]);
   }
-  const int64_t ntot = off[c->S];
-  std::vector<double> tgt((size_t)S * D * ntot), invn(c->S);
-  {
-    const double* src = L->target;
-    for (int s = 0; s < c->S; ++s) {
-      for (int q = 0; q < S; ++q)
-        for (int a = 0; a < D; ++a)
-          for (int64_t i = 0; i < n[s]; ++i) tgt[((size_t)q * D + a) * ntot + off[s] + i] = *src++;
-      invn[s] = 1.0 / ((double)S * (double)n[s]);
-    }
-  }
+  if (!L->target && (c->adj_tgt_saves != S || c->adj_tgt_n != n))
+    return fail(c, CBM_ERR_ARG, "no targets: the previous call's targets have another shape");
   const int64_t nck = (T + K - 1) / K;
   const size_t state = (size_t)D * c->C;
-  double *d_tgt = nullptr, *d_invn = nullptr, *ck = nullptr, *seg = nullptr, *lam = nullptr, *gpart = nullptr, *lpart = nullptr;
-  int64_t* d_off = nullptr;
-  auto release = [&]() {
-    for (void* p : {(void*)d_tgt, (void*)d_invn, (void*)ck, (void*)seg, (void*)lam, (void*)gpart, (void*)lpart, (void*)d_off})
-      if (p) (void)hipFree(p);
-  };
   hipError_t e = hipSuccess;
-  auto A = [&](void** p, size_t bytes) { if (e == hipSuccess) e = hipMalloc(p, bytes ? bytes : 8); };
-  A((void**)&d_tgt, sizeof(double) * tgt.size());
-  A((void**)&d_invn, sizeof(double) * c->S);
-  A((void**)&d_off, sizeof(int64_t) * c->S);
-  A((void**)&ck, sizeof(double) * state * nck);
-  A((void**)&seg, sizeof(double) * state * K);
-  A((void**)&lam, sizeof(double) * state * 2);
-  A((void**)&gpart, sizeof(double) * 3 * c->nblocks);
-  A((void**)&lpart, sizeof(double) * c->nblocks);
-  if (e == hipSuccess) e = hipMemcpyAsync(d_tgt, tgt.data(), sizeof(double) * tgt.size(), hipMemcpyHostToDevice, c->stream);
+  auto grow = [&](cbm_ctx::DevBuf& b, size_t bytes) {
+    if (e != hipSuccess || b.bytes >= bytes) return;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr; b.bytes = 0;
+    e = hipMalloc(&b.p, bytes ? bytes : 8);
+    if (e == hipSuccess) b.bytes = bytes;
+  };
+  grow(c->adj_tgt, sizeof(double) * (size_t)base[c->S]);
+  grow(c->adj_tbase, sizeof(int64_t) * c->S);
+  grow(c->adj_invn, sizeof(double) * c->S);
+  grow(c->adj_ck, sizeof(double) * state * nck);
+  grow(c->adj_seg, sizeof(double) * state * K);
+  grow(c->adj_lam, sizeof(double) * state * 2);
+  grow(c->adj_gpart, sizeof(double) * 3 * c->nblocks);
+  grow(c->adj_lpart, sizeof(double) * c->nblocks);
+  double* d_tgt = (double*)c->adj_tgt.p;
+  double* d_invn = (double*)c->adj_invn.p;
+  int64_t* d_base = (int64_t*)c->adj_tbase.p;
+  double* ck = (double*)c->adj_ck.p;
+  double* seg = (double*)c->adj_seg.p;
+  double* lam = (double*)c->adj_lam.p;
+  double* gpart = (double*)c->adj_gpart.p;
+  double* lpart = (double*)c->adj_lpart.p;
+  if (e == hipSuccess && L->target) {
+    c->adj_tgt_saves = 0;
+    e = hipMemcpyAsync(d_tgt, L->target, sizeof(double) * (size_t)base[c->S], hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) { c->adj_tgt_saves = S; c->adj_tgt_n = n; }
+  }
   if (e == hipSuccess) e = hipMemcpyAsync(d_invn, invn.data(), sizeof(double) * c->S, hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * c->S, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_base, base.data(), sizeof(int64_t) * c->S, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(gpart, 0, sizeof(double) * 3 * c->nblocks, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(lpart, 0, sizeof(double) * c->nblocks, c->stream);
-  if (e != hipSuccess) { release(); return fail(c, CBM_ERR_HIP, std::string("adjoint buffers: ") + hipGetErrorString(e)); }
-  auto ptrs = [&](double* base, double** out3) { for (int a = 0; a < 3; ++a) out3[a] = a < D ? base + (size_t)a * c->C : nullptr; };
+  if (e != hipSuccess) return fail(c, CBM_ERR_HIP, std::string("adjoint buffers: ") + hipGetErrorString(e));
+  auto ptrs = [&](double* b0, double** out3) { for (int a = 0; a < 3; ++a) out3[a] = a < D ? b0 + (size_t)a * c->C : nullptr; };
   int32_t* flag = c->flags + 5;
   double* X0[3];
   for (int a = 0; a < D; ++a) X0[a] = statecol(c, a, false);
@@ -1242,15 +1259,18 @@ int cbm_adjoint(cbm_ctx* c, const cbm_loss_desc* L, double* grad, double* loss) 
   int cur = 0;
   LossArgs la;
   std::memset(&la, 0, sizeof(la));
-  la.g = geom(c); la.slot = (const int32_t*)c->col[C_SLOT].cur(); la.target = d_tgt; la.tgt_off = d_off;
-  la.tgt_n = ntot; la.dims = D; la.lpart = lpart;
-  auto fwd_fail = [&](int code) { release(); return code; };
+  la.g = geom(c); la.slot = (const int32_t*)c->col[C_SLOT].cur(); la.target = d_tgt; la.tgt_base = d_base;
+  la.dims = D; la.lpart = lpart;
+  auto fwd_fail = [&](int code) { return code; };
   // ---- list snapshots.  Every list the forward pass builds is kept (while
   // half of the free device memory holds them), so the recompute and the
   // backward sweep run on exactly the forward's lists: no displacement
   // checks, no rebuilds.  Without room the lists are re-validated, and
-  // rebuilt where needed, along the backward sweep instead.
-  std::vector<int32_t*> snap_l, snap_n;
+  // rebuilt where needed, along the backward sweep instead.  The snapshot
+  // buffers stay in the context for the next call.
+  std::vector<int32_t*>& snap_l = c->snap_l;
+  std::vector<int32_t*>& snap_n = c->snap_n;
+  size_t used = 0;   // snapshots of this call
   std::vector<int32_t> epoch((size_t)T, 0);
   auto free_snaps = [&]() {
     for (int32_t* p : snap_l) (void)hipFree(p);
@@ -1259,21 +1279,25 @@ int cbm_adjoint(cbm_ctx* c, const cbm_loss_desc* L, double* grad, double* loss) 
   };
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
-  size_t budget = free_b / 2;
-  if (const char* mb = std::getenv("CBM_ADJ_SNAPSHOT_MB")) budget = (size_t)std::strtoull(mb, nullptr, 10) << 20;
   const size_t snap_bytes = sizeof(int32_t) * ((size_t)c->cap + 1) * (size_t)c->C;
+  size_t budget = free_b / 2 + snap_l.size() * snap_bytes;
+  if (const char* mb = std::getenv("CBM_ADJ_SNAPSHOT_MB")) budget = (size_t)std::strtoull(mb, nullptr, 10) << 20;
+
   auto new_snap = [&]() -> bool {
-    if ((snap_l.size() + 1) * snap_bytes > budget) return false;
-    int32_t *l = nullptr, *nn = nullptr;
-    if (hipMalloc((void**)&l, sizeof(int32_t) * (size_t)c->cap * (size_t)c->C) != hipSuccess) return false;
-    if (hipMalloc((void**)&nn, sizeof(int32_t) * (size_t)c->C) != hipSuccess) { (void)hipFree(l); return false; }
-    snap_l.push_back(l); snap_n.push_back(nn);
+    if ((used + 1) * snap_bytes > budget) return false;
+    if (used == snap_l.size()) {
+      int32_t *l = nullptr, *nn = nullptr;
+      if (hipMalloc((void**)&l, sizeof(int32_t) * (size_t)c->cap * (size_t)c->C) != hipSuccess) return false;
+      if (hipMalloc((void**)&nn, sizeof(int32_t) * (size_t)c->C) != hipSuccess) { (void)hipFree(l); return false; }
+      snap_l.push_back(l); snap_n.push_back(nn);
+    }
+    ++used;
     return true;
   };
   AdjointArgs aa;
   std::memset(&aa, 0, sizeof(aa));
   aa.g = geom(c); aa.ldl = c->C;
-  aa.slot = (const int32_t*)c->col[C_SLOT].cur(); aa.target = d_tgt; aa.tgt_off = d_off; aa.tgt_n = ntot;
+  aa.slot = (const int32_t*)c->col[C_SLOT].cur(); aa.target = d_tgt; aa.tgt_base = d_base;
   aa.periodic = m.periodic; aa.dt = m.dt; aa.inv_n = d_invn; aa.gpart = gpart;
   double* LAM[2][3];
   ptrs(lam, LAM[0]);
@@ -1385,6 +1409,7 @@ int cbm_adjoint(cbm_ctx* c, const cbm_loss_desc* L, double* grad, double* loss) 
   if (rc == ADJ_NO_ROOM) {
     // start again without snapshots from the initial state
     free_snaps();
+    used = 0;
     if (moved)
       for (int a = 0; a < D && e == hipSuccess; ++a)
         e = hipMemcpyAsync(P[0][a], ck + (size_t)a * c->C, sizeof(double) * c->C, hipMemcpyDeviceToDevice, c->stream);
@@ -1392,7 +1417,6 @@ int cbm_adjoint(cbm_ctx* c, const cbm_loss_desc* L, double* grad, double* loss) 
     if (e != hipSuccess) return fwd_fail(fail(c, CBM_ERR_HIP, "adjoint restart"));
     rc = run(false);
   }
-  free_snaps();
   if (rc) return fwd_fail(rc);
   // ---- per-simulation totals (blocks summed in order: deterministic)
   std::vector<double> hg((size_t)3 * c->nblocks), hl((size_t)c->nblocks);
@@ -1401,7 +1425,7 @@ int cbm_adjoint(cbm_ctx* c, const cbm_loss_desc* L, double* grad, double* loss) 
   // the community is left at its initial state
   for (int a = 0; a < D && e == hipSuccess; ++a) e = hipMemcpyAsync(X0[a], ck + (size_t)a * c->C, sizeof(double) * c->C, hipMemcpyDeviceToDevice, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (e != hipSuccess) { release(); return fail(c, CBM_ERR_HIP, std::string("adjoint readback: ") + hipGetErrorString(e)); }
+  if (e != hipSuccess) return fail(c, CBM_ERR_HIP, std::string("adjoint readback: ") + hipGetErrorString(e));
   for (int s = 0; s < c->S; ++s) { grad[3 * s] = grad[3 * s + 1] = grad[3 * s + 2] = 0.0; loss[s] = 0.0; }
   for (int b = 0; b < c->nblocks; ++b) {
     const int s = c->h_block_sim[b];
@@ -1409,7 +1433,6 @@ int cbm_adjoint(cbm_ctx* c, const cbm_loss_desc* L, double* grad, double* loss) 
     loss[s] += hl[(size_t)b];
   }
   for (int s = 0; s < c->S; ++s) loss[s] *= invn[s];
-  release();
   c->structure_valid = false;
   return check_flags(c);
 }

@@ -165,9 +165,8 @@ struct AdjointArgs {
   const int32_t* list_n;
   int64_t ldl;
   const int32_t* slot;
-  const double* target;         // save m, axis a, logical order of all sims: [(m*D + a)*ntot + off[sim] + slot]
-  const int64_t* tgt_off;       // per-sim offset into the logical order
-  int64_t tgt_n;                // agents of all sims (ntot)
+  const double* target;         // sim s, save m, axis a, slot i: [tgt_base[s] + (m*D + a)*N_s + i]
+  const int64_t* tgt_base;      // per-sim start of its targets
   int32_t save;                 // save index whose direct term enters lambda_t, or -1
   int32_t periodic;
   double dt;
@@ -180,9 +179,8 @@ struct LossArgs {
   RowGeom g;
   const double* X[3];
   const int32_t* slot;
-  const double* target;
-  const int64_t* tgt_off;
-  int64_t tgt_n;
+  const double* target;         // as in AdjointArgs
+  const int64_t* tgt_base;
   int32_t save;
   int32_t dims;
   double* lpart;                // per block running loss sums

@@ -654,11 +654,11 @@ __global__ __launch_bounds__(256) void k_adjoint_step(AdjointArgs a) {
     }
     double direct[3] = {0.0, 0.0, 0.0};
     if (a.save >= 0) {
-      const int64_t t = a.tgt_off[ri.sim] + a.slot[i];
+      const double* tg = a.target + a.tgt_base[ri.sim] + a.slot[i];
       const double w = 2.0 * a.inv_n[ri.sim];
 #pragma unroll
       for (int c = 0; c < D; ++c)
-        direct[c] = w * (xi[c] - a.target[((int64_t)a.save * D + c) * a.tgt_n + t]);
+        direct[c] = w * (xi[c] - tg[((int64_t)a.save * D + c) * N]);
     }
 #pragma unroll
     for (int c = 0; c < D; ++c) a.lam_out[c][i] = li[c] + a.dt * v[c] + direct[c];
@@ -677,8 +677,9 @@ __global__ __launch_bounds__(256) void k_adjoint_seed(AdjointArgs a) {
   for (int c = 0; c < D; ++c) {
     double v = 0.0;
     if (a.save >= 0) {
-      const int64_t t = a.tgt_off[ri.sim] + a.slot[i];
-      v = 2.0 * a.inv_n[ri.sim] * (a.X[c][i] - a.target[((int64_t)a.save * D + c) * a.tgt_n + t]);
+      const int64_t N = a.g.st[ri.sim].N;
+      const double* tg = a.target + a.tgt_base[ri.sim] + a.slot[i];
+      v = 2.0 * a.inv_n[ri.sim] * (a.X[c][i] - tg[((int64_t)a.save * D + c) * N]);
     }
     a.lam_out[c][i] = v;
   }
@@ -690,9 +691,10 @@ __global__ __launch_bounds__(256) void k_loss(LossArgs a) {
   double e = 0.0;
   if (ri.local < a.g.st[ri.sim].N) {
     const int64_t i = ri.row;
-    const int64_t t = a.tgt_off[ri.sim] + a.slot[i];
+    const int64_t N = a.g.st[ri.sim].N;
+    const double* tg = a.target + a.tgt_base[ri.sim] + a.slot[i];
     for (int c = 0; c < a.dims; ++c) {
-      const double d = a.X[c][i] - a.target[((int64_t)a.save * a.dims + c) * a.tgt_n + t];
+      const double d = a.X[c][i] - tg[((int64_t)a.save * a.dims + c) * N];
       e += d * d;
     }
   }

@@ -233,7 +233,10 @@ int cbm_count_neighbors(cbm_ctx* ctx, int32_t sim, double radius, int64_t* out);
  * backward pass.  The Verlet lists of the forward pass are kept too while
  * they fit in half of the free device memory (or in CBM_ADJ_SNAPSHOT_MB MiB
  * when that environment variable is set); otherwise the lists are checked
- * and rebuilt along the backward pass, with identical results.
+ * and rebuilt along the backward pass, with identical results.  target
+ * NULL reuses the targets of the previous call (same n_saves and N).  Work
+ * buffers and list snapshots stay in the context for the next call and are
+ * released by cbm_destroy.
  * grad[3*s + {0,1,2}] = dL_s/d(k, r_eq, eps); loss[s] = L_s. */
 typedef struct cbm_loss_desc {
   int64_t n_steps;
@@ -241,7 +244,7 @@ typedef struct cbm_loss_desc {
   int32_t n_saves;
   int32_t pad;
   const int64_t* save_steps;   /* n_saves step indices in (0, n_steps]     */
-  const double* target;        /* see above                                */
+  const double* target;        /* see above; NULL: the previous call's     */
 } cbm_loss_desc;
 int cbm_adjoint(cbm_ctx* ctx, const cbm_loss_desc* loss_desc, double* grad,
                 double* loss);
