@@ -96,9 +96,97 @@ def cpu_baseline(m, s, pos, budget_s):
         el = time.perf_counter() - t0
         if el > budget_s or steps >= 50:
             break
-    return {"value": s.n * steps / el, "unit": "agent-steps/s", "cores": O.threads(), "kind": "port",
-            "sample": f"{steps} steps of the same N={s.n} workload after 2 warm-up steps "
-                      f"(oracle/cbm_oracle.cpp, OpenMP, FP64)"}
+    cores = O.threads()
+    rec = {"value": s.n * steps / el, "unit": "agent-steps/s", "cores": cores, "kind": "port",
+           "sample": f"{steps} steps of the same N={s.n} workload after 2 warm-up steps "
+                     f"(oracle/cbm_oracle.cpp, OpenMP, FP64)",
+           "cpu_model": cpu_model()}
+    # the same oracle on one thread, a few steps
+    O.set_threads(1)
+    try:
+        steps1, t0 = 0, time.perf_counter()
+        while True:
+            o.step(1)
+            steps1 += 1
+            el1 = time.perf_counter() - t0
+            if el1 > budget_s / 4 or steps1 >= 5:
+                break
+    finally:
+        O.set_threads(cores)
+    rec["single_thread"] = {"value": s.n * steps1 / el1, "cores": 1, "sample": f"{steps1} further steps"}
+    return rec
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def run_em(n, steps=200):
+    """The C2 Euler-Maruyama variant (sigma = sqrt(2 D), D = 0.01, Philox noise)."""
+    from cbm.api import Context
+    m, s, pos, L = workload("em", n)
+    ctx = Context(m, [s])
+    for a in range(3):
+        ctx.upload(a, pos[a])
+    ctx.neighbors()
+    ctx.step(20)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ms, nl = ctx.profile(20)
+    ctx.close()
+    return {"value": s.n * steps / wall, "unit": "agent-steps/s", "steps": steps,
+            "ms_per_step": wall / steps * 1e3, "stage_ms": ms[1],
+            "config": f"C2 EM variant, N={s.n}, sigma=sqrt(0.02), Verlet r_list 2.7, FP64"}
+
+
+def run_c1(steps=1000):
+    """C1, the reference's CPU-sized case: 2-D N=1000 periodic 50 x 50, linear
+    repulsion k=1 for d < 2r, Euler dt=0.01, cell-linked (edge 2), 1000 steps,
+    on the device and on the oracle (all host threads)."""
+    import cbm._abi as A
+    from cbm.api import Context
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    n, L = 1000, 50.0
+    m = A.ModelDesc()
+    m.dims, m.integrator, m.neighbors, m.law, m.periodic = 2, A.EULER, A.CELL_LINKED, A.LAW_LINEAR, 1
+    m.dt, m.cell_edge, m.max_neighbors = 0.01, 2.0, 32
+    s = A.SimDesc()
+    s.n = s.n_max = n
+    s.box_hi[0] = s.box_hi[1] = L
+    s.p[0], s.p[1] = 1.0, 2.0
+    rng = np.random.default_rng(1)
+    pos = [rng.uniform(0.0, L, n), rng.uniform(0.0, L, n)]
+    ctx = Context(m, [s])
+    o = O.Oracle(m, s)
+    for a in range(2):
+        ctx.upload(a, pos[a])
+        o.set(a, pos[a])
+    ctx.neighbors()
+    o.init()
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.sync()
+    tg = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.step(steps)
+    tc = time.perf_counter() - t0
+    same = all(np.array_equal(ctx.download(a, n), o.get(a)) for a in range(2))
+    ctx.close()
+    return {"gpu_value": n * steps / tg, "cpu_value": n * steps / tc, "unit": "agent-steps/s",
+            "cpu_cores": O.threads(), "steps": steps, "bit_identical": bool(same),
+            "config": "C1 2-D N=1000 periodic 50x50, linear k=1, Euler dt=0.01, CL edge 2, FP64"}
 
 
 def sweep_workload(n_sims=512, seed=4):
@@ -309,6 +397,8 @@ def main():
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--workload", choices=["heun", "em"], default="heun")
     ap.add_argument("--graph", type=int, default=0, help="capture this many steps per hipGraph replay")
+    ap.add_argument("--em", type=int, default=1, help="also time the C2 Euler-Maruyama variant")
+    ap.add_argument("--c1", type=int, default=1, help="also time C1 on the device and the oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-steps", type=int, default=50)
@@ -406,6 +496,12 @@ def main():
     def phase(name):
         print(f"bench: {name}", file=sys.stderr, flush=True)
 
+    if args.em:
+        phase("C2 Euler-Maruyama variant")
+        rec["em"] = run_em(args.n)
+    if args.c1:
+        phase("C1 (2-D, N=1000, 1000 steps) on the device and the oracle")
+        rec["c1"] = run_c1()
     if args.sweep_steps:
         phase("batched sweep (512 sims)")
         rec["batched"] = run_sweep(args.sweep_steps, plain_sims=16, plain_steps=100)

@@ -80,6 +80,8 @@ struct cbm_ctx {
 
   // neighbor engine
   int32_t* crank = nullptr;               // per-sim cell -> Morton rank tables
+  int32_t* glin = nullptr;                // bucket -> rank_off + cell, -1 for other buckets
+  CellRange* cell_range = nullptr;        // rank_off + cell -> its rows after a rebuild
   int32_t* lin = nullptr;
   int32_t *key = nullptr, *count = nullptr, *cursor = nullptr, *start = nullptr, *tmp = nullptr, *cell_row = nullptr;
   unsigned long long* scan_status = nullptr;   // one status word per scan tile
@@ -332,7 +334,7 @@ static void destroy_ctx(cbm_ctx* c) {
   F(c->gr_new); F(c->vb_new); F(c->dead);
   F(c->n_add_slot); F(c->dead_slot); F(c->off_add); F(c->off_dead); F(c->holes);
   F(c->med_c[0]); F(c->med_c[1]); F(c->med_count);
-  F(c->key); F(c->lin); F(c->crank); F(c->count); F(c->cursor); F(c->start); F(c->tmp); F(c->cell_row);
+  F(c->key); F(c->lin); F(c->crank); F(c->glin); F(c->cell_range); F(c->count); F(c->cursor); F(c->start); F(c->tmp); F(c->cell_row);
   F(c->scan_status); F(c->scan_ticket); F(c->list); F(c->list_n); F(c->flags); F(c->counters);
   for (cbm_ctx::DevBuf* b : {&c->adj_tgt, &c->adj_tbase, &c->adj_invn, &c->adj_ck, &c->adj_seg, &c->adj_lam,
                              &c->adj_gpart, &c->adj_lpart}) F(b->p);
@@ -440,6 +442,17 @@ static int init_ctx(cbm_ctx* c) {
   HIPCHK(c, dalloc(&c->lin, c->C));
   HIPCHK(c, dalloc(&c->crank, c->NR));
   HIPCHK(c, hipMemcpy(c->crank, c->h_crank.data(), sizeof(int32_t) * c->NR, hipMemcpyHostToDevice));
+  {
+    std::vector<int32_t> gl((size_t)c->NB, -1);
+    for (int s = 0; s < c->S; ++s) {
+      const SimDev& sd = c->h_sims[s];
+      for (int64_t l = 0; l < sd.ncells; ++l)
+        gl[(size_t)(sd.cell_off + c->h_crank[(size_t)(sd.rank_off + l)])] = (int32_t)(sd.rank_off + l);
+    }
+    HIPCHK(c, dalloc(&c->glin, c->NB));
+    HIPCHK(c, hipMemcpy(c->glin, gl.data(), sizeof(int32_t) * c->NB, hipMemcpyHostToDevice));
+    HIPCHK(c, dalloc(&c->cell_range, c->NR));
+  }
   c->h_crank.clear();
   c->h_crank.shrink_to_fit();
   HIPCHK(c, dalloc(&c->tmp, c->C));
@@ -500,6 +513,7 @@ static RebuildArgs rebuild_args(cbm_ctx* c, const int32_t* flag, int32_t* flag_c
   for (int d = 0; d < c->dims; ++d) a.pos[d] = (const double*)c->col[C_X + d].cur();
   a.dead = c->dead;
   a.slot = (const int32_t*)c->col[C_SLOT].cur();
+  a.range = c->cell_range;
   a.key = c->key; a.lin = c->lin; a.crank = c->crank; a.count = c->count; a.cursor = c->cursor; a.start = c->start; a.tmp = c->tmp;
   a.cell_row = c->cell_row;
   a.dims = c->dims; a.periodic = c->m.periodic; a.full = c->full ? 1 : 0;
@@ -513,7 +527,8 @@ static RebuildArgs rebuild_args(cbm_ctx* c, const int32_t* flag, int32_t* flag_c
 // current buffers); otherwise the alternate buffers become current.
 // Exclusive scan of n int32 counts (see k_scan); refuses a scan with more
 // tiles than the status array holds.
-static int run_scan(cbm_ctx* c, const int32_t* flag, int32_t* count, int32_t* cursor, int32_t* start, int64_t n) {
+static int run_scan(cbm_ctx* c, const int32_t* flag, int32_t* count, int32_t* cursor, int32_t* start, int64_t n,
+                    bool cells = false) {
   if (scan_blocks(n) > c->scan_tiles)
     return fail(c, CBM_ERR_ARG, "scan longer than its status array");
   ScanArgs s;
@@ -521,6 +536,7 @@ static int run_scan(cbm_ctx* c, const int32_t* flag, int32_t* count, int32_t* cu
   s.flag = flag; s.count = count; s.cursor = cursor; s.start = start; s.n = n;
   s.status = c->scan_status; s.ticket = c->scan_ticket; s.done = c->scan_ticket + 1;
   s.epoch = c->scan_ticket + 2; s.nblocks = scan_blocks(n);
+  if (cells) { s.glin = c->glin; s.range = c->cell_range; }
   HIPCHK(c, launch_scan(c->stream, s));
   return CBM_OK;
 }
@@ -528,7 +544,7 @@ static int run_scan(cbm_ctx* c, const int32_t* flag, int32_t* count, int32_t* cu
 static int rebuild_chain(cbm_ctx* c, const int32_t* flag, int32_t* flag_clear, bool tick, bool copy_back) {
   RebuildArgs a = rebuild_args(c, flag, flag_clear, tick);
   HIPCHK(c, launch_keys(c->nblocks, c->block, c->stream, a));
-  int rc = run_scan(c, flag, c->count, c->cursor, c->start, c->NB);
+  int rc = run_scan(c, flag, c->count, c->cursor, c->start, c->NB, true);
   if (rc) return rc;
   HIPCHK(c, launch_scatter(c->nblocks, c->block, c->stream, a));
   PermuteArgs pa;
@@ -578,7 +594,7 @@ static StageArgs stage_base(cbm_ctx* c) {
   for (int d = 0; d < c->dims; ++d) a.xref[d] = c->xref[d];
   a.id = (const int64_t*)c->col[C_ID].cur();
   a.list = c->list; a.list_n = c->list_n; a.ldl = c->C;
-  a.cell_start = c->start; a.cell_row = c->cell_row; a.crank = c->crank;
+  a.cell_range = c->cell_range; a.cell_row = c->cell_row;
   a.dt = c->m.dt; a.sqdt = std::sqrt(c->m.dt); a.drift = c->m.drift; a.noise = c->m.noise; a.thr2 = sq_threshold(c->thr);
   a.seed = c->m.seed;
   a.step_ptr = c->counters + 1;
@@ -1133,7 +1149,7 @@ static int rebuild_in_place(cbm_ctx* c, double* const* X, const int32_t* flag,
   for (int d = 0; d < c->dims; ++d) a.pos[d] = X[d];
   a.dead = nullptr;
   HIPCHK(c, launch_keys(c->nblocks, c->block, c->stream, a));
-  int rc = run_scan(c, flag, c->count, c->cursor, c->start, c->NB);
+  int rc = run_scan(c, flag, c->count, c->cursor, c->start, c->NB, true);
   if (rc) return rc;
   HIPCHK(c, launch_scatter(c->nblocks, c->block, c->stream, a));
   VerletArgs va;

@@ -37,6 +37,7 @@ struct RebuildArgs {
   int32_t* start;             // per-bucket first row (exclusive scan), nb+1 entries
   int32_t* tmp;               // bucket-ordered source rows
   int32_t* cell_row;          // destination row -> linear local cell
+  const CellRange* range;     // rank_off + linear local cell -> rows, written by the scan
   int32_t dims, periodic, full, geometric;
 };
 
@@ -50,6 +51,8 @@ struct ScanArgs {
   unsigned int* ticket;
   unsigned int* done;
   unsigned int* epoch;        // device launch epoch (>= 1)
+  const int32_t* glin;        // bucket -> rank_off + linear cell, -1 for non-cell buckets
+  CellRange* range;           // written for cell buckets when not NULL
   int32_t nblocks;
 };
 
@@ -77,7 +80,7 @@ struct VerletArgs {
 };
 
 // LDS budget of the list build: lds_r * block * 8 bytes.
-constexpr int VERLET_LDS_BYTES = 32768;
+constexpr int VERLET_LDS_BYTES = 24576;
 
 struct XferArgs {
   RowGeom g;

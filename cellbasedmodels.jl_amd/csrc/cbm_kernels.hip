@@ -173,6 +173,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
     const int64_t idx = base + k;
     if (idx < a.n) {
       a.start[idx] = run;
+      if (a.range) {
+        const int32_t g = a.glin[idx];
+        if (g >= 0) a.range[g] = CellRange{run, run + v[k]};
+      }
       a.count[idx] = 0;
       if (a.cursor) a.cursor[idx] = 0;
       run += v[k];
@@ -290,14 +294,15 @@ __device__ __forceinline__ int32_t verlet_row(const VerletArgs& va, const SimDev
   for (int c = 0; c < D; ++c) xi[c] = va.apos[c][i];
   const int32_t c = va.nopermute ? a.lin[i] : a.cell_row[i];
   const int32_t cc[3] = {c % sd.gn[0], (c / sd.gn[0]) % sd.gn[1], c / (sd.gn[0] * sd.gn[1])};
-  const int32_t* __restrict__ rank = a.crank + sd.rank_off;
+  const CellRange* __restrict__ cr = a.range + sd.rank_off;
   const int R = va.lds_r;
   int32_t cnt = 0;
   // Columns of three x-adjacent cells; the candidates of a column are walked
-  // as one flattened range so that lanes of a wavefront stay in step.
+  // as one flattened range so that lanes of a wavefront stay in step.  The
+  // cell ranges of the next column are loaded before the current column's
+  // candidates are walked.
   constexpr int NCOL = D == 1 ? 1 : (D == 2 ? 3 : 9);
-#pragma unroll
-  for (int col = 0; col < NCOL; ++col) {
+  auto col_ranges = [&](int col, CellRange* r3) {
     int32_t yz = 0;
     bool ok = true;
     if (D > 1) {
@@ -311,24 +316,27 @@ __device__ __forceinline__ int32_t verlet_row(const VerletArgs& va, const SimDev
       }
       yz = sd.gn[0] * (q[0] + (D > 2 ? sd.gn[1] * q[1] : 0));
     }
-    if (!ok) continue;
-    int32_t s3[3], n3[3];
 #pragma unroll
     for (int dxc = 0; dxc < 3; ++dxc) {
       int32_t v = cc[0] + dxc - 1;
-      bool okx = true;
+      bool okx = ok;
       if (a.periodic) v = v < 0 ? v + sd.gn[0] : (v >= sd.gn[0] ? v - sd.gn[0] : v);
-      else okx = v >= 0 && v < sd.gn[0];
-      s3[dxc] = 0; n3[dxc] = 0;
-      if (okx) {
-        const int64_t gc = sd.cell_off + rank[yz + v];
-        s3[dxc] = a.start[gc];
-        n3[dxc] = a.start[gc + 1] - s3[dxc];
-      }
+      else okx = okx && v >= 0 && v < sd.gn[0];
+      r3[dxc] = CellRange{0, 0};
+      if (okx) r3[dxc] = cr[yz + v];
     }
+  };
+  CellRange nxt[3];
+  col_ranges(0, nxt);
+#pragma unroll
+  for (int col = 0; col < NCOL; ++col) {
+    int32_t s3[3], n3[3];
+#pragma unroll
+    for (int dxc = 0; dxc < 3; ++dxc) { s3[dxc] = nxt[dxc].s; n3[dxc] = nxt[dxc].e - nxt[dxc].s; }
+    if (col + 1 < NCOL) col_ranges(col + 1, nxt);
     // candidates in chunks of VC whose loads are issued together; slots past
     // the end load nothing and are skipped
-    constexpr int VC = 4;
+    constexpr int VC = 2;
     const int32_t n01 = n3[0] + n3[1], tot = n01 + n3[2];
     for (int32_t t0 = 0; t0 < tot; t0 += VC) {
       int32_t jj[VC];

@@ -71,13 +71,12 @@ __device__ __forceinline__ void for_candidates(const StageArgs& a, const SimDev&
     const int32_t cx = c % sd.gn[0];
     const int32_t cy = (c / sd.gn[0]) % sd.gn[1];
     const int32_t cz = c / (sd.gn[0] * sd.gn[1]);
-    const int32_t* __restrict__ rank = a.crank + sd.rank_off;
+    const CellRange* __restrict__ cr = a.cell_range + sd.rank_off;
     for (int k = 0; k < pow3(D); ++k) {
       const int32_t nc = neigh_cell<D>(sd, cx, cy, cz, k, a.periodic);
       if (nc < 0) continue;
-      const int64_t gc = sd.cell_off + rank[nc];
-      const int32_t j0 = a.cell_start[gc], j1 = a.cell_start[gc + 1];
-      for (int32_t j = j0; j < j1; ++j) if (j != i) fn((int64_t)j);
+      const CellRange r = cr[nc];
+      for (int32_t j = r.s; j < r.e; ++j) if (j != i) fn((int64_t)j);
     }
   }
 }

@@ -70,6 +70,9 @@ enum StageKind {
 
 enum NeighborKind { NB_FULL = 0, NB_CELL = 1, NB_LIST = 2 };
 
+// Rows [s, e) of one grid cell after a rebuild, indexed by linear cell.
+struct alignas(8) CellRange { int32_t s, e; };
+
 struct StageArgs {
   RowGeom g;
   const double* S[MAX_STATE];   // stage state read by the force (neighbors + self)
@@ -84,9 +87,8 @@ struct StageArgs {
   const int32_t* list;          // ELL Verlet list, entry q of row i at q*ldl + i
   const int32_t* list_n;
   int64_t ldl;
-  const int32_t* cell_start;    // bucket -> first global row
+  const CellRange* cell_range;  // rank_off + linear local cell -> its rows (cell-linked)
   const int32_t* cell_row;      // row -> linear local cell (cell-linked)
-  const int32_t* crank;         // linear local cell -> bucket rank (Morton order)
   int32_t* flag;                // rebuild flag written by the final stage
   double dt, sqdt, drift, noise;
   double thr2;                  // largest d2 with sqrt(d2) < rebuild displacement

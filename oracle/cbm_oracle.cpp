@@ -851,6 +851,15 @@ int orc_threads(void) {
 #endif
 }
 
+// OpenMP threads of the oracle's parallel loops (the CPU baseline at 1 thread).
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 // loss with double arithmetic; theta = (k, r_eq, eps).  Restores positions.
 double orc_traj_loss(void* h, const double* theta, int64_t n_steps, int32_t n_saves,
                      const int64_t* save_steps, const double* target) {

@@ -47,6 +47,7 @@ def lib():
             "orc_normals": (None, [C.c_uint64, C.c_int64, C.c_int64, C.c_int32, C.c_uint32, P]),
             "orc_floor_div": (C.c_int64, [C.c_double, C.c_double]),
             "orc_threads": (C.c_int, []),
+            "orc_set_threads": (None, [C.c_int]),
             "orc_traj_loss": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P]),
             "orc_traj_grad_dual": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P, P]),
         }
@@ -170,3 +171,7 @@ def floor_div(a, b):
 
 def threads():
     return lib().orc_threads()
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
