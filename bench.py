@@ -184,8 +184,21 @@ def run_c1(steps=1000):
     tc = time.perf_counter() - t0
     same = all(np.array_equal(ctx.download(a, n), o.get(a)) for a in range(2))
     ctx.close()
-    return {"gpu_value": n * steps / tg, "cpu_value": n * steps / tc, "unit": "agent-steps/s",
-            "cpu_cores": O.threads(), "steps": steps, "bit_identical": bool(same),
+    # the same run with the step loop captured in a hipGraph (100 steps, replayed)
+    gg = Context(m, [s])
+    for a in range(2):
+        gg.upload(a, pos[a])
+    gg.neighbors()
+    gg.sync()
+    t0 = time.perf_counter()
+    gg.step_graph(100, steps // 100)
+    gg.sync()
+    tgr = time.perf_counter() - t0
+    same_g = all(np.array_equal(gg.download(a, n), o.get(a)) for a in range(2))
+    gg.close()
+    return {"gpu_value": n * steps / tg, "gpu_graph_value": n * steps / tgr, "cpu_value": n * steps / tc,
+            "unit": "agent-steps/s", "cpu_cores": O.threads(), "steps": steps,
+            "bit_identical": bool(same), "graph_bit_identical": bool(same_g),
             "config": "C1 2-D N=1000 periodic 50x50, linear k=1, Euler dt=0.01, CL edge 2, FP64"}
 
 

@@ -16,7 +16,7 @@ ERRORS = {
     ERR_COMPILE: "CBM_ERR_COMPILE", ERR_HIP: "CBM_ERR_HIP", ERR_ARG: "CBM_ERR_ARG", ERR_NONFINITE: "CBM_ERR_NONFINITE",
 }
 
-EULER, HEUN, RK4, EM = 0, 1, 2, 3
+EULER, HEUN, RK4, EM, EULER_HEUN = 0, 1, 2, 3, 4
 FULL, CELL_LINKED, VERLET_DISPLACEMENT, VERLET_TIME, CLVD = 0, 1, 2, 3, 4
 LAW_NONE, LAW_LINEAR, LAW_REP_ADH, LAW_AGGREGATION, LAW_CUSTOM = 0, 1, 2, 3, 4
 RULE_NONE, RULE_GROWTH_DIVISION, RULE_PARITY, RULE_DEATH = 0, 1, 2, 3

@@ -95,7 +95,8 @@ class CBMNeighbors:
 
 # --------------------------------------------------------------- integrators
 class CBMIntegrators:
-    """src/integrators.jl:465 (Euler), :516 (Heun), :583 (RungeKutta4), :683 (EM)."""
+    """src/integrators.jl:465 (Euler), :516 (Heun), :583 (RungeKutta4), :683 (EM),
+    :744 (EulerHeun)."""
 
     class Euler:
         kind = A.EULER
@@ -108,6 +109,9 @@ class CBMIntegrators:
 
     class EM:
         kind = A.EM
+
+    class EulerHeun:
+        kind = A.EULER_HEUN
 
 
 # -------------------------------------------------------------------- models
@@ -174,7 +178,7 @@ class ABM:
     """Model description: ABM(dims; agentODE, agentSDE, agentRule, neighborsAlg, agentAlg, platform).
 
     agentODE    a CBMModels law (the interaction body); `drift` adds dt(x) = -drift*x
-    agentSDE    diagonal noise amplitude g (EM only): dt(x) = g in the SDE part
+    agentSDE    diagonal noise amplitude g (EM and EulerHeun): dt(x) = g in the SDE part
     lagged      dt(x) reads the previous step's committed force accumulator, the
                 reference's read-old/write-new idiom (agentStructure.jl:352-453)
     periodic    periodic box with minimum-image distances
@@ -372,9 +376,9 @@ class Context:
 
     def profile(self, nsteps):
         """Per-kernel average device times (ms) over nsteps steps, see cbm_profile."""
-        ms = np.zeros(10)
-        nl = np.zeros(10, dtype=np.int64)
-        self._chk(self.lib.cbm_profile(self.h, int(nsteps), ms.ctypes.data, nl.ctypes.data, 10))
+        ms = np.zeros(12)
+        nl = np.zeros(12, dtype=np.int64)
+        self._chk(self.lib.cbm_profile(self.h, int(nsteps), ms.ctypes.data, nl.ctypes.data, 12))
         return ms, nl
 
     def set_block(self, block):

@@ -178,7 +178,7 @@ static void prof_end(cbm_ctx* c) {
 static int validate(cbm_ctx* c) {
   const cbm_model_desc& m = c->m;
   if (m.dims < 1 || m.dims > 3) return fail(c, CBM_ERR_CONFIG, "dims must be 1, 2 or 3");
-  if (m.integrator < CBM_EULER || m.integrator > CBM_EM) return fail(c, CBM_ERR_CONFIG, "unknown integrator");
+  if (m.integrator < CBM_EULER || m.integrator > CBM_EULER_HEUN) return fail(c, CBM_ERR_CONFIG, "unknown integrator");
   if (m.neighbors < CBM_FULL || m.neighbors > CBM_CLVD) return fail(c, CBM_ERR_CONFIG, "unknown neighbor algorithm");
   if (m.law < CBM_LAW_NONE || m.law > CBM_LAW_CUSTOM) return fail(c, CBM_ERR_CONFIG, "unknown interaction law");
   if (!(m.dt > 0)) return fail(c, CBM_ERR_CONFIG, "dt must be positive");
@@ -431,10 +431,10 @@ static int init_ctx(cbm_ctx* c) {
     }
   }
   // stage buffers
-  int nst = (m.integrator == CBM_HEUN) ? 1 : (m.integrator == CBM_RK4 ? 2 : 0);
+  int nst = (m.integrator == CBM_HEUN || m.integrator == CBM_EULER_HEUN) ? 1 : (m.integrator == CBM_RK4 ? 2 : 0);
   for (int b = 0; b < nst; ++b)
     for (int v = 0; v < c->nvar; ++v) HIPCHK(c, dalloc(&c->h[b][v], c->C));
-  if (m.integrator == CBM_RK4)
+  if (m.integrator == CBM_RK4 || m.integrator == CBM_EULER_HEUN)
     for (int v = 0; v < c->nvar; ++v) HIPCHK(c, dalloc(&c->kacc[v], c->C));
 
   // neighbor engine
@@ -610,7 +610,9 @@ static StageArgs stage_base(cbm_ctx* c) {
 // Optional per-launch event pairs (cbm_profile): category = stage kind, or
 // PROF_CHAIN for the neighbor update of one step.
 constexpr int PROF_CHAIN = 8;
-constexpr int PROF_NCAT = 10;
+constexpr int PROF_NCAT = 12;
+// profile category of a stage kind: 0..7 as the kind, EulerHeun stages 10, 11
+static int prof_cat_of(int kind) { return kind == ST_EH1 ? 10 : (kind == ST_EH2 ? 11 : kind); }
 
 static void prof_begin(cbm_ctx* c, int cat);
 static void prof_end(cbm_ctx* c);
@@ -620,7 +622,7 @@ static void prof_end(cbm_ctx* c);
 static hipError_t stage_launch(cbm_ctx* c, const StageArgs& a) {
   if (c->m.law != CBM_LAW_CUSTOM)
     return launch_stage(c->dims, c->nb_kind, c->m.law, c->nblocks, c->block, c->stream, a);
-  hipFunction_t f = c->user_fn[a.kind == ST_EM ? 1 : 0];
+  hipFunction_t f = c->user_fn[(a.kind == ST_EM || a.kind == ST_EH1 || a.kind == ST_EH2) ? 1 : 0];
   if (!f) return hipErrorInvalidDeviceFunction;
   StageArgs copy = a;
   void* params[] = {(void*)&copy};
@@ -629,7 +631,7 @@ static hipError_t stage_launch(cbm_ctx* c, const StageArgs& a) {
 
 static int run_stage(cbm_ctx* c, StageArgs& a, int kind) {
   a.kind = kind;
-  prof_begin(c, kind);
+  prof_begin(c, prof_cat_of(kind));
   HIPCHK(c, stage_launch(c, a));
   prof_end(c);
   c->last_launches++;
@@ -753,6 +755,14 @@ static int step_once(cbm_ctx* c) {
       set_state(c->h[0], cur, cur);
       a.disp_check = disp ? 1 : 0;
       if ((rc = run_stage(c, a, ST_HEUN2))) return rc;
+      break;
+    case CBM_EULER_HEUN:
+      set_state(cur, cur, c->h[0]);
+      a.disp_check = 0;
+      if ((rc = run_stage(c, a, ST_EH1))) return rc;
+      set_state(c->h[0], cur, cur);
+      a.disp_check = disp ? 1 : 0;
+      if ((rc = run_stage(c, a, ST_EH2))) return rc;
       break;
     default:  // RK4
       a.disp_check = 0;

@@ -881,7 +881,7 @@ static hipError_t launch_stage_dims(int nb, int law, int periodic, dim3 grid, di
 
 hipError_t launch_stage(int dims, int nb, int law, int nblocks, int block, hipStream_t s, const StageArgs& a) {
   dim3 grid(nblocks), blk(block);
-  const bool noise = a.kind == ST_EM;
+  const bool noise = a.kind == ST_EM || a.kind == ST_EH1 || a.kind == ST_EH2;
   switch (dims) {
     case 1: return noise ? launch_stage_dims<1, 1>(nb, law, a.periodic, grid, blk, s, a) : launch_stage_dims<1, 0>(nb, law, a.periodic, grid, blk, s, a);
     case 2: return noise ? launch_stage_dims<2, 1>(nb, law, a.periodic, grid, blk, s, a) : launch_stage_dims<2, 0>(nb, law, a.periodic, grid, blk, s, a);

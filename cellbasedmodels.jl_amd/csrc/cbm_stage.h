@@ -197,15 +197,27 @@ __device__ __forceinline__ void stage_body(const StageArgs& a) {
   const double dt = a.dt;
   const int nv = grow ? D + 1 : D;
   double res[MAX_STATE];
-  if (NOISE) {                                // EM: (u + f dt) + (g xi) sqrt(dt)
+  if (NOISE) {
+    // EM and the EulerHeun predictor: (u + f dt) + (g xi) sqrt(dt), xi keyed
+    // 0x100/0x101; the EulerHeun corrector draws fresh noise (0x500/0x501)
     double z[4] = {0.0, 0.0, 0.0, 0.0};
     const int64_t id = a.id[i];
     const int64_t step = *a.step_ptr;
-    normal2(draw(a.seed, step, id, ri.sim, 0x100u), z[0], z[1]);
-    if (D > 2) normal2(draw(a.seed, step, id, ri.sim, 0x101u), z[2], z[3]);
+    const uint32_t pur = a.kind == ST_EH2 ? 0x500u : 0x100u;
+    normal2(draw(a.seed, step, id, ri.sim, pur), z[0], z[1]);
+    if (D > 2) normal2(draw(a.seed, step, id, ri.sim, pur + 1u), z[2], z[3]);
+    if (a.kind == ST_EH2) {
 #pragma unroll
-    for (int c = 0; c < D; ++c) res[c] = (a.U[c][i] + du[c] * dt) + a.noise * z[c] * a.sqdt;
-    if (grow) res[D] = a.U[D][i] + du[D] * dt;
+      for (int c = 0; c < D; ++c)
+        res[c] = a.U[c][i] + ((a.kacc[c][i] + du[c]) * dt + (a.noise + a.noise) * z[c] * a.sqdt) / 2;
+      if (grow) res[D] = a.U[D][i] + ((a.kacc[D][i] + du[D]) * dt) / 2;
+    } else {
+#pragma unroll
+      for (int c = 0; c < D; ++c) res[c] = (a.U[c][i] + du[c] * dt) + a.noise * z[c] * a.sqdt;
+      if (grow) res[D] = a.U[D][i] + du[D] * dt;
+      if (a.kind == ST_EH1)
+        for (int c = 0; c < nv; ++c) a.kacc[c][i] = du[c];
+    }
   } else {
     switch (a.kind) {
       case ST_EULER:
@@ -232,7 +244,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a) {
     }
   }
 
-  const bool final_stage = NOISE || (a.kind == ST_EULER || a.kind == ST_HEUN2 || a.kind == ST_RK4_4);
+  const bool final_stage = a.kind == ST_EULER || a.kind == ST_EM || a.kind == ST_HEUN2 ||
+                           a.kind == ST_RK4_4 || a.kind == ST_EH2;
   if (final_stage && a.periodic) {
 #pragma unroll
     for (int c = 0; c < D; ++c) {

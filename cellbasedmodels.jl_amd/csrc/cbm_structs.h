@@ -65,7 +65,9 @@ enum StageKind {
   ST_RK4_1 = 4,     // k = k1;        h = u + k1*dt/2         (:620-624)
   ST_RK4_2 = 5,     // k = k + 2*k2;  h = u + k2*dt/2
   ST_RK4_3 = 6,     // k = k + 2*k3;  h = u + k3*dt
-  ST_RK4_4 = 7      // u' = u + ((k + k4)*dt)/6                (:659-661)
+  ST_RK4_4 = 7,     // u' = u + ((k + k4)*dt)/6                (:659-661)
+  ST_EH1 = 8,       // k = f;  h = (u + f*dt) + (g*xi1)*sqrt(dt)   EulerHeun (:787-789)
+  ST_EH2 = 9        // u' = u + ((k + f)*dt + ((g + g)*xi2)*sqrt(dt))/2  (:801-803)
 };
 
 enum NeighborKind { NB_FULL = 0, NB_CELL = 1, NB_LIST = 2 };

@@ -42,8 +42,9 @@ enum cbm_status {
 };
 
 /* ---------------------------------------------------------- integrators */
-/* src/integrators.jl:465 (Euler), :516 (Heun), :583 (RungeKutta4), :683 (EM) */
-enum cbm_integrator { CBM_EULER = 0, CBM_HEUN = 1, CBM_RK4 = 2, CBM_EM = 3 };
+/* src/integrators.jl:465 (Euler), :516 (Heun), :583 (RungeKutta4), :683 (EM),
+ * :744 (EulerHeun: the CPU branch, whose corrector draws fresh noise) */
+enum cbm_integrator { CBM_EULER = 0, CBM_HEUN = 1, CBM_RK4 = 2, CBM_EM = 3, CBM_EULER_HEUN = 4 };
 
 /* ------------------------------------------------------ neighbor engines */
 /* src/neighbors.jl:27 (Full), :196 (VerletTime), :310 (VerletDisplacement),
@@ -267,7 +268,8 @@ int cbm_set_block(cbm_ctx* ctx, int32_t block);
  * avg_ms[k] / launches[k] for k = 0..7: the integrator stage kernels
  * (0 Euler, 1 EM, 2 Heun stage 1, 3 Heun stage 2, 4-7 RK4 stages 1-4);
  * k = 8: the neighbor update of one step (total, launches = steps);
- * k = 9: whole step.  ncat must be >= 10. */
+ * k = 9: whole step; k = 10, 11: EulerHeun stages 1 and 2.  ncat must be
+ * >= 12. */
 int cbm_profile(cbm_ctx* ctx, int64_t nsteps, double* avg_ms, int64_t* launches, int32_t ncat);
 /* Average device time (ms) of the last cbm_step call's force kernels, from
  * HIP events on the context stream; *launches = force-kernel launches timed. */

@@ -589,6 +589,31 @@ static void de_step(Orc& o, int64_t nA) {
     for (int v = 0; v < nv; ++v)
       for (int64_t i = 0; i < nA; ++i)
         U[v][i] = U[v][i] + (K1[v][i] + 2 * K2[v][i] + 2 * K3[v][i] + K4[v][i]) * dt / 6;
+  } else if (integ == CBM_EULER_HEUN) {
+    // src/integrators.jl:775-807 (CPU branch): predictor with noise z1, then
+    // the average of both stages with a fresh draw z2 (reference defect D8)
+    eval(U, K1);
+    const double sq = std::sqrt(dt);
+    const double gsig = o.m.noise;
+    for (int64_t i = 0; i < nA; ++i) {
+      double z[4];
+      U4 r0 = draw(o.m.seed, o.step, o.id[i], 0, 0x100u);
+      normal2(r0, z[0], z[1]);
+      if (d > 2) { U4 r1 = draw(o.m.seed, o.step, o.id[i], 0, 0x101u); normal2(r1, z[2], z[3]); }
+      for (int a = 0; a < d; ++a) H[a][i] = (U[a][i] + K1[a][i] * dt) + gsig * z[a] * sq;
+      if (o.growth) H[d][i] = U[d][i] + K1[d][i] * dt;
+    }
+    for (int v = 0; v < nv; ++v) K2[v].assign(nA, 0);
+    eval(H, K2);
+    for (int64_t i = 0; i < nA; ++i) {
+      double z[4];
+      U4 r0 = draw(o.m.seed, o.step, o.id[i], 0, 0x500u);
+      normal2(r0, z[0], z[1]);
+      if (d > 2) { U4 r1 = draw(o.m.seed, o.step, o.id[i], 0, 0x501u); normal2(r1, z[2], z[3]); }
+      for (int a = 0; a < d; ++a)
+        U[a][i] = U[a][i] + ((K1[a][i] + K2[a][i]) * dt + (gsig + gsig) * z[a] * sq) / 2;
+      if (o.growth) U[d][i] = U[d][i] + ((K1[d][i] + K2[d][i]) * dt) / 2;
+    }
   } else {  // EM
     eval(U, K1);
     double sq = std::sqrt(dt);

@@ -99,9 +99,10 @@ def test_ode_convergence_kat_on_device(gpu_available, integ):
         assert np.array_equal(x, o.get(A.F_X))
 
 
-def test_sde_variance_kat_on_device(gpu_available):
+@pytest.mark.parametrize("integ", [A.EM, A.EULER_HEUN])
+def test_sde_variance_kat_on_device(gpu_available, integ):
     g = golden("ref_sde_variance.json")
-    m = model(3, integrator=A.EM, neighbors=A.FULL, dt=g["dt"], noise=g["g"], seed=5)
+    m = model(3, integrator=integ, neighbors=A.FULL, dt=g["dt"], noise=g["g"], seed=5)
     s = sim(g["N"], g["simBox"])
     o, ctx = pair(m, s, [np.zeros(g["N"])] * 3)
     for k in range(1, g["steps"] + 1):
@@ -110,7 +111,7 @@ def test_sde_variance_kat_on_device(gpu_available):
         assert abs(0.1 * k - x.var()) < g["tol"]
     o.step(g["steps"])
     # same Philox stream as the oracle: Box-Muller transcendentals may differ by an ulp
-    assert_close(o, ctx, 3, 1e-12, "EM")
+    assert_close(o, ctx, 3, 1e-12, "SDE")
 
 
 # --------------------------------------------------------------- C1 parity
@@ -188,6 +189,18 @@ def test_c2_em_variant_parity(gpu_available):
     o.step(49)
     g.step(49)
     assert_close(o, g, 3, LONG_TOL, "EM step 50")
+
+
+def test_c2_euler_heun_parity(gpu_available):
+    m, s, pos = c2(n=2048, integ=A.EULER_HEUN, noise=np.sqrt(2 * 0.01))
+    o, g = pair(m, s, pos)
+    o.step(1)
+    g.step(1)
+    assert_close(o, g, 3, STEP_TOL, "EulerHeun step 1")
+    o.step(49)
+    g.step(49)
+    assert_close(o, g, 3, LONG_TOL, "EulerHeun step 50")
+    assert g.rebuilds() == o.rebuilds
 
 
 def test_rk4_cell_linked_3d_parity(gpu_available):

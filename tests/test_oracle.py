@@ -192,9 +192,11 @@ def test_linear_ode_amplification_and_kat(integ, factor):
     assert np.allclose(o.get(A.F_X), factor ** g["steps"], rtol=1e-12)
 
 
-def test_sde_variance_kat():
+@pytest.mark.parametrize("integ", [A.EM, A.EULER_HEUN])
+def test_sde_variance_kat(integ):
+    """test/testCommunity.jl:469-512 runs the same check for EM and EulerHeun."""
     g = golden("ref_sde_variance.json")
-    m = model(3, integrator=A.EM, neighbors=A.FULL, dt=g["dt"], noise=g["g"], seed=5)
+    m = model(3, integrator=integ, neighbors=A.FULL, dt=g["dt"], noise=g["g"], seed=5)
     o = O.Oracle(m, sim(g["N"], g["simBox"]))
     for a in range(3):
         o.set(a, np.zeros(g["N"]))
@@ -208,6 +210,34 @@ def test_sde_variance_kat():
     for a in range(3):
         v = o.get(a).var()
         assert abs(v - t) < 5 * t * np.sqrt(2.0 / g["N"])
+
+
+def test_euler_heun_deterministic_part_is_second_order():
+    """Without noise the EulerHeun step of dx = -a x is the trapezoidal
+    predictor-corrector: x1 = x0 (1 - a dt + (a dt)^2 / 2)."""
+    a, dt, x0 = 0.7, 0.1, 1.5
+    m = model(1, integrator=A.EULER_HEUN, neighbors=A.FULL, dt=dt, drift=a, seed=3)
+    o = O.Oracle(m, sim(4, [[-10.0, 10.0]]))
+    o.set(A.F_X, np.full(4, x0))
+    o.init()
+    o.step(1)
+    want = x0 * (1 - a * dt + (a * dt) ** 2 / 2)
+    assert np.allclose(o.get(A.F_X), want, rtol=1e-15, atol=0)
+
+
+def test_euler_heun_corrector_draws_fresh_noise():
+    """Reference defect D8 (src/integrators.jl:798-807): the corrector uses a
+    new normal draw, not the predictor's.  For dx = -a x dt + g dW from x = 0
+    one step has Var = g^2 dt (1 + (a dt)^2 / 4); reusing the predictor's
+    draw would give g^2 dt (1 - a dt / 2)^2.  With a dt = 1: 1.25 vs 0.25."""
+    n, dt, a, g = 20000, 0.5, 2.0, 1.0
+    m = model(1, integrator=A.EULER_HEUN, neighbors=A.FULL, dt=dt, drift=a, noise=g, seed=11)
+    o = O.Oracle(m, sim(n, [[-100.0, 100.0]]))
+    o.set(A.F_X, np.zeros(n))
+    o.init()
+    o.step(1)
+    v = o.get(A.F_X).var() / (g * g * dt)
+    assert abs(v - 1.25) < 5 * 1.25 * np.sqrt(2.0 / n), v
 
 
 # --------------------------------------------------------- force invariants
