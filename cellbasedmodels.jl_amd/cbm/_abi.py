@@ -18,11 +18,12 @@ ERRORS = {
 
 EULER, HEUN, RK4, EM, EULER_HEUN = 0, 1, 2, 3, 4
 FULL, CELL_LINKED, VERLET_DISPLACEMENT, VERLET_TIME, CLVD = 0, 1, 2, 3, 4
-LAW_NONE, LAW_LINEAR, LAW_REP_ADH, LAW_AGGREGATION, LAW_CUSTOM = 0, 1, 2, 3, 4
+LAW_NONE, LAW_LINEAR, LAW_REP_ADH, LAW_AGGREGATION, LAW_CUSTOM, LAW_SOFT_SPHERES = 0, 1, 2, 3, 4, 5
 RULE_NONE, RULE_GROWTH_DIVISION, RULE_PARITY, RULE_DEATH = 0, 1, 2, 3
 
 F_X, F_Y, F_Z, F_ID, F_FX, F_FY, F_FZ, F_VOL, F_GROWTH, F_VBIRTH, F_LF = range(11)
-INT_FIELDS = (F_ID, F_LF)
+F_VX, F_VY, F_VZ, F_RADIUS, F_MASS, F_FATE = range(11, 17)
+INT_FIELDS = (F_ID, F_LF, F_FATE)
 
 
 class ModelDesc(C.Structure):

@@ -152,6 +152,16 @@ class CBMModels:
         return law
 
     @staticmethod
+    def softSpheres(b, mu, f0):
+        """CBMModels.softSpheres2D/3D (src/models/softSpheres.jl:1-81): inertial
+        dynamics dt(v) = -b v/m + F/m, dt(x) = v with the pair force
+        f0[cellFate_i][cellFate_j] (r_ij/d - 1)(mu r_ij/d - 1)(x_i - x_j)/d for
+        d < mu r_ij (r_ij = r_i + r_j); f0 is 2 x 2 (cellFate 1 or 2).
+        Per-agent fields: vx, vy, vz, r, m, cellFate."""
+        f = np.asarray(f0, dtype=np.float64).reshape(2, 2)
+        return _Law(A.LAW_SOFT_SPHERES, [b, mu, f[0, 0], f[0, 1], f[1, 0], f[1, 1]])
+
+    @staticmethod
     def aggregation(rRep, fRep, rAtr, fAtr, walls=True):
         """The Aggregation model of docs/src/Aggregation.md:62-123 (with its wall push)."""
         return _Law(A.LAW_AGGREGATION, [rRep, fRep, rAtr, fAtr, 1.0 if walls else 0.0])
@@ -245,7 +255,8 @@ def sim_desc(abm, N, n_max, simBox, law_p=None):
 
 
 _FIELDS = {"x": A.F_X, "y": A.F_Y, "z": A.F_Z, "id": A.F_ID, "fx": A.F_FX, "fy": A.F_FY,
-           "fz": A.F_FZ, "vol": A.F_VOL, "growth": A.F_GROWTH, "vbirth": A.F_VBIRTH, "lf": A.F_LF}
+           "fz": A.F_FZ, "vol": A.F_VOL, "growth": A.F_GROWTH, "vbirth": A.F_VBIRTH, "lf": A.F_LF,
+           "vx": A.F_VX, "vy": A.F_VY, "vz": A.F_VZ, "r": A.F_RADIUS, "m": A.F_MASS, "cellFate": A.F_FATE}
 
 
 class Context:
@@ -408,6 +419,12 @@ class Community:
         if abm.lagged:
             for k in ("fx", "fy", "fz")[: abm.dims]:
                 self.params[k] = np.zeros(self.N)
+        if abm.agentODE.law == A.LAW_SOFT_SPHERES:
+            for k in ("vx", "vy", "vz")[: abm.dims]:
+                self.params[k] = np.zeros(self.N)
+            self.params["r"] = np.ones(self.N)
+            self.params["m"] = np.ones(self.N)
+            self.params["cellFate"] = np.ones(self.N, dtype=np.int64)
         for k, v in params.items():
             self[k] = v
         self.ctx = None

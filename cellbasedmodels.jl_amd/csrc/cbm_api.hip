@@ -32,7 +32,9 @@ struct Column {
 
 enum ColId {
   C_X = 0, C_Y, C_Z, C_VOL,            // DE state group (flip together)
+  C_VX, C_VY, C_VZ,                     // velocities (soft spheres), DE state too
   C_ID, C_SLOT, C_FX, C_FY, C_FZ, C_GR, C_VB, C_LF,
+  C_RAD, C_MASS, C_FATE,                // per-agent soft-sphere properties
   C_NCOLS
 };
 
@@ -42,6 +44,7 @@ struct cbm_ctx {
   int S = 1;
   int dims = 1, nvar = 1;
   bool growth = false, lagged = false, verlet = false, cl = false, full = false, vtime = false, rules = false;
+  bool inertial = false;                  // CBM_LAW_SOFT_SPHERES: velocities are DE variables
   int nb_kind = NB_FULL;
   int block = 256;
   int device = 0;
@@ -180,7 +183,9 @@ static int validate(cbm_ctx* c) {
   if (m.dims < 1 || m.dims > 3) return fail(c, CBM_ERR_CONFIG, "dims must be 1, 2 or 3");
   if (m.integrator < CBM_EULER || m.integrator > CBM_EULER_HEUN) return fail(c, CBM_ERR_CONFIG, "unknown integrator");
   if (m.neighbors < CBM_FULL || m.neighbors > CBM_CLVD) return fail(c, CBM_ERR_CONFIG, "unknown neighbor algorithm");
-  if (m.law < CBM_LAW_NONE || m.law > CBM_LAW_CUSTOM) return fail(c, CBM_ERR_CONFIG, "unknown interaction law");
+  if (m.law < CBM_LAW_NONE || m.law > CBM_LAW_SOFT_SPHERES) return fail(c, CBM_ERR_CONFIG, "unknown interaction law");
+  if (m.law == CBM_LAW_SOFT_SPHERES && m.rule != CBM_RULE_NONE)
+    return fail(c, CBM_ERR_CONFIG, "the soft-sphere model takes no agent rules");
   if (!(m.dt > 0)) return fail(c, CBM_ERR_CONFIG, "dt must be positive");
   if (m.neighbors == CBM_CELL_LINKED && !(m.cell_edge > 0)) return fail(c, CBM_ERR_CONFIG, "CellLinked needs cellEdge > 0");
   bool verlet = m.neighbors == CBM_VERLET_DISPLACEMENT || m.neighbors == CBM_VERLET_TIME || m.neighbors == CBM_CLVD;
@@ -356,7 +361,8 @@ static int init_ctx(cbm_ctx* c) {
   c->dims = m.dims;
   c->growth = m.rule == CBM_RULE_GROWTH_DIVISION;
   c->rules = m.rule != CBM_RULE_NONE;
-  c->nvar = c->dims + (c->growth ? 1 : 0);
+  c->inertial = m.law == CBM_LAW_SOFT_SPHERES;
+  c->nvar = c->inertial ? 2 * c->dims : c->dims + (c->growth ? 1 : 0);
   c->lagged = m.lagged != 0;
   c->full = m.neighbors == CBM_FULL;
   c->cl = m.neighbors == CBM_CELL_LINKED;
@@ -395,6 +401,19 @@ static int init_ctx(cbm_ctx* c) {
     if ((rc = alloc_column(c, C_VB, 8))) return rc;
     HIPCHK(c, dalloc(&c->gr_new, c->C));
     HIPCHK(c, dalloc(&c->vb_new, c->C));
+  }
+  if (c->inertial) {
+    for (int a = 0; a < c->dims; ++a) if ((rc = alloc_column(c, C_VX + a, 8))) return rc;
+    if ((rc = alloc_column(c, C_RAD, 8))) return rc;
+    if ((rc = alloc_column(c, C_MASS, 8))) return rc;
+    if ((rc = alloc_column(c, C_FATE, 8))) return rc;
+    std::vector<double> one((size_t)c->C, 1.0);
+    std::vector<int64_t> fate1((size_t)c->C, 1);
+    for (int b = 0; b < 2; ++b) {
+      HIPCHK(c, hipMemcpy(c->col[C_RAD].buf[b], one.data(), sizeof(double) * c->C, hipMemcpyHostToDevice));
+      HIPCHK(c, hipMemcpy(c->col[C_MASS].buf[b], one.data(), sizeof(double) * c->C, hipMemcpyHostToDevice));
+      HIPCHK(c, hipMemcpy(c->col[C_FATE].buf[b], fate1.data(), sizeof(int64_t) * c->C, hipMemcpyHostToDevice));
+    }
   }
   if (c->rules) {
     if ((rc = alloc_column(c, C_LF, 8))) return rc;
@@ -604,12 +623,18 @@ static StageArgs stage_base(cbm_ctx* c) {
   a.nvar = c->nvar;
   a.write_fnew = c->lagged ? 1 : 0;
   if (c->growth) a.growth = (const double*)c->col[C_GR].cur();
+  if (c->inertial) {
+    a.rad = (const double*)c->col[C_RAD].cur();
+    a.mass = (const double*)c->col[C_MASS].cur();
+    a.fate = (const int64_t*)c->col[C_FATE].cur();
+  }
   return a;
 }
 
 // Optional per-launch event pairs (cbm_profile): category = stage kind, or
 // PROF_CHAIN for the neighbor update of one step.
 constexpr int PROF_CHAIN = 8;
+constexpr int PROF_STEP = 9;
 constexpr int PROF_NCAT = 12;
 // profile category of a stage kind: 0..7 as the kind, EulerHeun stages 10, 11
 static int prof_cat_of(int kind) { return kind == ST_EH1 ? 10 : (kind == ST_EH2 ? 11 : kind); }
@@ -638,13 +663,19 @@ static int run_stage(cbm_ctx* c, StageArgs& a, int kind) {
   return CBM_OK;
 }
 
+// Column of DE variable v: positions, then velocities (soft spheres) or vol.
+static int statecol_k(const cbm_ctx* c, int v) {
+  if (v < c->dims) return C_X + v;
+  return c->inertial ? C_VX + (v - c->dims) : C_VOL;
+}
+
 static double* statecol(cbm_ctx* c, int v, bool alt) {
-  int k = (v < c->dims) ? C_X + v : C_VOL;
+  int k = statecol_k(c, v);
   return (double*)(alt ? c->col[k].alt() : c->col[k].cur());
 }
 
 static void flip_state(cbm_ctx* c) {
-  for (int v = 0; v < c->nvar; ++v) { int k = (v < c->dims) ? C_X + v : C_VOL; c->col[k].cidx ^= 1; }
+  for (int v = 0; v < c->nvar; ++v) c->col[statecol_k(c, v)].cidx ^= 1;
 }
 
 static int neighbor_update(cbm_ctx* c, int64_t s, bool disp);
@@ -908,6 +939,12 @@ static int field_col(cbm_ctx* c, int32_t field, int& k) {
     case CBM_F_FX: case CBM_F_FY: case CBM_F_FZ:
       if (!c->lagged || field - CBM_F_FX >= c->dims) return CBM_ERR_ARG;
       k = C_FX + (field - CBM_F_FX); return CBM_OK;
+    case CBM_F_VX: case CBM_F_VY: case CBM_F_VZ:
+      if (!c->inertial || field - CBM_F_VX >= c->dims) return CBM_ERR_ARG;
+      k = C_VX + (field - CBM_F_VX); return CBM_OK;
+    case CBM_F_RADIUS: if (!c->inertial) return CBM_ERR_ARG; k = C_RAD; return CBM_OK;
+    case CBM_F_MASS: if (!c->inertial) return CBM_ERR_ARG; k = C_MASS; return CBM_OK;
+    case CBM_F_FATE: if (!c->inertial) return CBM_ERR_ARG; k = C_FATE; return CBM_OK;
     default: return CBM_ERR_ARG;
   }
 }
@@ -919,6 +956,11 @@ static int xfer(cbm_ctx* c, int32_t field, void* host, int64_t n, int32_t sim, b
   if (field_col(c, field, k)) return fail(c, CBM_ERR_ARG, "field not present in this model");
   if (n < 0 || n > c->sims[sim].n_max) return fail(c, CBM_ERR_ARG, "n exceeds capacity");
   if (n == 0) return CBM_OK;
+  if (up && field == CBM_F_FATE)
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t f = ((const int64_t*)host)[i];
+      if (f < 1 || f > 2) return fail(c, CBM_ERR_ARG, "cellFate must be 1 or 2 (the f0 matrix is 2 x 2)");
+    }
   const int bytes = c->col[k].bytes;
   // rows of this simulation only: a one-simulation geometry view
   XferArgs a;
@@ -1571,11 +1613,11 @@ int cbm_profile(cbm_ctx* c, int64_t nsteps, double* avg_ms, int64_t* launches, i
     avg_ms[cat] += ms;
     launches[cat] += 1;
   }
-  for (int q = 0; q < PROF_NCAT - 1; ++q) if (launches[q]) avg_ms[q] /= (double)launches[q];
+  for (int q = 0; q < PROF_NCAT; ++q) if (q != PROF_STEP && launches[q]) avg_ms[q] /= (double)launches[q];
   float tot = 0.f;
   HIPCHK(c, hipEventElapsedTime(&tot, c->ev0, c->ev1));
-  avg_ms[PROF_NCAT - 1] = tot / (double)nsteps;
-  launches[PROF_NCAT - 1] = nsteps;
+  avg_ms[PROF_STEP] = tot / (double)nsteps;
+  launches[PROF_STEP] = nsteps;
   return check_flags(c);
 }
 

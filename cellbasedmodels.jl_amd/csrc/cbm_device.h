@@ -78,7 +78,10 @@ __device__ __forceinline__ int64_t floor_div(double a, double b) {
 // hoisted out of the pair loop.  operator() returns whether the pair
 // interacts (coincident agents, d == 0, never do) and always sets s, so the
 // caller can update its accumulators with selects instead of branches.
-enum { LAW_NONE = 0, LAW_LINEAR = 1, LAW_REP_ADH = 2, LAW_AGGREGATION = 3, LAW_CUSTOM = 4 };
+enum { LAW_NONE = 0, LAW_LINEAR = 1, LAW_REP_ADH = 2, LAW_AGGREGATION = 3, LAW_CUSTOM = 4, LAW_SOFT_SPHERES = 5 };
+
+template <class A, class B> struct same_type { static constexpr bool value = false; };
+template <class A> struct same_type<A, A> { static constexpr bool value = true; };
 
 template <int LAW> struct PairLaw;
 
@@ -165,6 +168,25 @@ struct RepAdhGrad {
     dth[0] = sk * id; dth[1] = sr * id; dth[2] = se * id;
     return d > 0.0 && d < dc;
   }
+};
+
+// src/models/softSpheres.jl:25-30 (2-D), :64-71 (3-D): with r_ij = r_i + r_j,
+// for 0 < d < mu r_ij the force on i along each axis gains
+// ((f0[fate_i][fate_j] (r_ij/d - 1)) (mu r_ij/d - 1) (x_i - x_j)) / d.
+// p = {b, mu, f0[1][1], f0[1][2], f0[2][1], f0[2][2]}.  The stage skeleton
+// handles this law apart (per-agent radius and fate, inertial variables).
+template <> struct PairLaw<LAW_SOFT_SPHERES> {
+  static constexpr bool active = true, per_axis_division = false, walls = false;
+  double mu;
+  const double* f0;
+  __device__ __forceinline__ explicit PairLaw(const double* p) : mu(p[1]), f0(p + 2) {}
+  __device__ __forceinline__ bool pair(double d, double ri, double rj, int64_t fi, int64_t fj, double& s) const {
+    const double rij = ri + rj;
+    const double f = f0[2 * (fi - 1) + (fj - 1)];
+    s = f * (rij / d - 1) * (mu * rij / d - 1);
+    return d < mu * rij && d > 0;
+  }
+  __device__ __forceinline__ bool operator()(double, double& s) const { s = 0.0; return false; }
 };
 
 template <> struct PairLaw<LAW_NONE> {

@@ -855,6 +855,7 @@ static hipError_t launch_stage_law(int law, dim3 grid, dim3 block, hipStream_t s
     case LAW_LINEAR: hipLaunchKernelGGL((k_stage<D, NB, LAW_LINEAR, PER, NOISE>), grid, block, 0, s, a); break;
     case LAW_REP_ADH: hipLaunchKernelGGL((k_stage<D, NB, LAW_REP_ADH, PER, NOISE>), grid, block, 0, s, a); break;
     case LAW_AGGREGATION: hipLaunchKernelGGL((k_stage<D, NB, LAW_AGGREGATION, PER, NOISE>), grid, block, 0, s, a); break;
+    case LAW_SOFT_SPHERES: hipLaunchKernelGGL((k_stage<D, NB, LAW_SOFT_SPHERES, PER, NOISE>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -103,16 +103,21 @@ __device__ __forceinline__ void stage_body(const StageArgs& a) {
 #pragma unroll
   for (int c = 0; c < D; ++c) xi[c] = a.S[c][i];
   double acc[3] = {0.0, 0.0, 0.0};
+  // the inertial soft-sphere model: per-agent radius and fate in the pair
+  // term, velocities as DE variables
+  constexpr bool SOFT = same_type<Law, PairLaw<LAW_SOFT_SPHERES>>::value;
 
   if (Law::active) {
     const Law law(sd.p);
+    const double rad_i = SOFT ? a.rad[i] : 0.0;
+    const int64_t fate_i = SOFT ? a.fate[i] : 0;
     double L[3] = {0.0, 0.0, 0.0}, hL[3] = {0.0, 0.0, 0.0};
     if (PER) {
 #pragma unroll
       for (int c = 0; c < D; ++c) { L[c] = sd.L[c]; hL[c] = 0.5 * sd.L[c]; }
     }
     // one neighbor at position xj (the agentODE interaction body)
-    auto body = [&](const double* xj, bool live) {
+    auto body = [&](const double* xj, int64_t j, bool live) {
       double dx[3];
 #pragma unroll
       for (int c = 0; c < D; ++c) {
@@ -124,8 +129,14 @@ __device__ __forceinline__ void stage_body(const StageArgs& a) {
       if (D > 2) s2 = s2 + dx[2] * dx[2];
       const double d = sqrt(s2);
       double s;
-      const bool on = law(d, s) && live;
-      if (Law::per_axis_division) {
+      bool on;
+      if constexpr (SOFT) on = law.pair(d, rad_i, a.rad[j], fate_i, a.fate[j], s) && live;
+      else on = law(d, s) && live;
+      if constexpr (SOFT) {
+        // ((s dx) / d) per axis, the reference's order (softSpheres.jl:28-29)
+#pragma unroll
+        for (int c = 0; c < D; ++c) { const double v = acc[c] + (s * dx[c]) / d; acc[c] = on ? v : acc[c]; }
+      } else if (Law::per_axis_division) {
 #pragma unroll
         for (int c = 0; c < D; ++c) { const double v = acc[c] + s * (dx[c] / d); acc[c] = on ? v : acc[c]; }
       } else {
@@ -153,22 +164,25 @@ __device__ __forceinline__ void stage_body(const StageArgs& a) {
       for (int u = 0; u < LCH; ++u) jn[u] = (u < cnt) ? lp[(int64_t)u * ldl] : (int32_t)i;
       for (int32_t q0 = 0; q0 < cnt; q0 += LCH) {
         double px[LCH][3];
+        int32_t jc[LCH];
 #pragma unroll
-        for (int u = 0; u < LCH; ++u)
+        for (int u = 0; u < LCH; ++u) {
+          jc[u] = jn[u];
 #pragma unroll
-          for (int c = 0; c < D; ++c) px[u][c] = a.S[c][jn[u]];
+          for (int c = 0; c < D; ++c) px[u][c] = a.S[c][jc[u]];
+        }
         lp += LCH * ldl;
 #pragma unroll
         for (int u = 0; u < LCH; ++u) jn[u] = (q0 + LCH + u < cnt) ? lp[(int64_t)u * ldl] : (int32_t)i;
 #pragma unroll
-        for (int u = 0; u < LCH; ++u) body(px[u], q0 + u < cnt);
+        for (int u = 0; u < LCH; ++u) body(px[u], jc[u], q0 + u < cnt);
       }
     } else {
       for_candidates<D, NB>(a, sd, N, i, [&](int64_t j) {
         double xj[3];
 #pragma unroll
         for (int c = 0; c < D; ++c) xj[c] = a.S[c][j];
-        body(xj, true);
+        body(xj, j, true);
       });
     }
     if (Law::walls && a.walls) {
@@ -184,18 +198,31 @@ __device__ __forceinline__ void stage_body(const StageArgs& a) {
 
   // derivative of each DE variable
   double du[MAX_STATE];
+  if constexpr (SOFT) {
+    // dt(v) = -b*v/m + f/m, dt(x) = v (src/models/softSpheres.jl:32-35)
+    const double b = sd.p[0], m = a.mass[i];
 #pragma unroll
-  for (int c = 0; c < D; ++c) {
-    double v = a.lagged ? a.F[c][i] : acc[c];
-    if (a.drift != 0.0) v = v - a.drift * xi[c];
-    du[c] = v;
-    if (a.write_fnew) a.Fnew[c][i] = acc[c];
+    for (int c = 0; c < D; ++c) {
+      const double f = a.lagged ? a.F[c][i] : acc[c];
+      const double v = a.S[D + c][i];
+      du[c] = v;
+      du[D + c] = -b * v / m + f / m;
+      if (a.write_fnew) a.Fnew[c][i] = acc[c];
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      double v = a.lagged ? a.F[c][i] : acc[c];
+      if (a.drift != 0.0) v = v - a.drift * xi[c];
+      du[c] = v;
+      if (a.write_fnew) a.Fnew[c][i] = acc[c];
+    }
   }
-  const bool grow = a.nvar > D;
+  const bool grow = !SOFT && a.nvar > D;
   if (grow) du[D] = a.growth[i] * a.S[D][i];
 
   const double dt = a.dt;
-  const int nv = grow ? D + 1 : D;
+  const int nv = SOFT ? 2 * D : (grow ? D + 1 : D);
   double res[MAX_STATE];
   if (NOISE) {
     // EM and the EulerHeun predictor: (u + f dt) + (g xi) sqrt(dt), xi keyed
@@ -210,11 +237,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& a) {
 #pragma unroll
       for (int c = 0; c < D; ++c)
         res[c] = a.U[c][i] + ((a.kacc[c][i] + du[c]) * dt + (a.noise + a.noise) * z[c] * a.sqdt) / 2;
-      if (grow) res[D] = a.U[D][i] + ((a.kacc[D][i] + du[D]) * dt) / 2;
+      for (int v = D; v < nv; ++v) res[v] = a.U[v][i] + ((a.kacc[v][i] + du[v]) * dt) / 2;
     } else {
 #pragma unroll
       for (int c = 0; c < D; ++c) res[c] = (a.U[c][i] + du[c] * dt) + a.noise * z[c] * a.sqdt;
-      if (grow) res[D] = a.U[D][i] + du[D] * dt;
+      for (int v = D; v < nv; ++v) res[v] = a.U[v][i] + du[v] * dt;   // no noise off the positions
       if (a.kind == ST_EH1)
         for (int c = 0; c < nv; ++c) a.kacc[c][i] = du[c];
     }

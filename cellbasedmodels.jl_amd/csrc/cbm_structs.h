@@ -17,7 +17,7 @@ using __hip_internal::uint8_t;
 namespace cbm {
 
 constexpr int MAX_COLS = 16;      // permutable per-agent columns
-constexpr int MAX_STATE = 4;      // DE variables: up to 3 positions + vol
+constexpr int MAX_STATE = 6;      // DE variables: 3 positions + vol, or 3 positions + 3 velocities
 
 // Per-simulation constants (device array, one entry per simulation).
 // Row space: simulation s owns rows [row_off, row_off + n_max).
@@ -85,6 +85,9 @@ struct StageArgs {
   double* Fnew[3];              // force accumulator written by this stage
   const double* xref[3];        // positions at the last Verlet build
   const double* growth;         // per-agent growth rate (vol variable)
+  const double* rad;            // soft spheres: radius, mass, cellFate (1-based) per row
+  const double* mass;
+  const int64_t* fate;
   const int64_t* id;
   const int32_t* list;          // ELL Verlet list, entry q of row i at q*ldl + i
   const int32_t* list_n;

@@ -70,13 +70,21 @@ enum cbm_neighbors {
  *   CBM_LAW_AGGREGATION  p = {rRep, fRep, rAtr, fAtr, walls}:
  *                          s = fRep (rRep - d)  d < rRep;  s = -fAtr (rAtr - d)  d < rAtr
  *                          plus the box-wall push fRep within rRep/2 of a wall
- *   CBM_LAW_CUSTOM       user body compiled with hipRTC (cbm_compile)          */
+ *   CBM_LAW_CUSTOM       user body compiled with hipRTC (cbm_compile)
+ *   CBM_LAW_SOFT_SPHERES p = {b, mu, f0[1][1], f0[1][2], f0[2][1], f0[2][2]}:
+ *     the inertial soft-sphere model of src/models/softSpheres.jl:1-81.  The
+ *     DE variables are the positions and the velocities (CBM_F_VX..VZ); each
+ *     agent has a radius (CBM_F_RADIUS), a mass (CBM_F_MASS) and a cellFate
+ *     in {1, 2} (CBM_F_FATE, int64).  With r_ij = r_i + r_j, for 0 < d < mu r_ij
+ *       F_i += (f0[fate_i][fate_j] (r_ij/d - 1)(mu r_ij/d - 1)(x_i - x_j))/d
+ *     and dv/dt = (-b v)/m + F/m, dx/dt = v.  No rules or adjoint.          */
 enum cbm_law {
   CBM_LAW_NONE = 0,
   CBM_LAW_LINEAR = 1,
   CBM_LAW_REP_ADH = 2,
   CBM_LAW_AGGREGATION = 3,
-  CBM_LAW_CUSTOM = 4
+  CBM_LAW_CUSTOM = 4,
+  CBM_LAW_SOFT_SPHERES = 5
 };
 
 /* ------------------------------------------------------------ agent rules */
@@ -106,7 +114,10 @@ enum cbm_field {
   CBM_F_FX = 4, CBM_F_FY = 5, CBM_F_FZ = 6,    /* committed force accumulator (lagged mode) */
   CBM_F_VOL = 7, CBM_F_GROWTH = 8, CBM_F_VBIRTH = 9,
   CBM_F_LF = 10,                               /* int64 payload (`lf` of the add/remove KAT) */
-  CBM_F_COUNT = 11
+  CBM_F_VX = 11, CBM_F_VY = 12, CBM_F_VZ = 13,  /* velocities (CBM_LAW_SOFT_SPHERES)        */
+  CBM_F_RADIUS = 14, CBM_F_MASS = 15,
+  CBM_F_FATE = 16,                             /* int64 cellFate, 1-based                   */
+  CBM_F_COUNT = 17
 };
 
 /* --------------------------------------------------- model description */

@@ -139,7 +139,10 @@ struct Orc {
   int64_t N = 0, nMax = 0, idMax = 0, step = 0;
   double t = 0;
   bool growth = false;
+  bool inertial = false;                      // CBM_LAW_SOFT_SPHERES: velocities are DE variables
   std::vector<double> X[3], F[3], Fnew[3];
+  std::vector<double> V[3], rad, mass;        // velocities, radius, mass (soft spheres)
+  std::vector<int64_t> fate;                  // cellFate, 1-based
   std::vector<double> vol, gr, vb, grNew, vbNew;
   std::vector<int64_t> id, lf;
   std::vector<char> alive;
@@ -301,13 +304,34 @@ static inline bool law_s(int law, const T* p, const T& d, T& s) {
   return false;
 }
 
+// Soft-sphere pair term of src/models/softSpheres.jl:25-30 (2-D) and :64-71
+// (3-D), in the reference's evaluation order: for 0 < d < mu r_ij, the
+// force on i along axis a gains ((f0 (r_ij/d - 1)) (mu r_ij/d - 1) dx_a) / d.
+static inline bool soft_s(const double* p, double d, double ri, double rj, int64_t fi, int64_t fj, double& s) {
+  const double rij = ri + rj;
+  if (!(d < p[1] * rij && d > 0)) return false;
+  const double f0 = p[2 + 2 * (fi - 1) + (fj - 1)];
+  s = f0 * (rij / d - 1) * (p[1] * rij / d - 1);
+  return true;
+}
+
 // The agentODE body evaluated on stage state S for agent i: returns the
-// derivative du (positions) and the new force accumulator acc.
+// derivative du (positions, then velocities for soft spheres) and the new
+// force accumulator acc.
 static void deriv(const Orc& o, int64_t i, const std::vector<double>* S, double* du, double* acc) {
   double a3[3] = {0, 0, 0};
   const double* p = o.s.p;
   int law = o.m.law;
-  if (law != CBM_LAW_NONE) {
+  if (law == CBM_LAW_SOFT_SPHERES) {
+    for_candidates(o, i, [&](int64_t j) {
+      double dx[3] = {0, 0, 0};
+      for (int a = 0; a < o.dims; ++a) dx[a] = o.mi(a, S[a][i] - S[a][j]);
+      double d = dist(o, dx);
+      double sv;
+      if (soft_s(p, d, o.rad[i], o.rad[j], o.fate[i], o.fate[j], sv))
+        for (int a = 0; a < o.dims; ++a) a3[a] = a3[a] + (sv * dx[a]) / d;
+    });
+  } else if (law != CBM_LAW_NONE) {
     for_candidates(o, i, [&](int64_t j) {
       double dx[3] = {0, 0, 0};
       for (int a = 0; a < o.dims; ++a) dx[a] = o.mi(a, S[a][i] - S[a][j]);
@@ -331,6 +355,18 @@ static void deriv(const Orc& o, int64_t i, const std::vector<double>* S, double*
         else if (x > o.s.box_hi[a] - p[0] / 2) a3[a] = a3[a] - p[1];
       }
     }
+  }
+  if (o.inertial) {
+    // dt(v) = -b*v/m + f/m, dt(x) = v (src/models/softSpheres.jl:32-35)
+    const double b = p[0], mi = o.mass[i];
+    for (int a = 0; a < o.dims; ++a) {
+      acc[a] = a3[a];
+      const double f = o.m.lagged ? o.F[a][i] : a3[a];
+      const double v = S[o.dims + a][i];
+      du[a] = v;
+      du[o.dims + a] = -b * v / mi + f / mi;
+    }
+    return;
   }
   for (int a = 0; a < o.dims; ++a) {
     acc[a] = a3[a];
@@ -552,20 +588,23 @@ static void compact(Orc& o, const std::vector<int64_t>& holes, int64_t nAdd) {
 static void de_step(Orc& o, int64_t nA) {
   const int d = o.dims;
   const double dt = o.m.dt;
-  const int nv = d + (o.growth ? 1 : 0);     // DE variables: positions (+ vol)
-  std::vector<double> U[4];
+  // DE variables: positions (+ vol), or positions then velocities
+  const int nv = o.inertial ? 2 * d : d + (o.growth ? 1 : 0);
+  std::vector<double> U[6];
   for (int a = 0; a < d; ++a) U[a] = o.X[a];
+  if (o.inertial) for (int a = 0; a < d; ++a) U[d + a] = o.V[a];
   if (o.growth) U[d] = o.vol;
   auto eval = [&](const std::vector<double>* S, std::vector<double>* K) {
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < nA; ++i) {
-      double du[3], acc[3];
+      double du[6], acc[3];
       deriv(o, i, S, du, acc);
       for (int a = 0; a < d; ++a) { K[a][i] = du[a]; o.Fnew[a][i] = acc[a]; }
+      if (o.inertial) for (int a = 0; a < d; ++a) K[d + a][i] = du[d + a];
       if (o.growth) K[d][i] = o.gr[i] * S[d][i];
     }
   };
-  std::vector<double> K1[4], K2[4], K3[4], K4[4], H[4];
+  std::vector<double> K1[6], K2[6], K3[6], K4[6], H[6];
   for (int v = 0; v < nv; ++v) { K1[v].assign(nA, 0); H[v] = U[v]; }
   int integ = o.m.integrator;
   if (integ == CBM_EULER) {
@@ -601,7 +640,7 @@ static void de_step(Orc& o, int64_t nA) {
       normal2(r0, z[0], z[1]);
       if (d > 2) { U4 r1 = draw(o.m.seed, o.step, o.id[i], 0, 0x101u); normal2(r1, z[2], z[3]); }
       for (int a = 0; a < d; ++a) H[a][i] = (U[a][i] + K1[a][i] * dt) + gsig * z[a] * sq;
-      if (o.growth) H[d][i] = U[d][i] + K1[d][i] * dt;
+      for (int v = d; v < nv; ++v) H[v][i] = U[v][i] + K1[v][i] * dt;    // no noise off the positions
     }
     for (int v = 0; v < nv; ++v) K2[v].assign(nA, 0);
     eval(H, K2);
@@ -612,7 +651,7 @@ static void de_step(Orc& o, int64_t nA) {
       if (d > 2) { U4 r1 = draw(o.m.seed, o.step, o.id[i], 0, 0x501u); normal2(r1, z[2], z[3]); }
       for (int a = 0; a < d; ++a)
         U[a][i] = U[a][i] + ((K1[a][i] + K2[a][i]) * dt + (gsig + gsig) * z[a] * sq) / 2;
-      if (o.growth) U[d][i] = U[d][i] + ((K1[d][i] + K2[d][i]) * dt) / 2;
+      for (int v = d; v < nv; ++v) U[v][i] = U[v][i] + ((K1[v][i] + K2[v][i]) * dt) / 2;
     }
   } else {  // EM
     eval(U, K1);
@@ -624,10 +663,11 @@ static void de_step(Orc& o, int64_t nA) {
       normal2(r0, z[0], z[1]);
       if (d > 2) { U4 r1 = draw(o.m.seed, o.step, o.id[i], 0, 0x101u); normal2(r1, z[2], z[3]); }
       for (int a = 0; a < d; ++a) U[a][i] = (U[a][i] + K1[a][i] * dt) + gsig * z[a] * sq;
-      if (o.growth) U[d][i] = U[d][i] + K1[d][i] * dt;
+      for (int v = d; v < nv; ++v) U[v][i] = U[v][i] + K1[v][i] * dt;
     }
   }
   for (int a = 0; a < d; ++a) std::copy(U[a].begin(), U[a].begin() + nA, o.X[a].begin());
+  if (o.inertial) for (int a = 0; a < d; ++a) std::copy(U[d + a].begin(), U[d + a].begin() + nA, o.V[a].begin());
   if (o.growth) std::copy(U[d].begin(), U[d].begin() + nA, o.vol.begin());
 }
 
@@ -660,7 +700,7 @@ static void one_step(Orc& o) {
   compute_neighbors(o, changed);               // computeNeighbors!
 }
 
-static int field_is_int(int f) { return f == CBM_F_ID || f == CBM_F_LF; }
+static int field_is_int(int f) { return f == CBM_F_ID || f == CBM_F_LF || f == CBM_F_FATE; }
 
 // -------------------------------------------------------------- adjoint
 // Overdamped Euler trajectory loss for the REP_ADH law (model-scope
@@ -724,9 +764,13 @@ void* orc_create(const cbm_model_desc* m, const cbm_sim_desc* s) {
   o->N = s->n; o->nMax = std::max<int64_t>(s->n_max, s->n);
   o->idMax = s->n;
   o->growth = (m->rule == CBM_RULE_GROWTH_DIVISION);
+  o->inertial = (m->law == CBM_LAW_SOFT_SPHERES);
+  if (o->inertial && m->rule != CBM_RULE_NONE) { o->err = CBM_ERR_CONFIG; o->msg = "soft spheres take no rules"; }
   for (int a = 0; a < 3; ++a) {
     o->X[a].assign(o->nMax, 0); o->F[a].assign(o->nMax, 0); o->Fnew[a].assign(o->nMax, 0); o->Xref[a].assign(o->nMax, 0);
+    o->V[a].assign(o->nMax, 0);
   }
+  o->rad.assign(o->nMax, 1.0); o->mass.assign(o->nMax, 1.0); o->fate.assign(o->nMax, 1);
   o->id.resize(o->nMax); o->lf.assign(o->nMax, 0); o->alive.assign(o->nMax, 1);
   for (int64_t i = 0; i < o->nMax; ++i) o->id[i] = i + 1;
   o->vol.assign(o->nMax, 1.0); o->gr.assign(o->nMax, 0.0); o->vb.assign(o->nMax, 1.0);
@@ -765,6 +809,10 @@ int orc_set(void* h, int field, const void* data, int64_t n) {
       case CBM_F_VBIRTH: o.vb[i] = v; o.vbNew[i] = v; break;
       case CBM_F_ID: o.id[i] = iv; break;
       case CBM_F_LF: o.lf[i] = iv; break;
+      case CBM_F_VX: case CBM_F_VY: case CBM_F_VZ: o.V[field - CBM_F_VX][i] = v; break;
+      case CBM_F_RADIUS: o.rad[i] = v; break;
+      case CBM_F_MASS: o.mass[i] = v; break;
+      case CBM_F_FATE: if (iv < 1 || iv > 2) return CBM_ERR_ARG; o.fate[i] = iv; break;
       default: return CBM_ERR_ARG;
     }
   }
@@ -785,6 +833,10 @@ int orc_get(void* h, int field, void* out, int64_t n) {
       case CBM_F_VBIRTH: dv[i] = o.vb[i]; break;
       case CBM_F_ID: iv[i] = o.id[i]; break;
       case CBM_F_LF: iv[i] = o.lf[i]; break;
+      case CBM_F_VX: case CBM_F_VY: case CBM_F_VZ: dv[i] = o.V[field - CBM_F_VX][i]; break;
+      case CBM_F_RADIUS: dv[i] = o.rad[i]; break;
+      case CBM_F_MASS: dv[i] = o.mass[i]; break;
+      case CBM_F_FATE: iv[i] = o.fate[i]; break;
       default: return CBM_ERR_ARG;
     }
   }

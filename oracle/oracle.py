@@ -59,7 +59,7 @@ def lib():
     return _lib
 
 
-INT_FIELDS = (3, 10)
+INT_FIELDS = (3, 10, 16)   # id, lf, cellFate (int64)
 
 
 class Oracle:

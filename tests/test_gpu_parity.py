@@ -259,3 +259,18 @@ def test_list_overflow_is_reported(gpu_available):
     with pytest.raises(A.CBMError) as e:
         g.sync()
     assert e.value.code == A.ERR_LIST_OVERFLOW
+
+
+def test_profile_categories(gpu_available):
+    """cbm_profile: stage kinds 0..7, neighbor update 8, whole step 9, EulerHeun 10/11."""
+    for integ, cats in ((A.HEUN, (2, 3)), (A.EULER_HEUN, (10, 11)), (A.EM, (1,))):
+        m, s, pos = c2(n=2048, integ=integ, noise=0.1 if integ != A.HEUN else 0.0)
+        g = Context(m, [s])
+        for a in range(3):
+            g.upload(a, pos[a])
+        g.neighbors()
+        ms, nl = g.profile(5)
+        for k in cats:
+            assert nl[k] == 5 and ms[k] > 0, (integ, k)
+        assert nl[8] == 5 and nl[9] == 5 and ms[9] > 0
+        assert sum(nl[k] for k in range(12) if k not in cats + (8, 9)) == 0
