@@ -10,7 +10,8 @@ gather + pair law + integrator), the displacement trigger and the
 the timed region.  Prints one JSON line.
 
 --workload em selects the Euler-Maruyama variant (isotropic noise D = 0.01,
-Philox keyed by (seed, step, agent id)).
+Philox keyed by (seed, step, agent id)); --workload heun_cl runs the same
+Heun model on a cell-linked grid of edge 2.4 rebuilt every step.
 """
 import argparse
 import json
@@ -48,7 +49,7 @@ def workload(kind, n):
     L = (n * (4.0 / 3.0) * np.pi / 0.3) ** (1.0 / 3.0)
     m = A.ModelDesc()
     m.dims = 3
-    m.integrator = A.HEUN if kind == "heun" else A.EM
+    m.integrator = A.EM if kind == "em" else A.HEUN
     m.neighbors = A.VERLET_DISPLACEMENT
     m.law = A.LAW_REP_ADH
     m.max_neighbors = 32
@@ -65,6 +66,10 @@ def workload(kind, n):
         s.box_hi[a] = L
     for q, v in enumerate([1.0, 1.0, 0.1, 2.4]):
         s.p[q] = v
+    if kind == "heun_cl":
+        # the same model on a cell grid of edge 2.4 (the law's cutoff), rebuilt every step
+        m.neighbors = A.CELL_LINKED
+        m.cell_edge = 2.4
     rng = np.random.default_rng(2)
     pos = [rng.uniform(0.0, L, n) for _ in range(3)]
     return m, s, pos, L
@@ -73,6 +78,8 @@ def workload(kind, n):
 def stage_bytes(kind, kbar):
     """Algorithmic HBM bytes per agent for each stage kernel (FP64 SoA, int32 ELL list)."""
     lst = 4.0 + 4.0 * kbar                      # list length + entries
+    if kind == "heun_cl":                       # cell-linked: read the row's cell, no list, no x_ref
+        return {2: 24 + 4 + 24, 3: 24 + 24 + 4 + 24}
     if kind == "heun":
         return {2: 24 + lst + 24,               # stage 1: read u, list; write h
                 3: 24 + 24 + lst + 24 + 24}     # stage 2: read h, u, list, x_ref; write u
@@ -408,7 +415,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=1048576)
-    ap.add_argument("--workload", choices=["heun", "em"], default="heun")
+    ap.add_argument("--workload", choices=["heun", "em", "heun_cl"], default="heun")
     ap.add_argument("--graph", type=int, default=0, help="capture this many steps per hipGraph replay")
     ap.add_argument("--em", type=int, default=1, help="also time the C2 Euler-Maruyama variant")
     ap.add_argument("--c1", type=int, default=1, help="also time C1 on the device and the oracle")
@@ -482,7 +489,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"C2 3-D cell mechanics, {args.workload.upper()}, Verlet r_list 2.7, rebuild at 0.15",
+        "config": {"workload": ("C2 3-D cell mechanics, HEUN, cell-linked (edge 2.4, rebuilt every step)"
+                                if args.workload == "heun_cl" else
+                                f"C2 3-D cell mechanics, {args.workload.upper()}, Verlet r_list 2.7, rebuild at 0.15"),
                    "N": s.n, "box": L, "dt": m.dt, "graph_steps": args.graph},
         "roofline": {
             "bound": "hbm",
@@ -491,7 +500,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom_name, s.n),
+            "traffic": measured_traffic(dom_name, s.n) if args.workload == "heun" else None,
             "alg_bytes_per_launch": sb[dom] * s.n,
             "avg_launch_ms": ms[dom],
             "mean_list_len": kbar,
