@@ -338,6 +338,13 @@ class Context:
         self._chk(self.lib.cbm_get_rebuilds(self.h, C.byref(v)))
         return v.value
 
+    def bricks(self):
+        """CellLinked LDS tiling: {'bdim', 'cap', 'nbricks', 'overflows'}."""
+        bd = np.zeros(3, dtype=np.int32)
+        cap, nb, over = C.c_int32(), C.c_int64(), C.c_int64()
+        self._chk(self.lib.cbm_get_bricks(self.h, bd.ctypes.data, C.byref(cap), C.byref(nb), C.byref(over)))
+        return {"bdim": bd.tolist(), "cap": cap.value, "nbricks": nb.value, "overflows": over.value}
+
     def neighbor_lists(self, radius, cap, sim=0):
         n = self.n(sim)
         counts = np.zeros(max(n, 1), dtype=np.int32)

@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <limits>
 #include <vector>
 #include <algorithm>
 #include "cbm_internal.h"
@@ -82,8 +83,7 @@ struct cbm_ctx {
   uint8_t* dead = nullptr;
 
   // neighbor engine
-  int32_t* crank = nullptr;               // per-sim cell -> Morton rank tables
-  int32_t* glin = nullptr;                // bucket -> rank_off + cell, -1 for other buckets
+  int32_t* gbucket = nullptr;             // rank_off + cell -> global bucket (cell_off + Morton rank)
   CellRange* cell_range = nullptr;        // rank_off + cell -> its rows after a rebuild
   int32_t* lin = nullptr;
   int32_t *key = nullptr, *count = nullptr, *cursor = nullptr, *start = nullptr, *tmp = nullptr, *cell_row = nullptr;
@@ -93,6 +93,11 @@ struct cbm_ctx {
   int32_t* list = nullptr;
   int32_t* list_n = nullptr;
   int cap = 0;
+  int32_t* bricks = nullptr;              // cell-linked brick blocks: sim, origin x, y, z
+  int64_t nbricks = 0;
+  int bdim[3] = {1, 1, 1};
+  int brick_cap = 0;                      // agents staged per brick block
+  size_t brick_lds = 0;                   // its LDS bytes
 
   // adjoint work buffers, kept across calls and grown on demand
   struct DevBuf { void* p = nullptr; size_t bytes = 0; };
@@ -101,7 +106,7 @@ struct cbm_ctx {
   std::vector<int64_t> adj_tgt_n;
   std::vector<int32_t*> snap_l, snap_n;   // Verlet-list snapshots of the forward pass
   int32_t* flags = nullptr;               // [0..1] rebuild flags by step parity, [2] list overflow, [3] capacity
-  int64_t* counters = nullptr;            // [0] rebuilds, [1] device step
+  int64_t* counters = nullptr;            // [0] rebuilds, [1] device step, [2] brick overflows
   bool structure_valid = false;
 
   // medium
@@ -250,6 +255,87 @@ static void morton_ranks(const int32_t* gn, int dims, std::vector<int32_t>& out)
   for (int64_t r = 0; r < nc; ++r) out[base + (size_t)key[(size_t)r].second] = (int32_t)r;
 }
 
+// Bricks of the cell-linked stage: 2^m cells, Morton-aligned (x takes the
+// extra bits first, so a brick's cells have consecutive Morton ranks and
+// its rows are one range).  m is the largest size whose expected population
+// (the densest simulation's mean density; capacity when rules change the
+// population) stays within ~1.25 blocks of 256 threads, lowered while the
+// LDS for 1.15x the expected halo population would exceed 64 KiB (a halo
+// above that walks global memory).  Bricks
+// are listed per simulation in Morton order of their coordinates.
+static int setup_bricks(cbm_ctx* c) {
+  const int D = c->dims;
+  const bool soft = c->inertial;
+  double rho = 0.0;
+  for (int s = 0; s < c->S; ++s) {
+    const double n = (double)std::max<int64_t>(c->rules ? c->sims[s].n_max : c->sims[s].n, 1);
+    rho = std::max(rho, n / (double)c->h_sims[s].ncells);
+  }
+  const int mmin = D, mmax = D == 3 ? 9 : 10;
+  int m = mmin;
+  while (m < mmax && rho * (double)(1 << (m + 1)) <= 320.0) ++m;
+  const int64_t budget = 64 * 1024;
+  const int64_t per = 8 * (D + (soft ? 1 : 0)) + (soft ? 4 : 0) + 2;   // LDS bytes per staged agent
+  int bd[3] = {1, 1, 1};
+  int cap = 0;
+  for (;; --m) {
+    bd[0] = 1 << ((m + D - 1) / D);
+    bd[1] = D > 1 ? 1 << ((m + D - 2) / D) : 1;
+    bd[2] = D > 2 ? 1 << ((m + D - 3) / D) : 1;
+    const int H = brick_halo_cells(D, bd[0], bd[1], bd[2]);
+    const int64_t fixed = brick_lds_bytes(D, soft, 0, H) + 16;
+    const int64_t capmax = (budget - fixed) / per;
+    const double want = 1.15 * rho * H + 32.0;
+    if (want <= (double)capmax || m == mmin) {
+      cap = (int)std::max<int64_t>(1, std::min<int64_t>(capmax, (int64_t)std::max(256.0, want)));
+      break;
+    }
+  }
+  for (int a = 0; a < 3; ++a) c->bdim[a] = bd[a];
+  c->brick_cap = cap;
+  c->brick_lds = (size_t)brick_lds_bytes(D, soft, cap, brick_halo_cells(D, bd[0], bd[1], bd[2]));
+  std::vector<int32_t> tab;
+  for (int s = 0; s < c->S; ++s) {
+    const SimDev& sd = c->h_sims[s];
+    int64_t nb[3];
+    for (int a = 0; a < 3; ++a) nb[a] = (sd.gn[a] + bd[a] - 1) / bd[a];
+    std::vector<std::pair<uint64_t, int64_t>> key;
+    key.reserve((size_t)(nb[0] * nb[1] * nb[2]));
+    for (int64_t z = 0; z < nb[2]; ++z)
+      for (int64_t y = 0; y < nb[1]; ++y)
+        for (int64_t x = 0; x < nb[0]; ++x) {
+          uint64_t code = spread_bits((uint64_t)x, D);
+          if (D > 1) code |= spread_bits((uint64_t)y, D) << 1;
+          if (D > 2) code |= spread_bits((uint64_t)z, D) << 2;
+          key.push_back({code, x + nb[0] * (y + nb[1] * z)});
+        }
+    std::sort(key.begin(), key.end());
+    for (const auto& k : key) {
+      const int64_t x = k.second % nb[0], y = (k.second / nb[0]) % nb[1], z = k.second / (nb[0] * nb[1]);
+      // the brick's cells have consecutive Morton ranks: its buckets are
+      // [cell_off + lowest rank, + number of cells)
+      const int32_t o[3] = {(int32_t)(x * bd[0]), (int32_t)(y * bd[1]), (int32_t)(z * bd[2])};
+      int64_t lo = std::numeric_limits<int64_t>::max(), cnt = 0;
+      for (int32_t cz = o[2]; cz < std::min(o[2] + bd[2], sd.gn[2]); ++cz)
+        for (int32_t cy = o[1]; cy < std::min(o[1] + bd[1], sd.gn[1]); ++cy)
+          for (int32_t cx = o[0]; cx < std::min(o[0] + bd[0], sd.gn[0]); ++cx) {
+            const int64_t l = cx + (int64_t)sd.gn[0] * (cy + (int64_t)sd.gn[1] * cz);
+            lo = std::min<int64_t>(lo, c->h_crank[(size_t)(sd.rank_off + l)]);
+            ++cnt;
+          }
+      tab.push_back(s);
+      for (int a = 0; a < 3; ++a) tab.push_back(o[a]);
+      tab.push_back((int32_t)(sd.cell_off + lo));
+      tab.push_back((int32_t)(sd.cell_off + lo + cnt));
+    }
+  }
+  c->nbricks = (int64_t)tab.size() / BRICK_WORDS;
+  if (c->nbricks > ((int64_t)1 << 31) - 1) return fail(c, CBM_ERR_CONFIG, "too many cell bricks; raise cellEdge");
+  HIPCHK(c, dalloc(&c->bricks, (int64_t)tab.size()));
+  HIPCHK(c, hipMemcpy(c->bricks, tab.data(), sizeof(int32_t) * tab.size(), hipMemcpyHostToDevice));
+  return CBM_OK;
+}
+
 static int setup_geometry(cbm_ctx* c) {
   const cbm_model_desc& m = c->m;
   c->h_sims.resize(c->S);
@@ -339,8 +425,8 @@ static void destroy_ctx(cbm_ctx* c) {
   F(c->gr_new); F(c->vb_new); F(c->dead);
   F(c->n_add_slot); F(c->dead_slot); F(c->off_add); F(c->off_dead); F(c->holes);
   F(c->med_c[0]); F(c->med_c[1]); F(c->med_count);
-  F(c->key); F(c->lin); F(c->crank); F(c->glin); F(c->cell_range); F(c->count); F(c->cursor); F(c->start); F(c->tmp); F(c->cell_row);
-  F(c->scan_status); F(c->scan_ticket); F(c->list); F(c->list_n); F(c->flags); F(c->counters);
+  F(c->key); F(c->lin); F(c->gbucket); F(c->cell_range); F(c->count); F(c->cursor); F(c->start); F(c->tmp); F(c->cell_row);
+  F(c->scan_status); F(c->scan_ticket); F(c->list); F(c->list_n); F(c->bricks); F(c->flags); F(c->counters);
   for (cbm_ctx::DevBuf* b : {&c->adj_tgt, &c->adj_tbase, &c->adj_invn, &c->adj_ck, &c->adj_seg, &c->adj_lam,
                              &c->adj_gpart, &c->adj_lpart}) F(b->p);
   for (int32_t* p : c->snap_l) F(p);
@@ -459,19 +545,18 @@ static int init_ctx(cbm_ctx* c) {
   // neighbor engine
   HIPCHK(c, dalloc(&c->key, c->C));
   HIPCHK(c, dalloc(&c->lin, c->C));
-  HIPCHK(c, dalloc(&c->crank, c->NR));
-  HIPCHK(c, hipMemcpy(c->crank, c->h_crank.data(), sizeof(int32_t) * c->NR, hipMemcpyHostToDevice));
   {
-    std::vector<int32_t> gl((size_t)c->NB, -1);
+    std::vector<int32_t> gb((size_t)c->NR);
     for (int s = 0; s < c->S; ++s) {
       const SimDev& sd = c->h_sims[s];
       for (int64_t l = 0; l < sd.ncells; ++l)
-        gl[(size_t)(sd.cell_off + c->h_crank[(size_t)(sd.rank_off + l)])] = (int32_t)(sd.rank_off + l);
+        gb[(size_t)(sd.rank_off + l)] = (int32_t)(sd.cell_off + c->h_crank[(size_t)(sd.rank_off + l)]);
     }
-    HIPCHK(c, dalloc(&c->glin, c->NB));
-    HIPCHK(c, hipMemcpy(c->glin, gl.data(), sizeof(int32_t) * c->NB, hipMemcpyHostToDevice));
+    HIPCHK(c, dalloc(&c->gbucket, c->NR));
+    HIPCHK(c, hipMemcpy(c->gbucket, gb.data(), sizeof(int32_t) * c->NR, hipMemcpyHostToDevice));
     HIPCHK(c, dalloc(&c->cell_range, c->NR));
   }
+  if (c->cl && (rc = setup_bricks(c))) return rc;
   c->h_crank.clear();
   c->h_crank.shrink_to_fit();
   HIPCHK(c, dalloc(&c->tmp, c->C));
@@ -533,7 +618,8 @@ static RebuildArgs rebuild_args(cbm_ctx* c, const int32_t* flag, int32_t* flag_c
   a.dead = c->dead;
   a.slot = (const int32_t*)c->col[C_SLOT].cur();
   a.range = c->cell_range;
-  a.key = c->key; a.lin = c->lin; a.crank = c->crank; a.count = c->count; a.cursor = c->cursor; a.start = c->start; a.tmp = c->tmp;
+  a.nr = c->NR;
+  a.key = c->key; a.lin = c->lin; a.gbucket = c->gbucket; a.count = c->count; a.cursor = c->cursor; a.start = c->start; a.tmp = c->tmp;
   a.cell_row = c->cell_row;
   a.dims = c->dims; a.periodic = c->m.periodic; a.full = c->full ? 1 : 0;
   a.geometric = c->verlet ? 1 : 0;
@@ -546,8 +632,7 @@ static RebuildArgs rebuild_args(cbm_ctx* c, const int32_t* flag, int32_t* flag_c
 // current buffers); otherwise the alternate buffers become current.
 // Exclusive scan of n int32 counts (see k_scan); refuses a scan with more
 // tiles than the status array holds.
-static int run_scan(cbm_ctx* c, const int32_t* flag, int32_t* count, int32_t* cursor, int32_t* start, int64_t n,
-                    bool cells = false) {
+static int run_scan(cbm_ctx* c, const int32_t* flag, int32_t* count, int32_t* cursor, int32_t* start, int64_t n) {
   if (scan_blocks(n) > c->scan_tiles)
     return fail(c, CBM_ERR_ARG, "scan longer than its status array");
   ScanArgs s;
@@ -555,7 +640,6 @@ static int run_scan(cbm_ctx* c, const int32_t* flag, int32_t* count, int32_t* cu
   s.flag = flag; s.count = count; s.cursor = cursor; s.start = start; s.n = n;
   s.status = c->scan_status; s.ticket = c->scan_ticket; s.done = c->scan_ticket + 1;
   s.epoch = c->scan_ticket + 2; s.nblocks = scan_blocks(n);
-  if (cells) { s.glin = c->glin; s.range = c->cell_range; }
   HIPCHK(c, launch_scan(c->stream, s));
   return CBM_OK;
 }
@@ -563,7 +647,8 @@ static int run_scan(cbm_ctx* c, const int32_t* flag, int32_t* count, int32_t* cu
 static int rebuild_chain(cbm_ctx* c, const int32_t* flag, int32_t* flag_clear, bool tick, bool copy_back) {
   RebuildArgs a = rebuild_args(c, flag, flag_clear, tick);
   HIPCHK(c, launch_keys(c->nblocks, c->block, c->stream, a));
-  int rc = run_scan(c, flag, c->count, c->cursor, c->start, c->NB, true);
+  int rc = run_scan(c, flag, c->count, c->cursor, c->start, c->NB);
+  if (!rc && !c->full) HIPCHK(c, launch_ranges(c->stream, a));
   if (rc) return rc;
   HIPCHK(c, launch_scatter(c->nblocks, c->block, c->stream, a));
   PermuteArgs pa;
@@ -614,6 +699,11 @@ static StageArgs stage_base(cbm_ctx* c) {
   a.id = (const int64_t*)c->col[C_ID].cur();
   a.list = c->list; a.list_n = c->list_n; a.ldl = c->C;
   a.cell_range = c->cell_range; a.cell_row = c->cell_row;
+  a.brick = c->bricks;
+  a.bucket_start = c->start;
+  a.brick_over = (unsigned long long*)(c->counters + 2);
+  for (int q = 0; q < 3; ++q) a.bdim[q] = c->bdim[q];
+  a.brick_cap = c->brick_cap;
   a.dt = c->m.dt; a.sqdt = std::sqrt(c->m.dt); a.drift = c->m.drift; a.noise = c->m.noise; a.thr2 = sq_threshold(c->thr);
   a.seed = c->m.seed;
   a.step_ptr = c->counters + 1;
@@ -646,12 +736,14 @@ static void prof_end(cbm_ctx* c);
 // module of a user law.
 static hipError_t stage_launch(cbm_ctx* c, const StageArgs& a) {
   if (c->m.law != CBM_LAW_CUSTOM)
-    return launch_stage(c->dims, c->nb_kind, c->m.law, c->nblocks, c->block, c->stream, a);
+    return launch_stage(c->dims, c->nb_kind, c->m.law, c->nblocks, c->block, c->stream, a, c->nbricks, c->brick_lds);
   hipFunction_t f = c->user_fn[(a.kind == ST_EM || a.kind == ST_EH1 || a.kind == ST_EH2) ? 1 : 0];
   if (!f) return hipErrorInvalidDeviceFunction;
   StageArgs copy = a;
   void* params[] = {(void*)&copy};
-  return hipModuleLaunchKernel(f, (unsigned)c->nblocks, 1, 1, (unsigned)c->block, 1, 1, 0, c->stream, params, nullptr);
+  const bool brick = c->nb_kind == NB_CELL;
+  return hipModuleLaunchKernel(f, brick ? (unsigned)c->nbricks : (unsigned)c->nblocks, 1, 1, (unsigned)c->block, 1, 1,
+                               brick ? (unsigned)c->brick_lds : 0u, c->stream, params, nullptr);
 }
 
 static int run_stage(cbm_ctx* c, StageArgs& a, int kind) {
@@ -893,10 +985,17 @@ int cbm_compile(cbm_ctx* c, const char* body) {
   src += "\n}  // namespace cbm\n";
   char inst[512];
   for (int noise = 0; noise < 2; ++noise) {
-    std::snprintf(inst, sizeof(inst),
-                  "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_stage_%d(cbm::StageArgs a) {\n"
-                  "  cbm::stage_body<%d, %d, cbm::UserLaw, %d, %d>(a);\n}\n",
-                  noise, c->dims, c->nb_kind, c->m.periodic ? 1 : 0, noise);
+    if (c->nb_kind == NB_CELL)   // the brick-tiled cell-linked stage
+      std::snprintf(inst, sizeof(inst),
+                    "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_stage_%d(cbm::StageArgs a) {\n"
+                    "  extern __shared__ unsigned char lds[];\n"
+                    "  cbm::brick_body<%d, cbm::UserLaw, %d, %d>(a, lds);\n}\n",
+                    noise, c->dims, c->m.periodic ? 1 : 0, noise);
+    else
+      std::snprintf(inst, sizeof(inst),
+                    "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_stage_%d(cbm::StageArgs a) {\n"
+                    "  cbm::stage_body<%d, %d, cbm::UserLaw, %d, %d>(a);\n}\n",
+                    noise, c->dims, c->nb_kind, c->m.periodic ? 1 : 0, noise);
     src += inst;
   }
   const char* headers[] = {cbm_rtc_cbm_device_h, cbm_rtc_cbm_structs_h, cbm_rtc_cbm_stage_h};
@@ -1125,6 +1224,18 @@ int cbm_get_t(cbm_ctx* c, double* t, int64_t* step) {
   return CBM_OK;
 }
 
+int cbm_get_bricks(cbm_ctx* c, int32_t* bdim, int32_t* cap, int64_t* nbricks, int64_t* overflows) {
+  if (!c || !bdim || !cap || !nbricks || !overflows) return CBM_ERR_ARG;
+  int64_t v[4];
+  HIPCHK(c, hipMemcpyAsync(v, c->counters, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int a = 0; a < 3; ++a) bdim[a] = c->cl ? c->bdim[a] : 0;
+  *cap = c->cl ? c->brick_cap : 0;
+  *nbricks = c->cl ? c->nbricks : 0;
+  *overflows = v[2];
+  return CBM_OK;
+}
+
 int cbm_get_rebuilds(cbm_ctx* c, int64_t* rebuilds) {
   if (!c || !rebuilds) return CBM_ERR_ARG;
   int64_t v[4];
@@ -1201,7 +1312,8 @@ static int rebuild_in_place(cbm_ctx* c, double* const* X, const int32_t* flag,
   for (int d = 0; d < c->dims; ++d) a.pos[d] = X[d];
   a.dead = nullptr;
   HIPCHK(c, launch_keys(c->nblocks, c->block, c->stream, a));
-  int rc = run_scan(c, flag, c->count, c->cursor, c->start, c->NB, true);
+  int rc = run_scan(c, flag, c->count, c->cursor, c->start, c->NB);
+  if (!rc && !c->full) HIPCHK(c, launch_ranges(c->stream, a));
   if (rc) return rc;
   HIPCHK(c, launch_scatter(c->nblocks, c->block, c->stream, a));
   VerletArgs va;

@@ -97,6 +97,7 @@ template <> struct PairLaw<LAW_LINEAR> {
     s = k * (c - d);
     return d > 0.0 && d < c;
   }
+  __device__ __forceinline__ double reach() const { return c; }
 };
 
 // c = 2 r_eq, dc = f r_eq, a4 = 4 eps / (dc - c)^2:
@@ -117,6 +118,7 @@ template <> struct PairLaw<LAW_REP_ADH> {
     s = d < c ? rep : adh;
     return d > 0.0 && d < dc;
   }
+  __device__ __forceinline__ double reach() const { return dc; }
 };
 
 // docs/src/Aggregation.md:88-98:
@@ -131,6 +133,7 @@ template <> struct PairLaw<LAW_AGGREGATION> {
     s = d < rRep ? rep : atr;
     return d > 0.0 && (d < rRep || d < rAtr);
   }
+  __device__ __forceinline__ double reach() const { return fmax(rRep, rAtr); }
 };
 
 // Derivatives of phi(d) = s(d)/d of PairLaw<LAW_REP_ADH> for the adjoint:

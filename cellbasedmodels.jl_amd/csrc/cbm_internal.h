@@ -31,13 +31,14 @@ struct RebuildArgs {
   const int32_t* slot;        // per-row logical slot (current)
   int32_t* key;               // per-row bucket
   int32_t* lin;               // per-row linear local cell
-  const int32_t* crank;       // linear local cell -> bucket rank (Morton order)
+  const int32_t* gbucket;     // rank_off + linear local cell -> global bucket (cell_off + Morton rank)
   int32_t* count;             // per-bucket count
   int32_t* cursor;            // per-bucket scatter cursor
   int32_t* start;             // per-bucket first row (exclusive scan), nb+1 entries
   int32_t* tmp;               // bucket-ordered source rows
   int32_t* cell_row;          // destination row -> linear local cell
-  const CellRange* range;     // rank_off + linear local cell -> rows, written by the scan
+  CellRange* range;           // rank_off + linear local cell -> rows, written by k_ranges
+  int64_t nr;                 // entries of gbucket / range (all simulations)
   int32_t dims, periodic, full, geometric;
 };
 
@@ -51,8 +52,6 @@ struct ScanArgs {
   unsigned int* ticket;
   unsigned int* done;
   unsigned int* epoch;        // device launch epoch (>= 1)
-  const int32_t* glin;        // bucket -> rank_off + linear cell, -1 for non-cell buckets
-  CellRange* range;           // written for cell buckets when not NULL
   int32_t nblocks;
 };
 
@@ -209,9 +208,12 @@ constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 inline int scan_blocks(int64_t n) { return (int)((n + SCAN_TILE - 1) / SCAN_TILE); }
 
-hipError_t launch_stage(int dims, int nb, int law, int nblocks, int block, hipStream_t s, const StageArgs& a);
+// NB_CELL launches one block per brick (nbricks) with brick_lds bytes of LDS
+hipError_t launch_stage(int dims, int nb, int law, int nblocks, int block, hipStream_t s, const StageArgs& a,
+                        int64_t nbricks, size_t brick_lds);
 hipError_t launch_keys(int nblocks, int block, hipStream_t s, const RebuildArgs& a);
 hipError_t launch_scan(hipStream_t s, const ScanArgs& a);
+hipError_t launch_ranges(hipStream_t s, const RebuildArgs& a);
 hipError_t launch_scatter(int nblocks, int block, hipStream_t s, const RebuildArgs& a);
 hipError_t launch_permute(int nblocks, int block, hipStream_t s, const PermuteArgs& a);
 hipError_t launch_verlet(int dims, int nblocks, int block, hipStream_t s, const VerletArgs& a);

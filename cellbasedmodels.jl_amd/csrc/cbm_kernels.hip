@@ -4,8 +4,12 @@
 //                      (the generated agentODE kernel of
 //                      src/AgentStructure/functionDE.jl:92-108 fused with the
 //                      CBMIntegrators broadcasts, src/integrators.jl:465-734)
+//   k_brick<D,LAW>     the same stage for the cell-linked walk on LDS tiles
+//                      of Morton-aligned cell bricks (cellPosNeigh order,
+//                      src/neighbors.jl:641-700)
 //   k_keys             cell key + bucket histogram   (assignCells, src/neighbors.jl:763-821)
 //   k_scan             single-pass exclusive scan    (cumsum, src/neighbors.jl:934-950)
+//   k_ranges           per-cell row ranges from the bucket starts
 //   k_scatter          bucket scatter                (sortAgentsInCells, src/neighbors.jl:837-899)
 //   k_permute          deterministic in-cell order + column permutation
 //   k_verlet           Verlet list build from cells (src/neighbors.jl:92-165, 1071-1183)
@@ -47,6 +51,13 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   stage_body<D, NB, PairLaw<LAW>, PER, NOISE>(a);
 }
 
+// cell-linked stage on LDS brick tiles (dynamic LDS: brick_lds_bytes)
+template <int D, int LAW, int PER, int NOISE>
+__global__ __launch_bounds__(256) void k_brick(StageArgs a) {
+  extern __shared__ unsigned char brick_lds[];
+  brick_body<D, PairLaw<LAW>, PER, NOISE>(a, brick_lds);
+}
+
 // --------------------------------------------------------- rebuild chain
 
 __global__ __launch_bounds__(256) void k_keys(RebuildArgs a) {
@@ -65,15 +76,14 @@ __global__ __launch_bounds__(256) void k_keys(RebuildArgs a) {
   if (ri.local >= st.n_tot) return;
   const int64_t r = ri.row;
   int32_t c, lin = 0;
-  if (a.dead && a.dead[r]) c = (int32_t)sd.ncells;
-  else if (a.full) c = 0;
+  if (a.dead && a.dead[r]) c = (int32_t)(sd.cell_off + sd.ncells);
+  else if (a.full) c = (int32_t)sd.cell_off;
   else {
     int32_t q[3] = {0, 0, 0};
     for (int d = 0; d < a.dims; ++d) q[d] = axis_cell(sd, d, a.pos[d][r], a.periodic, a.geometric);
     lin = q[0] + sd.gn[0] * (q[1] + sd.gn[1] * q[2]);
-    c = a.crank[sd.rank_off + lin];          // bucket = Morton rank of the cell
+    c = a.gbucket[sd.rank_off + lin];        // cell_off + Morton rank of the cell
   }
-  c += (int32_t)sd.cell_off;
   a.key[r] = c;
   a.lin[r] = lin;
   atomicAdd(&a.count[c], 1);
@@ -104,14 +114,22 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
   __syncthreads();
   const int64_t tile = s_tile;
   const int64_t base = tile * SCAN_TILE + (int64_t)tid * SCAN_ITEMS;
+  // a thread's items are contiguous: whole chunks move as 16-byte vectors
+  const bool whole = base + SCAN_ITEMS <= a.n;
   int32_t v[SCAN_ITEMS];
   int32_t sum = 0;
+  if (whole) {
 #pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; ++k) {
-    const int64_t idx = base + k;
-    v[k] = idx < a.n ? a.count[idx] : 0;
-    sum += v[k];
+    for (int k = 0; k < SCAN_ITEMS; k += 4) {
+      const int4 q = *(const int4*)(a.count + base + k);
+      v[k] = q.x; v[k + 1] = q.y; v[k + 2] = q.z; v[k + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) v[k] = base + k < a.n ? a.count[base + k] : 0;
   }
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) sum += v[k];
   // wavefront inclusive scan of per-thread sums
   const int lane = tid & 63, wave = tid >> 6;
   int32_t incl = sum;
@@ -168,19 +186,31 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
   }
   __syncthreads();
   int32_t run = s_prefix + wave_off + (incl - sum);
+  if (whole) {
+    const int4 z = make_int4(0, 0, 0, 0);
 #pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; ++k) {
-    const int64_t idx = base + k;
-    if (idx < a.n) {
-      a.start[idx] = run;
-      if (a.range) {
-        const int32_t g = a.glin[idx];
-        if (g >= 0) a.range[g] = CellRange{run, run + v[k]};
+    for (int k = 0; k < SCAN_ITEMS; k += 4) {
+      int4 q;
+      q.x = run; run += v[k];
+      q.y = run; run += v[k + 1];
+      q.z = run; run += v[k + 2];
+      q.w = run; run += v[k + 3];
+      *(int4*)(a.start + base + k) = q;
+      *(int4*)(a.count + base + k) = z;
+      if (a.cursor) *(int4*)(a.cursor + base + k) = z;
+    }
+    if (base + SCAN_ITEMS == a.n) a.start[a.n] = run;
+  } else {
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+      const int64_t idx = base + k;
+      if (idx < a.n) {
+        a.start[idx] = run;
+        a.count[idx] = 0;
+        if (a.cursor) a.cursor[idx] = 0;
+        run += v[k];
+        if (idx == a.n - 1) a.start[a.n] = run;
       }
-      a.count[idx] = 0;
-      if (a.cursor) a.cursor[idx] = 0;
-      run += v[k];
-      if (idx == a.n - 1) a.start[a.n] = run;
     }
   }
   if (tid == 0) {
@@ -193,6 +223,16 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
       __hip_atomic_store(a.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
+}
+
+// Rows [s, e) of every grid cell by linear cell (all simulations), gathered
+// from the bucket starts; written in table order, so the stores coalesce.
+__global__ __launch_bounds__(256) void k_ranges(RebuildArgs a) {
+  if (a.flag && *a.flag == 0) return;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.nr) return;
+  const int32_t b = a.gbucket[g];
+  a.range[g] = CellRange{a.start[b], a.start[b + 1]};
 }
 
 __global__ __launch_bounds__(256) void k_scatter(RebuildArgs a) {
@@ -849,44 +889,61 @@ __global__ __launch_bounds__(256) void k_export(ExportArgs e) {
 }
 
 // ------------------------------------------------------------- launchers
+struct StageLaunch { dim3 grid, block; size_t lds; hipStream_t s; };
+
 template <int D, int NB, int PER, int NOISE>
-static hipError_t launch_stage_law(int law, dim3 grid, dim3 block, hipStream_t s, const StageArgs& a) {
+static hipError_t launch_stage_law(int law, const StageLaunch& L, const StageArgs& a) {
+  if (NB == NB_CELL) {
+    switch (law) {
+      case LAW_LINEAR: hipLaunchKernelGGL((k_brick<D, LAW_LINEAR, PER, NOISE>), L.grid, L.block, L.lds, L.s, a); break;
+      case LAW_REP_ADH: hipLaunchKernelGGL((k_brick<D, LAW_REP_ADH, PER, NOISE>), L.grid, L.block, L.lds, L.s, a); break;
+      case LAW_AGGREGATION: hipLaunchKernelGGL((k_brick<D, LAW_AGGREGATION, PER, NOISE>), L.grid, L.block, L.lds, L.s, a); break;
+      case LAW_SOFT_SPHERES: hipLaunchKernelGGL((k_brick<D, LAW_SOFT_SPHERES, PER, NOISE>), L.grid, L.block, L.lds, L.s, a); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
   switch (law) {
-    case LAW_LINEAR: hipLaunchKernelGGL((k_stage<D, NB, LAW_LINEAR, PER, NOISE>), grid, block, 0, s, a); break;
-    case LAW_REP_ADH: hipLaunchKernelGGL((k_stage<D, NB, LAW_REP_ADH, PER, NOISE>), grid, block, 0, s, a); break;
-    case LAW_AGGREGATION: hipLaunchKernelGGL((k_stage<D, NB, LAW_AGGREGATION, PER, NOISE>), grid, block, 0, s, a); break;
-    case LAW_SOFT_SPHERES: hipLaunchKernelGGL((k_stage<D, NB, LAW_SOFT_SPHERES, PER, NOISE>), grid, block, 0, s, a); break;
+    case LAW_LINEAR: hipLaunchKernelGGL((k_stage<D, NB, LAW_LINEAR, PER, NOISE>), L.grid, L.block, 0, L.s, a); break;
+    case LAW_REP_ADH: hipLaunchKernelGGL((k_stage<D, NB, LAW_REP_ADH, PER, NOISE>), L.grid, L.block, 0, L.s, a); break;
+    case LAW_AGGREGATION: hipLaunchKernelGGL((k_stage<D, NB, LAW_AGGREGATION, PER, NOISE>), L.grid, L.block, 0, L.s, a); break;
+    case LAW_SOFT_SPHERES: hipLaunchKernelGGL((k_stage<D, NB, LAW_SOFT_SPHERES, PER, NOISE>), L.grid, L.block, 0, L.s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 template <int D, int PER, int NOISE>
-static hipError_t launch_stage_nb(int nb, int law, dim3 grid, dim3 block, hipStream_t s, const StageArgs& a) {
+static hipError_t launch_stage_nb(int nb, int law, const StageLaunch& L, const StageArgs& a) {
   switch (nb) {
-    case NB_FULL: return launch_stage_law<D, NB_FULL, PER, NOISE>(law, grid, block, s, a);
-    case NB_CELL: return launch_stage_law<D, NB_CELL, PER, NOISE>(law, grid, block, s, a);
-    default: return launch_stage_law<D, NB_LIST, PER, NOISE>(law, grid, block, s, a);
+    case NB_FULL: return launch_stage_law<D, NB_FULL, PER, NOISE>(law, L, a);
+    case NB_CELL: return launch_stage_law<D, NB_CELL, PER, NOISE>(law, L, a);
+    default: return launch_stage_law<D, NB_LIST, PER, NOISE>(law, L, a);
   }
 }
 
 template <int D, int NOISE>
-static hipError_t launch_stage_dims(int nb, int law, int periodic, dim3 grid, dim3 block, hipStream_t s, const StageArgs& a) {
+static hipError_t launch_stage_dims(int nb, int law, int periodic, const StageLaunch& L, const StageArgs& a) {
   if (law == LAW_NONE) {      // no interaction body: no neighbor walk at all
-    hipLaunchKernelGGL((k_stage<D, NB_FULL, LAW_NONE, 0, NOISE>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_stage<D, NB_FULL, LAW_NONE, 0, NOISE>), L.grid, L.block, 0, L.s, a);
     return hipGetLastError();
   }
-  return periodic ? launch_stage_nb<D, 1, NOISE>(nb, law, grid, block, s, a)
-                  : launch_stage_nb<D, 0, NOISE>(nb, law, grid, block, s, a);
+  return periodic ? launch_stage_nb<D, 1, NOISE>(nb, law, L, a) : launch_stage_nb<D, 0, NOISE>(nb, law, L, a);
 }
 
-hipError_t launch_stage(int dims, int nb, int law, int nblocks, int block, hipStream_t s, const StageArgs& a) {
-  dim3 grid(nblocks), blk(block);
+hipError_t launch_stage(int dims, int nb, int law, int nblocks, int block, hipStream_t s, const StageArgs& a,
+                        int64_t nbricks, size_t brick_lds) {
+  StageLaunch L;
+  const bool brick = nb == NB_CELL && law != LAW_NONE;
+  L.grid = dim3(brick ? (unsigned)nbricks : (unsigned)nblocks);
+  L.block = dim3(block);
+  L.lds = brick ? brick_lds : 0;
+  L.s = s;
   const bool noise = a.kind == ST_EM || a.kind == ST_EH1 || a.kind == ST_EH2;
   switch (dims) {
-    case 1: return noise ? launch_stage_dims<1, 1>(nb, law, a.periodic, grid, blk, s, a) : launch_stage_dims<1, 0>(nb, law, a.periodic, grid, blk, s, a);
-    case 2: return noise ? launch_stage_dims<2, 1>(nb, law, a.periodic, grid, blk, s, a) : launch_stage_dims<2, 0>(nb, law, a.periodic, grid, blk, s, a);
-    default: return noise ? launch_stage_dims<3, 1>(nb, law, a.periodic, grid, blk, s, a) : launch_stage_dims<3, 0>(nb, law, a.periodic, grid, blk, s, a);
+    case 1: return noise ? launch_stage_dims<1, 1>(nb, law, a.periodic, L, a) : launch_stage_dims<1, 0>(nb, law, a.periodic, L, a);
+    case 2: return noise ? launch_stage_dims<2, 1>(nb, law, a.periodic, L, a) : launch_stage_dims<2, 0>(nb, law, a.periodic, L, a);
+    default: return noise ? launch_stage_dims<3, 1>(nb, law, a.periodic, L, a) : launch_stage_dims<3, 0>(nb, law, a.periodic, L, a);
   }
 }
 
@@ -897,6 +954,11 @@ hipError_t launch_keys(int nblocks, int block, hipStream_t s, const RebuildArgs&
 
 hipError_t launch_scan(hipStream_t s, const ScanArgs& a) {
   hipLaunchKernelGGL(k_scan, dim3(a.nblocks), dim3(SCAN_THREADS), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_ranges(hipStream_t s, const RebuildArgs& a) {
+  hipLaunchKernelGGL(k_ranges, dim3((unsigned)((a.nr + 255) / 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

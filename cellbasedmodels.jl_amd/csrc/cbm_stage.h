@@ -87,112 +87,126 @@ __device__ __forceinline__ double wrap_dx(double d, double L, double hL) {
   return d > hL ? d - L : (d < -hL ? d + L : d);
 }
 
-// One integrator stage for every agent row: the agentODE body over the
-// neighbor candidates (Law = the pair law functor) fused with the stage
-// update.  Built-in laws instantiate it in libcbm_hip.so; user laws are
-// compiled around it with hipRTC.
-template <int D, int NB, class Law, int PER, int NOISE>
-__device__ __forceinline__ void stage_body(const StageArgs& a) {
-  const RowIdx ri = row_index(a.g);
-  const SimDev& sd = a.g.sims[ri.sim];
-  const int64_t N = a.g.st[ri.sim].N;
-  if (ri.local >= N) return;
-  const int64_t i = ri.row;
-
-  double xi[3] = {0.0, 0.0, 0.0};
+// One neighbor at position xj: the agentODE interaction body, added to acc
+// in the reference's per-axis order.  live = false (list padding, or the
+// agent itself) leaves acc unchanged.
+template <int D, class Law, int PER, bool SOFT>
+__device__ __forceinline__ void pair_into(const Law& law, const double* xi, const double* xj, const double* L,
+                                          const double* hL, double rad_i, double rad_j, int64_t fate_i,
+                                          int64_t fate_j, bool live, double* acc) {
+  double dx[3];
 #pragma unroll
-  for (int c = 0; c < D; ++c) xi[c] = a.S[c][i];
-  double acc[3] = {0.0, 0.0, 0.0};
-  // the inertial soft-sphere model: per-agent radius and fate in the pair
-  // term, velocities as DE variables
+  for (int c = 0; c < D; ++c) {
+    dx[c] = xi[c] - xj[c];
+    if (PER) dx[c] = wrap_dx(dx[c], L[c], hL[c]);
+  }
+  double s2 = dx[0] * dx[0];
+  if (D > 1) s2 = s2 + dx[1] * dx[1];
+  if (D > 2) s2 = s2 + dx[2] * dx[2];
+  const double d = sqrt(s2);
+  double s;
+  bool on;
+  if constexpr (SOFT) on = law.pair(d, rad_i, rad_j, fate_i, fate_j, s) && live;
+  else on = law(d, s) && live;
+  if constexpr (SOFT) {
+    // ((s dx) / d) per axis, the reference's order (softSpheres.jl:28-29)
+#pragma unroll
+    for (int c = 0; c < D; ++c) { const double v = acc[c] + (s * dx[c]) / d; acc[c] = on ? v : acc[c]; }
+  } else if (Law::per_axis_division) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) { const double v = acc[c] + s * (dx[c] / d); acc[c] = on ? v : acc[c]; }
+  } else {
+    // F = (s/d) (x_i - x_j).  An inactive pair adds 0*dx = +-0, which
+    // leaves acc unchanged: acc starts at +0 and a sum is -0 only when
+    // both terms are, so acc is never -0.
+    const double f = s / d;
+    const double fe = on ? f : 0.0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) acc[c] = acc[c] + fe * dx[c];
+  }
+}
+
+template <int D, int PER>
+__device__ __forceinline__ void box_halves(const SimDev& sd, double* L, double* hL) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) { L[c] = 0.0; hL[c] = 0.0; }
+  if (PER) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) { L[c] = sd.L[c]; hL[c] = 0.5 * sd.L[c]; }
+  }
+}
+
+// Pair sum of row i over its candidates in global memory (Verlet list,
+// all rows, or the cell-linked walk), in reference loop order.
+template <int D, int NB, class Law, int PER>
+__device__ __forceinline__ void gather_global(const StageArgs& a, const SimDev& sd, int64_t N, int64_t i,
+                                              const double* xi, double* acc) {
   constexpr bool SOFT = same_type<Law, PairLaw<LAW_SOFT_SPHERES>>::value;
+  const Law law(sd.p);
+  const double rad_i = SOFT ? a.rad[i] : 0.0;
+  const int64_t fate_i = SOFT ? a.fate[i] : 0;
+  double L[3], hL[3];
+  box_halves<D, PER>(sd, L, hL);
+  auto body = [&](const double* xj, int64_t j, bool live) {
+    pair_into<D, Law, PER, SOFT>(law, xi, xj, L, hL, rad_i, SOFT ? a.rad[j] : 0.0, fate_i,
+                                 SOFT ? a.fate[j] : 0, live, acc);
+  };
+  if (NB == NB_LIST) {
+    // Verlet list in chunks of LCH gathers issued together (memory-level
+    // parallelism per lane), branch-free: slots past the count read the
+    // agent itself (d = 0, never active) and add nothing.  The list
+    // indices of the next chunk are loaded before the current chunk's
+    // positions are used.
+    constexpr int LCH = CBM_LIST_CHUNK;
+    const int32_t cnt = a.list_n[i];
+    const int64_t ldl = a.ldl;
+    const int32_t* __restrict__ lp = a.list + i;     // entry q at lp[q * ldl]
+    int32_t jn[LCH];
+#pragma unroll
+    for (int u = 0; u < LCH; ++u) jn[u] = (u < cnt) ? lp[(int64_t)u * ldl] : (int32_t)i;
+    for (int32_t q0 = 0; q0 < cnt; q0 += LCH) {
+      double px[LCH][3];
+      int32_t jc[LCH];
+#pragma unroll
+      for (int u = 0; u < LCH; ++u) {
+        jc[u] = jn[u];
+#pragma unroll
+        for (int c = 0; c < D; ++c) px[u][c] = a.S[c][jc[u]];
+      }
+      lp += LCH * ldl;
+#pragma unroll
+      for (int u = 0; u < LCH; ++u) jn[u] = (q0 + LCH + u < cnt) ? lp[(int64_t)u * ldl] : (int32_t)i;
+#pragma unroll
+      for (int u = 0; u < LCH; ++u) body(px[u], jc[u], q0 + u < cnt);
+    }
+  } else {
+    for_candidates<D, NB>(a, sd, N, i, [&](int64_t j) {
+      double xj[3];
+#pragma unroll
+      for (int c = 0; c < D; ++c) xj[c] = a.S[c][j];
+      body(xj, j, true);
+    });
+  }
+}
 
-  if (Law::active) {
-    const Law law(sd.p);
-    const double rad_i = SOFT ? a.rad[i] : 0.0;
-    const int64_t fate_i = SOFT ? a.fate[i] : 0;
-    double L[3] = {0.0, 0.0, 0.0}, hL[3] = {0.0, 0.0, 0.0};
-    if (PER) {
+// The rest of the stage for row i once the pair sum acc is known: walls,
+// the derivative of every DE variable, the integrator update, periodic
+// wrap, the output write and the displacement check.
+template <int D, class Law, int NOISE>
+__device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& sd, int sim, int64_t i,
+                                             const double* xi, double* acc, const double* upre = nullptr) {
+  constexpr bool SOFT = same_type<Law, PairLaw<LAW_SOFT_SPHERES>>::value;
+  // u of variable c (positions possibly loaded early by the caller), and the
+  // stage state, whose positions are xi
+  auto Uc = [&](int c) { return (upre && c < D) ? upre[c] : a.U[c][i]; };
+  auto Sc = [&](int c) { return c < D ? xi[c] : a.S[c][i]; };
+  if (Law::active && Law::walls && a.walls) {
+    const double rRep = sd.p[0], fRep = sd.p[1];
 #pragma unroll
-      for (int c = 0; c < D; ++c) { L[c] = sd.L[c]; hL[c] = 0.5 * sd.L[c]; }
-    }
-    // one neighbor at position xj (the agentODE interaction body)
-    auto body = [&](const double* xj, int64_t j, bool live) {
-      double dx[3];
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        dx[c] = xi[c] - xj[c];
-        if (PER) dx[c] = wrap_dx(dx[c], L[c], hL[c]);
-      }
-      double s2 = dx[0] * dx[0];
-      if (D > 1) s2 = s2 + dx[1] * dx[1];
-      if (D > 2) s2 = s2 + dx[2] * dx[2];
-      const double d = sqrt(s2);
-      double s;
-      bool on;
-      if constexpr (SOFT) on = law.pair(d, rad_i, a.rad[j], fate_i, a.fate[j], s) && live;
-      else on = law(d, s) && live;
-      if constexpr (SOFT) {
-        // ((s dx) / d) per axis, the reference's order (softSpheres.jl:28-29)
-#pragma unroll
-        for (int c = 0; c < D; ++c) { const double v = acc[c] + (s * dx[c]) / d; acc[c] = on ? v : acc[c]; }
-      } else if (Law::per_axis_division) {
-#pragma unroll
-        for (int c = 0; c < D; ++c) { const double v = acc[c] + s * (dx[c] / d); acc[c] = on ? v : acc[c]; }
-      } else {
-        // F = (s/d) (x_i - x_j).  An inactive pair adds 0*dx = +-0, which
-        // leaves acc unchanged: acc starts at +0 and a sum is -0 only when
-        // both terms are, so acc is never -0.
-        const double f = s / d;
-        const double fe = on ? f : 0.0;
-#pragma unroll
-        for (int c = 0; c < D; ++c) acc[c] = acc[c] + fe * dx[c];
-      }
-    };
-    if (NB == NB_LIST) {
-      // Verlet list in chunks of LCH gathers issued together (memory-level
-      // parallelism per lane), branch-free: slots past the count read the
-      // agent itself (d = 0, never active) and add nothing.  The list
-      // indices of the next chunk are loaded before the current chunk's
-      // positions are used.
-      constexpr int LCH = CBM_LIST_CHUNK;
-      const int32_t cnt = a.list_n[i];
-      const int64_t ldl = a.ldl;
-      const int32_t* __restrict__ lp = a.list + i;     // entry q at lp[q * ldl]
-      int32_t jn[LCH];
-#pragma unroll
-      for (int u = 0; u < LCH; ++u) jn[u] = (u < cnt) ? lp[(int64_t)u * ldl] : (int32_t)i;
-      for (int32_t q0 = 0; q0 < cnt; q0 += LCH) {
-        double px[LCH][3];
-        int32_t jc[LCH];
-#pragma unroll
-        for (int u = 0; u < LCH; ++u) {
-          jc[u] = jn[u];
-#pragma unroll
-          for (int c = 0; c < D; ++c) px[u][c] = a.S[c][jc[u]];
-        }
-        lp += LCH * ldl;
-#pragma unroll
-        for (int u = 0; u < LCH; ++u) jn[u] = (q0 + LCH + u < cnt) ? lp[(int64_t)u * ldl] : (int32_t)i;
-#pragma unroll
-        for (int u = 0; u < LCH; ++u) body(px[u], jc[u], q0 + u < cnt);
-      }
-    } else {
-      for_candidates<D, NB>(a, sd, N, i, [&](int64_t j) {
-        double xj[3];
-#pragma unroll
-        for (int c = 0; c < D; ++c) xj[c] = a.S[c][j];
-        body(xj, j, true);
-      });
-    }
-    if (Law::walls && a.walls) {
-      const double rRep = sd.p[0], fRep = sd.p[1];
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        const double x = xi[c];
-        if (x < sd.lo[c] + rRep / 2) acc[c] = acc[c] + fRep;
-        else if (x > sd.hi[c] - rRep / 2) acc[c] = acc[c] - fRep;
-      }
+    for (int c = 0; c < D; ++c) {
+      const double x = xi[c];
+      if (x < sd.lo[c] + rRep / 2) acc[c] = acc[c] + fRep;
+      else if (x > sd.hi[c] - rRep / 2) acc[c] = acc[c] - fRep;
     }
   }
 
@@ -231,42 +245,42 @@ __device__ __forceinline__ void stage_body(const StageArgs& a) {
     const int64_t id = a.id[i];
     const int64_t step = *a.step_ptr;
     const uint32_t pur = a.kind == ST_EH2 ? 0x500u : 0x100u;
-    normal2(draw(a.seed, step, id, ri.sim, pur), z[0], z[1]);
-    if (D > 2) normal2(draw(a.seed, step, id, ri.sim, pur + 1u), z[2], z[3]);
+    normal2(draw(a.seed, step, id, sim, pur), z[0], z[1]);
+    if (D > 2) normal2(draw(a.seed, step, id, sim, pur + 1u), z[2], z[3]);
     if (a.kind == ST_EH2) {
 #pragma unroll
       for (int c = 0; c < D; ++c)
-        res[c] = a.U[c][i] + ((a.kacc[c][i] + du[c]) * dt + (a.noise + a.noise) * z[c] * a.sqdt) / 2;
-      for (int v = D; v < nv; ++v) res[v] = a.U[v][i] + ((a.kacc[v][i] + du[v]) * dt) / 2;
+        res[c] = Uc(c) + ((a.kacc[c][i] + du[c]) * dt + (a.noise + a.noise) * z[c] * a.sqdt) / 2;
+      for (int v = D; v < nv; ++v) res[v] = Uc(v) + ((a.kacc[v][i] + du[v]) * dt) / 2;
     } else {
 #pragma unroll
-      for (int c = 0; c < D; ++c) res[c] = (a.U[c][i] + du[c] * dt) + a.noise * z[c] * a.sqdt;
-      for (int v = D; v < nv; ++v) res[v] = a.U[v][i] + du[v] * dt;   // no noise off the positions
+      for (int c = 0; c < D; ++c) res[c] = (Uc(c) + du[c] * dt) + a.noise * z[c] * a.sqdt;
+      for (int v = D; v < nv; ++v) res[v] = Uc(v) + du[v] * dt;   // no noise off the positions
       if (a.kind == ST_EH1)
         for (int c = 0; c < nv; ++c) a.kacc[c][i] = du[c];
     }
   } else {
     switch (a.kind) {
       case ST_EULER:
-        for (int c = 0; c < nv; ++c) res[c] = a.U[c][i] + du[c] * dt;
+        for (int c = 0; c < nv; ++c) res[c] = Uc(c) + du[c] * dt;
         break;
       case ST_HEUN1:
-        for (int c = 0; c < nv; ++c) res[c] = a.U[c][i] + du[c] * dt;
+        for (int c = 0; c < nv; ++c) res[c] = Uc(c) + du[c] * dt;
         break;
       case ST_HEUN2:
-        for (int c = 0; c < nv; ++c) res[c] = (a.S[c][i] + (a.U[c][i] + du[c] * dt)) / 2;
+        for (int c = 0; c < nv; ++c) res[c] = (Sc(c) + (Uc(c) + du[c] * dt)) / 2;
         break;
       case ST_RK4_1:
-        for (int c = 0; c < nv; ++c) { a.kacc[c][i] = du[c]; res[c] = a.U[c][i] + du[c] * dt / 2; }
+        for (int c = 0; c < nv; ++c) { a.kacc[c][i] = du[c]; res[c] = Uc(c) + du[c] * dt / 2; }
         break;
       case ST_RK4_2:
-        for (int c = 0; c < nv; ++c) { a.kacc[c][i] = a.kacc[c][i] + 2 * du[c]; res[c] = a.U[c][i] + du[c] * dt / 2; }
+        for (int c = 0; c < nv; ++c) { a.kacc[c][i] = a.kacc[c][i] + 2 * du[c]; res[c] = Uc(c) + du[c] * dt / 2; }
         break;
       case ST_RK4_3:
-        for (int c = 0; c < nv; ++c) { a.kacc[c][i] = a.kacc[c][i] + 2 * du[c]; res[c] = a.U[c][i] + du[c] * dt; }
+        for (int c = 0; c < nv; ++c) { a.kacc[c][i] = a.kacc[c][i] + 2 * du[c]; res[c] = Uc(c) + du[c] * dt; }
         break;
       default:  // ST_RK4_4
-        for (int c = 0; c < nv; ++c) res[c] = a.U[c][i] + (a.kacc[c][i] + du[c]) * dt / 6;
+        for (int c = 0; c < nv; ++c) res[c] = Uc(c) + (a.kacc[c][i] + du[c]) * dt / 6;
         break;
     }
   }
@@ -295,6 +309,316 @@ __device__ __forceinline__ void stage_body(const StageArgs& a) {
     if (D > 1) s2 = s2 + dx[1] * dx[1];
     if (D > 2) s2 = s2 + dx[2] * dx[2];
     if (s2 > a.thr2) *a.flag = 1;   // sqrt(s2) >= thr, src/neighbors.jl:403 (>= skin/2)
+  }
+}
+
+// One integrator stage for every agent row: the agentODE body over the
+// neighbor candidates (Law = the pair law functor) fused with the stage
+// update.  Built-in laws instantiate it in libcbm_hip.so; user laws are
+// compiled around it with hipRTC.
+template <int D, int NB, class Law, int PER, int NOISE>
+__device__ __forceinline__ void stage_body(const StageArgs& a) {
+  const RowIdx ri = row_index(a.g);
+  const SimDev& sd = a.g.sims[ri.sim];
+  const int64_t N = a.g.st[ri.sim].N;
+  if (ri.local >= N) return;
+  const int64_t i = ri.row;
+
+  double xi[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < D; ++c) xi[c] = a.S[c][i];
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (Law::active) gather_global<D, NB, Law, PER>(a, sd, N, i, xi, acc);
+  stage_finish<D, Law, NOISE>(a, sd, ri.sim, i, xi, acc);
+}
+
+// ------------------------------------------------ cell-linked brick tiles
+// The cell-linked stage on LDS tiles.  Block b serves one brick of
+// bdim[0] x bdim[1] x bdim[2] grid cells, a Morton-aligned block (its edges
+// are powers of two with x >= y >= z >= x/2), so that the rows of its cells
+// are one contiguous range.  The agents of the brick and of its one-cell
+// halo are staged in LDS in halo-cell order (x fastest): the three
+// x-adjacent cells of a neighbor column are then one contiguous LDS run, and
+// each home agent walks its 3^D cells there in the reference order
+// (cellPosNeigh, x fastest; the rows of a cell by slot).  A brick whose halo
+// holds more than brick_cap agents walks global memory instead, in the same
+// order, so the result does not depend on the path.
+
+// Reach of a built-in law: a pair can be active only when d < reach (all of
+// them also need d > 0).  Laws without it (user bodies) are not filtered.
+template <class L, class = void> struct has_reach { static constexpr bool value = false; };
+template <class L> struct has_reach<L, decltype((void)&L::reach)> { static constexpr bool value = true; };
+
+// A bound on d^2 that keeps every pair with d = sqrt(d2) < reach: sqrt is
+// correctly rounded, so d < reach implies d2 < reach^2 (1 + 2^-51), and the
+// product below exceeds that.  A NaN or infinite reach keeps every pair.
+__device__ __forceinline__ double reach2_bound(double reach) {
+  const double r2 = (reach * reach) * (1.0 + 0x1p-44);
+  return (r2 == r2) ? r2 : __builtin_inf();
+}
+
+// Order-preserving int64 key of a double (for LDS atomicMax).
+__device__ __forceinline__ long long order_key(double x) {
+  const long long b = __double_as_longlong(x);
+  return b < 0 ? (long long)(~(unsigned long long)b | 0x8000000000000000ull) : b;
+}
+__device__ __forceinline__ double order_val(long long k) {
+  return __longlong_as_double(k < 0 ? (long long)(~(unsigned long long)k | 0x8000000000000000ull) : k);
+}
+
+template <int D, class Law, int PER, int NOISE>
+__device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* lds) {
+  constexpr bool SOFT = same_type<Law, PairLaw<LAW_SOFT_SPHERES>>::value;
+  // Built-in laws walk the candidates twice: a cheap pass keeps the LDS
+  // indices of the pairs within the law's reach (in walk order, up to
+  // BRICK_LIST per lane in an LDS queue), and the full pair body runs on
+  // those only.  Every pair left out is inactive, so the sum is unchanged.
+  constexpr bool FILTER = SOFT || has_reach<Law>::value;
+  constexpr int U4 = 4;               // loads issued together per thread while staging
+  const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  const int32_t* bt = a.brick + BRICK_WORDS * (int64_t)b;
+  const int sim = bt[0];
+  const int32_t o[3] = {bt[1], bt[2], bt[3]};
+  const SimDev& sd = a.g.sims[sim];
+  const int64_t N = a.g.st[sim].N;
+  // the brick's cells hold consecutive buckets [b0, b1): its rows are one range
+  const int32_t r0 = a.bucket_start[bt[4]], r1 = a.bucket_start[bt[5]];
+  if (r0 >= r1) return;               // no agent in this brick
+  const int B[3] = {a.bdim[0], D > 1 ? a.bdim[1] : 1, D > 2 ? a.bdim[2] : 1};
+  const int HX = B[0] + 2, HY = D > 1 ? B[1] + 2 : 1;
+  const int H = brick_halo_cells(D, B[0], B[1], B[2]);
+  const int cap = a.brick_cap;
+  const int tid = threadIdx.x, nt = blockDim.x;
+
+  int32_t* hdr = (int32_t*)lds;
+  long long* rmax_key = (long long*)(lds + 48);
+  double* pos[3];
+  unsigned char* p = lds + 4 * BRICK_HDR;
+#pragma unroll
+  for (int c = 0; c < D; ++c) { pos[c] = (double*)p; p += (size_t)8 * cap; }
+  double* rad = (double*)p;
+  if (SOFT) p += (size_t)8 * cap;
+  int32_t* fate = (int32_t*)p;
+  if (SOFT) p += (size_t)4 * cap;
+  int32_t* hs = (int32_t*)p;          // LDS offset of halo cell h's agents, hs[H] = total
+  p += (size_t)4 * (H + 1);
+  int32_t* hrow = (int32_t*)p;        // first global row of halo cell h
+  p += (size_t)4 * H;
+  uint16_t* lst = (uint16_t*)p;       // per-lane queue of kept candidates, entry u at [u * nt + tid]
+  p += (size_t)2 * BRICK_LIST * BRICK_MAX_BLOCK;
+  uint16_t* owner = (uint16_t*)p;     // halo cell of each staged agent, | 0x8000 at its home cell
+  p += (size_t)2 * cap;
+  uint16_t* home_li = (uint16_t*)p;   // LDS index of home row r0 + k
+
+  const CellRange* __restrict__ cr = a.cell_range + sd.rank_off;
+  if (SOFT && tid == 0) *rmax_key = order_key(-__builtin_inf());
+
+  // 1. rows and counts of the halo cells (wrapped on a periodic grid, empty
+  //    outside a bounded one), U4 cells per thread in flight
+  auto halo_cell = [&](int h, bool& home) {
+    const int hh[3] = {h % HX, (h / HX) % HY, h / (HX * HY)};
+    bool ok = true;
+    home = true;
+    int32_t q[3] = {0, 0, 0};
+#pragma unroll
+    for (int e = 0; e < D; ++e) {
+      int32_t v = o[e] + hh[e] - 1;
+      home = home && hh[e] >= 1 && hh[e] <= B[e] && v < sd.gn[e];
+      if (PER) {
+        if (v == -1) v = sd.gn[e] - 1;
+        else if (v == sd.gn[e]) v = 0;
+        else if (v > sd.gn[e]) ok = false;
+      } else {
+        ok = ok && v >= 0 && v < sd.gn[e];
+      }
+      q[e] = v;
+    }
+    return ok ? q[0] + sd.gn[0] * (q[1] + sd.gn[1] * q[2]) : -1;
+  };
+  for (int h0 = tid; h0 < H; h0 += U4 * nt) {
+    CellRange r[U4];
+#pragma unroll
+    for (int u = 0; u < U4; ++u) {
+      bool home;
+      const int h = h0 + u * nt;
+      const int32_t lin = h < H ? halo_cell(h, home) : -1;
+      r[u] = lin >= 0 ? cr[lin] : CellRange{0, 0};
+    }
+#pragma unroll
+    for (int u = 0; u < U4; ++u) {
+      const int h = h0 + u * nt;
+      if (h < H) { hs[h] = r[u].e - r[u].s; hrow[h] = r[u].s; }
+    }
+  }
+  __syncthreads();
+
+  // 2. exclusive scan of the halo counts (a contiguous chunk per thread)
+  {
+    const int K = (H + nt - 1) / nt;
+    const int c0 = min(tid * K, H), c1 = min(c0 + K, H);
+    int32_t sum = 0;
+    for (int h = c0; h < c1; ++h) sum += hs[h];
+    const int lane = tid & 63, w = tid >> 6, nw = (nt + 63) >> 6;
+    int32_t incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int32_t y = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += y;
+    }
+    if (lane == 63) hdr[4 + w] = incl;
+    __syncthreads();
+    int32_t run = incl - sum, tot = 0;
+    for (int q = 0; q < nw; ++q) {
+      const int32_t ws = hdr[4 + q];
+      if (q < w) run += ws;
+      tot += ws;
+    }
+    for (int h = c0; h < c1; ++h) { const int32_t v = hs[h]; hs[h] = run; run += v; }
+    if (tid == 0) { hs[H] = tot; hdr[2] = tot; }
+  }
+  __syncthreads();
+  const int32_t T = hdr[2];
+  const bool staged = T <= cap;
+  if (!staged && tid == 0) atomicAdd(a.brick_over, 1ull);
+
+  // 3. stage the halo agents (positions, and radius and fate for soft
+  //    spheres), U4 agents per thread in flight; each home row notes the LDS
+  //    index of its copy at its home cell
+  if (staged) {
+    for (int h = tid; h < H; h += nt) {
+      bool home;
+      (void)halo_cell(h, home);
+      const uint16_t tag = (uint16_t)(h | (home ? 0x8000 : 0));
+      for (int32_t q = hs[h]; q < hs[h + 1]; ++q) owner[q] = tag;
+    }
+    __syncthreads();
+    double rm = -__builtin_inf();
+    bool rnan = false;
+    for (int32_t q0 = tid; q0 < T; q0 += U4 * nt) {
+      double v[U4][3];
+      double vr[U4];
+      int32_t vf[U4];
+      int64_t row[U4];
+      int tag[U4];
+#pragma unroll
+      for (int u = 0; u < U4; ++u) {
+        const int32_t q = q0 + u * nt;
+        tag[u] = q < T ? owner[q] : 0;
+        const int h = tag[u] & 0x7FFF;
+        row[u] = q < T ? (int64_t)hrow[h] + (q - hs[h]) : -1;
+        const int64_t rr = row[u] < 0 ? (int64_t)r0 : row[u];
+#pragma unroll
+        for (int c = 0; c < D; ++c) v[u][c] = a.S[c][rr];
+        if (SOFT) { vr[u] = a.rad[rr]; vf[u] = (int32_t)a.fate[rr]; }
+      }
+#pragma unroll
+      for (int u = 0; u < U4; ++u) {
+        const int32_t q = q0 + u * nt;
+        if (q >= T) continue;
+#pragma unroll
+        for (int c = 0; c < D; ++c) pos[c][q] = v[u][c];
+        if (SOFT) {
+          rad[q] = vr[u];
+          fate[q] = vf[u];
+          rm = vr[u] > rm ? vr[u] : rm;
+          rnan = rnan || vr[u] != vr[u];
+        }
+        if (tag[u] & 0x8000) home_li[row[u] - r0] = (uint16_t)q;
+      }
+    }
+    if (SOFT) atomicMax(rmax_key, rnan ? order_key(__builtin_inf()) : order_key(rm));
+    __syncthreads();
+  }
+
+  // 4. the home agents, one row per thread
+  const Law law(sd.p);
+  double L[3], hL[3];
+  box_halves<D, PER>(sd, L, hL);
+  double reach2 = __builtin_inf();
+  if constexpr (FILTER && !SOFT) reach2 = reach2_bound(law.reach());
+  const double rmax = SOFT ? order_val(*rmax_key) : 0.0;
+  for (int64_t i = (int64_t)r0 + tid; i < (int64_t)r1; i += nt) {
+    if (i - sd.row_off >= N) continue;
+    double xi[3] = {0.0, 0.0, 0.0};
+    double acc[3] = {0.0, 0.0, 0.0};
+    double upre[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < D; ++c) upre[c] = a.U[c][i];   // consumed after the walk
+    if (staged) {
+      const int32_t li = home_li[i - r0];
+      const int h = owner[li] & 0x7FFF;
+#pragma unroll
+      for (int c = 0; c < D; ++c) xi[c] = pos[c][li];
+      const double rad_i = SOFT ? rad[li] : 0.0;
+      const int64_t fate_i = SOFT ? fate[li] : 0;
+      double keep2 = reach2;
+      if constexpr (SOFT) keep2 = law.mu >= 0.0 ? reach2_bound(law.mu * (rad_i + rmax)) : __builtin_inf();
+      // the 3^(D-1) runs of three x-adjacent cells (z outer, then y), walked
+      // as one flattened sequence so the lanes of a wavefront stay in step;
+      // run (ry, rz) covers halo cells hb, hb + 1, hb + 2
+      auto walk = [&](auto&& visit) {
+        int ry = 0, rz = 0;
+        int hb = h - 1 - (D > 1 ? HX : 0) - (D > 2 ? HX * HY : 0);
+        int32_t q = hs[hb], qe = hs[hb + 3];
+        while (true) {
+          if (q < qe) {
+            visit(q);
+            ++q;
+            continue;
+          }
+          if (D == 1) break;
+          if (++ry == 3) {
+            if (D == 2 || ++rz == 3) break;
+            ry = 0;
+            hb += HX * HY - 2 * HX;
+          } else {
+            hb += HX;
+          }
+          q = hs[hb];
+          qe = hs[hb + 3];
+        }
+      };
+      auto body = [&](int32_t q, bool live) {
+        double xj[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < D; ++c) xj[c] = pos[c][q];
+        pair_into<D, Law, PER, SOFT>(law, xi, xj, L, hL, rad_i, SOFT ? rad[q] : 0.0, fate_i,
+                                     SOFT ? (int64_t)fate[q] : 0, live, acc);
+      };
+      if constexpr (FILTER) {
+        // pass 1: LDS indices of the candidates within reach, in walk order
+        int nq = 0;
+        walk([&](int32_t q) {
+          double dx[3];
+#pragma unroll
+          for (int c = 0; c < D; ++c) {
+            dx[c] = xi[c] - pos[c][q];
+            if (PER) dx[c] = wrap_dx(dx[c], L[c], hL[c]);
+          }
+          double s2 = dx[0] * dx[0];
+          if (D > 1) s2 = s2 + dx[1] * dx[1];
+          if (D > 2) s2 = s2 + dx[2] * dx[2];
+          if (s2 <= keep2 && q != li) {
+            if (nq < BRICK_LIST) lst[nq * nt + tid] = (uint16_t)q;
+            ++nq;
+          }
+        });
+        // pass 2: the pair body on those; a lane whose queue overflowed
+        // walks every candidate again instead
+        if (nq <= BRICK_LIST) {
+          for (int u = 0; u < nq; ++u) body(lst[u * nt + tid], true);
+        } else {
+          walk([&](int32_t q) { body(q, q != li); });
+        }
+      } else {
+        walk([&](int32_t q) { body(q, q != li); });
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < D; ++c) xi[c] = a.S[c][i];
+      gather_global<D, NB_CELL, Law, PER>(a, sd, N, i, xi, acc);
+    }
+    stage_finish<D, Law, NOISE>(a, sd, sim, i, xi, acc, upre);
   }
 }
 

@@ -10,6 +10,7 @@ using __hip_internal::int64_t;
 using __hip_internal::uint32_t;
 using __hip_internal::uint64_t;
 using __hip_internal::uint8_t;
+using __hip_internal::uint16_t;
 #else
 #include <stdint.h>
 #endif
@@ -75,6 +76,30 @@ enum NeighborKind { NB_FULL = 0, NB_CELL = 1, NB_LIST = 2 };
 // Rows [s, e) of one grid cell after a rebuild, indexed by linear cell.
 struct alignas(8) CellRange { int32_t s, e; };
 
+// LDS of one brick block: a header of int32 words, then D position arrays
+// (and radius for soft spheres) of cap doubles, int32 fate (soft spheres),
+// the halo cells' LDS offsets (H + 1) and first rows (H), the per-lane
+// queues of kept candidates, the uint16 halo cell of each staged agent and
+// the uint16 LDS index of each home row.
+constexpr int BRICK_HDR = 16;        // int32 words: total, per-wave sums; int64 at byte 48
+constexpr int BRICK_LIST = 12;       // queue entries per lane
+constexpr int BRICK_MAX_BLOCK = 256;
+constexpr int BRICK_WORDS = 6;       // brick table entry: sim, origin x, y, z, first bucket, end bucket
+
+__host__ __device__ constexpr int brick_halo_cells(int D, int bx, int by, int bz) {
+  return (bx + 2) * (D > 1 ? by + 2 : 1) * (D > 2 ? bz + 2 : 1);
+}
+
+__host__ __device__ inline int64_t brick_lds_bytes(int D, bool soft, int cap, int H) {
+  int64_t b = 4 * BRICK_HDR;
+  b += (int64_t)8 * cap * (D + (soft ? 1 : 0));
+  if (soft) b += (int64_t)4 * cap;
+  b += (int64_t)4 * (2 * H + 1);
+  b += (int64_t)2 * BRICK_LIST * BRICK_MAX_BLOCK;
+  b += (int64_t)4 * cap;      // owner and home_li, uint16 each
+  return (b + 15) & ~(int64_t)15;
+}
+
 struct StageArgs {
   RowGeom g;
   const double* S[MAX_STATE];   // stage state read by the force (neighbors + self)
@@ -94,12 +119,17 @@ struct StageArgs {
   int64_t ldl;
   const CellRange* cell_range;  // rank_off + linear local cell -> its rows (cell-linked)
   const int32_t* cell_row;      // row -> linear local cell (cell-linked)
+  const int32_t* brick;         // cell-linked brick blocks: BRICK_WORDS per block
+  const int32_t* bucket_start;  // first row of each global bucket (the rebuild's scan)
+  unsigned long long* brick_over;   // count of brick blocks that walked global memory
   int32_t* flag;                // rebuild flag written by the final stage
   double dt, sqdt, drift, noise;
   double thr2;                  // largest d2 with sqrt(d2) < rebuild displacement
   uint64_t seed;
   const int64_t* step_ptr;     // device step counter (Philox counter word)
   int32_t kind, periodic, lagged, walls, nvar, disp_check, write_fnew, pad;
+  int32_t bdim[3];              // brick edge in cells per axis
+  int32_t brick_cap;            // agents a brick block can stage in LDS
 };
 
 }  // namespace cbm

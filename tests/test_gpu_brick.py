@@ -1,0 +1,147 @@
+"""The LDS-tiled cell-linked stage against the oracle on the cases that stress
+its tiling: a brick whose halo overflows the LDS capacity (it walks global
+memory instead), periodic grids of 3-5 cells per axis (a brick's halo wraps
+onto itself), 1-D grids, and block sizes smaller than a brick's population.
+The walk order is the reference's cellPosNeigh order (src/neighbors.jl), so
+the built-in laws are expected bit-identical to the oracle."""
+import numpy as np
+import pytest
+
+import cbm._abi as A
+import oracle as O
+from cbm.api import Context
+from helpers import model, rel_err, sim, uniform_positions
+
+pytestmark = pytest.mark.gpu
+
+LONG_TOL = 1e-9
+REP_ADH = [1.0, 1.0, 0.1, 2.4]
+
+
+def pair(m, s, pos, block=None):
+    o = O.Oracle(m, s)
+    g = Context(m, [s])
+    if block:
+        g.set_block(block)
+    for a, v in enumerate(pos):
+        o.set(a, v)
+        g.upload(a, v)
+    assert o.init() == 0
+    g.neighbors()
+    return o, g
+
+
+def err(o, g, dims):
+    n = o.n
+    return max(rel_err(g.download(a, n), o.get(a)) for a in range(dims))
+
+
+def test_brick_geometry_is_morton_aligned(gpu_available):
+    for dims, L in [(1, 2000.0), (2, 60.0), (3, 30.0)]:
+        n = 2000
+        box = [[0.0, L]] * dims
+        m = model(dims, integrator=A.EULER, neighbors=A.CELL_LINKED, law=A.LAW_REP_ADH, dt=0.01, cell_edge=2.4)
+        g = Context(m, [sim(n, box, REP_ADH)])
+        b = g.bricks()
+        bits = [int(np.log2(v)) for v in b["bdim"][:dims]]
+        assert all(2 ** q == v for q, v in zip(bits, b["bdim"][:dims]))
+        # x takes the extra bits first: x >= y >= z >= x - 1 (in bits)
+        assert bits == sorted(bits, reverse=True) and bits[0] - bits[-1] <= 1
+        assert b["cap"] >= 256 and b["nbricks"] >= 1 and b["overflows"] == 0
+
+
+def test_brick_overflow_falls_back_bit_exact(gpu_available):
+    # 1,500 of 2,000 agents packed in a 4.8-wide cube: the bricks around it
+    # hold far more agents than the capacity sized for the mean density
+    rng = np.random.default_rng(11)
+    n, L = 2000, 40.0
+    box = [[0.0, L]] * 3
+    pos = [rng.uniform(0.0, L, n) for _ in range(3)]
+    for a in range(3):
+        pos[a][:1500] = rng.uniform(17.6, 22.4, 1500)
+    m = model(3, integrator=A.EULER, neighbors=A.CELL_LINKED, law=A.LAW_REP_ADH, dt=1e-4, cell_edge=2.4)
+    o, g = pair(m, sim(n, box, REP_ADH), pos)
+    assert g.bricks()["cap"] < 1500
+    o.step(1)
+    g.step(1)
+    assert err(o, g, 3) == 0.0
+    assert g.bricks()["overflows"] > 0
+    o.step(19)
+    g.step(19)
+    assert err(o, g, 3) <= LONG_TOL
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_brick_dense_cluster_overflows_lane_queue(gpu_available, dims):
+    # 60 agents within 1.0 of a point: each has 59 candidates within the law's
+    # reach, more than a lane queues, while the brick still fits in LDS
+    rng = np.random.default_rng(7 + dims)
+    n, L = 460, 40.0
+    box = [[0.0, L]] * dims
+    pos = [rng.uniform(0.0, L, n) for _ in range(dims)]
+    u = rng.normal(size=(dims, 60))
+    u *= rng.uniform(0.0, 1.0, 60) ** (1.0 / dims) / np.linalg.norm(u, axis=0)
+    for a in range(dims):
+        pos[a][:60] = 20.0 + u[a]
+    m = model(dims, integrator=A.HEUN, neighbors=A.CELL_LINKED, law=A.LAW_REP_ADH, dt=1e-3, periodic=1,
+              cell_edge=2.4)
+    o, g = pair(m, sim(n, box, REP_ADH), pos)
+    o.step(1)
+    g.step(1)
+    assert err(o, g, dims) == 0.0
+    o.step(19)
+    g.step(19)
+    assert err(o, g, dims) <= LONG_TOL
+    assert g.bricks()["overflows"] == 0
+
+
+@pytest.mark.parametrize("gn", [3, 4, 5])
+def test_brick_small_periodic_grid(gpu_available, gn):
+    # a periodic grid narrower than a brick: the halo wraps onto the brick
+    L = gn * 2.4 + 0.01
+    n = int(0.9 * gn ** 3) + 5
+    box = [[0.0, L]] * 3
+    m = model(3, integrator=A.HEUN, neighbors=A.CELL_LINKED, law=A.LAW_REP_ADH, dt=0.005, periodic=1,
+              cell_edge=2.4)
+    o, g = pair(m, sim(n, box, REP_ADH), uniform_positions(n, box, 20 + gn))
+    assert g.bricks()["bdim"][0] > gn
+    o.step(1)
+    g.step(1)
+    assert err(o, g, 3) == 0.0
+    o.step(29)
+    g.step(29)
+    assert err(o, g, 3) <= LONG_TOL
+
+
+@pytest.mark.parametrize("periodic", [0, 1])
+def test_brick_1d(gpu_available, periodic):
+    n, L = 400, 300.0
+    box = [[0.0, L]]
+    m = model(1, integrator=A.EULER, neighbors=A.CELL_LINKED, law=A.LAW_LINEAR, dt=0.01, periodic=periodic,
+              cell_edge=2.0)
+    o, g = pair(m, sim(n, box, [1.0, 2.0]), uniform_positions(n, box, 3))
+    o.step(1)
+    g.step(1)
+    assert err(o, g, 1) == 0.0
+    o.step(49)
+    g.step(49)
+    assert err(o, g, 1) <= LONG_TOL
+
+
+def test_brick_block_sizes_agree(gpu_available):
+    # a brick of ~256 agents served by blocks of 64, 128 and 256 threads
+    n = 6000
+    box = [[0.0, 40.0]] * 3
+    pos = uniform_positions(n, box, 5)
+    m = model(3, integrator=A.RK4, neighbors=A.CELL_LINKED, law=A.LAW_REP_ADH, dt=0.01, periodic=1, cell_edge=2.4)
+    out = []
+    for block in (64, 128, 256):
+        o, g = pair(m, sim(n, box, REP_ADH), pos, block=block)
+        g.step(5)
+        out.append([g.download(a, n) for a in range(3)])
+    o.step(5)
+    ref = [o.get(a) for a in range(3)]
+    for got in out:
+        for a in range(3):
+            assert np.array_equal(got[a], out[0][a])
+            assert rel_err(got[a], ref[a]) == 0.0
