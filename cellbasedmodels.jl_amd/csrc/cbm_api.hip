@@ -93,7 +93,9 @@ struct cbm_ctx {
   int32_t* list = nullptr;
   int32_t* list_n = nullptr;
   int cap = 0;
-  int32_t* bricks = nullptr;              // cell-linked brick blocks: sim, origin x, y, z
+  int32_t* bricks = nullptr;              // cell-linked brick blocks (BRICK_WORDS each)
+  uint8_t* brick_skip = nullptr;          // per row: left to the row-parallel stage kernel
+  uint8_t* blk_flag = nullptr;            // per row-kernel block: holds such rows
   int64_t nbricks = 0;
   int bdim[3] = {1, 1, 1};
   int brick_cap = 0;                      // agents staged per brick block
@@ -136,7 +138,8 @@ struct cbm_ctx {
 
   // user interaction body compiled with hipRTC (law CBM_LAW_CUSTOM)
   hipModule_t user_mod = nullptr;
-  hipFunction_t user_fn[2] = {nullptr, nullptr};   // [0] deterministic stages, [1] EM
+  hipFunction_t user_fn[2] = {nullptr, nullptr};   // [0] deterministic stages, [1] noisy (EM, EulerHeun)
+  hipFunction_t user_brick[2] = {nullptr, nullptr};   // the same, cell-linked brick tiles
 
   std::string err;
 };
@@ -426,7 +429,7 @@ static void destroy_ctx(cbm_ctx* c) {
   F(c->n_add_slot); F(c->dead_slot); F(c->off_add); F(c->off_dead); F(c->holes);
   F(c->med_c[0]); F(c->med_c[1]); F(c->med_count);
   F(c->key); F(c->lin); F(c->gbucket); F(c->cell_range); F(c->count); F(c->cursor); F(c->start); F(c->tmp); F(c->cell_row);
-  F(c->scan_status); F(c->scan_ticket); F(c->list); F(c->list_n); F(c->bricks); F(c->flags); F(c->counters);
+  F(c->scan_status); F(c->scan_ticket); F(c->list); F(c->list_n); F(c->bricks); F(c->brick_skip); F(c->blk_flag); F(c->flags); F(c->counters);
   for (cbm_ctx::DevBuf* b : {&c->adj_tgt, &c->adj_tbase, &c->adj_invn, &c->adj_ck, &c->adj_seg, &c->adj_lam,
                              &c->adj_gpart, &c->adj_lpart}) F(b->p);
   for (int32_t* p : c->snap_l) F(p);
@@ -556,7 +559,11 @@ static int init_ctx(cbm_ctx* c) {
     HIPCHK(c, hipMemcpy(c->gbucket, gb.data(), sizeof(int32_t) * c->NR, hipMemcpyHostToDevice));
     HIPCHK(c, dalloc(&c->cell_range, c->NR));
   }
-  if (c->cl && (rc = setup_bricks(c))) return rc;
+  if (c->cl) {
+    if ((rc = setup_bricks(c))) return rc;
+    HIPCHK(c, dalloc(&c->brick_skip, c->C));
+    HIPCHK(c, dalloc(&c->blk_flag, c->C / 64 + 2 * (int64_t)c->S + 1));   // row blocks at the smallest block size
+  }
   c->h_crank.clear();
   c->h_crank.shrink_to_fit();
   HIPCHK(c, dalloc(&c->tmp, c->C));
@@ -700,6 +707,8 @@ static StageArgs stage_base(cbm_ctx* c) {
   a.list = c->list; a.list_n = c->list_n; a.ldl = c->C;
   a.cell_range = c->cell_range; a.cell_row = c->cell_row;
   a.brick = c->bricks;
+  a.brick_skip = c->brick_skip;
+  a.blk_flag = c->blk_flag;
   a.bucket_start = c->start;
   a.brick_over = (unsigned long long*)(c->counters + 2);
   for (int q = 0; q < 3; ++q) a.bdim[q] = c->bdim[q];
@@ -737,13 +746,18 @@ static void prof_end(cbm_ctx* c);
 static hipError_t stage_launch(cbm_ctx* c, const StageArgs& a) {
   if (c->m.law != CBM_LAW_CUSTOM)
     return launch_stage(c->dims, c->nb_kind, c->m.law, c->nblocks, c->block, c->stream, a, c->nbricks, c->brick_lds);
-  hipFunction_t f = c->user_fn[(a.kind == ST_EM || a.kind == ST_EH1 || a.kind == ST_EH2) ? 1 : 0];
+  const int noise = (a.kind == ST_EM || a.kind == ST_EH1 || a.kind == ST_EH2) ? 1 : 0;
+  hipFunction_t f = c->user_fn[noise];
   if (!f) return hipErrorInvalidDeviceFunction;
   StageArgs copy = a;
   void* params[] = {(void*)&copy};
-  const bool brick = c->nb_kind == NB_CELL;
-  return hipModuleLaunchKernel(f, brick ? (unsigned)c->nbricks : (unsigned)c->nblocks, 1, 1, (unsigned)c->block, 1, 1,
-                               brick ? (unsigned)c->brick_lds : 0u, c->stream, params, nullptr);
+  if (c->nb_kind == NB_CELL) {
+    if (!c->user_brick[noise]) return hipErrorInvalidDeviceFunction;
+    const hipError_t e = hipModuleLaunchKernel(c->user_brick[noise], (unsigned)c->nbricks, 1, 1, (unsigned)c->block,
+                                               1, 1, (unsigned)c->brick_lds, c->stream, params, nullptr);
+    if (e != hipSuccess) return e;
+  }
+  return hipModuleLaunchKernel(f, (unsigned)c->nblocks, 1, 1, (unsigned)c->block, 1, 1, 0, c->stream, params, nullptr);
 }
 
 static int run_stage(cbm_ctx* c, StageArgs& a, int kind) {
@@ -985,18 +999,19 @@ int cbm_compile(cbm_ctx* c, const char* body) {
   src += "\n}  // namespace cbm\n";
   char inst[512];
   for (int noise = 0; noise < 2; ++noise) {
-    if (c->nb_kind == NB_CELL)   // the brick-tiled cell-linked stage
+    std::snprintf(inst, sizeof(inst),
+                  "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_stage_%d(cbm::StageArgs a) {\n"
+                  "  cbm::stage_body<%d, %d, cbm::UserLaw, %d, %d>(a);\n}\n",
+                  noise, c->dims, c->nb_kind, c->m.periodic ? 1 : 0, noise);
+    src += inst;
+    if (c->nb_kind == NB_CELL) {   // the brick-tiled cell-linked stage, run before the row kernel
       std::snprintf(inst, sizeof(inst),
-                    "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_stage_%d(cbm::StageArgs a) {\n"
+                    "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_brick_%d(cbm::StageArgs a) {\n"
                     "  extern __shared__ unsigned char lds[];\n"
                     "  cbm::brick_body<%d, cbm::UserLaw, %d, %d>(a, lds);\n}\n",
                     noise, c->dims, c->m.periodic ? 1 : 0, noise);
-    else
-      std::snprintf(inst, sizeof(inst),
-                    "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_stage_%d(cbm::StageArgs a) {\n"
-                    "  cbm::stage_body<%d, %d, cbm::UserLaw, %d, %d>(a);\n}\n",
-                    noise, c->dims, c->nb_kind, c->m.periodic ? 1 : 0, noise);
-    src += inst;
+      src += inst;
+    }
   }
   const char* headers[] = {cbm_rtc_cbm_device_h, cbm_rtc_cbm_structs_h, cbm_rtc_cbm_stage_h};
   const char* names[] = {"cbm_device.h", "cbm_structs.h", "cbm_stage.h"};
@@ -1022,6 +1037,10 @@ int cbm_compile(cbm_ctx* c, const char* body) {
   HIPCHK(c, hipModuleLoadData(&c->user_mod, code.data()));
   HIPCHK(c, hipModuleGetFunction(&c->user_fn[0], c->user_mod, "cbm_user_stage_0"));
   HIPCHK(c, hipModuleGetFunction(&c->user_fn[1], c->user_mod, "cbm_user_stage_1"));
+  if (c->nb_kind == NB_CELL) {
+    HIPCHK(c, hipModuleGetFunction(&c->user_brick[0], c->user_mod, "cbm_user_brick_0"));
+    HIPCHK(c, hipModuleGetFunction(&c->user_brick[1], c->user_mod, "cbm_user_brick_1"));
+  }
   return CBM_OK;
 }
 

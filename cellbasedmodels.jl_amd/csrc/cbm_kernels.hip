@@ -889,19 +889,21 @@ __global__ __launch_bounds__(256) void k_export(ExportArgs e) {
 }
 
 // ------------------------------------------------------------- launchers
-struct StageLaunch { dim3 grid, block; size_t lds; hipStream_t s; };
+struct StageLaunch { dim3 grid, bgrid, block; size_t lds; hipStream_t s; };   // bgrid: one block per brick
 
 template <int D, int NB, int PER, int NOISE>
 static hipError_t launch_stage_law(int law, const StageLaunch& L, const StageArgs& a) {
   if (NB == NB_CELL) {
+    // bricks that fit in LDS, then the rows of those that did not
     switch (law) {
-      case LAW_LINEAR: hipLaunchKernelGGL((k_brick<D, LAW_LINEAR, PER, NOISE>), L.grid, L.block, L.lds, L.s, a); break;
-      case LAW_REP_ADH: hipLaunchKernelGGL((k_brick<D, LAW_REP_ADH, PER, NOISE>), L.grid, L.block, L.lds, L.s, a); break;
-      case LAW_AGGREGATION: hipLaunchKernelGGL((k_brick<D, LAW_AGGREGATION, PER, NOISE>), L.grid, L.block, L.lds, L.s, a); break;
-      case LAW_SOFT_SPHERES: hipLaunchKernelGGL((k_brick<D, LAW_SOFT_SPHERES, PER, NOISE>), L.grid, L.block, L.lds, L.s, a); break;
+      case LAW_LINEAR: hipLaunchKernelGGL((k_brick<D, LAW_LINEAR, PER, NOISE>), L.bgrid, L.block, L.lds, L.s, a); break;
+      case LAW_REP_ADH: hipLaunchKernelGGL((k_brick<D, LAW_REP_ADH, PER, NOISE>), L.bgrid, L.block, L.lds, L.s, a); break;
+      case LAW_AGGREGATION: hipLaunchKernelGGL((k_brick<D, LAW_AGGREGATION, PER, NOISE>), L.bgrid, L.block, L.lds, L.s, a); break;
+      case LAW_SOFT_SPHERES: hipLaunchKernelGGL((k_brick<D, LAW_SOFT_SPHERES, PER, NOISE>), L.bgrid, L.block, L.lds, L.s, a); break;
       default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
   }
   switch (law) {
     case LAW_LINEAR: hipLaunchKernelGGL((k_stage<D, NB, LAW_LINEAR, PER, NOISE>), L.grid, L.block, 0, L.s, a); break;
@@ -934,10 +936,10 @@ static hipError_t launch_stage_dims(int nb, int law, int periodic, const StageLa
 hipError_t launch_stage(int dims, int nb, int law, int nblocks, int block, hipStream_t s, const StageArgs& a,
                         int64_t nbricks, size_t brick_lds) {
   StageLaunch L;
-  const bool brick = nb == NB_CELL && law != LAW_NONE;
-  L.grid = dim3(brick ? (unsigned)nbricks : (unsigned)nblocks);
+  L.grid = dim3((unsigned)nblocks);
+  L.bgrid = dim3((unsigned)nbricks);
   L.block = dim3(block);
-  L.lds = brick ? brick_lds : 0;
+  L.lds = brick_lds;
   L.s = s;
   const bool noise = a.kind == ST_EM || a.kind == ST_EH1 || a.kind == ST_EH2;
   switch (dims) {

@@ -192,13 +192,17 @@ __device__ __forceinline__ void gather_global(const StageArgs& a, const SimDev& 
 // The rest of the stage for row i once the pair sum acc is known: walls,
 // the derivative of every DE variable, the integrator update, periodic
 // wrap, the output write and the displacement check.
-template <int D, class Law, int NOISE>
+template <int D, class Law, int NOISE, bool PRE = false>
 __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& sd, int sim, int64_t i,
-                                             const double* xi, double* acc, const double* upre = nullptr) {
+                                             const double* xi, double* acc, const double* upre) {
   constexpr bool SOFT = same_type<Law, PairLaw<LAW_SOFT_SPHERES>>::value;
-  // u of variable c (positions possibly loaded early by the caller), and the
-  // stage state, whose positions are xi
-  auto Uc = [&](int c) { return (upre && c < D) ? upre[c] : a.U[c][i]; };
+  // u of variable c (with PRE, the caller loaded the positions' u early into
+  // upre), and the stage state, whose positions are xi
+  const double u0 = upre[0], u1 = upre[1], u2 = upre[2];   // values, so no pointer is selected
+  auto Uc = [&](int c) -> double {
+    if (PRE && c < D) return c == 0 ? u0 : (c == 1 ? u1 : u2);
+    return a.U[c][i];
+  };
   auto Sc = [&](int c) { return c < D ? xi[c] : a.S[c][i]; };
   if (Law::active && Law::walls && a.walls) {
     const double rRep = sd.p[0], fRep = sd.p[1];
@@ -318,18 +322,31 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
 // compiled around it with hipRTC.
 template <int D, int NB, class Law, int PER, int NOISE>
 __device__ __forceinline__ void stage_body(const StageArgs& a) {
+  if (NB == NB_CELL) {
+    // cell-linked: only blocks holding rows of bricks too dense for LDS
+    const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    if (!a.blk_flag[b]) return;
+    __syncthreads();
+    if (threadIdx.x == 0) a.blk_flag[b] = 0;
+  }
   const RowIdx ri = row_index(a.g);
   const SimDev& sd = a.g.sims[ri.sim];
   const int64_t N = a.g.st[ri.sim].N;
-  if (ri.local >= N) return;
   const int64_t i = ri.row;
+  // cell-linked: the brick kernel has done every row of a brick that fit in
+  // LDS; the others are marked, and the mark is cleared here
+  if (NB == NB_CELL) {
+    if (ri.local >= sd.n_max || !a.brick_skip[i]) return;
+    a.brick_skip[i] = 0;
+  }
+  if (ri.local >= N) return;
 
   double xi[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int c = 0; c < D; ++c) xi[c] = a.S[c][i];
   double acc[3] = {0.0, 0.0, 0.0};
   if (Law::active) gather_global<D, NB, Law, PER>(a, sd, N, i, xi, acc);
-  stage_finish<D, Law, NOISE>(a, sd, ri.sim, i, xi, acc);
+  stage_finish<D, Law, NOISE>(a, sd, ri.sim, i, xi, acc, xi);
 }
 
 // ------------------------------------------------ cell-linked brick tiles
@@ -341,8 +358,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& a) {
 // x-adjacent cells of a neighbor column are then one contiguous LDS run, and
 // each home agent walks its 3^D cells there in the reference order
 // (cellPosNeigh, x fastest; the rows of a cell by slot).  A brick whose halo
-// holds more than brick_cap agents walks global memory instead, in the same
-// order, so the result does not depend on the path.
+// holds more than brick_cap agents marks its rows in brick_skip and leaves
+// them to the row-parallel stage kernel launched after it, which walks
+// global memory in the same order, so the result does not depend on the
+// path.
 
 // Reach of a built-in law: a pair can be active only when d < reach (all of
 // them also need d > 0).  Laws without it (user bodies) are not filtered.
@@ -415,8 +434,15 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
 
   // 1. rows and counts of the halo cells (wrapped on a periodic grid, empty
   //    outside a bounded one), U4 cells per thread in flight
+  // halo cell h -> (hx, hy, hz), x fastest.  h + 0.5 lies at least 0.5 from
+  // a multiple of the divisor, far beyond the float product's rounding for
+  // h < 2^16, so the truncated products are the exact quotients.
+  const float rHX = 1.0f / (float)HX, rHXY = 1.0f / (float)(HX * HY);
   auto halo_cell = [&](int h, bool& home) {
-    const int hh[3] = {h % HX, (h / HX) % HY, h / (HX * HY)};
+    const int hz = (int)(((float)h + 0.5f) * rHXY);
+    const int rem = h - hz * HX * HY;
+    const int hy = (int)(((float)rem + 0.5f) * rHX);
+    const int hh[3] = {rem - hy * HX, hy, hz};
     bool ok = true;
     home = true;
     int32_t q[3] = {0, 0, 0};
@@ -478,13 +504,22 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
   }
   __syncthreads();
   const int32_t T = hdr[2];
-  const bool staged = T <= cap;
-  if (!staged && tid == 0) atomicAdd(a.brick_over, 1ull);
+  if (T > cap) {
+    // too dense for LDS: the row-parallel kernel launched after this one
+    // takes the brick's rows
+    if (tid == 0) atomicAdd(a.brick_over, 1ull);
+    for (int64_t i = (int64_t)r0 + tid; i < (int64_t)r1; i += nt) a.brick_skip[i] = 1;
+    // wake the row-kernel blocks holding these rows (same block size)
+    const int64_t q0 = a.g.block_off[sim] + ((int64_t)r0 - sd.row_off) / nt;
+    const int64_t q1 = a.g.block_off[sim] + ((int64_t)r1 - 1 - sd.row_off) / nt;
+    for (int64_t q = q0 + tid; q <= q1; q += nt) a.blk_flag[q] = 1;
+    return;
+  }
 
   // 3. stage the halo agents (positions, and radius and fate for soft
   //    spheres), U4 agents per thread in flight; each home row notes the LDS
   //    index of its copy at its home cell
-  if (staged) {
+  {
     for (int h = tid; h < H; h += nt) {
       bool home;
       (void)halo_cell(h, home);
@@ -530,52 +565,62 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
     __syncthreads();
   }
 
-  // 4. the home agents, one row per thread
+  // 4. the home agents, one per thread
+  // candidates of a home agent in halo cell h: its 3^(D-1) runs of three
+  // x-adjacent cells (z outer, then y); run (ry, rz) covers cells hb .. hb + 2
+  auto run0 = [&](int h) { return h - 1 - (D > 1 ? HX : 0) - (D > 2 ? HX * HY : 0); };
+  auto ncand_of = [&](int h) {
+    const int hb0 = run0(h);
+    int32_t n = 0;
+#pragma unroll
+    for (int rz = 0; rz < (D > 2 ? 3 : 1); ++rz)
+#pragma unroll
+      for (int ry = 0; ry < (D > 1 ? 3 : 1); ++ry) {
+        const int hb = hb0 + rz * HX * HY + ry * HX;
+        n += hs[hb + 3] - hs[hb];
+      }
+    return n;
+  };
+  const int32_t nh = r1 - r0;
   const Law law(sd.p);
   double L[3], hL[3];
   box_halves<D, PER>(sd, L, hL);
   double reach2 = __builtin_inf();
   if constexpr (FILTER && !SOFT) reach2 = reach2_bound(law.reach());
   const double rmax = SOFT ? order_val(*rmax_key) : 0.0;
-  for (int64_t i = (int64_t)r0 + tid; i < (int64_t)r1; i += nt) {
+  for (int32_t k = tid; k < nh; k += nt) {
+    const int32_t li = home_li[k];
+    const int h = owner[li] & 0x7FFF;
+    const int64_t i = (int64_t)hrow[h] + (li - hs[h]);
     if (i - sd.row_off >= N) continue;
     double xi[3] = {0.0, 0.0, 0.0};
     double acc[3] = {0.0, 0.0, 0.0};
     double upre[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int c = 0; c < D; ++c) upre[c] = a.U[c][i];   // consumed after the walk
-    if (staged) {
-      const int32_t li = home_li[i - r0];
-      const int h = owner[li] & 0x7FFF;
+    {
 #pragma unroll
       for (int c = 0; c < D; ++c) xi[c] = pos[c][li];
       const double rad_i = SOFT ? rad[li] : 0.0;
       const int64_t fate_i = SOFT ? fate[li] : 0;
       double keep2 = reach2;
       if constexpr (SOFT) keep2 = law.mu >= 0.0 ? reach2_bound(law.mu * (rad_i + rmax)) : __builtin_inf();
-      // the 3^(D-1) runs of three x-adjacent cells (z outer, then y), walked
-      // as one flattened sequence so the lanes of a wavefront stay in step;
-      // run (ry, rz) covers halo cells hb, hb + 1, hb + 2
+      // the runs walked as one flattened sequence of ncand candidates, so
+      // the lanes of a wavefront stay in step
+      const int hb0 = run0(h);
+      const int32_t ncand = ncand_of(h);
       auto walk = [&](auto&& visit) {
-        int ry = 0, rz = 0;
-        int hb = h - 1 - (D > 1 ? HX : 0) - (D > 2 ? HX * HY : 0);
+        int ry = 0;
+        int hb = hb0;
         int32_t q = hs[hb], qe = hs[hb + 3];
-        while (true) {
-          if (q < qe) {
-            visit(q);
-            ++q;
-            continue;
+        for (int32_t k = 0; k < ncand; ++k) {
+          while (q == qe) {   // the next non-empty run; one exists while k < ncand
+            if (++ry == 3) { ry = 0; hb += HX * HY - 2 * HX; } else { hb += HX; }
+            q = hs[hb];
+            qe = hs[hb + 3];
           }
-          if (D == 1) break;
-          if (++ry == 3) {
-            if (D == 2 || ++rz == 3) break;
-            ry = 0;
-            hb += HX * HY - 2 * HX;
-          } else {
-            hb += HX;
-          }
-          q = hs[hb];
-          qe = hs[hb + 3];
+          visit(q);
+          ++q;
         }
       };
       auto body = [&](int32_t q, bool live) {
@@ -613,12 +658,8 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
       } else {
         walk([&](int32_t q) { body(q, q != li); });
       }
-    } else {
-#pragma unroll
-      for (int c = 0; c < D; ++c) xi[c] = a.S[c][i];
-      gather_global<D, NB_CELL, Law, PER>(a, sd, N, i, xi, acc);
     }
-    stage_finish<D, Law, NOISE>(a, sd, sim, i, xi, acc, upre);
+    stage_finish<D, Law, NOISE, true>(a, sd, sim, i, xi, acc, upre);
   }
 }
 

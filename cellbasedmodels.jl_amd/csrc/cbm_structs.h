@@ -121,7 +121,9 @@ struct StageArgs {
   const int32_t* cell_row;      // row -> linear local cell (cell-linked)
   const int32_t* brick;         // cell-linked brick blocks: BRICK_WORDS per block
   const int32_t* bucket_start;  // first row of each global bucket (the rebuild's scan)
-  unsigned long long* brick_over;   // count of brick blocks that walked global memory
+  unsigned long long* brick_over;   // count of brick blocks left to the row-parallel kernel
+  uint8_t* brick_skip;          // per row: 1 = left to the row-parallel kernel (cell-linked; it clears it)
+  uint8_t* blk_flag;            // per row-kernel block: holds rows left to it (cleared by it)
   int32_t* flag;                // rebuild flag written by the final stage
   double dt, sqdt, drift, noise;
   double thr2;                  // largest d2 with sqrt(d2) < rebuild displacement
