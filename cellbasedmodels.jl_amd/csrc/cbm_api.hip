@@ -263,8 +263,8 @@ static void morton_ranks(const int32_t* gn, int dims, std::vector<int32_t>& out)
 // its rows are one range).  m is the largest size whose expected population
 // (the densest simulation's mean density; capacity when rules change the
 // population) stays within ~1.25 blocks of 256 threads, lowered while the
-// LDS for 1.15x the expected halo population would exceed 64 KiB (a halo
-// above that walks global memory).  Bricks
+// LDS for 1.3x the expected halo population would exceed 64 KiB (a denser
+// brick is left to the row-parallel stage kernel).  Bricks
 // are listed per simulation in Morton order of their coordinates.
 static int setup_bricks(cbm_ctx* c) {
   const int D = c->dims;
@@ -288,7 +288,7 @@ static int setup_bricks(cbm_ctx* c) {
     const int H = brick_halo_cells(D, bd[0], bd[1], bd[2]);
     const int64_t fixed = brick_lds_bytes(D, soft, 0, H) + 16;
     const int64_t capmax = (budget - fixed) / per;
-    const double want = 1.15 * rho * H + 32.0;
+    const double want = 1.3 * rho * H + 32.0;
     if (want <= (double)capmax || m == mmin) {
       cap = (int)std::max<int64_t>(1, std::min<int64_t>(capmax, (int64_t)std::max(256.0, want)));
       break;

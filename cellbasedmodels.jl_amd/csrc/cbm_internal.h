@@ -203,7 +203,9 @@ hipError_t launch_compact_slots(int nblocks, int block, hipStream_t s, const Rul
 hipError_t launch_commit_counts(hipStream_t s, const RuleArgs& a, int32_t nsims);
 
 // ------------------------------------------------------------- launchers
-constexpr int SCAN_THREADS = 256;
+// 16 Ki counts per tile: the look-back chain (one tile after another) sets
+// the scan's time, so fewer, larger tiles are faster
+constexpr int SCAN_THREADS = 1024;
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 inline int scan_blocks(int64_t n) { return (int)((n + SCAN_TILE - 1) / SCAN_TILE); }
