@@ -278,7 +278,13 @@ static int setup_bricks(cbm_ctx* c) {
   int m = mmin;
   while (m < mmax && rho * (double)(1 << (m + 1)) <= 320.0) ++m;
   const int64_t budget = 64 * 1024;
-  const int64_t per = 8 * (D + (soft ? 1 : 0)) + (soft ? 4 : 0) + 2;   // LDS bytes per staged agent
+  // LDS bytes per staged agent and in all of a tile: the Verlet list build
+  // stages positions and slots, the cell-linked stage positions (and radius
+  // and fate for soft spheres); both add an owner and a home-row index
+  const int64_t per = c->verlet ? 8 * D + 4 + 4 : 8 * (D + (soft ? 1 : 0)) + (soft ? 4 : 0) + 4;
+  auto tile_bytes = [&](int cp, int H) {
+    return c->verlet ? vbrick_lds_bytes(D, cp, H) : brick_lds_bytes(D, soft, cp, H);
+  };
   int bd[3] = {1, 1, 1};
   int cap = 0;
   for (;; --m) {
@@ -286,7 +292,7 @@ static int setup_bricks(cbm_ctx* c) {
     bd[1] = D > 1 ? 1 << ((m + D - 2) / D) : 1;
     bd[2] = D > 2 ? 1 << ((m + D - 3) / D) : 1;
     const int H = brick_halo_cells(D, bd[0], bd[1], bd[2]);
-    const int64_t fixed = brick_lds_bytes(D, soft, 0, H) + 16;
+    const int64_t fixed = tile_bytes(0, H) + 16;
     const int64_t capmax = (budget - fixed) / per;
     const double want = 1.3 * rho * H + 32.0;
     if (want <= (double)capmax || m == mmin) {
@@ -296,7 +302,7 @@ static int setup_bricks(cbm_ctx* c) {
   }
   for (int a = 0; a < 3; ++a) c->bdim[a] = bd[a];
   c->brick_cap = cap;
-  c->brick_lds = (size_t)brick_lds_bytes(D, soft, cap, brick_halo_cells(D, bd[0], bd[1], bd[2]));
+  c->brick_lds = (size_t)tile_bytes(cap, brick_halo_cells(D, bd[0], bd[1], bd[2]));
   std::vector<int32_t> tab;
   for (int s = 0; s < c->S; ++s) {
     const SimDev& sd = c->h_sims[s];
@@ -559,7 +565,7 @@ static int init_ctx(cbm_ctx* c) {
     HIPCHK(c, hipMemcpy(c->gbucket, gb.data(), sizeof(int32_t) * c->NR, hipMemcpyHostToDevice));
     HIPCHK(c, dalloc(&c->cell_range, c->NR));
   }
-  if (c->cl) {
+  if (c->cl || c->verlet) {
     if ((rc = setup_bricks(c))) return rc;
     HIPCHK(c, dalloc(&c->brick_skip, c->C));
     HIPCHK(c, dalloc(&c->blk_flag, c->C / 64 + 2 * (int64_t)c->S + 1));   // row blocks at the smallest block size
@@ -651,6 +657,22 @@ static int run_scan(cbm_ctx* c, const int32_t* flag, int32_t* count, int32_t* cu
   return CBM_OK;
 }
 
+static BrickSrc brick_src(cbm_ctx* c) {
+  BrickSrc b;
+  std::memset(&b, 0, sizeof(b));
+  b.sims = c->d_sims;
+  b.block_off = c->d_block_off;
+  b.brick = c->bricks;
+  b.bucket_start = c->start;
+  b.cell_range = c->cell_range;
+  for (int a = 0; a < 3; ++a) b.bdim[a] = c->bdim[a];
+  b.cap = c->brick_cap;
+  b.brick_skip = c->brick_skip;
+  b.blk_flag = c->blk_flag;
+  b.brick_over = (unsigned long long*)(c->counters + 2);
+  return b;
+}
+
 static int rebuild_chain(cbm_ctx* c, const int32_t* flag, int32_t* flag_clear, bool tick, bool copy_back) {
   RebuildArgs a = rebuild_args(c, flag, flag_clear, tick);
   HIPCHK(c, launch_keys(c->nblocks, c->block, c->stream, a));
@@ -674,6 +696,12 @@ static int rebuild_chain(cbm_ctx* c, const int32_t* flag, int32_t* flag_clear, b
     va.rl2 = c->list_r2;
     va.lds_r = verlet_lds_r(c);
     va.build = c->verlet ? 1 : 0;
+    if (va.build && c->bricks) {
+      // brick tiles, then the rows of the bricks too dense for LDS
+      va.bs = brick_src(c);
+      HIPCHK(c, launch_verlet_brick(c->dims, c->m.periodic, c->nbricks, c->block, c->brick_lds, c->stream, va));
+      va.fixup = 1;
+    }
     HIPCHK(c, launch_verlet(c->dims, c->nblocks, c->block, c->stream, va));
   } else {
     for (int k = 0; k < C_NCOLS; ++k) if (c->has_col[k]) c->col[k].cidx ^= 1;
@@ -1248,9 +1276,10 @@ int cbm_get_bricks(cbm_ctx* c, int32_t* bdim, int32_t* cap, int64_t* nbricks, in
   int64_t v[4];
   HIPCHK(c, hipMemcpyAsync(v, c->counters, sizeof(v), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  for (int a = 0; a < 3; ++a) bdim[a] = c->cl ? c->bdim[a] : 0;
-  *cap = c->cl ? c->brick_cap : 0;
-  *nbricks = c->cl ? c->nbricks : 0;
+  const bool tiled = c->bricks != nullptr;
+  for (int a = 0; a < 3; ++a) bdim[a] = tiled ? c->bdim[a] : 0;
+  *cap = tiled ? c->brick_cap : 0;
+  *nbricks = tiled ? c->nbricks : 0;
   *overflows = v[2];
   return CBM_OK;
 }

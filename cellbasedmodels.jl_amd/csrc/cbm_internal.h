@@ -75,7 +75,8 @@ struct VerletArgs {
   int32_t build;              // 0: copy-back only (no list)
   int32_t nopermute;          // 1: rows stay in place; candidates via tmp, cells via lin
   int32_t lds_r;              // list entries a lane keeps sorted in LDS before spilling
-  int32_t pad;
+  int32_t fixup;              // 1: only the rows of bricks k_verlet_brick left (brick_skip)
+  BrickSrc bs;                // brick tiling (k_verlet_brick and its fixup)
 };
 
 // LDS budget of the list build: lds_r * block * 8 bytes.
@@ -219,6 +220,10 @@ hipError_t launch_ranges(hipStream_t s, const RebuildArgs& a);
 hipError_t launch_scatter(int nblocks, int block, hipStream_t s, const RebuildArgs& a);
 hipError_t launch_permute(int nblocks, int block, hipStream_t s, const PermuteArgs& a);
 hipError_t launch_verlet(int dims, int nblocks, int block, hipStream_t s, const VerletArgs& a);
+// Verlet list build on LDS brick tiles (one block per brick), to be followed
+// by launch_verlet with fixup = 1 for the bricks too dense for LDS.
+hipError_t launch_verlet_brick(int dims, int periodic, int64_t nbricks, int block, size_t lds, hipStream_t s,
+                               const VerletArgs& a);
 hipError_t launch_set_flag(hipStream_t s, int32_t* flag, int32_t v);
 hipError_t launch_tick(hipStream_t s, int64_t* counter);
 hipError_t launch_xfer(int nblocks, int block, hipStream_t s, const XferArgs& a);

@@ -441,11 +441,154 @@ __global__ __launch_bounds__(256) void k_verlet(VerletArgs va) {
   extern __shared__ int2 vbuf[];
   const RebuildArgs& a = va.rb;
   if (a.flag && *a.flag == 0) return;
+  if (va.fixup) {
+    // only blocks holding rows of bricks too dense for k_verlet_brick
+    const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    if (!va.bs.blk_flag[b]) return;
+    __syncthreads();
+    if (threadIdx.x == 0) va.bs.blk_flag[b] = 0;
+  }
   const RowIdx ri = row_index(a.g);
   const SimDev& sd = a.g.sims[ri.sim];
   const int64_t N = a.g.st[ri.sim].N;
+  if (va.fixup) {
+    if (ri.local >= sd.n_max || !va.bs.brick_skip[ri.row]) return;
+    va.bs.brick_skip[ri.row] = 0;
+  }
   if (ri.local >= N) return;
   (void)verlet_row<D>(va, sd, ri.row, vbuf + threadIdx.x, (int)blockDim.x);
+}
+
+// Batcher's odd-even merge sort of N keys held in registers (every index is
+// a compile-time constant once the loops unroll).
+template <int N>
+__device__ __forceinline__ void sort_net(int64_t* k) {
+#pragma unroll
+  for (int p = 1; p < N; p <<= 1)
+#pragma unroll
+    for (int kk = p; kk >= 1; kk >>= 1)
+#pragma unroll
+      for (int j = kk % p; j + kk < N; j += 2 * kk)
+#pragma unroll
+        for (int i = 0; i < kk && i + j + kk < N; ++i)
+          if ((i + j) / (2 * p) == (i + j + kk) / (2 * p)) {
+            const int64_t x = k[i + j], y = k[i + j + kk];
+            k[i + j] = x < y ? x : y;
+            k[i + j + kk] = x < y ? y : x;
+          }
+}
+
+// The list build on LDS brick tiles: the halo of a brick (positions and
+// slots of the sorted rows) is staged once; each home agent walks its 3^D
+// cells there, queues the candidates within the list radius (the same exact
+// d^2 test as k_verlet) in a per-lane LDS queue, sorts them by slot in
+// registers and writes them as rows.  A list longer than the queue is rebuilt by insertion into
+// the global list, as k_verlet does past its LDS buffer, so the lists equal
+// k_verlet's.  Also copies the sorted rows back and snapshots x_ref.
+template <int D, int PER>
+__global__ __launch_bounds__(256) void k_verlet_brick(VerletArgs va) {
+  extern __shared__ unsigned char vlds[];
+  const RebuildArgs& a = va.rb;
+  if (a.flag && *a.flag == 0) return;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int cap = va.bs.cap;
+  const int H = brick_halo_cells(D, va.bs.bdim[0], D > 1 ? va.bs.bdim[1] : 1, D > 2 ? va.bs.bdim[2] : 1);
+  BrickTile t;
+  t.hdr = (int32_t*)vlds;
+  double* pos[3] = {nullptr, nullptr, nullptr};
+  unsigned char* p = vlds + 4 * BRICK_HDR;
+#pragma unroll
+  for (int c = 0; c < D; ++c) { pos[c] = (double*)p; p += (size_t)8 * cap; }
+  int32_t* slot = (int32_t*)p;
+  p += (size_t)4 * cap;
+  t.hs = (int32_t*)p;
+  p += (size_t)4 * (H + 1);
+  t.hrow = (int32_t*)p;
+  p += (size_t)4 * H;
+  uint16_t* lst = (uint16_t*)p;       // lane queue, entry u at [u * nt + tid], sorted by slot
+  p += (size_t)2 * VBRICK_Q * BRICK_MAX_BLOCK;
+  t.owner = (uint16_t*)p;
+  p += (size_t)2 * cap;
+  t.home_li = (uint16_t*)p;
+
+  BrickCols cols;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) { cols.S[c] = va.apos[c]; cols.pos[c] = pos[c]; }
+  cols.xd = nullptr;
+  cols.xd_lds = nullptr;
+  cols.xi = va.aslot;
+  cols.xi_lds = slot;
+  double unused = 0.0;
+  const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  if (brick_stage<D, PER, 4, false, BX_I32>(va.bs, b, t, cols, unused) != 1) return;
+
+  const SimDev& sd = a.g.sims[t.sim];
+  const int64_t N = a.g.st[t.sim].N;
+  double L[3], hL[3];
+  box_halves<D, PER>(sd, L, hL);
+  auto row_of = [&](int32_t q) {
+    const int h = t.owner[q] & 0x7FFF;
+    return (int64_t)t.hrow[h] + (q - t.hs[h]);
+  };
+  const int32_t nh = t.r1 - t.r0;
+  for (int32_t k = tid; k < nh; k += nt) {
+    const int32_t li = t.home_li[k];
+    const int h = t.owner[li] & 0x7FFF;
+    const int64_t i = row_of(li);
+    if (i - sd.row_off >= N) continue;
+    double xi[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < D; ++c) xi[c] = pos[c][li];
+    auto within = [&](int32_t q) {
+      double dx[3];
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        dx[c] = xi[c] - pos[c][q];
+        if (PER) dx[c] = wrap_dx(dx[c], L[c], hL[c]);
+      }
+      double s2 = dx[0] * dx[0];
+      if (D > 1) s2 = s2 + dx[1] * dx[1];
+      if (D > 2) s2 = s2 + dx[2] * dx[2];
+      return s2 <= va.rl2 && q != li;   // sqrt(s2) < r_list, not the agent itself
+    };
+    int32_t nk = 0;
+    brick_walk<D>(t, h, [&](int32_t q) {
+      if (!within(q)) return;
+      if (nk < VBRICK_Q) lst[nk * nt + tid] = (uint16_t)q;
+      ++nk;
+    });
+    int32_t kept;
+    if (nk <= VBRICK_Q) {
+      // (slot, LDS index) keys; unused entries sort last
+      int64_t key[VBRICK_Q];
+#pragma unroll
+      for (int u = 0; u < VBRICK_Q; ++u) {
+        const int32_t q = u < nk ? lst[u * nt + tid] : 0;
+        key[u] = u < nk ? ((int64_t)slot[q] << 16) | q : INT64_MAX;
+      }
+      sort_net<VBRICK_Q>(key);
+      kept = nk < va.cap ? nk : va.cap;
+#pragma unroll
+      for (int u = 0; u < VBRICK_Q; ++u)
+        if (u < kept) va.list[(int64_t)u * va.ldl + i] = (int32_t)row_of((int32_t)(key[u] & 0xFFFF));
+      if (nk > va.cap) *va.overflow = 1;
+    } else {
+      int32_t cnt = 0;
+      brick_walk<D>(t, h, [&](int32_t q) {
+        if (within(q)) verlet_accept(va, i, nullptr, 0, 0, cnt, (int32_t)row_of(q), slot[q]);
+      });
+      kept = cnt < va.cap ? cnt : va.cap;
+    }
+    va.list_n[i] = kept;
+    for (int c = 0; c < va.cols.n; ++c) {
+      if (va.cols.bytes[c] == 8)
+        ((uint64_t*)va.cols.cur[c])[i] = ((const uint64_t*)va.cols.alt[c])[i];
+      else
+        ((uint32_t*)va.cols.cur[c])[i] = ((const uint32_t*)va.cols.alt[c])[i];
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) va.xref[c][i] = xi[c];
+  }
 }
 
 
@@ -980,6 +1123,20 @@ hipError_t launch_verlet(int dims, int nblocks, int block, hipStream_t s, const 
     case 1: hipLaunchKernelGGL(k_verlet<1>, dim3(nblocks), dim3(block), lds, s, a); break;
     case 2: hipLaunchKernelGGL(k_verlet<2>, dim3(nblocks), dim3(block), lds, s, a); break;
     default: hipLaunchKernelGGL(k_verlet<3>, dim3(nblocks), dim3(block), lds, s, a); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_verlet_brick(int dims, int periodic, int64_t nbricks, int block, size_t lds, hipStream_t s,
+                               const VerletArgs& a) {
+  const dim3 g((unsigned)nbricks), bl(block);
+  switch (dims * 2 + (periodic ? 1 : 0)) {
+    case 2: hipLaunchKernelGGL((k_verlet_brick<1, 0>), g, bl, lds, s, a); break;
+    case 3: hipLaunchKernelGGL((k_verlet_brick<1, 1>), g, bl, lds, s, a); break;
+    case 4: hipLaunchKernelGGL((k_verlet_brick<2, 0>), g, bl, lds, s, a); break;
+    case 5: hipLaunchKernelGGL((k_verlet_brick<2, 1>), g, bl, lds, s, a); break;
+    case 6: hipLaunchKernelGGL((k_verlet_brick<3, 0>), g, bl, lds, s, a); break;
+    default: hipLaunchKernelGGL((k_verlet_brick<3, 1>), g, bl, lds, s, a); break;
   }
   return hipGetLastError();
 }

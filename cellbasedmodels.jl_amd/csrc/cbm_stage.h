@@ -255,38 +255,40 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
 #pragma unroll
       for (int c = 0; c < D; ++c)
         res[c] = Uc(c) + ((a.kacc[c][i] + du[c]) * dt + (a.noise + a.noise) * z[c] * a.sqdt) / 2;
-      for (int v = D; v < nv; ++v) res[v] = Uc(v) + ((a.kacc[v][i] + du[v]) * dt) / 2;
+      for (int v = D; v < nv; ++v) res[v] = a.U[v][i] + ((a.kacc[v][i] + du[v]) * dt) / 2;
     } else {
 #pragma unroll
       for (int c = 0; c < D; ++c) res[c] = (Uc(c) + du[c] * dt) + a.noise * z[c] * a.sqdt;
-      for (int v = D; v < nv; ++v) res[v] = Uc(v) + du[v] * dt;   // no noise off the positions
+      for (int v = D; v < nv; ++v) res[v] = a.U[v][i] + du[v] * dt;   // no noise off the positions
       if (a.kind == ST_EH1)
         for (int c = 0; c < nv; ++c) a.kacc[c][i] = du[c];
     }
   } else {
-    switch (a.kind) {
-      case ST_EULER:
-        for (int c = 0; c < nv; ++c) res[c] = Uc(c) + du[c] * dt;
-        break;
-      case ST_HEUN1:
-        for (int c = 0; c < nv; ++c) res[c] = Uc(c) + du[c] * dt;
-        break;
-      case ST_HEUN2:
-        for (int c = 0; c < nv; ++c) res[c] = (Sc(c) + (Uc(c) + du[c] * dt)) / 2;
-        break;
-      case ST_RK4_1:
-        for (int c = 0; c < nv; ++c) { a.kacc[c][i] = du[c]; res[c] = Uc(c) + du[c] * dt / 2; }
-        break;
-      case ST_RK4_2:
-        for (int c = 0; c < nv; ++c) { a.kacc[c][i] = a.kacc[c][i] + 2 * du[c]; res[c] = Uc(c) + du[c] * dt / 2; }
-        break;
-      case ST_RK4_3:
-        for (int c = 0; c < nv; ++c) { a.kacc[c][i] = a.kacc[c][i] + 2 * du[c]; res[c] = Uc(c) + du[c] * dt; }
-        break;
-      default:  // ST_RK4_4
-        for (int c = 0; c < nv; ++c) res[c] = Uc(c) + (a.kacc[c][i] + du[c]) * dt / 6;
-        break;
-    }
+    // variable c from its u; positions first (c known at compile time, so
+    // an early-loaded u stays in registers), then the other DE variables
+    auto upd = [&](int c, double u) -> double {
+      switch (a.kind) {
+        case ST_EULER:
+        case ST_HEUN1:
+          return u + du[c] * dt;
+        case ST_HEUN2:
+          return (Sc(c) + (u + du[c] * dt)) / 2;
+        case ST_RK4_1:
+          a.kacc[c][i] = du[c];
+          return u + du[c] * dt / 2;
+        case ST_RK4_2:
+          a.kacc[c][i] = a.kacc[c][i] + 2 * du[c];
+          return u + du[c] * dt / 2;
+        case ST_RK4_3:
+          a.kacc[c][i] = a.kacc[c][i] + 2 * du[c];
+          return u + du[c] * dt;
+        default:  // ST_RK4_4
+          return u + (a.kacc[c][i] + du[c]) * dt / 6;
+      }
+    };
+#pragma unroll
+    for (int c = 0; c < D; ++c) res[c] = upd(c, Uc(c));
+    for (int v = D; v < nv; ++v) res[v] = upd(v, a.U[v][i]);
   }
 
   const bool final_stage = a.kind == ST_EULER || a.kind == ST_EM || a.kind == ST_HEUN2 ||
@@ -385,55 +387,61 @@ __device__ __forceinline__ double order_val(long long k) {
   return __longlong_as_double(k < 0 ? (long long)(~(unsigned long long)k | 0x8000000000000000ull) : k);
 }
 
-template <int D, class Law, int PER, int NOISE>
-__device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* lds) {
-  constexpr bool SOFT = same_type<Law, PairLaw<LAW_SOFT_SPHERES>>::value;
-  // Built-in laws walk the candidates twice: a cheap pass keeps the LDS
-  // indices of the pairs within the law's reach (in walk order, up to
-  // BRICK_LIST per lane in an LDS queue), and the full pair body runs on
-  // those only.  Every pair left out is inactive, so the sum is unchanged.
-  constexpr bool FILTER = SOFT || has_reach<Law>::value;
-  constexpr int U4 = 4;               // loads issued together per thread while staging
-  const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-  const int32_t* bt = a.brick + BRICK_WORDS * (int64_t)b;
-  const int sim = bt[0];
-  const int32_t o[3] = {bt[1], bt[2], bt[3]};
-  const SimDev& sd = a.g.sims[sim];
-  const int64_t N = a.g.st[sim].N;
-  // the brick's cells hold consecutive buckets [b0, b1): its rows are one range
-  const int32_t r0 = a.bucket_start[bt[4]], r1 = a.bucket_start[bt[5]];
-  if (r0 >= r1) return;               // no agent in this brick
-  const int B[3] = {a.bdim[0], D > 1 ? a.bdim[1] : 1, D > 2 ? a.bdim[2] : 1};
-  const int HX = B[0] + 2, HY = D > 1 ? B[1] + 2 : 1;
-  const int H = brick_halo_cells(D, B[0], B[1], B[2]);
-  const int cap = a.brick_cap;
+
+// One brick's LDS tile.  The caller points hdr, hs, hrow, owner and home_li
+// into its LDS; brick_stage fills the rest.
+struct BrickTile {
+  int sim;
+  int32_t o[3];                   // origin cell
+  int B[3], HX, HY, H;            // brick edge, halo extent, halo cells
+  int32_t r0, r1, T;              // home rows [r0, r1); staged agents
+  int32_t* hdr;                   // BRICK_HDR int32 words of scratch
+  int32_t* hs;                    // LDS offset of halo cell h's agents, hs[H] = T
+  int32_t* hrow;                  // first global row of halo cell h
+  uint16_t* owner;                // halo cell of each staged agent, | 0x8000 at its home cell
+  uint16_t* home_li;              // LDS index of home row r0 + k
+};
+
+// What brick_stage copies per agent besides the positions: an optional
+// double column (radius; its block maximum is returned, +inf if any is NaN)
+// and an optional integer column read as int64 (fate) or int32 (slot).
+enum { BX_NONE = 0, BX_I64 = 1, BX_I32 = 2 };
+struct BrickCols {
+  const double* S[3];       // positions, S[c][row]
+  double* pos[3];           // their LDS arrays
+  const double* xd;         // double column (or NULL)
+  double* xd_lds;
+  const void* xi;           // integer column
+  int32_t* xi_lds;
+};
+
+// Stages brick b: the rows and counts of its halo cells (wrapped on a
+// periodic grid, empty outside a bounded one), their exclusive scan, then
+// the columns of every halo agent, U4 agents per thread in flight, at its
+// LDS index q (halo-cell order, x fastest).  Returns 0 for an empty brick, 2
+// for one whose halo exceeds cap (its rows are then marked for the
+// row-parallel kernel), 1 when staged.
+template <int D, int PER, int U4, bool XD, int XI>
+__device__ __forceinline__ int brick_stage(const BrickSrc& src, int b, BrickTile& t, const BrickCols& cols,
+                                           double& xd_max) {
   const int tid = threadIdx.x, nt = blockDim.x;
+  const int32_t* bt = src.brick + BRICK_WORDS * (int64_t)b;
+  t.sim = bt[0];
+  t.o[0] = bt[1]; t.o[1] = bt[2]; t.o[2] = bt[3];
+  const SimDev& sd = src.sims[t.sim];
+  // the brick's cells hold consecutive buckets [b0, b1): its rows are one range
+  t.r0 = src.bucket_start[bt[4]];
+  t.r1 = src.bucket_start[bt[5]];
+  if (t.r0 >= t.r1) return 0;
+  t.B[0] = src.bdim[0]; t.B[1] = D > 1 ? src.bdim[1] : 1; t.B[2] = D > 2 ? src.bdim[2] : 1;
+  t.HX = t.B[0] + 2;
+  t.HY = D > 1 ? t.B[1] + 2 : 1;
+  t.H = brick_halo_cells(D, t.B[0], t.B[1], t.B[2]);
+  const int HX = t.HX, HY = t.HY, H = t.H;
+  int32_t* hs = t.hs;
+  int32_t* hrow = t.hrow;
+  const CellRange* __restrict__ cr = src.cell_range + sd.rank_off;
 
-  int32_t* hdr = (int32_t*)lds;
-  long long* rmax_key = (long long*)(lds + 48);
-  double* pos[3];
-  unsigned char* p = lds + 4 * BRICK_HDR;
-#pragma unroll
-  for (int c = 0; c < D; ++c) { pos[c] = (double*)p; p += (size_t)8 * cap; }
-  double* rad = (double*)p;
-  if (SOFT) p += (size_t)8 * cap;
-  int32_t* fate = (int32_t*)p;
-  if (SOFT) p += (size_t)4 * cap;
-  int32_t* hs = (int32_t*)p;          // LDS offset of halo cell h's agents, hs[H] = total
-  p += (size_t)4 * (H + 1);
-  int32_t* hrow = (int32_t*)p;        // first global row of halo cell h
-  p += (size_t)4 * H;
-  uint16_t* lst = (uint16_t*)p;       // per-lane queue of kept candidates, entry u at [u * nt + tid]
-  p += (size_t)2 * BRICK_LIST * BRICK_MAX_BLOCK;
-  uint16_t* owner = (uint16_t*)p;     // halo cell of each staged agent, | 0x8000 at its home cell
-  p += (size_t)2 * cap;
-  uint16_t* home_li = (uint16_t*)p;   // LDS index of home row r0 + k
-
-  const CellRange* __restrict__ cr = a.cell_range + sd.rank_off;
-  if (SOFT && tid == 0) *rmax_key = order_key(-__builtin_inf());
-
-  // 1. rows and counts of the halo cells (wrapped on a periodic grid, empty
-  //    outside a bounded one), U4 cells per thread in flight
   // halo cell h -> (hx, hy, hz), x fastest.  h + 0.5 lies at least 0.5 from
   // a multiple of the divisor, far beyond the float product's rounding for
   // h < 2^16, so the truncated products are the exact quotients.
@@ -448,8 +456,8 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
     int32_t q[3] = {0, 0, 0};
 #pragma unroll
     for (int e = 0; e < D; ++e) {
-      int32_t v = o[e] + hh[e] - 1;
-      home = home && hh[e] >= 1 && hh[e] <= B[e] && v < sd.gn[e];
+      int32_t v = t.o[e] + hh[e] - 1;
+      home = home && hh[e] >= 1 && hh[e] <= t.B[e] && v < sd.gn[e];
       if (PER) {
         if (v == -1) v = sd.gn[e] - 1;
         else if (v == sd.gn[e]) v = 0;
@@ -461,6 +469,7 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
     }
     return ok ? q[0] + sd.gn[0] * (q[1] + sd.gn[1] * q[2]) : -1;
   };
+  // 1. rows and counts of the halo cells, U4 cells per thread in flight
   for (int h0 = tid; h0 < H; h0 += U4 * nt) {
     CellRange r[U4];
 #pragma unroll
@@ -491,103 +500,191 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
       const int32_t y = __shfl_up(incl, off, 64);
       if (lane >= off) incl += y;
     }
-    if (lane == 63) hdr[4 + w] = incl;
+    if (lane == 63) t.hdr[4 + w] = incl;
     __syncthreads();
     int32_t run = incl - sum, tot = 0;
     for (int q = 0; q < nw; ++q) {
-      const int32_t ws = hdr[4 + q];
+      const int32_t ws = t.hdr[4 + q];
       if (q < w) run += ws;
       tot += ws;
     }
     for (int h = c0; h < c1; ++h) { const int32_t v = hs[h]; hs[h] = run; run += v; }
-    if (tid == 0) { hs[H] = tot; hdr[2] = tot; }
+    if (tid == 0) { hs[H] = tot; t.hdr[2] = tot; }
   }
   __syncthreads();
-  const int32_t T = hdr[2];
-  if (T > cap) {
+  t.T = t.hdr[2];
+  const int32_t r0 = t.r0, r1 = t.r1;
+  if (t.T > src.cap) {
     // too dense for LDS: the row-parallel kernel launched after this one
     // takes the brick's rows
-    if (tid == 0) atomicAdd(a.brick_over, 1ull);
-    for (int64_t i = (int64_t)r0 + tid; i < (int64_t)r1; i += nt) a.brick_skip[i] = 1;
+    if (tid == 0) atomicAdd(src.brick_over, 1ull);
+    for (int64_t i = (int64_t)r0 + tid; i < (int64_t)r1; i += nt) src.brick_skip[i] = 1;
     // wake the row-kernel blocks holding these rows (same block size)
-    const int64_t q0 = a.g.block_off[sim] + ((int64_t)r0 - sd.row_off) / nt;
-    const int64_t q1 = a.g.block_off[sim] + ((int64_t)r1 - 1 - sd.row_off) / nt;
-    for (int64_t q = q0 + tid; q <= q1; q += nt) a.blk_flag[q] = 1;
-    return;
+    const int64_t q0 = src.block_off[t.sim] + ((int64_t)r0 - sd.row_off) / nt;
+    const int64_t q1 = src.block_off[t.sim] + ((int64_t)r1 - 1 - sd.row_off) / nt;
+    for (int64_t q = q0 + tid; q <= q1; q += nt) src.blk_flag[q] = 1;
+    return 2;
   }
 
-  // 3. stage the halo agents (positions, and radius and fate for soft
-  //    spheres), U4 agents per thread in flight; each home row notes the LDS
-  //    index of its copy at its home cell
-  {
-    for (int h = tid; h < H; h += nt) {
-      bool home;
-      (void)halo_cell(h, home);
-      const uint16_t tag = (uint16_t)(h | (home ? 0x8000 : 0));
-      for (int32_t q = hs[h]; q < hs[h + 1]; ++q) owner[q] = tag;
-    }
-    __syncthreads();
-    double rm = -__builtin_inf();
-    bool rnan = false;
-    for (int32_t q0 = tid; q0 < T; q0 += U4 * nt) {
-      double v[U4][3];
-      double vr[U4];
-      int32_t vf[U4];
-      int64_t row[U4];
-      int tag[U4];
-#pragma unroll
-      for (int u = 0; u < U4; ++u) {
-        const int32_t q = q0 + u * nt;
-        tag[u] = q < T ? owner[q] : 0;
-        const int h = tag[u] & 0x7FFF;
-        row[u] = q < T ? (int64_t)hrow[h] + (q - hs[h]) : -1;
-        const int64_t rr = row[u] < 0 ? (int64_t)r0 : row[u];
-#pragma unroll
-        for (int c = 0; c < D; ++c) v[u][c] = a.S[c][rr];
-        if (SOFT) { vr[u] = a.rad[rr]; vf[u] = (int32_t)a.fate[rr]; }
-      }
-#pragma unroll
-      for (int u = 0; u < U4; ++u) {
-        const int32_t q = q0 + u * nt;
-        if (q >= T) continue;
-#pragma unroll
-        for (int c = 0; c < D; ++c) pos[c][q] = v[u][c];
-        if (SOFT) {
-          rad[q] = vr[u];
-          fate[q] = vf[u];
-          rm = vr[u] > rm ? vr[u] : rm;
-          rnan = rnan || vr[u] != vr[u];
-        }
-        if (tag[u] & 0x8000) home_li[row[u] - r0] = (uint16_t)q;
-      }
-    }
-    if (SOFT) atomicMax(rmax_key, rnan ? order_key(__builtin_inf()) : order_key(rm));
-    __syncthreads();
+  // 3. stage the halo agents; each home row notes the LDS index of its copy
+  //    at its home cell
+  for (int h = tid; h < H; h += nt) {
+    bool home;
+    (void)halo_cell(h, home);
+    const uint16_t tag = (uint16_t)(h | (home ? 0x8000 : 0));
+    for (int32_t q = hs[h]; q < hs[h + 1]; ++q) t.owner[q] = tag;
   }
+  __syncthreads();
+  const int32_t T = t.T;
+  double xm = -__builtin_inf();
+  bool xnan = false;
+  for (int32_t q0 = tid; q0 < T; q0 += U4 * nt) {
+    int64_t row[U4];
+    int tag[U4];
+    double v[U4][3], vd[U4];
+    int32_t vi[U4];
+#pragma unroll
+    for (int u = 0; u < U4; ++u) {
+      const int32_t q = q0 + u * nt;
+      tag[u] = q < T ? t.owner[q] : 0;
+      const int h = tag[u] & 0x7FFF;
+      row[u] = q < T ? (int64_t)hrow[h] + (q - hs[h]) : -1;
+      const int64_t rr = row[u] < 0 ? (int64_t)r0 : row[u];
+#pragma unroll
+      for (int c = 0; c < D; ++c) v[u][c] = cols.S[c][rr];
+      if (XD) vd[u] = cols.xd[rr];
+      if (XI == BX_I64) vi[u] = (int32_t)((const int64_t*)cols.xi)[rr];
+      if (XI == BX_I32) vi[u] = ((const int32_t*)cols.xi)[rr];
+    }
+#pragma unroll
+    for (int u = 0; u < U4; ++u) {
+      const int32_t q = q0 + u * nt;
+      if (q >= T) continue;
+#pragma unroll
+      for (int c = 0; c < D; ++c) cols.pos[c][q] = v[u][c];
+      if (XD) {
+        cols.xd_lds[q] = vd[u];
+        xm = vd[u] > xm ? vd[u] : xm;
+        xnan = xnan || vd[u] != vd[u];
+      }
+      if (XI != BX_NONE) cols.xi_lds[q] = vi[u];
+      if (tag[u] & 0x8000) t.home_li[row[u] - r0] = (uint16_t)q;
+    }
+  }
+  if (XD) {
+    // block maximum of the double column through an order-preserving key
+    long long* key = (long long*)(t.hdr + 12);
+    if (tid == 0) *key = order_key(-__builtin_inf());
+    __syncthreads();
+    atomicMax(key, xnan ? order_key(__builtin_inf()) : order_key(xm));
+  }
+  __syncthreads();
+  if (XD) xd_max = order_val(*(const long long*)(t.hdr + 12));
+  return 1;
+}
+
+// Candidates of a home agent in halo cell h: its 3^(D-1) runs of three
+// x-adjacent cells (z outer, then y); run (ry, rz) covers cells hb .. hb + 2.
+template <int D>
+__device__ __forceinline__ int brick_run0(const BrickTile& t, int h) {
+  return h - 1 - (D > 1 ? t.HX : 0) - (D > 2 ? t.HX * t.HY : 0);
+}
+template <int D>
+__device__ __forceinline__ int32_t brick_ncand(const BrickTile& t, int h) {
+  const int hb0 = brick_run0<D>(t, h);
+  int32_t n = 0;
+#pragma unroll
+  for (int rz = 0; rz < (D > 2 ? 3 : 1); ++rz)
+#pragma unroll
+    for (int ry = 0; ry < (D > 1 ? 3 : 1); ++ry) {
+      const int hb = hb0 + rz * t.HX * t.HY + ry * t.HX;
+      n += t.hs[hb + 3] - t.hs[hb];
+    }
+  return n;
+}
+// Visits the candidates of a home agent in halo cell h in walk order, as one
+// flattened sequence, so the lanes of a wavefront stay in step.
+template <int D, class Visit>
+__device__ __forceinline__ void brick_walk(const BrickTile& t, int h, Visit&& visit) {
+  const int32_t ncand = brick_ncand<D>(t, h);
+  int ry = 0;
+  int hb = brick_run0<D>(t, h);
+  int32_t q = t.hs[hb], qe = t.hs[hb + 3];
+  for (int32_t k = 0; k < ncand; ++k) {
+    while (q == qe) {   // the next non-empty run; one exists while k < ncand
+      if (++ry == 3) { ry = 0; hb += t.HX * t.HY - 2 * t.HX; } else { hb += t.HX; }
+      q = t.hs[hb];
+      qe = t.hs[hb + 3];
+    }
+    visit(q);
+    ++q;
+  }
+}
+
+template <int D, class Law, int PER, int NOISE>
+__device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* lds) {
+  constexpr bool SOFT = same_type<Law, PairLaw<LAW_SOFT_SPHERES>>::value;
+  // Built-in laws walk the candidates twice: a cheap pass keeps the LDS
+  // indices of the pairs within the law's reach (in walk order, up to
+  // BRICK_LIST per lane in an LDS queue), and the full pair body runs on
+  // those only.  Every pair left out is inactive, so the sum is unchanged.
+  constexpr bool FILTER = SOFT || has_reach<Law>::value;
+  constexpr int U4 = 4;               // loads issued together per thread while staging
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int cap = a.brick_cap;
+  BrickSrc src;
+  src.sims = a.g.sims; src.block_off = a.g.block_off; src.brick = a.brick; src.bucket_start = a.bucket_start;
+  src.cell_range = a.cell_range; src.cap = cap;
+  for (int e = 0; e < 3; ++e) src.bdim[e] = a.bdim[e];
+  src.brick_skip = a.brick_skip; src.blk_flag = a.blk_flag; src.brick_over = a.brick_over;
+  const int H = brick_halo_cells(D, a.bdim[0], D > 1 ? a.bdim[1] : 1, D > 2 ? a.bdim[2] : 1);
+
+  BrickTile t;
+  t.hdr = (int32_t*)lds;
+  double* pos[3];
+  unsigned char* p = lds + 4 * BRICK_HDR;
+#pragma unroll
+  for (int c = 0; c < D; ++c) { pos[c] = (double*)p; p += (size_t)8 * cap; }
+  double* rad = (double*)p;
+  if (SOFT) p += (size_t)8 * cap;
+  int32_t* fate = (int32_t*)p;
+  if (SOFT) p += (size_t)4 * cap;
+  t.hs = (int32_t*)p;
+  p += (size_t)4 * (H + 1);
+  t.hrow = (int32_t*)p;
+  p += (size_t)4 * H;
+  uint16_t* lst = (uint16_t*)p;       // per-lane queue of kept candidates, entry u at [u * nt + tid]
+  p += (size_t)2 * BRICK_LIST * BRICK_MAX_BLOCK;
+  t.owner = (uint16_t*)p;
+  p += (size_t)2 * cap;
+  t.home_li = (uint16_t*)p;
+
+  // staged columns: positions (and radius and fate for soft spheres)
+  BrickCols cols;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) { cols.S[c] = a.S[c]; cols.pos[c] = pos[c]; }
+  cols.xd = a.rad;
+  cols.xd_lds = rad;
+  cols.xi = a.fate;
+  cols.xi_lds = fate;
+  double rmax = 0.0;
+  const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  if (brick_stage<D, PER, U4, SOFT, SOFT ? BX_I64 : BX_NONE>(src, b, t, cols, rmax) != 1) return;
+  const int sim = t.sim;
+  const SimDev& sd = a.g.sims[sim];
+  const int64_t N = a.g.st[sim].N;
+  const int32_t* hs = t.hs;
+  const int32_t* hrow = t.hrow;
+  const uint16_t* owner = t.owner;
+  const uint16_t* home_li = t.home_li;
 
   // 4. the home agents, one per thread
-  // candidates of a home agent in halo cell h: its 3^(D-1) runs of three
-  // x-adjacent cells (z outer, then y); run (ry, rz) covers cells hb .. hb + 2
-  auto run0 = [&](int h) { return h - 1 - (D > 1 ? HX : 0) - (D > 2 ? HX * HY : 0); };
-  auto ncand_of = [&](int h) {
-    const int hb0 = run0(h);
-    int32_t n = 0;
-#pragma unroll
-    for (int rz = 0; rz < (D > 2 ? 3 : 1); ++rz)
-#pragma unroll
-      for (int ry = 0; ry < (D > 1 ? 3 : 1); ++ry) {
-        const int hb = hb0 + rz * HX * HY + ry * HX;
-        n += hs[hb + 3] - hs[hb];
-      }
-    return n;
-  };
-  const int32_t nh = r1 - r0;
+  const int32_t nh = t.r1 - t.r0;
   const Law law(sd.p);
   double L[3], hL[3];
   box_halves<D, PER>(sd, L, hL);
   double reach2 = __builtin_inf();
   if constexpr (FILTER && !SOFT) reach2 = reach2_bound(law.reach());
-  const double rmax = SOFT ? order_val(*rmax_key) : 0.0;
   for (int32_t k = tid; k < nh; k += nt) {
     const int32_t li = home_li[k];
     const int h = owner[li] & 0x7FFF;
@@ -605,24 +702,7 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
       const int64_t fate_i = SOFT ? fate[li] : 0;
       double keep2 = reach2;
       if constexpr (SOFT) keep2 = law.mu >= 0.0 ? reach2_bound(law.mu * (rad_i + rmax)) : __builtin_inf();
-      // the runs walked as one flattened sequence of ncand candidates, so
-      // the lanes of a wavefront stay in step
-      const int hb0 = run0(h);
-      const int32_t ncand = ncand_of(h);
-      auto walk = [&](auto&& visit) {
-        int ry = 0;
-        int hb = hb0;
-        int32_t q = hs[hb], qe = hs[hb + 3];
-        for (int32_t k = 0; k < ncand; ++k) {
-          while (q == qe) {   // the next non-empty run; one exists while k < ncand
-            if (++ry == 3) { ry = 0; hb += HX * HY - 2 * HX; } else { hb += HX; }
-            q = hs[hb];
-            qe = hs[hb + 3];
-          }
-          visit(q);
-          ++q;
-        }
-      };
+      auto walk = [&](auto&& visit) { brick_walk<D>(t, h, visit); };
       auto body = [&](int32_t q, bool live) {
         double xj[3] = {0.0, 0.0, 0.0};
 #pragma unroll

@@ -100,6 +100,36 @@ __host__ __device__ inline int64_t brick_lds_bytes(int D, bool soft, int cap, in
   return (b + 15) & ~(int64_t)15;
 }
 
+// Inputs of the kernels that tile cell bricks (the stage, and the Verlet
+// list build).
+struct BrickSrc {
+  const SimDev* sims;
+  const int32_t* block_off;       // row-kernel block of each simulation's first row
+  const int32_t* brick;           // BRICK_WORDS per brick
+  const int32_t* bucket_start;    // first row of each global bucket
+  const CellRange* cell_range;    // rank_off + linear cell -> rows
+  int32_t bdim[3];
+  int32_t cap;                    // agents a brick can stage
+  uint8_t* brick_skip;            // per row: left to the row-parallel kernel
+  uint8_t* blk_flag;              // per row-kernel block: holds such rows
+  unsigned long long* brick_over; // count of such bricks
+};
+
+// LDS of one Verlet-build brick block: the header, D position arrays of cap
+// doubles, int32 slots, the halo offsets and first rows, the per-lane
+// slot-sorted queues (VBRICK_Q uint16 LDS indices per lane), then the uint16
+// owner and home-row arrays.
+constexpr int VBRICK_Q = 16;
+
+__host__ __device__ inline int64_t vbrick_lds_bytes(int D, int cap, int H) {
+  int64_t b = 4 * BRICK_HDR;
+  b += (int64_t)8 * cap * D + (int64_t)4 * cap;
+  b += (int64_t)4 * (2 * H + 1);
+  b += (int64_t)2 * VBRICK_Q * BRICK_MAX_BLOCK;
+  b += (int64_t)4 * cap;
+  return (b + 15) & ~(int64_t)15;
+}
+
 struct StageArgs {
   RowGeom g;
   const double* S[MAX_STATE];   // stage state read by the force (neighbors + self)

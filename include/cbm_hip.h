@@ -222,12 +222,12 @@ int cbm_get_t(cbm_ctx* ctx, double* t, int64_t* step);      /* com.t      */
 int cbm_get_id_max(cbm_ctx* ctx, int32_t sim, int64_t* id_max); /* idMax_ */
 /* Number of Verlet rebuilds (or CL binnings) performed so far. */
 int cbm_get_rebuilds(cbm_ctx* ctx, int64_t* rebuilds);
-/* CellLinked launch geometry (no reference counterpart; diagnostics of the
- * LDS-tiled stage): the brick edge in cells per axis (bdim[3]), the agents
- * one brick block stages in LDS (cap), the number of bricks, and how many
- * brick launches so far found more than cap agents in their halo and walked
- * global memory instead (same results, slower).  Zeros for other
- * neighbor algorithms. */
+/* Cell-brick launch geometry (no reference counterpart; diagnostics of the
+ * LDS-tiled CellLinked stage and Verlet list build): the brick edge in cells
+ * per axis (bdim[3]), the agents one brick block stages in LDS (cap), the
+ * number of bricks, and how many brick launches so far found more than cap
+ * agents in their halo and left its rows to the row-parallel kernel (same
+ * results, slower).  Zeros for Full. */
 int cbm_get_bricks(cbm_ctx* ctx, int32_t* bdim, int32_t* cap, int64_t* nbricks,
                    int64_t* overflows);
 /* Neighbor lists of simulation `sim` in logical slots: counts[n] and

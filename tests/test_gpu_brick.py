@@ -145,3 +145,38 @@ def test_brick_block_sizes_agree(gpu_available):
         for a in range(3):
             assert np.array_equal(got[a], out[0][a])
             assert rel_err(got[a], ref[a]) == 0.0
+
+
+@pytest.mark.parametrize("cluster", [60, 1500])
+def test_verlet_brick_build_matches_oracle(gpu_available, cluster):
+    # Verlet lists built on brick tiles: 60 agents within 1.0 of a point give
+    # lists longer than a lane's sorted queue; 1,500 in a 4.8-wide cube give a
+    # halo beyond the tile capacity, left to the row-parallel build
+    rng = np.random.default_rng(100 + cluster)
+    n, L = 2000, 40.0
+    box = [[0.0, L]] * 3
+    pos = [rng.uniform(0.0, L, n) for _ in range(3)]
+    if cluster == 60:
+        u = rng.normal(size=(3, 60))
+        u *= rng.uniform(0.0, 1.0, 60) ** (1.0 / 3.0) / np.linalg.norm(u, axis=0)
+        for a in range(3):
+            pos[a][:60] = 20.0 + u[a]
+    else:
+        for a in range(3):
+            pos[a][:cluster] = rng.uniform(17.6, 22.4, cluster)
+    m = model(3, integrator=A.EULER, neighbors=A.VERLET_DISPLACEMENT, law=A.LAW_REP_ADH, dt=1e-4, periodic=1,
+              skin=2.7, max_neighbors=2048, rebuild_disp=0.05)
+    o, g = pair(m, sim(n, box, REP_ADH), pos)
+    b = g.bricks()
+    assert b["nbricks"] > 0
+    if cluster > 60:
+        assert b["cap"] < cluster and b["overflows"] > 0
+    else:
+        assert b["overflows"] == 0
+    o.step(1)
+    g.step(1)
+    assert err(o, g, 3) == 0.0
+    o.step(29)
+    g.step(29)
+    assert err(o, g, 3) <= LONG_TOL
+    assert g.rebuilds() == o.rebuilds
