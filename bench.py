@@ -476,6 +476,8 @@ def main():
     achieved = sb[dom] * s.n / (ms[dom] * 1e-3) / 1e9
     step_bytes = sum(sb.values()) * s.n
     dom_name = {2: "k_stage Heun stage 1", 3: "k_stage Heun stage 2", 1: "k_stage EM"}[dom]
+    if args.workload == "heun_cl":
+        dom_name = dom_name.replace("k_stage", "k_brick")
     rec = {
         "metric": "agent-steps/s @1M 3D",
         "value": value,
@@ -500,7 +502,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom_name, s.n) if args.workload == "heun" else None,
+            "traffic": measured_traffic(dom_name, s.n) if args.workload in ("heun", "heun_cl") else None,
             "alg_bytes_per_launch": sb[dom] * s.n,
             "avg_launch_ms": ms[dom],
             "mean_list_len": kbar,
