@@ -93,6 +93,8 @@ struct cbm_ctx {
   int32_t* list = nullptr;
   int32_t* list_n = nullptr;
   int cap = 0;
+  bool wl = false;                        // cell-linked two-stage: stage 2 walks lists written by stage 1
+  int32_t wcap = 0;
   int32_t* bricks = nullptr;              // cell-linked brick blocks (BRICK_WORDS each)
   uint8_t* brick_skip = nullptr;          // per row: left to the row-parallel stage kernel
   uint8_t* blk_flag = nullptr;            // per row-kernel block: holds such rows
@@ -591,6 +593,15 @@ static int init_ctx(cbm_ctx* c) {
   }
   HIPCHK(c, dalloc(&c->flags, 8));
   HIPCHK(c, dalloc(&c->counters, 4));
+  // cell-linked Heun / EulerHeun with a built-in law: stage 1 writes the
+  // candidate lists that stage 2 walks (flags[4] says whether they are exact)
+  c->wl = c->cl && (m.integrator == CBM_HEUN || m.integrator == CBM_EULER_HEUN) && !c->inertial &&
+          (m.law == CBM_LAW_LINEAR || m.law == CBM_LAW_REP_ADH || m.law == CBM_LAW_AGGREGATION);
+  if (c->wl) {
+    c->wcap = 32;
+    HIPCHK(c, dalloc(&c->list, (int64_t)c->wcap * c->C));
+    HIPCHK(c, dalloc(&c->list_n, c->C));
+  }
   if (c->verlet) {
     c->cap = m.max_neighbors;
     for (int a = 0; a < c->dims; ++a) HIPCHK(c, dalloc(&c->xref[a], c->C));
@@ -627,6 +638,7 @@ static RebuildArgs rebuild_args(cbm_ctx* c, const int32_t* flag, int32_t* flag_c
   a.flag_clear = flag_clear;
   a.rebuilds = c->counters;
   a.step_inc = tick ? c->counters + 1 : nullptr;
+  a.gate_set = c->wl ? c->flags + 4 : nullptr;
   for (int d = 0; d < c->dims; ++d) a.pos[d] = (const double*)c->col[C_X + d].cur();
   a.dead = c->dead;
   a.slot = (const int32_t*)c->col[C_SLOT].cur();
@@ -772,6 +784,16 @@ static void prof_end(cbm_ctx* c);
 // One integrator-stage launch: the built-in law kernels, or the hipRTC
 // module of a user law.
 static hipError_t stage_launch(cbm_ctx* c, const StageArgs& a) {
+  if (c->m.law != CBM_LAW_CUSTOM && a.gate) {
+    // stage 2 of a listed cell-linked step: the list walk when stage 1's
+    // lists are exact, else the brick tiles (each launch checks the gate)
+    StageArgs b = a;
+    b.gate_run = 1;
+    hipError_t e = launch_stage(c->dims, NB_LIST, c->m.law, c->nblocks, c->block, c->stream, b, c->nbricks, c->brick_lds);
+    if (e != hipSuccess) return e;
+    b.gate_run = 0;
+    return launch_stage(c->dims, NB_CELL, c->m.law, c->nblocks, c->block, c->stream, b, c->nbricks, c->brick_lds);
+  }
   if (c->m.law != CBM_LAW_CUSTOM)
     return launch_stage(c->dims, c->nb_kind, c->m.law, c->nblocks, c->block, c->stream, a, c->nbricks, c->brick_lds);
   const int noise = (a.kind == ST_EM || a.kind == ST_EH1 || a.kind == ST_EH2) ? 1 : 0;
@@ -786,6 +808,18 @@ static hipError_t stage_launch(cbm_ctx* c, const StageArgs& a) {
     if (e != hipSuccess) return e;
   }
   return hipModuleLaunchKernel(f, (unsigned)c->nblocks, 1, 1, (unsigned)c->block, 1, 1, 0, c->stream, params, nullptr);
+}
+
+// Cell-linked two-stage steps (c->wl): stage 1 writes the candidate lists
+// and may clear flags[4]; stage 2 reads it to pick the list walk.
+static void listed_stage(cbm_ctx* c, StageArgs& a, int stage) {
+  a.wlist = nullptr; a.wlist_n = nullptr; a.lgate = nullptr; a.gate = nullptr; a.wcap = 0;
+  if (!c->wl) return;
+  if (stage == 1) {
+    a.wlist = c->list; a.wlist_n = c->list_n; a.wcap = c->wcap; a.lgate = c->flags + 4;
+  } else {
+    a.gate = c->flags + 4;
+  }
 }
 
 static int run_stage(cbm_ctx* c, StageArgs& a, int kind) {
@@ -916,17 +950,21 @@ static int step_once(cbm_ctx* c) {
     case CBM_HEUN:
       set_state(cur, cur, c->h[0]);
       a.disp_check = 0;
+      listed_stage(c, a, 1);
       if ((rc = run_stage(c, a, ST_HEUN1))) return rc;
       set_state(c->h[0], cur, cur);
       a.disp_check = disp ? 1 : 0;
+      listed_stage(c, a, 2);
       if ((rc = run_stage(c, a, ST_HEUN2))) return rc;
       break;
     case CBM_EULER_HEUN:
       set_state(cur, cur, c->h[0]);
       a.disp_check = 0;
+      listed_stage(c, a, 1);
       if ((rc = run_stage(c, a, ST_EH1))) return rc;
       set_state(c->h[0], cur, cur);
       a.disp_check = disp ? 1 : 0;
+      listed_stage(c, a, 2);
       if ((rc = run_stage(c, a, ST_EH2))) return rc;
       break;
     default:  // RK4

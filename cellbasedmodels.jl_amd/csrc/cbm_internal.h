@@ -26,6 +26,7 @@ struct RebuildArgs {
   int32_t* flag_clear;        // flag of the next step, cleared by k_keys
   int64_t* rebuilds;          // rebuild counter
   int64_t* step_inc;          // device step counter advanced by k_keys (may be NULL)
+  int32_t* gate_set;          // set to 1 by k_keys: the next stage-2 lists start valid (may be NULL)
   const double* pos[3];       // current positions
   const uint8_t* dead;        // per-row removal mark (may be NULL)
   const int32_t* slot;        // per-row logical slot (current)

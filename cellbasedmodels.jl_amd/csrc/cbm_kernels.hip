@@ -66,6 +66,7 @@ __global__ __launch_bounds__(256) void k_keys(RebuildArgs a) {
     // clear the flag the next step's final stage will set
     if (a.step_inc) *a.step_inc += 1;
     if (a.flag_clear) *a.flag_clear = 0;
+    if (a.gate_set) *a.gate_set = 1;
   }
   if (a.flag && *a.flag == 0) return;
   const RowIdx ri = row_index(a.g);

@@ -194,7 +194,8 @@ __device__ __forceinline__ void gather_global(const StageArgs& a, const SimDev& 
 // wrap, the output write and the displacement check.
 template <int D, class Law, int NOISE, bool PRE = false>
 __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& sd, int sim, int64_t i,
-                                             const double* xi, double* acc, const double* upre) {
+                                             const double* xi, double* acc, const double* upre,
+                                             double disp2 = -1.0) {
   constexpr bool SOFT = same_type<Law, PairLaw<LAW_SOFT_SPHERES>>::value;
   // u of variable c (with PRE, the caller loaded the positions' u early into
   // upre), and the stage state, whose positions are xi
@@ -303,6 +304,13 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
     }
   }
   for (int c = 0; c < nv; ++c) a.out[c][i] = res[c];
+  if (disp2 >= 0.0) {
+    // the stage-2 lists stay exact while no agent moves more than half their margin
+    double s2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) { const double d = res[c] - Uc(c); s2 = s2 + d * d; }
+    if (!(s2 <= disp2)) *a.lgate = 0;
+  }
 
   if (final_stage && a.disp_check) {
     double dx[3];
@@ -324,6 +332,7 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
 // compiled around it with hipRTC.
 template <int D, int NB, class Law, int PER, int NOISE>
 __device__ __forceinline__ void stage_body(const StageArgs& a) {
+  if (a.gate && *a.gate != a.gate_run) return;
   if (NB == NB_CELL) {
     // cell-linked: only blocks holding rows of bricks too dense for LDS
     const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
@@ -630,6 +639,10 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
   // those only.  Every pair left out is inactive, so the sum is unchanged.
   constexpr bool FILTER = SOFT || has_reach<Law>::value;
   constexpr int U4 = 4;               // loads issued together per thread while staging
+  // stage-2 lists (built-in laws without per-agent radii)
+  constexpr bool WL = FILTER && !SOFT;
+  if (a.gate && *a.gate != a.gate_run) return;
+  const bool wl = WL && a.wlist != nullptr;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int cap = a.brick_cap;
   BrickSrc src;
@@ -669,7 +682,9 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
   cols.xi_lds = fate;
   double rmax = 0.0;
   const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-  if (brick_stage<D, PER, U4, SOFT, SOFT ? BX_I64 : BX_NONE>(src, b, t, cols, rmax) != 1) return;
+  const int staged = brick_stage<D, PER, U4, SOFT, SOFT ? BX_I64 : BX_NONE>(src, b, t, cols, rmax);
+  if (staged == 2 && wl && tid == 0) *a.lgate = 0;   // its rows get no list
+  if (staged != 1) return;
   const int sim = t.sim;
   const SimDev& sd = a.g.sims[sim];
   const int64_t N = a.g.st[sim].N;
@@ -684,7 +699,17 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
   double L[3], hL[3];
   box_halves<D, PER>(sd, L, hL);
   double reach2 = __builtin_inf();
-  if constexpr (FILTER && !SOFT) reach2 = reach2_bound(law.reach());
+  double disp2 = -1.0;
+  if constexpr (FILTER && !SOFT) {
+    reach2 = reach2_bound(law.reach());
+    if (wl) {
+      // lists to CL_LIST_GROW x reach: valid while displacements stay under
+      // half the margin (with a little slack for rounding)
+      reach2 = reach2_bound(law.reach() * CL_LIST_GROW);
+      const double half = 0.5 * (CL_LIST_GROW - 1.0) * law.reach() * (1.0 - 1e-9);
+      disp2 = half > 0.0 ? half * half : 0.0;
+    }
+  }
   for (int32_t k = tid; k < nh; k += nt) {
     const int32_t li = home_li[k];
     const int h = owner[li] & 0x7FFF;
@@ -711,9 +736,7 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
                                      SOFT ? (int64_t)fate[q] : 0, live, acc);
       };
       if constexpr (FILTER) {
-        // pass 1: LDS indices of the candidates within reach, in walk order
-        int nq = 0;
-        walk([&](int32_t q) {
+        auto near = [&](int32_t q) {
           double dx[3];
 #pragma unroll
           for (int c = 0; c < D; ++c) {
@@ -723,23 +746,48 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
           double s2 = dx[0] * dx[0];
           if (D > 1) s2 = s2 + dx[1] * dx[1];
           if (D > 2) s2 = s2 + dx[2] * dx[2];
-          if (s2 <= keep2 && q != li) {
+          return s2 <= keep2 && q != li;
+        };
+        auto row_of = [&](int32_t q) {
+          const int hq = owner[q] & 0x7FFF;
+          return (int32_t)(hrow[hq] + (q - hs[hq]));
+        };
+        // pass 1: LDS indices of the candidates within reach, in walk order
+        int nq = 0;
+        walk([&](int32_t q) {
+          if (near(q)) {
             if (nq < BRICK_LIST) lst[nq * nt + tid] = (uint16_t)q;
             ++nq;
           }
         });
-        // pass 2: the pair body on those; a lane whose queue overflowed
-        // walks every candidate again instead
+        // pass 2: the pair body on those (and their rows as the stage-2
+        // list); a lane whose queue overflowed walks every candidate again
         if (nq <= BRICK_LIST) {
-          for (int u = 0; u < nq; ++u) body(lst[u * nt + tid], true);
+          for (int u = 0; u < nq; ++u) {
+            const int32_t q = lst[u * nt + tid];
+            body(q, true);
+            if (wl) a.wlist[(int64_t)u * a.ldl + i] = row_of(q);
+          }
+          if (wl) a.wlist_n[i] = nq;
         } else {
-          walk([&](int32_t q) { body(q, q != li); });
+          int32_t cnt = 0;
+          walk([&](int32_t q) {
+            body(q, q != li);
+            if (wl && near(q)) {
+              if (cnt < a.wcap) a.wlist[(int64_t)cnt * a.ldl + i] = row_of(q);
+              ++cnt;
+            }
+          });
+          if (wl) {
+            a.wlist_n[i] = cnt < a.wcap ? cnt : a.wcap;
+            if (cnt > a.wcap) *a.lgate = 0;
+          }
         }
       } else {
         walk([&](int32_t q) { body(q, q != li); });
       }
     }
-    stage_finish<D, Law, NOISE, true>(a, sd, sim, i, xi, acc, upre);
+    stage_finish<D, Law, NOISE, true>(a, sd, sim, i, xi, acc, upre, disp2);
   }
 }
 

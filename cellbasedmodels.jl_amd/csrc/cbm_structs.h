@@ -84,7 +84,8 @@ struct alignas(8) CellRange { int32_t s, e; };
 constexpr int BRICK_HDR = 16;        // int32 words: total, per-wave sums; int64 at byte 48
 constexpr int BRICK_LIST = 12;       // queue entries per lane
 constexpr int BRICK_MAX_BLOCK = 256;
-constexpr int BRICK_WORDS = 6;       // brick table entry: sim, origin x, y, z, first bucket, end bucket
+constexpr int BRICK_WORDS = 6;
+constexpr double CL_LIST_GROW = 1.05;  // list radius of the stage-2 lists, in units of the law's reach       // brick table entry: sim, origin x, y, z, first bucket, end bucket
 
 __host__ __device__ constexpr int brick_halo_cells(int D, int bx, int by, int bz) {
   return (bx + 2) * (D > 1 ? by + 2 : 1) * (D > 2 ? bz + 2 : 1);
@@ -154,6 +155,16 @@ struct StageArgs {
   unsigned long long* brick_over;   // count of brick blocks left to the row-parallel kernel
   uint8_t* brick_skip;          // per row: 1 = left to the row-parallel kernel (cell-linked; it clears it)
   uint8_t* blk_flag;            // per row-kernel block: holds rows left to it (cleared by it)
+  // Cell-linked two-stage integrators: stage 1 writes, per agent, its
+  // candidates within CL_LIST_GROW x the law's reach (walk order) so that
+  // stage 2 can walk them as a list; lgate is cleared when they may miss a
+  // pair (a stage-1 displacement over half the margin, a list over wcap, a
+  // brick too dense for LDS).  A launch with gate != NULL runs only when
+  // *gate == gate_run.
+  int32_t* wlist;
+  int32_t* wlist_n;
+  int32_t* lgate;
+  const int32_t* gate;
   int32_t* flag;                // rebuild flag written by the final stage
   double dt, sqdt, drift, noise;
   double thr2;                  // largest d2 with sqrt(d2) < rebuild displacement
@@ -162,6 +173,8 @@ struct StageArgs {
   int32_t kind, periodic, lagged, walls, nvar, disp_check, write_fnew, pad;
   int32_t bdim[3];              // brick edge in cells per axis
   int32_t brick_cap;            // agents a brick block can stage in LDS
+  int32_t wcap;                 // entries per agent of wlist
+  int32_t gate_run;
 };
 
 }  // namespace cbm

@@ -180,3 +180,23 @@ def test_verlet_brick_build_matches_oracle(gpu_available, cluster):
     g.step(29)
     assert err(o, g, 3) <= LONG_TOL
     assert g.rebuilds() == o.rebuilds
+
+
+@pytest.mark.parametrize("integ,dt", [(A.HEUN, 0.005), (A.HEUN, 0.4), (A.EULER_HEUN, 0.005)])
+def test_cell_linked_two_stage_lists(gpu_available, integ, dt):
+    # stage 1 writes each agent's candidates within 1.05x the law's reach and
+    # stage 2 walks them as a list while no stage-1 displacement exceeds half
+    # the margin; dt = 0.4 moves overlapping agents farther than that, so
+    # stage 2 falls back to the brick tiles
+    n, L = 3000, 30.0
+    box = [[0.0, L]] * 3
+    m = model(3, integrator=integ, neighbors=A.CELL_LINKED, law=A.LAW_REP_ADH, dt=dt, periodic=1, cell_edge=2.4,
+              noise=0.05 if integ == A.EULER_HEUN else 0.0, seed=11)
+    o, g = pair(m, sim(n, box, REP_ADH), uniform_positions(n, box, 31))
+    o.step(1)
+    g.step(1)
+    # the noise goes through the device's log/sin/cos: 1e-12 per step, not bits
+    assert err(o, g, 3) <= (1e-12 if integ == A.EULER_HEUN else 0.0)
+    o.step(19)
+    g.step(19)
+    assert err(o, g, 3) <= LONG_TOL
