@@ -84,8 +84,8 @@ struct alignas(8) CellRange { int32_t s, e; };
 constexpr int BRICK_HDR = 16;        // int32 words: total, per-wave sums; int64 at byte 48
 constexpr int BRICK_LIST = 12;       // queue entries per lane
 constexpr int BRICK_MAX_BLOCK = 256;
-constexpr int BRICK_WORDS = 6;
-constexpr double CL_LIST_GROW = 1.05;  // list radius of the stage-2 lists, in units of the law's reach       // brick table entry: sim, origin x, y, z, first bucket, end bucket
+constexpr int BRICK_WORDS = 6;       // brick table entry: sim, origin x, y, z, first bucket, end bucket
+constexpr double CL_LIST_GROW = 1.05;  // radius of the stage-2 lists, in units of the law's reach
 
 __host__ __device__ constexpr int brick_halo_cells(int D, int bx, int by, int bz) {
   return (bx + 2) * (D > 1 ? by + 2 : 1) * (D > 2 ? bz + 2 : 1);
