@@ -188,7 +188,10 @@ class ABM:
     """Model description: ABM(dims; agentODE, agentSDE, agentRule, neighborsAlg, agentAlg, platform).
 
     agentODE    a CBMModels law (the interaction body); `drift` adds dt(x) = -drift*x
-    agentSDE    diagonal noise amplitude g (EM and EulerHeun): dt(x) = g in the SDE part
+    agentSDE    diagonal noise amplitude g (EM and EulerHeun): dt(x) = g in the SDE part;
+                a number (g on the positions) or a HIP C++ body defining
+                struct UserNoise (g of each DE variable at the agent's state),
+                compiled with hipRTC by loadToPlatform_
     lagged      dt(x) reads the previous step's committed force accumulator, the
                 reference's read-old/write-new idiom (agentStructure.jl:352-453)
     periodic    periodic box with minimum-image distances
@@ -201,7 +204,7 @@ class ABM:
             raise ValueError("dims must be 1, 2 or 3")
         self.dims = dims
         self.agentODE = agentODE if agentODE is not None else CBMModels.noInteraction()
-        self.agentSDE = float(agentSDE)
+        self.agentSDE = agentSDE if isinstance(agentSDE, str) else float(agentSDE)
         self.agentRule = agentRule
         self.drift = float(drift)
         self.neighbors = neighborsAlg if neighborsAlg is not None else CBMNeighbors.Full()
@@ -225,7 +228,7 @@ class ABM:
         m.lagged = 1 if self.lagged else 0
         m.dt = float(dt)
         m.drift = self.drift
-        m.noise = self.agentSDE
+        m.noise = 0.0 if isinstance(self.agentSDE, str) else self.agentSDE
         m.seed = self.seed
         if self.agentRule is not None:
             m.rule = self.agentRule[0]
@@ -368,6 +371,11 @@ class Context:
         """Compile a user interaction body (struct UserLaw, HIP C++) with hipRTC."""
         self._chk(self.lib.cbm_compile(self.h, source.encode()))
 
+    def compile_model(self, law=None, noise=None):
+        """Compile user agentODE (struct UserLaw) and/or agentSDE (struct UserNoise) bodies."""
+        enc = lambda s: s.encode() if s is not None else None
+        self._chk(self.lib.cbm_compile_model(self.h, enc(law), enc(noise), None))
+
     def set_params(self, p, sim=0):
         v = np.ascontiguousarray(p, dtype=np.float64)
         self._chk(self.lib.cbm_set_params(self.h, sim, v.ctypes.data, v.size))
@@ -467,8 +475,10 @@ def loadToPlatform_(com, preallocateAgents=0):
     model = abm.model_desc(com.dt)
     sim = sim_desc(abm, com.N, com.N + int(preallocateAgents), com.simBox.reshape(-1).tolist(), com.law_p)
     com.ctx = Context(model, [sim])
-    if abm.agentODE.law == A.LAW_CUSTOM:
-        com.ctx.compile(abm.agentODE.source)
+    law_src = abm.agentODE.source if abm.agentODE.law == A.LAW_CUSTOM else None
+    noise_src = abm.agentSDE if isinstance(abm.agentSDE, str) else None
+    if law_src is not None or noise_src is not None:
+        com.ctx.compile_model(law_src, noise_src)
     for k, arr in com.params.items():
         com.ctx.upload(_FIELDS[k], arr)
     com.ctx.neighbors()

@@ -142,6 +142,9 @@ struct cbm_ctx {
   hipModule_t user_mod = nullptr;
   hipFunction_t user_fn[2] = {nullptr, nullptr};   // [0] deterministic stages, [1] noisy (EM, EulerHeun)
   hipFunction_t user_brick[2] = {nullptr, nullptr};   // the same, cell-linked brick tiles
+  int user_nb = NB_FULL;                  // neighbor walk the module was built for
+  bool user_noise = false;                // the module holds a user agentSDE body
+  double* gacc[MAX_STATE] = {};           // EulerHeun with a user agentSDE body: g of stage 1
 
   std::string err;
 };
@@ -432,7 +435,7 @@ static void destroy_ctx(cbm_ctx* c) {
   auto F = [](void* p) { if (p) (void)hipFree(p); };
   for (int k = 0; k < C_NCOLS; ++k) { F(c->col[k].buf[0]); F(c->col[k].buf[1]); }
   for (int a = 0; a < 3; ++a) { F(c->Fnew[a]); F(c->xref[a]); }
-  for (int v = 0; v < MAX_STATE; ++v) { F(c->h[0][v]); F(c->h[1][v]); F(c->kacc[v]); }
+  for (int v = 0; v < MAX_STATE; ++v) { F(c->h[0][v]); F(c->h[1][v]); F(c->kacc[v]); F(c->gacc[v]); }
   F(c->gr_new); F(c->vb_new); F(c->dead);
   F(c->n_add_slot); F(c->dead_slot); F(c->off_add); F(c->off_dead); F(c->holes);
   F(c->med_c[0]); F(c->med_c[1]); F(c->med_count);
@@ -762,6 +765,7 @@ static StageArgs stage_base(cbm_ctx* c) {
   a.nvar = c->nvar;
   a.write_fnew = c->lagged ? 1 : 0;
   if (c->growth) a.growth = (const double*)c->col[C_GR].cur();
+  for (int v = 0; v < MAX_STATE; ++v) a.gacc[v] = c->gacc[v];
   if (c->inertial) {
     a.rad = (const double*)c->col[C_RAD].cur();
     a.mass = (const double*)c->col[C_MASS].cur();
@@ -784,7 +788,9 @@ static void prof_end(cbm_ctx* c);
 // One integrator-stage launch: the built-in law kernels, or the hipRTC
 // module of a user law.
 static hipError_t stage_launch(cbm_ctx* c, const StageArgs& a) {
-  if (c->m.law != CBM_LAW_CUSTOM && a.gate) {
+  const int noise = (a.kind == ST_EM || a.kind == ST_EH1 || a.kind == ST_EH2) ? 1 : 0;
+  const bool module = c->m.law == CBM_LAW_CUSTOM || (c->user_noise && noise);
+  if (!module && a.gate) {
     // stage 2 of a listed cell-linked step: the list walk when stage 1's
     // lists are exact, else the brick tiles (each launch checks the gate)
     StageArgs b = a;
@@ -794,14 +800,13 @@ static hipError_t stage_launch(cbm_ctx* c, const StageArgs& a) {
     b.gate_run = 0;
     return launch_stage(c->dims, NB_CELL, c->m.law, c->nblocks, c->block, c->stream, b, c->nbricks, c->brick_lds);
   }
-  if (c->m.law != CBM_LAW_CUSTOM)
+  if (!module)
     return launch_stage(c->dims, c->nb_kind, c->m.law, c->nblocks, c->block, c->stream, a, c->nbricks, c->brick_lds);
-  const int noise = (a.kind == ST_EM || a.kind == ST_EH1 || a.kind == ST_EH2) ? 1 : 0;
   hipFunction_t f = c->user_fn[noise];
   if (!f) return hipErrorInvalidDeviceFunction;
   StageArgs copy = a;
   void* params[] = {(void*)&copy};
-  if (c->nb_kind == NB_CELL) {
+  if (c->user_nb == NB_CELL) {
     if (!c->user_brick[noise]) return hipErrorInvalidDeviceFunction;
     const hipError_t e = hipModuleLaunchKernel(c->user_brick[noise], (unsigned)c->nbricks, 1, 1, (unsigned)c->block,
                                                1, 1, (unsigned)c->brick_lds, c->stream, params, nullptr);
@@ -1057,32 +1062,47 @@ const char* cbm_last_error(const cbm_ctx* c) { return c ? c->err.c_str() : "null
 // struct UserLaw, see include/cbm_hip.h) is compiled by hipRTC into the
 // same stage skeleton as the built-in laws, instantiated for this model's
 // dimension, neighbor walk and boundary.
-int cbm_compile(cbm_ctx* c, const char* body) {
-  if (!c || !body) return CBM_ERR_ARG;
-  if (c->m.law != CBM_LAW_CUSTOM) return fail(c, CBM_ERR_CONFIG, "cbm_compile needs a model with law CBM_LAW_CUSTOM");
+int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, const char* rule_body) {
+  if (!c) return CBM_ERR_ARG;
+  if (rule_body) return fail(c, CBM_ERR_CONFIG, "user agentRule bodies are not supported; use a built-in rule");
+  const bool custom = c->m.law == CBM_LAW_CUSTOM;
+  if (custom != (law_body != nullptr))
+    return fail(c, CBM_ERR_CONFIG, "a law body is required with law CBM_LAW_CUSTOM and only then");
+  if (noise_body && c->m.integrator != CBM_EM && c->m.integrator != CBM_EULER_HEUN)
+    return fail(c, CBM_ERR_CONFIG, "an agentSDE body needs the EM or EulerHeun integrator");
+  if (!law_body && !noise_body) return fail(c, CBM_ERR_ARG, "nothing to compile");
   std::string src = "#define CBM_HIPRTC 1\n#include \"cbm_stage.h\"\nnamespace cbm {\n";
-  src += body;
-  src += "\n}  // namespace cbm\n";
+  if (law_body) { src += law_body; src += "\n"; }
+  if (noise_body) { src += noise_body; src += "\n"; }
+  src += "}  // namespace cbm\n";
+  char line[256];
+  if (law_body) src += "using cbm_law_t = cbm::UserLaw;\n";
+  else { std::snprintf(line, sizeof(line), "using cbm_law_t = cbm::PairLaw<%d>;\n", c->m.law); src += line; }
+  src += noise_body ? "using cbm_noise_t = cbm::UserNoise;\n" : "using cbm_noise_t = cbm::ConstNoise;\n";
+  // the same skeletons as the built-in laws: no neighbor walk without a law
+  const bool walk = c->m.law != CBM_LAW_NONE;
+  const int nb = walk ? c->nb_kind : NB_FULL;
+  const int per = walk && c->m.periodic ? 1 : 0;
   char inst[512];
   for (int noise = 0; noise < 2; ++noise) {
     std::snprintf(inst, sizeof(inst),
                   "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_stage_%d(cbm::StageArgs a) {\n"
-                  "  cbm::stage_body<%d, %d, cbm::UserLaw, %d, %d>(a);\n}\n",
-                  noise, c->dims, c->nb_kind, c->m.periodic ? 1 : 0, noise);
+                  "  cbm::stage_body<%d, %d, cbm_law_t, %d, %d, cbm_noise_t>(a);\n}\n",
+                  noise, c->dims, nb, per, noise);
     src += inst;
-    if (c->nb_kind == NB_CELL) {   // the brick-tiled cell-linked stage, run before the row kernel
+    if (nb == NB_CELL) {   // the brick-tiled cell-linked stage, run before the row kernel
       std::snprintf(inst, sizeof(inst),
                     "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_brick_%d(cbm::StageArgs a) {\n"
                     "  extern __shared__ unsigned char lds[];\n"
-                    "  cbm::brick_body<%d, cbm::UserLaw, %d, %d>(a, lds);\n}\n",
-                    noise, c->dims, c->m.periodic ? 1 : 0, noise);
+                    "  cbm::brick_body<%d, cbm_law_t, %d, %d, cbm_noise_t>(a, lds);\n}\n",
+                    noise, c->dims, per, noise);
       src += inst;
     }
   }
   const char* headers[] = {cbm_rtc_cbm_device_h, cbm_rtc_cbm_structs_h, cbm_rtc_cbm_stage_h};
   const char* names[] = {"cbm_device.h", "cbm_structs.h", "cbm_stage.h"};
   hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, src.c_str(), "cbm_user_law.hip", 3, headers, names) != HIPRTC_SUCCESS)
+  if (hiprtcCreateProgram(&prog, src.c_str(), "cbm_user_bodies.hip", 3, headers, names) != HIPRTC_SUCCESS)
     return fail(c, CBM_ERR_COMPILE, "hiprtcCreateProgram failed");
   const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math"};
   hiprtcResult r = hiprtcCompileProgram(prog, 5, opts);
@@ -1100,14 +1120,28 @@ int cbm_compile(cbm_ctx* c, const char* body) {
   hiprtcGetCode(prog, code.data());
   hiprtcDestroyProgram(&prog);
   if (c->user_mod) { (void)hipModuleUnload(c->user_mod); c->user_mod = nullptr; }
+  for (int k = 0; k < 2; ++k) { c->user_fn[k] = nullptr; c->user_brick[k] = nullptr; }
   HIPCHK(c, hipModuleLoadData(&c->user_mod, code.data()));
   HIPCHK(c, hipModuleGetFunction(&c->user_fn[0], c->user_mod, "cbm_user_stage_0"));
   HIPCHK(c, hipModuleGetFunction(&c->user_fn[1], c->user_mod, "cbm_user_stage_1"));
-  if (c->nb_kind == NB_CELL) {
+  if (nb == NB_CELL) {
     HIPCHK(c, hipModuleGetFunction(&c->user_brick[0], c->user_mod, "cbm_user_brick_0"));
     HIPCHK(c, hipModuleGetFunction(&c->user_brick[1], c->user_mod, "cbm_user_brick_1"));
   }
+  c->user_nb = nb;
+  c->user_noise = noise_body != nullptr;
+  if (c->user_noise) {
+    c->wl = false;   // the noisy stages go through the module, whose stage 2 walks the cells
+    if (c->m.integrator == CBM_EULER_HEUN)
+      for (int v = 0; v < c->nvar; ++v)
+        if (!c->gacc[v]) HIPCHK(c, dalloc(&c->gacc[v], c->C));
+  }
   return CBM_OK;
+}
+
+int cbm_compile(cbm_ctx* c, const char* body) {
+  if (!c || !body) return CBM_ERR_ARG;
+  return cbm_compile_model(c, body, nullptr, nullptr);
 }
 
 static int field_col(cbm_ctx* c, int32_t field, int& k) {

@@ -192,7 +192,21 @@ __device__ __forceinline__ void gather_global(const StageArgs& a, const SimDev& 
 // The rest of the stage for row i once the pair sum acc is known: walls,
 // the derivative of every DE variable, the integrator update, periodic
 // wrap, the output write and the displacement check.
-template <int D, class Law, int NOISE, bool PRE = false>
+// Noise amplitude of the agentSDE part.  ConstNoise is the model's constant g
+// on the positions (StageArgs::noise).  A user body (hipRTC) defines
+//   struct UserNoise {
+//     __device__ explicit UserNoise(const double* p);          // the simulation's p[8]
+//     __device__ double operator()(int var, const double* u) const;
+//   };
+// giving g of DE variable var at the agent's state u (positions, then the
+// other DE variables), evaluated at the stage state as the reference's g
+// (src/integrators.jl:709-727 EM, :775-803 EulerHeun).
+struct ConstNoise {
+  __device__ explicit ConstNoise(const double*) {}
+  __device__ double operator()(int, const double*) const { return 0.0; }
+};
+
+template <int D, class Law, int NOISE, bool PRE = false, class Noise = ConstNoise>
 __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& sd, int sim, int64_t i,
                                              const double* xi, double* acc, const double* upre,
                                              double disp2 = -1.0) {
@@ -243,7 +257,36 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
   const double dt = a.dt;
   const int nv = SOFT ? 2 * D : (grow ? D + 1 : D);
   double res[MAX_STATE];
-  if (NOISE) {
+  if (NOISE && !same_type<Noise, ConstNoise>::value) {
+    // a user agentSDE body: g of every DE variable at the stage state, one
+    // normal per variable (pairs keyed pur, pur + 1, pur + 2); the
+    // EulerHeun corrector averages g of both stages, g of stage 1 kept in gacc
+    const int64_t id = a.id[i];
+    const int64_t step = *a.step_ptr;
+    const uint32_t pur = a.kind == ST_EH2 ? 0x500u : 0x100u;
+    double z[MAX_STATE], us[MAX_STATE];
+#pragma unroll
+    for (int k = 0; k < MAX_STATE / 2; ++k) {
+      z[2 * k] = 0.0;
+      z[2 * k + 1] = 0.0;
+      if (2 * k < nv) normal2(draw(a.seed, step, id, sim, pur + (uint32_t)k), z[2 * k], z[2 * k + 1]);
+    }
+#pragma unroll
+    for (int v = 0; v < MAX_STATE; ++v) us[v] = v < nv ? Sc(v) : 0.0;
+    const Noise gfn(sd.p);
+#pragma unroll
+    for (int v = 0; v < MAX_STATE; ++v) {
+      if (v >= nv) continue;
+      const double g = gfn(v, us);
+      const double u = v < D ? Uc(v) : a.U[v][i];
+      if (a.kind == ST_EH2) {
+        res[v] = u + ((a.kacc[v][i] + du[v]) * dt + (a.gacc[v][i] + g) * z[v] * a.sqdt) / 2;
+      } else {
+        res[v] = (u + du[v] * dt) + g * z[v] * a.sqdt;
+        if (a.kind == ST_EH1) { a.kacc[v][i] = du[v]; a.gacc[v][i] = g; }
+      }
+    }
+  } else if (NOISE) {
     // EM and the EulerHeun predictor: (u + f dt) + (g xi) sqrt(dt), xi keyed
     // 0x100/0x101; the EulerHeun corrector draws fresh noise (0x500/0x501)
     double z[4] = {0.0, 0.0, 0.0, 0.0};
@@ -330,7 +373,7 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
 // neighbor candidates (Law = the pair law functor) fused with the stage
 // update.  Built-in laws instantiate it in libcbm_hip.so; user laws are
 // compiled around it with hipRTC.
-template <int D, int NB, class Law, int PER, int NOISE>
+template <int D, int NB, class Law, int PER, int NOISE, class Noise = ConstNoise>
 __device__ __forceinline__ void stage_body(const StageArgs& a) {
   if (a.gate && *a.gate != a.gate_run) return;
   if (NB == NB_CELL) {
@@ -357,7 +400,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a) {
   for (int c = 0; c < D; ++c) xi[c] = a.S[c][i];
   double acc[3] = {0.0, 0.0, 0.0};
   if (Law::active) gather_global<D, NB, Law, PER>(a, sd, N, i, xi, acc);
-  stage_finish<D, Law, NOISE>(a, sd, ri.sim, i, xi, acc, xi);
+  stage_finish<D, Law, NOISE, false, Noise>(a, sd, ri.sim, i, xi, acc, xi);
 }
 
 // ------------------------------------------------ cell-linked brick tiles
@@ -630,7 +673,7 @@ __device__ __forceinline__ void brick_walk(const BrickTile& t, int h, Visit&& vi
   }
 }
 
-template <int D, class Law, int PER, int NOISE>
+template <int D, class Law, int PER, int NOISE, class Noise = ConstNoise>
 __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* lds) {
   constexpr bool SOFT = same_type<Law, PairLaw<LAW_SOFT_SPHERES>>::value;
   // Built-in laws walk the candidates twice: a cheap pass keeps the LDS
@@ -787,7 +830,7 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
         walk([&](int32_t q) { body(q, q != li); });
       }
     }
-    stage_finish<D, Law, NOISE, true>(a, sd, sim, i, xi, acc, upre, disp2);
+    stage_finish<D, Law, NOISE, true, Noise>(a, sd, sim, i, xi, acc, upre, disp2);
   }
 }
 

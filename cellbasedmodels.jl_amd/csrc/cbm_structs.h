@@ -137,6 +137,7 @@ struct StageArgs {
   const double* U[MAX_STATE];   // step-start state u
   double* out[MAX_STATE];       // stage output (h, or u')
   double* kacc[MAX_STATE];      // RK4 accumulator
+  double* gacc[MAX_STATE];      // EulerHeun with a user agentSDE body: g of stage 1
   const double* F[3];           // committed force accumulator (lagged mode)
   double* Fnew[3];              // force accumulator written by this stage
   const double* xref[3];        // positions at the last Verlet build

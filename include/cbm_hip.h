@@ -192,6 +192,27 @@ const char* cbm_version(void);
  * hipRTC log in cbm_last_error. */
 int cbm_compile(cbm_ctx* ctx, const char* pair_body_hip);
 
+/* compileABM! for the agentODE and agentSDE bodies together
+ * (src/AgentStructure/functionDE.jl:13-116 builds both f and g).  law_body
+ * as for cbm_compile (required with CBM_LAW_CUSTOM, NULL otherwise: the
+ * built-in law is compiled around the other body).  noise_body (EM or
+ * EulerHeun models; NULL = the model's constant `noise` on the positions)
+ * is HIP C++ defining, inside namespace cbm:
+ *   struct UserNoise {
+ *     __device__ explicit UserNoise(const double* p);   // the simulation's p[8]
+ *     __device__ double operator()(int var, const double* u) const;
+ *   };
+ * the diagonal noise amplitude g of DE variable `var` at the agent state u
+ * (positions, then velocities or volume), evaluated at each stage's state as
+ * the reference's g: EM u' = u + f dt + g xi sqrt(dt)
+ * (src/integrators.jl:709-727); EulerHeun averages g of both stages with a
+ * fresh normal (:775-803).  One normal per DE variable, keyed by (seed, step,
+ * agent id) as for the constant noise.  rule_body must be NULL (user
+ * agentRule bodies are not translated; the built-in rules cover growth,
+ * division and death). */
+int cbm_compile_model(cbm_ctx* ctx, const char* law_body, const char* noise_body,
+                      const char* rule_body);
+
 /* ------------------------------------------------------- data transfer */
 /* loadToPlatform! / bringFromPlatform! (communityStructure.jl:606-711).
  * `n` values in logical slot order for simulation `sim`.  Uploading CBM_F_X

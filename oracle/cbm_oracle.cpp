@@ -140,6 +140,7 @@ struct Orc {
   double t = 0;
   bool growth = false;
   bool inertial = false;                      // CBM_LAW_SOFT_SPHERES: velocities are DE variables
+  double noise_slope = 0.0;                   // state-dependent agentSDE g_a(x) = noise + slope x_a (positions)
   std::vector<double> X[3], F[3], Fnew[3];
   std::vector<double> V[3], rad, mass;        // velocities, radius, mass (soft spheres)
   std::vector<int64_t> fate;                  // cellFate, 1-based
@@ -634,12 +635,18 @@ static void de_step(Orc& o, int64_t nA) {
     eval(U, K1);
     const double sq = std::sqrt(dt);
     const double gsig = o.m.noise;
+    const double slope = o.noise_slope;
+    std::vector<double> G1[3];                 // g of the predictor (state-dependent noise)
+    for (int a = 0; a < d; ++a) G1[a].assign(nA, gsig);
     for (int64_t i = 0; i < nA; ++i) {
       double z[4];
       U4 r0 = draw(o.m.seed, o.step, o.id[i], 0, 0x100u);
       normal2(r0, z[0], z[1]);
       if (d > 2) { U4 r1 = draw(o.m.seed, o.step, o.id[i], 0, 0x101u); normal2(r1, z[2], z[3]); }
-      for (int a = 0; a < d; ++a) H[a][i] = (U[a][i] + K1[a][i] * dt) + gsig * z[a] * sq;
+      for (int a = 0; a < d; ++a) {
+        if (slope != 0.0) G1[a][i] = gsig + slope * U[a][i];
+        H[a][i] = (U[a][i] + K1[a][i] * dt) + G1[a][i] * z[a] * sq;
+      }
       for (int v = d; v < nv; ++v) H[v][i] = U[v][i] + K1[v][i] * dt;    // no noise off the positions
     }
     for (int v = 0; v < nv; ++v) K2[v].assign(nA, 0);
@@ -649,20 +656,27 @@ static void de_step(Orc& o, int64_t nA) {
       U4 r0 = draw(o.m.seed, o.step, o.id[i], 0, 0x500u);
       normal2(r0, z[0], z[1]);
       if (d > 2) { U4 r1 = draw(o.m.seed, o.step, o.id[i], 0, 0x501u); normal2(r1, z[2], z[3]); }
-      for (int a = 0; a < d; ++a)
-        U[a][i] = U[a][i] + ((K1[a][i] + K2[a][i]) * dt + (gsig + gsig) * z[a] * sq) / 2;
+      for (int a = 0; a < d; ++a) {
+        // (g(u) + g(h)) of the reference's corrector (src/integrators.jl:801-803)
+        const double g2 = slope != 0.0 ? gsig + slope * H[a][i] : gsig;
+        U[a][i] = U[a][i] + ((K1[a][i] + K2[a][i]) * dt + (G1[a][i] + g2) * z[a] * sq) / 2;
+      }
       for (int v = d; v < nv; ++v) U[v][i] = U[v][i] + ((K1[v][i] + K2[v][i]) * dt) / 2;
     }
   } else {  // EM
     eval(U, K1);
     double sq = std::sqrt(dt);
     double gsig = o.m.noise;
+    const double slope = o.noise_slope;
     for (int64_t i = 0; i < nA; ++i) {
       double z[4];
       U4 r0 = draw(o.m.seed, o.step, o.id[i], 0, 0x100u);
       normal2(r0, z[0], z[1]);
       if (d > 2) { U4 r1 = draw(o.m.seed, o.step, o.id[i], 0, 0x101u); normal2(r1, z[2], z[3]); }
-      for (int a = 0; a < d; ++a) U[a][i] = (U[a][i] + K1[a][i] * dt) + gsig * z[a] * sq;
+      for (int a = 0; a < d; ++a) {
+        const double g = slope != 0.0 ? gsig + slope * U[a][i] : gsig;   // g at u (src/integrators.jl:709-727)
+        U[a][i] = (U[a][i] + K1[a][i] * dt) + g * z[a] * sq;
+      }
       for (int v = d; v < nv; ++v) U[v][i] = U[v][i] + K1[v][i] * dt;
     }
   }
@@ -929,6 +943,10 @@ int orc_threads(void) {
 }
 
 // OpenMP threads of the oracle's parallel loops (the CPU baseline at 1 thread).
+// A state-dependent agentSDE term for the checks of user noise bodies:
+// g_a(x) = noise + slope * x_a on the positions.
+void orc_set_noise_slope(Orc* o, double slope) { if (o) o->noise_slope = slope; }
+
 void orc_set_threads(int n) {
 #ifdef _OPENMP
   if (n > 0) omp_set_num_threads(n);

@@ -48,6 +48,7 @@ def lib():
             "orc_floor_div": (C.c_int64, [C.c_double, C.c_double]),
             "orc_threads": (C.c_int, []),
             "orc_set_threads": (None, [C.c_int]),
+            "orc_set_noise_slope": (None, [P, C.c_double]),
             "orc_traj_loss": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P]),
             "orc_traj_grad_dual": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P, P]),
         }
@@ -99,6 +100,10 @@ class Oracle:
     @property
     def t(self):
         return self.lib.orc_t(self.h)
+
+    def set_noise_slope(self, slope):
+        """State-dependent agentSDE term g_a(x) = noise + slope * x_a on the positions."""
+        self.lib.orc_set_noise_slope(self.h, float(slope))
 
     @property
     def rebuilds(self):

@@ -240,6 +240,37 @@ def test_euler_heun_corrector_draws_fresh_noise():
     assert abs(v - 1.25) < 5 * 1.25 * np.sqrt(2.0 / n), v
 
 
+def _one_step_free(integ, g0, slope, x0, dt, seed=13):
+    m = model(1, integrator=integ, neighbors=A.FULL, dt=dt, noise=g0, seed=seed)
+    o = O.Oracle(m, sim(len(x0), [[-100.0, 100.0]]))
+    o.set_noise_slope(slope)
+    o.set(A.F_X, x0)
+    o.init()
+    o.step(1)
+    return o.get(A.F_X)
+
+
+def test_state_dependent_noise_closed_forms():
+    """g(x) = g0 + s x evaluated as the reference's g (src/integrators.jl:709-727,
+    775-803): EM at u; EulerHeun's predictor at u and its corrector averaging
+    g(u) and g(h) with a fresh normal.  The normals are read back from runs
+    with g = 1: z1 = EM increment / sqrt(dt), z2 = EulerHeun increment /
+    sqrt(dt).  Evaluating g at the new state, or one g for both EulerHeun
+    stages, fails these."""
+    dt, g0, s = 0.04, 0.3, 0.5
+    x0 = np.linspace(-2.0, 2.0, 9)
+    sq = np.sqrt(dt)
+    z1 = (_one_step_free(A.EM, 1.0, 0.0, np.zeros(9), dt) - 0.0) / sq
+    z2 = _one_step_free(A.EULER_HEUN, 1.0, 0.0, np.zeros(9), dt) / sq
+    em = _one_step_free(A.EM, g0, s, x0, dt)
+    assert np.allclose(em, x0 + (g0 + s * x0) * z1 * sq, rtol=1e-14, atol=1e-15)
+    eh = _one_step_free(A.EULER_HEUN, g0, s, x0, dt)
+    h = x0 + (g0 + s * x0) * z1 * sq
+    assert np.allclose(eh, x0 + ((g0 + s * x0) + (g0 + s * h)) * z2 * sq / 2, rtol=1e-14, atol=1e-15)
+    # slope 0: the constant noise
+    assert np.allclose(_one_step_free(A.EM, g0, 0.0, x0, dt), x0 + g0 * z1 * sq, rtol=1e-14, atol=1e-15)
+
+
 # --------------------------------------------------------- force invariants
 def test_newton_third_law_and_two_body_closed_form():
     box = np.array([[0.0, 30.0]] * 3)
