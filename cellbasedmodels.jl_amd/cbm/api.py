@@ -192,6 +192,10 @@ class ABM:
                 a number (g on the positions) or a HIP C++ body defining
                 struct UserNoise (g of each DE variable at the agent's state),
                 compiled with hipRTC by loadToPlatform_
+    agentRule   a built-in rule (CBMRules.*) or a HIP C++ body defining struct
+                UserRule (optionally as (body, rule_p)), compiled with hipRTC
+    agent       user agent parameters {name: float} (at most 4) for a UserRule
+                body: Community attributes, u[agent::name] in the body
     lagged      dt(x) reads the previous step's committed force accumulator, the
                 reference's read-old/write-new idiom (agentStructure.jl:352-453)
     periodic    periodic box with minimum-image distances
@@ -199,13 +203,29 @@ class ABM:
 
     def __init__(self, dims, agentODE=None, agentSDE=0.0, agentRule=None, drift=0.0,
                  neighborsAlg=None, agentAlg=None, platform=None, periodic=False,
-                 lagged=False, seed=0, medium=None):
+                 lagged=False, seed=0, medium=None, agent=None):
         if dims not in (1, 2, 3):
             raise ValueError("dims must be 1, 2 or 3")
         self.dims = dims
         self.agentODE = agentODE if agentODE is not None else CBMModels.noInteraction()
         self.agentSDE = agentSDE if isinstance(agentSDE, str) else float(agentSDE)
+        if isinstance(agentRule, str):
+            agentRule = (agentRule, [0.0] * 4)
+        if agentRule is not None and isinstance(agentRule[0], str):
+            self.rule_src = agentRule[0]
+            agentRule = (A.RULE_CUSTOM, list(agentRule[1]))
+        else:
+            self.rule_src = None
         self.agentRule = agentRule
+        agent = dict(agent or {})
+        if len(agent) > 4:
+            raise ValueError("at most 4 user agent parameters")
+        if agent and self.rule_src is None:
+            raise ValueError("user agent parameters need an agentRule body")
+        for name, typ in agent.items():
+            if typ is not float or name in _FIELDS or not name.isidentifier():
+                raise ValueError(f"user agent parameter {name!r} must be a new Float64 name")
+        self.user_fields = {name: A.F_U0 + k for k, name in enumerate(agent)}
         self.drift = float(drift)
         self.neighbors = neighborsAlg if neighborsAlg is not None else CBMNeighbors.Full()
         self.agentAlg = agentAlg if agentAlg is not None else CBMIntegrators.Euler()
@@ -371,10 +391,11 @@ class Context:
         """Compile a user interaction body (struct UserLaw, HIP C++) with hipRTC."""
         self._chk(self.lib.cbm_compile(self.h, source.encode()))
 
-    def compile_model(self, law=None, noise=None):
-        """Compile user agentODE (struct UserLaw) and/or agentSDE (struct UserNoise) bodies."""
+    def compile_model(self, law=None, noise=None, rule=None):
+        """Compile user agentODE (struct UserLaw), agentSDE (struct UserNoise) and
+        agentRule (struct UserRule) bodies, all of the model's in one call."""
         enc = lambda s: s.encode() if s is not None else None
-        self._chk(self.lib.cbm_compile_model(self.h, enc(law), enc(noise), None))
+        self._chk(self.lib.cbm_compile_model(self.h, enc(law), enc(noise), enc(rule)))
 
     def set_params(self, p, sim=0):
         v = np.ascontiguousarray(p, dtype=np.float64)
@@ -427,6 +448,7 @@ class Community:
         self.t = 0.0
         self.simBox = np.asarray(simBox, dtype=np.float64).reshape(-1, 2)[: abm.dims]
         self.law_p = lawParameters
+        self.fields = dict(_FIELDS, **abm.user_fields)
         self.params = {}
         for k in ("x", "y", "z")[: abm.dims]:
             self.params[k] = np.zeros(self.N)
@@ -440,6 +462,13 @@ class Community:
             self.params["r"] = np.ones(self.N)
             self.params["m"] = np.ones(self.N)
             self.params["cellFate"] = np.ones(self.N, dtype=np.int64)
+        if abm.rule_src is not None:
+            self.params["vol"] = np.ones(self.N)
+            self.params["growth"] = np.zeros(self.N)
+            self.params["vbirth"] = np.ones(self.N)
+            self.params["lf"] = np.zeros(self.N, dtype=np.int64)
+            for k in abm.user_fields:
+                self.params[k] = np.zeros(self.N)
         for k, v in params.items():
             self[k] = v
         self.ctx = None
@@ -449,9 +478,9 @@ class Community:
         return self.params[k]
 
     def __setitem__(self, k, v):
-        if k not in _FIELDS:
+        if k not in self.fields:
             raise KeyError(f"{k} is not an agent parameter of this model")
-        dt = np.int64 if _FIELDS[k] in A.INT_FIELDS else np.float64
+        dt = np.int64 if self.fields[k] in A.INT_FIELDS else np.float64
         arr = np.broadcast_to(np.asarray(v, dtype=dt), (self.N,)).copy()
         self.params[k] = arr
 
@@ -462,7 +491,7 @@ class Community:
         raise AttributeError(k)
 
     def __setattr__(self, k, v):
-        if k in _FIELDS and "params" in self.__dict__:
+        if "params" in self.__dict__ and k in self.__dict__["fields"]:
             self[k] = v
         else:
             object.__setattr__(self, k, v)
@@ -477,10 +506,14 @@ def loadToPlatform_(com, preallocateAgents=0):
     com.ctx = Context(model, [sim])
     law_src = abm.agentODE.source if abm.agentODE.law == A.LAW_CUSTOM else None
     noise_src = abm.agentSDE if isinstance(abm.agentSDE, str) else None
-    if law_src is not None or noise_src is not None:
-        com.ctx.compile_model(law_src, noise_src)
+    rule_src = None
+    if abm.rule_src is not None:
+        names = "".join(f"constexpr int {n} = {f - A.F_U0};\n" for n, f in abm.user_fields.items())
+        rule_src = "namespace agent {\n" + names + "}\n" + abm.rule_src
+    if law_src is not None or noise_src is not None or rule_src is not None:
+        com.ctx.compile_model(law_src, noise_src, rule_src)
     for k, arr in com.params.items():
-        com.ctx.upload(_FIELDS[k], arr)
+        com.ctx.upload(com.fields[k], arr)
     com.ctx.neighbors()
     return com
 
@@ -492,7 +525,7 @@ def bringFromPlatform_(com):
     object.__setattr__(com, "N", n)
     com.t, _ = ctx.time()
     for k in list(com.params):
-        com.params[k] = ctx.download(_FIELDS[k], n)
+        com.params[k] = ctx.download(com.fields[k], n)
     return com
 
 

@@ -5,10 +5,10 @@ HERE="$(cd "$(dirname "$0")" && pwd)"
 OUT="$HERE/../cbm/lib"
 mkdir -p "$OUT"
 HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
-# the stage-kernel headers, embedded for hipRTC compilation of user laws
+# the stage- and rule-kernel headers, embedded for hipRTC compilation of user bodies
 INC="$OUT/cbm_rtc_src.inc"
 {
-  for h in cbm_device.h cbm_structs.h cbm_stage.h; do
+  for h in cbm_device.h cbm_structs.h cbm_stage.h cbm_rule.h; do
     v=$(echo "$h" | tr '.' '_')
     printf 'static const char* cbm_rtc_%s = R"CBMRTC(' "$v"
     cat "$HERE/$h"

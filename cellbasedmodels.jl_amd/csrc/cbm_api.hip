@@ -36,6 +36,7 @@ enum ColId {
   C_VX, C_VY, C_VZ,                     // velocities (soft spheres), DE state too
   C_ID, C_SLOT, C_FX, C_FY, C_FZ, C_GR, C_VB, C_LF,
   C_RAD, C_MASS, C_FATE,                // per-agent soft-sphere properties
+  C_U0, C_U1, C_U2, C_U3,               // user agent parameters (CBM_RULE_CUSTOM)
   C_NCOLS
 };
 
@@ -142,6 +143,7 @@ struct cbm_ctx {
   hipModule_t user_mod = nullptr;
   hipFunction_t user_fn[2] = {nullptr, nullptr};   // [0] deterministic stages, [1] noisy (EM, EulerHeun)
   hipFunction_t user_brick[2] = {nullptr, nullptr};   // the same, cell-linked brick tiles
+  hipFunction_t user_rule[2] = {nullptr, nullptr};    // agentRule body: decide, apply
   int user_nb = NB_FULL;                  // neighbor walk the module was built for
   bool user_noise = false;                // the module holds a user agentSDE body
   double* gacc[MAX_STATE] = {};           // EulerHeun with a user agentSDE body: g of stage 1
@@ -205,7 +207,7 @@ static int validate(cbm_ctx* c) {
   if (verlet && !(m.skin > 0)) return fail(c, CBM_ERR_CONFIG, "Verlet algorithms need skin > 0");
   if (verlet && (m.max_neighbors < 1 || m.max_neighbors > 4096)) return fail(c, CBM_ERR_CONFIG, "nMaxNeighbors must be in [1, 4096]");
   if (m.neighbors == CBM_VERLET_TIME && !(m.dt_recompute > 0)) return fail(c, CBM_ERR_CONFIG, "VerletTime needs dtNeighborRecompute > 0");
-  if (m.rule < CBM_RULE_NONE || m.rule > CBM_RULE_DEATH) return fail(c, CBM_ERR_CONFIG, "unknown agent rule");
+  if (m.rule < CBM_RULE_NONE || m.rule > CBM_RULE_CUSTOM) return fail(c, CBM_ERR_CONFIG, "unknown agent rule");
   if (m.rule != CBM_RULE_NONE && c->S != 1) return fail(c, CBM_ERR_CONFIG, "agent rules (division/death) need a single simulation per context");
   if (m.medium_n[0] > 0) {
     if (c->S != 1) return fail(c, CBM_ERR_CONFIG, "a medium needs a single simulation per context");
@@ -459,7 +461,7 @@ static int init_ctx(cbm_ctx* c) {
   int rc = validate(c);
   if (rc) return rc;
   c->dims = m.dims;
-  c->growth = m.rule == CBM_RULE_GROWTH_DIVISION;
+  c->growth = m.rule == CBM_RULE_GROWTH_DIVISION || m.rule == CBM_RULE_CUSTOM;
   c->rules = m.rule != CBM_RULE_NONE;
   c->inertial = m.law == CBM_LAW_SOFT_SPHERES;
   c->nvar = c->inertial ? 2 * c->dims : c->dims + (c->growth ? 1 : 0);
@@ -517,6 +519,8 @@ static int init_ctx(cbm_ctx* c) {
   }
   if (c->rules) {
     if ((rc = alloc_column(c, C_LF, 8))) return rc;
+    if (m.rule == CBM_RULE_CUSTOM)
+      for (int k = 0; k < 4; ++k) if ((rc = alloc_column(c, C_U0 + k, 8))) return rc;
     const int64_t nm = c->sims[0].n_max;
     HIPCHK(c, dalloc(&c->dead, c->C));
     HIPCHK(c, dalloc(&c->n_add_slot, nm + 1));
@@ -870,6 +874,8 @@ static RuleArgs rule_args(cbm_ctx* c, int32_t* flag) {
   if (c->lagged) for (int d = 0; d < c->dims; ++d) { r.F[d] = (double*)c->col[C_FX + d].cur(); r.Fnew[d] = c->Fnew[d]; }
   r.id = (int64_t*)c->col[C_ID].cur();
   r.lf = (int64_t*)c->col[C_LF].cur();
+  if (c->m.rule == CBM_RULE_CUSTOM)
+    for (int k = 0; k < 4; ++k) r.u[k] = (double*)c->col[C_U0 + k].cur();
   r.slot = (int32_t*)c->col[C_SLOT].cur();
   r.dead = c->dead;
   r.n_add_slot = c->n_add_slot; r.dead_slot = c->dead_slot;
@@ -888,11 +894,18 @@ static int slot_scan(cbm_ctx* c, int32_t* in, int32_t* out, int64_t n) {
 static int rule_phase(cbm_ctx* c, int32_t* flag) {
   RuleArgs r = rule_args(c, flag);
   const int64_t nm = c->sims[0].n_max;
-  HIPCHK(c, launch_rule_decide(c->nblocks, c->block, c->stream, r));
+  const bool user = c->m.rule == CBM_RULE_CUSTOM;
+  if (user && !c->user_rule[0])
+    return fail(c, CBM_ERR_CONFIG, "CBM_RULE_CUSTOM needs cbm_compile_model with a rule body before stepping");
+  void* params[] = {(void*)&r};
+  auto user_launch = [&](hipFunction_t f) {
+    return hipModuleLaunchKernel(f, (unsigned)c->nblocks, 1, 1, (unsigned)c->block, 1, 1, 0, c->stream, params, nullptr);
+  };
+  HIPCHK(c, user ? user_launch(c->user_rule[0]) : launch_rule_decide(c->nblocks, c->block, c->stream, r));
   int rc;
   if ((rc = slot_scan(c, c->n_add_slot, c->off_add, nm))) return rc;
   if ((rc = slot_scan(c, c->dead_slot, c->off_dead, nm))) return rc;
-  HIPCHK(c, launch_rule_apply(c->nblocks, c->block, c->stream, r));
+  HIPCHK(c, user ? user_launch(c->user_rule[1]) : launch_rule_apply(c->nblocks, c->block, c->stream, r));
   return CBM_OK;
 }
 
@@ -1033,6 +1046,7 @@ static int check_flags(cbm_ctx* c) {
   HIPCHK(c, hipMemcpyAsync(f, c->flags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (f[2]) return fail(c, CBM_ERR_LIST_OVERFLOW, "Verlet list overflow: raise nMaxNeighbors");
+  if (f[3] & 2) return fail(c, CBM_ERR_CONFIG, "an agentRule body added more than 2 agents for one agent in one step");
   if (f[3]) return fail(c, CBM_ERR_CAPACITY, "agent capacity exceeded: raise preallocation (n_max)");
   return CBM_OK;
 }
@@ -1058,23 +1072,31 @@ void cbm_destroy(cbm_ctx* c) { destroy_ctx(c); }
 
 const char* cbm_last_error(const cbm_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
-// compileABM! for a user interaction body: the body (HIP C++ defining
-// struct UserLaw, see include/cbm_hip.h) is compiled by hipRTC into the
-// same stage skeleton as the built-in laws, instantiated for this model's
-// dimension, neighbor walk and boundary.
+// compileABM! for user bodies: each (HIP C++ defining struct UserLaw,
+// UserNoise or UserRule, see include/cbm_hip.h) is compiled by hipRTC into
+// the same skeletons as the built-in laws and rules, instantiated for this
+// model's dimension, neighbor walk and boundary.
 int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, const char* rule_body) {
   if (!c) return CBM_ERR_ARG;
-  if (rule_body) return fail(c, CBM_ERR_CONFIG, "user agentRule bodies are not supported; use a built-in rule");
   const bool custom = c->m.law == CBM_LAW_CUSTOM;
   if (custom != (law_body != nullptr))
     return fail(c, CBM_ERR_CONFIG, "a law body is required with law CBM_LAW_CUSTOM and only then");
+  if ((c->m.rule == CBM_RULE_CUSTOM) != (rule_body != nullptr))
+    return fail(c, CBM_ERR_CONFIG, "a rule body is required with rule CBM_RULE_CUSTOM and only then");
   if (noise_body && c->m.integrator != CBM_EM && c->m.integrator != CBM_EULER_HEUN)
     return fail(c, CBM_ERR_CONFIG, "an agentSDE body needs the EM or EulerHeun integrator");
-  if (!law_body && !noise_body) return fail(c, CBM_ERR_ARG, "nothing to compile");
-  std::string src = "#define CBM_HIPRTC 1\n#include \"cbm_stage.h\"\nnamespace cbm {\n";
+  if (!law_body && !noise_body && !rule_body) return fail(c, CBM_ERR_ARG, "nothing to compile");
+  const bool stages = law_body || noise_body;
+  std::string src = "#define CBM_HIPRTC 1\n#include \"cbm_rule.h\"\nnamespace cbm {\n";
   if (law_body) { src += law_body; src += "\n"; }
   if (noise_body) { src += noise_body; src += "\n"; }
+  if (rule_body) { src += rule_body; src += "\n"; }
   src += "}  // namespace cbm\n";
+  if (rule_body)
+    src += "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_rule_decide(cbm::RuleArgs a) {\n"
+           "  cbm::user_rule_decide<cbm::UserRule>(a);\n}\n"
+           "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_rule_apply(cbm::RuleArgs a) {\n"
+           "  cbm::user_rule_apply<cbm::UserRule>(a);\n}\n";
   char line[256];
   if (law_body) src += "using cbm_law_t = cbm::UserLaw;\n";
   else { std::snprintf(line, sizeof(line), "using cbm_law_t = cbm::PairLaw<%d>;\n", c->m.law); src += line; }
@@ -1084,7 +1106,7 @@ int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, 
   const int nb = walk ? c->nb_kind : NB_FULL;
   const int per = walk && c->m.periodic ? 1 : 0;
   char inst[512];
-  for (int noise = 0; noise < 2; ++noise) {
+  for (int noise = 0; stages && noise < 2; ++noise) {
     std::snprintf(inst, sizeof(inst),
                   "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_stage_%d(cbm::StageArgs a) {\n"
                   "  cbm::stage_body<%d, %d, cbm_law_t, %d, %d, cbm_noise_t>(a);\n}\n",
@@ -1099,10 +1121,10 @@ int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, 
       src += inst;
     }
   }
-  const char* headers[] = {cbm_rtc_cbm_device_h, cbm_rtc_cbm_structs_h, cbm_rtc_cbm_stage_h};
-  const char* names[] = {"cbm_device.h", "cbm_structs.h", "cbm_stage.h"};
+  const char* headers[] = {cbm_rtc_cbm_device_h, cbm_rtc_cbm_structs_h, cbm_rtc_cbm_stage_h, cbm_rtc_cbm_rule_h};
+  const char* names[] = {"cbm_device.h", "cbm_structs.h", "cbm_stage.h", "cbm_rule.h"};
   hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, src.c_str(), "cbm_user_bodies.hip", 3, headers, names) != HIPRTC_SUCCESS)
+  if (hiprtcCreateProgram(&prog, src.c_str(), "cbm_user_bodies.hip", 4, headers, names) != HIPRTC_SUCCESS)
     return fail(c, CBM_ERR_COMPILE, "hiprtcCreateProgram failed");
   const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math"};
   hiprtcResult r = hiprtcCompileProgram(prog, 5, opts);
@@ -1120,11 +1142,17 @@ int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, 
   hiprtcGetCode(prog, code.data());
   hiprtcDestroyProgram(&prog);
   if (c->user_mod) { (void)hipModuleUnload(c->user_mod); c->user_mod = nullptr; }
-  for (int k = 0; k < 2; ++k) { c->user_fn[k] = nullptr; c->user_brick[k] = nullptr; }
+  for (int k = 0; k < 2; ++k) { c->user_fn[k] = nullptr; c->user_brick[k] = nullptr; c->user_rule[k] = nullptr; }
   HIPCHK(c, hipModuleLoadData(&c->user_mod, code.data()));
-  HIPCHK(c, hipModuleGetFunction(&c->user_fn[0], c->user_mod, "cbm_user_stage_0"));
-  HIPCHK(c, hipModuleGetFunction(&c->user_fn[1], c->user_mod, "cbm_user_stage_1"));
-  if (nb == NB_CELL) {
+  if (rule_body) {
+    HIPCHK(c, hipModuleGetFunction(&c->user_rule[0], c->user_mod, "cbm_user_rule_decide"));
+    HIPCHK(c, hipModuleGetFunction(&c->user_rule[1], c->user_mod, "cbm_user_rule_apply"));
+  }
+  if (stages) {
+    HIPCHK(c, hipModuleGetFunction(&c->user_fn[0], c->user_mod, "cbm_user_stage_0"));
+    HIPCHK(c, hipModuleGetFunction(&c->user_fn[1], c->user_mod, "cbm_user_stage_1"));
+  }
+  if (stages && nb == NB_CELL) {
     HIPCHK(c, hipModuleGetFunction(&c->user_brick[0], c->user_mod, "cbm_user_brick_0"));
     HIPCHK(c, hipModuleGetFunction(&c->user_brick[1], c->user_mod, "cbm_user_brick_1"));
   }
@@ -1163,6 +1191,9 @@ static int field_col(cbm_ctx* c, int32_t field, int& k) {
     case CBM_F_RADIUS: if (!c->inertial) return CBM_ERR_ARG; k = C_RAD; return CBM_OK;
     case CBM_F_MASS: if (!c->inertial) return CBM_ERR_ARG; k = C_MASS; return CBM_OK;
     case CBM_F_FATE: if (!c->inertial) return CBM_ERR_ARG; k = C_FATE; return CBM_OK;
+    case CBM_F_U0: case CBM_F_U1: case CBM_F_U2: case CBM_F_U3:
+      if (c->m.rule != CBM_RULE_CUSTOM) return CBM_ERR_ARG;
+      k = C_U0 + (field - CBM_F_U0); return CBM_OK;
     default: return CBM_ERR_ARG;
   }
 }
@@ -1256,6 +1287,8 @@ int cbm_neighbors(cbm_ctx* c) {
 int cbm_step(cbm_ctx* c, int64_t nsteps) {
   if (!c || nsteps < 0) return CBM_ERR_ARG;
   if (c->m.law == CBM_LAW_CUSTOM && !c->user_fn[0]) return fail(c, CBM_ERR_CONFIG, "law CBM_LAW_CUSTOM: call cbm_compile first");
+  if (c->m.rule == CBM_RULE_CUSTOM && !c->user_rule[0])
+    return fail(c, CBM_ERR_CONFIG, "rule CBM_RULE_CUSTOM: call cbm_compile_model with a rule body first");
   c->last_launches = 0;
   if (!c->structure_valid) { int rc = force_rebuild(c); if (rc) return rc; }
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));

@@ -109,38 +109,6 @@ struct ExportArgs {
   int32_t pad;
 };
 
-// Agent rules (agentRule with @addAgent/@removeAgent) and the canonical
-// compaction of update!.  Arrays marked (slot) are indexed by logical slot.
-struct RuleArgs {
-  RowGeom g;
-  int32_t rule, dims, lagged, growth;
-  double rp[4];
-  double dt;
-  uint64_t seed;
-  const int64_t* step_ptr;
-  double* pos[3];               // current positions (DE state, written in place)
-  double* vol;                  // current volume (DE state)
-  double* pos_alt[3];           // the other DE-state buffers: new agents are
-  double* vol_alt;              // written to both (out-of-place stages flip them)
-  const double* gr;             // growth rate (read-old)
-  double* gr_new;               // growth rate (write-new)
-  const double* vb;             // birth volume (read-old)
-  double* vb_new;               // birth volume (write-new)
-  double* F[3];                 // committed force accumulator (lagged)
-  double* Fnew[3];
-  int64_t* id;
-  int64_t* lf;
-  int32_t* slot;
-  uint8_t* dead;                // per row
-  int32_t* n_add_slot;          // (slot) daughters of this agent, consumed by the scan
-  int32_t* dead_slot;           // (slot) 1 if removed, consumed by the scan
-  const int32_t* off_add;       // (slot) exclusive prefix of n_add_slot, [n_max] = total
-  const int32_t* off_dead;      // (slot) exclusive prefix of dead_slot, [n_max] = total
-  int32_t* holes;               // ascending hole slots below the new N
-  int32_t* err;                 // capacity error flag
-  int32_t* rebuild_flag;        // set when agents were added or removed
-};
-
 // Medium: one field on a cell-centred grid over the box of simulation 0,
 // stepped by explicit FTCS with zero-flux (mirror) boundaries and agent
 // uptake (mediumODE with @∂2, src/AgentStructure/macros.jl:203-257).

@@ -98,12 +98,17 @@ enum cbm_law {
  *     (test/testCommunity.jl:241-316): x += 1; odd id -> @removeAgent();
  *     even id -> @addAgent(lf = id).
  *   CBM_RULE_DEATH   rule_p = {rate}: each agent is removed with probability
- *     rate*dt per step (Philox uniform keyed by seed, step, id).            */
+ *     rate*dt per step (Philox uniform keyed by seed, step, id).
+ *   CBM_RULE_CUSTOM  a user agentRule body compiled by cbm_compile_model
+ *     (required before stepping).  The agents carry vol, growth, vbirth
+ *     (dvol/dt = growth*vol, as for CBM_RULE_GROWTH_DIVISION), lf and the
+ *     user parameters CBM_F_U0..U3 for the body to read and write.       */
 enum cbm_rule {
   CBM_RULE_NONE = 0,
   CBM_RULE_GROWTH_DIVISION = 1,
   CBM_RULE_PARITY = 2,
-  CBM_RULE_DEATH = 3
+  CBM_RULE_DEATH = 3,
+  CBM_RULE_CUSTOM = 4
 };
 
 /* ------------------------------------------------------ per-agent fields */
@@ -117,7 +122,8 @@ enum cbm_field {
   CBM_F_VX = 11, CBM_F_VY = 12, CBM_F_VZ = 13,  /* velocities (CBM_LAW_SOFT_SPHERES)        */
   CBM_F_RADIUS = 14, CBM_F_MASS = 15,
   CBM_F_FATE = 16,                             /* int64 cellFate, 1-based                   */
-  CBM_F_COUNT = 17
+  CBM_F_U0 = 17, CBM_F_U1 = 18, CBM_F_U2 = 19, CBM_F_U3 = 20,   /* user agent parameters (CBM_RULE_CUSTOM) */
+  CBM_F_COUNT = 21
 };
 
 /* --------------------------------------------------- model description */
@@ -207,9 +213,26 @@ int cbm_compile(cbm_ctx* ctx, const char* pair_body_hip);
  * the reference's g: EM u' = u + f dt + g xi sqrt(dt)
  * (src/integrators.jl:709-727); EulerHeun averages g of both stages with a
  * fresh normal (:775-803).  One normal per DE variable, keyed by (seed, step,
- * agent id) as for the constant noise.  rule_body must be NULL (user
- * agentRule bodies are not translated; the built-in rules cover growth,
- * division and death). */
+ * agent id) as for the constant noise.
+ * rule_body (functionRule.jl:121-160 with @addAgent/@removeAgent, :9-111;
+ * required with CBM_RULE_CUSTOM, NULL otherwise) is HIP C++ defining, inside
+ * namespace cbm:
+ *   struct UserRule {
+ *     __device__ UserRule(const double* rp, const double* p);  // rule_p[4], p[8]
+ *     __device__ void operator()(RuleAgent& a) const;
+ *   };
+ * run once per agent and step before the DE stages.  RuleAgent (cbm_rule.h)
+ * holds a.old, the agent's values at the start of the step (x[3], vol,
+ * growth, vbirth, u[4], lf), and a.nw, its new values (initialised to old);
+ * a.id, a.step, a.dt and a.dims; a.add() returns a new agent's record
+ * (initialised to the parent's old values, at most 2 per agent and step),
+ * a.remove() removes the agent; a.uniform(l0, l1), a.normal(mu, sigma) and
+ * a.exponential(theta) are the reference's CBMDistributions
+ * (src/random.jl:1-23) on Philox(seed, step, id, sim, 0x600 + k), k counting
+ * the agent's draws in the step.  New agents get ids idMax + 1 + (prefix of
+ * additions in slot order), the reference's serial append order.  The body
+ * must not read other agents.  Each call replaces the previous module, so
+ * pass every body of the model in one call. */
 int cbm_compile_model(cbm_ctx* ctx, const char* law_body, const char* noise_body,
                       const char* rule_body);
 

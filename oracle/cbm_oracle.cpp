@@ -146,7 +146,9 @@ struct Orc {
   std::vector<int64_t> fate;                  // cellFate, 1-based
   std::vector<double> vol, gr, vb, grNew, vbNew;
   std::vector<int64_t> id, lf;
+  std::vector<double> U[4];                   // user agent parameters (CBM_RULE_CUSTOM)
   std::vector<char> alive;
+  void* rule_fn = nullptr;                    // CBM_RULE_CUSTOM body (orc_rule_fn)
   // neighbor structures
   Grid g;
   std::vector<int64_t> cellOf, cellStart, cellCount, cellAgents;
@@ -500,11 +502,57 @@ static void medium_step(Orc& o, int64_t nAgents) {
 // ------------------------------------------------------------------ rules
 struct Births { std::vector<int64_t> parent; };
 
+// A CBM_RULE_CUSTOM body on the CPU: one agent's values at the start of the
+// step (old), its new values (nw, initialised to old) and up to two added
+// agents (add, each initialised to old by the caller); sets *n_add, *dead.
+struct OrcRuleRec { double x[3]; double vol, growth, vbirth; double u[4]; int64_t lf; };
+typedef void (*orc_rule_fn)(int64_t id, int64_t step, const OrcRuleRec* old, OrcRuleRec* nw, OrcRuleRec* add,
+                            int32_t* n_add, int32_t* dead);
+
+// The generated rule loop of functionRule.jl:121-160 around a custom body,
+// serially in ascending slot order: the new values are written in place (DE
+// state, committed growth/vbirth through the write-new buffers), @addAgent
+// appends at N + NAdd_ with id idMax + 1 + NAdd_ (functionRule.jl:59-69),
+// @removeAgent pushes the slot on the hole list (:98-101).
+static void apply_custom_rule(Orc& o, std::vector<int64_t>& holes, int64_t& nAdd) {
+  orc_rule_fn fn = (orc_rule_fn)o.rule_fn;
+  if (!fn) { o.err = CBM_ERR_CONFIG; o.msg = "no custom rule"; return; }
+  const int64_t N = o.N;
+  for (int64_t i = 0; i < N; ++i) {
+    OrcRuleRec old;
+    for (int a = 0; a < 3; ++a) old.x[a] = a < o.dims ? o.X[a][i] : 0.0;
+    old.vol = o.vol[i]; old.growth = o.gr[i]; old.vbirth = o.vb[i];
+    for (int k = 0; k < 4; ++k) old.u[k] = o.U[k][i];
+    old.lf = o.lf[i];
+    OrcRuleRec nw = old, add[2] = {old, old};
+    int32_t na = 0, dead = 0;
+    fn(o.id[i], o.step, &old, &nw, add, &na, &dead);
+    if (na > 2) { o.err = CBM_ERR_CONFIG; o.msg = "more than 2 additions"; na = 2; }
+    for (int a = 0; a < o.dims; ++a) o.X[a][i] = nw.x[a];
+    o.vol[i] = nw.vol; o.grNew[i] = nw.growth; o.vbNew[i] = nw.vbirth;
+    for (int k = 0; k < 4; ++k) o.U[k][i] = nw.u[k];
+    o.lf[i] = nw.lf;
+    if (dead) { holes.push_back(i); o.alive[i] = 0; }
+    for (int k = 0; k < na; ++k) {
+      const int64_t j = N + nAdd;
+      if (j >= o.nMax) { o.err = CBM_ERR_CAPACITY; o.msg = "agent capacity exceeded"; continue; }
+      const OrcRuleRec& d = add[k];
+      for (int a = 0; a < o.dims; ++a) { o.X[a][j] = d.x[a]; o.F[a][j] = o.F[a][i]; o.Fnew[a][j] = o.F[a][i]; }
+      o.vol[j] = d.vol; o.gr[j] = d.growth; o.grNew[j] = d.growth; o.vb[j] = d.vbirth; o.vbNew[j] = d.vbirth;
+      for (int q = 0; q < 4; ++q) o.U[q][j] = d.u[q];
+      o.lf[j] = d.lf;
+      o.id[j] = o.idMax + 1 + nAdd; o.alive[j] = 1;
+      nAdd++;
+    }
+  }
+}
+
 // The built-in agentRule bodies, executed serially in ascending slot order
 // (the single-thread order that makes @removeAgent's hole list ascending).
 static void apply_rule(Orc& o, std::vector<int64_t>& holes, int64_t& nAdd) {
   holes.clear(); nAdd = 0;
   if (o.m.rule == CBM_RULE_NONE) return;
+  if (o.m.rule == CBM_RULE_CUSTOM) { apply_custom_rule(o, holes, nAdd); return; }
   int64_t N = o.N;
   for (int64_t i = 0; i < N; ++i) {
     if (o.m.rule == CBM_RULE_PARITY) {
@@ -577,6 +625,7 @@ static void compact(Orc& o, const std::vector<int64_t>& holes, int64_t nAdd) {
       if (!(dst < src)) continue;
       for (int a = 0; a < o.dims; ++a) { o.X[a][dst] = o.X[a][src]; o.F[a][dst] = o.F[a][src]; o.Fnew[a][dst] = o.Fnew[a][src]; }
       o.id[dst] = o.id[src]; o.lf[dst] = o.lf[src];
+      for (int k = 0; k < 4; ++k) o.U[k][dst] = o.U[k][src];
       if (o.growth) { o.vol[dst] = o.vol[src]; o.gr[dst] = o.gr[src]; o.vb[dst] = o.vb[src]; o.grNew[dst] = o.grNew[src]; o.vbNew[dst] = o.vbNew[src]; }
     }
   }
@@ -777,7 +826,7 @@ void* orc_create(const cbm_model_desc* m, const cbm_sim_desc* s) {
   o->dims = m->dims;
   o->N = s->n; o->nMax = std::max<int64_t>(s->n_max, s->n);
   o->idMax = s->n;
-  o->growth = (m->rule == CBM_RULE_GROWTH_DIVISION);
+  o->growth = (m->rule == CBM_RULE_GROWTH_DIVISION || m->rule == CBM_RULE_CUSTOM);
   o->inertial = (m->law == CBM_LAW_SOFT_SPHERES);
   if (o->inertial && m->rule != CBM_RULE_NONE) { o->err = CBM_ERR_CONFIG; o->msg = "soft spheres take no rules"; }
   for (int a = 0; a < 3; ++a) {
@@ -789,6 +838,7 @@ void* orc_create(const cbm_model_desc* m, const cbm_sim_desc* s) {
   for (int64_t i = 0; i < o->nMax; ++i) o->id[i] = i + 1;
   o->vol.assign(o->nMax, 1.0); o->gr.assign(o->nMax, 0.0); o->vb.assign(o->nMax, 1.0);
   o->grNew.assign(o->nMax, 0.0); o->vbNew.assign(o->nMax, 1.0);
+  for (int k = 0; k < 4; ++k) o->U[k].assign(o->nMax, 0.0);
   double edge = m->cell_edge;
   if (m->neighbors == CBM_CELL_LINKED) {
     set_grid(*o, o->g, edge);
@@ -827,6 +877,7 @@ int orc_set(void* h, int field, const void* data, int64_t n) {
       case CBM_F_RADIUS: o.rad[i] = v; break;
       case CBM_F_MASS: o.mass[i] = v; break;
       case CBM_F_FATE: if (iv < 1 || iv > 2) return CBM_ERR_ARG; o.fate[i] = iv; break;
+      case CBM_F_U0: case CBM_F_U1: case CBM_F_U2: case CBM_F_U3: o.U[field - CBM_F_U0][i] = v; break;
       default: return CBM_ERR_ARG;
     }
   }
@@ -851,6 +902,7 @@ int orc_get(void* h, int field, void* out, int64_t n) {
       case CBM_F_RADIUS: dv[i] = o.rad[i]; break;
       case CBM_F_MASS: dv[i] = o.mass[i]; break;
       case CBM_F_FATE: iv[i] = o.fate[i]; break;
+      case CBM_F_U0: case CBM_F_U1: case CBM_F_U2: case CBM_F_U3: dv[i] = o.U[field - CBM_F_U0][i]; break;
       default: return CBM_ERR_ARG;
     }
   }
@@ -928,6 +980,10 @@ void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
   U4 r = philox(c, key[0], key[1]);
   for (int i = 0; i < 4; ++i) out[i] = r.v[i];
 }
+// CBM_RULE_CUSTOM: the rule body, called once per agent and step (test code
+// passes a restatement of the device body it checks).
+void orc_set_rule(void* h, void* fn) { ((Orc*)h)->rule_fn = fn; }
+
 void orc_normals(uint64_t seed, int64_t step, int64_t id, int32_t sim, uint32_t purpose, double* z2) {
   U4 r = draw(seed, step, id, sim, purpose);
   normal2(r, z2[0], z2[1]);

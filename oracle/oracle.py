@@ -4,6 +4,7 @@ __graft_entry__.smoke() and by bench.py's cpu_baseline leg, always as the
 checker, never as the product path.
 """
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -49,6 +50,7 @@ def lib():
             "orc_threads": (C.c_int, []),
             "orc_set_threads": (None, [C.c_int]),
             "orc_set_noise_slope": (None, [P, C.c_double]),
+            "orc_set_rule": (None, [P, P]),
             "orc_traj_loss": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P]),
             "orc_traj_grad_dual": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P, P]),
         }
@@ -61,6 +63,68 @@ def lib():
 
 
 INT_FIELDS = (3, 10, 16)   # id, lf, cellFate (int64)
+
+
+class RuleRec(C.Structure):
+    """One agent's values as a CBM_RULE_CUSTOM body sees them (OrcRuleRec)."""
+    _fields_ = [("x", C.c_double * 3), ("vol", C.c_double), ("growth", C.c_double), ("vbirth", C.c_double),
+                ("u", C.c_double * 4), ("lf", C.c_int64)]
+
+
+RULE_FN = C.CFUNCTYPE(None, C.c_int64, C.c_int64, C.POINTER(RuleRec), C.POINTER(RuleRec), C.POINTER(RuleRec),
+                      C.POINTER(C.c_int32), C.POINTER(C.c_int32))
+RULE_PURPOSE = 0x600
+
+
+def u53(a, b):
+    return ((a >> 5) * 67108864.0 + (b >> 6)) * (1.0 / 9007199254740992.0)
+
+
+class RuleAgent:
+    """CPU restatement of a rule body's view of its agent (old, nw, child(),
+    add(), remove(), and the CBMDistributions of src/random.jl:1-23 on the agent's
+    k-th Philox draw of the step, purpose RULE_PURPOSE + k)."""
+
+    def __init__(self, seed, id, step, dims, dt, old, nw, add):
+        self.seed, self.id, self.step, self.dims, self.dt = seed, id, step, dims, dt
+        self.old, self.nw, self._add = old, nw, add
+        self.n_add, self.dead, self._k = 0, 0, 0
+
+    def child(self):
+        return RuleRec.from_buffer_copy(self.old)
+
+    def add(self, d):
+        if self.n_add >= 2:
+            raise RuntimeError("more than 2 additions")
+        C.memmove(C.addressof(self._add[self.n_add]), C.addressof(d), C.sizeof(RuleRec))
+        self.n_add += 1
+
+    def remove(self):
+        self.dead = 1
+
+    def _words(self):
+        ctr = np.array([self.step & 0xFFFFFFFF, self.id & 0xFFFFFFFF, 0, RULE_PURPOSE + self._k], dtype=np.uint32)
+        key = np.array([self.seed & 0xFFFFFFFF, self.seed >> 32], dtype=np.uint32)
+        out = np.zeros(4, dtype=np.uint32)
+        lib().orc_philox(ctr.ctypes.data, key.ctypes.data, out.ctypes.data)
+        self._k += 1
+        return [int(w) for w in out]
+
+    def rand(self):
+        w = self._words()
+        return u53(w[0], w[1])
+
+    def uniform(self, l0, l1):
+        return (l1 - l0) * self.rand() + l0
+
+    def normal(self, mu, sigma):
+        z = np.zeros(2)
+        lib().orc_normals(self.seed, self.step, self.id, 0, RULE_PURPOSE + self._k, z.ctypes.data)
+        self._k += 1
+        return sigma * float(z[0]) + mu
+
+    def exponential(self, theta):
+        return -math.log(1.0 - self.rand()) * theta
 
 
 class Oracle:
@@ -100,6 +164,20 @@ class Oracle:
     @property
     def t(self):
         return self.lib.orc_t(self.h)
+
+    def set_rule(self, body):
+        """CBM_RULE_CUSTOM: body(agent) restates the device rule on a RuleAgent."""
+        m = self.model
+
+        def cb(id_, step, old, nw, add, n_add, dead):
+            ag = RuleAgent(int(m.seed), int(id_), int(step), int(m.dims), float(m.dt), old.contents, nw.contents,
+                           [add[0], add[1]])
+            body(ag)
+            n_add[0] = ag.n_add
+            dead[0] = ag.dead
+
+        self._rule_cb = RULE_FN(cb)
+        self.lib.orc_set_rule(self.h, C.cast(self._rule_cb, C.c_void_p))
 
     def set_noise_slope(self, slope):
         """State-dependent agentSDE term g_a(x) = noise + slope * x_a on the positions."""

@@ -1,0 +1,110 @@
+"""agentRule bodies for the hipRTC rule tests, each as HIP C++ (struct
+UserRule, compiled into the library's rule skeleton) and as its CPU
+restatement on oracle.RuleAgent (the oracle's CBM_RULE_CUSTOM callback)."""
+import math
+
+# test/testCommunity.jl:241-316: x = x + 1; odd ids removed, even ids add one (lf = id)
+PARITY = """
+struct UserRule {
+  __device__ UserRule(const double*, const double*) {}
+  __device__ void operator()(RuleAgent& a) const {
+    a.nw.x[0] = a.old.x[0] + 1.0;
+    if (a.id % 2 == 1) a.remove();
+    else { RuleRec d = a.child(); d.lf = a.id; a.add(d); }
+  }
+};
+"""
+
+
+def parity(a):
+    a.nw.x[0] = a.old.x[0] + 1.0
+    if a.id % 2 == 1:
+        a.remove()
+    else:
+        d = a.child()
+        d.lf = a.id
+        a.add(d)
+
+
+# test/testCommunity.jl:319-356: CBMDistributions drawn into agent parameters
+RANDOM = """
+struct UserRule {
+  __device__ UserRule(const double*, const double*) {}
+  __device__ void operator()(RuleAgent& a) const {
+    a.nw.u[0] = a.normal(1.0, 2.0);
+    a.nw.u[1] = a.uniform(1.0, 2.0);
+    a.nw.u[2] = a.exponential(3.0);
+  }
+};
+"""
+
+
+def random_draws(a):
+    a.nw.u[0] = a.normal(1.0, 2.0)
+    a.nw.u[1] = a.uniform(1.0, 2.0)
+    a.nw.u[2] = a.exponential(3.0)
+
+
+# examples/Bacteries.ipynb agentRule: at vol >= ratio * vbirth, two new
+# agents +-split along a random axis, each with half the volume and a fresh
+# growth rate, and the parent removed; u[0] counts generations
+DIVIDE_2D = """
+struct UserRule {
+  double ratio, split, gm, gs;
+  __device__ UserRule(const double* rp, const double*) : ratio(rp[0]), split(rp[1]), gm(rp[2]), gs(rp[3]) {}
+  __device__ void operator()(RuleAgent& a) const {
+    if (!(a.old.vol >= ratio * a.old.vbirth)) return;
+    const double th = a.uniform(0.0, 6.283185307179586);
+    const double c = cos(th), s = sin(th);
+    const double half = a.old.vol / 2.0;
+    for (int k = 0; k < 2; ++k) {
+      RuleRec d = a.child();
+      const double sg = k == 0 ? 1.0 : -1.0;
+      d.x[0] = a.old.x[0] + sg * split * c;
+      d.x[1] = a.old.x[1] + sg * split * s;
+      d.vol = half;
+      d.vbirth = half;
+      d.growth = a.uniform(gm * (1.0 - gs), gm * (1.0 + gs));
+      d.lf = a.id;
+      d.u[0] = a.old.u[0] + 1.0;
+      a.add(d);
+    }
+    a.remove();
+  }
+};
+"""
+
+
+def divide_2d(rp):
+    ratio, split, gm, gs = rp
+
+    def body(a):
+        if not (a.old.vol >= ratio * a.old.vbirth):
+            return
+        th = a.uniform(0.0, 6.283185307179586)
+        c, s = math.cos(th), math.sin(th)
+        half = a.old.vol / 2.0
+        for k in range(2):
+            d = a.child()
+            sg = 1.0 if k == 0 else -1.0
+            d.x[0] = a.old.x[0] + sg * split * c
+            d.x[1] = a.old.x[1] + sg * split * s
+            d.vol = half
+            d.vbirth = half
+            d.growth = a.uniform(gm * (1.0 - gs), gm * (1.0 + gs))
+            d.lf = a.id
+            d.u[0] = a.old.u[0] + 1.0
+            a.add(d)
+        a.remove()
+    return body
+
+
+# three @addAgent calls: one more than a rule may make per agent and step
+THREE_ADDS = """
+struct UserRule {
+  __device__ UserRule(const double*, const double*) {}
+  __device__ void operator()(RuleAgent& a) const {
+    if (a.id == 1) { a.add(a.child()); a.add(a.child()); a.add(a.child()); }
+  }
+};
+"""
