@@ -63,7 +63,7 @@ SIGNATURES = {
     "cbm_last_error": (C.c_char_p, [_P]),
     "cbm_version": (C.c_char_p, []),
     "cbm_compile": (C.c_int, [_P, C.c_char_p]),
-    "cbm_compile_model": (C.c_int, [_P, C.c_char_p, C.c_char_p, C.c_char_p]),
+    "cbm_compile_model": (C.c_int, [_P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p]),
     "cbm_upload": (C.c_int, [_P, C.c_int32, _P, C.c_int64, C.c_int32]),
     "cbm_download": (C.c_int, [_P, C.c_int32, _P, C.c_int64, C.c_int32]),
     "cbm_set_n": (C.c_int, [_P, C.c_int32, C.c_int64]),

@@ -196,6 +196,9 @@ class ABM:
                 UserRule (optionally as (body, rule_p)), compiled with hipRTC
     agent       user agent parameters {name: float} (at most 4) for a UserRule
                 body: Community attributes, u[agent::name] in the body
+    medium      {"NMedium": [...], "D", "consume", "c0"} for the built-in
+                diffusion with uptake; "mediumODE": a HIP C++ body defining
+                struct UserMedium (dt(c) at a node) replaces that law
     lagged      dt(x) reads the previous step's committed force accumulator, the
                 reference's read-old/write-new idiom (agentStructure.jl:352-453)
     periodic    periodic box with minimum-image distances
@@ -391,11 +394,12 @@ class Context:
         """Compile a user interaction body (struct UserLaw, HIP C++) with hipRTC."""
         self._chk(self.lib.cbm_compile(self.h, source.encode()))
 
-    def compile_model(self, law=None, noise=None, rule=None):
-        """Compile user agentODE (struct UserLaw), agentSDE (struct UserNoise) and
-        agentRule (struct UserRule) bodies, all of the model's in one call."""
+    def compile_model(self, law=None, noise=None, rule=None, medium=None):
+        """Compile user agentODE (struct UserLaw), agentSDE (struct UserNoise),
+        agentRule (struct UserRule) and mediumODE (struct UserMedium) bodies, all of
+        the model's in one call."""
         enc = lambda s: s.encode() if s is not None else None
-        self._chk(self.lib.cbm_compile_model(self.h, enc(law), enc(noise), enc(rule)))
+        self._chk(self.lib.cbm_compile_model(self.h, enc(law), enc(noise), enc(rule), enc(medium)))
 
     def set_params(self, p, sim=0):
         v = np.ascontiguousarray(p, dtype=np.float64)
@@ -510,8 +514,9 @@ def loadToPlatform_(com, preallocateAgents=0):
     if abm.rule_src is not None:
         names = "".join(f"constexpr int {n} = {f - A.F_U0};\n" for n, f in abm.user_fields.items())
         rule_src = "namespace agent {\n" + names + "}\n" + abm.rule_src
-    if law_src is not None or noise_src is not None or rule_src is not None:
-        com.ctx.compile_model(law_src, noise_src, rule_src)
+    medium_src = abm.medium.get("mediumODE") if abm.medium is not None else None
+    if law_src is not None or noise_src is not None or rule_src is not None or medium_src is not None:
+        com.ctx.compile_model(law_src, noise_src, rule_src, medium_src)
     for k, arr in com.params.items():
         com.ctx.upload(com.fields[k], arr)
     com.ctx.neighbors()

@@ -144,6 +144,7 @@ struct cbm_ctx {
   hipFunction_t user_fn[2] = {nullptr, nullptr};   // [0] deterministic stages, [1] noisy (EM, EulerHeun)
   hipFunction_t user_brick[2] = {nullptr, nullptr};   // the same, cell-linked brick tiles
   hipFunction_t user_rule[2] = {nullptr, nullptr};    // agentRule body: decide, apply
+  hipFunction_t user_medium = nullptr;                // mediumODE body: the FTCS step
   int user_nb = NB_FULL;                  // neighbor walk the module was built for
   bool user_noise = false;                // the module holds a user agentSDE body
   double* gacc[MAX_STATE] = {};           // EulerHeun with a user agentSDE body: g of stage 1
@@ -937,7 +938,13 @@ static int medium_phase(cbm_ctx* c) {
   md.D = c->m.medium_D; md.uptake = c->m.medium_consume; md.dt = c->m.dt;
   md.nodes = c->med_nodes;
   HIPCHK(c, launch_medium_count(c->nblocks, c->block, c->stream, md));
-  HIPCHK(c, launch_medium_step(c->stream, md));
+  if (c->user_medium) {
+    void* params[] = {(void*)&md};
+    HIPCHK(c, hipModuleLaunchKernel(c->user_medium, (unsigned)((md.nodes + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream,
+                                    params, nullptr));
+  } else {
+    HIPCHK(c, launch_medium_step(c->stream, md));
+  }
   c->med_cur ^= 1;
   return CBM_OK;
 }
@@ -1076,7 +1083,8 @@ const char* cbm_last_error(const cbm_ctx* c) { return c ? c->err.c_str() : "null
 // UserNoise or UserRule, see include/cbm_hip.h) is compiled by hipRTC into
 // the same skeletons as the built-in laws and rules, instantiated for this
 // model's dimension, neighbor walk and boundary.
-int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, const char* rule_body) {
+int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, const char* rule_body,
+                      const char* medium_body) {
   if (!c) return CBM_ERR_ARG;
   const bool custom = c->m.law == CBM_LAW_CUSTOM;
   if (custom != (law_body != nullptr))
@@ -1085,13 +1093,18 @@ int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, 
     return fail(c, CBM_ERR_CONFIG, "a rule body is required with rule CBM_RULE_CUSTOM and only then");
   if (noise_body && c->m.integrator != CBM_EM && c->m.integrator != CBM_EULER_HEUN)
     return fail(c, CBM_ERR_CONFIG, "an agentSDE body needs the EM or EulerHeun integrator");
-  if (!law_body && !noise_body && !rule_body) return fail(c, CBM_ERR_ARG, "nothing to compile");
+  if (medium_body && !c->medium) return fail(c, CBM_ERR_CONFIG, "a mediumODE body needs a medium (medium_n)");
+  if (!law_body && !noise_body && !rule_body && !medium_body) return fail(c, CBM_ERR_ARG, "nothing to compile");
   const bool stages = law_body || noise_body;
-  std::string src = "#define CBM_HIPRTC 1\n#include \"cbm_rule.h\"\nnamespace cbm {\n";
+  std::string src = "#define CBM_HIPRTC 1\n#include \"cbm_rule.h\"\n#include \"cbm_medium.h\"\nnamespace cbm {\n";
   if (law_body) { src += law_body; src += "\n"; }
   if (noise_body) { src += noise_body; src += "\n"; }
   if (rule_body) { src += rule_body; src += "\n"; }
+  if (medium_body) { src += medium_body; src += "\n"; }
   src += "}  // namespace cbm\n";
+  if (medium_body)
+    src += "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_medium_step(cbm::MediumArgs m) {\n"
+           "  cbm::medium_step_body<cbm::UserMedium>(m);\n}\n";
   if (rule_body)
     src += "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_rule_decide(cbm::RuleArgs a) {\n"
            "  cbm::user_rule_decide<cbm::UserRule>(a);\n}\n"
@@ -1121,10 +1134,11 @@ int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, 
       src += inst;
     }
   }
-  const char* headers[] = {cbm_rtc_cbm_device_h, cbm_rtc_cbm_structs_h, cbm_rtc_cbm_stage_h, cbm_rtc_cbm_rule_h};
-  const char* names[] = {"cbm_device.h", "cbm_structs.h", "cbm_stage.h", "cbm_rule.h"};
+  const char* headers[] = {cbm_rtc_cbm_device_h, cbm_rtc_cbm_structs_h, cbm_rtc_cbm_stage_h, cbm_rtc_cbm_rule_h,
+                           cbm_rtc_cbm_medium_h};
+  const char* names[] = {"cbm_device.h", "cbm_structs.h", "cbm_stage.h", "cbm_rule.h", "cbm_medium.h"};
   hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, src.c_str(), "cbm_user_bodies.hip", 4, headers, names) != HIPRTC_SUCCESS)
+  if (hiprtcCreateProgram(&prog, src.c_str(), "cbm_user_bodies.hip", 5, headers, names) != HIPRTC_SUCCESS)
     return fail(c, CBM_ERR_COMPILE, "hiprtcCreateProgram failed");
   const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math"};
   hiprtcResult r = hiprtcCompileProgram(prog, 5, opts);
@@ -1143,7 +1157,9 @@ int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, 
   hiprtcDestroyProgram(&prog);
   if (c->user_mod) { (void)hipModuleUnload(c->user_mod); c->user_mod = nullptr; }
   for (int k = 0; k < 2; ++k) { c->user_fn[k] = nullptr; c->user_brick[k] = nullptr; c->user_rule[k] = nullptr; }
+  c->user_medium = nullptr;
   HIPCHK(c, hipModuleLoadData(&c->user_mod, code.data()));
+  if (medium_body) HIPCHK(c, hipModuleGetFunction(&c->user_medium, c->user_mod, "cbm_user_medium_step"));
   if (rule_body) {
     HIPCHK(c, hipModuleGetFunction(&c->user_rule[0], c->user_mod, "cbm_user_rule_decide"));
     HIPCHK(c, hipModuleGetFunction(&c->user_rule[1], c->user_mod, "cbm_user_rule_apply"));
@@ -1169,7 +1185,7 @@ int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, 
 
 int cbm_compile(cbm_ctx* c, const char* body) {
   if (!c || !body) return CBM_ERR_ARG;
-  return cbm_compile_model(c, body, nullptr, nullptr);
+  return cbm_compile_model(c, body, nullptr, nullptr, nullptr);
 }
 
 static int field_col(cbm_ctx* c, int32_t field, int& k) {

@@ -109,23 +109,6 @@ struct ExportArgs {
   int32_t pad;
 };
 
-// Medium: one field on a cell-centred grid over the box of simulation 0,
-// stepped by explicit FTCS with zero-flux (mirror) boundaries and agent
-// uptake (mediumODE with @∂2, src/AgentStructure/macros.jl:203-257).
-struct MediumArgs {
-  RowGeom g;
-  const double* pos[3];         // agent positions after the DE stages
-  const double* c;              // concentration (read)
-  double* cn;                   // concentration (write)
-  int32_t* count;               // agents per node, consumed (zeroed) by the stencil
-  int32_t n[3];                 // NMedium
-  int32_t dims;
-  double dx[3];                 // (hi - lo) / n per axis
-  double lo[3], hi[3];
-  double D, uptake, dt;
-  int64_t nodes;
-};
-
 // Adjoint of the overdamped-Euler mechanics step (no reference counterpart).
 // Rows are not permuted while it runs, so lambda keeps its row mapping.
 struct AdjointArgs {

@@ -21,6 +21,7 @@
 #include "cbm_internal.h"
 #include "cbm_device.h"
 #include "cbm_stage.h"
+#include "cbm_medium.h"
 
 namespace cbm {
 
@@ -933,25 +934,7 @@ __global__ __launch_bounds__(256) void k_medium_count(MediumArgs m) {
 
 // FTCS: c' = c + dt (D sum_a (c[+a] - 2c + c[-a])/dx_a^2 - (uptake c) n_agents),
 // ghost nodes mirroring the boundary node (zero flux, conserves mass).
-__global__ __launch_bounds__(256) void k_medium_step(MediumArgs m) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m.nodes) return;
-  const int64_t nx = m.n[0], ny = m.n[1];
-  const int64_t p[3] = {k % nx, (k / nx) % ny, k / (nx * ny)};
-  const int64_t stride[3] = {1, nx, nx * ny};
-  const double cc = m.c[k];
-  double lap = 0.0;
-  for (int a = 0; a < m.dims; ++a) {
-    const double cp = (p[a] + 1 < m.n[a]) ? m.c[k + stride[a]] : cc;
-    const double cm = (p[a] > 0) ? m.c[k - stride[a]] : cc;
-    const double l = ((cp - 2 * cc) + cm) / (m.dx[a] * m.dx[a]);
-    lap = (a == 0) ? l : lap + l;
-  }
-  const int32_t na = m.count[k];
-  m.count[k] = 0;
-  const double f = m.D * lap - (m.uptake * cc) * (double)na;
-  m.cn[k] = cc + m.dt * f;
-}
+__global__ __launch_bounds__(256) void k_medium_step(MediumArgs m) { medium_step_body<BuiltinMedium>(m); }
 
 // N' = N + NAdd_ - NRemove_ and idMax_ += NAdd_ (update.jl:204-214), after
 // every compaction kernel has read the old counts; flags the neighbor

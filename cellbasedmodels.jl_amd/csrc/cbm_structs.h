@@ -212,6 +212,25 @@ struct RuleArgs {
   int32_t* rebuild_flag;        // set when agents were added or removed
 };
 
+// Medium: one field on a cell-centred grid over the box of simulation 0,
+// stepped by explicit FTCS with zero-flux (mirror) boundaries; dt(c) is the
+// built-in diffusion with agent uptake or a user mediumODE body (@∂2,
+// src/AgentStructure/macros.jl:203-257).  Also the argument of the hipRTC
+// user medium kernel.
+struct MediumArgs {
+  RowGeom g;
+  const double* pos[3];         // agent positions after the DE stages
+  const double* c;              // concentration (read)
+  double* cn;                   // concentration (write)
+  int32_t* count;               // agents per node, consumed (zeroed) by the stencil
+  int32_t n[3];                 // NMedium
+  int32_t dims;
+  double dx[3];                 // (hi - lo) / n per axis
+  double lo[3], hi[3];
+  double D, uptake, dt;
+  int64_t nodes;
+};
+
 }  // namespace cbm
 
 #endif  // CBM_STRUCTS_H

@@ -231,10 +231,23 @@ int cbm_compile(cbm_ctx* ctx, const char* pair_body_hip);
  * (src/random.jl:1-23) on Philox(seed, step, id, sim, 0x600 + k), k counting
  * the agent's draws in the step.  New agents get ids idMax + 1 + (prefix of
  * additions in slot order), the reference's serial append order.  The body
- * must not read other agents.  Each call replaces the previous module, so
- * pass every body of the model in one call. */
+ * must not read other agents.
+ * medium_body (mediumODE, functionDE.jl:13-116 with the medium loop of
+ * auxiliar.jl:168-172,184-210 and @∂2 of macros.jl:203-257; models with a
+ * medium, NULL = the built-in diffusion with uptake) is HIP C++ defining,
+ * inside namespace cbm:
+ *   struct UserMedium {
+ *     __device__ UserMedium(double D, double consume, const double* p);   // medium_D, medium_consume, p[8]
+ *     __device__ double operator()(const MediumNode& m) const;            // dt(c) at the node
+ *   };
+ * MediumNode (cbm_medium.h) holds c, lap[3] (@∂2 per axis with zero-flux
+ * mirror ghosts), laplacian(), the node centre x[3] and index idx[3], the
+ * number of agents in the node (cellInMesh, metrics.jl:18-20) and dt; the
+ * step is c' = c + dt * dt(c) for every node.
+ * Each call replaces the previous module, so pass every body of the model in
+ * one call. */
 int cbm_compile_model(cbm_ctx* ctx, const char* law_body, const char* noise_body,
-                      const char* rule_body);
+                      const char* rule_body, const char* medium_body);
 
 /* ------------------------------------------------------- data transfer */
 /* loadToPlatform! / bringFromPlatform! (communityStructure.jl:606-711).

@@ -149,6 +149,7 @@ struct Orc {
   std::vector<double> U[4];                   // user agent parameters (CBM_RULE_CUSTOM)
   std::vector<char> alive;
   void* rule_fn = nullptr;                    // CBM_RULE_CUSTOM body (orc_rule_fn)
+  void* medium_fn = nullptr;                  // user mediumODE body (orc_medium_fn), else built-in
   // neighbor structures
   Grid g;
   std::vector<int64_t> cellOf, cellStart, cellCount, cellAgents;
@@ -461,6 +462,15 @@ static void compute_neighbors(Orc& o, bool force) {
 }
 
 // --------------------------------------------------------------- medium
+// A user mediumODE body on the CPU: dt(c) at one node.
+struct OrcMediumNode {
+  double c, lap[3], x[3];
+  int64_t idx[3];
+  int32_t agents, dims;
+  double dt;
+};
+typedef double (*orc_medium_fn)(const OrcMediumNode* node);
+
 static void medium_step(Orc& o, int64_t nAgents) {
   int64_t nn = o.mn[0] * o.mn[1] * o.mn[2];
   o.nodeCount.assign(nn, 0);
@@ -480,6 +490,26 @@ static void medium_step(Orc& o, int64_t nAgents) {
   }
   o.cnew.resize(nn);
   const double D = o.m.medium_D, dt = o.m.dt, u = o.m.medium_consume;
+  if (o.medium_fn) {   // a user body (test code's restatement), serially
+    orc_medium_fn fn = (orc_medium_fn)o.medium_fn;
+    for (int64_t k = 0; k < nn; ++k) {
+      OrcMediumNode nd;
+      int64_t p[3] = {k % o.mn[0], (k / o.mn[0]) % o.mn[1], k / (o.mn[0] * o.mn[1])};
+      int64_t stride[3] = {1, o.mn[0], o.mn[0] * o.mn[1]};
+      nd.c = o.c[k]; nd.agents = o.nodeCount[k]; nd.dims = o.dims; nd.dt = dt;
+      for (int a = 0; a < 3; ++a) {
+        nd.idx[a] = p[a]; nd.lap[a] = 0.0; nd.x[a] = 0.0;
+        if (a >= o.dims) continue;
+        double cp = (p[a] + 1 < o.mn[a]) ? o.c[k + stride[a]] : nd.c;
+        double cm = (p[a] > 0) ? o.c[k - stride[a]] : nd.c;
+        nd.lap[a] = ((cp - 2 * nd.c) + cm) / (o.mdx[a] * o.mdx[a]);
+        nd.x[a] = o.s.box_lo[a] + ((double)p[a] + 0.5) * o.mdx[a];
+      }
+      o.cnew[k] = nd.c + dt * fn(&nd);
+    }
+    o.c.swap(o.cnew);
+    return;
+  }
 #pragma omp parallel for schedule(static)
   for (int64_t k = 0; k < nn; ++k) {
     int64_t p[3] = {k % o.mn[0], (k / o.mn[0]) % o.mn[1], k / (o.mn[0] * o.mn[1])};
@@ -734,11 +764,11 @@ static void de_step(Orc& o, int64_t nA) {
   if (o.growth) std::copy(U[d].begin(), U[d].begin() + nA, o.vol.begin());
 }
 
-static void wrap_positions(Orc& o) {
+static void wrap_positions(Orc& o, int64_t n) {
   if (!o.m.periodic) return;
   for (int a = 0; a < o.dims; ++a) {
     double lo = o.s.box_lo[a], L = o.s.box_hi[a] - o.s.box_lo[a];
-    for (int64_t i = 0; i < o.N; ++i) {
+    for (int64_t i = 0; i < n; ++i) {
       double x = o.X[a][i];
       if (x >= lo + L) x = x - L; else if (x < lo) x = x + L;
       o.X[a][i] = x;
@@ -751,6 +781,7 @@ static void one_step(Orc& o) {
   apply_rule(o, holes, nAdd);                 // agentStepRule!
   int64_t nA = o.N;                            // DE loop runs over 1:N
   de_step(o, nA);                              // agentStepDE!
+  wrap_positions(o, nA);                       // periodic wrap after the final stage: the medium sees it
   if (o.m.medium_n[0] > 0) medium_step(o, nA); // mediumStepDE!
   // update!: commit the write-new buffers, then compact
   for (int a = 0; a < o.dims; ++a) for (int64_t i = 0; i < nA; ++i) o.F[a][i] = o.Fnew[a][i];
@@ -759,7 +790,7 @@ static void one_step(Orc& o) {
   compact(o, holes, nAdd);
   o.t = o.t + o.m.dt;
   o.step++;
-  wrap_positions(o);
+  wrap_positions(o, o.N);                      // new agents placed outside the box
   compute_neighbors(o, changed);               // computeNeighbors!
 }
 
@@ -983,6 +1014,8 @@ void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
 // CBM_RULE_CUSTOM: the rule body, called once per agent and step (test code
 // passes a restatement of the device body it checks).
 void orc_set_rule(void* h, void* fn) { ((Orc*)h)->rule_fn = fn; }
+// A user mediumODE body in place of the built-in diffusion with uptake.
+void orc_set_medium_body(void* h, void* fn) { ((Orc*)h)->medium_fn = fn; }
 
 void orc_normals(uint64_t seed, int64_t step, int64_t id, int32_t sim, uint32_t purpose, double* z2) {
   U4 r = draw(seed, step, id, sim, purpose);

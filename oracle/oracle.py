@@ -51,6 +51,7 @@ def lib():
             "orc_set_threads": (None, [C.c_int]),
             "orc_set_noise_slope": (None, [P, C.c_double]),
             "orc_set_rule": (None, [P, P]),
+            "orc_set_medium_body": (None, [P, P]),
             "orc_traj_loss": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P]),
             "orc_traj_grad_dual": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P, P]),
         }
@@ -74,6 +75,23 @@ class RuleRec(C.Structure):
 RULE_FN = C.CFUNCTYPE(None, C.c_int64, C.c_int64, C.POINTER(RuleRec), C.POINTER(RuleRec), C.POINTER(RuleRec),
                       C.POINTER(C.c_int32), C.POINTER(C.c_int32))
 RULE_PURPOSE = 0x600
+
+
+class MediumNode(C.Structure):
+    """What a mediumODE body sees at one node (OrcMediumNode)."""
+    _fields_ = [("c", C.c_double), ("lap", C.c_double * 3), ("x", C.c_double * 3), ("idx", C.c_int64 * 3),
+                ("agents", C.c_int32), ("dims", C.c_int32), ("dt", C.c_double)]
+
+    def laplacian(self):
+        lap = self.lap[0]
+        if self.dims > 1:
+            lap = lap + self.lap[1]
+        if self.dims > 2:
+            lap = lap + self.lap[2]
+        return lap
+
+
+MEDIUM_FN = C.CFUNCTYPE(C.c_double, C.POINTER(MediumNode))
 
 
 def u53(a, b):
@@ -178,6 +196,11 @@ class Oracle:
 
         self._rule_cb = RULE_FN(cb)
         self.lib.orc_set_rule(self.h, C.cast(self._rule_cb, C.c_void_p))
+
+    def set_medium_body(self, body):
+        """A user mediumODE body: body(node) returns dt(c) at a MediumNode."""
+        self._medium_cb = MEDIUM_FN(lambda nd: float(body(nd.contents)))
+        self.lib.orc_set_medium_body(self.h, C.cast(self._medium_cb, C.c_void_p))
 
     def set_noise_slope(self, slope):
         """State-dependent agentSDE term g_a(x) = noise + slope * x_a on the positions."""
