@@ -863,6 +863,7 @@ static RuleArgs rule_args(cbm_ctx* c, int32_t* flag) {
   std::memset(&r, 0, sizeof(r));
   r.g = geom(c);
   r.rule = c->m.rule; r.dims = c->dims; r.lagged = c->lagged ? 1 : 0; r.growth = c->growth ? 1 : 0;
+  r.periodic = c->m.periodic ? 1 : 0;
   for (int q = 0; q < 4; ++q) r.rp[q] = c->m.rule_p[q];
   r.dt = c->m.dt; r.seed = c->m.seed; r.step_ptr = c->counters + 1;
   for (int d = 0; d < c->dims; ++d) { r.pos[d] = (double*)c->col[C_X + d].cur(); r.pos_alt[d] = (double*)c->col[C_X + d].alt(); }

@@ -698,7 +698,8 @@ __global__ __launch_bounds__(256) void k_rule_apply(RuleArgs a) {
     const double g2 = gm * (1.0 + gs * (2.0 * u53(rg.z, rg.w) - 1.0));
     for (int c = 0; c < D; ++c) {
       const double x = a.pos[c][r];
-      const double xn = x + split * ax[c];
+      double xn = x + split * ax[c];
+      if (a.periodic) xn = wrap_box(xn, sd.lo[c], sd.L[c]);
       a.pos[c][dst] = xn;
       a.pos_alt[c][dst] = xn;
       a.pos[c][r] = x - split * ax[c];

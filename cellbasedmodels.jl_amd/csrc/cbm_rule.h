@@ -162,8 +162,9 @@ __device__ void user_rule_apply(const RuleArgs& a) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       if (c >= D) break;
-      a.pos[c][dst] = d.x[c];
-      a.pos_alt[c][dst] = d.x[c];
+      const double x = a.periodic ? wrap_box(d.x[c], sd.lo[c], sd.L[c]) : d.x[c];
+      a.pos[c][dst] = x;
+      a.pos_alt[c][dst] = x;
       if (a.lagged) { a.F[c][dst] = a.F[c][r]; a.Fnew[c][dst] = a.F[c][r]; }
     }
     a.vol[dst] = d.vol; a.vol_alt[dst] = d.vol;

@@ -81,6 +81,13 @@ __device__ __forceinline__ void for_candidates(const StageArgs& a, const SimDev&
   }
 }
 
+// Periodic box: a coordinate brought back into [lo, lo + L) (one period at most).
+__device__ __forceinline__ double wrap_box(double x, double lo, double L) {
+  if (x >= lo + L) return x - L;
+  if (x < lo) return x + L;
+  return x;
+}
+
 // ----------------------------------------------------------- stage kernel
 // Minimum image (periodic boxes): the same comparisons as the oracle, as selects.
 __device__ __forceinline__ double wrap_dx(double d, double L, double hL) {
@@ -339,12 +346,7 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
                            a.kind == ST_RK4_4 || a.kind == ST_EH2;
   if (final_stage && a.periodic) {
 #pragma unroll
-    for (int c = 0; c < D; ++c) {
-      double x = res[c];
-      if (x >= sd.lo[c] + sd.L[c]) x = x - sd.L[c];
-      else if (x < sd.lo[c]) x = x + sd.L[c];
-      res[c] = x;
-    }
+    for (int c = 0; c < D; ++c) res[c] = wrap_box(res[c], sd.lo[c], sd.L[c]);
   }
   for (int c = 0; c < nv; ++c) a.out[c][i] = res[c];
   if (disp2 >= 0.0) {

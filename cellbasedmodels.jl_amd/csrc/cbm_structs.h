@@ -184,6 +184,7 @@ struct StageArgs {
 struct RuleArgs {
   RowGeom g;
   int32_t rule, dims, lagged, growth;
+  int32_t periodic, pad;        // periodic box: new agents' positions are wrapped into it
   double rp[4];
   double dt;
   uint64_t seed;

@@ -126,3 +126,29 @@ def test_rule_scan_longer_than_bucket_table(gpu_available):
     g.step(40)
     same_slots(o, g, 2, 1e-9)
     assert o.n > n0
+
+
+@pytest.mark.parametrize("nb", [A.CELL_LINKED, A.VERLET_DISPLACEMENT])
+def test_periodic_division_across_the_boundary(gpu_available, nb):
+    """Agents dividing next to the faces of a periodic box: a new agent placed
+    past a face is wrapped into the box, as the oracle's end-of-step wrap does."""
+    n0, L = 300, 30.0
+    rng = np.random.default_rng(12)
+    pos = [rng.uniform(0.0, L, n0), rng.uniform(0.0, L, n0)]
+    pos[0][:100] = rng.uniform(0.0, 0.4, 100)        # against the x faces
+    pos[1][100:200] = rng.uniform(L - 0.4, L, 100)   # against the y faces
+    kw = dict(cell_edge=2.0) if nb == A.CELL_LINKED else dict(skin=2.6, max_neighbors=256, rebuild_disp=0.3)
+    m = model(2, integrator=A.EULER, neighbors=nb, law=A.LAW_LINEAR, dt=0.05, periodic=1,
+              rule=A.RULE_GROWTH_DIVISION, rule_p=(0.5, 0.1, 0.5, 2.0), seed=21, **kw)
+    fields = {A.F_X: pos[0], A.F_Y: pos[1], A.F_VOL: rng.uniform(1.9, 2.0, n0), A.F_VBIRTH: np.ones(n0),
+              A.F_GROWTH: rng.uniform(0.45, 0.55, n0)}
+    o, g = pair(m, sim(n0, [[0.0, L], [0.0, L]], [1.0, 2.0], n_max=4000), fields)
+    for _ in range(4):
+        o.step(5)
+        g.step(5)
+        n = o.n
+        same_slots(o, g, 2, 1e-9)
+        for a in range(2):
+            x = g.download(a, n)
+            assert x.min() >= 0.0 and x.max() < L
+    assert o.n > 1.5 * n0
