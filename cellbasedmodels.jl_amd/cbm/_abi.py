@@ -69,6 +69,13 @@ SIGNATURES = {
     "cbm_set_n": (C.c_int, [_P, C.c_int32, C.c_int64]),
     "cbm_upload_medium": (C.c_int, [_P, _P, C.c_int64]),
     "cbm_download_medium": (C.c_int, [_P, _P, C.c_int64]),
+    "cbm_snapshot_config": (C.c_int, [_P, _P, C.c_int32, C.c_int32]),
+    "cbm_snapshot_take": (C.c_int, [_P]),
+    "cbm_step_save": (C.c_int, [_P, C.c_int64, C.c_int64]),
+    "cbm_snapshot_count": (C.c_int, [_P, C.POINTER(C.c_int32)]),
+    "cbm_snapshot_get": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _P, C.c_int64, C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_double)]),
+    "cbm_snapshot_get_medium": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     "cbm_neighbors": (C.c_int, [_P]),
     "cbm_step": (C.c_int, [_P, C.c_int64]),
     "cbm_step_graph": (C.c_int, [_P, C.c_int64, C.c_int64]),

@@ -401,6 +401,36 @@ class Context:
         enc = lambda s: s.encode() if s is not None else None
         self._chk(self.lib.cbm_compile_model(self.h, enc(law), enc(noise), enc(rule), enc(medium)))
 
+    # evolve!(saveEach) snapshots kept on the device (cbm_snapshot_*)
+    def snapshot_config(self, fields, capacity):
+        f = np.ascontiguousarray(fields, dtype=np.int32)
+        self._chk(self.lib.cbm_snapshot_config(self.h, f.ctypes.data, f.size, int(capacity)))
+
+    def snapshot_take(self):
+        self._chk(self.lib.cbm_snapshot_take(self.h))
+
+    def step_save(self, n, save_each):
+        self._chk(self.lib.cbm_step_save(self.h, int(n), int(save_each)))
+
+    def snapshot_count(self):
+        k = C.c_int32()
+        self._chk(self.lib.cbm_snapshot_count(self.h, C.byref(k)))
+        return k.value
+
+    def snapshot(self, k, field, sim=0):
+        """(values in slot order, N, t) of snapshot k."""
+        cap = int(self.sims[sim].n_max)
+        v = np.empty(max(cap, 1), dtype=np.int64 if field in A.INT_FIELDS else np.float64)
+        n, t = C.c_int64(), C.c_double()
+        self._chk(self.lib.cbm_snapshot_get(self.h, int(k), int(field), int(sim), v.ctypes.data, cap,
+                                            C.byref(n), C.byref(t)))
+        return v[: n.value].copy(), n.value, t.value
+
+    def snapshot_medium(self, k, nodes):
+        v = np.empty(int(nodes))
+        self._chk(self.lib.cbm_snapshot_get_medium(self.h, int(k), v.ctypes.data, v.size))
+        return v
+
     def set_params(self, p, sim=0):
         v = np.ascontiguousarray(p, dtype=np.float64)
         self._chk(self.lib.cbm_set_params(self.h, sim, v.ctypes.data, v.size))
@@ -545,21 +575,35 @@ def step_(com, n=1):
     com.t, _ = com.ctx.time()
 
 
-def evolve_(com, steps, saveEach=None, preallocateAgents=0):
-    """evolve! (step.jl:124-155): loads if needed, runs `steps` steps, snapshots every
-    saveEach steps into com.pastTimes (the reference's saveRAM!)."""
+def evolve_(com, steps, saveEach=None, preallocateAgents=0, saveCurrentState=False):
+    """evolve! (step.jl:124-155): loads if needed, runs `steps` steps and, after
+    every step i with i % saveEach == 0, saves the community into com.pastTimes
+    (the reference's saveRAM!, IO.jl:6-31).  The snapshots are taken on the
+    device while the steps run and read back at the end."""
     if com.ctx is None:
         loadToPlatform_(com, preallocateAgents)
-    if not saveEach:
+    ctx = com.ctx
+    if not saveEach and not saveCurrentState:
         step_(com, steps)
     else:
-        done = 0
-        while done < steps:
-            k = min(saveEach, steps - done)
-            step_(com, k)
-            done += k
-            bringFromPlatform_(com)
-            com.pastTimes.append({"t": com.t, **{k2: v.copy() for k2, v in com.params.items()}})
+        names = list(com.params)
+        every = int(saveEach) if saveEach else steps + 1
+        ctx.snapshot_config([com.fields[k] for k in names], steps // every + (1 if saveCurrentState else 0))
+        if saveCurrentState:
+            ctx.snapshot_take()
+        ctx.step_save(steps, every)
+        com.t, _ = ctx.time()
+        med = com.abm.medium is not None
+        nodes = int(np.prod(com.abm.medium["NMedium"])) if med else 0
+        for k in range(ctx.snapshot_count()):
+            snap = {}
+            for name in names:
+                snap[name], n, t = ctx.snapshot(k, com.fields[name])
+            snap["t"], snap["N"] = t, n
+            if med:
+                snap["medium"] = ctx.snapshot_medium(k, nodes)
+            com.pastTimes.append(snap)
+        ctx.snapshot_config([], 0)
     bringFromPlatform_(com)
     return com
 

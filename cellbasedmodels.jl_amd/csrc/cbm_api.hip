@@ -123,6 +123,14 @@ struct cbm_ctx {
 
   // staging for transfers
   void* d_staging = nullptr;
+  // evolve!(saveEach) snapshots on the device: per field capacity x C values
+  // at [k * C + row_off + slot], N per simulation, the medium, t on the host
+  std::vector<int32_t> snap_fields;
+  std::vector<void*> snap_buf;
+  int64_t* snap_cnt = nullptr;
+  double* snap_med = nullptr;
+  int32_t snap_cap = 0, snap_count = 0;
+  std::vector<double> snap_t;
   int64_t staging_bytes = 0;
 
   // time
@@ -448,6 +456,8 @@ static void destroy_ctx(cbm_ctx* c) {
                              &c->adj_gpart, &c->adj_lpart}) F(b->p);
   for (int32_t* p : c->snap_l) F(p);
   for (int32_t* p : c->snap_n) F(p);
+  for (void* p : c->snap_buf) F(p);
+  F(c->snap_cnt); F(c->snap_med);
   F(c->d_staging); F(c->d_block_sim); F(c->d_block_off); F(c->d_sims); F(c->d_st);
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1403,6 +1413,121 @@ int cbm_get_bricks(cbm_ctx* c, int32_t* bdim, int32_t* cap, int64_t* nbricks, in
   *cap = tiled ? c->brick_cap : 0;
   *nbricks = tiled ? c->nbricks : 0;
   *overflows = v[2];
+  return CBM_OK;
+}
+
+// ------------------------------------------------------------- snapshots
+static void snapshot_free(cbm_ctx* c) {
+  for (void* p : c->snap_buf) if (p) (void)hipFree(p);
+  c->snap_buf.clear();
+  c->snap_fields.clear();
+  if (c->snap_cnt) { (void)hipFree(c->snap_cnt); c->snap_cnt = nullptr; }
+  if (c->snap_med) { (void)hipFree(c->snap_med); c->snap_med = nullptr; }
+  c->snap_cap = 0; c->snap_count = 0;
+  c->snap_t.clear();
+}
+
+int cbm_snapshot_config(cbm_ctx* c, const int32_t* fields, int32_t nfields, int32_t capacity) {
+  if (!c || nfields < 0 || capacity < 0 || (nfields > 0 && !fields)) return CBM_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  snapshot_free(c);
+  for (int32_t q = 0; q < nfields; ++q) {
+    int k;
+    if (field_col(c, fields[q], k)) return fail(c, CBM_ERR_ARG, "snapshot field not present in this model");
+    if (c->col[k].bytes != 8) return fail(c, CBM_ERR_ARG, "snapshot fields are 8-byte columns");
+  }
+  if (capacity == 0) return CBM_OK;
+  for (int32_t q = 0; q < nfields; ++q) {
+    void* p = nullptr;
+    HIPCHK(c, hipMalloc(&p, (size_t)capacity * c->C * 8));
+    c->snap_buf.push_back(p);
+    c->snap_fields.push_back(fields[q]);
+  }
+  HIPCHK(c, dalloc(&c->snap_cnt, (int64_t)capacity * c->S));
+  if (c->medium) HIPCHK(c, dalloc(&c->snap_med, (int64_t)capacity * c->med_nodes));
+  c->snap_cap = capacity;
+  return CBM_OK;
+}
+
+// saveRAM!: queues the gathers of the current state as snapshot snap_count.
+static int snapshot_take(cbm_ctx* c) {
+  if (c->snap_count >= c->snap_cap) return fail(c, CBM_ERR_CAPACITY, "snapshot capacity exhausted");
+  const int64_t k = c->snap_count;
+  SnapArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.g = geom(c);
+  a.slot = (const int32_t*)c->col[C_SLOT].cur();
+  a.n_out = c->snap_cnt + k * c->S;
+  for (size_t q = 0; q < c->snap_fields.size(); ++q) {
+    int col;
+    (void)field_col(c, c->snap_fields[q], col);
+    a.field = (const uint64_t*)c->col[col].cur();
+    a.out = (uint64_t*)c->snap_buf[q] + k * c->C;
+    HIPCHK(c, launch_snapshot(c->nblocks, c->block, c->stream, a));
+    a.n_out = nullptr;
+  }
+  if (c->snap_fields.empty()) HIPCHK(c, launch_snapshot(c->nblocks, c->block, c->stream, a));   // N only
+  if (c->medium)
+    HIPCHK(c, hipMemcpyAsync(c->snap_med + k * c->med_nodes, c->med_c[c->med_cur], sizeof(double) * c->med_nodes,
+                             hipMemcpyDeviceToDevice, c->stream));
+  c->snap_t.push_back(c->t);
+  c->snap_count++;
+  return CBM_OK;
+}
+
+int cbm_snapshot_take(cbm_ctx* c) {
+  if (!c) return CBM_ERR_ARG;
+  return snapshot_take(c);
+}
+
+int cbm_step_save(cbm_ctx* c, int64_t nsteps, int64_t save_each) {
+  if (!c || nsteps < 0 || save_each < 1) return CBM_ERR_ARG;
+  if (c->snap_count + nsteps / save_each > c->snap_cap)
+    return fail(c, CBM_ERR_CAPACITY, "more snapshots than cbm_snapshot_config's capacity");
+  for (int64_t i = 1; i <= nsteps; ++i) {
+    int rc = cbm_step(c, 1);
+    if (rc) return rc;
+    if (i % save_each == 0 && (rc = snapshot_take(c))) return rc;
+  }
+  return CBM_OK;
+}
+
+int cbm_snapshot_count(cbm_ctx* c, int32_t* count) {
+  if (!c || !count) return CBM_ERR_ARG;
+  *count = c->snap_count;
+  return CBM_OK;
+}
+
+int cbm_snapshot_get(cbm_ctx* c, int32_t k, int32_t field, int32_t sim, void* host, int64_t n_cap, int64_t* n_out,
+                     double* t_out) {
+  if (!c) return CBM_ERR_ARG;
+  if (k < 0 || k >= c->snap_count) return fail(c, CBM_ERR_ARG, "snapshot index out of range");
+  if (sim < 0 || sim >= c->S) return fail(c, CBM_ERR_ARG, "simulation index out of range");
+  int q = -1;
+  for (size_t f = 0; f < c->snap_fields.size(); ++f) if (c->snap_fields[f] == field) q = (int)f;
+  if (q < 0) return fail(c, CBM_ERR_ARG, "field not in the snapshot configuration");
+  int64_t n = 0;
+  HIPCHK(c, hipMemcpyAsync(&n, c->snap_cnt + (int64_t)k * c->S + sim, sizeof(n), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (n_out) *n_out = n;
+  if (t_out) *t_out = c->snap_t[k];
+  if (n > n_cap) return fail(c, CBM_ERR_ARG, "host buffer smaller than the snapshot's N");
+  if (n > 0) {
+    const uint64_t* src = (const uint64_t*)c->snap_buf[q] + (int64_t)k * c->C + c->h_sims[sim].row_off;
+    HIPCHK(c, hipMemcpyAsync(host, src, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return check_flags(c);
+}
+
+int cbm_snapshot_get_medium(cbm_ctx* c, int32_t k, double* host, int64_t n) {
+  if (!c) return CBM_ERR_ARG;
+  if (!c->medium) return fail(c, CBM_ERR_CONFIG, "model has no medium");
+  if (k < 0 || k >= c->snap_count) return fail(c, CBM_ERR_ARG, "snapshot index out of range");
+  if (!host || n != c->med_nodes) return fail(c, CBM_ERR_ARG, "medium snapshot needs prod(NMedium) values");
+  HIPCHK(c, hipMemcpyAsync(host, c->snap_med + (int64_t)k * c->med_nodes, sizeof(double) * n, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return CBM_OK;
 }
 

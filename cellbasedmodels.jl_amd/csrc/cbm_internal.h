@@ -95,6 +95,16 @@ struct XferArgs {
   int32_t pad;
 };
 
+// One field of every simulation gathered into slot order at
+// out[row_off + slot] (evolve!(saveEach) snapshots); n_out[sim] = N.
+struct SnapArgs {
+  RowGeom g;
+  const int32_t* slot;
+  const uint64_t* field;
+  uint64_t* out;        // NULL: the counts only
+  int64_t* n_out;       // may be NULL
+};
+
 struct ExportArgs {
   StageArgs s;
   const int32_t* slot;
@@ -179,6 +189,7 @@ hipError_t launch_verlet_brick(int dims, int periodic, int64_t nbricks, int bloc
 hipError_t launch_set_flag(hipStream_t s, int32_t* flag, int32_t v);
 hipError_t launch_tick(hipStream_t s, int64_t* counter);
 hipError_t launch_xfer(int nblocks, int block, hipStream_t s, const XferArgs& a);
+hipError_t launch_snapshot(int nblocks, int block, hipStream_t s, const SnapArgs& a);
 hipError_t launch_export(int dims, int nb, int nblocks, int block, hipStream_t s, const ExportArgs& e);
 
 }  // namespace cbm

@@ -972,6 +972,16 @@ __global__ __launch_bounds__(256) void k_xfer(XferArgs a) {
   }
 }
 
+// saveRAM! (src/CommunityStructure/IO.jl:6-31) on the device: the committed
+// agents' values of one field in slot order, every simulation at once.
+__global__ __launch_bounds__(256) void k_snapshot(SnapArgs a) {
+  const RowIdx ri = row_index(a.g);
+  const SimState& st = a.g.st[ri.sim];
+  if (a.n_out && ri.local == 0) a.n_out[ri.sim] = st.N;
+  if (!a.out || ri.local >= st.N) return;
+  a.out[a.g.sims[ri.sim].row_off + a.slot[ri.row]] = a.field[ri.row];
+}
+
 // Neighbor-set export (mode 0: lists in slots, mode 1: counts only) for the
 // sorted-set parity checks and the neighbor-count agentRule of the
 // reference KAT (test/testCommunity.jl:361-415).
@@ -1197,6 +1207,11 @@ hipError_t launch_tick(hipStream_t s, int64_t* counter) {
 
 hipError_t launch_xfer(int nblocks, int block, hipStream_t s, const XferArgs& a) {
   hipLaunchKernelGGL(k_xfer, dim3(nblocks), dim3(block), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_snapshot(int nblocks, int block, hipStream_t s, const SnapArgs& a) {
+  hipLaunchKernelGGL(k_snapshot, dim3(nblocks), dim3(block), 0, s, a);
   return hipGetLastError();
 }
 

@@ -249,6 +249,29 @@ int cbm_compile(cbm_ctx* ctx, const char* pair_body_hip);
 int cbm_compile_model(cbm_ctx* ctx, const char* law_body, const char* noise_body,
                       const char* rule_body, const char* medium_body);
 
+/* ------------------------------------------------------------- snapshots */
+/* evolve!(saveEach) with saveRAM! (src/CommunityStructure/step.jl:124-155,
+ * IO.jl:6-31) kept on the device: the gathers of each snapshot are queued on
+ * the context's stream and nothing synchronises the host until a snapshot is
+ * read back.
+ * cbm_snapshot_config: room for `capacity` snapshots of the listed fields
+ *   (8-byte fields only; every simulation; plus the medium when the model has
+ *   one).  Replaces an earlier configuration and discards its snapshots.
+ * cbm_snapshot_take: a snapshot of the current state (saveCurrentState).
+ * cbm_step_save: nsteps steps as cbm_step, with a snapshot after step i for
+ *   every i % save_each == 0 (i = 1..nsteps).  CBM_ERR_CAPACITY before any
+ *   step if the snapshots would not fit.
+ * cbm_snapshot_get: snapshot k's values of `field` for simulation `sim` in
+ *   slot order (its N values; *n_out = N, *t_out = t at the snapshot);
+ *   CBM_ERR_ARG if N > n_cap. */
+int cbm_snapshot_config(cbm_ctx* ctx, const int32_t* fields, int32_t nfields, int32_t capacity);
+int cbm_snapshot_take(cbm_ctx* ctx);
+int cbm_step_save(cbm_ctx* ctx, int64_t nsteps, int64_t save_each);
+int cbm_snapshot_count(cbm_ctx* ctx, int32_t* count);
+int cbm_snapshot_get(cbm_ctx* ctx, int32_t k, int32_t field, int32_t sim, void* host, int64_t n_cap,
+                     int64_t* n_out, double* t_out);
+int cbm_snapshot_get_medium(cbm_ctx* ctx, int32_t k, double* host, int64_t n);
+
 /* ------------------------------------------------------- data transfer */
 /* loadToPlatform! / bringFromPlatform! (communityStructure.jl:606-711).
  * `n` values in logical slot order for simulation `sim`.  Uploading CBM_F_X
