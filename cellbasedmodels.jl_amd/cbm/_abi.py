@@ -20,6 +20,7 @@ EULER, HEUN, RK4, EM, EULER_HEUN = 0, 1, 2, 3, 4
 FULL, CELL_LINKED, VERLET_DISPLACEMENT, VERLET_TIME, CLVD = 0, 1, 2, 3, 4
 LAW_NONE, LAW_LINEAR, LAW_REP_ADH, LAW_AGGREGATION, LAW_CUSTOM, LAW_SOFT_SPHERES = 0, 1, 2, 3, 4, 5
 RULE_NONE, RULE_GROWTH_DIVISION, RULE_PARITY, RULE_DEATH, RULE_CUSTOM = 0, 1, 2, 3, 4
+MEDIUM_FTCS, MEDIUM_DGADI = 0, 1
 
 F_X, F_Y, F_Z, F_ID, F_FX, F_FY, F_FZ, F_VOL, F_GROWTH, F_VBIRTH, F_LF = range(11)
 F_VX, F_VY, F_VZ, F_RADIUS, F_MASS, F_FATE = range(11, 17)
@@ -34,7 +35,7 @@ class ModelDesc(C.Structure):
         ("dt", C.c_double), ("cell_edge", C.c_double), ("skin", C.c_double), ("rebuild_disp", C.c_double),
         ("dt_recompute", C.c_double), ("drift", C.c_double), ("noise", C.c_double),
         ("rule_p", C.c_double * 4), ("seed", C.c_uint64),
-        ("medium_n", C.c_int32 * 3), ("medium_pad", C.c_int32),
+        ("medium_n", C.c_int32 * 3), ("medium_alg", C.c_int32),
         ("medium_D", C.c_double), ("medium_consume", C.c_double), ("medium_c0", C.c_double),
     ]
 

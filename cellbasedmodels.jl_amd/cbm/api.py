@@ -198,7 +198,8 @@ class ABM:
                 body: Community attributes, u[agent::name] in the body
     medium      {"NMedium": [...], "D", "consume", "c0"} for the built-in
                 diffusion with uptake; "mediumODE": a HIP C++ body defining
-                struct UserMedium (dt(c) at a node) replaces that law
+                struct UserMedium (dt(c) at a node) replaces that law;
+                "mediumAlg": "FTCS" (default) or "DGADI" (Douglas-Gunn ADI)
     lagged      dt(x) reads the previous step's committed force accumulator, the
                 reference's read-old/write-new idiom (agentStructure.jl:352-453)
     periodic    periodic box with minimum-image distances
@@ -263,6 +264,7 @@ class ABM:
             m.medium_D = float(self.medium.get("D", 0.0))
             m.medium_consume = float(self.medium.get("consume", 0.0))
             m.medium_c0 = float(self.medium.get("c0", 0.0))
+            m.medium_alg = {"FTCS": A.MEDIUM_FTCS, "DGADI": A.MEDIUM_DGADI}[self.medium.get("mediumAlg", "FTCS")]
         return m
 
 

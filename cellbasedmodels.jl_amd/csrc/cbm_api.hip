@@ -120,6 +120,13 @@ struct cbm_ctx {
   int med_cur = 0;
   int32_t* med_count = nullptr;
   int64_t med_nodes = 0;
+  // DGADI: the initial field (held on the boundary), a sweep buffer, and per
+  // axis c*[i] and m[i] (n entries each) of the constant-diagonal Thomas solve
+  bool adi = false;
+  double* med_init = nullptr;
+  double* med_tmp = nullptr;
+  double* adi_tab[3] = {nullptr, nullptr, nullptr};
+  double adi_sub = 0.0, adi_main = 0.0, adi_sup = 0.0;
 
   // staging for transfers
   void* d_staging = nullptr;
@@ -223,6 +230,11 @@ static int validate(cbm_ctx* c) {
     for (int a = 0; a < m.dims; ++a)
       if (m.medium_n[a] < 1) return fail(c, CBM_ERR_CONFIG, "NMedium must be >= 1 on every axis");
     if (!(m.medium_D >= 0.0) || !(m.medium_consume >= 0.0)) return fail(c, CBM_ERR_CONFIG, "medium D and uptake must be >= 0");
+    if (m.medium_alg != CBM_MEDIUM_FTCS && m.medium_alg != CBM_MEDIUM_DGADI)
+      return fail(c, CBM_ERR_CONFIG, "unknown medium solver");
+    if (m.medium_alg == CBM_MEDIUM_DGADI)
+      for (int a = 0; a < m.dims; ++a)
+        if (m.medium_n[a] < 4) return fail(c, CBM_ERR_CONFIG, "DGADI needs NMedium >= 4 on every axis");
   }
   for (int s = 0; s < c->S; ++s) {
     const cbm_sim_desc& sd = c->sims[s];
@@ -450,6 +462,7 @@ static void destroy_ctx(cbm_ctx* c) {
   F(c->gr_new); F(c->vb_new); F(c->dead);
   F(c->n_add_slot); F(c->dead_slot); F(c->off_add); F(c->off_dead); F(c->holes);
   F(c->med_c[0]); F(c->med_c[1]); F(c->med_count);
+  F(c->med_init); F(c->med_tmp); F(c->adi_tab[0]); F(c->adi_tab[1]); F(c->adi_tab[2]);
   F(c->key); F(c->lin); F(c->gbucket); F(c->cell_range); F(c->count); F(c->cursor); F(c->start); F(c->tmp); F(c->cell_row);
   F(c->scan_status); F(c->scan_ticket); F(c->list); F(c->list_n); F(c->bricks); F(c->brick_skip); F(c->blk_flag); F(c->flags); F(c->counters);
   for (cbm_ctx::DevBuf* b : {&c->adj_tgt, &c->adj_tbase, &c->adj_invn, &c->adj_ck, &c->adj_seg, &c->adj_lam,
@@ -549,6 +562,33 @@ static int init_ctx(cbm_ctx* c) {
     HIPCHK(c, dalloc(&c->med_count, c->med_nodes));
     std::vector<double> c0((size_t)c->med_nodes, m.medium_c0);
     HIPCHK(c, hipMemcpy(c->med_c[0], c0.data(), sizeof(double) * c->med_nodes, hipMemcpyHostToDevice));
+    if (m.medium_alg == CBM_MEDIUM_DGADI) {
+      c->adi = true;
+      HIPCHK(c, dalloc(&c->med_init, c->med_nodes));
+      HIPCHK(c, dalloc(&c->med_tmp, c->med_nodes));
+      HIPCHK(c, hipMemcpy(c->med_init, c0.data(), sizeof(double) * c->med_nodes, hipMemcpyHostToDevice));
+      // diagonals -D r, 1 + 2 D r with r = dt / (dims dx^2) of the last axis
+      // for every sweep (integrators.jl:850, 875-877 and the sweeps' column dims)
+      const int la = c->dims - 1;
+      const double dxl = (c->sims[0].box_hi[la] - c->sims[0].box_lo[la]) / (double)m.medium_n[la];
+      const double r = m.dt / ((double)c->dims * (dxl * dxl));
+      c->adi_sub = -m.medium_D * r;
+      c->adi_main = 1 + 2 * m.medium_D * r;
+      c->adi_sup = -m.medium_D * r;
+      for (int a = 0; a < c->dims; ++a) {
+        const int n = m.medium_n[a];
+        std::vector<double> tab(2 * (size_t)n, 0.0);   // c*[i] at [i], m[i] at [n + i]
+        tab[1] = c->adi_sup / c->adi_main;
+        for (int i = 2; i <= n - 3; ++i) {
+          const double mi = 1.0 / (c->adi_main - c->adi_sub * tab[i - 1]);
+          tab[n + i] = mi;
+          tab[i] = c->adi_sup * mi;
+        }
+        tab[n + n - 2] = 1.0 / (c->adi_main - c->adi_sub * tab[n - 3]);
+        HIPCHK(c, dalloc(&c->adi_tab[a], 2 * (int64_t)n));
+        HIPCHK(c, hipMemcpy(c->adi_tab[a], tab.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+      }
+    }
   }
   // slot = local row, id = local row + 1 (id = 1:N, BASEPARAMETERS constants.jl:21)
   {
@@ -930,7 +970,38 @@ static int compact_phase(cbm_ctx* c, int32_t* flag) {
   return CBM_OK;
 }
 
-// mediumStepDE!: agents per node, then one FTCS step of the field.
+// DGADI sweeps after the reaction step (which wrote R = med_c[1 - cur]); the
+// last sweep lands in C = med_c[cur].  Every sweep writes the interior lines
+// of its destination, then the boundary nodes from the initial field.
+static int adi_sweeps(cbm_ctx* c) {
+  double* C = c->med_c[c->med_cur];
+  double* R = c->med_c[1 - c->med_cur];
+  double* T = c->med_tmp;
+  const double* src[3];
+  double* dst[3];
+  if (c->dims == 1) { src[0] = R; dst[0] = C; }
+  else if (c->dims == 2) { src[0] = R; dst[0] = T; src[1] = T; dst[1] = C; }
+  else { src[0] = R; dst[0] = T; src[1] = T; dst[1] = R; src[2] = R; dst[2] = C; }
+  int32_t n[3];
+  for (int a = 0; a < 3; ++a) n[a] = a < c->dims ? c->m.medium_n[a] : 1;
+  for (int ax = 0; ax < c->dims; ++ax) {
+    AdiArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.src = src[ax]; a.dst = dst[ax];
+    a.cst = c->adi_tab[ax]; a.mst = c->adi_tab[ax] + n[ax];
+    for (int b = 0; b < 3; ++b) a.n[b] = n[b];
+    a.dims = c->dims; a.axis = ax;
+    a.sub = c->adi_sub; a.main = c->adi_main; a.sup = c->adi_sup;
+    a.lines = 1;
+    for (int b = 0; b < c->dims; ++b) if (b != ax) a.lines *= n[b] - 2;
+    HIPCHK(c, launch_adi_sweep(c->stream, a));
+    HIPCHK(c, launch_adi_border(c->stream, dst[ax], c->med_init, n, c->dims, c->med_nodes));
+  }
+  return CBM_OK;
+}
+
+// mediumStepDE!: agents per node, then one FTCS step of the field (or the
+// reaction part and the ADI sweeps).
 static int medium_phase(cbm_ctx* c) {
   MediumArgs md;
   std::memset(&md, 0, sizeof(md));
@@ -948,6 +1019,7 @@ static int medium_phase(cbm_ctx* c) {
   }
   md.D = c->m.medium_D; md.uptake = c->m.medium_consume; md.dt = c->m.dt;
   md.nodes = c->med_nodes;
+  md.lap = c->adi ? 0 : 1;
   HIPCHK(c, launch_medium_count(c->nblocks, c->block, c->stream, md));
   if (c->user_medium) {
     void* params[] = {(void*)&md};
@@ -956,6 +1028,7 @@ static int medium_phase(cbm_ctx* c) {
   } else {
     HIPCHK(c, launch_medium_step(c->stream, md));
   }
+  if (c->adi) return adi_sweeps(c);
   c->med_cur ^= 1;
   return CBM_OK;
 }
@@ -1290,6 +1363,8 @@ int cbm_upload_medium(cbm_ctx* c, const double* host, int64_t n) {
   if (!c->medium) return fail(c, CBM_ERR_CONFIG, "model has no medium");
   if (!host || n != c->med_nodes) return fail(c, CBM_ERR_ARG, "medium upload needs prod(NMedium) values");
   HIPCHK(c, hipMemcpyAsync(c->med_c[c->med_cur], host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  if (c->adi)   // DGADI holds the boundary at this field
+    HIPCHK(c, hipMemcpyAsync(c->med_init, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return CBM_OK;
 }

@@ -105,6 +105,21 @@ struct SnapArgs {
   int64_t* n_out;       // may be NULL
 };
 
+// One implicit ADI sweep along `axis` (the reference's tridiagonal solvers,
+// src/integrators.jl:141-428): every line of interior nodes solved by the
+// Thomas algorithm with constant diagonals; cst/mst hold c*[i] and
+// m[i] = 1 / (main - sub c*[i-1]) along the axis, the same for every line.
+struct AdiArgs {
+  const double* src;
+  double* dst;
+  const double* cst;
+  const double* mst;
+  int32_t n[3];
+  int32_t dims, axis;
+  double sub, main, sup;
+  int64_t lines;
+};
+
 struct ExportArgs {
   StageArgs s;
   const int32_t* slot;
@@ -158,6 +173,10 @@ hipError_t launch_disp_check(int dims, int nblocks, int block, hipStream_t s, co
 
 hipError_t launch_medium_count(int nblocks, int block, hipStream_t s, const MediumArgs& a);
 hipError_t launch_medium_step(hipStream_t s, const MediumArgs& a);
+hipError_t launch_adi_sweep(hipStream_t s, const AdiArgs& a);
+// boundary nodes of dst (any coordinate 0 or n-1) set from init
+hipError_t launch_adi_border(hipStream_t s, double* dst, const double* init, const int32_t* n, int32_t dims,
+                             int64_t nodes);
 
 hipError_t launch_rule_decide(int nblocks, int block, hipStream_t s, const RuleArgs& a);
 hipError_t launch_rule_apply(int nblocks, int block, hipStream_t s, const RuleArgs& a);

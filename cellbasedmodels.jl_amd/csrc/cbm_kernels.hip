@@ -937,6 +937,50 @@ __global__ __launch_bounds__(256) void k_medium_count(MediumArgs m) {
 // ghost nodes mirroring the boundary node (zero flux, conserves mass).
 __global__ __launch_bounds__(256) void k_medium_step(MediumArgs m) { medium_step_body<BuiltinMedium>(m); }
 
+// DGADI sweep: one thread per line of interior nodes (lines over the other
+// axes' interior indices), forward elimination into dst, then back
+// substitution in place, in the reference's order of operations.
+__global__ __launch_bounds__(256) void k_adi_sweep(AdiArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.lines) return;
+  const int64_t stride[3] = {1, a.n[0], (int64_t)a.n[0] * a.n[1]};
+  int64_t base = 0, rem = t;
+  for (int b = 0; b < a.dims; ++b) {
+    if (b == a.axis) continue;
+    const int64_t m = a.n[b] - 2;
+    base += (1 + rem % m) * stride[b];
+    rem /= m;
+  }
+  const int64_t s = stride[a.axis];
+  const int n = a.n[a.axis];
+  const double* x = a.src + base;
+  double* y = a.dst + base;
+  double d = (x[s] - a.sub * x[0]) / a.main;
+  y[s] = d;
+  for (int i = 2; i <= n - 3; ++i) {
+    d = (x[i * s] - a.sub * d) * a.mst[i];
+    y[i * s] = d;
+  }
+  d = ((x[(int64_t)(n - 2) * s] - a.sup * x[(int64_t)(n - 1) * s]) - a.sub * d) * a.mst[n - 2];
+  y[(int64_t)(n - 2) * s] = d;
+  for (int i = n - 3; i >= 1; --i) {
+    d = y[i * s] - a.cst[i] * d;
+    y[i * s] = d;
+  }
+}
+
+struct BorderArgs { double* dst; const double* init; int32_t n[3]; int32_t dims; int64_t nodes; };
+
+__global__ __launch_bounds__(256) void k_adi_border(BorderArgs a) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.nodes) return;
+  const int64_t nx = a.n[0], ny = a.n[1];
+  const int64_t p[3] = {k % nx, (k / nx) % ny, k / (nx * ny)};
+  bool edge = false;
+  for (int b = 0; b < a.dims; ++b) edge = edge || p[b] == 0 || p[b] == a.n[b] - 1;
+  if (edge) a.dst[k] = a.init[k];
+}
+
 // N' = N + NAdd_ - NRemove_ and idMax_ += NAdd_ (update.jl:204-214), after
 // every compaction kernel has read the old counts; flags the neighbor
 // rebuild when the population changed.
@@ -1171,6 +1215,20 @@ hipError_t launch_medium_count(int nblocks, int block, hipStream_t s, const Medi
 }
 hipError_t launch_medium_step(hipStream_t s, const MediumArgs& a) {
   hipLaunchKernelGGL(k_medium_step, dim3((unsigned)((a.nodes + 255) / 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_adi_sweep(hipStream_t s, const AdiArgs& a) {
+  hipLaunchKernelGGL(k_adi_sweep, dim3((unsigned)((a.lines + 255) / 256)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_adi_border(hipStream_t s, double* dst, const double* init, const int32_t* n, int32_t dims,
+                             int64_t nodes) {
+  BorderArgs a;
+  a.dst = dst; a.init = init; a.dims = dims; a.nodes = nodes;
+  for (int b = 0; b < 3; ++b) a.n[b] = n[b];
+  hipLaunchKernelGGL(k_adi_border, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

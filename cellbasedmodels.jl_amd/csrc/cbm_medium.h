@@ -53,9 +53,11 @@ __device__ __forceinline__ void medium_step_body(const MediumArgs& m) {
     nd.lap[a] = 0.0;
     nd.x[a] = 0.0;
     if (a < m.dims) {
-      const double cp = (p[a] + 1 < m.n[a]) ? m.c[k + stride[a]] : nd.c;
-      const double cm = (p[a] > 0) ? m.c[k - stride[a]] : nd.c;
-      nd.lap[a] = ((cp - 2 * nd.c) + cm) / (m.dx[a] * m.dx[a]);
+      if (m.lap) {
+        const double cp = (p[a] + 1 < m.n[a]) ? m.c[k + stride[a]] : nd.c;
+        const double cm = (p[a] > 0) ? m.c[k - stride[a]] : nd.c;
+        nd.lap[a] = ((cp - 2 * nd.c) + cm) / (m.dx[a] * m.dx[a]);
+      }
       nd.x[a] = m.lo[a] + ((double)p[a] + 0.5) * m.dx[a];
     }
   }

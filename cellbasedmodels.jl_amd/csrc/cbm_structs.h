@@ -226,6 +226,7 @@ struct MediumArgs {
   int32_t* count;               // agents per node, consumed (zeroed) by the stencil
   int32_t n[3];                 // NMedium
   int32_t dims;
+  int32_t lap, pad;             // lap = 0: the body's Laplacian terms are zero (ADI reaction part)
   double dx[3];                 // (hi - lo) / n per axis
   double lo[3], hi[3];
   double D, uptake, dt;

@@ -87,6 +87,21 @@ enum cbm_law {
   CBM_LAW_SOFT_SPHERES = 5
 };
 
+/* ---------------------------------------------------------- medium solvers */
+/* CBM_MEDIUM_FTCS   explicit c' = c + dt f(c) with zero-flux mirror ghosts.
+ * CBM_MEDIUM_DGADI  the reference's Douglas-Gunn ADI (src/integrators.jl:
+ *   821-935 with the tridiagonal sweeps of :141-428): the reaction part of
+ *   the body (its Laplacian terms zero) as c += dt f, then one implicit sweep
+ *   per axis (x, then y, then z) solving (1 + 2 D r) c_i - D r (c_{i-1} +
+ *   c_{i+1}) = c_i^old on the interior nodes by the Thomas algorithm, with
+ *   r = dt / (dims dx^2) of the last axis for every sweep and boundary nodes
+ *   held at their initial values (the field at creation or at the last
+ *   cbm_upload_medium).  Needs >= 4 nodes per axis. */
+enum cbm_medium_alg {
+  CBM_MEDIUM_FTCS = 0,
+  CBM_MEDIUM_DGADI = 1
+};
+
 /* ------------------------------------------------------------ agent rules */
 /* Built-in agentRule bodies (src/AgentStructure/functionRule.jl:9-111).
  *   CBM_RULE_GROWTH_DIVISION rule_p = {g_mean, g_spread, split, ratio}:
@@ -152,9 +167,9 @@ typedef struct cbm_model_desc {
   double noise;          /* agentSDE diagonal term g (EM only)               */
   double rule_p[4];      /* rule parameters (see enum cbm_rule)              */
   uint64_t seed;         /* Philox4x32-10 key for all in-kernel randomness   */
-  /* medium (explicit FTCS diffusion of one field on a cell-centred grid) */
+  /* medium (diffusion of one field on a cell-centred grid) */
   int32_t medium_n[3];   /* NMedium; all zero = no medium                    */
-  int32_t medium_pad;
+  int32_t medium_alg;    /* enum cbm_medium_alg                              */
   double medium_D;       /* diffusion coefficient                            */
   double medium_consume; /* uptake rate per agent in a node: dc/dt -= rate*c*n */
   double medium_c0;      /* initial concentration                            */

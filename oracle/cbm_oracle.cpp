@@ -163,6 +163,7 @@ struct Orc {
   int64_t mn[3] = {1, 1, 1};
   double mdx[3] = {1, 1, 1};
   std::vector<double> c, cnew;
+  std::vector<double> cinit;                  // DGADI: the field at creation / last orc_set_medium
   std::vector<int32_t> nodeCount;
   int err = 0;
   std::string msg;
@@ -471,7 +472,52 @@ struct OrcMediumNode {
 };
 typedef double (*orc_medium_fn)(const OrcMediumNode* node);
 
+// Douglas-Gunn ADI (src/integrators.jl:864-935): after the reaction step,
+// one implicit sweep per axis with the reference's tridiagonal solver
+// (:141-428): interior nodes of every line of interior nodes, diagonals
+// -D r, 1 + 2 D r, -D r with r = dt / (dims dx^2) of the last axis (the
+// sweeps all read the coefficient column `dims`), c*[i] recomputed per line;
+// every node a sweep does not solve takes its value from u_new, which the
+// reference never writes there, i.e. the initial field.
+static void adi_sweeps(Orc& o) {
+  const int dims = o.dims;
+  const int la = dims - 1;
+  const double dxl = (o.s.box_hi[la] - o.s.box_lo[la]) / (double)o.mn[la];
+  const double r = o.m.dt / ((double)dims * (dxl * dxl));
+  const double sub = -o.m.medium_D * r, main = 1 + 2 * o.m.medium_D * r, sup = -o.m.medium_D * r;
+  const int64_t stride[3] = {1, o.mn[0], o.mn[0] * o.mn[1]};
+  const int64_t nn = o.mn[0] * o.mn[1] * o.mn[2];
+  std::vector<double> cst, dstar;
+  for (int ax = 0; ax < dims; ++ax) {
+    std::vector<double> xnew = o.cinit;
+    const std::vector<double>& x = o.c;
+    const int64_t n = o.mn[ax], s = stride[ax];
+    cst.assign(n, 0.0);
+    dstar.assign(n, 0.0);
+    for (int64_t k = 0; k < nn; ++k) {
+      const int64_t p[3] = {k % o.mn[0], (k / o.mn[0]) % o.mn[1], k / (o.mn[0] * o.mn[1])};
+      bool line = p[ax] == 0;
+      for (int b = 0; b < dims; ++b)
+        if (b != ax) line = line && p[b] >= 1 && p[b] <= o.mn[b] - 2;
+      if (!line) continue;
+      cst[1] = sup / main;
+      dstar[1] = (x[k + s] - sub * x[k]) / main;
+      for (int64_t i = 2; i <= n - 3; ++i) {
+        const double m = 1.0 / (main - sub * cst[i - 1]);
+        cst[i] = sup * m;
+        dstar[i] = (x[k + i * s] - sub * dstar[i - 1]) * m;
+      }
+      const double m = 1.0 / (main - sub * cst[n - 3]);
+      dstar[n - 2] = ((x[k + (n - 2) * s] - sup * x[k + (n - 1) * s]) - sub * dstar[n - 3]) * m;
+      xnew[k + (n - 2) * s] = dstar[n - 2];
+      for (int64_t i = n - 3; i >= 1; --i) xnew[k + i * s] = dstar[i] - cst[i] * xnew[k + (i + 1) * s];
+    }
+    o.c.swap(xnew);
+  }
+}
+
 static void medium_step(Orc& o, int64_t nAgents) {
+  const bool adi = o.m.medium_alg == CBM_MEDIUM_DGADI;   // reaction part here, diffusion in the sweeps
   int64_t nn = o.mn[0] * o.mn[1] * o.mn[2];
   o.nodeCount.assign(nn, 0);
   for (int64_t i = 0; i < nAgents; ++i) {
@@ -502,12 +548,13 @@ static void medium_step(Orc& o, int64_t nAgents) {
         if (a >= o.dims) continue;
         double cp = (p[a] + 1 < o.mn[a]) ? o.c[k + stride[a]] : nd.c;
         double cm = (p[a] > 0) ? o.c[k - stride[a]] : nd.c;
-        nd.lap[a] = ((cp - 2 * nd.c) + cm) / (o.mdx[a] * o.mdx[a]);
+        if (!adi) nd.lap[a] = ((cp - 2 * nd.c) + cm) / (o.mdx[a] * o.mdx[a]);
         nd.x[a] = o.s.box_lo[a] + ((double)p[a] + 0.5) * o.mdx[a];
       }
       o.cnew[k] = nd.c + dt * fn(&nd);
     }
     o.c.swap(o.cnew);
+    if (adi) adi_sweeps(o);
     return;
   }
 #pragma omp parallel for schedule(static)
@@ -516,7 +563,7 @@ static void medium_step(Orc& o, int64_t nAgents) {
     double cc = o.c[k];
     double lap = 0.0;
     int64_t stride[3] = {1, o.mn[0], o.mn[0] * o.mn[1]};
-    for (int a = 0; a < o.dims; ++a) {
+    for (int a = 0; a < o.dims && !adi; ++a) {
       // zero-flux ghost mirror: c[-1] = c[0], c[n] = c[n-1]
       double cp = (p[a] + 1 < o.mn[a]) ? o.c[k + stride[a]] : cc;
       double cm = (p[a] > 0) ? o.c[k - stride[a]] : cc;
@@ -527,6 +574,7 @@ static void medium_step(Orc& o, int64_t nAgents) {
     o.cnew[k] = cc + dt * f;
   }
   o.c.swap(o.cnew);
+  if (adi) adi_sweeps(o);
 }
 
 // ------------------------------------------------------------------ rules
@@ -884,6 +932,10 @@ void* orc_create(const cbm_model_desc* m, const cbm_sim_desc* s) {
       o->mdx[a] = (s->box_hi[a] - s->box_lo[a]) / (double)m->medium_n[a];   // communityStructure.jl:274
     }
     o->c.assign(o->mn[0] * o->mn[1] * o->mn[2], m->medium_c0);
+    o->cinit = o->c;
+    if (m->medium_alg == CBM_MEDIUM_DGADI)
+      for (int a = 0; a < o->dims; ++a)
+        if (o->mn[a] < 4) { o->err = CBM_ERR_CONFIG; o->msg = "DGADI needs NMedium >= 4 on every axis"; }
   }
   return o;
 }
@@ -1002,6 +1054,7 @@ int orc_set_medium(void* h, const double* in, int64_t n) {
   Orc& o = *(Orc*)h;
   if (n > (int64_t)o.c.size()) return CBM_ERR_ARG;
   std::memcpy(o.c.data(), in, sizeof(double) * n);
+  o.cinit = o.c;
   return 0;
 }
 
