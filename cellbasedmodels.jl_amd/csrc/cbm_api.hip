@@ -160,6 +160,7 @@ struct cbm_ctx {
   hipFunction_t user_brick[2] = {nullptr, nullptr};   // the same, cell-linked brick tiles
   hipFunction_t user_rule[2] = {nullptr, nullptr};    // agentRule body: decide, apply
   hipFunction_t user_medium = nullptr;                // mediumODE body: the FTCS step
+  double* rule_nb[8] = {};                // a neighbor-looping rule body's view: x, y, z, vol, u0..u3
   int user_nb = NB_FULL;                  // neighbor walk the module was built for
   bool user_noise = false;                // the module holds a user agentSDE body
   double* gacc[MAX_STATE] = {};           // EulerHeun with a user agentSDE body: g of stage 1
@@ -462,6 +463,7 @@ static void destroy_ctx(cbm_ctx* c) {
   F(c->gr_new); F(c->vb_new); F(c->dead);
   F(c->n_add_slot); F(c->dead_slot); F(c->off_add); F(c->off_dead); F(c->holes);
   F(c->med_c[0]); F(c->med_c[1]); F(c->med_count);
+  for (int k = 0; k < 8; ++k) F(c->rule_nb[k]);
   F(c->med_init); F(c->med_tmp); F(c->adi_tab[0]); F(c->adi_tab[1]); F(c->adi_tab[2]);
   F(c->key); F(c->lin); F(c->gbucket); F(c->cell_range); F(c->count); F(c->cursor); F(c->start); F(c->tmp); F(c->cell_row);
   F(c->scan_status); F(c->scan_ticket); F(c->list); F(c->list_n); F(c->bricks); F(c->brick_skip); F(c->blk_flag); F(c->flags); F(c->counters);
@@ -928,6 +930,12 @@ static RuleArgs rule_args(cbm_ctx* c, int32_t* flag) {
   r.lf = (int64_t*)c->col[C_LF].cur();
   if (c->m.rule == CBM_RULE_CUSTOM)
     for (int k = 0; k < 4; ++k) r.u[k] = (double*)c->col[C_U0 + k].cur();
+  r.list = c->list; r.list_n = c->list_n; r.ldl = c->C;
+  r.cell_row = c->cell_row; r.cell_range = c->cell_range;
+  r.nb_kind = c->nb_kind;
+  for (int d = 0; d < 3; ++d) r.nb_x[d] = c->rule_nb[d];
+  r.nb_vol = c->rule_nb[3];
+  for (int k = 0; k < 4; ++k) r.nb_u[k] = c->rule_nb[4 + k];
   r.slot = (int32_t*)c->col[C_SLOT].cur();
   r.dead = c->dead;
   r.n_add_slot = c->n_add_slot; r.dead_slot = c->dead_slot;
@@ -949,6 +957,7 @@ static int rule_phase(cbm_ctx* c, int32_t* flag) {
   const bool user = c->m.rule == CBM_RULE_CUSTOM;
   if (user && !c->user_rule[0])
     return fail(c, CBM_ERR_CONFIG, "CBM_RULE_CUSTOM needs cbm_compile_model with a rule body before stepping");
+  if (user && c->rule_nb[0]) HIPCHK(c, launch_rule_nb_copy(c->nblocks, c->block, c->stream, r, c->rule_nb));
   void* params[] = {(void*)&r};
   auto user_launch = [&](hipFunction_t f) {
     return hipModuleLaunchKernel(f, (unsigned)c->nblocks, 1, 1, (unsigned)c->block, 1, 1, 0, c->stream, params, nullptr);
@@ -1247,6 +1256,11 @@ int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, 
   if (rule_body) {
     HIPCHK(c, hipModuleGetFunction(&c->user_rule[0], c->user_mod, "cbm_user_rule_decide"));
     HIPCHK(c, hipModuleGetFunction(&c->user_rule[1], c->user_mod, "cbm_user_rule_apply"));
+    // a body with a neighbor loop reads its neighbors from a copy taken
+    // before the rule's passes
+    if (std::strstr(rule_body, "forNeighbors"))
+      for (int k = 0; k < 8; ++k)
+        if (!c->rule_nb[k]) HIPCHK(c, dalloc(&c->rule_nb[k], c->C));
   }
   if (stages) {
     HIPCHK(c, hipModuleGetFunction(&c->user_fn[0], c->user_mod, "cbm_user_stage_0"));

@@ -179,6 +179,9 @@ hipError_t launch_adi_border(hipStream_t s, double* dst, const double* init, con
                              int64_t nodes);
 
 hipError_t launch_rule_decide(int nblocks, int block, hipStream_t s, const RuleArgs& a);
+// copies positions, vol and the user parameters of the committed agents into
+// the rule's neighbor view (RuleArgs::nb_*; dst = those pointers)
+hipError_t launch_rule_nb_copy(int nblocks, int block, hipStream_t s, const RuleArgs& a, double* const* dst);
 hipError_t launch_rule_apply(int nblocks, int block, hipStream_t s, const RuleArgs& a);
 hipError_t launch_compact_holes(int nblocks, int block, hipStream_t s, const RuleArgs& a);
 hipError_t launch_compact_slots(int nblocks, int block, hipStream_t s, const RuleArgs& a);

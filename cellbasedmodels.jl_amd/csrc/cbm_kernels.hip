@@ -1232,6 +1232,25 @@ hipError_t launch_adi_border(hipStream_t s, double* dst, const double* init, con
   return hipGetLastError();
 }
 
+struct NbCopy { double* dst[8]; };
+
+__global__ __launch_bounds__(256) void k_rule_nb_copy(RuleArgs a, NbCopy d) {
+  const RowIdx ri = row_index(a.g);
+  if (ri.local >= a.g.st[ri.sim].N) return;
+  const int64_t r = ri.row;
+  for (int c = 0; c < a.dims; ++c) d.dst[c][r] = a.pos[c][r];
+  d.dst[3][r] = a.vol[r];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) d.dst[4 + k][r] = a.u[k][r];
+}
+
+hipError_t launch_rule_nb_copy(int nblocks, int block, hipStream_t s, const RuleArgs& a, double* const* dst) {
+  NbCopy d;
+  for (int k = 0; k < 8; ++k) d.dst[k] = dst[k];
+  hipLaunchKernelGGL(k_rule_nb_copy, dim3(nblocks), dim3(block), 0, s, a, d);
+  return hipGetLastError();
+}
+
 hipError_t launch_rule_decide(int nblocks, int block, hipStream_t s, const RuleArgs& a) {
   hipLaunchKernelGGL(k_rule_decide, dim3(nblocks), dim3(block), 0, s, a);
   return hipGetLastError();

@@ -29,6 +29,18 @@ struct RuleRec {
   int64_t lf;
 };
 
+// One neighbor as a rule body's neighbor loop sees it (values at the start
+// of the step).
+struct RuleNeighbor {
+  int64_t row, id;
+  double x[3];                 // its position
+  double dx[3];                // the agent's position minus it (minimum image in a periodic box)
+  double d;                    // euclidean distance (src/metrics.jl:34-36)
+  const RuleArgs* args;
+  __device__ double vol() const { return args->nb_vol[row]; }
+  __device__ double u(int k) const { return args->nb_u[k][row]; }
+};
+
 // What a UserRule body receives: the agent's values at the start of the step
 // (old), its new values (nw, initialised to old) and the new agents it adds.
 // New agents are passed by value and add() merges the record into every
@@ -43,6 +55,8 @@ struct RuleAgent {
   RuleRec dn[RULE_MAX_ADD];
   int32_t n_add, dead, over;
   uint32_t n_draw;
+  const RuleArgs* args;
+  int64_t row;
 
   // a new agent's values before the body sets any: the parent's old values
   // (@addAgent copies every parameter it is not given, functionRule.jl:46-55)
@@ -78,6 +92,46 @@ struct RuleAgent {
     return sigma * z0 + mu;
   }
   __device__ double exponential(double theta) { return -log(1.0 - rand()) * theta; }
+
+  // @loopOverNeighbors (src/AgentStructure/macros.jl:97-126): f(RuleNeighbor)
+  // for every candidate of the neighbor structure, in its walk order
+  template <class F>
+  __device__ void forNeighbors(F&& f) const {
+    const SimDev& sd = args->g.sims[sim];
+    const int64_t N = args->g.st[sim].N;
+    auto visit = [&](int64_t j) {
+      RuleNeighbor n;
+      n.row = j;
+      n.id = args->id[j];
+      n.args = args;
+      double s2 = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        n.x[c] = 0.0;
+        n.dx[c] = 0.0;
+        if (c < dims) {
+          n.x[c] = args->nb_x[c][j];
+          double e = old.x[c] - n.x[c];
+          if (args->periodic) e = min_image(e, sd.L[c]);
+          n.dx[c] = e;
+          s2 = c == 0 ? e * e : s2 + e * e;
+        }
+      }
+      n.d = sqrt(s2);
+      f(n);
+    };
+    switch (args->nb_kind * 4 + dims) {
+      case NB_FULL * 4 + 1: for_candidates<1, NB_FULL>(*args, sd, N, row, visit); break;
+      case NB_FULL * 4 + 2: for_candidates<2, NB_FULL>(*args, sd, N, row, visit); break;
+      case NB_FULL * 4 + 3: for_candidates<3, NB_FULL>(*args, sd, N, row, visit); break;
+      case NB_CELL * 4 + 1: for_candidates<1, NB_CELL>(*args, sd, N, row, visit); break;
+      case NB_CELL * 4 + 2: for_candidates<2, NB_CELL>(*args, sd, N, row, visit); break;
+      case NB_CELL * 4 + 3: for_candidates<3, NB_CELL>(*args, sd, N, row, visit); break;
+      case NB_LIST * 4 + 1: for_candidates<1, NB_LIST>(*args, sd, N, row, visit); break;
+      case NB_LIST * 4 + 2: for_candidates<2, NB_LIST>(*args, sd, N, row, visit); break;
+      default: for_candidates<3, NB_LIST>(*args, sd, N, row, visit); break;
+    }
+  }
 };
 
 __device__ __forceinline__ void rule_load(const RuleArgs& a, int64_t r, RuleRec& v) {
@@ -100,6 +154,8 @@ __device__ __forceinline__ void rule_eval(const RuleArgs& a, int sim, int64_t r,
   ag.dt = a.dt;
   ag.seed = a.seed;
   ag.n_add = 0; ag.dead = 0; ag.over = 0; ag.n_draw = 0;
+  ag.args = &a;
+  ag.row = r;
   rule_load(a, r, ag.old);
   ag.nw = ag.old;
   const Rule rule(a.rp, a.g.sims[sim].p);

@@ -57,9 +57,11 @@ __device__ __forceinline__ int32_t neigh_cell(const SimDev& sd, int32_t cx, int3
 
 __host__ __device__ constexpr int pow3(int d) { return d == 1 ? 3 : (d == 2 ? 9 : 27); }
 
-// Walk the candidate neighbors of row i in reference loop order.
-template <int D, int NB, class Fn>
-__device__ __forceinline__ void for_candidates(const StageArgs& a, const SimDev& sd, int64_t N, int64_t i, Fn&& fn) {
+// Walk the candidate neighbors of row i in reference loop order (Args: the
+// stage or rule arguments, with the list and cell tables of the last
+// neighbor update).
+template <int D, int NB, class Args, class Fn>
+__device__ __forceinline__ void for_candidates(const Args& a, const SimDev& sd, int64_t N, int64_t i, Fn&& fn) {
   if (NB == NB_LIST) {
     const int32_t cnt = a.list_n[i];
     for (int32_t q = 0; q < cnt; ++q) fn((int64_t)a.list[(int64_t)q * a.ldl + i]);

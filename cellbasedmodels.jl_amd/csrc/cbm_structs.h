@@ -211,6 +211,18 @@ struct RuleArgs {
   int32_t* holes;               // ascending hole slots below the new N
   int32_t* err;                 // capacity error flag
   int32_t* rebuild_flag;        // set when agents were added or removed
+  // neighbor loops of a user rule: the structure of the last neighbor update
+  // and a copy of the values a neighbor exposes, taken before the rule's
+  // passes (the apply pass writes the agents' own values in place)
+  const int32_t* list;
+  const int32_t* list_n;
+  int64_t ldl;
+  const int32_t* cell_row;
+  const CellRange* cell_range;
+  int32_t nb_kind, pad2;
+  const double* nb_x[3];
+  const double* nb_vol;
+  const double* nb_u[4];
 };
 
 // Medium: one field on a cell-centred grid over the box of simulation 0,

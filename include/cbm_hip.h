@@ -239,14 +239,17 @@ int cbm_compile(cbm_ctx* ctx, const char* pair_body_hip);
  * run once per agent and step before the DE stages.  RuleAgent (cbm_rule.h)
  * holds a.old, the agent's values at the start of the step (x[3], vol,
  * growth, vbirth, u[4], lf), and a.nw, its new values (initialised to old);
- * a.id, a.step, a.dt and a.dims; a.add() returns a new agent's record
- * (initialised to the parent's old values, at most 2 per agent and step),
- * a.remove() removes the agent; a.uniform(l0, l1), a.normal(mu, sigma) and
- * a.exponential(theta) are the reference's CBMDistributions
- * (src/random.jl:1-23) on Philox(seed, step, id, sim, 0x600 + k), k counting
- * the agent's draws in the step.  New agents get ids idMax + 1 + (prefix of
- * additions in slot order), the reference's serial append order.  The body
- * must not read other agents.
+ * a.id, a.step, a.dt and a.dims; a.child() is a new agent's record
+ * (the parent's old values) and a.add(d) adds it (@addAgent; at most 2 per
+ * agent and step); a.remove() removes the agent (@removeAgent);
+ * a.uniform(l0, l1), a.normal(mu, sigma) and a.exponential(theta) are the
+ * reference's CBMDistributions (src/random.jl:1-23) on Philox(seed, step,
+ * id, sim, 0x600 + k), k counting the agent's draws in the step;
+ * a.forNeighbors(f) is @loopOverNeighbors (macros.jl:97-126): f(const
+ * RuleNeighbor& n) for every candidate of the neighbor structure in its walk
+ * order, with n.x, n.dx (minimum image), n.d, n.id, n.vol() and n.u(k) as
+ * they were before the rule ran.  New agents get ids idMax + 1 + (prefix of
+ * additions in slot order), the reference's serial append order.
  * medium_body (mediumODE, functionDE.jl:13-116 with the medium loop of
  * auxiliar.jl:168-172,184-210 and @∂2 of macros.jl:203-257; models with a
  * medium, NULL = the built-in diffusion with uptake) is HIP C++ defining,

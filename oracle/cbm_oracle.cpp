@@ -149,6 +149,7 @@ struct Orc {
   std::vector<double> U[4];                   // user agent parameters (CBM_RULE_CUSTOM)
   std::vector<char> alive;
   void* rule_fn = nullptr;                    // CBM_RULE_CUSTOM body (orc_rule_fn)
+  std::vector<double> ruleX[3], ruleVol, ruleU[4];   // the rule's neighbor view: values at the step's start
   void* medium_fn = nullptr;                  // user mediumODE body (orc_medium_fn), else built-in
   // neighbor structures
   Grid g;
@@ -585,7 +586,7 @@ struct Births { std::vector<int64_t> parent; };
 // agents (add, each initialised to old by the caller); sets *n_add, *dead.
 struct OrcRuleRec { double x[3]; double vol, growth, vbirth; double u[4]; int64_t lf; };
 typedef void (*orc_rule_fn)(int64_t id, int64_t step, const OrcRuleRec* old, OrcRuleRec* nw, OrcRuleRec* add,
-                            int32_t* n_add, int32_t* dead);
+                            int32_t* n_add, int32_t* dead, int64_t slot);
 
 // The generated rule loop of functionRule.jl:121-160 around a custom body,
 // serially in ascending slot order: the new values are written in place (DE
@@ -596,6 +597,11 @@ static void apply_custom_rule(Orc& o, std::vector<int64_t>& holes, int64_t& nAdd
   orc_rule_fn fn = (orc_rule_fn)o.rule_fn;
   if (!fn) { o.err = CBM_ERR_CONFIG; o.msg = "no custom rule"; return; }
   const int64_t N = o.N;
+  // neighbors are read as they were when the loop started (the reference's
+  // reads of x[i2] see the committed values; writes go to x__)
+  for (int a = 0; a < 3; ++a) o.ruleX[a] = o.X[a];
+  o.ruleVol = o.vol;
+  for (int k = 0; k < 4; ++k) o.ruleU[k] = o.U[k];
   for (int64_t i = 0; i < N; ++i) {
     OrcRuleRec old;
     for (int a = 0; a < 3; ++a) old.x[a] = a < o.dims ? o.X[a][i] : 0.0;
@@ -604,7 +610,7 @@ static void apply_custom_rule(Orc& o, std::vector<int64_t>& holes, int64_t& nAdd
     old.lf = o.lf[i];
     OrcRuleRec nw = old, add[2] = {old, old};
     int32_t na = 0, dead = 0;
-    fn(o.id[i], o.step, &old, &nw, add, &na, &dead);
+    fn(o.id[i], o.step, &old, &nw, add, &na, &dead, i);
     if (na > 2) { o.err = CBM_ERR_CONFIG; o.msg = "more than 2 additions"; na = 2; }
     for (int a = 0; a < o.dims; ++a) o.X[a][i] = nw.x[a];
     o.vol[i] = nw.vol; o.grNew[i] = nw.growth; o.vbNew[i] = nw.vbirth;
@@ -1067,6 +1073,23 @@ void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
 // CBM_RULE_CUSTOM: the rule body, called once per agent and step (test code
 // passes a restatement of the device body it checks).
 void orc_set_rule(void* h, void* fn) { ((Orc*)h)->rule_fn = fn; }
+// A custom rule's neighbor loop: the candidates of slot i in the structure's
+// walk order, and their values as the rule's loop started (field 0..2
+// positions, 3 vol, 4..7 the user parameters, 8 id).
+int64_t orc_rule_neighbors(void* h, int64_t i, int64_t* out, int64_t cap) {
+  Orc& o = *(Orc*)h;
+  int64_t k = 0;
+  for_candidates(o, i, [&](int64_t j) { if (k < cap) out[k] = j; k++; });
+  return k;
+}
+double orc_rule_value(void* h, int field, int64_t j) {
+  Orc& o = *(Orc*)h;
+  if (field < 3) return o.ruleX[field][j];
+  if (field == 3) return o.ruleVol[j];
+  if (field < 8) return o.ruleU[field - 4][j];
+  return (double)o.id[j];
+}
+
 // A user mediumODE body in place of the built-in diffusion with uptake.
 void orc_set_medium_body(void* h, void* fn) { ((Orc*)h)->medium_fn = fn; }
 

@@ -51,6 +51,8 @@ def lib():
             "orc_set_threads": (None, [C.c_int]),
             "orc_set_noise_slope": (None, [P, C.c_double]),
             "orc_set_rule": (None, [P, P]),
+            "orc_rule_neighbors": (C.c_int64, [P, C.c_int64, P, C.c_int64]),
+            "orc_rule_value": (C.c_double, [P, C.c_int, C.c_int64]),
             "orc_set_medium_body": (None, [P, P]),
             "orc_traj_loss": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P]),
             "orc_traj_grad_dual": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P, P]),
@@ -73,7 +75,34 @@ class RuleRec(C.Structure):
 
 
 RULE_FN = C.CFUNCTYPE(None, C.c_int64, C.c_int64, C.POINTER(RuleRec), C.POINTER(RuleRec), C.POINTER(RuleRec),
-                      C.POINTER(C.c_int32), C.POINTER(C.c_int32))
+                      C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int64)
+
+
+class RuleNeighbor:
+    """A neighbor in a rule's loop (values as the loop started)."""
+
+    def __init__(self, oracle, ag, j):
+        lib_, h = oracle.lib, oracle.h
+        self.row = j
+        self.id = int(lib_.orc_rule_value(h, 8, j))
+        self.x = [lib_.orc_rule_value(h, c, j) if c < ag.dims else 0.0 for c in range(3)]
+        self.dx = [0.0, 0.0, 0.0]
+        s2 = 0.0
+        for c in range(ag.dims):
+            e = ag.old.x[c] - self.x[c]
+            if oracle.model.periodic:
+                L = oracle.sim.box_hi[c] - oracle.sim.box_lo[c]
+                e = e - L if e > 0.5 * L else (e + L if e < -0.5 * L else e)
+            self.dx[c] = e
+            s2 = e * e if c == 0 else s2 + e * e
+        self.d = math.sqrt(s2)
+        self._o, self._j = oracle, j
+
+    def vol(self):
+        return self._o.lib.orc_rule_value(self._o.h, 3, self._j)
+
+    def u(self, k):
+        return self._o.lib.orc_rule_value(self._o.h, 4 + k, self._j)
 RULE_PURPOSE = 0x600
 
 
@@ -187,9 +216,10 @@ class Oracle:
         """CBM_RULE_CUSTOM: body(agent) restates the device rule on a RuleAgent."""
         m = self.model
 
-        def cb(id_, step, old, nw, add, n_add, dead):
+        def cb(id_, step, old, nw, add, n_add, dead, slot):
             ag = RuleAgent(int(m.seed), int(id_), int(step), int(m.dims), float(m.dt), old.contents, nw.contents,
                            [add[0], add[1]])
+            ag.neighbors = lambda: self._rule_neighbors(ag, int(slot))
             body(ag)
             n_add[0] = ag.n_add
             dead[0] = ag.dead
@@ -201,6 +231,12 @@ class Oracle:
         """A user mediumODE body: body(node) returns dt(c) at a MediumNode."""
         self._medium_cb = MEDIUM_FN(lambda nd: float(body(nd.contents)))
         self.lib.orc_set_medium_body(self.h, C.cast(self._medium_cb, C.c_void_p))
+
+    def _rule_neighbors(self, ag, slot):
+        out = np.zeros(4096, dtype=np.int64)
+        k = self.lib.orc_rule_neighbors(self.h, slot, out.ctypes.data, out.size)
+        assert k <= out.size
+        return [RuleNeighbor(self, ag, int(j)) for j in out[:k]]
 
     def set_noise_slope(self, slope):
         """State-dependent agentSDE term g_a(x) = noise + slope * x_a on the positions."""

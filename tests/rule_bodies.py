@@ -108,3 +108,81 @@ struct UserRule {
   }
 };
 """
+
+
+# testCommunity.jl:361-415: nn = number of neighbors closer than rp[0], by
+# @loopOverNeighbors, into u[0]
+COUNT_NEIGHBORS = """
+struct UserRule {
+  double r;
+  __device__ UserRule(const double* rp, const double*) : r(rp[0]) {}
+  __device__ void operator()(RuleAgent& a) const {
+    double nn = 0.0;
+    a.forNeighbors([&](const RuleNeighbor& n) { if (n.d < r) nn += 1.0; });
+    a.nw.u[0] = nn;
+  }
+};
+"""
+
+
+def count_neighbors(r):
+    def body(a):
+        nn = 0.0
+        for n in a.neighbors():
+            if n.d < r:
+                nn += 1.0
+        a.nw.u[0] = nn
+    return body
+
+
+# contact inhibition: an agent with fewer than rp[1] neighbors closer than
+# rp[0] adds one at +0.3 along x; u[1] = distance to its nearest neighbor,
+# u[2] = the neighbors' u[3] summed in walk order; every agent is pushed
+# 0.01 * (its neighbor count) along x
+INHIBIT = """
+struct UserRule {
+  double r, nmax;
+  __device__ UserRule(const double* rp, const double*) : r(rp[0]), nmax(rp[1]) {}
+  __device__ void operator()(RuleAgent& a) const {
+    double nn = 0.0, dmin = 1e9, su = 0.0;
+    a.forNeighbors([&](const RuleNeighbor& n) {
+      if (n.d < r) { nn += 1.0; su = su + n.u(3); }
+      dmin = n.d < dmin ? n.d : dmin;
+    });
+    a.nw.u[0] = nn;
+    a.nw.u[1] = dmin;
+    a.nw.u[2] = su;
+    a.nw.x[0] = a.old.x[0] + 0.01 * nn;
+    if (nn < nmax) {
+      RuleRec d = a.child();
+      d.x[0] = a.old.x[0] + 0.3;
+      d.u[3] = a.old.u[3] + 1.0;
+      d.lf = a.id;
+      a.add(d);
+    }
+  }
+};
+"""
+
+
+def inhibit(rp):
+    r, nmax = rp[0], rp[1]
+
+    def body(a):
+        nn, dmin, su = 0.0, 1e9, 0.0
+        for n in a.neighbors():
+            if n.d < r:
+                nn += 1.0
+                su = su + n.u(3)
+            dmin = n.d if n.d < dmin else dmin
+        a.nw.u[0] = nn
+        a.nw.u[1] = dmin
+        a.nw.u[2] = su
+        a.nw.x[0] = a.old.x[0] + 0.01 * nn
+        if nn < nmax:
+            d = a.child()
+            d.x[0] = a.old.x[0] + 0.3
+            d.u[3] = a.old.u[3] + 1.0
+            d.lf = a.id
+            a.add(d)
+    return body

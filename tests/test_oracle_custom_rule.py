@@ -115,3 +115,22 @@ def test_custom_divide_replaces_every_agent_with_two():
         assert abs((x[k[0]] + x[k[1]]) / 2 - pos[0][p - 1]) < 1e-12
     assert abs(o.get(A.F_VOL).sum() - 2.0 * N) < 1e-12
     assert np.all(o.get(A.F_U0) == 1.0)
+
+
+@pytest.mark.parametrize("dims", [1, 2, 3])
+def test_custom_rule_neighbor_loop_lattice_kat(dims):
+    # the reference's neighbor KAT (testCommunity.jl:361-415) as a rule body
+    # with a neighbor loop, for every neighbor algorithm
+    from test_oracle import lattice_case
+    from helpers import lattice
+    g = golden("ref_neighbors_lattice.json")
+    for alg in g["algorithms"]:
+        m, s = lattice_case(dims, alg)
+        m.rule = A.RULE_CUSTOM
+        o = O.Oracle(m, s)
+        o.set_rule(RB.count_neighbors(g["radius"]))
+        for a, v in enumerate(lattice(dims)):
+            o.set(a, v)
+        assert o.init() == 0
+        o.step(1)
+        assert o.get(A.F_U0).tolist() == [float(v) for v in g["expected"][str(dims)]], alg["name"]
