@@ -749,9 +749,8 @@ static int rebuild_chain(cbm_ctx* c, const int32_t* flag, int32_t* flag_clear, b
   RebuildArgs a = rebuild_args(c, flag, flag_clear, tick);
   HIPCHK(c, launch_keys(c->nblocks, c->block, c->stream, a));
   int rc = run_scan(c, flag, c->count, c->cursor, c->start, c->NB);
-  if (!rc && !c->full) HIPCHK(c, launch_ranges(c->stream, a));
   if (rc) return rc;
-  HIPCHK(c, launch_scatter(c->nblocks, c->block, c->stream, a));
+  HIPCHK(c, launch_scatter(c->nblocks, c->block, c->stream, a, !c->full));
   PermuteArgs pa;
   pa.rb = a;
   pa.cols = perm_cols(c);
@@ -1697,9 +1696,8 @@ static int rebuild_in_place(cbm_ctx* c, double* const* X, const int32_t* flag,
   a.dead = nullptr;
   HIPCHK(c, launch_keys(c->nblocks, c->block, c->stream, a));
   int rc = run_scan(c, flag, c->count, c->cursor, c->start, c->NB);
-  if (!rc && !c->full) HIPCHK(c, launch_ranges(c->stream, a));
   if (rc) return rc;
-  HIPCHK(c, launch_scatter(c->nblocks, c->block, c->stream, a));
+  HIPCHK(c, launch_scatter(c->nblocks, c->block, c->stream, a, !c->full));
   VerletArgs va;
   std::memset(&va, 0, sizeof(va));
   va.rb = a;

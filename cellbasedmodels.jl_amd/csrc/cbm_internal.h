@@ -38,7 +38,7 @@ struct RebuildArgs {
   int32_t* start;             // per-bucket first row (exclusive scan), nb+1 entries
   int32_t* tmp;               // bucket-ordered source rows
   int32_t* cell_row;          // destination row -> linear local cell
-  CellRange* range;           // rank_off + linear local cell -> rows, written by k_ranges
+  CellRange* range;           // rank_off + linear local cell -> rows, written by k_scatter's range blocks
   int64_t nr;                 // entries of gbucket / range (all simulations)
   int32_t dims, periodic, full, geometric;
 };
@@ -200,8 +200,8 @@ hipError_t launch_stage(int dims, int nb, int law, int nblocks, int block, hipSt
                         int64_t nbricks, size_t brick_lds);
 hipError_t launch_keys(int nblocks, int block, hipStream_t s, const RebuildArgs& a);
 hipError_t launch_scan(hipStream_t s, const ScanArgs& a);
-hipError_t launch_ranges(hipStream_t s, const RebuildArgs& a);
-hipError_t launch_scatter(int nblocks, int block, hipStream_t s, const RebuildArgs& a);
+// the row scatter, and with ranges the cell range table (one launch)
+hipError_t launch_scatter(int nblocks, int block, hipStream_t s, const RebuildArgs& a, bool ranges);
 hipError_t launch_permute(int nblocks, int block, hipStream_t s, const PermuteArgs& a);
 hipError_t launch_verlet(int dims, int nblocks, int block, hipStream_t s, const VerletArgs& a);
 // Verlet list build on LDS brick tiles (one block per brick), to be followed

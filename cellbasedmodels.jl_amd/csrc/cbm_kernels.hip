@@ -9,8 +9,8 @@
 //                      src/neighbors.jl:641-700)
 //   k_keys             cell key + bucket histogram   (assignCells, src/neighbors.jl:763-821)
 //   k_scan             single-pass exclusive scan    (cumsum, src/neighbors.jl:934-950)
-//   k_ranges           per-cell row ranges from the bucket starts
 //   k_scatter          bucket scatter                (sortAgentsInCells, src/neighbors.jl:837-899)
+//                      and, in the same launch, per-cell row ranges from the bucket starts
 //   k_permute          deterministic in-cell order + column permutation
 //   k_verlet           Verlet list build from cells (src/neighbors.jl:92-165, 1071-1183)
 //
@@ -227,19 +227,20 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(ScanArgs a) {
   }
 }
 
-// Rows [s, e) of every grid cell by linear cell (all simulations), gathered
-// from the bucket starts; written in table order, so the stores coalesce.
-__global__ __launch_bounds__(256) void k_ranges(RebuildArgs a) {
+// One launch after the scan: blocks [0, nrow) scatter the rows into bucket
+// order; blocks past them gather the rows [s, e) of every grid cell by linear
+// cell (all simulations) from the bucket starts, written in table order so
+// the stores coalesce.
+__global__ __launch_bounds__(256) void k_scatter(RebuildArgs a, int nrow) {
   if (a.flag && *a.flag == 0) return;
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.nr) return;
-  const int32_t b = a.gbucket[g];
-  a.range[g] = CellRange{a.start[b], a.start[b + 1]};
-}
-
-__global__ __launch_bounds__(256) void k_scatter(RebuildArgs a) {
-  if (a.flag && *a.flag == 0) return;
-  const RowIdx ri = row_index(a.g);
+  if ((int)blockIdx.x >= nrow) {
+    const int64_t g = (int64_t)(blockIdx.x - nrow) * blockDim.x + threadIdx.x;
+    if (g >= a.nr) return;
+    const int32_t b = a.gbucket[g];
+    a.range[g] = CellRange{a.start[b], a.start[b + 1]};
+    return;
+  }
+  const RowIdx ri = row_index_of(a.g, (int)blockIdx.x, nrow);
   const SimDev& sd = a.g.sims[ri.sim];
   if (ri.local >= a.g.st[ri.sim].n_tot) return;
   const int64_t r = ri.row;
@@ -1141,13 +1142,9 @@ hipError_t launch_scan(hipStream_t s, const ScanArgs& a) {
   return hipGetLastError();
 }
 
-hipError_t launch_ranges(hipStream_t s, const RebuildArgs& a) {
-  hipLaunchKernelGGL(k_ranges, dim3((unsigned)((a.nr + 255) / 256)), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-hipError_t launch_scatter(int nblocks, int block, hipStream_t s, const RebuildArgs& a) {
-  hipLaunchKernelGGL(k_scatter, dim3(nblocks), dim3(block), 0, s, a);
+hipError_t launch_scatter(int nblocks, int block, hipStream_t s, const RebuildArgs& a, bool ranges) {
+  const int64_t nrb = ranges ? (a.nr + block - 1) / block : 0;
+  hipLaunchKernelGGL(k_scatter, dim3((unsigned)(nblocks + nrb)), dim3(block), 0, s, a, nblocks);
   return hipGetLastError();
 }
 

@@ -25,6 +25,17 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
 }
 
+// Row of this thread for block bx of a launch whose first nb blocks serve
+// the rows.
+__device__ __forceinline__ RowIdx row_index_of(const RowGeom& g, int bx, int nb) {
+  RowIdx r;
+  const int b = xcd_tile(bx, nb);
+  r.sim = g.block_sim[b];
+  r.local = (int64_t)(b - g.block_off[r.sim]) * blockDim.x + threadIdx.x;
+  r.row = g.sims[r.sim].row_off + r.local;
+  return r;
+}
+
 __device__ __forceinline__ RowIdx row_index(const RowGeom& g) {
   RowIdx r;
   const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
