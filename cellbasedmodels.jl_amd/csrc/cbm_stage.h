@@ -68,6 +68,12 @@ __device__ __forceinline__ int32_t neigh_cell(const SimDev& sd, int32_t cx, int3
 
 __host__ __device__ constexpr int pow3(int d) { return d == 1 ? 3 : (d == 2 ? 9 : 27); }
 
+// A law with a per-agent term: `void self(const double* x, double* f) const`
+// adds f (D values, zeroed before the call) to dt(x) at the stage state x,
+// the agentODE body's terms that are not pair forces.
+template <class L, class = void> struct has_self { static constexpr bool value = false; };
+template <class L> struct has_self<L, decltype((void)&L::self)> { static constexpr bool value = true; };
+
 // Walk the candidate neighbors of row i in reference loop order (Args: the
 // stage or rule arguments, with the list and cell tables of the last
 // neighbor update).
@@ -263,10 +269,19 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
       if (a.write_fnew) a.Fnew[c][i] = acc[c];
     }
   } else {
+    double fs[3] = {0.0, 0.0, 0.0};
+    if constexpr (has_self<Law>::value) {
+      double xs[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int c = 0; c < D; ++c) xs[c] = xi[c];
+      const Law law(sd.p);
+      law.self(xs, fs);
+    }
 #pragma unroll
     for (int c = 0; c < D; ++c) {
       double v = a.lagged ? a.F[c][i] : acc[c];
       if (a.drift != 0.0) v = v - a.drift * xi[c];
+      if constexpr (has_self<Law>::value) v = v + fs[c];
       du[c] = v;
       if (a.write_fnew) a.Fnew[c][i] = acc[c];
     }

@@ -209,7 +209,10 @@ const char* cbm_version(void);
  *     __device__ bool operator()(double d, double& s) const;  // s(d); true if
  *   };                                                // the pair interacts
  * The force on agent i is sum_j (s/d)(x_i - x_j) (s*((x_i - x_j)/d) per axis
- * when per_axis_division).  Compile errors return CBM_ERR_COMPILE with the
+ * when per_axis_division).  A UserLaw may also define
+ *     __device__ void self(const double* x, double* f) const;
+ * the agentODE body's per-agent terms: f (zeroed, D values) is added to
+ * dt(x) at the stage positions x, after the pair sum and `drift`.  Compile errors return CBM_ERR_COMPILE with the
  * hipRTC log in cbm_last_error. */
 int cbm_compile(cbm_ctx* ctx, const char* pair_body_hip);
 
