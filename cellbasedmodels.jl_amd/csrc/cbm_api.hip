@@ -1146,6 +1146,7 @@ static int check_flags(cbm_ctx* c) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (f[2]) return fail(c, CBM_ERR_LIST_OVERFLOW, "Verlet list overflow: raise nMaxNeighbors");
   if (f[3] & 2) return fail(c, CBM_ERR_CONFIG, "an agentRule body added more than 2 agents for one agent in one step");
+  if (f[3] & 4) return fail(c, CBM_ERR_CONFIG, "an agentRule body looped over neighbors without naming forNeighbors");
   if (f[3]) return fail(c, CBM_ERR_CAPACITY, "agent capacity exceeded: raise preallocation (n_max)");
   return CBM_OK;
 }

@@ -20,6 +20,7 @@ constexpr uint32_t RULE_PURPOSE = 0x600u; // Philox purpose word of the k-th dra
 // Rule error bits in RuleArgs::err.
 constexpr int32_t RULE_ERR_CAPACITY = 1;  // more agents than n_max
 constexpr int32_t RULE_ERR_ADDS = 2;      // more than RULE_MAX_ADD additions by one agent
+constexpr int32_t RULE_ERR_NBVIEW = 4;    // a neighbor loop without the neighbor view (no loop run)
 
 // The values of one agent a rule body reads and writes.
 struct RuleRec {
@@ -97,6 +98,8 @@ struct RuleAgent {
   // for every candidate of the neighbor structure, in its walk order
   template <class F>
   __device__ void forNeighbors(F&& f) const {
+    // the host sets up the view when the body's text names forNeighbors
+    if (!args->nb_vol) { atomicOr(args->err, RULE_ERR_NBVIEW); return; }
     const SimDev& sd = args->g.sims[sim];
     const int64_t N = args->g.st[sim].N;
     auto visit = [&](int64_t j) {
