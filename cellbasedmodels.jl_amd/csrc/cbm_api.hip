@@ -323,7 +323,7 @@ static int setup_bricks(cbm_ctx* c) {
     const int H = brick_halo_cells(D, bd[0], bd[1], bd[2]);
     const int64_t fixed = tile_bytes(0, H) + 16;
     const int64_t capmax = (budget - fixed) / per;
-    const double want = 1.3 * rho * H + 32.0;
+    const double want = 1.2 * rho * H + 32.0;
     if (want <= (double)capmax || m == mmin) {
       cap = (int)std::max<int64_t>(1, std::min<int64_t>(capmax, (int64_t)std::max(256.0, want)));
       break;
