@@ -1319,6 +1319,10 @@ static int xfer(cbm_ctx* c, int32_t field, void* host, int64_t n, int32_t sim, b
   if (field_col(c, field, k)) return fail(c, CBM_ERR_ARG, "field not present in this model");
   if (n < 0 || n > c->sims[sim].n_max) return fail(c, CBM_ERR_ARG, "n exceeds capacity");
   if (n == 0) return CBM_OK;
+  if (up && !(field == CBM_F_ID || field == CBM_F_LF || field == CBM_F_FATE))
+    for (int64_t i = 0; i < n; ++i)
+      if (!std::isfinite(((const double*)host)[i]))
+        return fail(c, CBM_ERR_NONFINITE, "non-finite value in an uploaded agent field");
   if (up && field == CBM_F_FATE)
     for (int64_t i = 0; i < n; ++i) {
       const int64_t f = ((const int64_t*)host)[i];
@@ -1376,6 +1380,8 @@ int cbm_upload_medium(cbm_ctx* c, const double* host, int64_t n) {
   if (!c) return CBM_ERR_ARG;
   if (!c->medium) return fail(c, CBM_ERR_CONFIG, "model has no medium");
   if (!host || n != c->med_nodes) return fail(c, CBM_ERR_ARG, "medium upload needs prod(NMedium) values");
+  for (int64_t i = 0; i < n; ++i)
+    if (!std::isfinite(host[i])) return fail(c, CBM_ERR_NONFINITE, "non-finite value in the uploaded medium");
   HIPCHK(c, hipMemcpyAsync(c->med_c[c->med_cur], host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   if (c->adi)   // DGADI holds the boundary at this field
     HIPCHK(c, hipMemcpyAsync(c->med_init, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));

@@ -38,7 +38,7 @@ enum cbm_status {
   CBM_ERR_COMPILE = -4,       /* hipRTC rejected a user interaction body    */
   CBM_ERR_HIP = -5,           /* HIP runtime error                          */
   CBM_ERR_ARG = -6,           /* bad argument (field id, size, sim index)   */
-  CBM_ERR_NONFINITE = -7      /* NaN/Inf found in agent state                */
+  CBM_ERR_NONFINITE = -7      /* NaN/Inf in an uploaded agent field or medium */
 };
 
 /* ---------------------------------------------------------- integrators */
@@ -297,7 +297,8 @@ int cbm_snapshot_get_medium(cbm_ctx* ctx, int32_t k, double* host, int64_t n);
 /* loadToPlatform! / bringFromPlatform! (communityStructure.jl:606-711).
  * `n` values in logical slot order for simulation `sim`.  Uploading CBM_F_X
  * of simulation `sim` does not change N; N is set at creation or with
- * cbm_set_n.  Int64 fields take/return int64_t, the others double. */
+ * cbm_set_n.  Int64 fields take/return int64_t, the others double; a NaN or
+ * infinite double is refused with CBM_ERR_NONFINITE (nothing uploaded). */
 int cbm_upload(cbm_ctx* ctx, int32_t field, const void* host, int64_t n, int32_t sim);
 int cbm_download(cbm_ctx* ctx, int32_t field, void* host, int64_t n, int32_t sim);
 int cbm_set_n(cbm_ctx* ctx, int32_t sim, int64_t n);

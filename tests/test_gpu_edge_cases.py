@@ -58,3 +58,17 @@ def test_batched_with_empty_and_single_simulations(gpu_available, nb, kw):
         alone.step(20)
         for a in range(2):
             assert np.array_equal(g.download(a, n, sim=s), alone.download(a, n))
+
+
+def test_non_finite_uploads_are_refused(gpu_available):
+    box = [[0.0, 20.0]] * 2
+    m = model(2, integrator=A.EULER, neighbors=A.CELL_LINKED, law=A.LAW_LINEAR, dt=0.01, cell_edge=2.0)
+    g = Context(m, [sim(5, box, [1.0, 2.0])])
+    for bad in (np.nan, np.inf, -np.inf):
+        x = np.full(5, 3.0)
+        x[2] = bad
+        with pytest.raises(A.CBMError) as e:
+            g.upload(A.F_X, x)
+        assert e.value.code == A.ERR_NONFINITE
+    g.upload(A.F_X, np.full(5, 3.0))   # the context stays usable
+    assert np.array_equal(g.download(A.F_X, 5), np.full(5, 3.0))
