@@ -429,7 +429,7 @@ def main():
                     help="first run a plain device-to-device copy of MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "bench.py measures one GPU; runs on several GPUs are not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

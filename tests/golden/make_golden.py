@@ -8,8 +8,8 @@ transcribed from the reference file named beside it (data only):
   - SDE variance KAT              test/testCommunity.jl:469-512
   - Aggregation printed outputs   docs/src/Aggregation.md:146-195, :243-279
   - Philox4x32-10 known answers   Random123 kat_vectors (Salmon et al., SC'11),
-    the generator the build uses for in-kernel noise (the reference's own RNG
-    streams are not reproducible)
+    the generator the library uses for in-kernel noise (the reference's own
+    Xoshiro and CUDA.jl device streams cannot be reproduced bit for bit)
 Run: python tests/golden/make_golden.py
 """
 import json
