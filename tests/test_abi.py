@@ -57,3 +57,23 @@ def test_capacity_below_n_is_rejected():
     h = C.c_void_p()
     assert A.lib().cbm_create(C.byref(h), C.byref(m), 1, C.byref(s)) == A.ERR_CONFIG
     A.lib().cbm_destroy(h)
+
+
+def _create(m, s):
+    h = C.c_void_p()
+    rc = A.lib().cbm_create(C.byref(h), C.byref(m), 1, C.byref(s))
+    A.lib().cbm_destroy(h)
+    return rc
+
+
+@pytest.mark.parametrize("n,alg", [((3, 8), A.MEDIUM_DGADI), ((8, 8), 7)])
+def test_bad_medium_is_rejected(n, alg):
+    m = model(2, neighbors=A.FULL)
+    m.medium_n[0], m.medium_n[1] = n
+    m.medium_alg = alg
+    assert _create(m, sim(4, [[0, 8]] * 2)) == A.ERR_CONFIG
+
+
+def test_unknown_rule_is_rejected():
+    m = model(2, neighbors=A.FULL, rule=9)
+    assert _create(m, sim(4, [[0, 8]] * 2)) == A.ERR_CONFIG
