@@ -207,7 +207,7 @@ class ABM:
 
     def __init__(self, dims, agentODE=None, agentSDE=0.0, agentRule=None, drift=0.0,
                  neighborsAlg=None, agentAlg=None, platform=None, periodic=False,
-                 lagged=False, seed=0, medium=None, agent=None):
+                 lagged=False, seed=0, medium=None, agent=None, deVariables=()):
         if dims not in (1, 2, 3):
             raise ValueError("dims must be 1, 2 or 3")
         self.dims = dims
@@ -224,8 +224,15 @@ class ABM:
         agent = dict(agent or {})
         if len(agent) > 4:
             raise ValueError("at most 4 user agent parameters")
-        if agent and self.rule_src is None:
-            raise ValueError("user agent parameters need an agentRule body")
+        de = tuple(deVariables)
+        if any(k not in agent for k in de):
+            raise ValueError("deVariables must name user agent parameters")
+        if de and self.agentODE.law != A.LAW_CUSTOM:
+            raise ValueError("deVariables need a custom agentODE body defining UserLaw::de")
+        if agent and self.rule_src is None and not de:
+            raise ValueError("user agent parameters need an agentRule body or deVariables")
+        agent = dict([(k, agent[k]) for k in de] + [(k, v) for k, v in agent.items() if k not in de])
+        self.user_de = len(de)
         for name, typ in agent.items():
             if typ is not float or name in _FIELDS or not name.isidentifier():
                 raise ValueError(f"user agent parameter {name!r} must be a new Float64 name")
@@ -254,6 +261,7 @@ class ABM:
         m.drift = self.drift
         m.noise = 0.0 if isinstance(self.agentSDE, str) else self.agentSDE
         m.seed = self.seed
+        m.user_de = self.user_de
         if self.agentRule is not None:
             m.rule = self.agentRule[0]
             for q, v in enumerate(self.agentRule[1]):
@@ -540,12 +548,14 @@ def loadToPlatform_(com, preallocateAgents=0):
     model = abm.model_desc(com.dt)
     sim = sim_desc(abm, com.N, com.N + int(preallocateAgents), com.simBox.reshape(-1).tolist(), com.law_p)
     com.ctx = Context(model, [sim])
-    law_src = abm.agentODE.source if abm.agentODE.law == A.LAW_CUSTOM else None
+    # user parameter names as indices of RuleAgent u[] and of UserLaw::de's u/du
+    names = "".join(f"constexpr int {n} = {f - A.F_U0};\n" for n, f in abm.user_fields.items())
+    names = "namespace agent {\n" + names + "}\n" if names else ""
+    law_src = names + abm.agentODE.source if abm.agentODE.law == A.LAW_CUSTOM else None
     noise_src = abm.agentSDE if isinstance(abm.agentSDE, str) else None
     rule_src = None
-    if abm.rule_src is not None:
-        names = "".join(f"constexpr int {n} = {f - A.F_U0};\n" for n, f in abm.user_fields.items())
-        rule_src = "namespace agent {\n" + names + "}\n" + abm.rule_src
+    if abm.rule_src is not None:   # one definition of the names per program
+        rule_src = (names if law_src is None else "") + abm.rule_src
     medium_src = abm.medium.get("mediumODE") if abm.medium is not None else None
     if law_src is not None or noise_src is not None or rule_src is not None or medium_src is not None:
         com.ctx.compile_model(law_src, noise_src, rule_src, medium_src)

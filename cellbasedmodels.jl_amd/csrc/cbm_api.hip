@@ -219,6 +219,8 @@ static int validate(cbm_ctx* c) {
   if (m.law == CBM_LAW_SOFT_SPHERES && m.rule != CBM_RULE_NONE)
     return fail(c, CBM_ERR_CONFIG, "the soft-sphere model takes no agent rules");
   if (!(m.dt > 0)) return fail(c, CBM_ERR_CONFIG, "dt must be positive");
+  if (m.user_de < 0 || m.user_de > 4 || (m.user_de > 0 && m.law != CBM_LAW_CUSTOM))
+    return fail(c, CBM_ERR_CONFIG, "user_de is 0..4 and needs a custom law (UserLaw::de)");
   if (m.neighbors == CBM_CELL_LINKED && !(m.cell_edge > 0)) return fail(c, CBM_ERR_CONFIG, "CellLinked needs cellEdge > 0");
   bool verlet = m.neighbors == CBM_VERLET_DISPLACEMENT || m.neighbors == CBM_VERLET_TIME || m.neighbors == CBM_CLVD;
   if (verlet && !(m.skin > 0)) return fail(c, CBM_ERR_CONFIG, "Verlet algorithms need skin > 0");
@@ -443,6 +445,8 @@ static int setup_blocks(cbm_ctx* c) {
   return CBM_OK;
 }
 
+static int statecol_k(const cbm_ctx* c, int v);
+
 static int alloc_column(cbm_ctx* c, int k, int bytes) {
   c->has_col[k] = true;
   c->col[k].bytes = bytes;
@@ -490,7 +494,7 @@ static int init_ctx(cbm_ctx* c) {
   c->growth = m.rule == CBM_RULE_GROWTH_DIVISION || m.rule == CBM_RULE_CUSTOM;
   c->rules = m.rule != CBM_RULE_NONE;
   c->inertial = m.law == CBM_LAW_SOFT_SPHERES;
-  c->nvar = c->inertial ? 2 * c->dims : c->dims + (c->growth ? 1 : 0);
+  c->nvar = c->inertial ? 2 * c->dims : c->dims + (c->growth ? 1 : 0) + c->m.user_de;
   c->lagged = m.lagged != 0;
   c->full = m.neighbors == CBM_FULL;
   c->cl = m.neighbors == CBM_CELL_LINKED;
@@ -543,10 +547,14 @@ static int init_ctx(cbm_ctx* c) {
       HIPCHK(c, hipMemcpy(c->col[C_FATE].buf[b], fate1.data(), sizeof(int64_t) * c->C, hipMemcpyHostToDevice));
     }
   }
+  // user agent parameters: read by a rule body, or DE variables of the law body
+  if (m.rule == CBM_RULE_CUSTOM || m.user_de > 0)
+    for (int k = 0; k < 4; ++k) if ((rc = alloc_column(c, C_U0 + k, 8))) return rc;
+  // every DE variable's column exists before any stage addresses it
+  for (int v = 0; v < c->nvar; ++v)
+    if (!c->has_col[statecol_k(c, v)]) return fail(c, CBM_ERR_CONFIG, "DE variable without a column");
   if (c->rules) {
     if ((rc = alloc_column(c, C_LF, 8))) return rc;
-    if (m.rule == CBM_RULE_CUSTOM)
-      for (int k = 0; k < 4; ++k) if ((rc = alloc_column(c, C_U0 + k, 8))) return rc;
     const int64_t nm = c->sims[0].n_max;
     HIPCHK(c, dalloc(&c->dead, c->C));
     HIPCHK(c, dalloc(&c->n_add_slot, nm + 1));
@@ -895,7 +903,8 @@ static int run_stage(cbm_ctx* c, StageArgs& a, int kind) {
 // Column of DE variable v: positions, then velocities (soft spheres) or vol.
 static int statecol_k(const cbm_ctx* c, int v) {
   if (v < c->dims) return C_X + v;
-  return c->inertial ? C_VX + (v - c->dims) : C_VOL;
+  if (c->inertial) return C_VX + (v - c->dims);
+  return c->growth && v == c->dims ? C_VOL : C_U0 + (v - c->dims - (c->growth ? 1 : 0));
 }
 
 static double* statecol(cbm_ctx* c, int v, bool alt) {
@@ -1189,8 +1198,14 @@ int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, 
   if (medium_body && !c->medium) return fail(c, CBM_ERR_CONFIG, "a mediumODE body needs a medium (medium_n)");
   if (!law_body && !noise_body && !rule_body && !medium_body) return fail(c, CBM_ERR_ARG, "nothing to compile");
   const bool stages = law_body || noise_body;
-  std::string src = "#define CBM_HIPRTC 1\n#include \"cbm_rule.h\"\n#include \"cbm_medium.h\"\nnamespace cbm {\n";
+  std::string src = "#define CBM_HIPRTC 1\n";
+  if (c->m.user_de > 0)
+    src += "#define CBM_USER_DE " + std::to_string(c->m.user_de) + "\n#define CBM_DE_BASE " +
+           std::to_string(c->dims + (c->growth ? 1 : 0)) + "\n";
+  src += "#include \"cbm_rule.h\"\n#include \"cbm_medium.h\"\nnamespace cbm {\n";
   if (law_body) { src += law_body; src += "\n"; }
+  if (law_body && c->m.user_de > 0)
+    src += "static_assert(has_de<UserLaw>::value, \"user_de > 0 needs UserLaw::de\");\n";
   if (noise_body) { src += noise_body; src += "\n"; }
   if (rule_body) { src += rule_body; src += "\n"; }
   if (medium_body) { src += medium_body; src += "\n"; }
@@ -1306,7 +1321,7 @@ static int field_col(cbm_ctx* c, int32_t field, int& k) {
     case CBM_F_MASS: if (!c->inertial) return CBM_ERR_ARG; k = C_MASS; return CBM_OK;
     case CBM_F_FATE: if (!c->inertial) return CBM_ERR_ARG; k = C_FATE; return CBM_OK;
     case CBM_F_U0: case CBM_F_U1: case CBM_F_U2: case CBM_F_U3:
-      if (c->m.rule != CBM_RULE_CUSTOM) return CBM_ERR_ARG;
+      if (!c->has_col[C_U0]) return CBM_ERR_ARG;
       k = C_U0 + (field - CBM_F_U0); return CBM_OK;
     default: return CBM_ERR_ARG;
   }
@@ -1757,7 +1772,7 @@ int cbm_adjoint(cbm_ctx* c, const cbm_loss_desc* L, double* grad, double* loss) 
   if (!c || !L || !grad || !loss) return CBM_ERR_ARG;
   const cbm_model_desc& m = c->m;
   if (m.integrator != CBM_EULER || m.law != CBM_LAW_REP_ADH || !c->verlet || c->vtime || c->rules ||
-      c->medium || c->lagged || m.drift != 0.0)
+      c->medium || c->lagged || m.drift != 0.0 || m.user_de > 0)
     return fail(c, CBM_ERR_CONFIG, "the adjoint covers overdamped Euler with the REP_ADH law on Verlet lists (no rules, medium, lag or drift)");
   const int64_t T = L->n_steps, K = L->checkpoint_every;
   const int S = L->n_saves;

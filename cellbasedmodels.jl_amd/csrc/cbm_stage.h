@@ -73,6 +73,15 @@ __host__ __device__ constexpr int pow3(int d) { return d == 1 ? 3 : (d == 2 ? 9 
 // the agentODE body's terms that are not pair forces.
 template <class L, class = void> struct has_self { static constexpr bool value = false; };
 template <class L> struct has_self<L, decltype((void)&L::self)> { static constexpr bool value = true; };
+template <class L, class = void> struct has_de { static constexpr bool value = false; };
+template <class L> struct has_de<L, decltype((void)&L::de)> { static constexpr bool value = true; };
+
+// A program around a body with user DE variables defines CBM_USER_DE (their
+// count) and CBM_DE_BASE (the DE index of the first: dims, + 1 with vol).
+#ifndef CBM_USER_DE
+#define CBM_USER_DE 0
+#define CBM_DE_BASE 0
+#endif
 
 // Walk the candidate neighbors of row i in reference loop order (Args: the
 // stage or rule arguments, with the list and cell tables of the last
@@ -256,7 +265,9 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
   }
 
   // derivative of each DE variable
-  double du[MAX_STATE];
+  constexpr bool UDE = CBM_USER_DE > 0 && has_de<Law>::value;
+  constexpr int NS = UDE ? MAX_STATE : BUILTIN_STATE;
+  double du[NS];
   if constexpr (SOFT) {
     // dt(v) = -b*v/m + f/m, dt(x) = v (src/models/softSpheres.jl:32-35)
     const double b = sd.p[0], m = a.mass[i];
@@ -286,12 +297,24 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
       if (a.write_fnew) a.Fnew[c][i] = acc[c];
     }
   }
-  const bool grow = !SOFT && a.nvar > D;
+  const bool grow = UDE ? CBM_DE_BASE > D : !SOFT && a.nvar > D;
   if (grow) du[D] = a.growth[i] * a.S[D][i];
+  if constexpr (UDE) {
+    // dt of the user DE variables at the stage state
+    double xs[3] = {0.0, 0.0, 0.0}, us[4] = {0.0, 0.0, 0.0, 0.0}, ds[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < D; ++c) xs[c] = xi[c];
+#pragma unroll
+    for (int k = 0; k < CBM_USER_DE; ++k) us[k] = a.S[CBM_DE_BASE + k][i];
+    const Law law(sd.p);
+    law.de(xs, us, ds);
+#pragma unroll
+    for (int k = 0; k < CBM_USER_DE; ++k) du[CBM_DE_BASE + k] = ds[k];
+  }
 
   const double dt = a.dt;
-  const int nv = SOFT ? 2 * D : (grow ? D + 1 : D);
-  double res[MAX_STATE];
+  const int nv = UDE ? CBM_DE_BASE + CBM_USER_DE : (SOFT ? 2 * D : (grow ? D + 1 : D));
+  double res[NS];
   if (NOISE && !same_type<Noise, ConstNoise>::value) {
     // a user agentSDE body: g of every DE variable at the stage state, one
     // normal per variable (pairs keyed pur, pur + 1, pur + 2); the
@@ -299,18 +322,18 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
     const int64_t id = a.id[i];
     const int64_t step = *a.step_ptr;
     const uint32_t pur = a.kind == ST_EH2 ? 0x500u : 0x100u;
-    double z[MAX_STATE], us[MAX_STATE];
+    double z[NS], us[NS];
 #pragma unroll
-    for (int k = 0; k < MAX_STATE / 2; ++k) {
+    for (int k = 0; k < NS / 2; ++k) {
       z[2 * k] = 0.0;
       z[2 * k + 1] = 0.0;
       if (2 * k < nv) normal2(draw(a.seed, step, id, sim, pur + (uint32_t)k), z[2 * k], z[2 * k + 1]);
     }
 #pragma unroll
-    for (int v = 0; v < MAX_STATE; ++v) us[v] = v < nv ? Sc(v) : 0.0;
+    for (int v = 0; v < NS; ++v) us[v] = v < nv ? Sc(v) : 0.0;
     const Noise gfn(sd.p);
 #pragma unroll
-    for (int v = 0; v < MAX_STATE; ++v) {
+    for (int v = 0; v < NS; ++v) {
       if (v >= nv) continue;
       const double g = gfn(v, us);
       const double u = v < D ? Uc(v) : a.U[v][i];

@@ -18,7 +18,8 @@ using __hip_internal::uint16_t;
 namespace cbm {
 
 constexpr int MAX_COLS = 20;      // permutable per-agent columns
-constexpr int MAX_STATE = 6;      // DE variables: 3 positions + vol, or 3 positions + 3 velocities
+constexpr int MAX_STATE = 8;      // DE variables: 3 positions + vol + 4 user, or 3 positions + 3 velocities
+constexpr int BUILTIN_STATE = 6;  // without user DE variables
 
 // Per-simulation constants (device array, one entry per simulation).
 // Row space: simulation s owns rows [row_off, row_off + n_max).

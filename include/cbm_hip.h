@@ -173,6 +173,10 @@ typedef struct cbm_model_desc {
   double medium_D;       /* diffusion coefficient                            */
   double medium_consume; /* uptake rate per agent in a node: dc/dt -= rate*c*n */
   double medium_c0;      /* initial concentration                            */
+  int32_t user_de;       /* U0..U(user_de-1) are DE variables of the agentODE
+                            body (UserLaw::de, dt(u) of functionDE.jl:13-116);
+                            0..4, needs CBM_LAW_CUSTOM, no soft spheres     */
+  int32_t pad1;
 } cbm_model_desc;
 
 /* Per-simulation description.  One context may hold many independent
@@ -212,7 +216,12 @@ const char* cbm_version(void);
  * when per_axis_division).  A UserLaw may also define
  *     __device__ void self(const double* x, double* f) const;
  * the agentODE body's per-agent terms: f (zeroed, D values) is added to
- * dt(x) at the stage positions x, after the pair sum and `drift`.  Compile errors return CBM_ERR_COMPILE with the
+ * dt(x) at the stage positions x, after the pair sum and `drift`.  With
+ * user_de > 0 it defines
+ *     __device__ void de(const double* x, const double* u, double* du) const;
+ * dt of the user DE variables: du[k] (zeroed) = dt(U_k) at the stage
+ * positions x and stage values u[k] of U_k, k < user_de, integrated by the
+ * model's integrator beside the positions (one normal each under EM).  Compile errors return CBM_ERR_COMPILE with the
  * hipRTC log in cbm_last_error. */
 int cbm_compile(cbm_ctx* ctx, const char* pair_body_hip);
 

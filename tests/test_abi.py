@@ -77,3 +77,10 @@ def test_bad_medium_is_rejected(n, alg):
 def test_unknown_rule_is_rejected():
     m = model(2, neighbors=A.FULL, rule=9)
     assert _create(m, sim(4, [[0, 8]] * 2)) == A.ERR_CONFIG
+
+
+@pytest.mark.parametrize("law,k", [(A.LAW_CUSTOM, 5), (A.LAW_LINEAR, 1), (A.LAW_CUSTOM, -1)])
+def test_bad_user_de_is_rejected(law, k):
+    m = model(2, neighbors=A.FULL, law=law)
+    m.user_de = k
+    assert _create(m, sim(4, [[0, 8]] * 2)) == A.ERR_CONFIG
