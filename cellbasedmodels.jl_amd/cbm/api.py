@@ -196,6 +196,8 @@ class ABM:
                 UserRule (optionally as (body, rule_p)), compiled with hipRTC
     agent       user agent parameters {name: float} (at most 4) for a UserRule
                 body: Community attributes, u[agent::name] in the body
+    deVariables names of user agent parameters that are DE variables of a custom
+                agentODE body (UserLaw::de; the dt(var) lines of functionDE.jl)
     medium      {"NMedium": [...], "D", "consume", "c0"} for the built-in
                 diffusion with uptake; "mediumODE": a HIP C++ body defining
                 struct UserMedium (dt(c) at a node) replaces that law;

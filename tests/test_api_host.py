@@ -43,3 +43,14 @@ def test_medium_and_noise_options():
     assert m.medium_alg == A.MEDIUM_DGADI and list(m.medium_n)[:2] == [16, 8]
     assert m.noise == 0.0 and m.integrator == A.EM and m.neighbors == A.CELL_LINKED
     assert ABM(2, medium={"NMedium": [8, 8]}).model_desc(0.1).medium_alg == A.MEDIUM_FTCS
+
+
+def test_de_variables_come_first_and_set_user_de():
+    law = CBMModels.custom("struct UserLaw {};", [0.5])
+    abm = ABM(2, agentODE=law, agent={"a": float, "age": float}, deVariables=("age",))
+    assert abm.user_fields == {"age": A.F_U0, "a": A.F_U1}
+    assert abm.model_desc(0.1).user_de == 1 and abm.model_desc(0.1).law == A.LAW_CUSTOM
+    with pytest.raises(ValueError):
+        ABM(2, agent={"age": float}, deVariables=("age",))     # built-in law: no de body
+    with pytest.raises(ValueError):
+        ABM(2, agentODE=law, agent={"a": float}, deVariables=("b",))
