@@ -75,6 +75,13 @@ def workload(kind, n):
     return m, s, pos, L
 
 
+# Cell-linked rebuild of every step, per agent (D = 3): cell key from the
+# positions (8D), key write and read (8), the sort's passes over (key, row)
+# pairs (16 x 3), the permutation of the positions (16D) and the cell starts (8):
+# 24 D + 64 bytes.
+CL_REBUILD_BYTES = 24 * 3 + 64
+
+
 def stage_bytes(kind, kbar):
     """Algorithmic HBM bytes per agent for each stage kernel (FP64 SoA, int32 ELL list)."""
     lst = 4.0 + 4.0 * kbar                      # list length + entries
@@ -84,6 +91,78 @@ def stage_bytes(kind, kbar):
         return {2: 24 + lst + 24,               # stage 1: read u, list; write h
                 3: 24 + 24 + lst + 24 + 24}     # stage 2: read h, u, list, x_ref; write u
     return {1: 24 + lst + 8 + 24 + 24}          # EM: read u, list, id, x_ref; write u
+
+
+def measure(kind, n, steps, warmup, graph, profile_steps):
+    """The C2 model (Verlet "heun"/"em", or cell-linked "heun_cl") at N = n:
+    agent-steps/s over `steps` timed steps after `warmup`, the dominant stage
+    kernel's roofline from HIP events on the library stream, and the whole
+    step against its algorithmic bytes."""
+    from cbm.api import Context
+    m, s, pos, L = workload(kind, n)
+    ctx = Context(m, [s])
+    for a in range(3):
+        ctx.upload(a, pos[a])
+    ctx.neighbors()
+    ctx.step(warmup)
+    ctx.sync()
+    rb0 = ctx.rebuilds()
+
+    t0 = time.perf_counter()
+    if graph:
+        ctx.step_graph(graph, steps // graph)
+        steps = (steps // graph) * graph
+    else:
+        ctx.step(steps)
+    ctx.sync()
+    t1 = time.perf_counter()
+    dev_ms, _, _ = ctx.last_timing()
+    rebuilds = ctx.rebuilds() - rb0
+    wall = t1 - t0
+
+    # per-kernel times from HIP events on the library stream
+    ms, nl = ctx.profile(profile_steps)
+    kbar = float(ctx.count_neighbors(0.0).mean()) if kind != "heun_cl" else 0.0
+    sb = stage_bytes(kind, kbar)
+    dom = max(sb, key=lambda k: ms[k])
+    achieved = sb[dom] * s.n / (ms[dom] * 1e-3) / 1e9
+    step_bytes = sum(sb.values()) * s.n
+    if kind == "heun_cl":
+        step_bytes += CL_REBUILD_BYTES * s.n
+    dom_name = {2: "k_stage Heun stage 1", 3: "k_stage Heun stage 2", 1: "k_stage EM"}[dom]
+    if kind == "heun_cl":
+        dom_name = dom_name.replace("k_stage", "k_brick")
+    ctx.close()
+    return {
+        "value": s.n * steps / wall,
+        "steps": steps,
+        "ms_per_step": wall / steps * 1e3,
+        "config": {"workload": ("C2 3-D cell mechanics, HEUN, cell-linked (edge 2.4, rebuilt every step)"
+                                if kind == "heun_cl" else
+                                f"C2 3-D cell mechanics, {kind.upper()}, Verlet r_list 2.7, rebuild at 0.15"),
+                   "N": s.n, "box": L, "dt": m.dt, "graph_steps": graph},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(dom_name, s.n) if kind in ("heun", "heun_cl") else None,
+            "alg_bytes_per_launch": sb[dom] * s.n,
+            "avg_launch_ms": ms[dom],
+            "mean_list_len": kbar,
+        },
+        "step_roofline": {
+            "alg_bytes_per_step": step_bytes,
+            "alg_bytes_per_agent_step": step_bytes / s.n,
+            "achieved_GBs": step_bytes / (wall / steps) / 1e9,
+            "frac": step_bytes / (wall / steps) / 1e9 / HBM_PEAK_GBS,
+            "neighbor_update_ms": ms[8],
+            "device_ms_per_step": dev_ms / steps if not graph else None,
+            "rebuilds_in_timed_steps": rebuilds,
+        },
+    }
 
 
 def cpu_baseline(m, s, pos, budget_s):
@@ -417,6 +496,8 @@ def main():
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--workload", choices=["heun", "em", "heun_cl"], default="heun")
     ap.add_argument("--graph", type=int, default=0, help="capture this many steps per hipGraph replay")
+    ap.add_argument("--cl", type=int, default=500, metavar="STEPS",
+                    help="also time the same model on a cell-linked grid rebuilt every step (0 skips)")
     ap.add_argument("--em", type=int, default=1, help="also time the C2 Euler-Maruyama variant")
     ap.add_argument("--c1", type=int, default=1, help="also time C1 on the device and the oracle")
     ap.add_argument("--no-cpu", action="store_true")
@@ -442,84 +523,33 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    from cbm.api import Context
     print(f"bench: C2 {args.workload} N={args.n}", file=sys.stderr, flush=True)
-    m, s, pos, L = workload(args.workload, args.n)
-    ctx = Context(m, [s])
-    for a in range(3):
-        ctx.upload(a, pos[a])
-    ctx.neighbors()
-    ctx.step(args.warmup)
-    ctx.sync()
-    rb0 = ctx.rebuilds()
-
-    t0 = time.perf_counter()
-    if args.graph:
-        ctx.step_graph(args.graph, args.steps // args.graph)
-        steps = (args.steps // args.graph) * args.graph
-    else:
-        ctx.step(args.steps)
-        steps = args.steps
-    ctx.sync()
-    t1 = time.perf_counter()
-    dev_ms, _, _ = ctx.last_timing()
-    rebuilds = ctx.rebuilds() - rb0
-    wall = t1 - t0
-    value = s.n * steps / wall
-
-    # per-kernel times from HIP events on the library stream
-    ms, nl = ctx.profile(args.profile_steps)
-    lens = ctx.count_neighbors(0.0)
-    kbar = float(lens.mean())
-    sb = stage_bytes(args.workload, kbar)
-    dom = max(sb, key=lambda k: ms[k])
-    achieved = sb[dom] * s.n / (ms[dom] * 1e-3) / 1e9
-    step_bytes = sum(sb.values()) * s.n
-    dom_name = {2: "k_stage Heun stage 1", 3: "k_stage Heun stage 2", 1: "k_stage EM"}[dom]
-    if args.workload == "heun_cl":
-        dom_name = dom_name.replace("k_stage", "k_brick")
+    r = measure(args.workload, args.n, args.steps, args.warmup, args.graph, args.profile_steps)
     rec = {
         "metric": "agent-steps/s @1M 3D",
-        "value": value,
+        "value": r["value"],
         "unit": "agent-steps/s",
         "n_gpus": 1,
-        "steps": steps,
+        "steps": r["steps"],
         "warmup": args.warmup,
-        "ms_per_step": wall / steps * 1e3,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": ("C2 3-D cell mechanics, HEUN, cell-linked (edge 2.4, rebuilt every step)"
-                                if args.workload == "heun_cl" else
-                                f"C2 3-D cell mechanics, {args.workload.upper()}, Verlet r_list 2.7, rebuild at 0.15"),
-                   "N": s.n, "box": L, "dt": m.dt, "graph_steps": args.graph},
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom_name,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom_name, s.n) if args.workload in ("heun", "heun_cl") else None,
-            "alg_bytes_per_launch": sb[dom] * s.n,
-            "avg_launch_ms": ms[dom],
-            "mean_list_len": kbar,
-        },
-        "step_roofline": {
-            "alg_bytes_per_step": step_bytes,
-            "achieved_GBs": step_bytes / (wall / steps) / 1e9,
-            "frac": step_bytes / (wall / steps) / 1e9 / HBM_PEAK_GBS,
-            "neighbor_update_ms": ms[8],
-            "device_ms_per_step": dev_ms / steps if not args.graph else None,
-            "rebuilds_in_timed_steps": rebuilds,
-        },
+        "config": r["config"],
+        "roofline": r["roofline"],
+        "step_roofline": r["step_roofline"],
     }
-    ctx.close()
+    m, s, pos, L = workload(args.workload, args.n)
+
     def phase(name):
         print(f"bench: {name}", file=sys.stderr, flush=True)
 
+    if args.cl and args.workload != "heun_cl":
+        phase("C2 on a cell-linked grid rebuilt every step")
+        rec["cl"] = measure("heun_cl", args.n, args.cl, args.warmup, 0, args.profile_steps)
     if args.em:
         phase("C2 Euler-Maruyama variant")
         rec["em"] = run_em(args.n)

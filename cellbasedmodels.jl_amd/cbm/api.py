@@ -227,6 +227,8 @@ class ABM:
         if len(agent) > 4:
             raise ValueError("at most 4 user agent parameters")
         de = tuple(deVariables)
+        if len(set(de)) != len(de):
+            raise ValueError("deVariables names a parameter more than once")
         if any(k not in agent for k in de):
             raise ValueError("deVariables must name user agent parameters")
         if de and self.agentODE.law != A.LAW_CUSTOM:

@@ -936,8 +936,11 @@ static RuleArgs rule_args(cbm_ctx* c, int32_t* flag) {
   if (c->lagged) for (int d = 0; d < c->dims; ++d) { r.F[d] = (double*)c->col[C_FX + d].cur(); r.Fnew[d] = c->Fnew[d]; }
   r.id = (int64_t*)c->col[C_ID].cur();
   r.lf = (int64_t*)c->col[C_LF].cur();
-  if (c->m.rule == CBM_RULE_CUSTOM)
-    for (int k = 0; k < 4; ++k) r.u[k] = (double*)c->col[C_U0 + k].cur();
+  if (c->has_col[C_U0])
+    for (int k = 0; k < 4; ++k) {
+      r.u[k] = (double*)c->col[C_U0 + k].cur();
+      r.u_alt[k] = (double*)c->col[C_U0 + k].alt();
+    }
   r.list = c->list; r.list_n = c->list_n; r.ldl = c->C;
   r.cell_row = c->cell_row; r.cell_range = c->cell_range;
   r.nb_kind = c->nb_kind;
@@ -1263,7 +1266,12 @@ int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, 
   std::vector<char> code(code_n);
   hiprtcGetCode(prog, code.data());
   hiprtcDestroyProgram(&prog);
-  if (c->user_mod) { (void)hipModuleUnload(c->user_mod); c->user_mod = nullptr; }
+  if (c->user_mod) {
+    // steps queued on the stream may still run kernels of the old module
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipModuleUnload(c->user_mod);
+    c->user_mod = nullptr;
+  }
   for (int k = 0; k < 2; ++k) { c->user_fn[k] = nullptr; c->user_brick[k] = nullptr; c->user_rule[k] = nullptr; }
   c->user_medium = nullptr;
   HIPCHK(c, hipModuleLoadData(&c->user_mod, code.data()));

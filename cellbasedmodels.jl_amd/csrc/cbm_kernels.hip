@@ -602,6 +602,15 @@ __global__ __launch_bounds__(256) void k_verlet_brick(VerletArgs va) {
 // apply passes agree and the result is independent of thread order.
 struct RuleDecision { int32_t add; int32_t dead; };
 
+// A daughter's user parameters: the parent's (@addAgent copies every
+// parameter it is not given, src/AgentStructure/functionRule.jl:46-55), in
+// both buffers, since user DE variables flip with the positions.
+__device__ __forceinline__ void rule_copy_u(const RuleArgs& a, int64_t r, int64_t dst) {
+  if (!a.u[0]) return;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) { const double v = a.u[q][r]; a.u[q][dst] = v; a.u_alt[q][dst] = v; }
+}
+
 __device__ __forceinline__ RuleDecision rule_decide(const RuleArgs& a, int sim, int64_t r, int64_t step) {
   RuleDecision d{0, 0};
   const int64_t id = a.id[r];
@@ -670,6 +679,7 @@ __global__ __launch_bounds__(256) void k_rule_apply(RuleArgs a) {
       if (a.lagged) for (int c = 0; c < D; ++c) { a.F[c][dst] = a.F[c][r]; a.Fnew[c][dst] = a.F[c][r]; }
       a.id[dst] = st.id_max + 1 + o;
       a.lf[dst] = a.id[r];
+      rule_copy_u(a, r, dst);
       a.slot[dst] = (int32_t)nw;
       a.dead[dst] = 0;
     }
@@ -711,6 +721,7 @@ __global__ __launch_bounds__(256) void k_rule_apply(RuleArgs a) {
     a.gr_new[r] = g1; a.gr_new[dst] = g2; ((double*)a.gr)[dst] = g2;
     a.id[dst] = st.id_max + 1 + o;
     a.lf[dst] = a.lf[r];
+    rule_copy_u(a, r, dst);
     a.slot[dst] = (int32_t)nw;
     a.dead[dst] = 0;
   }

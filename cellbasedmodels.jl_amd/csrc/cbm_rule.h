@@ -230,7 +230,7 @@ __device__ void user_rule_apply(const RuleArgs& a) {
     ((double*)a.gr)[dst] = d.growth; a.gr_new[dst] = d.growth;
     ((double*)a.vb)[dst] = d.vbirth; a.vb_new[dst] = d.vbirth;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) a.u[q][dst] = d.u[q];
+    for (int q = 0; q < 4; ++q) { a.u[q][dst] = d.u[q]; a.u_alt[q][dst] = d.u[q]; }
     a.lf[dst] = d.lf;
     a.id[dst] = st.id_max + 1 + o + k;
     a.slot[dst] = (int32_t)nw;

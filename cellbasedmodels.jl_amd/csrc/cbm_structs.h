@@ -202,7 +202,9 @@ struct RuleArgs {
   double* Fnew[3];
   int64_t* id;
   int64_t* lf;
-  double* u[4];                 // user agent parameters (CBM_RULE_CUSTOM), written in place
+  double* u[4];                 // user agent parameters (user rules, user DE variables), written in place
+  double* u_alt[4];             // their other buffers: new agents are written to both (user DE
+                                // variables flip with the positions under Euler and EM)
   int32_t* slot;
   uint8_t* dead;                // per row
   int32_t* n_add_slot;          // (slot) daughters of this agent, consumed by the scan

@@ -54,3 +54,5 @@ def test_de_variables_come_first_and_set_user_de():
         ABM(2, agent={"age": float}, deVariables=("age",))     # built-in law: no de body
     with pytest.raises(ValueError):
         ABM(2, agentODE=law, agent={"a": float}, deVariables=("b",))
+    with pytest.raises(ValueError, match="more than once"):
+        ABM(2, agentODE=law, agent={"a": float, "b": float}, deVariables=("a", "a"))
