@@ -85,8 +85,12 @@ CL_REBUILD_BYTES = 24 * 3 + 64
 def stage_bytes(kind, kbar):
     """Algorithmic HBM bytes per agent for each stage kernel (FP64 SoA, int32 ELL list)."""
     lst = 4.0 + 4.0 * kbar                      # list length + entries
-    if kind == "heun_cl":                       # cell-linked: read the row's cell, no list, no x_ref
-        return {2: 24 + 4 + 24, 3: 24 + 24 + 4 + 24}
+    if kind == "heun_cl":
+        # CellLinked on a skin list: stage 1 reads u, the skin list, the
+        # agent's cell and x_ref (its distance from the list's positions) and
+        # writes h and the ordered stage-2 list; stage 2 reads h, u, that
+        # list and x_ref and writes u' and the next cell
+        return {2: 24 + lst + 4 + 24 + 24 + lst, 3: 24 + 24 + lst + 24 + 24 + 4}
     if kind == "heun":
         return {2: 24 + lst + 24,               # stage 1: read u, list; write h
                 3: 24 + 24 + lst + 24 + 24}     # stage 2: read h, u, list, x_ref; write u
@@ -122,16 +126,17 @@ def measure(kind, n, steps, warmup, graph, profile_steps):
 
     # per-kernel times from HIP events on the library stream
     ms, nl = ctx.profile(profile_steps)
-    kbar = float(ctx.count_neighbors(0.0).mean()) if kind != "heun_cl" else 0.0
+    # mean list length: the Verlet list, or the skin list of the cell-linked
+    # model (its candidates within the list radius, reach 2.4 x 1.125)
+    kbar = float(ctx.count_neighbors(2.4 * 1.125 if kind == "heun_cl" else 0.0).mean())
     sb = stage_bytes(kind, kbar)
     dom = max(sb, key=lambda k: ms[k])
     achieved = sb[dom] * s.n / (ms[dom] * 1e-3) / 1e9
-    step_bytes = sum(sb.values()) * s.n
-    if kind == "heun_cl":
-        step_bytes += CL_REBUILD_BYTES * s.n
-    dom_name = {2: "k_stage Heun stage 1", 3: "k_stage Heun stage 2", 1: "k_stage EM"}[dom]
-    if kind == "heun_cl":
-        dom_name = dom_name.replace("k_stage", "k_brick")
+    # the skin-list rebuilds, amortised: the cell sort (24 D + 64), the list
+    # (4 + 4 kbar) and x_ref (8 D) per agent and rebuild
+    rebuild_bytes = (CL_REBUILD_BYTES + 4 + 4 * kbar + 24) * rebuilds / steps
+    step_bytes = (sum(sb.values()) + rebuild_bytes) * s.n
+    dom_name = {2: "k_flat Heun stage 1", 3: "k_flat Heun stage 2", 1: "k_flat EM"}[dom]
     ctx.close()
     return {
         "value": s.n * steps / wall,
@@ -162,6 +167,12 @@ def measure(kind, n, steps, warmup, graph, profile_steps):
             "device_ms_per_step": dev_ms / steps if not graph else None,
             "rebuilds_in_timed_steps": rebuilds,
         },
+        # the same time against the bytes of a cell-linked step that sorts
+        # the grid every step (both stages, 52 + 76 B, plus 24 D + 64 B)
+        **({"step_roofline_grid_sorted_every_step": {
+            "alg_bytes_per_agent_step": 52 + 76 + CL_REBUILD_BYTES,
+            "frac": (52 + 76 + CL_REBUILD_BYTES) * s.n / (wall / steps) / 1e9 / HBM_PEAK_GBS}}
+           if kind == "heun_cl" else {}),
     }
 
 

@@ -37,7 +37,7 @@ class ModelDesc(C.Structure):
         ("rule_p", C.c_double * 4), ("seed", C.c_uint64),
         ("medium_n", C.c_int32 * 3), ("medium_alg", C.c_int32),
         ("medium_D", C.c_double), ("medium_consume", C.c_double), ("medium_c0", C.c_double),
-        ("user_de", C.c_int32), ("pad1", C.c_int32),
+        ("user_de", C.c_int32), ("walk", C.c_int32),
     ]
 
 

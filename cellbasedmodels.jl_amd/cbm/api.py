@@ -471,9 +471,9 @@ class Context:
 
     def profile(self, nsteps):
         """Per-kernel average device times (ms) over nsteps steps, see cbm_profile."""
-        ms = np.zeros(12)
-        nl = np.zeros(12, dtype=np.int64)
-        self._chk(self.lib.cbm_profile(self.h, int(nsteps), ms.ctypes.data, nl.ctypes.data, 12))
+        ms = np.zeros(16)
+        nl = np.zeros(16, dtype=np.int64)
+        self._chk(self.lib.cbm_profile(self.h, int(nsteps), ms.ctypes.data, nl.ctypes.data, 16))
         return ms, nl
 
     def set_block(self, block):

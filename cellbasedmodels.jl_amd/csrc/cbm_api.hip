@@ -37,6 +37,7 @@ enum ColId {
   C_ID, C_SLOT, C_FX, C_FY, C_FZ, C_GR, C_VB, C_LF,
   C_RAD, C_MASS, C_FATE,                // per-agent soft-sphere properties
   C_U0, C_U1, C_U2, C_U3,               // user agent parameters (CBM_RULE_CUSTOM)
+  C_CCELL,                              // CellLinked on a skin list: packed cell of the step start (int32)
   C_NCOLS
 };
 
@@ -47,6 +48,16 @@ struct cbm_ctx {
   int dims = 1, nvar = 1;
   bool growth = false, lagged = false, verlet = false, cl = false, full = false, vtime = false, rules = false;
   bool inertial = false;                  // CBM_LAW_SOFT_SPHERES: velocities are DE variables
+  // CellLinked executed on a skin list (cl && clv): no per-step cell sort;
+  // each stage orders an agent's list entries by its cell among the 3^D
+  // neighbor cells (the reference loop order) from the agents' packed cells
+  bool clv = false;
+  double clv_reach = 0.0, clv_skin = 0.0;  // largest law reach at creation, list margin
+  unsigned long long* emax = nullptr;       // max |h - x_ref|^2 of stage 1 (bit pattern)
+  int32_t* wlist = nullptr;                 // the step's lists in cell-walk order (clv)
+  int32_t* wlist_n = nullptr;
+  int32_t* fix_rows = nullptr;              // agents deferred to the exact walk, and
+  int32_t* fix_n = nullptr;                 // their counts per stage
   int nb_kind = NB_FULL;
   int block = 256;
   int device = 0;
@@ -219,6 +230,8 @@ static int validate(cbm_ctx* c) {
   if (m.law == CBM_LAW_SOFT_SPHERES && m.rule != CBM_RULE_NONE)
     return fail(c, CBM_ERR_CONFIG, "the soft-sphere model takes no agent rules");
   if (!(m.dt > 0)) return fail(c, CBM_ERR_CONFIG, "dt must be positive");
+  if (m.walk != 0 && m.walk != 1 && (m.walk < 512 || m.walk > 2048))
+    return fail(c, CBM_ERR_CONFIG, "walk is 0, 1 or a round of 512..2048 list entries");
   if (m.user_de < 0 || m.user_de > 4 || (m.user_de > 0 && m.law != CBM_LAW_CUSTOM))
     return fail(c, CBM_ERR_CONFIG, "user_de is 0..4 and needs a custom law (UserLaw::de)");
   if (m.neighbors == CBM_CELL_LINKED && !(m.cell_edge > 0)) return fail(c, CBM_ERR_CONFIG, "CellLinked needs cellEdge > 0");
@@ -312,9 +325,10 @@ static int setup_bricks(cbm_ctx* c) {
   // LDS bytes per staged agent and in all of a tile: the Verlet list build
   // stages positions and slots, the cell-linked stage positions (and radius
   // and fate for soft spheres); both add an owner and a home-row index
-  const int64_t per = c->verlet ? 8 * D + 4 + 4 : 8 * (D + (soft ? 1 : 0)) + (soft ? 4 : 0) + 4;
+  const bool build = c->verlet || c->clv;   // the bricks serve the list build
+  const int64_t per = build ? 8 * D + 4 + 4 : 8 * (D + (soft ? 1 : 0)) + (soft ? 4 : 0) + 4;
   auto tile_bytes = [&](int cp, int H) {
-    return c->verlet ? vbrick_lds_bytes(D, cp, H) : brick_lds_bytes(D, soft, cp, H);
+    return build ? vbrick_lds_bytes(D, cp, H) : brick_lds_bytes(D, soft, cp, H);
   };
   int bd[3] = {1, 1, 1};
   int cap = 0;
@@ -376,12 +390,63 @@ static int setup_bricks(cbm_ctx* c) {
   return CBM_OK;
 }
 
+// Reach of a built-in law from a simulation's parameters: no pair at
+// d >= reach interacts (the PairLaw::reach of the kernels).
+static double law_reach(int law, const double* p) {
+  switch (law) {
+    case CBM_LAW_LINEAR: return p[1];
+    case CBM_LAW_REP_ADH: return p[3] * p[1];
+    case CBM_LAW_AGGREGATION: return std::max(p[0], p[2]);
+    default: return std::numeric_limits<double>::infinity();
+  }
+}
+
+// CellLinked on a skin list: for a built-in law with a reach, Euler or Heun,
+// no rules, and grids the packed cells can hold.  The list radius is the
+// largest reach plus a skin of reach/8, rebuilt when an agent has moved
+// half the skin; the list grid has cells of at least the list radius.
+static void choose_clv(cbm_ctx* c) {
+  const cbm_model_desc& m = c->m;
+  c->clv = c->cl && m.walk != 1 && !c->rules && !c->inertial &&
+           (m.law == CBM_LAW_LINEAR || m.law == CBM_LAW_REP_ADH || m.law == CBM_LAW_AGGREGATION) &&
+           (m.integrator == CBM_EULER || m.integrator == CBM_HEUN);
+  if (!c->clv) return;
+  double reach = 0.0;
+  for (int s = 0; s < c->S; ++s) reach = std::max(reach, law_reach(m.law, c->sims[s].p));
+  const double rl = (reach + 0.125 * reach) * (1.0 + 1e-9);
+  bool ok = reach > 0.0 && std::isfinite(reach);
+  for (int s = 0; s < c->S && ok; ++s)
+    for (int a = 0; a < m.dims && ok; ++a) {
+      const double L = c->sims[s].box_hi[a] - c->sims[s].box_lo[a];
+      const double n = m.periodic ? std::floor(L / m.cell_edge) : std::ceil(L / m.cell_edge + 2.0);
+      ok = n <= 1023.0 && (!m.periodic || std::floor(L / rl) >= 3.0);
+    }
+  if (!ok) { c->clv = false; return; }
+  c->clv_reach = reach;
+  c->clv_skin = 0.125 * reach;
+  c->list_radius = rl;
+  c->list_r2 = sq_threshold(rl);
+  c->thr = 0.5 * c->clv_skin;
+  c->nb_kind = NB_LIST;
+  // list capacity from the densest simulation's expected list length
+  double lam = 0.0;
+  for (int s = 0; s < c->S; ++s) {
+    double vol = 1.0;
+    for (int a = 0; a < m.dims; ++a) vol *= c->sims[s].box_hi[a] - c->sims[s].box_lo[a];
+    const double ball = m.dims == 1 ? 2.0 * rl : (m.dims == 2 ? M_PI * rl * rl : 4.0 / 3.0 * M_PI * rl * rl * rl);
+    lam = std::max(lam, (double)c->sims[s].n_max / vol * ball);
+  }
+  const int want = (int)std::ceil(lam + 6.0 * std::sqrt(lam) + 8.0);
+  c->cap = std::min(FLAT_MAX_CAP, std::max(16, (want + 7) / 8 * 8));
+}
+
 static int setup_geometry(cbm_ctx* c) {
   const cbm_model_desc& m = c->m;
   c->h_sims.resize(c->S);
   int64_t row = 0, cell = 0;
   double edge = m.cell_edge;
   if (c->verlet) edge = std::max(m.cell_edge, m.skin);
+  if (c->clv) edge = c->list_radius;
   for (int s = 0; s < c->S; ++s) {
     const cbm_sim_desc& in = c->sims[s];
     SimDev& sd = c->h_sims[s];
@@ -392,6 +457,19 @@ static int setup_geometry(cbm_ctx* c) {
       sd.gn[a] = 1; sd.gedge[a] = 1.0;
     }
     for (int q = 0; q < 8; ++q) sd.p[q] = in.p[q];
+    for (int a = 0; a < 3; ++a) { sd.cgn[a] = 1; sd.cedge[a] = 1.0; }
+    if (c->clv)
+      for (int a = 0; a < m.dims; ++a) {   // the model's own grid (src/neighbors.jl:604), for the cell order
+        const double L = sd.L[a];
+        if (m.periodic) {
+          const int64_t n = (int64_t)std::floor(L / m.cell_edge);
+          sd.cgn[a] = (int32_t)n;
+          sd.cedge[a] = L / (double)n;
+        } else {
+          sd.cgn[a] = (int32_t)std::ceil(L / m.cell_edge + 2.0);
+          sd.cedge[a] = m.cell_edge;
+        }
+      }
     if (!c->full) {
       for (int a = 0; a < m.dims; ++a) {
         double L = sd.L[a];
@@ -470,6 +548,7 @@ static void destroy_ctx(cbm_ctx* c) {
   for (int k = 0; k < 8; ++k) F(c->rule_nb[k]);
   F(c->med_init); F(c->med_tmp); F(c->adi_tab[0]); F(c->adi_tab[1]); F(c->adi_tab[2]);
   F(c->key); F(c->lin); F(c->gbucket); F(c->cell_range); F(c->count); F(c->cursor); F(c->start); F(c->tmp); F(c->cell_row);
+  F(c->emax); F(c->wlist); F(c->wlist_n); F(c->fix_rows); F(c->fix_n);
   F(c->scan_status); F(c->scan_ticket); F(c->list); F(c->list_n); F(c->bricks); F(c->brick_skip); F(c->blk_flag); F(c->flags); F(c->counters);
   for (cbm_ctx::DevBuf* b : {&c->adj_tgt, &c->adj_tbase, &c->adj_invn, &c->adj_ck, &c->adj_seg, &c->adj_lam,
                              &c->adj_gpart, &c->adj_lpart}) F(b->p);
@@ -504,6 +583,7 @@ static int init_ctx(cbm_ctx* c) {
   c->list_radius = m.skin;
   c->list_r2 = sq_threshold(m.skin);
   c->thr = m.rebuild_disp > 0 ? m.rebuild_disp : m.skin / 2;   // src/neighbors.jl:403
+  choose_clv(c);
   HIPCHK(c, hipGetDevice(&c->device));
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(c, hipEventCreate(&c->ev0));
@@ -553,6 +633,7 @@ static int init_ctx(cbm_ctx* c) {
   // every DE variable's column exists before any stage addresses it
   for (int v = 0; v < c->nvar; ++v)
     if (!c->has_col[statecol_k(c, v)]) return fail(c, CBM_ERR_CONFIG, "DE variable without a column");
+  if (c->clv && (rc = alloc_column(c, C_CCELL, 4))) return rc;
   if (c->rules) {
     if ((rc = alloc_column(c, C_LF, 8))) return rc;
     const int64_t nm = c->sims[0].n_max;
@@ -663,18 +744,25 @@ static int init_ctx(cbm_ctx* c) {
   HIPCHK(c, dalloc(&c->counters, 4));
   // cell-linked Heun / EulerHeun with a built-in law: stage 1 writes the
   // candidate lists that stage 2 walks (flags[4] says whether they are exact)
-  c->wl = c->cl && (m.integrator == CBM_HEUN || m.integrator == CBM_EULER_HEUN) && !c->inertial &&
+  c->wl = c->cl && !c->clv && (m.integrator == CBM_HEUN || m.integrator == CBM_EULER_HEUN) && !c->inertial &&
           (m.law == CBM_LAW_LINEAR || m.law == CBM_LAW_REP_ADH || m.law == CBM_LAW_AGGREGATION);
   if (c->wl) {
     c->wcap = 32;
     HIPCHK(c, dalloc(&c->list, (int64_t)c->wcap * c->C));
     HIPCHK(c, dalloc(&c->list_n, c->C));
   }
-  if (c->verlet) {
-    c->cap = m.max_neighbors;
+  if (c->verlet || c->clv) {
+    if (c->verlet) c->cap = m.max_neighbors;
     for (int a = 0; a < c->dims; ++a) HIPCHK(c, dalloc(&c->xref[a], c->C));
     HIPCHK(c, dalloc(&c->list, (int64_t)c->cap * c->C));
     HIPCHK(c, dalloc(&c->list_n, c->C));
+  }
+  if (c->clv) {
+    HIPCHK(c, dalloc(&c->emax, 2));
+    HIPCHK(c, dalloc(&c->wlist, (int64_t)c->cap * c->C));
+    HIPCHK(c, dalloc(&c->wlist_n, c->C));
+    HIPCHK(c, dalloc(&c->fix_rows, c->C));
+    HIPCHK(c, dalloc(&c->fix_n, 2));
   }
   int64_t maxn = 1;
   for (int s = 0; s < c->S; ++s) maxn = std::max(maxn, c->sims[s].n_max);
@@ -715,7 +803,9 @@ static RebuildArgs rebuild_args(cbm_ctx* c, const int32_t* flag, int32_t* flag_c
   a.key = c->key; a.lin = c->lin; a.gbucket = c->gbucket; a.count = c->count; a.cursor = c->cursor; a.start = c->start; a.tmp = c->tmp;
   a.cell_row = c->cell_row;
   a.dims = c->dims; a.periodic = c->m.periodic; a.full = c->full ? 1 : 0;
-  a.geometric = c->verlet ? 1 : 0;
+  a.geometric = (c->verlet || c->clv) ? 1 : 0;
+  a.emax_clear = c->emax;
+  a.fix_clear = c->fix_n;
   return a;
 }
 
@@ -771,10 +861,12 @@ static int rebuild_chain(cbm_ctx* c, const int32_t* flag, int32_t* flag_clear, b
     for (int d = 0; d < c->dims; ++d) { va.apos[d] = (const double*)c->col[C_X + d].alt(); va.xref[d] = c->xref[d]; }
     va.aslot = (const int32_t*)c->col[C_SLOT].alt();
     va.list = c->list; va.list_n = c->list_n; va.ldl = c->C; va.cap = c->cap;
-    va.overflow = c->flags + 2;
+    // a CellLinked skin list past its capacity is no error: its agent takes
+    // the exact walk (flat_body), so the overflow word is one nobody reads
+    va.overflow = c->flags + (c->clv ? 6 : 2);
     va.rl2 = c->list_r2;
     va.lds_r = verlet_lds_r(c);
-    va.build = c->verlet ? 1 : 0;
+    va.build = (c->verlet || c->clv) ? 1 : 0;
     if (va.build && c->bricks) {
       // brick tiles, then the rows of the bricks too dense for LDS
       va.bs = brick_src(c);
@@ -789,13 +881,22 @@ static int rebuild_chain(cbm_ctx* c, const int32_t* flag, int32_t* flag_clear, b
 }
 
 // Forced (re)build before step c->step: computeNeighbors! of loadToPlatform!
+static StageArgs stage_base(cbm_ctx* c);
+
 static int force_rebuild(cbm_ctx* c) {
-  if (c->verlet) {
+  if (c->verlet || c->clv) {
     int32_t* f = c->flags + ((c->step + 1) & 1);
     HIPCHK(c, launch_set_flag(c->stream, f, 1));
     int rc = rebuild_chain(c, f, c->flags + (c->step & 1), false, true);
     if (rc) return rc;
     if (c->vtime) c->t_next = c->t + c->m.dt_recompute;
+    if (c->clv) {
+      // the cells of the sorted rows; every final stage writes the next ones
+      StageArgs a = stage_base(c);
+      for (int d = 0; d < c->dims; ++d) a.S[d] = (const double*)c->col[C_X + d].cur();
+      a.ccell_out = (int32_t*)c->col[C_CCELL].cur();
+      HIPCHK(c, launch_clcell(c->dims, c->nblocks, c->block, c->stream, a));
+    }
   } else {
     int rc = rebuild_chain(c, nullptr, nullptr, false, false);
     if (rc) return rc;
@@ -820,6 +921,24 @@ static StageArgs stage_base(cbm_ctx* c) {
   a.brick_over = (unsigned long long*)(c->counters + 2);
   for (int q = 0; q < 3; ++q) a.bdim[q] = c->bdim[q];
   a.brick_cap = c->brick_cap;
+  // flattened list stages: lists of at most FLAT_MAX_CAP entries
+  {
+    const int lcap = (c->verlet || c->clv) ? c->cap : (c->wl ? c->wcap : 0);
+    const int chunk = c->m.walk >= 512 ? c->m.walk : 0;
+    if (lcap > 0 && lcap <= FLAT_MAX_CAP && chunk > 0) { a.fchunk = chunk; a.fcap = lcap; }
+  }
+  if (c->clv) {
+    a.ccell = (const int32_t*)c->col[C_CCELL].cur();
+    a.cslot = (const int32_t*)c->col[C_SLOT].cur();
+    a.clv_e1 = c->thr * (1.0 + 1e-6);
+    a.clv_skin = c->clv_skin * (1.0 - 1e-9);
+    a.clv_half = 0.5 * c->clv_skin * (1.0 + 1e-9);
+    a.clv_half2 = 0.25 * c->clv_skin * c->clv_skin;
+    a.fix_rows = c->fix_rows;
+    a.fix_n = c->fix_n;
+    a.nsims = c->S;
+    a.lcap = c->cap;
+  }
   a.dt = c->m.dt; a.sqdt = std::sqrt(c->m.dt); a.drift = c->m.drift; a.noise = c->m.noise; a.thr2 = sq_threshold(c->thr);
   a.seed = c->m.seed;
   a.step_ptr = c->counters + 1;
@@ -842,7 +961,8 @@ static StageArgs stage_base(cbm_ctx* c) {
 // PROF_CHAIN for the neighbor update of one step.
 constexpr int PROF_CHAIN = 8;
 constexpr int PROF_STEP = 9;
-constexpr int PROF_NCAT = 12;
+constexpr int PROF_ORDER = 12;   // CellLinked on a skin list: the step's ordered lists
+constexpr int PROF_NCAT = 13;
 // profile category of a stage kind: 0..7 as the kind, EulerHeun stages 10, 11
 static int prof_cat_of(int kind) { return kind == ST_EH1 ? 10 : (kind == ST_EH2 ? 11 : kind); }
 
@@ -1060,7 +1180,7 @@ static int step_once(cbm_ctx* c) {
   if (c->rules && (rc = rule_phase(c, c->flags + (s & 1)))) return rc;
   StageArgs a = stage_base(c);
   int32_t* flag_now = c->flags + (s & 1);
-  const bool disp = c->verlet && !c->vtime;
+  const bool disp = (c->verlet && !c->vtime) || c->clv;
   auto set_state = [&](double* const* S, double* const* U, double* const* out) {
     for (int v = 0; v < c->nvar; ++v) { a.S[v] = S[v]; a.U[v] = U[v]; a.out[v] = out[v]; a.kacc[v] = c->kacc[v]; }
   };
@@ -1068,11 +1188,23 @@ static int step_once(cbm_ctx* c) {
   double* alt[MAX_STATE] = {};
   for (int v = 0; v < c->nvar; ++v) { cur[v] = statecol(c, v, false); alt[v] = statecol(c, v, true); }
   a.flag = flag_now;
+  // CellLinked on a skin list: the step's lists in cell-walk order, walked by
+  // every stage; the final stage writes the next cells
+  int32_t* ccell_next = c->clv ? (int32_t*)c->col[C_CCELL].alt() : nullptr;
+  if (c->clv) {
+    StageArgs o = a;
+    o.wlist = c->wlist; o.wlist_n = c->wlist_n;
+    prof_begin(c, PROF_ORDER);
+    HIPCHK(c, launch_clv_order(c->dims, c->m.periodic, c->nblocks, c->block, c->stream, o));
+    prof_end(c);
+    a.clv = 1; a.list = c->wlist; a.list_n = c->wlist_n;
+  }
   switch (c->m.integrator) {
     case CBM_EULER:
     case CBM_EM:
       set_state(cur, cur, alt);
       a.disp_check = disp ? 1 : 0;
+      if (c->clv) { a.fix_stage = 0; a.ccell_out = ccell_next; }
       if ((rc = run_stage(c, a, c->m.integrator == CBM_EULER ? ST_EULER : ST_EM))) return rc;
       flip_state(c);
       break;
@@ -1080,10 +1212,12 @@ static int step_once(cbm_ctx* c) {
       set_state(cur, cur, c->h[0]);
       a.disp_check = 0;
       listed_stage(c, a, 1);
+      if (c->clv) { a.fix_stage = 0; a.emax = c->emax; a.list = c->wlist; a.list_n = c->wlist_n; }
       if ((rc = run_stage(c, a, ST_HEUN1))) return rc;
       set_state(c->h[0], cur, cur);
       a.disp_check = disp ? 1 : 0;
       listed_stage(c, a, 2);
+      if (c->clv) { a.fix_stage = 1; a.emax = c->emax; a.ccell_out = ccell_next; a.list = c->wlist; a.list_n = c->wlist_n; }
       if ((rc = run_stage(c, a, ST_HEUN2))) return rc;
       break;
     case CBM_EULER_HEUN:
@@ -1115,6 +1249,7 @@ static int step_once(cbm_ctx* c) {
     col.buf[col.cidx] = fresh;
     fresh = (double*)committed;
   };
+  if (c->clv) c->col[C_CCELL].cidx ^= 1;
   if (c->lagged)
     for (int d = 0; d < c->dims; ++d) commit(c->col[C_FX + d], c->Fnew[d]);
   if (c->growth) {
@@ -1135,7 +1270,7 @@ static int step_once(cbm_ctx* c) {
 static int neighbor_update(cbm_ctx* c, int64_t s, bool disp) {
   int rc;
   int32_t* flag_now = c->flags + (s & 1);
-  if (c->cl) {
+  if (c->cl && !c->clv) {
     if ((rc = rebuild_chain(c, nullptr, nullptr, true, false))) return rc;
   } else if (disp) {
     if ((rc = rebuild_chain(c, flag_now, c->flags + ((s + 1) & 1), true, true))) return rc;
@@ -1681,6 +1816,8 @@ static int export_sets(cbm_ctx* c, int32_t sim, double radius, int32_t* counts, 
   e.slot = (const int32_t*)c->col[C_SLOT].cur();
   e.counts = d_counts; e.lists = d_lists; e.counts64 = d_c64; e.cap = cap; e.radius = radius;
   e.take_all = (c->verlet && (lists || radius <= 0.0)) ? 1 : 0;
+  e.clv = c->clv ? 1 : 0;
+  e.e_bound = c->thr * (1.0 + 1e-6);
   e.overflow = d_ovf;
   e.only_sim = sim;
   int ovf = 0;
@@ -1709,6 +1846,13 @@ int cbm_count_neighbors(cbm_ctx* c, int32_t sim, double radius, int64_t* out) {
 
 int cbm_set_params(cbm_ctx* c, int32_t sim, const double* p, int32_t n) {
   if (!c || !p || sim < 0 || sim >= c->S || n < 1 || n > 8) return CBM_ERR_ARG;
+  if (c->clv) {
+    double q[8];
+    for (int k = 0; k < 8; ++k) q[k] = k < n ? p[k] : c->sims[sim].p[k];
+    if (!(law_reach(c->m.law, q) <= c->clv_reach))
+      return fail(c, CBM_ERR_CONFIG, "the law's reach grew past the one the cell-linked lists were sized for "
+                                     "(create the context with the largest parameters, or walk = 1)");
+  }
   for (int q = 0; q < n; ++q) { c->sims[sim].p[q] = p[q]; c->h_sims[sim].p[q] = p[q]; }
   HIPCHK(c, hipMemcpyAsync(c->d_sims + sim, &c->h_sims[sim], sizeof(SimDev), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -41,6 +41,8 @@ struct RebuildArgs {
   CellRange* range;           // rank_off + linear local cell -> rows, written by k_scatter's range blocks
   int64_t nr;                 // entries of gbucket / range (all simulations)
   int32_t dims, periodic, full, geometric;
+  unsigned long long* emax_clear;   // cleared by k_keys every step (may be NULL)
+  int32_t* fix_clear;               // two counters cleared by k_keys every step (may be NULL)
 };
 
 struct ScanArgs {
@@ -131,7 +133,8 @@ struct ExportArgs {
   int32_t take_all;     // Verlet: export the list itself
   int32_t* overflow;
   int32_t only_sim;
-  int32_t pad;
+  int32_t clv;          // CellLinked on a skin list: exact walk in the cell order (clv_each)
+  double e_bound;       // clv: bound on |x - x_ref| of every agent
 };
 
 // Adjoint of the overdamped-Euler mechanics step (no reference counterpart).
@@ -213,6 +216,8 @@ hipError_t launch_tick(hipStream_t s, int64_t* counter);
 hipError_t launch_xfer(int nblocks, int block, hipStream_t s, const XferArgs& a);
 hipError_t launch_snapshot(int nblocks, int block, hipStream_t s, const SnapArgs& a);
 hipError_t launch_export(int dims, int nb, int nblocks, int block, hipStream_t s, const ExportArgs& e);
+hipError_t launch_clcell(int dims, int nblocks, int block, hipStream_t s, const StageArgs& a);
+hipError_t launch_clv_order(int dims, int periodic, int nblocks, int block, hipStream_t s, const StageArgs& a);
 
 }  // namespace cbm
 

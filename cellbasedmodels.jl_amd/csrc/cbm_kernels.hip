@@ -59,6 +59,34 @@ __global__ __launch_bounds__(256) void k_brick(StageArgs a) {
   brick_body<D, PairLaw<LAW>, PER, NOISE>(a, brick_lds);
 }
 
+// list stage with the pair work flattened over the block (dynamic LDS:
+// flat_lds_bytes)
+template <int D, int LAW, int PER, int NOISE>
+__global__ __launch_bounds__(256) void k_flat(StageArgs a) {
+  extern __shared__ unsigned char flat_lds[];
+  flat_body<D, PairLaw<LAW>, PER, NOISE>(a, flat_lds);
+}
+
+// CellLinked on a skin list: the step's ordered lists (dynamic LDS: the
+// per-lane key queues, blockDim x lcap uint16), the stages over them, and
+// the exact walk of the agents they deferred
+template <int D, int PER>
+__global__ __launch_bounds__(256) void k_clv_order(StageArgs a) {
+  extern __shared__ unsigned char clv_lds[];
+  clv_order_body<D, PER>(a, clv_lds);
+}
+
+template <int D, int LAW, int PER, int NOISE>
+__global__ __launch_bounds__(256) void k_clv(StageArgs a) {
+  clv_stage_body<D, PairLaw<LAW>, PER, NOISE>(a);
+}
+
+template <int D, int LAW, int PER, int NOISE>
+__global__ __launch_bounds__(256) void k_clv_fix(StageArgs a) {
+  if (a.fix_n[a.fix_stage] == 0) return;
+  clv_fix_body<D, PairLaw<LAW>, PER, NOISE>(a);
+}
+
 // --------------------------------------------------------- rebuild chain
 
 __global__ __launch_bounds__(256) void k_keys(RebuildArgs a) {
@@ -67,6 +95,8 @@ __global__ __launch_bounds__(256) void k_keys(RebuildArgs a) {
     // clear the flag the next step's final stage will set
     if (a.step_inc) *a.step_inc += 1;
     if (a.flag_clear) *a.flag_clear = 0;
+    if (a.emax_clear) *a.emax_clear = 0ull;
+    if (a.fix_clear) { a.fix_clear[0] = 0; a.fix_clear[1] = 0; }
     if (a.gate_set) *a.gate_set = 1;
   }
   if (a.flag && *a.flag == 0) return;
@@ -460,25 +490,6 @@ __global__ __launch_bounds__(256) void k_verlet(VerletArgs va) {
   }
   if (ri.local >= N) return;
   (void)verlet_row<D>(va, sd, ri.row, vbuf + threadIdx.x, (int)blockDim.x);
-}
-
-// Batcher's odd-even merge sort of N keys held in registers (every index is
-// a compile-time constant once the loops unroll).
-template <int N>
-__device__ __forceinline__ void sort_net(int64_t* k) {
-#pragma unroll
-  for (int p = 1; p < N; p <<= 1)
-#pragma unroll
-    for (int kk = p; kk >= 1; kk >>= 1)
-#pragma unroll
-      for (int j = kk % p; j + kk < N; j += 2 * kk)
-#pragma unroll
-        for (int i = 0; i < kk && i + j + kk < N; ++i)
-          if ((i + j) / (2 * p) == (i + j + kk) / (2 * p)) {
-            const int64_t x = k[i + j], y = k[i + j + kk];
-            k[i + j] = x < y ? x : y;
-            k[i + j + kk] = x < y ? y : x;
-          }
 }
 
 // The list build on LDS brick tiles: the halo of a brick (positions and
@@ -1082,6 +1093,46 @@ __global__ __launch_bounds__(256) void k_export(ExportArgs e) {
   if (e.counts64) e.counts64[si] = k;
 }
 
+// The export of a CellLinked model run on a skin list: every candidate of the
+// model's cells within the radius, in its order (clv_each).
+template <int D, int PER>
+__global__ __launch_bounds__(256) void k_export_clv(ExportArgs e) {
+  const StageArgs& a = e.s;
+  const RowIdx ri = row_index(a.g);
+  if (ri.sim != e.only_sim) return;
+  const SimDev& sd = a.g.sims[ri.sim];
+  const int64_t N = a.g.st[ri.sim].N;
+  if (ri.local >= N) return;
+  const int64_t i = ri.row;
+  const int64_t si = e.slot[i];
+  double xi[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < D; ++c) xi[c] = a.S[c][i];
+  int64_t k = 0;
+  clv_each<D, PER>(a, sd, i, xi, reach2_bound(e.radius), e.radius + e.e_bound, [&](int64_t j, double s2) {
+    if (!(sqrt(s2) < e.radius)) return;
+    if (e.lists) {
+      if (k < e.cap) e.lists[si * e.cap + k] = e.slot[j];
+      else *e.overflow = 1;
+    }
+    ++k;
+  });
+  if (e.counts) e.counts[si] = (int32_t)k;
+  if (e.counts64) e.counts64[si] = k;
+}
+
+// Packed cells (cl_pack) of every row's position a.S into a.ccell_out.
+template <int D>
+__global__ __launch_bounds__(256) void k_clcell(StageArgs a) {
+  const RowIdx ri = row_index(a.g);
+  const SimDev& sd = a.g.sims[ri.sim];
+  if (ri.local >= a.g.st[ri.sim].n_tot) return;
+  double x[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < D; ++c) x[c] = a.S[c][ri.row];
+  a.ccell_out[ri.row] = cl_pack<D>(sd, x, a.periodic);
+}
+
 // ------------------------------------------------------------- launchers
 struct StageLaunch { dim3 grid, bgrid, block; size_t lds; hipStream_t s; };   // bgrid: one block per brick
 
@@ -1109,8 +1160,46 @@ static hipError_t launch_stage_law(int law, const StageLaunch& L, const StageArg
   return hipGetLastError();
 }
 
+// a stage of CellLinked on a skin list, then the exact walk of the agents it
+// deferred (a grid of at most 64 blocks; it exits at once when none were)
+template <int D, int PER, int NOISE>
+static hipError_t launch_clv_stage(int law, const StageLaunch& L, const StageArgs& a) {
+  const dim3 fg(L.grid.x < 64 ? L.grid.x : 64);
+  switch (law) {
+    case LAW_LINEAR:
+      hipLaunchKernelGGL((k_clv<D, LAW_LINEAR, PER, NOISE>), L.grid, L.block, 0, L.s, a);
+      hipLaunchKernelGGL((k_clv_fix<D, LAW_LINEAR, PER, NOISE>), fg, L.block, 0, L.s, a);
+      break;
+    case LAW_REP_ADH:
+      hipLaunchKernelGGL((k_clv<D, LAW_REP_ADH, PER, NOISE>), L.grid, L.block, 0, L.s, a);
+      hipLaunchKernelGGL((k_clv_fix<D, LAW_REP_ADH, PER, NOISE>), fg, L.block, 0, L.s, a);
+      break;
+    case LAW_AGGREGATION:
+      hipLaunchKernelGGL((k_clv<D, LAW_AGGREGATION, PER, NOISE>), L.grid, L.block, 0, L.s, a);
+      hipLaunchKernelGGL((k_clv_fix<D, LAW_AGGREGATION, PER, NOISE>), fg, L.block, 0, L.s, a);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int D, int PER, int NOISE>
+static hipError_t launch_flat(int law, const StageLaunch& L, const StageArgs& a) {
+  const size_t lds = (size_t)flat_lds_bytes(D, a.fchunk, a.fcap);
+  switch (law) {
+    case LAW_LINEAR: hipLaunchKernelGGL((k_flat<D, LAW_LINEAR, PER, NOISE>), L.grid, L.block, lds, L.s, a); break;
+    case LAW_REP_ADH: hipLaunchKernelGGL((k_flat<D, LAW_REP_ADH, PER, NOISE>), L.grid, L.block, lds, L.s, a); break;
+    case LAW_AGGREGATION: hipLaunchKernelGGL((k_flat<D, LAW_AGGREGATION, PER, NOISE>), L.grid, L.block, lds, L.s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <int D, int PER, int NOISE>
 static hipError_t launch_stage_nb(int nb, int law, const StageLaunch& L, const StageArgs& a) {
+  const bool reach_law = law == LAW_LINEAR || law == LAW_REP_ADH || law == LAW_AGGREGATION;
+  if (nb == NB_LIST && a.clv && reach_law) return launch_clv_stage<D, PER, NOISE>(law, L, a);
+  if (nb == NB_LIST && a.fchunk > 0 && reach_law) return launch_flat<D, PER, NOISE>(law, L, a);
   switch (nb) {
     case NB_FULL: return launch_stage_law<D, NB_FULL, PER, NOISE>(law, L, a);
     case NB_CELL: return launch_stage_law<D, NB_CELL, PER, NOISE>(law, L, a);
@@ -1310,8 +1399,41 @@ static hipError_t launch_export_nb(int nb, dim3 grid, dim3 block, hipStream_t s,
   return hipGetLastError();
 }
 
+hipError_t launch_clv_order(int dims, int periodic, int nblocks, int block, hipStream_t s, const StageArgs& a) {
+  const size_t lds = (size_t)block * (size_t)a.lcap * 2;
+  switch (dims * 2 + (periodic ? 1 : 0)) {
+    case 2: hipLaunchKernelGGL((k_clv_order<1, 0>), dim3(nblocks), dim3(block), lds, s, a); break;
+    case 3: hipLaunchKernelGGL((k_clv_order<1, 1>), dim3(nblocks), dim3(block), lds, s, a); break;
+    case 4: hipLaunchKernelGGL((k_clv_order<2, 0>), dim3(nblocks), dim3(block), lds, s, a); break;
+    case 5: hipLaunchKernelGGL((k_clv_order<2, 1>), dim3(nblocks), dim3(block), lds, s, a); break;
+    case 6: hipLaunchKernelGGL((k_clv_order<3, 0>), dim3(nblocks), dim3(block), lds, s, a); break;
+    default: hipLaunchKernelGGL((k_clv_order<3, 1>), dim3(nblocks), dim3(block), lds, s, a); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_clcell(int dims, int nblocks, int block, hipStream_t s, const StageArgs& a) {
+  switch (dims) {
+    case 1: hipLaunchKernelGGL(k_clcell<1>, dim3(nblocks), dim3(block), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_clcell<2>, dim3(nblocks), dim3(block), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_clcell<3>, dim3(nblocks), dim3(block), 0, s, a); break;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_export(int dims, int nb, int nblocks, int block, hipStream_t s, const ExportArgs& e) {
   dim3 grid(nblocks), blk(block);
+  if (e.clv) {
+    switch (dims * 2 + (e.s.periodic ? 1 : 0)) {
+      case 2: hipLaunchKernelGGL((k_export_clv<1, 0>), grid, blk, 0, s, e); break;
+      case 3: hipLaunchKernelGGL((k_export_clv<1, 1>), grid, blk, 0, s, e); break;
+      case 4: hipLaunchKernelGGL((k_export_clv<2, 0>), grid, blk, 0, s, e); break;
+      case 5: hipLaunchKernelGGL((k_export_clv<2, 1>), grid, blk, 0, s, e); break;
+      case 6: hipLaunchKernelGGL((k_export_clv<3, 0>), grid, blk, 0, s, e); break;
+      default: hipLaunchKernelGGL((k_export_clv<3, 1>), grid, blk, 0, s, e); break;
+    }
+    return hipGetLastError();
+  }
   switch (dims) {
     case 1: return launch_export_nb<1>(nb, grid, blk, s, e);
     case 2: return launch_export_nb<2>(nb, grid, blk, s, e);

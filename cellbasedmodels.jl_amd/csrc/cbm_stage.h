@@ -66,6 +66,45 @@ __device__ __forceinline__ int32_t neigh_cell(const SimDev& sd, int32_t cx, int3
   return q[0] + sd.gn[0] * (q[1] + sd.gn[1] * q[2]);
 }
 
+// Cell of x on the model's grid along axis a: cellPos (src/neighbors.jl:
+// 636-638, 0-based) on the padded grid, clamped on the periodic one.
+__device__ __forceinline__ int32_t cl_axis(const SimDev& sd, int a, double x, int periodic) {
+  if (periodic) {
+    int64_t c = floor_div(x - sd.lo[a], sd.cedge[a]);
+    c = c < 0 ? 0 : (c > sd.cgn[a] - 1 ? sd.cgn[a] - 1 : c);
+    return (int32_t)c;
+  }
+  if (x > sd.hi[a]) return sd.cgn[a] - 1;
+  if (x < sd.lo[a]) return 0;
+  return (int32_t)(floor_div(x - sd.lo[a], sd.cedge[a]) + 1);
+}
+
+// The cell packed as 10 bits per axis (grids of at most 1023 cells per axis).
+template <int D>
+__device__ __forceinline__ int32_t cl_pack(const SimDev& sd, const double* x, int periodic) {
+  int32_t v = 0;
+#pragma unroll
+  for (int a = 0; a < D; ++a) v |= cl_axis(sd, a, x[a], periodic) << (10 * a);
+  return v;
+}
+
+// Position of cell pj among the 3^D neighbor cells of pi (cellPosNeigh
+// order), or 27 when pj is not one of them.
+template <int D, int PER>
+__device__ __forceinline__ int cl_key(const SimDev& sd, int32_t pi, int32_t pj) {
+  int k = 0, w = 1;
+  bool adj = true;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    int d = ((pj >> (10 * a)) & 1023) - ((pi >> (10 * a)) & 1023);
+    if (PER) d = d == sd.cgn[a] - 1 ? -1 : (d == 1 - sd.cgn[a] ? 1 : d);
+    adj = adj && d >= -1 && d <= 1;
+    k += (d + 1) * w;
+    w *= 3;
+  }
+  return adj ? k : 27;
+}
+
 __host__ __device__ constexpr int pow3(int d) { return d == 1 ? 3 : (d == 2 ? 9 : 27); }
 
 // A law with a per-agent term: `void self(const double* x, double* f) const`
@@ -244,7 +283,7 @@ struct ConstNoise {
 template <int D, class Law, int NOISE, bool PRE = false, class Noise = ConstNoise>
 __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& sd, int sim, int64_t i,
                                              const double* xi, double* acc, const double* upre,
-                                             double disp2 = -1.0) {
+                                             double disp2 = -1.0, double* e2_out = nullptr) {
   constexpr bool SOFT = same_type<Law, PairLaw<LAW_SOFT_SPHERES>>::value;
   // u of variable c (with PRE, the caller loaded the positions' u early into
   // upre), and the stage state, whose positions are xi
@@ -254,7 +293,7 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
     return a.U[c][i];
   };
   auto Sc = [&](int c) { return c < D ? xi[c] : a.S[c][i]; };
-  if (Law::active && Law::walls && a.walls) {
+  if (Law::active && Law::walls && a.walls && sd.p[4] != 0.0) {   // p[4]: the walls switch
     const double rRep = sd.p[0], fRep = sd.p[1];
 #pragma unroll
     for (int c = 0; c < D; ++c) {
@@ -400,6 +439,23 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
     for (int c = 0; c < D; ++c) res[c] = wrap_box(res[c], sd.lo[c], sd.L[c]);
   }
   for (int c = 0; c < nv; ++c) a.out[c][i] = res[c];
+  if (a.clv) {
+    // CellLinked on a skin list: the next step's cells, or this stage's
+    // distance from the list's reference positions
+    if (final_stage && a.ccell_out) a.ccell_out[i] = cl_pack<D>(sd, res, a.periodic);
+    if (!final_stage && e2_out) {
+      double dx[3];
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        dx[c] = res[c] - a.xref[c][i];
+        if (a.periodic) dx[c] = min_image(dx[c], sd.L[c]);
+      }
+      double s2 = dx[0] * dx[0];
+      if (D > 1) s2 = s2 + dx[1] * dx[1];
+      if (D > 2) s2 = s2 + dx[2] * dx[2];
+      *e2_out = s2;
+    }
+  }
   if (disp2 >= 0.0) {
     // the stage-2 lists stay exact while no agent moves more than half their margin
     double s2 = 0.0;
@@ -419,6 +475,19 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
     if (D > 1) s2 = s2 + dx[1] * dx[1];
     if (D > 2) s2 = s2 + dx[2] * dx[2];
     if (s2 > a.thr2) *a.flag = 1;   // sqrt(s2) >= thr, src/neighbors.jl:403 (>= skin/2)
+    if (a.clv) {
+      // CellLinked on a skin list rebuilds one step early: when this step's
+      // displacement, repeated 1.5 times, would carry the agent past the
+      // threshold, so that the next step's stages rarely leave half the skin
+      double st = 0.0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        double d = res[c] - Uc(c);
+        if (a.periodic) d = min_image(d, sd.L[c]);
+        st = st + d * d;
+      }
+      if (!(sqrt(s2) + 1.5 * sqrt(st) <= a.clv_half)) *a.flag = 1;
+    }
   }
 }
 
@@ -454,6 +523,57 @@ __device__ __forceinline__ void stage_body(const StageArgs& a) {
   double acc[3] = {0.0, 0.0, 0.0};
   if (Law::active) gather_global<D, NB, Law, PER>(a, sd, N, i, xi, acc);
   stage_finish<D, Law, NOISE, false, Noise>(a, sd, ri.sim, i, xi, acc, xi);
+}
+
+// ------------------------------------------------ flattened list stage
+// The pair term of one list entry: what pair_into adds to the accumulator,
+// returned instead (+0 for an inactive pair; the accumulator never holds
+// -0, so adding +0 or -0 leaves it unchanged and the sums equal pair_into's).
+template <int D, class Law, int PER>
+__device__ __forceinline__ void pair_term(const Law& law, const double* xi, const double* xj, const double* L,
+                                          const double* hL, double* t) {
+  double dx[3];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    dx[c] = xi[c] - xj[c];
+    if (PER) dx[c] = wrap_dx(dx[c], L[c], hL[c]);
+  }
+  double s2 = dx[0] * dx[0];
+  if (D > 1) s2 = s2 + dx[1] * dx[1];
+  if (D > 2) s2 = s2 + dx[2] * dx[2];
+  const double d = sqrt(s2);
+  double s;
+  const bool on = law(d, s);
+  if (Law::per_axis_division) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) t[c] = on ? s * (dx[c] / d) : 0.0;
+  } else {
+    const double f = s / d;
+    const double fe = on ? f : 0.0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) t[c] = fe * dx[c];
+  }
+}
+
+// Exclusive block scan of one int32 per thread; the total lands in *total.
+__device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t* wsum, int32_t* total) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = (blockDim.x + 63) >> 6;
+  int32_t incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int32_t y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
+  }
+  if (lane == 63 || tid == (int)blockDim.x - 1) wsum[w] = incl;
+  __syncthreads();
+  int32_t before = 0, tot = 0;
+  for (int q = 0; q < nw; ++q) {
+    const int32_t ws = wsum[q];
+    if (q < w) before += ws;
+    tot += ws;
+  }
+  if (tid == 0) *total = tot;
+  return before + incl - v;
 }
 
 // ------------------------------------------------ cell-linked brick tiles
@@ -885,6 +1005,422 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
     }
     stage_finish<D, Law, NOISE, true, Noise>(a, sd, sim, i, xi, acc, upre, disp2);
   }
+}
+
+// Batcher's odd-even merge sort of N keys held in registers (every index is
+// a compile-time constant once the loops unroll).
+template <int N, class T>
+__device__ __forceinline__ void sort_net(T* k) {
+#pragma unroll
+  for (int p = 1; p < N; p <<= 1)
+#pragma unroll
+    for (int kk = p; kk >= 1; kk >>= 1)
+#pragma unroll
+      for (int j = kk % p; j + kk < N; j += 2 * kk)
+#pragma unroll
+        for (int i = 0; i < kk && i + j + kk < N; ++i)
+          if ((i + j) / (2 * p) == (i + j + kk) / (2 * p)) {
+            const T x = k[i + j], y = k[i + j + kk];
+            k[i + j] = x < y ? x : y;
+            k[i + j + kk] = x < y ? y : x;
+          }
+}
+
+// ------------------------------------------ CellLinked on a skin list
+// CellLinked (src/neighbors.jl:587-747) walks, for agent i, the 3^D cells
+// around i's cell in cellPosNeigh order (x fastest) and each cell's agents by
+// slot.  Its active pairs (d < the law's reach) therefore come in the order
+// (k, slot j), k the position of j's cell among i's neighbor cells.  With a
+// skin list (radius reach + skin, rebuilt when an agent has moved skin/2)
+// holding every pair within reach, the step needs no cell sort: each agent's
+// list entries are ordered by (k, slot) from the agents' packed cells of the
+// step start.
+
+// Every candidate j of row i in the reference's CellLinked order whose
+// stage position lies within sqrt(bound2) of xi (fn(j, s2)), found without
+// the skin list: repeated minimum selection of the key (k, slot) over the
+// list-build grid cells within R of xi, where R bounds the distance of every
+// such j's position at the last list build.  Exact for any displacement;
+// used for the few agents whose list may miss a pair, and for exports.
+template <int D, int PER, class Fn>
+__device__ void clv_each(const StageArgs& a, const SimDev& sd, int64_t i, const double* xi, double bound2, double R,
+                         Fn&& fn) {
+  double L[3], hL[3];
+  box_halves<D, PER>(sd, L, hL);
+  const int32_t ci = a.ccell[i];
+  int32_t c0[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+#pragma unroll
+  for (int e = 0; e < D; ++e) {
+    const double x = xi[e];
+    int64_t c = x < sd.lo[e] ? 0 : floor_div(x - sd.lo[e], sd.gedge[e]) + (PER ? 0 : 1);
+    c = c < 0 ? 0 : c;
+    c0[e] = (int32_t)(c > sd.gn[e] - 1 ? sd.gn[e] - 1 : c);
+    const double q = R / sd.gedge[e];
+    const int32_t m = (q == q && q < (double)sd.gn[e]) ? (int32_t)ceil(q) : sd.gn[e];
+    if (PER && 2 * m + 1 >= sd.gn[e]) { lo[e] = 0; hi[e] = sd.gn[e] - 1; c0[e] = 0; }
+    else { lo[e] = -m; hi[e] = m; }
+  }
+  const CellRange* __restrict__ cr = a.cell_range + sd.rank_off;
+  unsigned long long from = 0;
+  for (;;) {
+    unsigned long long best = ~0ull;
+    int64_t bj = -1;
+    double bs2 = 0.0;
+    for (int32_t oz = D > 2 ? lo[2] : 0; oz <= (D > 2 ? hi[2] : 0); ++oz)
+      for (int32_t oy = D > 1 ? lo[1] : 0; oy <= (D > 1 ? hi[1] : 0); ++oy)
+        for (int32_t ox = lo[0]; ox <= hi[0]; ++ox) {
+          const int32_t o[3] = {ox, oy, oz};
+          int32_t q[3] = {0, 0, 0};
+          bool ok = true;
+#pragma unroll
+          for (int e = 0; e < D; ++e) {
+            int32_t v = c0[e] + o[e];
+            if (PER) v = ((v % sd.gn[e]) + sd.gn[e]) % sd.gn[e];
+            else ok = ok && v >= 0 && v < sd.gn[e];
+            q[e] = v;
+          }
+          if (!ok) continue;
+          const CellRange r = cr[q[0] + sd.gn[0] * (q[1] + sd.gn[1] * q[2])];
+          for (int32_t j = r.s; j < r.e; ++j) {
+            if (j == i) continue;
+            const int k = cl_key<D, PER>(sd, ci, a.ccell[j]);
+            if (k == 27) continue;
+            double dx[3];
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+              dx[c] = xi[c] - a.S[c][j];
+              if (PER) dx[c] = wrap_dx(dx[c], L[c], hL[c]);
+            }
+            double s2 = dx[0] * dx[0];
+            if (D > 1) s2 = s2 + dx[1] * dx[1];
+            if (D > 2) s2 = s2 + dx[2] * dx[2];
+            if (!(s2 <= bound2)) continue;
+            const unsigned long long key = ((unsigned long long)k << 32) | (uint32_t)a.cslot[j];
+            if (key >= from && key < best) { best = key; bj = j; bs2 = s2; }
+          }
+        }
+    if (bj < 0) return;
+    fn(bj, bs2);
+    from = best + 1;
+  }
+}
+
+// The pair sum of row i by clv_each within the law's reach.
+template <int D, class Law, int PER>
+__device__ void clv_walk(const StageArgs& a, const SimDev& sd, int64_t i, const double* xi, double E, double* acc) {
+  const Law law(sd.p);
+  double L[3], hL[3];
+  box_halves<D, PER>(sd, L, hL);
+  const double reach = law.reach();
+  double s[3] = {0.0, 0.0, 0.0};
+  clv_each<D, PER>(a, sd, i, xi, reach2_bound(reach), reach + E, [&](int64_t j, double) {
+    double xj[3] = {0.0, 0.0, 0.0}, t[3];
+#pragma unroll
+    for (int c = 0; c < D; ++c) xj[c] = a.S[c][j];
+    pair_term<D, Law, PER>(law, xi, xj, L, hL, t);
+#pragma unroll
+    for (int c = 0; c < D; ++c) s[c] = s[c] + t[c];
+  });
+#pragma unroll
+  for (int c = 0; c < D; ++c) acc[c] = s[c];
+}
+
+// CellLinked on a skin list, one agent per lane.  Per step:
+//   clv_order_body: each agent's skin-list entries inside its 3^D neighbor
+//     cells are put in CellLinked order -- cell k among the neighbor cells,
+//     then list position (= slot order) -- by insertion of the 16-bit keys
+//     (k << 6 | position) into the lane's LDS queue (stride blockDim, a.lcap
+//     entries) and written as the step's list (a.wlist); a list at capacity
+//     (possibly truncated) is marked -1.
+//   clv_stage_body: every stage walks that list.  An agent whose list may
+//     miss a pair -- marked -1, or (stage 2 of a two-stage integrator) whose
+//     distance e from the reference positions plus the largest, E, over all
+//     agents exceeds the skin -- writes nothing and is queued for
+//   clv_fix_body, which sums its pairs by clv_walk and finishes its stage.
+// Stage 1 of a two-stage integrator records max e^2 (only agents past half
+// the skin report; E is at least half the skin).
+
+// One lane's queued agent (a.fix_rows), counted per stage in a.fix_n[stage].
+__device__ __forceinline__ void clv_defer(const StageArgs& a, int64_t i) {
+  const int32_t q = atomicAdd(a.fix_n + a.fix_stage, 1);
+  a.fix_rows[q] = (int32_t)i;
+}
+
+// e^2 of a stage-1 output past half the skin, into the device maximum.
+__device__ __forceinline__ void clv_report(const StageArgs& a, double e2) {
+  if (a.emax && !(e2 <= a.clv_half2)) atomicMax(a.emax, (unsigned long long)__double_as_longlong(e2));
+}
+
+template <int D, int PER>
+__device__ __forceinline__ void clv_order_body(const StageArgs& a, unsigned char* lds) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const RowIdx ri = row_index(a.g);
+  const SimDev& sd = a.g.sims[ri.sim];
+  if (ri.local >= a.g.st[ri.sim].N) return;
+  const int64_t i = ri.row;
+  const int32_t* __restrict__ lp = a.list + i;       // entry q at lp[q * ldl]
+  const int64_t ldl = a.ldl;
+  const int32_t n = a.list_n[i];
+  if (n >= a.lcap) { a.wlist_n[i] = -1; return; }
+  uint16_t* qk = (uint16_t*)lds + tid;                // this lane's queue, entry u at qk[u * nt]
+  const int32_t ci = a.ccell[i];
+  int32_t kept = 0;
+  auto insert = [&](uint16_t key) {
+    int32_t pos = kept;
+    while (pos > 0) {
+      const uint16_t prev = qk[(pos - 1) * nt];
+      if (prev < key) break;
+      qk[pos * nt] = prev;
+      --pos;
+    }
+    qk[pos * nt] = key;
+    ++kept;
+  };
+  constexpr int W = 16;
+  if (n <= W) {
+    // every list read and cell gather in flight, the keys sorted by a
+    // network in registers, the rows picked from registers
+    // the list holds lcap >= W entries per row, so all W loads are in bounds;
+    // entries past the length are replaced by the agent itself, unread
+    int32_t jj[W], key[W];
+    const int32_t* e = lp;
+#pragma unroll
+    for (int u = 0; u < W; ++u, e += ldl) {
+      const int32_t raw = *e;
+      jj[u] = u < n ? raw : (int32_t)i;
+    }
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      const int32_t cj = a.ccell[jj[u]];
+      const int k = u < n ? cl_key<D, PER>(sd, ci, cj) : 27;
+      key[u] = (k << 6) | u;                       // k = 27: outside the 3^D cells, sorts last
+      kept += k < 27 ? 1 : 0;
+    }
+    sort_net<W>(key);
+    int32_t* w = a.wlist + i;
+#pragma unroll
+    for (int v = 0; v < W; ++v, w += ldl) {
+      const int q = key[v] & 63;
+      int32_t j = jj[0];
+#pragma unroll
+      for (int u = 1; u < W; ++u) j = q == u ? jj[u] : j;
+      if (v < kept) *w = j;
+    }
+  } else {
+    for (int32_t q0 = 0; q0 < n; q0 += 4) {
+      int32_t jj[4], cc[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) jj[u] = q0 + u < n ? lp[(int64_t)(q0 + u) * ldl] : (int32_t)i;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) cc[u] = a.ccell[jj[u]];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (q0 + u >= n) continue;
+        const int k = cl_key<D, PER>(sd, ci, cc[u]);
+        if (k < 27) insert((uint16_t)((k << 6) | (q0 + u)));
+      }
+    }
+    for (int32_t v0 = 0; v0 < kept; v0 += 4) {
+      int32_t jj[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) jj[u] = v0 + u < kept ? lp[(int64_t)(qk[(v0 + u) * nt] & 63) * ldl] : 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (v0 + u < kept) a.wlist[(int64_t)(v0 + u) * ldl + i] = jj[u];
+    }
+  }
+  a.wlist_n[i] = kept;
+}
+
+template <int D, class Law, int PER, int NOISE>
+__device__ __forceinline__ void clv_stage_body(const StageArgs& a) {
+  const RowIdx ri = row_index(a.g);
+  const SimDev& sd = a.g.sims[ri.sim];
+  if (ri.local >= a.g.st[ri.sim].N) return;
+  const int64_t i = ri.row;
+  const int32_t n = a.list_n[i];
+  double xi[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < D; ++c) xi[c] = a.S[c][i];
+  if (n < 0) { clv_defer(a, i); return; }
+  if (a.fix_stage == 1) {
+    // stage 2: could a pair active here lie outside the list?
+    double dx[3];
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      dx[c] = xi[c] - a.xref[c][i];
+      if (PER) dx[c] = min_image(dx[c], sd.L[c]);
+    }
+    double s2 = dx[0] * dx[0];
+    if (D > 1) s2 = s2 + dx[1] * dx[1];
+    if (D > 2) s2 = s2 + dx[2] * dx[2];
+    const double E = fmax(sqrt(__longlong_as_double((long long)*a.emax)), a.clv_half);
+    if (!(sqrt(s2) + E <= a.clv_skin)) { clv_defer(a, i); return; }
+  }
+  double upre[3] = {0.0, 0.0, 0.0}, acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < D; ++c) upre[c] = a.U[c][i];
+  const Law law(sd.p);
+  double L[3], hL[3];
+  box_halves<D, PER>(sd, L, hL);
+  const int32_t* __restrict__ lp = a.list + i;
+  const int64_t ldl = a.ldl;
+  int32_t jn[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) jn[u] = u < n ? lp[(int64_t)u * ldl] : (int32_t)i;
+  for (int32_t q0 = 0; q0 < n; q0 += 2) {
+    int32_t jc[2];
+    double xj[2][3];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      jc[u] = jn[u];
+#pragma unroll
+      for (int c = 0; c < D; ++c) xj[u][c] = a.S[c][jc[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) jn[u] = q0 + 2 + u < n ? lp[(int64_t)(q0 + 2 + u) * ldl] : (int32_t)i;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      double t[3];
+      pair_term<D, Law, PER>(law, xi, xj[u], L, hL, t);
+      const bool v = q0 + u < n;
+#pragma unroll
+      for (int c = 0; c < D; ++c) acc[c] = acc[c] + (v ? t[c] : 0.0);
+    }
+  }
+  double e2 = 0.0;
+  stage_finish<D, Law, NOISE, true>(a, sd, ri.sim, i, xi, acc, upre, -1.0, &e2);
+  clv_report(a, e2);
+}
+
+template <int D, class Law, int PER, int NOISE>
+__device__ void clv_fix_body(const StageArgs& a) {
+  const int32_t cnt = a.fix_n[a.fix_stage];
+  for (int32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x) {
+    const int64_t i = a.fix_rows[q];
+    int sim = 0;
+    while (sim + 1 < a.nsims && a.g.sims[sim + 1].row_off <= i) ++sim;
+    const SimDev& sd = a.g.sims[sim];
+    double xi[3] = {0.0, 0.0, 0.0}, upre[3] = {0.0, 0.0, 0.0}, acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < D; ++c) { xi[c] = a.S[c][i]; upre[c] = a.U[c][i]; }
+    const double E = a.fix_stage == 1 ? fmax(sqrt(__longlong_as_double((long long)*a.emax)), a.clv_half) : a.clv_e1;
+    clv_walk<D, Law, PER>(a, sd, i, xi, E, acc);
+    double e2 = 0.0;
+    stage_finish<D, Law, NOISE, true>(a, sd, sim, i, xi, acc, upre, -1.0, &e2);
+    clv_report(a, e2);
+  }
+}
+
+// One integrator stage over the Verlet lists with the pair work spread
+// evenly over the block's lanes.  The entries of the block's agents are
+// numbered agent-major (a block-wide scan of the list lengths) and processed
+// in rounds of whole agents holding about `fchunk` entries: every lane
+// evaluates entries of the round -- gather, square root, division and law
+// of one pair per lane step -- and stores the term in LDS; then each
+// agent's thread adds its terms in list order (the order of the per-agent
+// walk, so the sums are bit-identical) and finishes the stage.  A wavefront
+// thus no longer runs to its longest list.  Lists are at most FLAT_MAX_CAP
+// long.
+template <int D, class Law, int PER, int NOISE>
+__device__ __forceinline__ void flat_body(const StageArgs& a, unsigned char* lds) {
+  if (a.gate && *a.gate != a.gate_run) return;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int chunk = a.fchunk;
+  const int ch = chunk + a.fcap;
+  double* term[3];
+  unsigned char* p = lds;
+#pragma unroll
+  for (int c = 0; c < D; ++c) { term[c] = (double*)p; p += (size_t)8 * ch; }
+  double* hx[3];
+#pragma unroll
+  for (int c = 0; c < D; ++c) { hx[c] = (double*)p; p += (size_t)8 * BRICK_MAX_BLOCK; }
+  int32_t* offs = (int32_t*)p;
+  p += (size_t)4 * (BRICK_MAX_BLOCK + 1);
+  int32_t* rfirst = (int32_t*)p;
+  p += (size_t)4 * (FLAT_MAX_ROUNDS + 1);
+  int32_t* wsum = (int32_t*)p;
+  p += 4 * 8;
+  uint8_t* owner = (uint8_t*)p;
+
+  const RowIdx ri = row_index(a.g);
+  const SimDev& sd = a.g.sims[ri.sim];
+  const int64_t N = a.g.st[ri.sim].N;
+  const int64_t i = ri.row;
+  const int64_t row0 = i - tid;                  // the block's rows are consecutive
+  const bool live = ri.local < N;
+  double xi[3] = {0.0, 0.0, 0.0}, upre[3] = {0.0, 0.0, 0.0};
+  int32_t n = 0;
+  if (live) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) xi[c] = a.S[c][i];
+    n = a.list_n[i];
+#pragma unroll
+    for (int c = 0; c < D; ++c) upre[c] = a.U[c][i];   // consumed by the stage update
+  }
+#pragma unroll
+  for (int c = 0; c < D; ++c) hx[c][tid] = xi[c];
+  for (int r = tid; r <= FLAT_MAX_ROUNDS; r += nt) rfirst[r] = 0x7FFFFFFF;
+  const int32_t off = block_excl_scan(n, wsum, wsum + 7);
+  offs[tid] = off;
+  // round of an agent: the chunk its first entry falls in; a round's entries
+  // run from its first agent's first entry to the next round's
+  const int myr = n > 0 ? off / chunk : -1;
+  __syncthreads();
+  if (n > 0) atomicMin(&rfirst[myr], off);
+  const int32_t P = wsum[7];
+  __syncthreads();
+
+  const Law law(sd.p);
+  double L[3], hL[3];
+  box_halves<D, PER>(sd, L, hL);
+  const int32_t* __restrict__ list = a.list;
+  const int64_t ldl = a.ldl;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int r = 0; r < FLAT_MAX_ROUNDS; ++r) {
+    const int32_t lo = rfirst[r];
+    if (lo >= P) break;
+    const int32_t nx = rfirst[r + 1];
+    const int32_t hi = nx < P ? nx : P;
+    if (myr == r)
+      for (int32_t k = 0; k < n; ++k) owner[off - lo + k] = (uint8_t)tid;
+    __syncthreads();
+    // every lane evaluates entries of the round, two in flight
+    for (int32_t p0 = lo + tid; p0 < hi; p0 += 2 * nt) {
+      int32_t pp[2], jj[2], tt[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        pp[u] = p0 + u * nt;
+        const bool v = pp[u] < hi;
+        tt[u] = v ? owner[pp[u] - lo] : tid;
+        const int32_t q = pp[u] - offs[tt[u]];
+        jj[u] = v ? list[(int64_t)q * ldl + row0 + tt[u]] : (int32_t)i;
+      }
+      double xj[2][3];
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int c = 0; c < D; ++c) xj[u][c] = a.S[c][jj[u]];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        if (pp[u] >= hi) continue;
+        double xo[3] = {0.0, 0.0, 0.0}, t[3];
+#pragma unroll
+        for (int c = 0; c < D; ++c) xo[c] = hx[c][tt[u]];
+        pair_term<D, Law, PER>(law, xo, xj[u], L, hL, t);
+#pragma unroll
+        for (int c = 0; c < D; ++c) term[c][pp[u] - lo] = t[c];
+      }
+    }
+    __syncthreads();
+    if (myr == r)
+      for (int32_t k = off - lo; k < off - lo + n; ++k)
+#pragma unroll
+        for (int c = 0; c < D; ++c) acc[c] = acc[c] + term[c][k];
+    __syncthreads();
+  }
+  if (!live) return;
+  stage_finish<D, Law, NOISE, true>(a, sd, ri.sim, i, xi, acc, upre);
 }
 
 }  // namespace cbm

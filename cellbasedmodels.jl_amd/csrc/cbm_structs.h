@@ -37,6 +37,13 @@ struct SimDev {
   int64_t cell_off, ncells;
   int64_t rank_off;     // offset of this simulation's cell -> Morton-rank table
   int64_t pad1;
+  // CellLinked on a skin list: the model's own cell grid (the reference's
+  // padded grid of edge cellEdge, or the periodic grid of edge L/n), whose
+  // cells order each agent's candidates; gedge/gn above are then the list
+  // build's grid
+  double cedge[3];
+  int32_t cgn[3];
+  int32_t pad2;
 };
 
 // Per-simulation mutable scalars (device array).
@@ -177,7 +184,47 @@ struct StageArgs {
   int32_t brick_cap;            // agents a brick block can stage in LDS
   int32_t wcap;                 // entries per agent of wlist
   int32_t gate_run;
+  int32_t fchunk;               // flattened list stage: pairs per LDS round
+  int32_t fcap;                 // flattened list stage: the list capacity (entries per agent)
+  // CellLinked on a skin list (clv != 0): packed cells (cl_pack) of the
+  // step-start positions on the model's grid, read by the stages; the final
+  // stage writes those of its output to ccell_out.  Stage 1 of a two-stage
+  // integrator adds its agents' |h - x_ref|^2 to *emax (bit pattern, max);
+  // stage 2 reads it.  Agents whose list may miss a pair take the exact
+  // walk (clv_walk): clv_e1 bounds |x - x_ref| at step start, clv_skin is the
+  // list margin.
+  const int32_t* ccell;
+  int32_t* ccell_out;
+  unsigned long long* emax;
+  const int32_t* cslot;         // logical slot of every row (clv_walk's order)
+  double clv_e1, clv_skin;
+  double clv_half, clv_half2;   // skin/2 (a little above) and its square: stage 1 reports
+                                // only agents past it, and E is at least clv_half
+  int32_t clv, lcap;            // lcap: entries of a full (possibly truncated) skin list
+  int32_t* fix_rows;            // agents deferred to the exact walk (clv_fix_body)
+  int32_t* fix_n;               // their count, per stage ([0] stage 1, [1] stage 2)
+  int32_t fix_stage, nsims;
 };
+
+// Flattened list stage (flat_body): per block, the agents' list entries are
+// numbered agent-major; every lane evaluates pairs of that sequence and an
+// agent's terms are then added in list order.  LDS: the pair terms of one
+// round (chunk + cap entries: a round takes whole agents), the home
+// positions, the agents' pair offsets, the rounds' first pairs, the scan's
+// wave sums and the owner (agent) of each pair of the round.
+constexpr int FLAT_MAX_CAP = 64;       // longer lists take the per-agent walk
+constexpr int FLAT_MAX_ROUNDS = (BRICK_MAX_BLOCK * FLAT_MAX_CAP) / 512 + 2;
+
+__host__ __device__ inline int64_t flat_lds_bytes(int D, int chunk, int cap) {
+  const int64_t ch = (int64_t)chunk + cap;
+  int64_t b = 8 * (int64_t)D * ch;                    // pair terms
+  b += 8 * (int64_t)D * BRICK_MAX_BLOCK;              // home positions
+  b += 4 * (int64_t)(BRICK_MAX_BLOCK + 1);            // pair offsets of the agents
+  b += 4 * (int64_t)(FLAT_MAX_ROUNDS + 1);            // first pair of every round
+  b += 4 * 8;                                         // wave sums, total
+  b += ch;                                            // owner of each pair (uint8)
+  return (b + 15) & ~(int64_t)15;
+}
 
 // Agent rules (agentRule with @addAgent/@removeAgent) and the canonical
 // compaction of update!; also the argument of the hipRTC user rule kernels.

@@ -176,7 +176,18 @@ typedef struct cbm_model_desc {
   int32_t user_de;       /* U0..U(user_de-1) are DE variables of the agentODE
                             body (UserLaw::de, dt(u) of functionDE.jl:13-116);
                             0..4, needs CBM_LAW_CUSTOM, no soft spheres     */
-  int32_t pad1;
+  int32_t walk;          /* 0: neighbor lists walked one agent per lane; a
+                            CellLinked model with a built-in law (LINEAR,
+                            REP_ADH, AGGREGATION) under Euler or Heun and no
+                            rules runs on a skin list (no per-step cell sort;
+                            each agent's entries ordered as the cell walk).
+                            1: as 0, but CellLinked sorts its grid every step
+                            and walks it on LDS brick tiles.  512..2048: as
+                            0, with list stages flattened over each block's
+                            lanes in rounds of that many list entries (a
+                            round's terms sit in LDS: 2048 keeps a 3-D block
+                            under 64 KiB).  The results are the same bits for
+                            every value.                                    */
 } cbm_model_desc;
 
 /* Per-simulation description.  One context may hold many independent
@@ -398,8 +409,9 @@ int cbm_set_block(cbm_ctx* ctx, int32_t block);
  * avg_ms[k] / launches[k] for k = 0..7: the integrator stage kernels
  * (0 Euler, 1 EM, 2 Heun stage 1, 3 Heun stage 2, 4-7 RK4 stages 1-4);
  * k = 8: the neighbor update of one step (total, launches = steps);
- * k = 9: whole step; k = 10, 11: EulerHeun stages 1 and 2.  ncat must be
- * >= 12. */
+ * k = 9: whole step; k = 10, 11: EulerHeun stages 1 and 2; k = 12: the
+ * step's cell-ordered lists of CellLinked on a skin list.  ncat must be
+ * >= 13. */
 int cbm_profile(cbm_ctx* ctx, int64_t nsteps, double* avg_ms, int64_t* launches, int32_t ncat);
 /* Average device time (ms) of the last cbm_step call's force kernels, from
  * HIP events on the context stream; *launches = force-kernel launches timed. */

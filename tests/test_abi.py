@@ -84,3 +84,14 @@ def test_bad_user_de_is_rejected(law, k):
     m = model(2, neighbors=A.FULL, law=law)
     m.user_de = k
     assert _create(m, sim(4, [[0, 8]] * 2)) == A.ERR_CONFIG
+
+
+@pytest.mark.parametrize("walk", [2, 300, 4096])
+def test_walk_option_outside_its_range_is_rejected(walk):
+    m = model(3, neighbors=A.VERLET_DISPLACEMENT, skin=2.0, max_neighbors=8)
+    m.walk = walk
+    s = sim(10, [[0, 10]] * 3)
+    h = C.c_void_p()
+    rc = A.lib().cbm_create(C.byref(h), C.byref(m), 1, C.byref(s))
+    assert rc == A.ERR_CONFIG
+    assert b"walk" in A.lib().cbm_last_error(h)

@@ -3,7 +3,13 @@ its tiling: a brick whose halo overflows the LDS capacity (it walks global
 memory instead), periodic grids of 3-5 cells per axis (a brick's halo wraps
 onto itself), 1-D grids, and block sizes smaller than a brick's population.
 The walk order is the reference's cellPosNeigh order (src/neighbors.jl), so
-the built-in laws are expected bit-identical to the oracle."""
+the built-in laws are expected bit-identical to the oracle.
+
+The same cases also run with walk = 0, where an Euler or Heun model takes
+CellLinked on a skin list (no per-step cell sort; each agent's skin-list
+entries are put in the cell walk's order): there the dense clusters give
+lists past the skin list's capacity and large stage-1 steps move agents past
+the list margin, and those agents take the exact cell-order walk."""
 import numpy as np
 import pytest
 
@@ -18,7 +24,11 @@ LONG_TOL = 1e-9
 REP_ADH = [1.0, 1.0, 0.1, 2.4]
 
 
-def pair(m, s, pos, block=None):
+WALKS = pytest.mark.parametrize("walk", [1, 0], ids=["bricks", "skinlist"])
+
+
+def pair(m, s, pos, block=None, walk=1):
+    m.walk = walk
     o = O.Oracle(m, s)
     g = Context(m, [s])
     if block:
@@ -50,7 +60,8 @@ def test_brick_geometry_is_morton_aligned(gpu_available):
         assert b["cap"] >= 256 and b["nbricks"] >= 1 and b["overflows"] == 0
 
 
-def test_brick_overflow_falls_back_bit_exact(gpu_available):
+@WALKS
+def test_brick_overflow_falls_back_bit_exact(gpu_available, walk):
     # 1,500 of 2,000 agents packed in a 4.8-wide cube: the bricks around it
     # hold far more agents than the capacity sized for the mean density
     rng = np.random.default_rng(11)
@@ -60,7 +71,7 @@ def test_brick_overflow_falls_back_bit_exact(gpu_available):
     for a in range(3):
         pos[a][:1500] = rng.uniform(17.6, 22.4, 1500)
     m = model(3, integrator=A.EULER, neighbors=A.CELL_LINKED, law=A.LAW_REP_ADH, dt=1e-4, cell_edge=2.4)
-    o, g = pair(m, sim(n, box, REP_ADH), pos)
+    o, g = pair(m, sim(n, box, REP_ADH), pos, walk=walk)
     assert g.bricks()["cap"] < 1500
     o.step(1)
     g.step(1)
@@ -71,8 +82,9 @@ def test_brick_overflow_falls_back_bit_exact(gpu_available):
     assert err(o, g, 3) <= LONG_TOL
 
 
+@WALKS
 @pytest.mark.parametrize("dims", [2, 3])
-def test_brick_dense_cluster_overflows_lane_queue(gpu_available, dims):
+def test_brick_dense_cluster_overflows_lane_queue(gpu_available, dims, walk):
     # 60 agents within 1.0 of a point: each has 59 candidates within the law's
     # reach, more than a lane queues, while the brick still fits in LDS
     rng = np.random.default_rng(7 + dims)
@@ -85,26 +97,29 @@ def test_brick_dense_cluster_overflows_lane_queue(gpu_available, dims):
         pos[a][:60] = 20.0 + u[a]
     m = model(dims, integrator=A.HEUN, neighbors=A.CELL_LINKED, law=A.LAW_REP_ADH, dt=1e-3, periodic=1,
               cell_edge=2.4)
-    o, g = pair(m, sim(n, box, REP_ADH), pos)
+    o, g = pair(m, sim(n, box, REP_ADH), pos, walk=walk)
     o.step(1)
     g.step(1)
     assert err(o, g, dims) == 0.0
     o.step(19)
     g.step(19)
     assert err(o, g, dims) <= LONG_TOL
-    assert g.bricks()["overflows"] == 0
+    if walk:
+        assert g.bricks()["overflows"] == 0
 
 
+@WALKS
 @pytest.mark.parametrize("gn", [3, 4, 5])
-def test_brick_small_periodic_grid(gpu_available, gn):
+def test_brick_small_periodic_grid(gpu_available, gn, walk):
     # a periodic grid narrower than a brick: the halo wraps onto the brick
     L = gn * 2.4 + 0.01
     n = int(0.9 * gn ** 3) + 5
     box = [[0.0, L]] * 3
     m = model(3, integrator=A.HEUN, neighbors=A.CELL_LINKED, law=A.LAW_REP_ADH, dt=0.005, periodic=1,
               cell_edge=2.4)
-    o, g = pair(m, sim(n, box, REP_ADH), uniform_positions(n, box, 20 + gn))
-    assert g.bricks()["bdim"][0] > gn
+    o, g = pair(m, sim(n, box, REP_ADH), uniform_positions(n, box, 20 + gn), walk=walk)
+    if walk:
+        assert g.bricks()["bdim"][0] > gn
     o.step(1)
     g.step(1)
     assert err(o, g, 3) == 0.0
@@ -113,13 +128,14 @@ def test_brick_small_periodic_grid(gpu_available, gn):
     assert err(o, g, 3) <= LONG_TOL
 
 
+@WALKS
 @pytest.mark.parametrize("periodic", [0, 1])
-def test_brick_1d(gpu_available, periodic):
+def test_brick_1d(gpu_available, periodic, walk):
     n, L = 400, 300.0
     box = [[0.0, L]]
     m = model(1, integrator=A.EULER, neighbors=A.CELL_LINKED, law=A.LAW_LINEAR, dt=0.01, periodic=periodic,
               cell_edge=2.0)
-    o, g = pair(m, sim(n, box, [1.0, 2.0]), uniform_positions(n, box, 3))
+    o, g = pair(m, sim(n, box, [1.0, 2.0]), uniform_positions(n, box, 3), walk=walk)
     o.step(1)
     g.step(1)
     assert err(o, g, 1) == 0.0
@@ -182,8 +198,9 @@ def test_verlet_brick_build_matches_oracle(gpu_available, cluster):
     assert g.rebuilds() == o.rebuilds
 
 
+@WALKS
 @pytest.mark.parametrize("integ,dt", [(A.HEUN, 0.005), (A.HEUN, 0.4), (A.EULER_HEUN, 0.005)])
-def test_cell_linked_two_stage_lists(gpu_available, integ, dt):
+def test_cell_linked_two_stage_lists(gpu_available, integ, dt, walk):
     # stage 1 writes each agent's candidates within 1.05x the law's reach and
     # stage 2 walks them as a list while no stage-1 displacement exceeds half
     # the margin; dt = 0.4 moves overlapping agents farther than that, so
@@ -192,7 +209,7 @@ def test_cell_linked_two_stage_lists(gpu_available, integ, dt):
     box = [[0.0, L]] * 3
     m = model(3, integrator=integ, neighbors=A.CELL_LINKED, law=A.LAW_REP_ADH, dt=dt, periodic=1, cell_edge=2.4,
               noise=0.05 if integ == A.EULER_HEUN else 0.0, seed=11)
-    o, g = pair(m, sim(n, box, REP_ADH), uniform_positions(n, box, 31))
+    o, g = pair(m, sim(n, box, REP_ADH), uniform_positions(n, box, 31), walk=walk)
     o.step(1)
     g.step(1)
     # the noise goes through the device's log/sin/cos: 1e-12 per step, not bits

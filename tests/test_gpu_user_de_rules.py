@@ -44,13 +44,19 @@ struct UserRule {
 """
 
 
-def decay(u, a, dt, steps):
+def decay(u, a, dt, steps, integ=A.EULER):
+    """dt(u) = -a u stepped by Euler / EM (no noise off the positions) or
+    Heun (src/integrators.jl:496-498, :724-727, :551-566)."""
     for _ in range(steps):
-        u = u + (0.0 - a * u) * dt
+        if integ == A.HEUN:
+            h = u + (0.0 - a * u) * dt
+            u = (h + (u + (0.0 - a * h) * dt)) / 2
+        else:
+            u = u + (0.0 - a * u) * dt
     return u
 
 
-@pytest.mark.parametrize("integ", [A.EULER, A.EM, A.HEUN])
+@pytest.mark.parametrize("integ", [A.EULER, A.EM, A.HEUN], ids=["euler", "em", "heun"])
 def test_daughters_of_a_user_rule_carry_their_user_de_values(gpu_available, integ):
     n, a, dt, steps = 300, 0.7, 0.05, 6
     box = [[0.0, 10.0]] * 2
@@ -73,16 +79,16 @@ def test_daughters_of_a_user_rule_carry_their_user_de_values(gpu_available, inte
     x = g.download(A.F_X, n2)
     # parents: slots 0..n-1, integrated every step
     assert np.array_equal(ids[:n], np.arange(1, n + 1))
-    assert np.array_equal(u[:n], decay(u0, a, dt, steps))
+    assert np.array_equal(u[:n], decay(u0, a, dt, steps, integ))
     # daughters: slot n + k for the k-th even id, added in the first step
     # with 2 u_parent at its start, then integrated steps - 1 times
     par = np.arange(1, n, 2)
     assert np.array_equal(ids[n:], n + 1 + np.arange(n // 2))
-    assert np.array_equal(u[n:], decay(2.0 * u0[par], a, dt, steps - 1))
+    assert np.array_equal(u[n:], decay(2.0 * u0[par], a, dt, steps - 1, integ))
     assert np.array_equal(x[n:], pos[0][par] + 0.25)
 
 
-@pytest.mark.parametrize("integ", [A.EULER, A.EM])
+@pytest.mark.parametrize("integ", [A.EULER, A.EM], ids=["euler", "em"])
 def test_growth_division_daughters_inherit_user_de_values(gpu_available, integ):
     """The built-in growth/division rule copies the parent's user variables
     into the daughter; with dt(u) = 0 every agent keeps its founder's u."""
