@@ -89,6 +89,7 @@ SIGNATURES = {
     "cbm_get_bricks": (C.c_int, [_P, _P, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "cbm_get_neighbor_lists": (C.c_int, [_P, C.c_int32, C.c_double, _P, _P, C.c_int64]),
     "cbm_count_neighbors": (C.c_int, [_P, C.c_int32, C.c_double, _P]),
+    "cbm_count_neighbors_noalloc": (C.c_int, [_P, C.c_int32, C.c_double, _P]),
     "cbm_adjoint": (C.c_int, [_P, C.POINTER(LossDesc), _P, _P]),
     "cbm_set_params": (C.c_int, [_P, C.c_int32, _P, C.c_int32]),
     "cbm_autotune": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int32)]),

@@ -12,6 +12,7 @@
 #include <limits>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 #include "cbm_internal.h"
 #include "cbm_device.h"
 #include <hip/hiprtc.h>
@@ -1298,6 +1299,37 @@ static int check_flags(cbm_ctx* c) {
   return CBM_OK;
 }
 
+// ------------------------------------------------- allocation-free counts
+// cbm_count_neighbors_noalloc writes its per-agent counts into this
+// file-scope device array instead of memory allocated per call.  Every
+// device the library runs on holds one copy (8 MiB) until the process ends.
+// In host code `scratch` names a host-side stand-in, not the device copy:
+// its device address is taken once per device with hipGetSymbolAddress at
+// cbm_create (that device current) and only that address is used.
+constexpr int64_t NOALLOC_MAX_AGENTS = 1 << 20;
+static __device__ __align__(256) int64_t scratch[NOALLOC_MAX_AGENTS];
+
+static std::mutex g_scratch_mu;
+static std::vector<int64_t*> g_scratch_ptr;   // by device ordinal; nullptr until set up
+
+// Device address of this device's scratch array (the current device is c->device).
+static int scratch_setup(cbm_ctx* c) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= c->device) g_scratch_ptr.resize((size_t)c->device + 1, nullptr);
+  if (g_scratch_ptr[(size_t)c->device]) return CBM_OK;
+  int64_t* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(c, CBM_ERR_HIP, std::string("scratch array address: ") + hipGetErrorString(e));
+  g_scratch_ptr[(size_t)c->device] = p;
+  return CBM_OK;
+}
+
+static int64_t* scratch_of(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return device >= 0 && device < (int)g_scratch_ptr.size() ? g_scratch_ptr[(size_t)device] : nullptr;
+}
+
 // ======================================================================= ABI
 extern "C" {
 
@@ -1311,6 +1343,7 @@ int cbm_create(cbm_ctx** out, const cbm_model_desc* model, int32_t n_sims, const
   c->S = n_sims;
   c->sims.assign(sims, sims + n_sims);
   int rc = init_ctx(c);
+  if (rc == CBM_OK) rc = scratch_setup(c);
   *out = c;   // returned even on failure so cbm_last_error can explain
   return rc;
 }
@@ -1842,6 +1875,39 @@ int cbm_get_neighbor_lists(cbm_ctx* c, int32_t sim, double radius, int32_t* coun
 int cbm_count_neighbors(cbm_ctx* c, int32_t sim, double radius, int64_t* out) {
   if (!out) return CBM_ERR_ARG;
   return export_sets(c, sim, radius, nullptr, nullptr, 0, out);
+}
+
+int cbm_count_neighbors_noalloc(cbm_ctx* c, int32_t sim, double radius, int64_t* out) {
+  if (!c || !out || sim < 0 || sim >= c->S) return CBM_ERR_ARG;
+  int64_t* counts = scratch_of(c->device);
+  if (!counts) return fail(c, CBM_ERR_HIP, "scratch array of this device not set up");
+  int64_t N;
+  int rc = cbm_get_n(c, sim, &N);
+  if (rc) return rc;
+  if (N > NOALLOC_MAX_AGENTS)
+    return fail(c, CBM_ERR_ARG, "cbm_count_neighbors_noalloc: " + std::to_string(N) +
+                                    " agents exceed its scratch array of 1048576; use cbm_count_neighbors");
+  if (!c->structure_valid && (rc = force_rebuild(c))) return rc;
+  // the kernel writes counts[slot] for every slot below N, so nothing left by
+  // an earlier call is read; the overflow word is unused without lists
+  ExportArgs e;
+  std::memset(&e, 0, sizeof(e));
+  e.s = stage_base(c);
+  for (int d = 0; d < c->dims; ++d) e.s.S[d] = (const double*)c->col[C_X + d].cur();
+  e.slot = (const int32_t*)c->col[C_SLOT].cur();
+  e.counts64 = counts;
+  e.radius = radius;
+  e.take_all = (c->verlet && radius <= 0.0) ? 1 : 0;
+  e.clv = c->clv ? 1 : 0;
+  e.e_bound = c->thr * (1.0 + 1e-6);
+  e.overflow = c->flags + 4;
+  e.only_sim = sim;
+  hipError_t le = launch_export(c->dims, c->nb_kind, c->nblocks, c->block, c->stream, e);
+  if (le == hipSuccess && N > 0)
+    le = hipMemcpyAsync(out, counts, sizeof(int64_t) * (size_t)N, hipMemcpyDeviceToHost, c->stream);
+  if (le == hipSuccess) le = hipStreamSynchronize(c->stream);
+  if (le != hipSuccess) return fail(c, CBM_ERR_HIP, std::string("neighbor counts: ") + hipGetErrorString(le));
+  return CBM_OK;
 }
 
 int cbm_set_params(cbm_ctx* c, int32_t sim, const double* p, int32_t n) {

@@ -363,6 +363,15 @@ int cbm_get_neighbor_lists(cbm_ctx* ctx, int32_t sim, double radius,
  * For Verlet models, radius <= 0 returns the list lengths instead. */
 int cbm_count_neighbors(cbm_ctx* ctx, int32_t sim, double radius, int64_t* out);
 
+/* cbm_count_neighbors without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code that must not call
+ * hipMalloc/hipFree, e.g. around graph capture or in tight serving loops).
+ * The counts go through one file-scope device array per GPU (8 MiB, held
+ * until the process ends), so at most 1,048,576 agents per call; a larger
+ * population returns CBM_ERR_ARG.  Calls that overlap on one GPU (contexts
+ * used from several host threads) share that array and race. */
+int cbm_count_neighbors_noalloc(cbm_ctx* ctx, int32_t sim, double radius, int64_t* out);
+
 /* --------------------------------------------------------------- adjoint */
 /* Gradient of a trajectory loss through the overdamped-Euler mechanics step
  * (no reference counterpart: the reference has no AD).  The loss is
