@@ -11,7 +11,8 @@ the timed region.  Prints one JSON line.
 
 --workload em selects the Euler-Maruyama variant (isotropic noise D = 0.01,
 Philox keyed by (seed, step, agent id)); --workload heun_cl runs the same
-Heun model on a cell-linked grid of edge 2.4 rebuilt every step.
+Heun model on a cell-linked grid of edge 2.4 (CellLinked loop order, walked
+on a skin list of radius 2.7).
 """
 import argparse
 import json
@@ -67,7 +68,8 @@ def workload(kind, n):
     for q, v in enumerate([1.0, 1.0, 0.1, 2.4]):
         s.p[q] = v
     if kind == "heun_cl":
-        # the same model on a cell grid of edge 2.4 (the law's cutoff), rebuilt every step
+        # the same model on a cell grid of edge 2.4 (the law's cutoff), walked
+        # in CellLinked order on a skin list
         m.neighbors = A.CELL_LINKED
         m.cell_edge = 2.4
     rng = np.random.default_rng(2)
@@ -136,13 +138,15 @@ def measure(kind, n, steps, warmup, graph, profile_steps):
     # (4 + 4 kbar) and x_ref (8 D) per agent and rebuild
     rebuild_bytes = (CL_REBUILD_BYTES + 4 + 4 * kbar + 24) * rebuilds / steps
     step_bytes = (sum(sb.values()) + rebuild_bytes) * s.n
-    dom_name = {2: "k_flat Heun stage 1", 3: "k_flat Heun stage 2", 1: "k_flat EM"}[dom]
+    kern = "k_clv" if kind == "heun_cl" else "k_stage"
+    dom_name = {2: f"{kern} Heun stage 1", 3: f"{kern} Heun stage 2", 1: f"{kern} EM"}[dom]
     ctx.close()
     return {
         "value": s.n * steps / wall,
         "steps": steps,
         "ms_per_step": wall / steps * 1e3,
-        "config": {"workload": ("C2 3-D cell mechanics, HEUN, cell-linked (edge 2.4, rebuilt every step)"
+        "config": {"workload": ("C2 3-D cell mechanics, HEUN, cell-linked (edge 2.4) on a skin list "
+                                "(radius 2.7, rebuilt at 0.15 displacement)"
                                 if kind == "heun_cl" else
                                 f"C2 3-D cell mechanics, {kind.upper()}, Verlet r_list 2.7, rebuild at 0.15"),
                    "N": s.n, "box": L, "dt": m.dt, "graph_steps": graph},
@@ -508,7 +512,7 @@ def main():
     ap.add_argument("--workload", choices=["heun", "em", "heun_cl"], default="heun")
     ap.add_argument("--graph", type=int, default=0, help="capture this many steps per hipGraph replay")
     ap.add_argument("--cl", type=int, default=500, metavar="STEPS",
-                    help="also time the same model on a cell-linked grid rebuilt every step (0 skips)")
+                    help="also time the same model on a cell-linked grid of edge 2.4 (0 skips)")
     ap.add_argument("--em", type=int, default=1, help="also time the C2 Euler-Maruyama variant")
     ap.add_argument("--c1", type=int, default=1, help="also time C1 on the device and the oracle")
     ap.add_argument("--no-cpu", action="store_true")
@@ -559,7 +563,7 @@ def main():
         print(f"bench: {name}", file=sys.stderr, flush=True)
 
     if args.cl and args.workload != "heun_cl":
-        phase("C2 on a cell-linked grid rebuilt every step")
+        phase("C2 on a cell-linked grid")
         rec["cl"] = measure("heun_cl", args.n, args.cl, args.warmup, 0, args.profile_steps)
     if args.em:
         phase("C2 Euler-Maruyama variant")

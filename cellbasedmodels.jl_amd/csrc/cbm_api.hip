@@ -13,10 +13,19 @@
 #include <vector>
 #include <algorithm>
 #include <mutex>
+#include <chrono>
 #include "cbm_internal.h"
 #include "cbm_device.h"
 #include <hip/hiprtc.h>
 #include "cbm_rtc_src.inc"
+
+// Skin-list builds scheduled by the host (clv_schedule): the host runs at
+// most CLV_HOST_LEAD steps ahead of the device, and a final stage cues a
+// build when CLV_AHEAD more displacements like its step's would carry an
+// agent past half the skin (the lead, the step that ends with the build and
+// one and a half more).
+constexpr int64_t CLV_HOST_LEAD = 3;
+constexpr double CLV_AHEAD = 2.5 + (double)CLV_HOST_LEAD;
 
 
 using namespace cbm;
@@ -59,6 +68,16 @@ struct cbm_ctx {
   int32_t* wlist_n = nullptr;
   int32_t* fix_rows = nullptr;              // agents deferred to the exact walk, and
   int32_t* fix_n = nullptr;                 // their counts per stage
+  unsigned long long* efin = nullptr;       // displacement bound and build cue since the last build
+  // the host schedules the skin-list builds of eager steps from the record
+  // the order kernel writes each step (mapped host memory); graph capture
+  // keeps the device-side trigger
+  ClvRecord* rec_host = nullptr;
+  ClvRecord* rec_dev = nullptr;
+  int64_t last_build_step = -1;             // step whose end saw the last build
+  int64_t rec_first = 0;                    // first step whose record the device writes
+  bool capturing = false;
+  bool build_now = false;
   int nb_kind = NB_FULL;
   int block = 256;
   int device = 0;
@@ -412,6 +431,11 @@ static void choose_clv(cbm_ctx* c) {
            (m.law == CBM_LAW_LINEAR || m.law == CBM_LAW_REP_ADH || m.law == CBM_LAW_AGGREGATION) &&
            (m.integrator == CBM_EULER || m.integrator == CBM_HEUN);
   if (!c->clv) return;
+  // the order kernel packs a row into 23 bits and addresses the lists by
+  // 32-bit byte offsets: at most 2^23 rows
+  int64_t rows = 0;
+  for (int s = 0; s < c->S; ++s) rows += c->sims[s].n_max;
+  if (rows > ((int64_t)1 << 23)) { c->clv = false; return; }
   double reach = 0.0;
   for (int s = 0; s < c->S; ++s) reach = std::max(reach, law_reach(m.law, c->sims[s].p));
   const double rl = (reach + 0.125 * reach) * (1.0 + 1e-9);
@@ -561,6 +585,8 @@ static void destroy_ctx(cbm_ctx* c) {
   for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  F(c->efin);
+  if (c->rec_host) (void)hipHostFree(c->rec_host);
   if (c->user_mod) (void)hipModuleUnload(c->user_mod);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -764,6 +790,12 @@ static int init_ctx(cbm_ctx* c) {
     HIPCHK(c, dalloc(&c->wlist_n, c->C));
     HIPCHK(c, dalloc(&c->fix_rows, c->C));
     HIPCHK(c, dalloc(&c->fix_n, 2));
+    HIPCHK(c, dalloc(&c->efin, 2));
+    HIPCHK(c, hipMemset(c->efin, 0, 2 * sizeof(unsigned long long)));
+    HIPCHK(c, hipHostMalloc((void**)&c->rec_host, sizeof(ClvRecord), hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(c->rec_host, 0, sizeof(ClvRecord));
+    c->rec_host->step = -1;
+    HIPCHK(c, hipHostGetDevicePointer((void**)&c->rec_dev, c->rec_host, 0));
   }
   int64_t maxn = 1;
   for (int s = 0; s < c->S; ++s) maxn = std::max(maxn, c->sims[s].n_max);
@@ -807,6 +839,7 @@ static RebuildArgs rebuild_args(cbm_ctx* c, const int32_t* flag, int32_t* flag_c
   a.geometric = (c->verlet || c->clv) ? 1 : 0;
   a.emax_clear = c->emax;
   a.fix_clear = c->fix_n;
+  a.efin_clear = c->efin;
   return a;
 }
 
@@ -897,6 +930,9 @@ static int force_rebuild(cbm_ctx* c) {
       for (int d = 0; d < c->dims; ++d) a.S[d] = (const double*)c->col[C_X + d].cur();
       a.ccell_out = (int32_t*)c->col[C_CCELL].cur();
       HIPCHK(c, launch_clcell(c->dims, c->nblocks, c->block, c->stream, a));
+      // records from this step on describe the new lists
+      c->last_build_step = c->step - 1;
+      c->rec_first = c->step;
     }
   } else {
     int rc = rebuild_chain(c, nullptr, nullptr, false, false);
@@ -939,6 +975,8 @@ static StageArgs stage_base(cbm_ctx* c) {
     a.fix_n = c->fix_n;
     a.nsims = c->S;
     a.lcap = c->cap;
+    a.efin = c->efin;
+    a.clv_ahead = CLV_AHEAD;
   }
   a.dt = c->m.dt; a.sqdt = std::sqrt(c->m.dt); a.drift = c->m.drift; a.noise = c->m.noise; a.thr2 = sq_threshold(c->thr);
   a.seed = c->m.seed;
@@ -1174,14 +1212,37 @@ static int medium_phase(cbm_ctx* c) {
   return CBM_OK;
 }
 
+// Skin-list builds of eager steps.  The host stays at most CLV_HOST_LEAD
+// steps ahead of the device: before step s it waits until the order kernel
+// of step s - CLV_HOST_LEAD has written its record.  When a record written
+// after the last build carries the cue (a final stage predicted an agent to
+// pass half the skin within CLV_AHEAD step displacements, enough for the
+// lead and one step and a half more), the build is enqueued at the end of
+// step s.  A late build costs time, never exactness: stage 1 bounds every
+// displacement by the reported maximum and sends agents whose list may miss
+// a pair to the exact walk.
+static void clv_schedule(cbm_ctx* c) {
+  c->build_now = false;
+  volatile ClvRecord* r = c->rec_host;
+  const int64_t want = c->step - CLV_HOST_LEAD;
+  if (want >= c->rec_first) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int64_t spins = 1; r->step < want; ++spins)
+      if (spins % 4096 == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(10)) break;
+  }
+  if (r->step > c->last_build_step && r->soon != 0ull) c->build_now = true;
+}
+
 static int step_once(cbm_ctx* c) {
   int rc;
   if (!c->structure_valid && (rc = force_rebuild(c))) return rc;
   const int64_t s = c->step;
+  const bool clv_host = c->clv && !c->capturing;
+  if (clv_host) clv_schedule(c);
   if (c->rules && (rc = rule_phase(c, c->flags + (s & 1)))) return rc;
   StageArgs a = stage_base(c);
   int32_t* flag_now = c->flags + (s & 1);
-  const bool disp = (c->verlet && !c->vtime) || c->clv;
+  const bool disp = (c->verlet && !c->vtime) || (c->clv && !clv_host);
   auto set_state = [&](double* const* S, double* const* U, double* const* out) {
     for (int v = 0; v < c->nvar; ++v) { a.S[v] = S[v]; a.U[v] = U[v]; a.out[v] = out[v]; a.kacc[v] = c->kacc[v]; }
   };
@@ -1195,6 +1256,7 @@ static int step_once(cbm_ctx* c) {
   if (c->clv) {
     StageArgs o = a;
     o.wlist = c->wlist; o.wlist_n = c->wlist_n;
+    if (clv_host) { o.book = 1; o.rec = c->rec_dev; o.step_no = s; o.emax = c->emax; }
     prof_begin(c, PROF_ORDER);
     HIPCHK(c, launch_clv_order(c->dims, c->m.periodic, c->nblocks, c->block, c->stream, o));
     prof_end(c);
@@ -1205,7 +1267,7 @@ static int step_once(cbm_ctx* c) {
     case CBM_EM:
       set_state(cur, cur, alt);
       a.disp_check = disp ? 1 : 0;
-      if (c->clv) { a.fix_stage = 0; a.ccell_out = ccell_next; }
+      if (c->clv) { a.fix_stage = 0; a.ccell_out = ccell_next; a.tick = clv_host ? c->counters + 1 : nullptr; }
       if ((rc = run_stage(c, a, c->m.integrator == CBM_EULER ? ST_EULER : ST_EM))) return rc;
       flip_state(c);
       break;
@@ -1218,7 +1280,10 @@ static int step_once(cbm_ctx* c) {
       set_state(c->h[0], cur, cur);
       a.disp_check = disp ? 1 : 0;
       listed_stage(c, a, 2);
-      if (c->clv) { a.fix_stage = 1; a.emax = c->emax; a.ccell_out = ccell_next; a.list = c->wlist; a.list_n = c->wlist_n; }
+      if (c->clv) {
+        a.fix_stage = 1; a.emax = c->emax; a.ccell_out = ccell_next; a.list = c->wlist; a.list_n = c->wlist_n;
+        a.tick = clv_host ? c->counters + 1 : nullptr;
+      }
       if ((rc = run_stage(c, a, ST_HEUN2))) return rc;
       break;
     case CBM_EULER_HEUN:
@@ -1273,6 +1338,14 @@ static int neighbor_update(cbm_ctx* c, int64_t s, bool disp) {
   int32_t* flag_now = c->flags + (s & 1);
   if (c->cl && !c->clv) {
     if ((rc = rebuild_chain(c, nullptr, nullptr, true, false))) return rc;
+  } else if (c->clv && !c->capturing) {
+    // skin list, eager steps: the build the host scheduled, unconditional
+    // (the step's last fix kernel has advanced the device step)
+    if (c->build_now) {
+      if ((rc = rebuild_chain(c, nullptr, nullptr, false, true))) return rc;
+      c->last_build_step = s;
+      c->build_now = false;
+    }
   } else if (disp) {
     if ((rc = rebuild_chain(c, flag_now, c->flags + ((s + 1) & 1), true, true))) return rc;
   } else if (c->vtime && c->t_next <= c->t) {   // src/neighbors.jl:255
@@ -1631,7 +1704,9 @@ int cbm_step_graph(cbm_ctx* c, int64_t nsteps, int64_t replays) {
   hipGraphExec_t exec = nullptr;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int rc = CBM_OK;
+  c->capturing = true;   // the device decides every build in a graph
   for (int64_t k = 0; k < nsteps && rc == CBM_OK; ++k) rc = step_once(c);
+  c->capturing = false;
   hipError_t ce = hipStreamEndCapture(c->stream, &graph);
   if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
   if (ce != hipSuccess) return fail(c, CBM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
@@ -1655,6 +1730,10 @@ int cbm_step_graph(cbm_ctx* c, int64_t nsteps, int64_t replays) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   (void)hipGraphExecDestroy(exec);
   (void)hipGraphDestroy(graph);
+  // eager steps after the replays: their records describe the lists the
+  // replays left (the bound and the cue count from the last build)
+  c->last_build_step = c->step - 1;
+  c->rec_first = c->step;
   return CBM_OK;
 }
 
@@ -1826,6 +1905,17 @@ int cbm_get_rebuilds(cbm_ctx* c, int64_t* rebuilds) {
   return CBM_OK;
 }
 
+// Bound on every agent's distance from its skin-list reference position:
+// half the skin (a little over), or the largest displacement the final
+// stages reported past it since the last build (read from the device).
+static int clv_bound(cbm_ctx* c, double* eb) {
+  unsigned long long e = 0ull;
+  HIPCHK(c, hipMemcpyAsync(&e, c->efin, sizeof(e), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *eb = std::max(c->thr * (1.0 + 1e-6), std::sqrt(__builtin_bit_cast(double, e)) * (1.0 + 1e-9));
+  return CBM_OK;
+}
+
 static int export_sets(cbm_ctx* c, int32_t sim, double radius, int32_t* counts, int64_t* lists, int64_t cap, int64_t* counts64) {
   if (!c || sim < 0 || sim >= c->S) return CBM_ERR_ARG;
   if (!c->structure_valid) { int rc = force_rebuild(c); if (rc) return rc; }
@@ -1850,7 +1940,7 @@ static int export_sets(cbm_ctx* c, int32_t sim, double radius, int32_t* counts, 
   e.counts = d_counts; e.lists = d_lists; e.counts64 = d_c64; e.cap = cap; e.radius = radius;
   e.take_all = (c->verlet && (lists || radius <= 0.0)) ? 1 : 0;
   e.clv = c->clv ? 1 : 0;
-  e.e_bound = c->thr * (1.0 + 1e-6);
+  if (c->clv && (rc = clv_bound(c, &e.e_bound))) return rc;
   e.overflow = d_ovf;
   e.only_sim = sim;
   int ovf = 0;
@@ -1899,7 +1989,7 @@ int cbm_count_neighbors_noalloc(cbm_ctx* c, int32_t sim, double radius, int64_t*
   e.radius = radius;
   e.take_all = (c->verlet && radius <= 0.0) ? 1 : 0;
   e.clv = c->clv ? 1 : 0;
-  e.e_bound = c->thr * (1.0 + 1e-6);
+  if (c->clv && (rc = clv_bound(c, &e.e_bound))) return rc;
   e.overflow = c->flags + 4;
   e.only_sim = sim;
   hipError_t le = launch_export(c->dims, c->nb_kind, c->nblocks, c->block, c->stream, e);

@@ -43,6 +43,7 @@ struct RebuildArgs {
   int32_t dims, periodic, full, geometric;
   unsigned long long* emax_clear;   // cleared by k_keys every step (may be NULL)
   int32_t* fix_clear;               // two counters cleared by k_keys every step (may be NULL)
+  unsigned long long* efin_clear;   // two words cleared by k_keys when it rebuilds (may be NULL)
 };
 
 struct ScanArgs {

@@ -73,6 +73,16 @@ __global__ __launch_bounds__(256) void k_flat(StageArgs a) {
 template <int D, int PER>
 __global__ __launch_bounds__(256) void k_clv_order(StageArgs a) {
   extern __shared__ unsigned char clv_lds[];
+  if (a.book && blockIdx.x == 0 && threadIdx.x == 0) {
+    // the step's bookkeeping, and the record the host reads (system-scope
+    // stores into mapped host memory, the step last)
+    *a.emax = 0ull;
+    a.fix_n[0] = 0;
+    a.fix_n[1] = 0;
+    __hip_atomic_store(&a.rec->soon, a.efin[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&a.rec->e2max, a.efin[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&a.rec->step, a.step_no, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
   clv_order_body<D, PER>(a, clv_lds);
 }
 
@@ -83,6 +93,7 @@ __global__ __launch_bounds__(256) void k_clv(StageArgs a) {
 
 template <int D, int LAW, int PER, int NOISE>
 __global__ __launch_bounds__(256) void k_clv_fix(StageArgs a) {
+  if (a.tick && blockIdx.x == 0 && threadIdx.x == 0) *a.tick += 1;   // the step's last kernel
   if (a.fix_n[a.fix_stage] == 0) return;
   clv_fix_body<D, PairLaw<LAW>, PER, NOISE>(a);
 }
@@ -103,7 +114,10 @@ __global__ __launch_bounds__(256) void k_keys(RebuildArgs a) {
   const RowIdx ri = row_index(a.g);
   const SimDev& sd = a.g.sims[ri.sim];
   const SimState& st = a.g.st[ri.sim];
-  if (blockIdx.x == 0 && threadIdx.x == 0 && a.rebuilds) *a.rebuilds += 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (a.rebuilds) *a.rebuilds += 1;
+    if (a.efin_clear) { a.efin_clear[0] = 0ull; a.efin_clear[1] = 0ull; }
+  }
   if (ri.local == 0) a.count[sd.cell_off + sd.ncells + 1] = (int32_t)(sd.n_max - st.n_tot);
   if (ri.local >= st.n_tot) return;
   const int64_t r = ri.row;

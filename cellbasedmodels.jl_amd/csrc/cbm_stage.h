@@ -464,7 +464,7 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
     if (!(s2 <= disp2)) *a.lgate = 0;
   }
 
-  if (final_stage && a.disp_check) {
+  if (final_stage && (a.disp_check || (a.clv && a.efin))) {
     double dx[3];
 #pragma unroll
     for (int c = 0; c < D; ++c) {
@@ -474,19 +474,32 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
     double s2 = dx[0] * dx[0];
     if (D > 1) s2 = s2 + dx[1] * dx[1];
     if (D > 2) s2 = s2 + dx[2] * dx[2];
-    if (s2 > a.thr2) *a.flag = 1;   // sqrt(s2) >= thr, src/neighbors.jl:403 (>= skin/2)
+    double st = 0.0;   // this step's displacement, squared
     if (a.clv) {
-      // CellLinked on a skin list rebuilds one step early: when this step's
-      // displacement, repeated 1.5 times, would carry the agent past the
-      // threshold, so that the next step's stages rarely leave half the skin
-      double st = 0.0;
 #pragma unroll
       for (int c = 0; c < D; ++c) {
         double d = res[c] - Uc(c);
         if (a.periodic) d = min_image(d, sd.L[c]);
         st = st + d * d;
       }
-      if (!(sqrt(s2) + 1.5 * sqrt(st) <= a.clv_half)) *a.flag = 1;
+    }
+    if (a.disp_check) {
+      if (s2 > a.thr2) *a.flag = 1;   // sqrt(s2) >= thr, src/neighbors.jl:403 (>= skin/2)
+      // CellLinked on a skin list rebuilds one step early: when this step's
+      // displacement, repeated 1.5 times, would carry the agent past the
+      // threshold, so that the next step's stages rarely leave half the skin
+      if (a.clv && !(sqrt(s2) + 1.5 * sqrt(st) <= a.clv_half)) *a.flag = 1;
+    }
+    if (a.clv && a.efin) {
+      // the displacement bound of the next step's stage 1 (agents past half
+      // the skin; a value at or below the running maximum adds nothing), and
+      // the host's cue to schedule a build: an agent that clv_ahead more
+      // displacements like this step's would carry past half the skin
+      if (s2 > a.clv_half2) {
+        const unsigned long long e = (unsigned long long)__double_as_longlong(s2);
+        if (e > *(volatile unsigned long long*)a.efin) atomicMax(a.efin, e);
+      }
+      if (!(sqrt(s2) + a.clv_ahead * sqrt(st) <= a.clv_half)) a.efin[1] = 1ull;
     }
   }
 }
@@ -1007,6 +1020,18 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
   }
 }
 
+// An element at a 32-bit byte offset from a base that all lanes share (a
+// kernel argument): the access takes the base from scalar registers and the
+// offset from one vector register, with no 64-bit address arithmetic.
+template <class T>
+__device__ __forceinline__ T ld_b(const T* base, uint32_t off) {
+  return *(const T*)((const char*)base + off);
+}
+template <class T>
+__device__ __forceinline__ void st_b(T* base, uint32_t off, T v) {
+  *(T*)((char*)base + off) = v;
+}
+
 // Batcher's odd-even merge sort of N keys held in registers (every index is
 // a compile-time constant once the loops unroll).
 template <int N, class T>
@@ -1128,10 +1153,11 @@ __device__ void clv_walk(const StageArgs& a, const SimDev& sd, int64_t i, const 
 // CellLinked on a skin list, one agent per lane.  Per step:
 //   clv_order_body: each agent's skin-list entries inside its 3^D neighbor
 //     cells are put in CellLinked order -- cell k among the neighbor cells,
-//     then list position (= slot order) -- by insertion of the 16-bit keys
+//     then list position (= slot order) -- and written as the step's list
+//     (a.wlist): lists of at most 16 entries by a sorting network over keys
+//     (k, position, row) in registers, longer ones by insertion of the keys
 //     (k << 6 | position) into the lane's LDS queue (stride blockDim, a.lcap
-//     entries) and written as the step's list (a.wlist); a list at capacity
-//     (possibly truncated) is marked -1.
+//     entries); a list at capacity (possibly truncated) is marked -1.
 //   clv_stage_body: every stage walks that list.  An agent whose list may
 //     miss a pair -- marked -1, or (stage 2 of a two-stage integrator) whose
 //     distance e from the reference positions plus the largest, E, over all
@@ -1149,6 +1175,42 @@ __device__ __forceinline__ void clv_defer(const StageArgs& a, int64_t i) {
 // e^2 of a stage-1 output past half the skin, into the device maximum.
 __device__ __forceinline__ void clv_report(const StageArgs& a, double e2) {
   if (a.emax && !(e2 <= a.clv_half2)) atomicMax(a.emax, (unsigned long long)__double_as_longlong(e2));
+}
+
+// The ordered list of an agent with n <= W skin-list entries (see
+// clv_order_body); returns its length.  The list holds lcap >= 16 entries
+// per row, so all W loads are in bounds; entries past the length are
+// replaced by the agent itself and sorted last.
+template <int W, int D, int PER>
+__device__ __forceinline__ int32_t clv_order_small(const StageArgs& a, const SimDev& sd, int64_t i, int32_t n,
+                                                   int32_t ci) {
+  uint32_t key[W];
+  const uint32_t i4 = (uint32_t)i * 4u, l4 = (uint32_t)a.ldl * 4u;
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+    const int32_t raw = ld_b(a.list, i4 + (uint32_t)u * l4);
+    key[u] = (uint32_t)(u < n ? raw : (int32_t)i);
+  }
+  // all W gathers are issued before the first key: the empty asm uses each
+  // loaded value unconditionally, so the compiler cannot move a load into
+  // the (u < n) branch behind its own wait
+  int32_t cj[W];
+#pragma unroll
+  for (int u = 0; u < W; ++u) cj[u] = ld_b(a.ccell, key[u] * 4u);
+#pragma unroll
+  for (int u = 0; u < W; ++u) __asm__ volatile("" ::"v"(cj[u]));
+  int32_t kept = 0;
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+    const int k = u < n ? cl_key<D, PER>(sd, ci, cj[u]) : 27;   // 27: outside the 3^D cells, sorts last
+    key[u] |= ((uint32_t)k << 27) | ((uint32_t)u << 23);
+    kept += k < 27 ? 1 : 0;
+  }
+  sort_net<W>(key);
+#pragma unroll
+  for (int v = 0; v < W; ++v)
+    if (v < kept) st_b(a.wlist, i4 + (uint32_t)v * l4, (int32_t)(key[v] & 0x7FFFFFu));
+  return kept;
 }
 
 template <int D, int PER>
@@ -1176,37 +1238,14 @@ __device__ __forceinline__ void clv_order_body(const StageArgs& a, unsigned char
     qk[pos * nt] = key;
     ++kept;
   };
-  constexpr int W = 16;
-  if (n <= W) {
-    // every list read and cell gather in flight, the keys sorted by a
-    // network in registers, the rows picked from registers
-    // the list holds lcap >= W entries per row, so all W loads are in bounds;
-    // entries past the length are replaced by the agent itself, unread
-    int32_t jj[W], key[W];
-    const int32_t* e = lp;
-#pragma unroll
-    for (int u = 0; u < W; ++u, e += ldl) {
-      const int32_t raw = *e;
-      jj[u] = u < n ? raw : (int32_t)i;
-    }
-#pragma unroll
-    for (int u = 0; u < W; ++u) {
-      const int32_t cj = a.ccell[jj[u]];
-      const int k = u < n ? cl_key<D, PER>(sd, ci, cj) : 27;
-      key[u] = (k << 6) | u;                       // k = 27: outside the 3^D cells, sorts last
-      kept += k < 27 ? 1 : 0;
-    }
-    sort_net<W>(key);
-    int32_t* w = a.wlist + i;
-#pragma unroll
-    for (int v = 0; v < W; ++v, w += ldl) {
-      const int q = key[v] & 63;
-      int32_t j = jj[0];
-#pragma unroll
-      for (int u = 1; u < W; ++u) j = q == u ? jj[u] : j;
-      if (v < kept) *w = j;
-    }
-  } else {
+  // lists of at most 16 entries: every list read and cell gather in flight,
+  // and the keys (k << 27 | position << 23 | row) sorted by a network in
+  // registers; the row rides in the key (the set-up allows this form below
+  // 2^23 rows), and 32-bit byte offsets address from the kernel-argument
+  // bases.  (A form for 8 entries beside it gains nothing: the lanes of a
+  // wave rarely all hold lists that short.)
+  if (n <= 16) kept = clv_order_small<16, D, PER>(a, sd, i, n, ci);
+  else {
     for (int32_t q0 = 0; q0 < n; q0 += 4) {
       int32_t jj[4], cc[4];
 #pragma unroll
@@ -1243,6 +1282,23 @@ __device__ __forceinline__ void clv_stage_body(const StageArgs& a) {
 #pragma unroll
   for (int c = 0; c < D; ++c) xi[c] = a.S[c][i];
   if (n < 0) { clv_defer(a, i); return; }
+  if (a.fix_stage == 0 && a.efin) {
+    // stage 1: every agent is within E0 of its reference position, E0 the
+    // largest reported displacement (agents past half the skin report) or a
+    // little over half the skin; with no report all pairs are in the lists
+    const unsigned long long e0 = *a.efin;
+    if (e0 != 0ull) {
+      const double E0 = fmax(sqrt(__longlong_as_double((long long)e0)), a.clv_e1);
+      double s2 = 0.0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        double d = xi[c] - a.xref[c][i];
+        if (PER) d = min_image(d, sd.L[c]);
+        s2 = s2 + d * d;
+      }
+      if (!(sqrt(s2) + E0 <= a.clv_skin)) { clv_defer(a, i); return; }
+    }
+  }
   if (a.fix_stage == 1) {
     // stage 2: could a pair active here lie outside the list?
     double dx[3];
@@ -1304,7 +1360,8 @@ __device__ void clv_fix_body(const StageArgs& a) {
     double xi[3] = {0.0, 0.0, 0.0}, upre[3] = {0.0, 0.0, 0.0}, acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int c = 0; c < D; ++c) { xi[c] = a.S[c][i]; upre[c] = a.U[c][i]; }
-    const double E = a.fix_stage == 1 ? fmax(sqrt(__longlong_as_double((long long)*a.emax)), a.clv_half) : a.clv_e1;
+    const double E = a.fix_stage == 1 ? fmax(sqrt(__longlong_as_double((long long)*a.emax)), a.clv_half)
+                                      : fmax(a.efin ? sqrt(__longlong_as_double((long long)*a.efin)) : 0.0, a.clv_e1);
     clv_walk<D, Law, PER>(a, sd, i, xi, E, acc);
     double e2 = 0.0;
     stage_finish<D, Law, NOISE, true>(a, sd, sim, i, xi, acc, upre, -1.0, &e2);

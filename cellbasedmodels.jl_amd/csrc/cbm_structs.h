@@ -139,6 +139,18 @@ __host__ __device__ inline int64_t vbrick_lds_bytes(int D, int cap, int H) {
   return (b + 15) & ~(int64_t)15;
 }
 
+// What the skin-list order kernel reports to the host each step (host
+// memory mapped into the device): the step it ran for and, as of the end of
+// the previous step and since the last list build, whether a final stage
+// predicted an agent to pass half the skin soon (the host then schedules a
+// build) and the largest squared displacement reported (bit pattern).
+struct ClvRecord {
+  int64_t step;
+  unsigned long long soon;
+  unsigned long long e2max;
+  int64_t pad;
+};
+
 struct StageArgs {
   RowGeom g;
   const double* S[MAX_STATE];   // stage state read by the force (neighbors + self)
@@ -204,6 +216,20 @@ struct StageArgs {
   int32_t* fix_rows;            // agents deferred to the exact walk (clv_fix_body)
   int32_t* fix_n;               // their count, per stage ([0] stage 1, [1] stage 2)
   int32_t fix_stage, nsims;
+  // since the last list build (both cleared by it): efin[0], the largest
+  // |x - x_ref|^2 past half the skin of a final stage (bit pattern, max;
+  // stage 1 bounds the step-start displacement by it); efin[1], set when a
+  // final stage predicts an agent to pass half the skin within clv_ahead
+  // step displacements
+  unsigned long long* efin;
+  double clv_ahead;
+  // per-step bookkeeping when the host schedules the list builds: the order
+  // kernel clears emax and fix_n and writes rec (step_no, efin) for the
+  // host; the last fix kernel advances *tick (the device step)
+  ClvRecord* rec;
+  int64_t* tick;
+  int64_t step_no;
+  int32_t book, pad3;
 };
 
 // Flattened list stage (flat_body): per block, the agents' list entries are

@@ -138,3 +138,37 @@ def test_law_parameters_may_shrink_but_not_outgrow_the_lists(gpu_available):
     o.step(20)
     for a in range(3):
         assert np.array_equal(g.download(a, s.n), o.get(a)), a
+
+
+def test_eager_steps_and_graph_replays_agree(gpu_available):
+    """Eager steps schedule the list builds from the host (the record the
+    order kernel writes), graph replays from the device flag: the two may
+    build at different steps, and the positions must still be equal."""
+    m, s, pos = c2_cl(6000, seed=31)
+    g = device(m, [s], [pos], 0)
+    h = device(m, [s], [pos], 0)
+    g.step(120)
+    h.step_graph(4, 30)
+    for a in range(3):
+        assert np.array_equal(g.download(a, s.n), h.download(a, s.n)), a
+    assert g.rebuilds() >= 2 and h.rebuilds() >= 2
+
+
+def test_late_builds_stay_exact(gpu_available):
+    """Agents that cover a fifth of the skin per step outrun any build the
+    host can schedule a few steps ahead: stage 1 bounds every displacement
+    by the reported maximum and the agents whose list may miss a pair take
+    the exact walk, so the result is the oracle's cell walk bit for bit."""
+    n, L = 1500, 45.0
+    m = model(2, integrator=A.EULER, neighbors=A.CELL_LINKED, law=A.LAW_LINEAR, dt=0.02, periodic=1,
+              cell_edge=2.0)
+    s = sim(n, [[0.0, L]] * 2, [3.0, 2.0])
+    pos = uniform_positions(n, [[0.0, L]] * 2, 33)
+    g = device(m, [s], [pos], 0)
+    o = oracle(m, s, pos)
+    for k in (3, 17, 40):
+        g.step(k)
+        o.step(k)
+        for a in range(2):
+            assert np.array_equal(g.download(a, n), o.get(a)), (k, a)
+    assert g.rebuilds() >= 4
