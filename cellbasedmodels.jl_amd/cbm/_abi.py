@@ -21,6 +21,7 @@ FULL, CELL_LINKED, VERLET_DISPLACEMENT, VERLET_TIME, CLVD = 0, 1, 2, 3, 4
 LAW_NONE, LAW_LINEAR, LAW_REP_ADH, LAW_AGGREGATION, LAW_CUSTOM, LAW_SOFT_SPHERES = 0, 1, 2, 3, 4, 5
 RULE_NONE, RULE_GROWTH_DIVISION, RULE_PARITY, RULE_DEATH, RULE_CUSTOM = 0, 1, 2, 3, 4
 MEDIUM_FTCS, MEDIUM_DGADI = 0, 1
+BORDER_MIRROR, BORDER_DIRICHLET, BORDER_COPY = 0, 1, 2
 
 F_X, F_Y, F_Z, F_ID, F_FX, F_FY, F_FZ, F_VOL, F_GROWTH, F_VBIRTH, F_LF = range(11)
 F_VX, F_VY, F_VZ, F_RADIUS, F_MASS, F_FATE = range(11, 17)
@@ -38,6 +39,7 @@ class ModelDesc(C.Structure):
         ("medium_n", C.c_int32 * 3), ("medium_alg", C.c_int32),
         ("medium_D", C.c_double), ("medium_consume", C.c_double), ("medium_c0", C.c_double),
         ("user_de", C.c_int32), ("walk", C.c_int32),
+        ("medium_border", C.c_int32 * 6), ("medium_border_value", C.c_double * 6),
     ]
 
 
@@ -90,6 +92,7 @@ SIGNATURES = {
     "cbm_get_neighbor_lists": (C.c_int, [_P, C.c_int32, C.c_double, _P, _P, C.c_int64]),
     "cbm_count_neighbors": (C.c_int, [_P, C.c_int32, C.c_double, _P]),
     "cbm_count_neighbors_noalloc": (C.c_int, [_P, C.c_int32, C.c_double, _P]),
+    "cbm_compile_check": (C.c_int, [_P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, _P, C.c_int64]),
     "cbm_adjoint": (C.c_int, [_P, C.POINTER(LossDesc), _P, _P]),
     "cbm_set_params": (C.c_int, [_P, C.c_int32, _P, C.c_int32]),
     "cbm_autotune": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int32)]),

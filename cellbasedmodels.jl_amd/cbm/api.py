@@ -201,7 +201,11 @@ class ABM:
     medium      {"NMedium": [...], "D", "consume", "c0"} for the built-in
                 diffusion with uptake; "mediumODE": a HIP C++ body defining
                 struct UserMedium (dt(c) at a node) replaces that law;
-                "mediumAlg": "FTCS" (default) or "DGADI" (Douglas-Gunn ADI)
+                "mediumAlg": "FTCS" (default) or "DGADI" (Douglas-Gunn ADI);
+                "borders": one entry per axis side (x-, x+, y-, y+, z-, z+;
+                missing ones "mirror"): "mirror" (zero flux), "copy" (the
+                reference's `p = p[2, i2_]` idiom: the inward neighbor's
+                value) or a number (a Dirichlet value, `p = v`)
     lagged      dt(x) reads the previous step's committed force accumulator, the
                 reference's read-old/write-new idiom (agentStructure.jl:352-453)
     periodic    periodic box with minimum-image distances
@@ -277,6 +281,17 @@ class ABM:
             m.medium_consume = float(self.medium.get("consume", 0.0))
             m.medium_c0 = float(self.medium.get("c0", 0.0))
             m.medium_alg = {"FTCS": A.MEDIUM_FTCS, "DGADI": A.MEDIUM_DGADI}[self.medium.get("mediumAlg", "FTCS")]
+            borders = list(self.medium.get("borders", ()))
+            if len(borders) > 2 * self.dims:
+                raise ValueError(f"borders: at most {2 * self.dims} sides for a {self.dims}-D medium")
+            for k, b in enumerate(borders):
+                if isinstance(b, str):
+                    if b not in ("mirror", "copy"):
+                        raise ValueError(f"unknown medium border {b!r}")
+                    m.medium_border[k] = A.BORDER_COPY if b == "copy" else A.BORDER_MIRROR
+                else:
+                    m.medium_border[k] = A.BORDER_DIRICHLET
+                    m.medium_border_value[k] = float(b)
         return m
 
 

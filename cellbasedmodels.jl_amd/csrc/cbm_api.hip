@@ -69,6 +69,7 @@ struct cbm_ctx {
   int32_t* fix_rows = nullptr;              // agents deferred to the exact walk, and
   int32_t* fix_n = nullptr;                 // their counts per stage
   unsigned long long* efin = nullptr;       // displacement bound and build cue since the last build
+  unsigned long long* med_add = nullptr;    // a rule body's medium additions per node (32.32 fixed point)
   // the host schedules the skin-list builds of eager steps from the record
   // the order kernel writes each step (mapped host memory); graph capture
   // keeps the device-side trigger
@@ -271,6 +272,15 @@ static int validate(cbm_ctx* c) {
     if (m.medium_alg == CBM_MEDIUM_DGADI)
       for (int a = 0; a < m.dims; ++a)
         if (m.medium_n[a] < 4) return fail(c, CBM_ERR_CONFIG, "DGADI needs NMedium >= 4 on every axis");
+    for (int k = 0; k < 6; ++k) {
+      const int32_t b = m.medium_border[k];
+      if (b < CBM_BORDER_MIRROR || b > CBM_BORDER_COPY) return fail(c, CBM_ERR_CONFIG, "unknown medium border");
+      if (b != CBM_BORDER_MIRROR && k >= 2 * m.dims) return fail(c, CBM_ERR_CONFIG, "medium border on an axis the model does not have");
+      if (b != CBM_BORDER_MIRROR && m.medium_alg == CBM_MEDIUM_DGADI)
+        return fail(c, CBM_ERR_CONFIG, "DGADI holds its boundary at the initial field: medium borders need FTCS");
+      if (b == CBM_BORDER_DIRICHLET && !std::isfinite(m.medium_border_value[k]))
+        return fail(c, CBM_ERR_CONFIG, "a Dirichlet medium border needs a finite value");
+    }
   }
   for (int s = 0; s < c->S; ++s) {
     const cbm_sim_desc& sd = c->sims[s];
@@ -586,6 +596,7 @@ static void destroy_ctx(cbm_ctx* c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   F(c->efin);
+  F(c->med_add);
   if (c->rec_host) (void)hipHostFree(c->rec_host);
   if (c->user_mod) (void)hipModuleUnload(c->user_mod);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -785,11 +796,13 @@ static int init_ctx(cbm_ctx* c) {
     HIPCHK(c, dalloc(&c->list_n, c->C));
   }
   if (c->clv) {
-    HIPCHK(c, dalloc(&c->emax, 2));
+    HIPCHK(c, dalloc(&c->emax, 2));   // the stage-1 maximum of |h - x_ref|^2, by step parity
+    HIPCHK(c, hipMemset(c->emax, 0, 2 * sizeof(unsigned long long)));
     HIPCHK(c, dalloc(&c->wlist, (int64_t)c->cap * c->C));
     HIPCHK(c, dalloc(&c->wlist_n, c->C));
     HIPCHK(c, dalloc(&c->fix_rows, c->C));
     HIPCHK(c, dalloc(&c->fix_n, 2));
+    HIPCHK(c, hipMemset(c->fix_n, 0, 2 * sizeof(int32_t)));
     HIPCHK(c, dalloc(&c->efin, 2));
     HIPCHK(c, hipMemset(c->efin, 0, 2 * sizeof(unsigned long long)));
     HIPCHK(c, hipHostMalloc((void**)&c->rec_host, sizeof(ClvRecord), hipHostMallocMapped | hipHostMallocCoherent));
@@ -837,8 +850,9 @@ static RebuildArgs rebuild_args(cbm_ctx* c, const int32_t* flag, int32_t* flag_c
   a.cell_row = c->cell_row;
   a.dims = c->dims; a.periodic = c->m.periodic; a.full = c->full ? 1 : 0;
   a.geometric = (c->verlet || c->clv) ? 1 : 0;
-  a.emax_clear = c->emax;
-  a.fix_clear = c->fix_n;
+  // the skin-list stages clear their own words; other modes keep none
+  a.emax_clear = c->clv ? nullptr : c->emax;
+  a.fix_clear = c->clv ? nullptr : c->fix_n;
   a.efin_clear = c->efin;
   return a;
 }
@@ -1000,7 +1014,7 @@ static StageArgs stage_base(cbm_ctx* c) {
 // PROF_CHAIN for the neighbor update of one step.
 constexpr int PROF_CHAIN = 8;
 constexpr int PROF_STEP = 9;
-constexpr int PROF_ORDER = 12;   // CellLinked on a skin list: the step's ordered lists
+constexpr int PROF_ORDER = 12;   // no launch of its own now: the first skin-list stage orders the lists
 constexpr int PROF_NCAT = 13;
 // profile category of a stage kind: 0..7 as the kind, EulerHeun stages 10, 11
 static int prof_cat_of(int kind) { return kind == ST_EH1 ? 10 : (kind == ST_EH2 ? 11 : kind); }
@@ -1113,6 +1127,16 @@ static RuleArgs rule_args(cbm_ctx* c, int32_t* flag) {
   r.holes = c->holes;
   r.err = c->flags + 3;
   r.rebuild_flag = flag;
+  if (c->medium) {
+    r.med_c = c->med_c[c->med_cur];
+    r.med_add = c->med_add;
+    for (int a = 0; a < 3; ++a) {
+      r.med_n[a] = a < c->dims ? c->m.medium_n[a] : 1;
+      r.med_lo[a] = c->sims[0].box_lo[a];
+      r.med_hi[a] = c->sims[0].box_hi[a];
+      r.med_dx[a] = a < c->dims ? (r.med_hi[a] - r.med_lo[a]) / (double)r.med_n[a] : 1.0;   // communityStructure.jl:274
+    }
+  }
   return r;
 }
 
@@ -1199,6 +1223,8 @@ static int medium_phase(cbm_ctx* c) {
   md.D = c->m.medium_D; md.uptake = c->m.medium_consume; md.dt = c->m.dt;
   md.nodes = c->med_nodes;
   md.lap = c->adi ? 0 : 1;
+  for (int k = 0; k < 6; ++k) { md.bmode[k] = c->m.medium_border[k]; md.bval[k] = c->m.medium_border_value[k]; }
+  if (c->med_add) HIPCHK(c, launch_medium_inject(c->stream, c->med_c[c->med_cur], c->med_add, c->med_nodes));
   HIPCHK(c, launch_medium_count(c->nblocks, c->block, c->stream, md));
   if (c->user_medium) {
     void* params[] = {(void*)&md};
@@ -1250,24 +1276,32 @@ static int step_once(cbm_ctx* c) {
   double* alt[MAX_STATE] = {};
   for (int v = 0; v < c->nvar; ++v) { cur[v] = statecol(c, v, false); alt[v] = statecol(c, v, true); }
   a.flag = flag_now;
-  // CellLinked on a skin list: the step's lists in cell-walk order, walked by
-  // every stage; the final stage writes the next cells
+  // CellLinked on a skin list: the first stage orders the skin list into the
+  // step's lists in cell-walk order (a.wlist), every stage walks them; the
+  // final stage writes the next cells
   int32_t* ccell_next = c->clv ? (int32_t*)c->col[C_CCELL].alt() : nullptr;
-  if (c->clv) {
-    StageArgs o = a;
-    o.wlist = c->wlist; o.wlist_n = c->wlist_n;
-    if (clv_host) { o.book = 1; o.rec = c->rec_dev; o.step_no = s; o.emax = c->emax; }
-    prof_begin(c, PROF_ORDER);
-    HIPCHK(c, launch_clv_order(c->dims, c->m.periodic, c->nblocks, c->block, c->stream, o));
-    prof_end(c);
-    a.clv = 1; a.list = c->wlist; a.list_n = c->wlist_n;
-  }
+  auto clv_args = [&](int first, int final_stage, int queue) {
+    if (!c->clv) return;
+    a.clv = 1;
+    a.list = c->list; a.list_n = c->list_n;
+    a.wlist = c->wlist; a.wlist_n = c->wlist_n;
+    a.emax = c->emax + (s & 1);
+    a.emax_next = c->emax + ((s + 1) & 1);
+    a.clv_order = first;
+    a.fix_stage = first ? 0 : 1;
+    a.fix_q = queue;
+    a.book = (first && clv_host) ? 1 : 0;
+    a.rec = c->rec_dev;
+    a.step_no = s;
+    a.ccell_out = final_stage ? ccell_next : nullptr;
+    a.tick = (final_stage && clv_host) ? c->counters + 1 : nullptr;
+  };
   switch (c->m.integrator) {
     case CBM_EULER:
     case CBM_EM:
       set_state(cur, cur, alt);
       a.disp_check = disp ? 1 : 0;
-      if (c->clv) { a.fix_stage = 0; a.ccell_out = ccell_next; a.tick = clv_host ? c->counters + 1 : nullptr; }
+      clv_args(1, 1, (int)(s & 1));
       if ((rc = run_stage(c, a, c->m.integrator == CBM_EULER ? ST_EULER : ST_EM))) return rc;
       flip_state(c);
       break;
@@ -1275,15 +1309,12 @@ static int step_once(cbm_ctx* c) {
       set_state(cur, cur, c->h[0]);
       a.disp_check = 0;
       listed_stage(c, a, 1);
-      if (c->clv) { a.fix_stage = 0; a.emax = c->emax; a.list = c->wlist; a.list_n = c->wlist_n; }
+      clv_args(1, 0, 0);
       if ((rc = run_stage(c, a, ST_HEUN1))) return rc;
       set_state(c->h[0], cur, cur);
       a.disp_check = disp ? 1 : 0;
       listed_stage(c, a, 2);
-      if (c->clv) {
-        a.fix_stage = 1; a.emax = c->emax; a.ccell_out = ccell_next; a.list = c->wlist; a.list_n = c->wlist_n;
-        a.tick = clv_host ? c->counters + 1 : nullptr;
-      }
+      clv_args(0, 1, 1);
       if ((rc = run_stage(c, a, ST_HEUN2))) return rc;
       break;
     case CBM_EULER_HEUN:
@@ -1368,6 +1399,7 @@ static int check_flags(cbm_ctx* c) {
   if (f[2]) return fail(c, CBM_ERR_LIST_OVERFLOW, "Verlet list overflow: raise nMaxNeighbors");
   if (f[3] & 2) return fail(c, CBM_ERR_CONFIG, "an agentRule body added more than 2 agents for one agent in one step");
   if (f[3] & 4) return fail(c, CBM_ERR_CONFIG, "an agentRule body looped over neighbors without naming forNeighbors");
+  if (f[3] & 8) return fail(c, CBM_ERR_CONFIG, "an agentRule body used the medium in a model without a medium");
   if (f[3]) return fail(c, CBM_ERR_CAPACITY, "agent capacity exceeded: raise preallocation (n_max)");
   return CBM_OK;
 }
@@ -1429,26 +1461,21 @@ const char* cbm_last_error(const cbm_ctx* c) { return c ? c->err.c_str() : "null
 // UserNoise or UserRule, see include/cbm_hip.h) is compiled by hipRTC into
 // the same skeletons as the built-in laws and rules, instantiated for this
 // model's dimension, neighbor walk and boundary.
-int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, const char* rule_body,
-                      const char* medium_body) {
-  if (!c) return CBM_ERR_ARG;
-  const bool custom = c->m.law == CBM_LAW_CUSTOM;
-  if (custom != (law_body != nullptr))
-    return fail(c, CBM_ERR_CONFIG, "a law body is required with law CBM_LAW_CUSTOM and only then");
-  if ((c->m.rule == CBM_RULE_CUSTOM) != (rule_body != nullptr))
-    return fail(c, CBM_ERR_CONFIG, "a rule body is required with rule CBM_RULE_CUSTOM and only then");
-  if (noise_body && c->m.integrator != CBM_EM && c->m.integrator != CBM_EULER_HEUN)
-    return fail(c, CBM_ERR_CONFIG, "an agentSDE body needs the EM or EulerHeun integrator");
-  if (medium_body && !c->medium) return fail(c, CBM_ERR_CONFIG, "a mediumODE body needs a medium (medium_n)");
-  if (!law_body && !noise_body && !rule_body && !medium_body) return fail(c, CBM_ERR_ARG, "nothing to compile");
+}  // extern "C"
+
+// The hipRTC program of cbm_compile_model: the user bodies in the skeletons
+// of a model of this dimension, neighbor walk (nb, one of NB_*) and box.
+static std::string user_source(const cbm_model_desc& m, int nb_kind, bool growth, const char* law_body,
+                               const char* noise_body, const char* rule_body, const char* medium_body) {
+  const int dims = m.dims;
   const bool stages = law_body || noise_body;
   std::string src = "#define CBM_HIPRTC 1\n";
-  if (c->m.user_de > 0)
-    src += "#define CBM_USER_DE " + std::to_string(c->m.user_de) + "\n#define CBM_DE_BASE " +
-           std::to_string(c->dims + (c->growth ? 1 : 0)) + "\n";
+  if (m.user_de > 0)
+    src += "#define CBM_USER_DE " + std::to_string(m.user_de) + "\n#define CBM_DE_BASE " +
+           std::to_string(dims + (growth ? 1 : 0)) + "\n";
   src += "#include \"cbm_rule.h\"\n#include \"cbm_medium.h\"\nnamespace cbm {\n";
   if (law_body) { src += law_body; src += "\n"; }
-  if (law_body && c->m.user_de > 0)
+  if (law_body && m.user_de > 0)
     src += "static_assert(has_de<UserLaw>::value, \"user_de > 0 needs UserLaw::de\");\n";
   if (noise_body) { src += noise_body; src += "\n"; }
   if (rule_body) { src += rule_body; src += "\n"; }
@@ -1464,49 +1491,96 @@ int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, 
            "  cbm::user_rule_apply<cbm::UserRule>(a);\n}\n";
   char line[256];
   if (law_body) src += "using cbm_law_t = cbm::UserLaw;\n";
-  else { std::snprintf(line, sizeof(line), "using cbm_law_t = cbm::PairLaw<%d>;\n", c->m.law); src += line; }
+  else { std::snprintf(line, sizeof(line), "using cbm_law_t = cbm::PairLaw<%d>;\n", m.law); src += line; }
   src += noise_body ? "using cbm_noise_t = cbm::UserNoise;\n" : "using cbm_noise_t = cbm::ConstNoise;\n";
   // the same skeletons as the built-in laws: no neighbor walk without a law
-  const bool walk = c->m.law != CBM_LAW_NONE;
-  const int nb = walk ? c->nb_kind : NB_FULL;
-  const int per = walk && c->m.periodic ? 1 : 0;
+  const bool walk = m.law != CBM_LAW_NONE;
+  const int nb = walk ? nb_kind : NB_FULL;
+  const int per = walk && m.periodic ? 1 : 0;
   char inst[512];
   for (int noise = 0; stages && noise < 2; ++noise) {
     std::snprintf(inst, sizeof(inst),
                   "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_stage_%d(cbm::StageArgs a) {\n"
                   "  cbm::stage_body<%d, %d, cbm_law_t, %d, %d, cbm_noise_t>(a);\n}\n",
-                  noise, c->dims, nb, per, noise);
+                  noise, dims, nb, per, noise);
     src += inst;
     if (nb == NB_CELL) {   // the brick-tiled cell-linked stage, run before the row kernel
       std::snprintf(inst, sizeof(inst),
                     "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_brick_%d(cbm::StageArgs a) {\n"
                     "  extern __shared__ unsigned char lds[];\n"
                     "  cbm::brick_body<%d, cbm_law_t, %d, %d, cbm_noise_t>(a, lds);\n}\n",
-                    noise, c->dims, per, noise);
+                    noise, dims, per, noise);
       src += inst;
     }
   }
+  return src;
+}
+
+// hipRTC of a user-body program for gfx950; the log on failure.
+static hiprtcResult user_compile(const std::string& src, std::vector<char>* code, std::string* log) {
   const char* headers[] = {cbm_rtc_cbm_device_h, cbm_rtc_cbm_structs_h, cbm_rtc_cbm_stage_h, cbm_rtc_cbm_rule_h,
                            cbm_rtc_cbm_medium_h};
   const char* names[] = {"cbm_device.h", "cbm_structs.h", "cbm_stage.h", "cbm_rule.h", "cbm_medium.h"};
   hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, src.c_str(), "cbm_user_bodies.hip", 5, headers, names) != HIPRTC_SUCCESS)
-    return fail(c, CBM_ERR_COMPILE, "hiprtcCreateProgram failed");
+  hiprtcResult r = hiprtcCreateProgram(&prog, src.c_str(), "cbm_user_bodies.hip", 5, headers, names);
+  if (r != HIPRTC_SUCCESS) { *log = "hiprtcCreateProgram failed"; return r; }
   const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math"};
-  hiprtcResult r = hiprtcCompileProgram(prog, 5, opts);
+  r = hiprtcCompileProgram(prog, 5, opts);
   size_t logn = 0;
   hiprtcGetProgramLogSize(prog, &logn);
-  std::string log(logn + 1, '\0');
-  if (logn) hiprtcGetProgramLog(prog, &log[0]);
-  if (r != HIPRTC_SUCCESS) {
-    hiprtcDestroyProgram(&prog);
-    return fail(c, CBM_ERR_COMPILE, std::string("hipRTC: ") + log.c_str());
+  std::string l(logn + 1, '\0');
+  if (logn) hiprtcGetProgramLog(prog, &l[0]);
+  *log = l.c_str();
+  if (r == HIPRTC_SUCCESS && code) {
+    size_t code_n = 0;
+    hiprtcGetCodeSize(prog, &code_n);
+    code->resize(code_n);
+    hiprtcGetCode(prog, code->data());
   }
-  size_t code_n = 0;
-  hiprtcGetCodeSize(prog, &code_n);
-  std::vector<char> code(code_n);
-  hiprtcGetCode(prog, code.data());
   hiprtcDestroyProgram(&prog);
+  return r;
+}
+
+extern "C" {
+
+int cbm_compile_check(const cbm_model_desc* model, const char* law_body, const char* noise_body,
+                      const char* rule_body, const char* medium_body, char* log, int64_t log_cap) {
+  if (!model || (log && log_cap < 1)) return CBM_ERR_ARG;
+  if (!law_body && !noise_body && !rule_body && !medium_body) return CBM_ERR_ARG;
+  const cbm_model_desc& m = *model;
+  if (m.dims < 1 || m.dims > 3 || m.neighbors < CBM_FULL || m.neighbors > CBM_CLVD) return CBM_ERR_CONFIG;
+  const int nb = m.neighbors == CBM_FULL ? NB_FULL : (m.neighbors == CBM_CELL_LINKED ? NB_CELL : NB_LIST);
+  const bool growth = m.rule == CBM_RULE_GROWTH_DIVISION || m.rule == CBM_RULE_CUSTOM;
+  std::string lg;
+  const hiprtcResult r = user_compile(user_source(m, nb, growth, law_body, noise_body, rule_body, medium_body),
+                                      nullptr, &lg);
+  if (log) {
+    const size_t n = std::min(lg.size(), (size_t)log_cap - 1);
+    std::memcpy(log, lg.data(), n);
+    log[n] = '\0';
+  }
+  return r == HIPRTC_SUCCESS ? CBM_OK : CBM_ERR_COMPILE;
+}
+
+int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, const char* rule_body,
+                      const char* medium_body) {
+  if (!c) return CBM_ERR_ARG;
+  const bool custom = c->m.law == CBM_LAW_CUSTOM;
+  if (custom != (law_body != nullptr))
+    return fail(c, CBM_ERR_CONFIG, "a law body is required with law CBM_LAW_CUSTOM and only then");
+  if ((c->m.rule == CBM_RULE_CUSTOM) != (rule_body != nullptr))
+    return fail(c, CBM_ERR_CONFIG, "a rule body is required with rule CBM_RULE_CUSTOM and only then");
+  if (noise_body && c->m.integrator != CBM_EM && c->m.integrator != CBM_EULER_HEUN)
+    return fail(c, CBM_ERR_CONFIG, "an agentSDE body needs the EM or EulerHeun integrator");
+  if (medium_body && !c->medium) return fail(c, CBM_ERR_CONFIG, "a mediumODE body needs a medium (medium_n)");
+  if (!law_body && !noise_body && !rule_body && !medium_body) return fail(c, CBM_ERR_ARG, "nothing to compile");
+  std::vector<char> code;
+  std::string log;
+  const hiprtcResult r = user_compile(user_source(c->m, c->nb_kind, c->growth, law_body, noise_body, rule_body,
+                                                  medium_body), &code, &log);
+  if (r != HIPRTC_SUCCESS) return fail(c, CBM_ERR_COMPILE, std::string("hipRTC: ") + log);
+  const bool stages = law_body || noise_body;
+  const int nb = c->m.law != CBM_LAW_NONE ? c->nb_kind : NB_FULL;
   if (c->user_mod) {
     // steps queued on the stream may still run kernels of the old module
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1525,6 +1599,11 @@ int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, 
     if (std::strstr(rule_body, "forNeighbors"))
       for (int k = 0; k < 8; ++k)
         if (!c->rule_nb[k]) HIPCHK(c, dalloc(&c->rule_nb[k], c->C));
+    // a body adding to the medium sums its additions per node
+    if (std::strstr(rule_body, "medium_add") && c->medium && !c->med_add) {
+      HIPCHK(c, dalloc(&c->med_add, c->med_nodes));
+      HIPCHK(c, hipMemset(c->med_add, 0, sizeof(unsigned long long) * (size_t)c->med_nodes));
+    }
   }
   if (stages) {
     HIPCHK(c, hipModuleGetFunction(&c->user_fn[0], c->user_mod, "cbm_user_stage_0"));

@@ -175,6 +175,7 @@ hipError_t launch_adjoint_seed(int dims, int nblocks, int block, hipStream_t s, 
 hipError_t launch_loss(int nblocks, int block, hipStream_t s, const LossArgs& a);
 hipError_t launch_disp_check(int dims, int nblocks, int block, hipStream_t s, const StageArgs& a);
 
+hipError_t launch_medium_inject(hipStream_t s, double* c, unsigned long long* add, int64_t nodes);
 hipError_t launch_medium_count(int nblocks, int block, hipStream_t s, const MediumArgs& a);
 hipError_t launch_medium_step(hipStream_t s, const MediumArgs& a);
 hipError_t launch_adi_sweep(hipStream_t s, const AdiArgs& a);
@@ -218,7 +219,6 @@ hipError_t launch_xfer(int nblocks, int block, hipStream_t s, const XferArgs& a)
 hipError_t launch_snapshot(int nblocks, int block, hipStream_t s, const SnapArgs& a);
 hipError_t launch_export(int dims, int nb, int nblocks, int block, hipStream_t s, const ExportArgs& e);
 hipError_t launch_clcell(int dims, int nblocks, int block, hipStream_t s, const StageArgs& a);
-hipError_t launch_clv_order(int dims, int periodic, int nblocks, int block, hipStream_t s, const StageArgs& a);
 
 }  // namespace cbm
 

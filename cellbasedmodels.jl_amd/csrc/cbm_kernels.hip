@@ -70,31 +70,16 @@ __global__ __launch_bounds__(256) void k_flat(StageArgs a) {
 // CellLinked on a skin list: the step's ordered lists (dynamic LDS: the
 // per-lane key queues, blockDim x lcap uint16), the stages over them, and
 // the exact walk of the agents they deferred
-template <int D, int PER>
-__global__ __launch_bounds__(256) void k_clv_order(StageArgs a) {
-  extern __shared__ unsigned char clv_lds[];
-  if (a.book && blockIdx.x == 0 && threadIdx.x == 0) {
-    // the step's bookkeeping, and the record the host reads (system-scope
-    // stores into mapped host memory, the step last)
-    *a.emax = 0ull;
-    a.fix_n[0] = 0;
-    a.fix_n[1] = 0;
-    __hip_atomic_store(&a.rec->soon, a.efin[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_store(&a.rec->e2max, a.efin[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_store(&a.rec->step, a.step_no, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-  }
-  clv_order_body<D, PER>(a, clv_lds);
-}
-
 template <int D, int LAW, int PER, int NOISE>
 __global__ __launch_bounds__(256) void k_clv(StageArgs a) {
-  clv_stage_body<D, PairLaw<LAW>, PER, NOISE>(a);
+  extern __shared__ unsigned char clv_lds[];
+  clv_stage_body<D, PairLaw<LAW>, PER, NOISE>(a, clv_lds);
 }
 
 template <int D, int LAW, int PER, int NOISE>
 __global__ __launch_bounds__(256) void k_clv_fix(StageArgs a) {
   if (a.tick && blockIdx.x == 0 && threadIdx.x == 0) *a.tick += 1;   // the step's last kernel
-  if (a.fix_n[a.fix_stage] == 0) return;
+  if (a.fix_n[a.fix_q] == 0) return;
   clv_fix_body<D, PairLaw<LAW>, PER, NOISE>(a);
 }
 
@@ -970,6 +955,18 @@ __global__ __launch_bounds__(256) void k_medium_count(MediumArgs m) {
   atomicAdd(&m.count[q[0] + (int64_t)m.n[0] * (q[1] + (int64_t)m.n[1] * q[2])], 1);
 }
 
+// The rules' medium additions of the step (32.32 fixed point per node) land
+// in the field before its update; the sums are cleared for the next step.
+__global__ __launch_bounds__(256) void k_medium_inject(double* c, unsigned long long* add, int64_t nodes) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nodes) return;
+  const long long v = (long long)add[k];
+  if (v != 0) {
+    c[k] = c[k] + (double)v * (1.0 / MEDIUM_ADD_SCALE);
+    add[k] = 0;
+  }
+}
+
 // FTCS: c' = c + dt (D sum_a (c[+a] - 2c + c[-a])/dx_a^2 - (uptake c) n_agents),
 // ghost nodes mirroring the boundary node (zero flux, conserves mass).
 __global__ __launch_bounds__(256) void k_medium_step(MediumArgs m) { medium_step_body<BuiltinMedium>(m); }
@@ -1179,17 +1176,19 @@ static hipError_t launch_stage_law(int law, const StageLaunch& L, const StageArg
 template <int D, int PER, int NOISE>
 static hipError_t launch_clv_stage(int law, const StageLaunch& L, const StageArgs& a) {
   const dim3 fg(L.grid.x < 64 ? L.grid.x : 64);
+  // the ordering stage keeps lists over 16 entries in LDS (16-bit keys)
+  const size_t lds = a.clv_order ? (size_t)L.block.x * (size_t)a.lcap * 2 : 0;
   switch (law) {
     case LAW_LINEAR:
-      hipLaunchKernelGGL((k_clv<D, LAW_LINEAR, PER, NOISE>), L.grid, L.block, 0, L.s, a);
+      hipLaunchKernelGGL((k_clv<D, LAW_LINEAR, PER, NOISE>), L.grid, L.block, lds, L.s, a);
       hipLaunchKernelGGL((k_clv_fix<D, LAW_LINEAR, PER, NOISE>), fg, L.block, 0, L.s, a);
       break;
     case LAW_REP_ADH:
-      hipLaunchKernelGGL((k_clv<D, LAW_REP_ADH, PER, NOISE>), L.grid, L.block, 0, L.s, a);
+      hipLaunchKernelGGL((k_clv<D, LAW_REP_ADH, PER, NOISE>), L.grid, L.block, lds, L.s, a);
       hipLaunchKernelGGL((k_clv_fix<D, LAW_REP_ADH, PER, NOISE>), fg, L.block, 0, L.s, a);
       break;
     case LAW_AGGREGATION:
-      hipLaunchKernelGGL((k_clv<D, LAW_AGGREGATION, PER, NOISE>), L.grid, L.block, 0, L.s, a);
+      hipLaunchKernelGGL((k_clv<D, LAW_AGGREGATION, PER, NOISE>), L.grid, L.block, lds, L.s, a);
       hipLaunchKernelGGL((k_clv_fix<D, LAW_AGGREGATION, PER, NOISE>), fg, L.block, 0, L.s, a);
       break;
     default: return hipErrorInvalidValue;
@@ -1324,6 +1323,10 @@ hipError_t launch_medium_count(int nblocks, int block, hipStream_t s, const Medi
   hipLaunchKernelGGL(k_medium_count, dim3(nblocks), dim3(block), 0, s, a);
   return hipGetLastError();
 }
+hipError_t launch_medium_inject(hipStream_t s, double* c, unsigned long long* add, int64_t nodes) {
+  hipLaunchKernelGGL(k_medium_inject, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, s, c, add, nodes);
+  return hipGetLastError();
+}
 hipError_t launch_medium_step(hipStream_t s, const MediumArgs& a) {
   hipLaunchKernelGGL(k_medium_step, dim3((unsigned)((a.nodes + 255) / 256)), dim3(256), 0, s, a);
   return hipGetLastError();
@@ -1409,19 +1412,6 @@ static hipError_t launch_export_nb(int nb, dim3 grid, dim3 block, hipStream_t s,
     case NB_FULL: hipLaunchKernelGGL((k_export<D, NB_FULL>), grid, block, 0, s, e); break;
     case NB_CELL: hipLaunchKernelGGL((k_export<D, NB_CELL>), grid, block, 0, s, e); break;
     default: hipLaunchKernelGGL((k_export<D, NB_LIST>), grid, block, 0, s, e); break;
-  }
-  return hipGetLastError();
-}
-
-hipError_t launch_clv_order(int dims, int periodic, int nblocks, int block, hipStream_t s, const StageArgs& a) {
-  const size_t lds = (size_t)block * (size_t)a.lcap * 2;
-  switch (dims * 2 + (periodic ? 1 : 0)) {
-    case 2: hipLaunchKernelGGL((k_clv_order<1, 0>), dim3(nblocks), dim3(block), lds, s, a); break;
-    case 3: hipLaunchKernelGGL((k_clv_order<1, 1>), dim3(nblocks), dim3(block), lds, s, a); break;
-    case 4: hipLaunchKernelGGL((k_clv_order<2, 0>), dim3(nblocks), dim3(block), lds, s, a); break;
-    case 5: hipLaunchKernelGGL((k_clv_order<2, 1>), dim3(nblocks), dim3(block), lds, s, a); break;
-    case 6: hipLaunchKernelGGL((k_clv_order<3, 0>), dim3(nblocks), dim3(block), lds, s, a); break;
-    default: hipLaunchKernelGGL((k_clv_order<3, 1>), dim3(nblocks), dim3(block), lds, s, a); break;
   }
   return hipGetLastError();
 }

@@ -21,6 +21,24 @@ constexpr uint32_t RULE_PURPOSE = 0x600u; // Philox purpose word of the k-th dra
 constexpr int32_t RULE_ERR_CAPACITY = 1;  // more agents than n_max
 constexpr int32_t RULE_ERR_ADDS = 2;      // more than RULE_MAX_ADD additions by one agent
 constexpr int32_t RULE_ERR_NBVIEW = 4;    // a neighbor loop without the neighbor view (no loop run)
+constexpr int32_t RULE_ERR_MEDIUM = 8;    // medium access in a model without a medium (reads 0, adds nothing)
+
+
+// The medium node of a position: cellInMesh (src/metrics.jl:18-20) per axis,
+// clamped to the grid (x beyond the box takes the edge node).
+__device__ __forceinline__ int64_t medium_node(const RuleArgs& a, const double* x) {
+  int64_t q[3] = {0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (c >= a.dims) break;
+    int64_t v;
+    if (x[c] > a.med_hi[c]) v = a.med_n[c] - 1;
+    else if (x[c] < a.med_lo[c]) v = 0;
+    else v = floor_div(x[c] - a.med_lo[c], a.med_dx[c]);
+    q[c] = v > a.med_n[c] - 1 ? a.med_n[c] - 1 : v;
+  }
+  return q[0] + (int64_t)a.med_n[0] * (q[1] + (int64_t)a.med_n[1] * q[2]);
+}
 
 // The values of one agent a rule body reads and writes.
 struct RuleRec {
@@ -58,6 +76,7 @@ struct RuleAgent {
   uint32_t n_draw;
   const RuleArgs* args;
   int64_t row;
+  double madd;                 // this agent's medium additions, in call order
 
   // a new agent's values before the body sets any: the parent's old values
   // (@addAgent copies every parameter it is not given, functionRule.jl:46-55)
@@ -80,6 +99,16 @@ struct RuleAgent {
   }
   // @removeAgent
   __device__ void remove() { dead = 1; }
+
+  // the medium at the agent's node, as the step started (a medium variable
+  // in agent code, vectorizeMediumInAgents: auxiliar.jl:353-365)
+  __device__ double medium() const {
+    if (!args->med_c) { atomicOr(args->err, RULE_ERR_MEDIUM); return 0.0; }
+    return args->med_c[medium_node(*args, old.x)];
+  }
+  // `p += v` on the agent's node (the docs' secretion idiom, Usage_Agent.md
+  // :390-395); lands in the field before the step's medium update
+  __device__ void medium_add(double v) { madd = madd + v; }
 
   // CBMDistributions (src/random.jl:1-23) on this agent's k-th draw of the step
   __device__ double rand() {
@@ -159,6 +188,7 @@ __device__ __forceinline__ void rule_eval(const RuleArgs& a, int sim, int64_t r,
   ag.n_add = 0; ag.dead = 0; ag.over = 0; ag.n_draw = 0;
   ag.args = &a;
   ag.row = r;
+  ag.madd = 0.0;
   rule_load(a, r, ag.old);
   ag.nw = ag.old;
   const Rule rule(a.rp, a.g.sims[sim].p);
@@ -197,6 +227,11 @@ __device__ void user_rule_apply(const RuleArgs& a) {
   RuleAgent ag;
   rule_eval<Rule>(a, ri.sim, r, *a.step_ptr, ag);
   const int D = a.dims;
+  if (ag.madd != 0.0) {
+    if (a.med_add)
+      atomicAdd(a.med_add + medium_node(a, ag.old.x), (unsigned long long)llrint(ag.madd * MEDIUM_ADD_SCALE));
+    else atomicOr(a.err, RULE_ERR_MEDIUM);
+  }
   a.dead[r] = (uint8_t)ag.dead;
   // positions and vol are DE state (written in place, read by the stages);
   // growth and vbirth are write-new, committed by update!

@@ -1151,24 +1151,26 @@ __device__ void clv_walk(const StageArgs& a, const SimDev& sd, int64_t i, const 
 }
 
 // CellLinked on a skin list, one agent per lane.  Per step:
-//   clv_order_body: each agent's skin-list entries inside its 3^D neighbor
-//     cells are put in CellLinked order -- cell k among the neighbor cells,
-//     then list position (= slot order) -- and written as the step's list
-//     (a.wlist): lists of at most 16 entries by a sorting network over keys
-//     (k, position, row) in registers, longer ones by insertion of the keys
-//     (k << 6 | position) into the lane's LDS queue (stride blockDim, a.lcap
-//     entries); a list at capacity (possibly truncated) is marked -1.
+//   clv_order_row (inside the step's first stage): each agent's skin-list
+//     entries inside its 3^D neighbor cells are put in CellLinked order --
+//     cell k among the neighbor cells, then list position (= slot order) --
+//     and written as the step's list (a.wlist): lists of at most 16 entries
+//     by a sorting network over keys (k, position, row) in registers, longer
+//     ones by insertion of the keys (k << 6 | position) into the lane's LDS
+//     queue (stride blockDim, a.lcap entries); a list at capacity (possibly
+//     truncated) is marked -1.
 //   clv_stage_body: every stage walks that list.  An agent whose list may
-//     miss a pair -- marked -1, or (stage 2 of a two-stage integrator) whose
-//     distance e from the reference positions plus the largest, E, over all
-//     agents exceeds the skin -- writes nothing and is queued for
+//     miss a pair -- marked -1, or whose distance e from its reference
+//     position plus the largest, E, over all agents exceeds the skin --
+//     writes nothing and is queued for
 //   clv_fix_body, which sums its pairs by clv_walk and finishes its stage.
-// Stage 1 of a two-stage integrator records max e^2 (only agents past half
-// the skin report; E is at least half the skin).
+// Stage 1 of a two-stage integrator records max e^2 for stage 2, the final
+// stage for the next step's stage 1 (only agents past half the skin report;
+// E is at least half the skin).
 
-// One lane's queued agent (a.fix_rows), counted per stage in a.fix_n[stage].
+// One lane's queued agent (a.fix_rows), counted per queue in a.fix_n[fix_q].
 __device__ __forceinline__ void clv_defer(const StageArgs& a, int64_t i) {
-  const int32_t q = atomicAdd(a.fix_n + a.fix_stage, 1);
+  const int32_t q = atomicAdd(a.fix_n + a.fix_q, 1);
   a.fix_rows[q] = (int32_t)i;
 }
 
@@ -1178,7 +1180,7 @@ __device__ __forceinline__ void clv_report(const StageArgs& a, double e2) {
 }
 
 // The ordered list of an agent with n <= W skin-list entries (see
-// clv_order_body); returns its length.  The list holds lcap >= 16 entries
+// clv_order_row); returns its length.  The list holds lcap >= 16 entries
 // per row, so all W loads are in bounds; entries past the length are
 // replaced by the agent itself and sorted last.
 template <int W, int D, int PER>
@@ -1213,19 +1215,26 @@ __device__ __forceinline__ int32_t clv_order_small(const StageArgs& a, const Sim
   return kept;
 }
 
+// Row i's skin-list entries (a.list, a.list_n) in CellLinked order, written
+// to a.wlist; returns their count, or -1 for a list at capacity (possibly
+// truncated).  LDS: blockDim * a.lcap 16-bit keys (lists over 16 entries).
 template <int D, int PER>
-__device__ __forceinline__ void clv_order_body(const StageArgs& a, unsigned char* lds) {
+__device__ __forceinline__ int32_t clv_order_row(const StageArgs& a, const SimDev& sd, int64_t i,
+                                                 unsigned char* lds) {
   const int tid = threadIdx.x, nt = blockDim.x;
-  const RowIdx ri = row_index(a.g);
-  const SimDev& sd = a.g.sims[ri.sim];
-  if (ri.local >= a.g.st[ri.sim].N) return;
-  const int64_t i = ri.row;
   const int32_t* __restrict__ lp = a.list + i;       // entry q at lp[q * ldl]
   const int64_t ldl = a.ldl;
   const int32_t n = a.list_n[i];
-  if (n >= a.lcap) { a.wlist_n[i] = -1; return; }
-  uint16_t* qk = (uint16_t*)lds + tid;                // this lane's queue, entry u at qk[u * nt]
+  if (n >= a.lcap) return -1;
   const int32_t ci = a.ccell[i];
+  // lists of at most 16 entries: every list read and cell gather in flight,
+  // and the keys (k << 27 | position << 23 | row) sorted by a network in
+  // registers; the row rides in the key (the set-up allows this form below
+  // 2^23 rows), and 32-bit byte offsets address from the kernel-argument
+  // bases.  (A form for 8 entries beside it gains nothing: the lanes of a
+  // wave rarely all hold lists that short.)
+  if (n <= 16) return clv_order_small<16, D, PER>(a, sd, i, n, ci);
+  uint16_t* qk = (uint16_t*)lds + tid;                // this lane's queue, entry u at qk[u * nt]
   int32_t kept = 0;
   auto insert = [&](uint16_t key) {
     int32_t pos = kept;
@@ -1238,69 +1247,86 @@ __device__ __forceinline__ void clv_order_body(const StageArgs& a, unsigned char
     qk[pos * nt] = key;
     ++kept;
   };
-  // lists of at most 16 entries: every list read and cell gather in flight,
-  // and the keys (k << 27 | position << 23 | row) sorted by a network in
-  // registers; the row rides in the key (the set-up allows this form below
-  // 2^23 rows), and 32-bit byte offsets address from the kernel-argument
-  // bases.  (A form for 8 entries beside it gains nothing: the lanes of a
-  // wave rarely all hold lists that short.)
-  if (n <= 16) kept = clv_order_small<16, D, PER>(a, sd, i, n, ci);
-  else {
-    for (int32_t q0 = 0; q0 < n; q0 += 4) {
-      int32_t jj[4], cc[4];
+  for (int32_t q0 = 0; q0 < n; q0 += 4) {
+    int32_t jj[4], cc[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) jj[u] = q0 + u < n ? lp[(int64_t)(q0 + u) * ldl] : (int32_t)i;
+    for (int u = 0; u < 4; ++u) jj[u] = q0 + u < n ? lp[(int64_t)(q0 + u) * ldl] : (int32_t)i;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) cc[u] = a.ccell[jj[u]];
+    for (int u = 0; u < 4; ++u) cc[u] = a.ccell[jj[u]];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (q0 + u >= n) continue;
-        const int k = cl_key<D, PER>(sd, ci, cc[u]);
-        if (k < 27) insert((uint16_t)((k << 6) | (q0 + u)));
-      }
-    }
-    for (int32_t v0 = 0; v0 < kept; v0 += 4) {
-      int32_t jj[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) jj[u] = v0 + u < kept ? lp[(int64_t)(qk[(v0 + u) * nt] & 63) * ldl] : 0;
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (v0 + u < kept) a.wlist[(int64_t)(v0 + u) * ldl + i] = jj[u];
+    for (int u = 0; u < 4; ++u) {
+      if (q0 + u >= n) continue;
+      const int k = cl_key<D, PER>(sd, ci, cc[u]);
+      if (k < 27) insert((uint16_t)((k << 6) | (q0 + u)));
     }
   }
-  a.wlist_n[i] = kept;
+  for (int32_t v0 = 0; v0 < kept; v0 += 4) {
+    int32_t jj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) jj[u] = v0 + u < kept ? lp[(int64_t)(qk[(v0 + u) * nt] & 63) * ldl] : 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (v0 + u < kept) a.wlist[(int64_t)(v0 + u) * ldl + i] = jj[u];
+  }
+  return kept;
 }
 
 template <int D, class Law, int PER, int NOISE>
-__device__ __forceinline__ void clv_stage_body(const StageArgs& a) {
+__device__ __forceinline__ void clv_stage_body(const StageArgs& a, unsigned char* lds) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // the step's bookkeeping, where no other kernel is using the words: the
+    // other queue (its fix kernel is done, the next user is a later stage),
+    // and in the first stage the next step's stage-1 maximum
+    a.fix_n[1 - a.fix_q] = 0;
+    if (a.clv_order) {
+      *a.emax_next = 0ull;
+      if (a.book) {
+        // the record the host reads (system-scope stores into mapped host
+        // memory, the step last)
+        __hip_atomic_store(&a.rec->soon, a.efin[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&a.rec->e2max, a.efin[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&a.rec->step, a.step_no, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+    }
+  }
   const RowIdx ri = row_index(a.g);
   const SimDev& sd = a.g.sims[ri.sim];
   if (ri.local >= a.g.st[ri.sim].N) return;
   const int64_t i = ri.row;
-  const int32_t n = a.list_n[i];
   double xi[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int c = 0; c < D; ++c) xi[c] = a.S[c][i];
-  if (n < 0) { clv_defer(a, i); return; }
-  if (a.fix_stage == 0 && a.efin) {
-    // stage 1: every agent is within E0 of its reference position, E0 the
-    // largest reported displacement (agents past half the skin report) or a
-    // little over half the skin; with no report all pairs are in the lists
-    const unsigned long long e0 = *a.efin;
-    if (e0 != 0ull) {
-      const double E0 = fmax(sqrt(__longlong_as_double((long long)e0)), a.clv_e1);
-      double s2 = 0.0;
+  int32_t n;
+  if (a.clv_order) {
+    // the step's first stage: is the agent within reach of every pair its
+    // list must hold?  Then its entries in CellLinked order, written for
+    // the later stages and walked below
+    if (a.efin) {
+      // every agent is within E0 of its reference position, E0 the largest
+      // reported displacement (agents past half the skin report) or a
+      // little over half the skin; with no report all pairs are in the lists
+      const unsigned long long e0 = *a.efin;
+      if (e0 != 0ull) {
+        const double E0 = fmax(sqrt(__longlong_as_double((long long)e0)), a.clv_e1);
+        double s2 = 0.0;
 #pragma unroll
-      for (int c = 0; c < D; ++c) {
-        double d = xi[c] - a.xref[c][i];
-        if (PER) d = min_image(d, sd.L[c]);
-        s2 = s2 + d * d;
+        for (int c = 0; c < D; ++c) {
+          double d = xi[c] - a.xref[c][i];
+          if (PER) d = min_image(d, sd.L[c]);
+          s2 = s2 + d * d;
+        }
+        if (!(sqrt(s2) + E0 <= a.clv_skin)) { a.wlist_n[i] = -1; clv_defer(a, i); return; }
       }
-      if (!(sqrt(s2) + E0 <= a.clv_skin)) { clv_defer(a, i); return; }
     }
+    n = clv_order_row<D, PER>(a, sd, i, lds);
+    a.wlist_n[i] = n;
+  } else {
+    n = a.wlist_n[i];
   }
-  if (a.fix_stage == 1) {
-    // stage 2: could a pair active here lie outside the list?
+  if (n < 0) { clv_defer(a, i); return; }
+  if (a.fix_stage == 1 && *a.emax != 0ull) {
+    // stage 2: could a pair active here lie outside the list?  (With no
+    // stage-1 report every agent is within half the skin: none can.)
     double dx[3];
 #pragma unroll
     for (int c = 0; c < D; ++c) {
@@ -1319,7 +1345,7 @@ __device__ __forceinline__ void clv_stage_body(const StageArgs& a) {
   const Law law(sd.p);
   double L[3], hL[3];
   box_halves<D, PER>(sd, L, hL);
-  const int32_t* __restrict__ lp = a.list + i;
+  const int32_t* __restrict__ lp = a.wlist + i;       // entry q at lp[q * ldl]
   const int64_t ldl = a.ldl;
   int32_t jn[2];
 #pragma unroll
@@ -1351,7 +1377,7 @@ __device__ __forceinline__ void clv_stage_body(const StageArgs& a) {
 
 template <int D, class Law, int PER, int NOISE>
 __device__ void clv_fix_body(const StageArgs& a) {
-  const int32_t cnt = a.fix_n[a.fix_stage];
+  const int32_t cnt = a.fix_n[a.fix_q];
   for (int32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x) {
     const int64_t i = a.fix_rows[q];
     int sim = 0;

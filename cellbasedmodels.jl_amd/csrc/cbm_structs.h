@@ -223,13 +223,17 @@ struct StageArgs {
   // step displacements
   unsigned long long* efin;
   double clv_ahead;
-  // per-step bookkeeping when the host schedules the list builds: the order
-  // kernel clears emax and fix_n and writes rec (step_no, efin) for the
-  // host; the last fix kernel advances *tick (the device step)
+  // clv_order: this stage (the step's first) orders the skin list (a.list,
+  // a.list_n) into a.wlist/a.wlist_n itself, and clears *emax_next (the
+  // next step's stage-1 maximum).  Every stage queues its deferred agents
+  // in fix_n[fix_q] and clears fix_n[1 - fix_q].  book: the first stage
+  // also writes rec (step_no, efin) for the host, and the final stage's
+  // fix kernel advances *tick (the device step).
+  unsigned long long* emax_next;
   ClvRecord* rec;
   int64_t* tick;
   int64_t step_no;
-  int32_t book, pad3;
+  int32_t book, clv_order, fix_q, pad3;
 };
 
 // Flattened list stage (flat_body): per block, the agents' list entries are
@@ -255,6 +259,12 @@ __host__ __device__ inline int64_t flat_lds_bytes(int D, int chunk, int cap) {
 // Agent rules (agentRule with @addAgent/@removeAgent) and the canonical
 // compaction of update!; also the argument of the hipRTC user rule kernels.
 // Arrays marked (slot) are indexed by logical slot.
+// Medium additions of rule bodies are summed per node in 32.32 fixed point:
+// exact and the same in any order (the reference's unsynchronized `p += v`
+// of several agents in one node has no defined sum).  |sum of one agent's
+// additions in a step| < 2^31.
+constexpr double MEDIUM_ADD_SCALE = 4294967296.0;
+
 struct RuleArgs {
   RowGeom g;
   int32_t rule, dims, lagged, growth;
@@ -299,6 +309,14 @@ struct RuleArgs {
   const double* nb_x[3];
   const double* nb_vol;
   const double* nb_u[4];
+  // the medium at the agent's node (cellInMesh of its step-start position,
+  // src/AgentStructure/auxiliar.jl:353-365): read from med_c (the field at
+  // the step start, NULL without a medium); additions are summed per node
+  // in med_add as 32.32 fixed point by the apply pass
+  const double* med_c;
+  unsigned long long* med_add;  // two's complement
+  int32_t med_n[3], med_pad;
+  double med_lo[3], med_hi[3], med_dx[3];
 };
 
 // Medium: one field on a cell-centred grid over the box of simulation 0,
@@ -319,6 +337,8 @@ struct MediumArgs {
   double lo[3], hi[3];
   double D, uptake, dt;
   int64_t nodes;
+  int32_t bmode[6];             // border per axis side x-, x+, y-, y+, z-, z+ (cbm_medium_border)
+  double bval[6];               // Dirichlet values
 };
 
 }  // namespace cbm

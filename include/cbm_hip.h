@@ -102,6 +102,18 @@ enum cbm_medium_alg {
   CBM_MEDIUM_DGADI = 1
 };
 
+/* Border of the medium on one side of one axis (cbm_model_desc.medium_border).
+ * MIRROR: the border nodes integrate like the others, with a ghost node
+ *   mirroring them (zero flux).
+ * DIRICHLET: the border nodes take medium_border_value each step (`p = v`).
+ * COPY: the border nodes take the value of their inward neighbor along that
+ *   axis at the step start (`p = p[2, i2_]`, the reference's reflective
+ *   idiom).
+ * A node on borders of several sides takes the first of them in the order
+ * x-, x+, y-, y+, z-, z+ (the order of an if/elseif chain over the sides);
+ * inside nodes read the border nodes' step-start values. */
+enum cbm_medium_border { CBM_BORDER_MIRROR = 0, CBM_BORDER_DIRICHLET = 1, CBM_BORDER_COPY = 2 };
+
 /* ------------------------------------------------------------ agent rules */
 /* Built-in agentRule bodies (src/AgentStructure/functionRule.jl:9-111).
  *   CBM_RULE_GROWTH_DIVISION rule_p = {g_mean, g_spread, split, ratio}:
@@ -188,6 +200,14 @@ typedef struct cbm_model_desc {
                             round's terms sit in LDS: 2048 keeps a 3-D block
                             under 64 KiB).  The results are the same bits for
                             every value.                                    */
+  /* medium borders, per axis and side in the order x-, x+, y-, y+, z-, z+
+   * (enum cbm_medium_border): the nodes of a side with a Dirichlet or copy
+   * border are assigned each step instead of integrated, as a mediumODE
+   * body's `@mediumBorder(axis, side)` branch assigns them
+   * (src/AgentStructure/macros.jl:284-311; docs/src/Usage_Agent.md:397-403)
+   * while `@mediumInside()` nodes integrate dt(c).  FTCS only.           */
+  int32_t medium_border[6];
+  double medium_border_value[6];  /* the Dirichlet values                    */
 } cbm_model_desc;
 
 /* Per-simulation description.  One context may hold many independent
@@ -236,6 +256,15 @@ const char* cbm_version(void);
  * hipRTC log in cbm_last_error. */
 int cbm_compile(cbm_ctx* ctx, const char* pair_body_hip);
 
+/* Compile user bodies as cbm_compile_model would for a model described by
+ * `model` (its dims, neighbors, law, rule, periodic and user_de), without a
+ * context or a GPU: the check of compileABM! (src/AgentStructure/
+ * functionRule.jl:121-160, functionDE.jl:13-116) before a run.  Returns
+ * CBM_OK, or CBM_ERR_COMPILE with the compiler log in `log` (log_cap bytes
+ * with the terminating NUL; log may be NULL). */
+int cbm_compile_check(const cbm_model_desc* model, const char* law_body, const char* noise_body,
+                      const char* rule_body, const char* medium_body, char* log, int64_t log_cap);
+
 /* compileABM! for the agentODE and agentSDE bodies together
  * (src/AgentStructure/functionDE.jl:13-116 builds both f and g).  law_body
  * as for cbm_compile (required with CBM_LAW_CUSTOM, NULL otherwise: the
@@ -273,6 +302,12 @@ int cbm_compile(cbm_ctx* ctx, const char* pair_body_hip);
  * order, with n.x, n.dx (minimum image), n.d, n.id, n.vol() and n.u(k) as
  * they were before the rule ran.  New agents get ids idMax + 1 + (prefix of
  * additions in slot order), the reference's serial append order.
+ * a.medium() is the medium at the agent's node (cellInMesh of its position)
+ * as the step started, and a.medium_add(v) is `p += v` there (a medium
+ * variable in agent code, auxiliar.jl:353-365; the secretion idiom of
+ * docs/src/Usage_Agent.md:390-395): the additions of all agents are summed
+ * per node in 32.32 fixed point (exact, so their order does not matter)
+ * and land in the field before the step's medium update.
  * medium_body (mediumODE, functionDE.jl:13-116 with the medium loop of
  * auxiliar.jl:168-172,184-210 and @∂2 of macros.jl:203-257; models with a
  * medium, NULL = the built-in diffusion with uptake) is HIP C++ defining,
@@ -283,8 +318,13 @@ int cbm_compile(cbm_ctx* ctx, const char* pair_body_hip);
  *   };
  * MediumNode (cbm_medium.h) holds c, lap[3] (@∂2 per axis with zero-flux
  * mirror ghosts), laplacian(), the node centre x[3] and index idx[3], the
- * number of agents in the node (cellInMesh, metrics.jl:18-20) and dt; the
- * step is c' = c + dt * dt(c) for every node.
+ * number of agents in the node (cellInMesh, metrics.jl:18-20), dt, the
+ * neighbors' values nb[axis][0 below / 1 above], border[axis] (-1 / +1 on a
+ * lower / upper border, @mediumBorder) and inside() (@mediumInside); the
+ * step is c' = c + dt * dt(c) for every node, except nodes of a side with a
+ * Dirichlet or copy border (cbm_model_desc.medium_border) and nodes for
+ * which an optional `__device__ bool assign(const MediumNode&, double& v)
+ * const` returns true: those are set to v (the `p = ...` branches).
  * Each call replaces the previous module, so pass every body of the model in
  * one call. */
 int cbm_compile_model(cbm_ctx* ctx, const char* law_body, const char* noise_body,
@@ -418,9 +458,9 @@ int cbm_set_block(cbm_ctx* ctx, int32_t block);
  * avg_ms[k] / launches[k] for k = 0..7: the integrator stage kernels
  * (0 Euler, 1 EM, 2 Heun stage 1, 3 Heun stage 2, 4-7 RK4 stages 1-4);
  * k = 8: the neighbor update of one step (total, launches = steps);
- * k = 9: whole step; k = 10, 11: EulerHeun stages 1 and 2; k = 12: the
- * step's cell-ordered lists of CellLinked on a skin list.  ncat must be
- * >= 13. */
+ * k = 9: whole step; k = 10, 11: EulerHeun stages 1 and 2; k = 12 is
+ * reserved (no launches: CellLinked on a skin list orders its lists inside
+ * the first stage).  ncat must be >= 13. */
 int cbm_profile(cbm_ctx* ctx, int64_t nsteps, double* avg_ms, int64_t* launches, int32_t ncat);
 /* Average device time (ms) of the last cbm_step call's force kernels, from
  * HIP events on the context stream; *launches = force-kernel launches timed. */

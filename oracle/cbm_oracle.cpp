@@ -18,9 +18,11 @@
 //   update! compaction + commit       src/CommunityStructure/update.jl:15-303
 //   step! order                       src/CommunityStructure/step.jl:96-107
 //   @∂2 medium stencil                src/AgentStructure/macros.jl:203-257
+//   @mediumInside / @mediumBorder     src/AgentStructure/macros.jl:259-311
 // with the spec decisions the GPU path shares (exact floor quotients,
 // periodic minimum image, canonical ascending compaction order, Philox noise,
-// zero-flux ghost-mirror medium, d>0 guard on pair laws).
+// zero-flux ghost-mirror medium, border nodes assigned from the step-start
+// field, d>0 guard on pair laws).
 //
 // Compiled with -ffp-contract=off: every expression below is evaluated in
 // the written order with IEEE double rounding.
@@ -151,6 +153,7 @@ struct Orc {
   void* rule_fn = nullptr;                    // CBM_RULE_CUSTOM body (orc_rule_fn)
   std::vector<double> ruleX[3], ruleVol, ruleU[4];   // the rule's neighbor view: values at the step's start
   void* medium_fn = nullptr;                  // user mediumODE body (orc_medium_fn), else built-in
+  void* medium_assign = nullptr;              // its assignment branches (orc_medium_assign_fn), optional
   // neighbor structures
   Grid g;
   std::vector<int64_t> cellOf, cellStart, cellCount, cellAgents;
@@ -165,6 +168,7 @@ struct Orc {
   double mdx[3] = {1, 1, 1};
   std::vector<double> c, cnew;
   std::vector<double> cinit;                  // DGADI: the field at creation / last orc_set_medium
+  std::vector<int64_t> medAdd;                // the rules' additions per node (32.32 fixed point)
   std::vector<int32_t> nodeCount;
   int err = 0;
   std::string msg;
@@ -470,8 +474,35 @@ struct OrcMediumNode {
   int64_t idx[3];
   int32_t agents, dims;
   double dt;
+  double nb[3][2];       // the neighbors' step-start values per axis, below / above (the node past the edge)
+  int32_t border[3];     // -1 / +1 on the lower / upper border of that axis, else 0
+  int32_t pad;
 };
 typedef double (*orc_medium_fn)(const OrcMediumNode* node);
+// Optionally claims a node for assignment (the `p = ...` branches of a
+// mediumODE body): returns nonzero and sets *v.
+typedef int (*orc_medium_assign_fn)(const OrcMediumNode* node, double* v);
+
+// The value a node on a side with a Dirichlet or copy border is assigned:
+// the first such side in the order x-, x+, y-, y+, z-, z+ that the node lies
+// on (`@mediumBorder(axis, side)` branches of an if/elseif chain,
+// macros.jl:284-311); a copy takes the inward neighbor's step-start value
+// (the reference's reflective idiom `p = p[2, i2_]`, docs/src/Usage_Agent.md
+// :397-403).  Returns false for a node on no such side.
+static bool border_value(const Orc& o, int64_t k, const int64_t* p, const int64_t* stride, double* v) {
+  for (int a = 0; a < o.dims; ++a)
+    for (int side = 0; side < 2; ++side) {
+      const bool on = side == 0 ? p[a] == 0 : p[a] == o.mn[a] - 1;
+      const int32_t mode = o.m.medium_border[2 * a + side];
+      if (!on || mode == CBM_BORDER_MIRROR) continue;
+      if (mode == CBM_BORDER_DIRICHLET) { *v = o.m.medium_border_value[2 * a + side]; return true; }
+      // copy: the inward neighbor (the node itself on a one-node axis)
+      const int64_t q = side == 0 ? p[a] + 1 : p[a] - 1;
+      *v = (q >= 0 && q < o.mn[a]) ? o.c[k + (q - p[a]) * stride[a]] : o.c[k];
+      return true;
+    }
+  return false;
+}
 
 // Douglas-Gunn ADI (src/integrators.jl:864-935): after the reaction step,
 // one implicit sweep per axis with the reference's tridiagonal solver
@@ -517,9 +548,30 @@ static void adi_sweeps(Orc& o) {
   }
 }
 
+// The medium node of a position: cellInMesh (src/metrics.jl:18-20, 0-based;
+// x == xMax clamped), per axis.
+static int64_t medium_node(const Orc& o, const double* x) {
+  int64_t q[3] = {0, 0, 0};
+  for (int a = 0; a < o.dims; ++a) {
+    int64_t v;
+    if (x[a] > o.s.box_hi[a]) v = o.mn[a] - 1;
+    else if (x[a] < o.s.box_lo[a]) v = 0;
+    else v = floor_div(x[a] - o.s.box_lo[a], o.mdx[a]);
+    if (v > o.mn[a] - 1) v = o.mn[a] - 1;
+    q[a] = v;
+  }
+  return q[0] + o.mn[0] * (q[1] + o.mn[1] * q[2]);
+}
+
 static void medium_step(Orc& o, int64_t nAgents) {
   const bool adi = o.m.medium_alg == CBM_MEDIUM_DGADI;   // reaction part here, diffusion in the sweeps
   int64_t nn = o.mn[0] * o.mn[1] * o.mn[2];
+  // the rules' additions of this step land first
+  for (int64_t k = 0; k < nn && !o.medAdd.empty(); ++k)
+    if (o.medAdd[k] != 0) {
+      o.c[k] = o.c[k] + (double)o.medAdd[k] * (1.0 / 4294967296.0);
+      o.medAdd[k] = 0;
+    }
   o.nodeCount.assign(nn, 0);
   for (int64_t i = 0; i < nAgents; ++i) {
     int64_t q[3] = {0, 0, 0};
@@ -539,8 +591,10 @@ static void medium_step(Orc& o, int64_t nAgents) {
   const double D = o.m.medium_D, dt = o.m.dt, u = o.m.medium_consume;
   if (o.medium_fn) {   // a user body (test code's restatement), serially
     orc_medium_fn fn = (orc_medium_fn)o.medium_fn;
+    orc_medium_assign_fn asg = (orc_medium_assign_fn)o.medium_assign;
     for (int64_t k = 0; k < nn; ++k) {
       OrcMediumNode nd;
+      std::memset(&nd, 0, sizeof(nd));
       int64_t p[3] = {k % o.mn[0], (k / o.mn[0]) % o.mn[1], k / (o.mn[0] * o.mn[1])};
       int64_t stride[3] = {1, o.mn[0], o.mn[0] * o.mn[1]};
       nd.c = o.c[k]; nd.agents = o.nodeCount[k]; nd.dims = o.dims; nd.dt = dt;
@@ -549,10 +603,16 @@ static void medium_step(Orc& o, int64_t nAgents) {
         if (a >= o.dims) continue;
         double cp = (p[a] + 1 < o.mn[a]) ? o.c[k + stride[a]] : nd.c;
         double cm = (p[a] > 0) ? o.c[k - stride[a]] : nd.c;
+        nd.nb[a][0] = cm;
+        nd.nb[a][1] = cp;
+        nd.border[a] = p[a] == 0 ? -1 : (p[a] == o.mn[a] - 1 ? 1 : 0);
         if (!adi) nd.lap[a] = ((cp - 2 * nd.c) + cm) / (o.mdx[a] * o.mdx[a]);
         nd.x[a] = o.s.box_lo[a] + ((double)p[a] + 0.5) * o.mdx[a];
       }
-      o.cnew[k] = nd.c + dt * fn(&nd);
+      double v;
+      if (border_value(o, k, p, stride, &v)) o.cnew[k] = v;
+      else if (asg && asg(&nd, &v)) o.cnew[k] = v;
+      else o.cnew[k] = nd.c + dt * fn(&nd);
     }
     o.c.swap(o.cnew);
     if (adi) adi_sweeps(o);
@@ -572,7 +632,8 @@ static void medium_step(Orc& o, int64_t nAgents) {
       lap = (a == 0) ? l : lap + l;
     }
     double f = D * lap - (u * cc) * (double)o.nodeCount[k];
-    o.cnew[k] = cc + dt * f;
+    double v;
+    o.cnew[k] = border_value(o, k, p, stride, &v) ? v : cc + dt * f;
   }
   o.c.swap(o.cnew);
   if (adi) adi_sweeps(o);
@@ -584,7 +645,11 @@ struct Births { std::vector<int64_t> parent; };
 // A CBM_RULE_CUSTOM body on the CPU: one agent's values at the start of the
 // step (old), its new values (nw, initialised to old) and up to two added
 // agents (add, each initialised to old by the caller); sets *n_add, *dead.
-struct OrcRuleRec { double x[3]; double vol, growth, vbirth; double u[4]; int64_t lf; };
+struct OrcRuleRec {
+  double x[3]; double vol, growth, vbirth; double u[4]; int64_t lf;
+  double medium;   // old: the medium at the agent's node as the step started
+  double madd;     // nw: the body's additions to that node, in call order
+};
 typedef void (*orc_rule_fn)(int64_t id, int64_t step, const OrcRuleRec* old, OrcRuleRec* nw, OrcRuleRec* add,
                             int32_t* n_add, int32_t* dead, int64_t slot);
 
@@ -602,16 +667,26 @@ static void apply_custom_rule(Orc& o, std::vector<int64_t>& holes, int64_t& nAdd
   for (int a = 0; a < 3; ++a) o.ruleX[a] = o.X[a];
   o.ruleVol = o.vol;
   for (int k = 0; k < 4; ++k) o.ruleU[k] = o.U[k];
+  const bool med = o.m.medium_n[0] > 0;
   for (int64_t i = 0; i < N; ++i) {
     OrcRuleRec old;
     for (int a = 0; a < 3; ++a) old.x[a] = a < o.dims ? o.X[a][i] : 0.0;
     old.vol = o.vol[i]; old.growth = o.gr[i]; old.vbirth = o.vb[i];
     for (int k = 0; k < 4; ++k) old.u[k] = o.U[k][i];
     old.lf = o.lf[i];
+    const int64_t node = med ? medium_node(o, old.x) : 0;
+    old.medium = med ? o.c[node] : 0.0;
+    old.madd = 0.0;
     OrcRuleRec nw = old, add[2] = {old, old};
     int32_t na = 0, dead = 0;
     fn(o.id[i], o.step, &old, &nw, add, &na, &dead, i);
     if (na > 2) { o.err = CBM_ERR_CONFIG; o.msg = "more than 2 additions"; na = 2; }
+    // `p += v` of the body on its node: summed per node in 32.32 fixed
+    // point (exact in any order), added to the field before its update
+    if (nw.madd != 0.0) {
+      if (!med) { o.err = CBM_ERR_CONFIG; o.msg = "medium access in a model without a medium"; }
+      else o.medAdd[node] += std::llrint(nw.madd * 4294967296.0);
+    }
     for (int a = 0; a < o.dims; ++a) o.X[a][i] = nw.x[a];
     o.vol[i] = nw.vol; o.grNew[i] = nw.growth; o.vbNew[i] = nw.vbirth;
     for (int k = 0; k < 4; ++k) o.U[k][i] = nw.u[k];
@@ -938,6 +1013,7 @@ void* orc_create(const cbm_model_desc* m, const cbm_sim_desc* s) {
       o->mdx[a] = (s->box_hi[a] - s->box_lo[a]) / (double)m->medium_n[a];   // communityStructure.jl:274
     }
     o->c.assign(o->mn[0] * o->mn[1] * o->mn[2], m->medium_c0);
+    o->medAdd.assign(o->c.size(), 0);
     o->cinit = o->c;
     if (m->medium_alg == CBM_MEDIUM_DGADI)
       for (int a = 0; a < o->dims; ++a)
@@ -1092,6 +1168,8 @@ double orc_rule_value(void* h, int field, int64_t j) {
 
 // A user mediumODE body in place of the built-in diffusion with uptake.
 void orc_set_medium_body(void* h, void* fn) { ((Orc*)h)->medium_fn = fn; }
+// Its assignment branches (orc_medium_assign_fn; NULL: none).
+void orc_set_medium_assign(void* h, void* fn) { ((Orc*)h)->medium_assign = fn; }
 
 void orc_normals(uint64_t seed, int64_t step, int64_t id, int32_t sim, uint32_t purpose, double* z2) {
   U4 r = draw(seed, step, id, sim, purpose);

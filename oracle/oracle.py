@@ -54,6 +54,7 @@ def lib():
             "orc_rule_neighbors": (C.c_int64, [P, C.c_int64, P, C.c_int64]),
             "orc_rule_value": (C.c_double, [P, C.c_int, C.c_int64]),
             "orc_set_medium_body": (None, [P, P]),
+            "orc_set_medium_assign": (None, [P, P]),
             "orc_traj_loss": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P]),
             "orc_traj_grad_dual": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P, P]),
         }
@@ -71,7 +72,7 @@ INT_FIELDS = (3, 10, 16)   # id, lf, cellFate (int64)
 class RuleRec(C.Structure):
     """One agent's values as a CBM_RULE_CUSTOM body sees them (OrcRuleRec)."""
     _fields_ = [("x", C.c_double * 3), ("vol", C.c_double), ("growth", C.c_double), ("vbirth", C.c_double),
-                ("u", C.c_double * 4), ("lf", C.c_int64)]
+                ("u", C.c_double * 4), ("lf", C.c_int64), ("medium", C.c_double), ("madd", C.c_double)]
 
 
 RULE_FN = C.CFUNCTYPE(None, C.c_int64, C.c_int64, C.POINTER(RuleRec), C.POINTER(RuleRec), C.POINTER(RuleRec),
@@ -109,7 +110,12 @@ RULE_PURPOSE = 0x600
 class MediumNode(C.Structure):
     """What a mediumODE body sees at one node (OrcMediumNode)."""
     _fields_ = [("c", C.c_double), ("lap", C.c_double * 3), ("x", C.c_double * 3), ("idx", C.c_int64 * 3),
-                ("agents", C.c_int32), ("dims", C.c_int32), ("dt", C.c_double)]
+                ("agents", C.c_int32), ("dims", C.c_int32), ("dt", C.c_double),
+                ("nb", (C.c_double * 2) * 3), ("border", C.c_int32 * 3), ("pad", C.c_int32)]
+
+    def inside(self):
+        """@mediumInside(): on no border of any axis."""
+        return all(self.border[a] == 0 for a in range(self.dims))
 
     def laplacian(self):
         lap = self.lap[0]
@@ -121,6 +127,7 @@ class MediumNode(C.Structure):
 
 
 MEDIUM_FN = C.CFUNCTYPE(C.c_double, C.POINTER(MediumNode))
+MEDIUM_ASSIGN_FN = C.CFUNCTYPE(C.c_int, C.POINTER(MediumNode), C.POINTER(C.c_double))
 
 
 def u53(a, b):
@@ -148,6 +155,14 @@ class RuleAgent:
 
     def remove(self):
         self.dead = 1
+
+    def medium(self):
+        """The medium at the agent's node as the step started."""
+        return self.old.medium
+
+    def medium_add(self, v):
+        """`p += v` on the agent's node (summed per node in 32.32 fixed point)."""
+        self.nw.madd = self.nw.madd + v
 
     def _words(self):
         ctr = np.array([self.step & 0xFFFFFFFF, self.id & 0xFFFFFFFF, 0, RULE_PURPOSE + self._k], dtype=np.uint32)
@@ -227,10 +242,21 @@ class Oracle:
         self._rule_cb = RULE_FN(cb)
         self.lib.orc_set_rule(self.h, C.cast(self._rule_cb, C.c_void_p))
 
-    def set_medium_body(self, body):
-        """A user mediumODE body: body(node) returns dt(c) at a MediumNode."""
+    def set_medium_body(self, body, assign=None):
+        """A user mediumODE body: body(node) returns dt(c) at a MediumNode;
+        assign(node), if given, returns the value a node is assigned (its
+        `p = ...` branches) or None."""
         self._medium_cb = MEDIUM_FN(lambda nd: float(body(nd.contents)))
         self.lib.orc_set_medium_body(self.h, C.cast(self._medium_cb, C.c_void_p))
+        if assign is not None:
+            def cb(nd, v):
+                r = assign(nd.contents)
+                if r is None:
+                    return 0
+                v[0] = float(r)
+                return 1
+            self._medium_assign_cb = MEDIUM_ASSIGN_FN(cb)
+            self.lib.orc_set_medium_assign(self.h, C.cast(self._medium_assign_cb, C.c_void_p))
 
     def _rule_neighbors(self, ag, slot):
         out = np.zeros(4096, dtype=np.int64)

@@ -43,3 +43,36 @@ struct UserMedium {
 
 def geometry(m):
     return m.x[0] + 1000.0 * float(m.idx[1])
+
+
+# the reference's mediumODE idiom (docs/src/Usage_Agent.md:390-403): inside
+# nodes diffuse, the x- border reflects (p = p[2, i2_]), the x+ border
+# absorbs (p = 0); other nodes (the y borders) diffuse through the mirror
+IDIOM = """
+struct UserMedium {
+  double D;
+  __device__ UserMedium(double D_, double, const double*) : D(D_) {}
+  __device__ double operator()(const MediumNode& m) const { return D * m.laplacian(); }
+  __device__ bool assign(const MediumNode& m, double& v) const {
+    if (m.inside()) return false;
+    if (m.border[0] == -1) { v = m.nb[0][1]; return true; }
+    if (m.border[0] == 1) { v = 0.0; return true; }
+    return false;
+  }
+};
+"""
+
+
+def idiom(D):
+    def body(m):
+        return D * m.laplacian()
+
+    def assign(m):
+        if m.inside():
+            return None
+        if m.border[0] == -1:
+            return m.nb[0][1]
+        if m.border[0] == 1:
+            return 0.0
+        return None
+    return body, assign

@@ -186,3 +186,26 @@ def inhibit(rp):
             d.lf = a.id
             a.add(d)
     return body
+
+
+# the docs' secretion idiom (Usage_Agent.md:390-395: `p += secrete*dt/dx*dy`
+# in agentRule, p a medium variable at the agent's node): each agent adds
+# rp[0] * dt to its node and keeps in u[0] what the node held as the step
+# started (a medium read in agent code, auxiliar.jl:353-365)
+SECRETE = """
+struct UserRule {
+  double s;
+  __device__ UserRule(const double* rp, const double*) : s(rp[0]) {}
+  __device__ void operator()(RuleAgent& a) const {
+    a.nw.u[0] = a.medium();
+    a.medium_add(s * a.dt);
+  }
+};
+"""
+
+
+def secrete(s):
+    def body(a):
+        a.nw.u[0] = a.medium()
+        a.medium_add(s * a.dt)
+    return body

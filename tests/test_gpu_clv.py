@@ -172,3 +172,22 @@ def test_late_builds_stay_exact(gpu_available):
         for a in range(2):
             assert np.array_equal(g.download(a, n), o.get(a)), (k, a)
     assert g.rebuilds() >= 4
+
+
+def test_heun_steps_ten_times_longer(gpu_available):
+    """With a step ten times C2's, agents move past half the skin within a
+    step or two: stage 2 sends the agents whose list may miss a pair to the
+    exact walk, builds come every few steps, and the result is still the
+    oracle's cell walk, bit for bit."""
+    n = 4000
+    L = (n * (4.0 / 3.0) * np.pi / 0.3) ** (1.0 / 3.0)
+    m = model(3, integrator=A.HEUN, neighbors=A.CELL_LINKED, law=A.LAW_REP_ADH, dt=0.05, cell_edge=2.4)
+    s = sim(n, [[0.0, L]] * 3, REP_ADH)
+    pos = uniform_positions(n, [[0.0, L]] * 3, 41)
+    g = device(m, [s], [pos], 0)
+    o = oracle(m, s, pos)
+    for k in (1, 4, 10):
+        g.step(k)
+        o.step(k)
+        for a in range(3):
+            assert np.array_equal(g.download(a, n), o.get(a)), (k, a)

@@ -134,3 +134,47 @@ def test_custom_rule_neighbor_loop_lattice_kat(dims):
         assert o.init() == 0
         o.step(1)
         assert o.get(A.F_U0).tolist() == [float(v) for v in g["expected"][str(dims)]], alg["name"]
+
+
+def medium_rule_model(n, D, dt, c0, rp0):
+    m = model(2, neighbors=A.FULL, dt=dt, rule=A.RULE_CUSTOM)
+    m.rule_p[0] = rp0
+    m.medium_n[0], m.medium_n[1] = n
+    m.medium_D, m.medium_c0 = D, c0
+    return m
+
+
+def test_medium_additions_and_reads_by_hand():
+    """Agents add s dt = 0.25 to their node and read it back: with no
+    diffusion a node gains 0.25 per agent and step (exact in 32.32 fixed
+    point), and u0 holds the node's value at the step start."""
+    import rule_bodies as RB
+    m = medium_rule_model((4, 4), 0.0, 0.5, 1.0, 0.5)        # s dt = 0.25
+    o = O.Oracle(m, sim(3, [[0.0, 4.0], [0.0, 4.0]], n_max=8))
+    o.set(A.F_X, [1.5, 1.2, 3.9])    # two agents in node (1, 2), one in (3, 0)
+    o.set(A.F_Y, [2.5, 2.1, 0.2])
+    o.set_rule(RB.secrete(0.5))
+    assert o.init() == 0
+    o.step(1)
+    c = o.medium(16).reshape(4, 4)
+    assert c[2, 1] == 1.5 and c[0, 3] == 1.25 and c.sum() == 16.0 + 0.75
+    assert np.array_equal(o.get(A.F_U0), [1.0, 1.0, 1.0])
+    o.step(2)
+    c = o.medium(16).reshape(4, 4)
+    assert c[2, 1] == 2.5 and c[0, 3] == 1.75
+    assert np.array_equal(o.get(A.F_U0), [2.0, 2.0, 1.5])     # read at the third step's start
+
+
+def test_medium_additions_are_fixed_point_sums():
+    """0.1 is not a multiple of 2^-32: each agent's addition is rounded to
+    32.32 fixed point once and the node's sum is exact in any order."""
+    import rule_bodies as RB
+    m = medium_rule_model((2, 2), 0.0, 1.0, 0.0, 0.1)
+    o = O.Oracle(m, sim(5, [[0.0, 2.0], [0.0, 2.0]], n_max=8))
+    o.set(A.F_X, [0.5] * 5)
+    o.set(A.F_Y, [0.5] * 5)
+    o.set_rule(RB.secrete(0.1))
+    assert o.init() == 0
+    o.step(1)
+    q = round(0.1 * 2.0 ** 32)
+    assert o.medium(4)[0] == (5 * q) / 2.0 ** 32
