@@ -18,7 +18,7 @@ ERRORS = {
 
 EULER, HEUN, RK4, EM, EULER_HEUN = 0, 1, 2, 3, 4
 FULL, CELL_LINKED, VERLET_DISPLACEMENT, VERLET_TIME, CLVD = 0, 1, 2, 3, 4
-LAW_NONE, LAW_LINEAR, LAW_REP_ADH, LAW_AGGREGATION, LAW_CUSTOM, LAW_SOFT_SPHERES = 0, 1, 2, 3, 4, 5
+LAW_NONE, LAW_LINEAR, LAW_REP_ADH, LAW_AGGREGATION, LAW_CUSTOM, LAW_SOFT_SPHERES, LAW_RODS = 0, 1, 2, 3, 4, 5, 6
 RULE_NONE, RULE_GROWTH_DIVISION, RULE_PARITY, RULE_DEATH, RULE_CUSTOM = 0, 1, 2, 3, 4
 MEDIUM_FTCS, MEDIUM_DGADI = 0, 1
 BORDER_MIRROR, BORDER_DIRICHLET, BORDER_COPY = 0, 1, 2
@@ -26,6 +26,7 @@ BORDER_MIRROR, BORDER_DIRICHLET, BORDER_COPY = 0, 1, 2
 F_X, F_Y, F_Z, F_ID, F_FX, F_FY, F_FZ, F_VOL, F_GROWTH, F_VBIRTH, F_LF = range(11)
 F_VX, F_VY, F_VZ, F_RADIUS, F_MASS, F_FATE = range(11, 17)
 F_U0, F_U1, F_U2, F_U3 = range(17, 21)   # user agent parameters (RULE_CUSTOM)
+F_THETA, F_OMEGA, F_DIAMETER, F_LENGTH, F_TORQUE, F_PRESSURE = range(21, 27)   # rods
 INT_FIELDS = (F_ID, F_LF, F_FATE)
 
 

@@ -162,6 +162,18 @@ class CBMModels:
         return _Law(A.LAW_SOFT_SPHERES, [b, mu, f[0, 0], f[0, 1], f[1, 0], f[1, 1]])
 
     @staticmethod
+    def rods2D(kn, gamma_n, gamma_t, mu_cc, mu_cw, beta, beta_omega):
+        """CBMModels.rods2D (src/models/rods.jl:61-121): 2-D rods of diameter d,
+        length l and mass m at angle theta; the contact force and torque of
+        rodForces summed over the neighbors into fx, fy, W and pressure, with
+        dt(v) = -beta v + f/m, dt(theta) = omega and
+        dt(omega) = W/(m (d + l)^2/12 + m d^2) - beta_omega omega.
+        Per-agent fields: vx, vy, theta, ω, d, l, m (inputs) and fx, fy, W,
+        pressure (the last evaluation's sums).  mu_cw is carried, unused, as
+        in the reference."""
+        return _Law(A.LAW_RODS, [kn, gamma_n, gamma_t, mu_cc, mu_cw, beta, beta_omega])
+
+    @staticmethod
     def aggregation(rRep, fRep, rAtr, fAtr, walls=True):
         """The Aggregation model of docs/src/Aggregation.md:62-123 (with its wall push)."""
         return _Law(A.LAW_AGGREGATION, [rRep, fRep, rAtr, fAtr, 1.0 if walls else 0.0])
@@ -311,7 +323,9 @@ def sim_desc(abm, N, n_max, simBox, law_p=None):
 
 _FIELDS = {"x": A.F_X, "y": A.F_Y, "z": A.F_Z, "id": A.F_ID, "fx": A.F_FX, "fy": A.F_FY,
            "fz": A.F_FZ, "vol": A.F_VOL, "growth": A.F_GROWTH, "vbirth": A.F_VBIRTH, "lf": A.F_LF,
-           "vx": A.F_VX, "vy": A.F_VY, "vz": A.F_VZ, "r": A.F_RADIUS, "m": A.F_MASS, "cellFate": A.F_FATE}
+           "vx": A.F_VX, "vy": A.F_VY, "vz": A.F_VZ, "r": A.F_RADIUS, "m": A.F_MASS, "cellFate": A.F_FATE,
+           "theta": A.F_THETA, "ω": A.F_OMEGA, "d": A.F_DIAMETER, "l": A.F_LENGTH, "W": A.F_TORQUE,
+           "pressure": A.F_PRESSURE}
 
 
 class Context:
@@ -525,6 +539,11 @@ class Community:
             self.params["r"] = np.ones(self.N)
             self.params["m"] = np.ones(self.N)
             self.params["cellFate"] = np.ones(self.N, dtype=np.int64)
+        if abm.agentODE.law == A.LAW_RODS:
+            for k in ("vx", "vy", "theta", "ω", "fx", "fy", "W", "pressure"):
+                self.params[k] = np.zeros(self.N)
+            for k in ("d", "l", "m"):
+                self.params[k] = np.ones(self.N)
         if abm.rule_src is not None:
             self.params["vol"] = np.ones(self.N)
             self.params["growth"] = np.zeros(self.N)

@@ -47,6 +47,8 @@ enum ColId {
   C_ID, C_SLOT, C_FX, C_FY, C_FZ, C_GR, C_VB, C_LF,
   C_RAD, C_MASS, C_FATE,                // per-agent soft-sphere properties
   C_U0, C_U1, C_U2, C_U3,               // user agent parameters (CBM_RULE_CUSTOM)
+  C_TH, C_OM,                           // rods: angle and its rate, DE state too
+  C_DIAM, C_LEN, C_TQ, C_PR,            // rods: diameter, length, torque and pressure
   C_CCELL,                              // CellLinked on a skin list: packed cell of the step start (int32)
   C_NCOLS
 };
@@ -58,6 +60,7 @@ struct cbm_ctx {
   int dims = 1, nvar = 1;
   bool growth = false, lagged = false, verlet = false, cl = false, full = false, vtime = false, rules = false;
   bool inertial = false;                  // CBM_LAW_SOFT_SPHERES: velocities are DE variables
+  bool rods = false;                      // CBM_LAW_RODS: x, y, vx, vy, theta, omega are DE variables
   // CellLinked executed on a skin list (cl && clv): no per-step cell sort;
   // each stage orders an agent's list entries by its cell among the 3^D
   // neighbor cells (the reference loop order) from the agents' packed cells
@@ -247,9 +250,20 @@ static int validate(cbm_ctx* c) {
   if (m.dims < 1 || m.dims > 3) return fail(c, CBM_ERR_CONFIG, "dims must be 1, 2 or 3");
   if (m.integrator < CBM_EULER || m.integrator > CBM_EULER_HEUN) return fail(c, CBM_ERR_CONFIG, "unknown integrator");
   if (m.neighbors < CBM_FULL || m.neighbors > CBM_CLVD) return fail(c, CBM_ERR_CONFIG, "unknown neighbor algorithm");
-  if (m.law < CBM_LAW_NONE || m.law > CBM_LAW_SOFT_SPHERES) return fail(c, CBM_ERR_CONFIG, "unknown interaction law");
+  if (m.law < CBM_LAW_NONE || m.law > CBM_LAW_RODS) return fail(c, CBM_ERR_CONFIG, "unknown interaction law");
   if (m.law == CBM_LAW_SOFT_SPHERES && m.rule != CBM_RULE_NONE)
     return fail(c, CBM_ERR_CONFIG, "the soft-sphere model takes no agent rules");
+  if (m.law == CBM_LAW_RODS) {
+    // rods2D (src/models/rods.jl:61-121): a 2-D agentODE without rules
+    if (m.dims != 2) return fail(c, CBM_ERR_CONFIG, "the rod model is 2-D");
+    if (m.rule != CBM_RULE_NONE) return fail(c, CBM_ERR_CONFIG, "the rod model takes no agent rules");
+    if (m.periodic) return fail(c, CBM_ERR_CONFIG, "the rod model has no periodic box (its contact geometry uses raw coordinates)");
+    if (m.lagged) return fail(c, CBM_ERR_CONFIG, "the rod model sums its contacts at the stage state (no lagged forces)");
+    if (m.integrator != CBM_EULER && m.integrator != CBM_HEUN && m.integrator != CBM_RK4)
+      return fail(c, CBM_ERR_CONFIG, "the rod model is an agentODE: Euler, Heun or RK4");
+    if (m.medium_n[0] > 0) return fail(c, CBM_ERR_CONFIG, "the rod model takes no medium");
+    if (m.drift != 0.0) return fail(c, CBM_ERR_CONFIG, "the rod model takes no drift term");
+  }
   if (!(m.dt > 0)) return fail(c, CBM_ERR_CONFIG, "dt must be positive");
   if (m.walk != 0 && m.walk != 1 && (m.walk < 512 || m.walk > 2048))
     return fail(c, CBM_ERR_CONFIG, "walk is 0, 1 or a round of 512..2048 list entries");
@@ -611,7 +625,8 @@ static int init_ctx(cbm_ctx* c) {
   c->growth = m.rule == CBM_RULE_GROWTH_DIVISION || m.rule == CBM_RULE_CUSTOM;
   c->rules = m.rule != CBM_RULE_NONE;
   c->inertial = m.law == CBM_LAW_SOFT_SPHERES;
-  c->nvar = c->inertial ? 2 * c->dims : c->dims + (c->growth ? 1 : 0) + c->m.user_de;
+  c->rods = m.law == CBM_LAW_RODS;
+  c->nvar = c->rods ? 6 : (c->inertial ? 2 * c->dims : c->dims + (c->growth ? 1 : 0) + c->m.user_de);
   c->lagged = m.lagged != 0;
   c->full = m.neighbors == CBM_FULL;
   c->cl = m.neighbors == CBM_CELL_LINKED;
@@ -664,6 +679,16 @@ static int init_ctx(cbm_ctx* c) {
       HIPCHK(c, hipMemcpy(c->col[C_MASS].buf[b], one.data(), sizeof(double) * c->C, hipMemcpyHostToDevice));
       HIPCHK(c, hipMemcpy(c->col[C_FATE].buf[b], fate1.data(), sizeof(int64_t) * c->C, hipMemcpyHostToDevice));
     }
+  }
+  if (c->rods) {
+    // DE velocities, angle and rate (0 until set); diameter, length and
+    // mass 1 until set; the last stage's fx, fy, W and pressure
+    const int cols[] = {C_VX, C_VY, C_TH, C_OM, C_DIAM, C_LEN, C_MASS, C_FX, C_FY, C_TQ, C_PR};
+    for (int k : cols) if ((rc = alloc_column(c, k, 8))) return rc;
+    std::vector<double> one((size_t)c->C, 1.0);
+    for (int k : {C_DIAM, C_LEN, C_MASS})
+      for (int b = 0; b < 2; ++b)
+        HIPCHK(c, hipMemcpy(c->col[k].buf[b], one.data(), sizeof(double) * c->C, hipMemcpyHostToDevice));
   }
   // user agent parameters: read by a rule body, or DE variables of the law body
   if (m.rule == CBM_RULE_CUSTOM || m.user_de > 0)
@@ -1007,6 +1032,14 @@ static StageArgs stage_base(cbm_ctx* c) {
     a.mass = (const double*)c->col[C_MASS].cur();
     a.fate = (const int64_t*)c->col[C_FATE].cur();
   }
+  if (c->rods) {
+    a.mass = (const double*)c->col[C_MASS].cur();
+    a.rod_d = (const double*)c->col[C_DIAM].cur();
+    a.rod_l = (const double*)c->col[C_LEN].cur();
+    const int outs[4] = {C_FX, C_FY, C_TQ, C_PR};
+    for (int k = 0; k < 4; ++k) a.rod_out[k] = (double*)c->col[outs[k]].cur();
+    a.rod_err = c->flags + 7;
+  }
   return a;
 }
 
@@ -1076,6 +1109,7 @@ static int run_stage(cbm_ctx* c, StageArgs& a, int kind) {
 // Column of DE variable v: positions, then velocities (soft spheres) or vol.
 static int statecol_k(const cbm_ctx* c, int v) {
   if (v < c->dims) return C_X + v;
+  if (c->rods) return v < 4 ? C_VX + (v - 2) : C_TH + (v - 4);
   if (c->inertial) return C_VX + (v - c->dims);
   return c->growth && v == c->dims ? C_VOL : C_U0 + (v - c->dims - (c->growth ? 1 : 0));
 }
@@ -1401,6 +1435,9 @@ static int check_flags(cbm_ctx* c) {
   if (f[3] & 4) return fail(c, CBM_ERR_CONFIG, "an agentRule body looped over neighbors without naming forNeighbors");
   if (f[3] & 8) return fail(c, CBM_ERR_CONFIG, "an agentRule body used the medium in a model without a medium");
   if (f[3]) return fail(c, CBM_ERR_CAPACITY, "agent capacity exceeded: raise preallocation (n_max)");
+  if (f[7])
+    return fail(c, CBM_ERR_NONFINITE, "rod contact with d_i < r_ij < (d_i + d_j)/2: the overlap d_i - r_ij is negative "
+                                      "and (d_i - r_ij)^1.5 has no real value");
   return CBM_OK;
 }
 
@@ -1640,13 +1677,19 @@ static int field_col(cbm_ctx* c, int32_t field, int& k) {
     case CBM_F_VBIRTH: if (!c->growth) return CBM_ERR_ARG; k = C_VB; return CBM_OK;
     case CBM_F_LF: if (!c->rules) return CBM_ERR_ARG; k = C_LF; return CBM_OK;
     case CBM_F_FX: case CBM_F_FY: case CBM_F_FZ:
-      if (!c->lagged || field - CBM_F_FX >= c->dims) return CBM_ERR_ARG;
+      if (!(c->lagged || c->rods) || field - CBM_F_FX >= c->dims) return CBM_ERR_ARG;
       k = C_FX + (field - CBM_F_FX); return CBM_OK;
     case CBM_F_VX: case CBM_F_VY: case CBM_F_VZ:
-      if (!c->inertial || field - CBM_F_VX >= c->dims) return CBM_ERR_ARG;
+      if (!(c->inertial || c->rods) || field - CBM_F_VX >= c->dims) return CBM_ERR_ARG;
       k = C_VX + (field - CBM_F_VX); return CBM_OK;
     case CBM_F_RADIUS: if (!c->inertial) return CBM_ERR_ARG; k = C_RAD; return CBM_OK;
-    case CBM_F_MASS: if (!c->inertial) return CBM_ERR_ARG; k = C_MASS; return CBM_OK;
+    case CBM_F_MASS: if (!(c->inertial || c->rods)) return CBM_ERR_ARG; k = C_MASS; return CBM_OK;
+    case CBM_F_THETA: if (!c->rods) return CBM_ERR_ARG; k = C_TH; return CBM_OK;
+    case CBM_F_OMEGA: if (!c->rods) return CBM_ERR_ARG; k = C_OM; return CBM_OK;
+    case CBM_F_DIAMETER: if (!c->rods) return CBM_ERR_ARG; k = C_DIAM; return CBM_OK;
+    case CBM_F_LENGTH: if (!c->rods) return CBM_ERR_ARG; k = C_LEN; return CBM_OK;
+    case CBM_F_TORQUE: if (!c->rods) return CBM_ERR_ARG; k = C_TQ; return CBM_OK;
+    case CBM_F_PRESSURE: if (!c->rods) return CBM_ERR_ARG; k = C_PR; return CBM_OK;
     case CBM_F_FATE: if (!c->inertial) return CBM_ERR_ARG; k = C_FATE; return CBM_OK;
     case CBM_F_U0: case CBM_F_U1: case CBM_F_U2: case CBM_F_U3:
       if (!c->has_col[C_U0]) return CBM_ERR_ARG;

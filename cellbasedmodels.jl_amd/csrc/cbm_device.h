@@ -78,7 +78,8 @@ __device__ __forceinline__ int64_t floor_div(double a, double b) {
 // hoisted out of the pair loop.  operator() returns whether the pair
 // interacts (coincident agents, d == 0, never do) and always sets s, so the
 // caller can update its accumulators with selects instead of branches.
-enum { LAW_NONE = 0, LAW_LINEAR = 1, LAW_REP_ADH = 2, LAW_AGGREGATION = 3, LAW_CUSTOM = 4, LAW_SOFT_SPHERES = 5 };
+enum { LAW_NONE = 0, LAW_LINEAR = 1, LAW_REP_ADH = 2, LAW_AGGREGATION = 3, LAW_CUSTOM = 4, LAW_SOFT_SPHERES = 5,
+       LAW_RODS = 6 };
 
 template <class A, class B> struct same_type { static constexpr bool value = false; };
 template <class A> struct same_type<A, A> { static constexpr bool value = true; };
@@ -191,6 +192,118 @@ template <> struct PairLaw<LAW_SOFT_SPHERES> {
   }
   __device__ __forceinline__ bool operator()(double, double& s) const { s = 0.0; return false; }
 };
+
+// ------------------------------------------------------------------ rods
+// The 2-D rod model (src/models/rods.jl:21-124).  A rod is the segment of
+// length l through its centre (x, y) at angle theta, thickened to diameter
+// d.  Two rods touch when the closest "virtual spheres" on their axes
+// (rodIntersection, src/metrics.jl:199-246) are closer than the mean
+// diameter; the stage skeleton handles this law apart (rod_stage_body).
+struct RodLaw {
+  static constexpr bool active = true, per_axis_division = false, walls = false;
+};
+
+// A rod's axis: centre, length, cos and sin of the angle (each evaluated
+// once; the reference recomputes them per call, to the same values).
+struct RodAxis { double x, y, l, c, s; };
+
+// intersection2lines (src/metrics.jl:148-166): the meeting point of the two
+// axis lines, or the midpoint of the centres when |sin(t2 - t1)| <= 1e-7.
+__device__ __forceinline__ void rod_meet(const RodAxis& p, const RodAxis& q, double& px, double& py) {
+  const double cx = p.x - q.x, cy = p.y - q.y;
+  const double nrm = p.c * q.s - p.s * q.c;
+  if (fabs(nrm) > 0.0000001) {
+    const double k = (-q.s * cx + q.c * cy) / nrm;
+    px = k * p.c + p.x;
+    py = k * p.s + p.y;
+  } else {
+    px = (p.x + q.x) / 2;
+    py = (p.y + q.y) / 2;
+  }
+}
+
+// point2line (src/metrics.jl:175-183): (ax, ay) projected onto q's axis line.
+__device__ __forceinline__ void rod_project(double ax, double ay, const RodAxis& q, double& ox, double& oy) {
+  const double cx = q.x - ax, cy = q.y - ay;
+  const double k = q.s * cx - q.c * cy;
+  ox = k * q.s + ax;
+  oy = -k * q.c + ay;
+}
+
+// pointInsideRod (src/metrics.jl:194-202) with separation 0.99: a point on
+// q's axis pulled to 0.99 of its distance from the centre, at most l/2.
+__device__ __forceinline__ void rod_clamp(const RodAxis& q, double ax, double ay, double& ox, double& oy) {
+  const double ex = q.x - ax, ey = q.y - ay;
+  const double di = fmax(sqrt(ex * ex + ey * ey), 0.00000001);
+  const double ux = (ax - q.x) / di, uy = (ay - q.y) / di;
+  const double r = 0.99 * fmin(di, q.l / 2);
+  ox = r * ux + q.x;
+  oy = r * uy + q.y;
+}
+
+// rodIntersection (src/metrics.jl:210-246): the contact points (ax, ay) on p
+// and (bx, by) on q, branch for branch.
+__device__ __forceinline__ void rod_closest(const RodAxis& p, const RodAxis& q, double& ax, double& ay, double& bx,
+                                            double& by) {
+  double px, py;
+  rod_meet(p, q, px, py);
+  const double di = sqrt((p.x - px) * (p.x - px) + (p.y - py) * (p.y - py));
+  double dj = sqrt((q.x - px) * (q.x - px) + (q.y - py) * (q.y - py));
+  const double nrm = p.c * q.s - p.s * q.c;
+  if (fabs(nrm) < 0.000001) {
+    rod_project(px, py, p, ax, ay);
+    rod_clamp(p, ax, ay, ax, ay);
+    rod_project(px, py, q, bx, by);
+    rod_clamp(q, bx, by, bx, by);
+  } else if (di <= p.l / 2 && dj <= q.l / 2) {
+    rod_clamp(p, px, py, ax, ay);
+    rod_clamp(q, px, py, bx, by);
+  } else if (di <= p.l / 2 && dj > q.l / 2) {
+    rod_clamp(q, px, py, bx, by);
+    rod_project(bx, by, p, ax, ay);
+  } else if (di > p.l / 2 && dj <= q.l / 2) {
+    rod_clamp(p, px, py, ax, ay);
+    rod_project(ax, ay, q, bx, by);
+  } else {
+    rod_clamp(q, px, py, bx, by);
+    rod_project(bx, by, p, ax, ay);
+    dj = sqrt((p.x - ax) * (p.x - ax) + (p.y - ay) * (p.y - ay));
+    if (dj > p.l / 2) {
+      rod_clamp(p, px, py, ax, ay);
+      double tx, ty;
+      rod_project(ax, ay, q, tx, ty);
+      dj = sqrt((q.x - tx) * (q.x - tx) + (q.y - ty) * (q.y - ty));
+      if (dj < q.l / 2) { bx = tx; by = ty; }
+    }
+  }
+}
+
+// One rod of a pair: axis, diameter, mass, velocity.
+struct RodState { RodAxis a; double d, m, vx, vy; };
+
+// rodForces (src/models/rods.jl:21-59): the force (fx, fy) and torque w on
+// rod i from rod j; false (and nothing set) without contact.  The overlap
+// uses d_i, the effective mass m_i / 2.  An overlap below zero (d_i < r_ij <
+// (d_i + d_j)/2) sets *domain: the reference's delta^1.5 has no real value.
+__device__ __forceinline__ bool rod_force(const RodState& i, const RodState& j, double kn, double gn, double gt,
+                                          double mucc, double& fx, double& fy, double& w, bool& domain) {
+  double ax, ay, bx, by;
+  rod_closest(i.a, j.a, ax, ay, bx, by);
+  const double rij = sqrt((ax - bx) * (ax - bx) + (ay - by) * (ay - by));
+  if (!(rij > 0.0 && rij < (i.d + j.d) / 2)) return false;
+  const double del = i.d - rij;
+  if (del < 0.0) domain = true;
+  const double me = i.m / 2;
+  const double nx = (ax - bx) / rij, ny = (ay - by) / rij;
+  const double vx = i.vx - j.vx, vy = i.vy - j.vy;
+  const double vn = nx * vx + ny * vy;
+  const double fn = kn * pow(del, 1.5) - gn * me * del * vn;
+  const double ft = -fmin(gt * me * pow(del, 0.5), mucc * fn);
+  fx = fn * nx + ft * (vx - vn * nx);
+  fy = fn * ny + ft * (vy - vn * ny);
+  w = (ax - i.a.x) * fy - (ay - i.a.y) * fx;
+  return true;
+}
 
 template <> struct PairLaw<LAW_NONE> {
   static constexpr bool active = false, per_axis_division = false, walls = false;

@@ -52,6 +52,12 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   stage_body<D, NB, PairLaw<LAW>, PER, NOISE>(a);
 }
 
+// a stage of the 2-D rod model (no periodic box, no noise)
+template <int NB>
+__global__ __launch_bounds__(256) void k_rods(StageArgs a) {
+  rod_stage_body<NB>(a);
+}
+
 // cell-linked stage on LDS brick tiles (dynamic LDS: brick_lds_bytes)
 template <int D, int LAW, int PER, int NOISE>
 __global__ __launch_bounds__(256) void k_brick(StageArgs a) {
@@ -1222,6 +1228,15 @@ static hipError_t launch_stage_nb(int nb, int law, const StageLaunch& L, const S
 
 template <int D, int NOISE>
 static hipError_t launch_stage_dims(int nb, int law, int periodic, const StageLaunch& L, const StageArgs& a) {
+  if (law == LAW_RODS) {      // 2-D, no periodic box, no noise: the host refuses the rest
+    if (D != 2 || NOISE || periodic) return hipErrorInvalidValue;
+    switch (nb) {
+      case NB_FULL: hipLaunchKernelGGL((k_rods<NB_FULL>), L.grid, L.block, 0, L.s, a); break;
+      case NB_CELL: hipLaunchKernelGGL((k_rods<NB_CELL>), L.grid, L.block, 0, L.s, a); break;
+      default: hipLaunchKernelGGL((k_rods<NB_LIST>), L.grid, L.block, 0, L.s, a); break;
+    }
+    return hipGetLastError();
+  }
   if (law == LAW_NONE) {      // no interaction body: no neighbor walk at all
     hipLaunchKernelGGL((k_stage<D, NB_FULL, LAW_NONE, 0, NOISE>), L.grid, L.block, 0, L.s, a);
     return hipGetLastError();

@@ -285,6 +285,7 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
                                              const double* xi, double* acc, const double* upre,
                                              double disp2 = -1.0, double* e2_out = nullptr) {
   constexpr bool SOFT = same_type<Law, PairLaw<LAW_SOFT_SPHERES>>::value;
+  constexpr bool RODS = same_type<Law, RodLaw>::value;
   // u of variable c (with PRE, the caller loaded the positions' u early into
   // upre), and the stage state, whose positions are xi
   const double u0 = upre[0], u1 = upre[1], u2 = upre[2];   // values, so no pointer is selected
@@ -307,7 +308,21 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
   constexpr bool UDE = CBM_USER_DE > 0 && has_de<Law>::value;
   constexpr int NS = UDE ? MAX_STATE : BUILTIN_STATE;
   double du[NS];
-  if constexpr (SOFT) {
+  if constexpr (RODS) {
+    // acc = {fx, fy, W, pressure}; the DE variables x, y, vx, vy, theta,
+    // omega (src/models/rods.jl:111-116)
+    const double beta = sd.p[5], bw = sd.p[6];
+    const double m = a.mass[i], d = a.rod_d[i], l = a.rod_l[i];
+    const double vx = a.S[2][i], vy = a.S[3][i], om = a.S[5][i];
+    du[0] = vx;
+    du[1] = vy;
+    du[2] = -beta * vx + acc[0] / m;
+    du[3] = -beta * vy + acc[1] / m;
+    du[4] = om;
+    du[5] = acc[2] / (m * ((d + l) * (d + l)) / 12 + m * (d * d)) - bw * om;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a.rod_out[k][i] = acc[k];
+  } else if constexpr (SOFT) {
     // dt(v) = -b*v/m + f/m, dt(x) = v (src/models/softSpheres.jl:32-35)
     const double b = sd.p[0], m = a.mass[i];
 #pragma unroll
@@ -336,7 +351,7 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
       if (a.write_fnew) a.Fnew[c][i] = acc[c];
     }
   }
-  const bool grow = UDE ? CBM_DE_BASE > D : !SOFT && a.nvar > D;
+  const bool grow = UDE ? CBM_DE_BASE > D : !SOFT && !RODS && a.nvar > D;
   if (grow) du[D] = a.growth[i] * a.S[D][i];
   if constexpr (UDE) {
     // dt of the user DE variables at the stage state
@@ -352,7 +367,7 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
   }
 
   const double dt = a.dt;
-  const int nv = UDE ? CBM_DE_BASE + CBM_USER_DE : (SOFT ? 2 * D : (grow ? D + 1 : D));
+  const int nv = UDE ? CBM_DE_BASE + CBM_USER_DE : (RODS ? 6 : (SOFT ? 2 * D : (grow ? D + 1 : D)));
   double res[NS];
   if (NOISE && !same_type<Noise, ConstNoise>::value) {
     // a user agentSDE body: g of every DE variable at the stage state, one
@@ -536,6 +551,48 @@ __device__ __forceinline__ void stage_body(const StageArgs& a) {
   double acc[3] = {0.0, 0.0, 0.0};
   if (Law::active) gather_global<D, NB, Law, PER>(a, sd, N, i, xi, acc);
   stage_finish<D, Law, NOISE, false, Noise>(a, sd, ri.sim, i, xi, acc, xi);
+}
+
+// One integrator stage of the rod model for every agent row: the contact
+// sums over the neighbor candidates in reference loop order (pairs out of
+// contact add exactly nothing: the reference adds +0 to sums that start at
+// +0), then the stage update.  2-D, no periodic box.
+template <int NB>
+__device__ __forceinline__ void rod_stage_body(const StageArgs& a) {
+  const RowIdx ri = row_index(a.g);
+  const SimDev& sd = a.g.sims[ri.sim];
+  const int64_t N = a.g.st[ri.sim].N;
+  const int64_t i = ri.row;
+  if (ri.local >= N) return;
+  const double kn = sd.p[0], gn = sd.p[1], gt = sd.p[2], mucc = sd.p[3];
+  auto rod_at = [&](int64_t j) {
+    RodState r;
+    r.a.x = a.S[0][j];
+    r.a.y = a.S[1][j];
+    r.vx = a.S[2][j];
+    r.vy = a.S[3][j];
+    sincos(a.S[4][j], &r.a.s, &r.a.c);
+    r.a.l = a.rod_l[j];
+    r.d = a.rod_d[j];
+    r.m = a.mass[j];
+    return r;
+  };
+  const RodState me = rod_at(i);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  bool domain = false;
+  for_candidates<2, NB>(a, sd, N, i, [&](int64_t j) {
+    const RodState other = rod_at(j);
+    double fx, fy, w;
+    if (rod_force(me, other, kn, gn, gt, mucc, fx, fy, w, domain)) {
+      acc[0] = acc[0] + fx;
+      acc[1] = acc[1] + fy;
+      acc[2] = acc[2] + w;
+      acc[3] = acc[3] + sqrt(fx * fx + fy * fy);
+    }
+  });
+  if (domain) *a.rod_err = 1;
+  double xi[3] = {me.a.x, me.a.y, 0.0};
+  stage_finish<2, RodLaw, 0>(a, sd, ri.sim, i, xi, acc, xi);
 }
 
 // ------------------------------------------------ flattened list stage

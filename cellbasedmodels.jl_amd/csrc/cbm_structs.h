@@ -165,6 +165,10 @@ struct StageArgs {
   const double* rad;            // soft spheres: radius, mass, cellFate (1-based) per row
   const double* mass;
   const int64_t* fate;
+  const double* rod_d;          // rods: diameter and length per row (mass in mass)
+  const double* rod_l;
+  double* rod_out[4];           // rods: fx, fy, W, pressure of this stage
+  int32_t* rod_err;             // rods: set when a contact's overlap d_i - r_ij is negative
   const int64_t* id;
   const int32_t* list;          // ELL Verlet list, entry q of row i at q*ldl + i
   const int32_t* list_n;

@@ -77,14 +77,27 @@ enum cbm_neighbors {
  *     agent has a radius (CBM_F_RADIUS), a mass (CBM_F_MASS) and a cellFate
  *     in {1, 2} (CBM_F_FATE, int64).  With r_ij = r_i + r_j, for 0 < d < mu r_ij
  *       F_i += (f0[fate_i][fate_j] (r_ij/d - 1)(mu r_ij/d - 1)(x_i - x_j))/d
- *     and dv/dt = (-b v)/m + F/m, dx/dt = v.  No rules or adjoint.          */
+ *     and dv/dt = (-b v)/m + F/m, dx/dt = v.  No rules or adjoint.
+ *   CBM_LAW_RODS p = {kn, gamma_n, gamma_t, mu_cc, mu_cw, beta, beta_w}:
+ *     the 2-D rod model of Volfson et al. (src/models/rods.jl:21-124, the
+ *     rod geometry of src/metrics.jl:138-246).  The DE variables are x, y,
+ *     the velocities (CBM_F_VX, VY), the angle theta (CBM_F_THETA) and its
+ *     rate omega (CBM_F_OMEGA); each rod has a diameter (CBM_F_DIAMETER), a
+ *     length (CBM_F_LENGTH) and a mass (CBM_F_MASS).  Over the neighbors in
+ *     walk order the closest virtual spheres of the two rods give the
+ *     contact force (Fx, Fy) and torque W of rodForces, summed into fx, fy,
+ *     W and pressure (+= |F|), written every stage (CBM_F_FX, FY,
+ *     CBM_F_TORQUE, CBM_F_PRESSURE); then dt(v) = -beta v + f/m, dt(theta) =
+ *     omega, dt(omega) = W/(m (d + l)^2/12 + m d^2) - beta_w omega.  2-D,
+ *     no periodic box, no rules, Euler, Heun or RK4.                      */
 enum cbm_law {
   CBM_LAW_NONE = 0,
   CBM_LAW_LINEAR = 1,
   CBM_LAW_REP_ADH = 2,
   CBM_LAW_AGGREGATION = 3,
   CBM_LAW_CUSTOM = 4,
-  CBM_LAW_SOFT_SPHERES = 5
+  CBM_LAW_SOFT_SPHERES = 5,
+  CBM_LAW_RODS = 6
 };
 
 /* ---------------------------------------------------------- medium solvers */
@@ -150,7 +163,10 @@ enum cbm_field {
   CBM_F_RADIUS = 14, CBM_F_MASS = 15,
   CBM_F_FATE = 16,                             /* int64 cellFate, 1-based                   */
   CBM_F_U0 = 17, CBM_F_U1 = 18, CBM_F_U2 = 19, CBM_F_U3 = 20,   /* user agent parameters (CBM_RULE_CUSTOM) */
-  CBM_F_COUNT = 21
+  CBM_F_THETA = 21, CBM_F_OMEGA = 22,          /* rods: angle and its rate (DE variables)  */
+  CBM_F_DIAMETER = 23, CBM_F_LENGTH = 24,      /* rods: per-rod geometry                    */
+  CBM_F_TORQUE = 25, CBM_F_PRESSURE = 26,      /* rods: W and pressure of the last stage    */
+  CBM_F_COUNT = 27
 };
 
 /* --------------------------------------------------- model description */

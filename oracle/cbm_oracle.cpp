@@ -19,6 +19,8 @@
 //   step! order                       src/CommunityStructure/step.jl:96-107
 //   @∂2 medium stencil                src/AgentStructure/macros.jl:203-257
 //   @mediumInside / @mediumBorder     src/AgentStructure/macros.jl:259-311
+//   rods2D / rodForces                src/models/rods.jl:21-121
+//   rodIntersection and helpers       src/metrics.jl:138-246
 // with the spec decisions the GPU path shares (exact floor quotients,
 // periodic minimum image, canonical ascending compaction order, Philox noise,
 // zero-flux ghost-mirror medium, border nodes assigned from the step-start
@@ -142,6 +144,9 @@ struct Orc {
   double t = 0;
   bool growth = false;
   bool inertial = false;                      // CBM_LAW_SOFT_SPHERES: velocities are DE variables
+  bool rods = false;                          // CBM_LAW_RODS: x, y, vx, vy, theta, omega are DE variables
+  std::vector<double> TH, OM, rodD, rodL;     // rods: angle, its rate, diameter, length
+  std::vector<double> rodW, rodP;             // rods: torque and pressure of the last evaluation
   double noise_slope = 0.0;                   // state-dependent agentSDE g_a(x) = noise + slope x_a (positions)
   std::vector<double> X[3], F[3], Fnew[3];
   std::vector<double> V[3], rad, mass;        // velocities, radius, mass (soft spheres)
@@ -323,6 +328,133 @@ static inline bool soft_s(const double* p, double d, double ri, double rj, int64
   const double f0 = p[2 + 2 * (fi - 1) + (fj - 1)];
   s = f0 * (rij / d - 1) * (p[1] * rij / d - 1);
   return true;
+}
+
+// ------------------------------------------------------------------ rods
+// A rod's axis line: centre (x, y), length l, angle t.
+struct Rod { double x, y, l, t; };
+
+// intersection2lines, src/metrics.jl:148-166.
+static void line_meet(const Rod& a, const Rod& b, double& px, double& py) {
+  const double cxA = a.x - b.x, cyA = a.y - b.y;
+  const double nrm = std::cos(a.t) * std::sin(b.t) - std::sin(a.t) * std::cos(b.t);
+  if (std::fabs(nrm) > 0.0000001) {
+    const double sc = (-std::sin(b.t) * cxA + std::cos(b.t) * cyA) / nrm;
+    px = sc * std::cos(a.t) + a.x;
+    py = sc * std::sin(a.t) + a.y;
+  } else {
+    px = (a.x + b.x) / 2;
+    py = (a.y + b.y) / 2;
+  }
+}
+
+// point2line, src/metrics.jl:175-183: the foot of (x1, y1) on the line
+// through (x2, y2) at angle t2.
+static void foot(double x1, double y1, double x2, double y2, double t2, double& ox, double& oy) {
+  const double cx = x2 - x1, cy = y2 - y1;
+  ox = (std::sin(t2) * cx - std::cos(t2) * cy) * std::sin(t2) + x1;
+  oy = -(std::sin(t2) * cx - std::cos(t2) * cy) * std::cos(t2) + y1;
+}
+
+// pointInsideRod, src/metrics.jl:194-202.
+static void inside(const Rod& r, double px, double py, double sep, double& ox, double& oy) {
+  const double di = std::max(std::sqrt((r.x - px) * (r.x - px) + (r.y - py) * (r.y - py)), 0.00000001);
+  const double dxi = (px - r.x) / di, dyi = (py - r.y) / di;
+  ox = sep * std::min(di, r.l / 2) * dxi + r.x;
+  oy = sep * std::min(di, r.l / 2) * dyi + r.y;
+}
+
+// rodIntersection, src/metrics.jl:210-246 (separation 0.99): the closest
+// virtual spheres (x1a, y1a) on rod 1 and (x2a, y2a) on rod 2.
+static void rod_intersection(const Rod& r1, const Rod& r2, double* out) {
+  const double sep = 0.99;
+  double x1a = r1.x, y1a = r1.y, x2a = r2.x, y2a = r2.y;
+  double px, py;
+  line_meet(r1, r2, px, py);
+  const double di = std::sqrt((r1.x - px) * (r1.x - px) + (r1.y - py) * (r1.y - py));
+  double dj = std::sqrt((r2.x - px) * (r2.x - px) + (r2.y - py) * (r2.y - py));
+  const double nrm = std::cos(r1.t) * std::sin(r2.t) - std::sin(r1.t) * std::cos(r2.t);
+  if (std::fabs(nrm) < 0.000001) {
+    foot(px, py, r1.x, r1.y, r1.t, x1a, y1a);
+    inside(r1, x1a, y1a, sep, x1a, y1a);
+    foot(px, py, r2.x, r2.y, r2.t, x2a, y2a);
+    inside(r2, x2a, y2a, sep, x2a, y2a);
+  } else if (di <= r1.l / 2 && dj <= r2.l / 2) {
+    inside(r1, px, py, sep, x1a, y1a);
+    inside(r2, px, py, sep, x2a, y2a);
+  } else if (di <= r1.l / 2 && dj > r2.l / 2) {
+    inside(r2, px, py, sep, x2a, y2a);
+    foot(x2a, y2a, r1.x, r1.y, r1.t, x1a, y1a);
+  } else if (di > r1.l / 2 && dj <= r2.l / 2) {
+    inside(r1, px, py, sep, x1a, y1a);
+    foot(x1a, y1a, r2.x, r2.y, r2.t, x2a, y2a);
+  } else {
+    inside(r2, px, py, sep, x2a, y2a);
+    foot(x2a, y2a, r1.x, r1.y, r1.t, x1a, y1a);
+    dj = std::sqrt((r1.x - x1a) * (r1.x - x1a) + (r1.y - y1a) * (r1.y - y1a));
+    if (dj > r1.l / 2) {
+      inside(r1, px, py, sep, x1a, y1a);
+      double x2b, y2b;
+      foot(x1a, y1a, r2.x, r2.y, r2.t, x2b, y2b);
+      dj = std::sqrt((r2.x - x2b) * (r2.x - x2b) + (r2.y - y2b) * (r2.y - y2b));
+      if (dj < r2.l / 2) { x2a = x2b; y2a = y2b; }
+    }
+  }
+  out[0] = x1a; out[1] = y1a; out[2] = x2a; out[3] = y2a;
+}
+
+// rodForces, src/models/rods.jl:21-59.  r = {x, y, d, l, theta, vx, vy, m}
+// of each rod, k = {kn, gamma_n, gamma_t, mu_cc}.  Writes {Fx, Fy, W};
+// returns -1 when the overlap d - r_ij of a contact is negative (the
+// reference's delta^1.5 raises a DomainError there), else 0.
+static int rod_forces(const double* r1, const double* r2, const double* k, double* out) {
+  out[0] = 0.; out[1] = 0.; out[2] = 0.;
+  double q[4];
+  rod_intersection(Rod{r1[0], r1[1], r1[3], r1[4]}, Rod{r2[0], r2[1], r2[3], r2[4]}, q);
+  const double rij = std::sqrt((q[0] - q[2]) * (q[0] - q[2]) + (q[1] - q[3]) * (q[1] - q[3]));
+  if (rij > 0. && rij < (r1[2] + r2[2]) / 2) {
+    const double delta = r1[2] - rij;
+    if (delta < 0) return -1;
+    const double me = r1[7] / 2;
+    const double nx = (q[0] - q[2]) / rij, ny = (q[1] - q[3]) / rij;
+    const double vx = r1[5] - r2[5], vy = r1[6] - r2[6];
+    const double vn = nx * vx + ny * vy;
+    const double fn = k[0] * std::pow(delta, 1.5) - k[1] * me * delta * vn;
+    const double ft = -std::min(k[2] * me * std::pow(delta, .5), k[3] * fn);
+    out[0] = fn * nx + ft * (vx - vn * nx);
+    out[1] = fn * ny + ft * (vy - vn * ny);
+    out[2] = (q[0] - r1[0]) * out[1] - (q[1] - r1[1]) * out[0];
+  }
+  return 0;
+}
+
+// The rods2D agentODE body (src/models/rods.jl:86-117) for agent i at stage
+// state S (x, y, vx, vy, theta, omega): du of the six DE variables, and
+// acc = {fx, fy, W, pressure}.  Returns -1 on a negative overlap.
+static int rod_deriv(const Orc& o, int64_t i, const std::vector<double>* S, double* du, double* acc) {
+  const double* p = o.s.p;
+  double fx = 0, fy = 0, W = 0, pressure = 0;
+  int bad = 0;
+  const double ri[8] = {S[0][i], S[1][i], o.rodD[i], o.rodL[i], S[4][i], S[2][i], S[3][i], o.mass[i]};
+  for_candidates(o, i, [&](int64_t j) {
+    const double rj[8] = {S[0][j], S[1][j], o.rodD[j], o.rodL[j], S[4][j], S[2][j], S[3][j], o.mass[j]};
+    double f[3];
+    if (rod_forces(ri, rj, p, f)) bad = -1;
+    fx += f[0];
+    fy += f[1];
+    W += f[2];
+    pressure += std::sqrt(f[0] * f[0] + f[1] * f[1]);
+  });
+  const double m = o.mass[i], d = o.rodD[i], l = o.rodL[i];
+  const double vx = S[2][i], vy = S[3][i], w = S[5][i];
+  du[0] = vx;
+  du[1] = vy;
+  du[2] = -p[5] * vx + fx / m;
+  du[3] = -p[5] * vy + fy / m;
+  du[4] = w;
+  du[5] = W / (m * ((d + l) * (d + l)) / 12 + m * (d * d)) - p[6] * w;
+  acc[0] = fx; acc[1] = fy; acc[2] = W; acc[3] = pressure;
+  return bad;
 }
 
 // The agentODE body evaluated on stage state S for agent i: returns the
@@ -797,16 +929,26 @@ static void compact(Orc& o, const std::vector<int64_t>& holes, int64_t nAdd) {
 static void de_step(Orc& o, int64_t nA) {
   const int d = o.dims;
   const double dt = o.m.dt;
-  // DE variables: positions (+ vol), or positions then velocities
-  const int nv = o.inertial ? 2 * d : d + (o.growth ? 1 : 0);
+  // DE variables: positions (+ vol), or positions then velocities, or the
+  // rods' x, y, vx, vy, theta, omega
+  const int nv = o.rods ? 6 : (o.inertial ? 2 * d : d + (o.growth ? 1 : 0));
   std::vector<double> U[6];
   for (int a = 0; a < d; ++a) U[a] = o.X[a];
-  if (o.inertial) for (int a = 0; a < d; ++a) U[d + a] = o.V[a];
+  if (o.inertial || o.rods) for (int a = 0; a < d; ++a) U[d + a] = o.V[a];
+  if (o.rods) { U[4] = o.TH; U[5] = o.OM; }
   if (o.growth) U[d] = o.vol;
+  std::vector<char> bad(o.rods ? nA : 0, 0);
   auto eval = [&](const std::vector<double>* S, std::vector<double>* K) {
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < nA; ++i) {
-      double du[6], acc[3];
+      double du[6], acc[4];
+      if (o.rods) {
+        // every evaluation leaves its fx, fy, W and pressure in the agent
+        if (rod_deriv(o, i, S, du, acc)) bad[i] = 1;
+        for (int v = 0; v < 6; ++v) K[v][i] = du[v];
+        o.Fnew[0][i] = acc[0]; o.Fnew[1][i] = acc[1]; o.rodW[i] = acc[2]; o.rodP[i] = acc[3];
+        continue;
+      }
       deriv(o, i, S, du, acc);
       for (int a = 0; a < d; ++a) { K[a][i] = du[a]; o.Fnew[a][i] = acc[a]; }
       if (o.inertial) for (int a = 0; a < d; ++a) K[d + a][i] = du[d + a];
@@ -889,7 +1031,15 @@ static void de_step(Orc& o, int64_t nA) {
     }
   }
   for (int a = 0; a < d; ++a) std::copy(U[a].begin(), U[a].begin() + nA, o.X[a].begin());
-  if (o.inertial) for (int a = 0; a < d; ++a) std::copy(U[d + a].begin(), U[d + a].begin() + nA, o.V[a].begin());
+  if (o.inertial || o.rods) for (int a = 0; a < d; ++a) std::copy(U[d + a].begin(), U[d + a].begin() + nA, o.V[a].begin());
+  if (o.rods) {
+    std::copy(U[4].begin(), U[4].begin() + nA, o.TH.begin());
+    std::copy(U[5].begin(), U[5].begin() + nA, o.OM.begin());
+    if (std::find(bad.begin(), bad.end(), 1) != bad.end()) {
+      o.err = CBM_ERR_NONFINITE;
+      o.msg = "rod contact with a negative overlap d_i - r_ij";
+    }
+  }
   if (o.growth) std::copy(U[d].begin(), U[d].begin() + nA, o.vol.begin());
 }
 
@@ -989,11 +1139,19 @@ void* orc_create(const cbm_model_desc* m, const cbm_sim_desc* s) {
   o->growth = (m->rule == CBM_RULE_GROWTH_DIVISION || m->rule == CBM_RULE_CUSTOM);
   o->inertial = (m->law == CBM_LAW_SOFT_SPHERES);
   if (o->inertial && m->rule != CBM_RULE_NONE) { o->err = CBM_ERR_CONFIG; o->msg = "soft spheres take no rules"; }
+  o->rods = (m->law == CBM_LAW_RODS);
+  if (o->rods && (m->dims != 2 || m->rule != CBM_RULE_NONE || m->periodic || m->lagged || m->medium_n[0] > 0 ||
+                  m->drift != 0.0 || !(m->integrator == CBM_EULER || m->integrator == CBM_HEUN || m->integrator == CBM_RK4))) {
+    o->err = CBM_ERR_CONFIG;
+    o->msg = "the rod model is 2-D, without rules, periodic box, lagged forces, medium or drift; Euler, Heun or RK4";
+  }
   for (int a = 0; a < 3; ++a) {
     o->X[a].assign(o->nMax, 0); o->F[a].assign(o->nMax, 0); o->Fnew[a].assign(o->nMax, 0); o->Xref[a].assign(o->nMax, 0);
     o->V[a].assign(o->nMax, 0);
   }
   o->rad.assign(o->nMax, 1.0); o->mass.assign(o->nMax, 1.0); o->fate.assign(o->nMax, 1);
+  o->TH.assign(o->nMax, 0.0); o->OM.assign(o->nMax, 0.0); o->rodD.assign(o->nMax, 1.0); o->rodL.assign(o->nMax, 1.0);
+  o->rodW.assign(o->nMax, 0.0); o->rodP.assign(o->nMax, 0.0);
   o->id.resize(o->nMax); o->lf.assign(o->nMax, 0); o->alive.assign(o->nMax, 1);
   for (int64_t i = 0; i < o->nMax; ++i) o->id[i] = i + 1;
   o->vol.assign(o->nMax, 1.0); o->gr.assign(o->nMax, 0.0); o->vb.assign(o->nMax, 1.0);
@@ -1024,6 +1182,16 @@ void* orc_create(const cbm_model_desc* m, const cbm_sim_desc* s) {
 
 void orc_destroy(void* h) { delete (Orc*)h; }
 
+// The rod geometry and contact force alone, for their pins: r = {x, y, l,
+// theta} per rod -> the two contact points; r = {x, y, d, l, theta, vx, vy,
+// m} per rod and k = {kn, gamma_n, gamma_t, mu_cc} -> {Fx, Fy, W}.
+void orc_rod_intersection(const double* r1, const double* r2, double* out4) {
+  rod_intersection(Rod{r1[0], r1[1], r1[2], r1[3]}, Rod{r2[0], r2[1], r2[2], r2[3]}, out4);
+}
+int orc_rod_forces(const double* r1, const double* r2, const double* k, double* out3) {
+  return rod_forces(r1, r2, k, out3);
+}
+
 int orc_set(void* h, int field, const void* data, int64_t n) {
   Orc& o = *(Orc*)h;
   if (n > o.nMax) return CBM_ERR_ARG;
@@ -1043,6 +1211,12 @@ int orc_set(void* h, int field, const void* data, int64_t n) {
       case CBM_F_MASS: o.mass[i] = v; break;
       case CBM_F_FATE: if (iv < 1 || iv > 2) return CBM_ERR_ARG; o.fate[i] = iv; break;
       case CBM_F_U0: case CBM_F_U1: case CBM_F_U2: case CBM_F_U3: o.U[field - CBM_F_U0][i] = v; break;
+      case CBM_F_THETA: o.TH[i] = v; break;
+      case CBM_F_OMEGA: o.OM[i] = v; break;
+      case CBM_F_DIAMETER: o.rodD[i] = v; break;
+      case CBM_F_LENGTH: o.rodL[i] = v; break;
+      case CBM_F_TORQUE: o.rodW[i] = v; break;
+      case CBM_F_PRESSURE: o.rodP[i] = v; break;
       default: return CBM_ERR_ARG;
     }
   }
@@ -1068,6 +1242,12 @@ int orc_get(void* h, int field, void* out, int64_t n) {
       case CBM_F_MASS: dv[i] = o.mass[i]; break;
       case CBM_F_FATE: iv[i] = o.fate[i]; break;
       case CBM_F_U0: case CBM_F_U1: case CBM_F_U2: case CBM_F_U3: dv[i] = o.U[field - CBM_F_U0][i]; break;
+      case CBM_F_THETA: dv[i] = o.TH[i]; break;
+      case CBM_F_OMEGA: dv[i] = o.OM[i]; break;
+      case CBM_F_DIAMETER: dv[i] = o.rodD[i]; break;
+      case CBM_F_LENGTH: dv[i] = o.rodL[i]; break;
+      case CBM_F_TORQUE: dv[i] = o.rodW[i]; break;
+      case CBM_F_PRESSURE: dv[i] = o.rodP[i]; break;
       default: return CBM_ERR_ARG;
     }
   }

@@ -57,6 +57,8 @@ def lib():
             "orc_set_medium_assign": (None, [P, P]),
             "orc_traj_loss": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P]),
             "orc_traj_grad_dual": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P, P]),
+            "orc_rod_intersection": (None, [P, P, P]),
+            "orc_rod_forces": (C.c_int, [P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(h, name)
@@ -343,3 +345,23 @@ def threads():
 
 def set_threads(n):
     lib().orc_set_threads(int(n))
+
+
+def rod_intersection(r1, r2):
+    """Contact points of two rods r = (x, y, l, theta): (x1, y1, x2, y2)."""
+    a = np.ascontiguousarray(r1, dtype=np.float64)
+    b = np.ascontiguousarray(r2, dtype=np.float64)
+    o = np.zeros(4)
+    lib().orc_rod_intersection(a.ctypes.data, b.ctypes.data, o.ctypes.data)
+    return o
+
+
+def rod_forces(r1, r2, k):
+    """rodForces on rod 1 from rod 2, r = (x, y, d, l, theta, vx, vy, m) and
+    k = (kn, gamma_n, gamma_t, mu_cc): ((Fx, Fy, W), status)."""
+    a = np.ascontiguousarray(r1, dtype=np.float64)
+    b = np.ascontiguousarray(r2, dtype=np.float64)
+    kk = np.ascontiguousarray(k, dtype=np.float64)
+    o = np.zeros(3)
+    rc = lib().orc_rod_forces(a.ctypes.data, b.ctypes.data, kk.ctypes.data, o.ctypes.data)
+    return o, rc
