@@ -1026,6 +1026,8 @@ static StageArgs stage_base(cbm_ctx* c) {
   a.nvar = c->nvar;
   a.write_fnew = c->lagged ? 1 : 0;
   if (c->growth) a.growth = (const double*)c->col[C_GR].cur();
+  if (c->has_col[C_U0])
+    for (int k = 0; k < 4; ++k) a.upar[k] = (const double*)c->col[C_U0 + k].cur();
   for (int v = 0; v < MAX_STATE; ++v) a.gacc[v] = c->gacc[v];
   if (c->inertial) {
     a.rad = (const double*)c->col[C_RAD].cur();
@@ -1513,7 +1515,8 @@ static std::string user_source(const cbm_model_desc& m, int nb_kind, bool growth
   src += "#include \"cbm_rule.h\"\n#include \"cbm_medium.h\"\nnamespace cbm {\n";
   if (law_body) { src += law_body; src += "\n"; }
   if (law_body && m.user_de > 0)
-    src += "static_assert(has_de<UserLaw>::value, \"user_de > 0 needs UserLaw::de\");\n";
+    src += "static_assert(has_de<UserLaw>::value || general_body<UserLaw>::value, "
+           "\"user_de > 0 needs UserLaw::de, or UserLaw::pair and ode\");\n";
   if (noise_body) { src += noise_body; src += "\n"; }
   if (rule_body) { src += rule_body; src += "\n"; }
   if (medium_body) { src += medium_body; src += "\n"; }
@@ -1527,7 +1530,9 @@ static std::string user_source(const cbm_model_desc& m, int nb_kind, bool growth
            "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_rule_apply(cbm::RuleArgs a) {\n"
            "  cbm::user_rule_apply<cbm::UserRule>(a);\n}\n";
   char line[256];
-  if (law_body) src += "using cbm_law_t = cbm::UserLaw;\n";
+  if (law_body)
+    src += "using cbm_law_t = cbm::UserLaw;\n"
+           "extern \"C\" __device__ int cbm_law_general = cbm::general_body<cbm_law_t>::value ? 1 : 0;\n";
   else { std::snprintf(line, sizeof(line), "using cbm_law_t = cbm::PairLaw<%d>;\n", m.law); src += line; }
   src += noise_body ? "using cbm_noise_t = cbm::UserNoise;\n" : "using cbm_noise_t = cbm::ConstNoise;\n";
   // the same skeletons as the built-in laws: no neighbor walk without a law
@@ -1538,14 +1543,14 @@ static std::string user_source(const cbm_model_desc& m, int nb_kind, bool growth
   for (int noise = 0; stages && noise < 2; ++noise) {
     std::snprintf(inst, sizeof(inst),
                   "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_stage_%d(cbm::StageArgs a) {\n"
-                  "  cbm::stage_body<%d, %d, cbm_law_t, %d, %d, cbm_noise_t>(a);\n}\n",
+                  "  cbm::user_stage<%d, %d, cbm_law_t, %d, %d, cbm_noise_t>(a);\n}\n",
                   noise, dims, nb, per, noise);
     src += inst;
     if (nb == NB_CELL) {   // the brick-tiled cell-linked stage, run before the row kernel
       std::snprintf(inst, sizeof(inst),
                     "extern \"C\" __global__ __launch_bounds__(256) void cbm_user_brick_%d(cbm::StageArgs a) {\n"
                     "  extern __shared__ unsigned char lds[];\n"
-                    "  cbm::brick_body<%d, cbm_law_t, %d, %d, cbm_noise_t>(a, lds);\n}\n",
+                    "  cbm::user_brick<%d, cbm_law_t, %d, %d, cbm_noise_t>(a, lds);\n}\n",
                     noise, dims, per, noise);
       src += inst;
     }
@@ -1627,6 +1632,20 @@ int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, 
   for (int k = 0; k < 2; ++k) { c->user_fn[k] = nullptr; c->user_brick[k] = nullptr; c->user_rule[k] = nullptr; }
   c->user_medium = nullptr;
   HIPCHK(c, hipModuleLoadData(&c->user_mod, code.data()));
+  if (law_body && c->lagged) {
+    // a general agentODE body (UserLaw::ode) writes its own equations: there
+    // is no force accumulator to lag
+    hipDeviceptr_t gp = nullptr;
+    size_t gb = 0;
+    int general = 0;
+    if (hipModuleGetGlobal(&gp, &gb, c->user_mod, "cbm_law_general") == hipSuccess && gb == sizeof(int))
+      HIPCHK(c, hipMemcpyDtoH(&general, gp, sizeof(int)));
+    if (general) {
+      (void)hipModuleUnload(c->user_mod);
+      c->user_mod = nullptr;
+      return fail(c, CBM_ERR_CONFIG, "a general agentODE body (UserLaw::ode) takes no lagged forces");
+    }
+  }
   if (medium_body) HIPCHK(c, hipModuleGetFunction(&c->user_medium, c->user_mod, "cbm_user_medium_step"));
   if (rule_body) {
     HIPCHK(c, hipModuleGetFunction(&c->user_rule[0], c->user_mod, "cbm_user_rule_decide"));

@@ -122,6 +122,51 @@ template <class L> struct has_de<L, decltype((void)&L::de)> { static constexpr b
 #define CBM_DE_BASE 0
 #endif
 
+// A general agentODE body (src/AgentStructure/functionDE.jl:13-116): a
+// UserLaw with
+//   static constexpr int sums = K;     // neighbor sums, 1..8
+//   __device__ void pair(const LawAgent& i, const LawAgent& j, const double* dx, double d, double* s) const;
+//   __device__ void ode(const LawAgent& i, const double* s, double* dxdt, double* dudt) const;
+// The skeleton zeroes s[0..K), calls pair for every neighbor candidate j of
+// agent i in reference loop order (dx = x_i - x_j, minimum image in a
+// periodic box; d = |dx|), each call adding its terms to s in whatever order
+// the body writes, and then ode with the sums: dxdt[0..D) = dt of the
+// positions, dudt[k] = dt of user DE variable k (both zeroed before the call).
+// The agents are seen at the stage state.
+template <class L, class = void> struct general_body { static constexpr bool value = false; };
+template <class L> struct general_body<L, decltype((void)&L::ode)> { static constexpr bool value = true; };
+template <class L, class = void> struct law_sums { static constexpr int value = 1; };
+template <class L> struct law_sums<L, decltype((void)L::sums)> { static constexpr int value = L::sums; };
+// the Aggregation wall push, for laws that declare it
+template <class L, class = void> struct law_walls { static constexpr bool value = false; };
+template <class L> struct law_walls<L, decltype((void)L::walls)> {
+  static constexpr bool value = L::active && L::walls;
+};
+
+// One agent of a pair as a general agentODE body sees it.
+struct LawAgent {
+  int64_t row, id;
+  double x[3];              // position at the stage state
+  double uv[4];             // user parameters U0..U3: DE variables at the stage state, others as stored
+  double volv;              // vol at the stage state (1 without the vol variable)
+  __device__ double u(int k) const { return uv[k]; }
+  __device__ double vol() const { return volv; }
+};
+
+template <int D, class Args>
+__device__ __forceinline__ LawAgent law_agent(const Args& a, int64_t row, const double* x) {
+  LawAgent r;
+  r.row = row;
+  r.id = a.id[row];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) r.x[c] = c < D ? x[c] : 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    r.uv[k] = k < CBM_USER_DE ? a.S[CBM_DE_BASE + k][row] : (a.upar[k] ? a.upar[k][row] : 0.0);
+  r.volv = a.growth ? a.S[D][row] : 1.0;
+  return r;
+}
+
 // Walk the candidate neighbors of row i in reference loop order (Args: the
 // stage or rule arguments, with the list and cell tables of the last
 // neighbor update).
@@ -294,7 +339,7 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
     return a.U[c][i];
   };
   auto Sc = [&](int c) { return c < D ? xi[c] : a.S[c][i]; };
-  if (Law::active && Law::walls && a.walls && sd.p[4] != 0.0) {   // p[4]: the walls switch
+  if (law_walls<Law>::value && a.walls && sd.p[4] != 0.0) {   // p[4]: the walls switch
     const double rRep = sd.p[0], fRep = sd.p[1];
 #pragma unroll
     for (int c = 0; c < D; ++c) {
@@ -305,10 +350,24 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
   }
 
   // derivative of each DE variable
-  constexpr bool UDE = CBM_USER_DE > 0 && has_de<Law>::value;
+  constexpr bool PAIRB = general_body<Law>::value;
+  constexpr bool UDE = CBM_USER_DE > 0 && (has_de<Law>::value || PAIRB);
   constexpr int NS = UDE ? MAX_STATE : BUILTIN_STATE;
   double du[NS];
-  if constexpr (RODS) {
+  double dudt[4] = {0.0, 0.0, 0.0, 0.0};
+  if constexpr (PAIRB) {
+    // a general agentODE body: its equations from the neighbor sums acc
+    const LawAgent me = law_agent<D>(a, i, xi);
+    double dxdt[3] = {0.0, 0.0, 0.0};
+    const Law law(sd.p);
+    law.ode(me, acc, dxdt, dudt);
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      double v = dxdt[c];
+      if (a.drift != 0.0) v = v - a.drift * xi[c];
+      du[c] = v;
+    }
+  } else if constexpr (RODS) {
     // acc = {fx, fy, W, pressure}; the DE variables x, y, vx, vy, theta,
     // omega (src/models/rods.jl:111-116)
     const double beta = sd.p[5], bw = sd.p[6];
@@ -355,15 +414,20 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
   if (grow) du[D] = a.growth[i] * a.S[D][i];
   if constexpr (UDE) {
     // dt of the user DE variables at the stage state
-    double xs[3] = {0.0, 0.0, 0.0}, us[4] = {0.0, 0.0, 0.0, 0.0}, ds[4] = {0.0, 0.0, 0.0, 0.0};
+    if constexpr (PAIRB) {
 #pragma unroll
-    for (int c = 0; c < D; ++c) xs[c] = xi[c];
+      for (int k = 0; k < CBM_USER_DE; ++k) du[CBM_DE_BASE + k] = dudt[k];
+    } else {
+      double xs[3] = {0.0, 0.0, 0.0}, us[4] = {0.0, 0.0, 0.0, 0.0}, ds[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < CBM_USER_DE; ++k) us[k] = a.S[CBM_DE_BASE + k][i];
-    const Law law(sd.p);
-    law.de(xs, us, ds);
+      for (int c = 0; c < D; ++c) xs[c] = xi[c];
 #pragma unroll
-    for (int k = 0; k < CBM_USER_DE; ++k) du[CBM_DE_BASE + k] = ds[k];
+      for (int k = 0; k < CBM_USER_DE; ++k) us[k] = a.S[CBM_DE_BASE + k][i];
+      const Law law(sd.p);
+      law.de(xs, us, ds);
+#pragma unroll
+      for (int k = 0; k < CBM_USER_DE; ++k) du[CBM_DE_BASE + k] = ds[k];
+    }
   }
 
   const double dt = a.dt;
@@ -593,6 +657,55 @@ __device__ __forceinline__ void rod_stage_body(const StageArgs& a) {
   if (domain) *a.rod_err = 1;
   double xi[3] = {me.a.x, me.a.y, 0.0};
   stage_finish<2, RodLaw, 0>(a, sd, ri.sim, i, xi, acc, xi);
+}
+
+// One integrator stage of a general agentODE body (UserLaw::ode) for every agent
+// row: the body's neighbor sums over the candidates in reference loop order
+// (Full, the cell-linked walk or the Verlet list), then its equations.
+template <int D, int NB, class Law, int PER, int NOISE, class Noise>
+__device__ __forceinline__ void pair_stage_body(const StageArgs& a) {
+  const RowIdx ri = row_index(a.g);
+  const SimDev& sd = a.g.sims[ri.sim];
+  const int64_t N = a.g.st[ri.sim].N;
+  const int64_t i = ri.row;
+  if (ri.local >= N) return;
+  double xi[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < D; ++c) xi[c] = a.S[c][i];
+  const LawAgent me = law_agent<D>(a, i, xi);
+  constexpr int K = law_sums<Law>::value;
+  static_assert(K >= 1 && K <= 8, "UserLaw::sums is 1..8");
+  double s[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) s[k] = 0.0;
+  double L[3], hL[3];
+  box_halves<D, PER>(sd, L, hL);
+  const Law law(sd.p);
+  for_candidates<D, NB>(a, sd, N, i, [&](int64_t j) {
+    double xj[3] = {0.0, 0.0, 0.0}, dx[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      xj[c] = a.S[c][j];
+      dx[c] = xi[c] - xj[c];
+      if (PER) dx[c] = wrap_dx(dx[c], L[c], hL[c]);
+    }
+    double s2 = dx[0] * dx[0];
+    if (D > 1) s2 = s2 + dx[1] * dx[1];
+    if (D > 2) s2 = s2 + dx[2] * dx[2];
+    const LawAgent nb = law_agent<D>(a, j, xj);
+    law.pair(me, nb, dx, sqrt(s2), s);
+  });
+  stage_finish<D, Law, NOISE, false, Noise>(a, sd, ri.sim, i, xi, s, xi);
+}
+
+// The stage and brick kernels of a hipRTC module: a general agentODE body
+// walks its candidates row by row (no brick tiles: its brick kernel returns
+// at once and the row kernel takes every row); a pair law takes the
+// built-in skeletons.
+template <int D, int NB, class Law, int PER, int NOISE, class Noise>
+__device__ __forceinline__ void user_stage(const StageArgs& a) {
+  if constexpr (general_body<Law>::value) pair_stage_body<D, NB, Law, PER, NOISE, Noise>(a);
+  else stage_body<D, NB, Law, PER, NOISE, Noise>(a);
 }
 
 // ------------------------------------------------ flattened list stage
@@ -1075,6 +1188,13 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
     }
     stage_finish<D, Law, NOISE, true, Noise>(a, sd, sim, i, xi, acc, upre, disp2);
   }
+}
+
+// The brick kernel of a hipRTC module: nothing for a general agentODE body
+// (its row kernel walks every row).
+template <int D, class Law, int PER, int NOISE, class Noise>
+__device__ __forceinline__ void user_brick(const StageArgs& a, unsigned char* lds) {
+  if constexpr (!general_body<Law>::value) brick_body<D, Law, PER, NOISE, Noise>(a, lds);
 }
 
 // An element at a 32-bit byte offset from a base that all lanes share (a

@@ -169,6 +169,7 @@ struct StageArgs {
   const double* rod_l;
   double* rod_out[4];           // rods: fx, fy, W, pressure of this stage
   int32_t* rod_err;             // rods: set when a contact's overlap d_i - r_ij is negative
+  const double* upar[4];        // user agent parameter columns U0..U3 (null when absent)
   const int64_t* id;
   const int32_t* list;          // ELL Verlet list, entry q of row i at q*ldl + i
   const int32_t* list_n;

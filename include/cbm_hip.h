@@ -268,8 +268,24 @@ const char* cbm_version(void);
  *     __device__ void de(const double* x, const double* u, double* du) const;
  * dt of the user DE variables: du[k] (zeroed) = dt(U_k) at the stage
  * positions x and stage values u[k] of U_k, k < user_de, integrated by the
- * model's integrator beside the positions (one normal each under EM).  Compile errors return CBM_ERR_COMPILE with the
- * hipRTC log in cbm_last_error. */
+ * model's integrator beside the positions (one normal each under EM).
+ * A general agentODE body (src/AgentStructure/functionDE.jl:13-116: neighbor
+ * loops that read the neighbors' parameters and sum into any variable)
+ * defines instead of operator():
+ *     static constexpr int sums = K;                     // 1..8
+ *     __device__ void pair(const LawAgent& i, const LawAgent& j,
+ *                          const double* dx, double d, double* s) const;
+ *     __device__ void ode(const LawAgent& i, const double* s,
+ *                         double* dxdt, double* dudt) const;
+ * pair is called for every neighbor candidate j of agent i in the reference
+ * loop order (dx = x_i - x_j, minimum image in a periodic box, d = |dx|)
+ * and adds its terms to the sums s (zeroed first) in the order it writes
+ * them; ode then gives dt of the positions (dxdt) and of the user DE
+ * variables (dudt) from the sums.  A LawAgent holds row, id, x[3] and
+ * u(k), vol(): U_k and vol at the stage state for DE variables, as stored
+ * otherwise.  Such a body walks its candidates row by row (no brick
+ * tiles) and takes no lagged forces.  Compile errors return
+ * CBM_ERR_COMPILE with the hipRTC log in cbm_last_error. */
 int cbm_compile(cbm_ctx* ctx, const char* pair_body_hip);
 
 /* Compile user bodies as cbm_compile_model would for a model described by

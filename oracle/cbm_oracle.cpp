@@ -159,6 +159,9 @@ struct Orc {
   std::vector<double> ruleX[3], ruleVol, ruleU[4];   // the rule's neighbor view: values at the step's start
   void* medium_fn = nullptr;                  // user mediumODE body (orc_medium_fn), else built-in
   void* medium_assign = nullptr;              // its assignment branches (orc_medium_assign_fn), optional
+  void* law_pair = nullptr;                   // a general agentODE body (orc_law_pair_fn, orc_law_ode_fn)
+  void* law_ode = nullptr;
+  int law_sums = 0;
   // neighbor structures
   Grid g;
   std::vector<int64_t> cellOf, cellStart, cellCount, cellAgents;
@@ -328,6 +331,53 @@ static inline bool soft_s(const double* p, double d, double ri, double rj, int64
   const double f0 = p[2 + 2 * (fi - 1) + (fj - 1)];
   s = f0 * (rij / d - 1) * (p[1] * rij / d - 1);
   return true;
+}
+
+// ------------------------------------------------- general agentODE bodies
+// An agent as a general agentODE body sees it (functionDE.jl:13-116): the
+// stage state of its position, of the vol variable and of the user DE
+// variables, and the other user parameters as stored.
+struct OrcLawAgent {
+  int64_t row, id;
+  double x[3];
+  double u[4];
+  double vol;
+};
+typedef void (*orc_law_pair_fn)(const OrcLawAgent* i, const OrcLawAgent* j, const double* dx, double d, double* s);
+typedef void (*orc_law_ode_fn)(const OrcLawAgent* i, const double* s, double* dxdt, double* dudt);
+
+// The body's derivative of agent i at stage state S: the sums over the
+// neighbor candidates in loop order, then its equations; du holds the
+// positions' and the user DE variables' derivatives at their DE indices
+// (the caller adds vol's).
+static void body_deriv(const Orc& o, int64_t i, const std::vector<double>* S, double* du) {
+  const int d = o.dims;
+  const int base = d + (o.growth ? 1 : 0);
+  auto agent = [&](int64_t j) {
+    OrcLawAgent r;
+    r.row = j;
+    r.id = o.id[j];
+    for (int a = 0; a < 3; ++a) r.x[a] = a < d ? S[a][j] : 0.0;
+    for (int k = 0; k < 4; ++k) r.u[k] = k < o.m.user_de ? S[base + k][j] : o.U[k][j];
+    r.vol = o.growth ? S[d][j] : 1.0;
+    return r;
+  };
+  const OrcLawAgent me = agent(i);
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for_candidates(o, i, [&](int64_t j) {
+    const OrcLawAgent nb = agent(j);
+    double dx[3] = {0, 0, 0};
+    for (int a = 0; a < d; ++a) dx[a] = o.mi(a, S[a][i] - S[a][j]);
+    ((orc_law_pair_fn)o.law_pair)(&me, &nb, dx, dist(o, dx), s);
+  });
+  double dxdt[3] = {0, 0, 0}, dudt[4] = {0, 0, 0, 0};
+  ((orc_law_ode_fn)o.law_ode)(&me, s, dxdt, dudt);
+  for (int a = 0; a < d; ++a) {
+    double v = dxdt[a];
+    if (o.m.drift != 0.0) v = v - o.m.drift * S[a][i];
+    du[a] = v;
+  }
+  for (int k = 0; k < o.m.user_de; ++k) du[base + k] = dudt[k];
 }
 
 // ------------------------------------------------------------------ rods
@@ -931,17 +981,28 @@ static void de_step(Orc& o, int64_t nA) {
   const double dt = o.m.dt;
   // DE variables: positions (+ vol), or positions then velocities, or the
   // rods' x, y, vx, vy, theta, omega
-  const int nv = o.rods ? 6 : (o.inertial ? 2 * d : d + (o.growth ? 1 : 0));
-  std::vector<double> U[6];
+  const bool body = o.law_ode != nullptr;
+  const int ude = body ? o.m.user_de : 0;
+  const int base = d + (o.growth ? 1 : 0);
+  const int nv = o.rods ? 6 : (o.inertial ? 2 * d : base + ude);
+  std::vector<double> U[8];
   for (int a = 0; a < d; ++a) U[a] = o.X[a];
   if (o.inertial || o.rods) for (int a = 0; a < d; ++a) U[d + a] = o.V[a];
   if (o.rods) { U[4] = o.TH; U[5] = o.OM; }
   if (o.growth) U[d] = o.vol;
+  for (int k = 0; k < ude; ++k) U[base + k] = o.U[k];
   std::vector<char> bad(o.rods ? nA : 0, 0);
   auto eval = [&](const std::vector<double>* S, std::vector<double>* K) {
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < nA; ++i) {
-      double du[6], acc[4];
+      double du[8], acc[4];
+      if (body) {
+        body_deriv(o, i, S, du);
+        for (int a = 0; a < d; ++a) K[a][i] = du[a];
+        for (int k = 0; k < ude; ++k) K[base + k][i] = du[base + k];
+        if (o.growth) K[d][i] = o.gr[i] * S[d][i];
+        continue;
+      }
       if (o.rods) {
         // every evaluation leaves its fx, fy, W and pressure in the agent
         if (rod_deriv(o, i, S, du, acc)) bad[i] = 1;
@@ -955,7 +1016,7 @@ static void de_step(Orc& o, int64_t nA) {
       if (o.growth) K[d][i] = o.gr[i] * S[d][i];
     }
   };
-  std::vector<double> K1[6], K2[6], K3[6], K4[6], H[6];
+  std::vector<double> K1[8], K2[8], K3[8], K4[8], H[8];
   for (int v = 0; v < nv; ++v) { K1[v].assign(nA, 0); H[v] = U[v]; }
   int integ = o.m.integrator;
   if (integ == CBM_EULER) {
@@ -1041,6 +1102,7 @@ static void de_step(Orc& o, int64_t nA) {
     }
   }
   if (o.growth) std::copy(U[d].begin(), U[d].begin() + nA, o.vol.begin());
+  for (int k = 0; k < ude; ++k) std::copy(U[base + k].begin(), U[base + k].begin() + nA, o.U[k].begin());
 }
 
 static void wrap_positions(Orc& o, int64_t n) {
@@ -1350,6 +1412,19 @@ double orc_rule_value(void* h, int field, int64_t j) {
 void orc_set_medium_body(void* h, void* fn) { ((Orc*)h)->medium_fn = fn; }
 // Its assignment branches (orc_medium_assign_fn; NULL: none).
 void orc_set_medium_assign(void* h, void* fn) { ((Orc*)h)->medium_assign = fn; }
+
+// A general agentODE body: pair (orc_law_pair_fn) adds to `sums` accumulators
+// per neighbor, ode (orc_law_ode_fn) gives dt of the positions and of the
+// model's user_de user variables (U0..).  Returns CBM_ERR_ARG for sums
+// outside 1..8.
+int orc_set_law_body(void* h, void* pair, void* ode, int sums) {
+  if (sums < 1 || sums > 8) return CBM_ERR_ARG;
+  Orc& o = *(Orc*)h;
+  o.law_pair = pair;
+  o.law_ode = ode;
+  o.law_sums = sums;
+  return 0;
+}
 
 void orc_normals(uint64_t seed, int64_t step, int64_t id, int32_t sim, uint32_t purpose, double* z2) {
   U4 r = draw(seed, step, id, sim, purpose);

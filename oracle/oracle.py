@@ -58,6 +58,7 @@ def lib():
             "orc_traj_loss": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P]),
             "orc_traj_grad_dual": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P, P]),
             "orc_rod_intersection": (None, [P, P, P]),
+            "orc_set_law_body": (C.c_int, [P, P, P, C.c_int]),
             "orc_rod_forces": (C.c_int, [P, P, P, P]),
         }
         for name, (res, args) in sig.items():
@@ -75,6 +76,19 @@ class RuleRec(C.Structure):
     """One agent's values as a CBM_RULE_CUSTOM body sees them (OrcRuleRec)."""
     _fields_ = [("x", C.c_double * 3), ("vol", C.c_double), ("growth", C.c_double), ("vbirth", C.c_double),
                 ("u", C.c_double * 4), ("lf", C.c_int64), ("medium", C.c_double), ("madd", C.c_double)]
+
+
+class LawAgent(C.Structure):
+    """One agent as a general agentODE body sees it (OrcLawAgent): stage-state
+    position, user parameters u[0..4) (DE variables at the stage state), vol."""
+    _fields_ = [("row", C.c_int64), ("id", C.c_int64), ("x", C.c_double * 3), ("u", C.c_double * 4),
+                ("vol", C.c_double)]
+
+
+LAW_PAIR_FN = C.CFUNCTYPE(None, C.POINTER(LawAgent), C.POINTER(LawAgent), C.POINTER(C.c_double), C.c_double,
+                          C.POINTER(C.c_double))
+LAW_ODE_FN = C.CFUNCTYPE(None, C.POINTER(LawAgent), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                         C.POINTER(C.c_double))
 
 
 RULE_FN = C.CFUNCTYPE(None, C.c_int64, C.c_int64, C.POINTER(RuleRec), C.POINTER(RuleRec), C.POINTER(RuleRec),
@@ -243,6 +257,16 @@ class Oracle:
 
         self._rule_cb = RULE_FN(cb)
         self.lib.orc_set_rule(self.h, C.cast(self._rule_cb, C.c_void_p))
+
+    def set_law_body(self, pair, ode, sums):
+        """A general agentODE body: pair(i, j, dx, d, s) adds neighbor j's terms
+        to the sums s[0..sums); ode(i, s, dxdt, dudt) writes dt of the
+        positions and of the user DE variables (LawAgent records)."""
+        self._law_cb = (LAW_PAIR_FN(lambda i, j, dx, d, s: pair(i.contents, j.contents, dx, d, s)),
+                        LAW_ODE_FN(lambda i, s, dxdt, dudt: ode(i.contents, s, dxdt, dudt)))
+        rc = self.lib.orc_set_law_body(self.h, C.cast(self._law_cb[0], C.c_void_p),
+                                       C.cast(self._law_cb[1], C.c_void_p), int(sums))
+        assert rc == 0, rc
 
     def set_medium_body(self, body, assign=None):
         """A user mediumODE body: body(node) returns dt(c) at a MediumNode;

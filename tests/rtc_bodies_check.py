@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 for p in (HERE, ROOT, os.path.join(ROOT, "cellbasedmodels.jl_amd")):
     sys.path.insert(0, p)
 
-MODULES = ["medium_bodies", "rule_bodies", "test_api_host", "test_gpu_custom_law", "test_gpu_law_self",
+MODULES = ["law_bodies", "medium_bodies", "rule_bodies", "test_api_host", "test_gpu_custom_law", "test_gpu_law_self",
            "test_gpu_sde_body", "test_gpu_user_de", "test_gpu_user_de_rules", "test_gpu_user_rule"]
 
 
@@ -43,7 +43,7 @@ def check(A, label, body):
         args[1] = body
     else:
         m = model(3, neighbors=A.CELL_LINKED, law=A.LAW_CUSTOM, cell_edge=2.0)
-        m.user_de = 1 if " de(" in body else 0
+        m.user_de = 4 if " ode(" in body else (1 if " de(" in body else 0)
         args[0] = body
     enc = [a.encode() if a is not None else None for a in args]
     print(label, flush=True)
