@@ -96,6 +96,9 @@ SIGNATURES = {
     "cbm_compile_check": (C.c_int, [_P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, _P, C.c_int64]),
     "cbm_adjoint": (C.c_int, [_P, C.POINTER(LossDesc), _P, _P]),
     "cbm_set_params": (C.c_int, [_P, C.c_int32, _P, C.c_int32]),
+    "cbm_get_params": (C.c_int, [_P, C.c_int32, _P, C.c_int32]),
+    "cbm_compile_model_scope": (C.c_int, [_P, C.c_char_p]),
+    "cbm_compile_model_scope_check": (C.c_int, [C.c_char_p, _P, C.c_int64]),
     "cbm_autotune": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int32)]),
     "cbm_set_block": (C.c_int, [_P, C.c_int32]),
     "cbm_profile": (C.c_int, [_P, C.c_int64, _P, _P, C.c_int32]),

@@ -221,14 +221,19 @@ class ABM:
     lagged      dt(x) reads the previous step's committed force accumulator, the
                 reference's read-old/write-new idiom (agentStructure.jl:352-453)
     periodic    periodic box with minimum-image distances
+    modelScope  a HIP C++ body defining struct UserModel: the modelRule,
+                modelODE and modelSDE of the reference over the simulation's
+                parameters p[8] (cbm_compile_model_scope), run once per step
+                after the medium; com.ctx.params() reads them back
     """
 
     def __init__(self, dims, agentODE=None, agentSDE=0.0, agentRule=None, drift=0.0,
                  neighborsAlg=None, agentAlg=None, platform=None, periodic=False,
-                 lagged=False, seed=0, medium=None, agent=None, deVariables=()):
+                 lagged=False, seed=0, medium=None, agent=None, deVariables=(), modelScope=None):
         if dims not in (1, 2, 3):
             raise ValueError("dims must be 1, 2 or 3")
         self.dims = dims
+        self.modelScope = modelScope
         self.agentODE = agentODE if agentODE is not None else CBMModels.noInteraction()
         self.agentSDE = agentSDE if isinstance(agentSDE, str) else float(agentSDE)
         if isinstance(agentRule, str):
@@ -478,6 +483,17 @@ class Context:
         v = np.ascontiguousarray(p, dtype=np.float64)
         self._chk(self.lib.cbm_set_params(self.h, sim, v.ctypes.data, v.size))
 
+    def params(self, sim=0, n=8):
+        """The model-scope parameters p[0..n) as the model-scope body has left them."""
+        v = np.zeros(int(n))
+        self._chk(self.lib.cbm_get_params(self.h, sim, v.ctypes.data, v.size))
+        return v
+
+    def compile_model_scope(self, body):
+        """A modelRule / modelODE / modelSDE body (struct UserModel), run once per
+        simulation and step after the medium step (cbm_compile_model_scope)."""
+        self._chk(self.lib.cbm_compile_model_scope(self.h, body.encode()))
+
     def adjoint(self, n_steps, checkpoint_every, save_steps, target):
         """Loss and d(loss)/d(k, r_eq, eps) per simulation (cbm_adjoint).
 
@@ -597,6 +613,8 @@ def loadToPlatform_(com, preallocateAgents=0):
     medium_src = abm.medium.get("mediumODE") if abm.medium is not None else None
     if law_src is not None or noise_src is not None or rule_src is not None or medium_src is not None:
         com.ctx.compile_model(law_src, noise_src, rule_src, medium_src)
+    if abm.modelScope is not None:
+        com.ctx.compile_model_scope(abm.modelScope)
     for k, arr in com.params.items():
         com.ctx.upload(com.fields[k], arr)
     com.ctx.neighbors()

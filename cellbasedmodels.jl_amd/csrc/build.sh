@@ -8,7 +8,7 @@ HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
 # the stage- and rule-kernel headers, embedded for hipRTC compilation of user bodies
 INC="$OUT/cbm_rtc_src.inc"
 {
-  for h in cbm_device.h cbm_structs.h cbm_stage.h cbm_rule.h cbm_medium.h; do
+  for h in cbm_device.h cbm_structs.h cbm_stage.h cbm_rule.h cbm_medium.h cbm_model.h; do
     v=$(echo "$h" | tr '.' '_')
     printf 'static const char* cbm_rtc_%s = R"CBMRTC(' "$v"
     cat "$HERE/$h"

@@ -61,6 +61,10 @@ struct cbm_ctx {
   bool growth = false, lagged = false, verlet = false, cl = false, full = false, vtime = false, rules = false;
   bool inertial = false;                  // CBM_LAW_SOFT_SPHERES: velocities are DE variables
   bool rods = false;                      // CBM_LAW_RODS: x, y, vx, vy, theta, omega are DE variables
+  // model scope (modelStepRule!/modelStepDE!): a hipRTC module of its own
+  hipModule_t model_mod = nullptr;
+  hipFunction_t model_fn = nullptr;
+  double* model_t = nullptr;              // the step's start time by step parity (device)
   // CellLinked executed on a skin list (cl && clv): no per-step cell sort;
   // each stage orders an agent's list entries by its cell among the 3^D
   // neighbor cells (the reference loop order) from the agents' packed cells
@@ -613,6 +617,8 @@ static void destroy_ctx(cbm_ctx* c) {
   F(c->med_add);
   if (c->rec_host) (void)hipHostFree(c->rec_host);
   if (c->user_mod) (void)hipModuleUnload(c->user_mod);
+  if (c->model_mod) (void)hipModuleUnload(c->model_mod);
+  if (c->model_t) (void)hipFree(c->model_t);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1295,6 +1301,24 @@ static void clv_schedule(cbm_ctx* c) {
   if (r->step > c->last_build_step && r->soon != 0ull) c->build_now = true;
 }
 
+// modelStepRule! and modelStepDE! of every simulation (one thread each),
+// committing the parameters the next step reads.
+static int model_phase(cbm_ctx* c) {
+  ModelArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.sims = c->d_sims;
+  a.st = c->d_st;
+  a.step_ptr = c->counters + 1;
+  a.t_buf = c->model_t;
+  a.dt = c->m.dt;
+  a.seed = c->m.seed;
+  a.S = c->S;
+  void* params[] = {(void*)&a};
+  HIPCHK(c, hipModuleLaunchKernel(c->model_fn, (unsigned)((c->S + 63) / 64), 1, 1, 64, 1, 1, 0, c->stream, params,
+                                  nullptr));
+  return CBM_OK;
+}
+
 static int step_once(cbm_ctx* c) {
   int rc;
   if (!c->structure_valid && (rc = force_rebuild(c))) return rc;
@@ -1390,6 +1414,7 @@ static int step_once(cbm_ctx* c) {
     commit(c->col[C_VB], c->vb_new);
   }
   if (c->medium && (rc = medium_phase(c))) return rc;   // mediumStepDE!
+  if (c->model_fn && (rc = model_phase(c))) return rc;   // modelStepRule!, modelStepDE!
   if (c->rules && (rc = compact_phase(c, c->flags + (s & 1)))) return rc;
   c->t = c->t + c->m.dt;
   c->step++;
@@ -1561,10 +1586,10 @@ static std::string user_source(const cbm_model_desc& m, int nb_kind, bool growth
 // hipRTC of a user-body program for gfx950; the log on failure.
 static hiprtcResult user_compile(const std::string& src, std::vector<char>* code, std::string* log) {
   const char* headers[] = {cbm_rtc_cbm_device_h, cbm_rtc_cbm_structs_h, cbm_rtc_cbm_stage_h, cbm_rtc_cbm_rule_h,
-                           cbm_rtc_cbm_medium_h};
-  const char* names[] = {"cbm_device.h", "cbm_structs.h", "cbm_stage.h", "cbm_rule.h", "cbm_medium.h"};
+                           cbm_rtc_cbm_medium_h, cbm_rtc_cbm_model_h};
+  const char* names[] = {"cbm_device.h", "cbm_structs.h", "cbm_stage.h", "cbm_rule.h", "cbm_medium.h", "cbm_model.h"};
   hiprtcProgram prog;
-  hiprtcResult r = hiprtcCreateProgram(&prog, src.c_str(), "cbm_user_bodies.hip", 5, headers, names);
+  hiprtcResult r = hiprtcCreateProgram(&prog, src.c_str(), "cbm_user_bodies.hip", 6, headers, names);
   if (r != HIPRTC_SUCCESS) { *log = "hiprtcCreateProgram failed"; return r; }
   const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math"};
   r = hiprtcCompileProgram(prog, 5, opts);
@@ -1602,6 +1627,56 @@ int cbm_compile_check(const cbm_model_desc* model, const char* law_body, const c
     log[n] = '\0';
   }
   return r == HIPRTC_SUCCESS ? CBM_OK : CBM_ERR_COMPILE;
+}
+
+static int params_from_device(cbm_ctx* c, int32_t sim);
+
+static std::string model_source(const char* body) {
+  std::string src = "#define CBM_HIPRTC 1\n#include \"cbm_model.h\"\nnamespace cbm {\n";
+  src += body;
+  src += "\n}  // namespace cbm\n"
+         "extern \"C\" __global__ __launch_bounds__(64) void cbm_user_model(cbm::ModelArgs a) {\n"
+         "  cbm::model_step_body<cbm::UserModel>(a);\n}\n";
+  return src;
+}
+
+int cbm_compile_model_scope_check(const char* body, char* log, int64_t log_cap) {
+  if (!body || (log && log_cap < 1)) return CBM_ERR_ARG;
+  std::string lg;
+  const hiprtcResult r = user_compile(model_source(body), nullptr, &lg);
+  if (log) {
+    const size_t n = std::min(lg.size(), (size_t)log_cap - 1);
+    std::memcpy(log, lg.data(), n);
+    log[n] = '\0';
+  }
+  return r == HIPRTC_SUCCESS ? CBM_OK : CBM_ERR_COMPILE;
+}
+
+int cbm_compile_model_scope(cbm_ctx* c, const char* body) {
+  if (!c || !body) return CBM_ERR_ARG;
+  if (c->clv)
+    return fail(c, CBM_ERR_CONFIG, "a model-scope body moves the law's parameters on the device, past what the "
+                                   "cell-linked skin lists were sized for: create the context with walk = 1");
+  std::vector<char> code;
+  std::string log;
+  if (user_compile(model_source(body), &code, &log) != HIPRTC_SUCCESS)
+    return fail(c, CBM_ERR_COMPILE, std::string("hipRTC: ") + log);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < c->S; ++s) {
+    int rc = params_from_device(c, s);
+    if (rc) return rc;
+  }
+  if (c->model_mod) {
+    (void)hipModuleUnload(c->model_mod);
+    c->model_mod = nullptr;
+    c->model_fn = nullptr;
+  }
+  HIPCHK(c, hipModuleLoadData(&c->model_mod, code.data()));
+  HIPCHK(c, hipModuleGetFunction(&c->model_fn, c->model_mod, "cbm_user_model"));
+  if (!c->model_t) HIPCHK(c, dalloc(&c->model_t, 2));
+  // the current step's start time in its parity slot (the device step counter equals c->step)
+  HIPCHK(c, hipMemcpy(c->model_t + (c->step & 1), &c->t, sizeof(double), hipMemcpyHostToDevice));
+  return CBM_OK;
 }
 
 int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, const char* rule_body,
@@ -2141,8 +2216,30 @@ int cbm_count_neighbors_noalloc(cbm_ctx* c, int32_t sim, double radius, int64_t*
   return CBM_OK;
 }
 
+// The parameters a model-scope body has moved on the device, into the host copies.
+static int params_from_device(cbm_ctx* c, int32_t sim) {
+  if (!c->model_fn) return CBM_OK;
+  SimDev d;
+  HIPCHK(c, hipMemcpyAsync(&d, c->d_sims + sim, sizeof(SimDev), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 8; ++k) { c->h_sims[sim].p[k] = d.p[k]; c->sims[sim].p[k] = d.p[k]; }
+  return CBM_OK;
+}
+
+int cbm_get_params(cbm_ctx* c, int32_t sim, double* p, int32_t n) {
+  if (!c || !p || sim < 0 || sim >= c->S || n < 1 || n > 8) return CBM_ERR_ARG;
+  int rc = params_from_device(c, sim);
+  if (rc) return rc;
+  for (int k = 0; k < n; ++k) p[k] = c->h_sims[sim].p[k];
+  return CBM_OK;
+}
+
 int cbm_set_params(cbm_ctx* c, int32_t sim, const double* p, int32_t n) {
   if (!c || !p || sim < 0 || sim >= c->S || n < 1 || n > 8) return CBM_ERR_ARG;
+  {
+    int rc = params_from_device(c, sim);   // the entries past n keep their device values
+    if (rc) return rc;
+  }
   if (c->clv) {
     double q[8];
     for (int k = 0; k < 8; ++k) q[k] = k < n ? p[k] : c->sims[sim].p[k];
@@ -2221,8 +2318,8 @@ int cbm_adjoint(cbm_ctx* c, const cbm_loss_desc* L, double* grad, double* loss) 
   if (!c || !L || !grad || !loss) return CBM_ERR_ARG;
   const cbm_model_desc& m = c->m;
   if (m.integrator != CBM_EULER || m.law != CBM_LAW_REP_ADH || !c->verlet || c->vtime || c->rules ||
-      c->medium || c->lagged || m.drift != 0.0 || m.user_de > 0)
-    return fail(c, CBM_ERR_CONFIG, "the adjoint covers overdamped Euler with the REP_ADH law on Verlet lists (no rules, medium, lag or drift)");
+      c->medium || c->lagged || m.drift != 0.0 || m.user_de > 0 || c->model_fn)
+    return fail(c, CBM_ERR_CONFIG, "the adjoint covers overdamped Euler with the REP_ADH law on Verlet lists (no rules, medium, lag, drift or model scope)");
   const int64_t T = L->n_steps, K = L->checkpoint_every;
   const int S = L->n_saves;
   if (T < 1 || K < 1 || S < 1 || !L->save_steps) return fail(c, CBM_ERR_ARG, "bad loss description");

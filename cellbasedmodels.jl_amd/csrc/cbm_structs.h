@@ -346,6 +346,18 @@ struct MediumArgs {
   double bval[6];               // Dirichlet values
 };
 
+// Arguments of the model-scope kernel (modelStepRule! and modelStepDE!,
+// src/CommunityStructure/step.jl:21-30, :66-75): one thread per simulation.
+struct ModelArgs {
+  SimDev* sims;                 // the simulations' parameters p[8] (read, then committed)
+  const SimState* st;           // N per simulation
+  const int64_t* step_ptr;      // the device step counter
+  double* t_buf;                // the time at the start of step s in t_buf[s & 1]
+  double dt;
+  uint64_t seed;
+  int32_t S, pad;
+};
+
 }  // namespace cbm
 
 #endif  // CBM_STRUCTS_H

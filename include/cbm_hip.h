@@ -477,6 +477,38 @@ int cbm_adjoint(cbm_ctx* ctx, const cbm_loss_desc* loss_desc, double* grad,
  * (test/testFitting.jl:4-29). */
 int cbm_set_params(cbm_ctx* ctx, int32_t sim, const double* p, int32_t n);
 
+/* The current model-scope parameters p[0..n-1] of simulation `sim`, as a
+ * model-scope body has left them (com.k etc. after a run). */
+int cbm_get_params(cbm_ctx* ctx, int32_t sim, double* p, int32_t n);
+
+/* modelRule / modelODE / modelSDE (src/AgentStructure/agentStructure.jl:43-45,
+ * run by modelStepRule! and modelStepDE! after the medium step,
+ * src/CommunityStructure/step.jl:96-107): compile a model-scope body with
+ * hipRTC; every later step runs it once per simulation.  The model
+ * variables are the simulation's parameters p[8].  The body is HIP C++
+ * defining, inside namespace cbm, with every member optional:
+ *   struct UserModel {
+ *     __device__ void rule(ModelScope& m) const;                   // modelRule
+ *     static constexpr unsigned de = MASK;                         // bit k: p[k] is a DE variable
+ *     __device__ void ode(const ModelScope& m, double* dp) const;  // modelODE: dp[k] = dt(p[k])
+ *     __device__ void noise(const ModelScope& m, double* g) const; // modelSDE: g of p[k]
+ *   };
+ * A ModelScope holds t (the step's start time), dt, N, step, sim, p[8] (the
+ * values at the step's start) and pn[8] (the new values, initialised to p;
+ * the rule writes them).  DE parameters step by Euler, or Euler-Maruyama
+ * with noise: pn[k] = (p[k] + dp[k] dt) + g[k] z sqrt(dt), z normal from
+ * Philox keyed (seed, step, 0, sim, 0x600 + k/2).  pn is committed at the
+ * step's end, as update! commits write-new values: agents and the medium
+ * read p during the step and pn from the next step on.  CBM_ERR_CONFIG in
+ * a context running CellLinked on a skin list (the lists are sized for the
+ * law's reach at creation: create it with walk = 1); CBM_ERR_COMPILE with
+ * the hipRTC log in cbm_last_error. */
+int cbm_compile_model_scope(cbm_ctx* ctx, const char* model_body_hip);
+
+/* cbm_compile_model_scope's compilation alone, without a context or GPU;
+ * the log as for cbm_compile_check. */
+int cbm_compile_model_scope_check(const char* model_body_hip, char* log, int64_t log_cap);
+
 /* -------------------------------------------------------------- autotune */
 /* Pick block size (and, for Verlet models, list radius margin) by timing a
  * few steps; the choice is cached per (model, device) in `cache_path` (may

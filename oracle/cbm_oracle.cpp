@@ -20,6 +20,7 @@
 //   @∂2 medium stencil                src/AgentStructure/macros.jl:203-257
 //   @mediumInside / @mediumBorder     src/AgentStructure/macros.jl:259-311
 //   rods2D / rodForces                src/models/rods.jl:21-121
+//   modelStepRule! / modelStepDE!     src/CommunityStructure/step.jl:21-30, 66-75, 96-107
 //   rodIntersection and helpers       src/metrics.jl:138-246
 // with the spec decisions the GPU path shares (exact floor quotients,
 // periodic minimum image, canonical ascending compaction order, Philox noise,
@@ -145,6 +146,7 @@ struct Orc {
   bool growth = false;
   bool inertial = false;                      // CBM_LAW_SOFT_SPHERES: velocities are DE variables
   bool rods = false;                          // CBM_LAW_RODS: x, y, vx, vy, theta, omega are DE variables
+  int32_t sim = 0;                            // its index in a batched context: the noise streams' key
   std::vector<double> TH, OM, rodD, rodL;     // rods: angle, its rate, diameter, length
   std::vector<double> rodW, rodP;             // rods: torque and pressure of the last evaluation
   double noise_slope = 0.0;                   // state-dependent agentSDE g_a(x) = noise + slope x_a (positions)
@@ -159,6 +161,10 @@ struct Orc {
   std::vector<double> ruleX[3], ruleVol, ruleU[4];   // the rule's neighbor view: values at the step's start
   void* medium_fn = nullptr;                  // user mediumODE body (orc_medium_fn), else built-in
   void* medium_assign = nullptr;              // its assignment branches (orc_medium_assign_fn), optional
+  void* model_rule = nullptr;                 // model scope (orc_model_rule_fn, orc_model_ode_fn,
+  void* model_ode = nullptr;                  // orc_model_noise_fn) and its DE parameters
+  void* model_noise = nullptr;
+  unsigned model_de = 0;
   void* law_pair = nullptr;                   // a general agentODE body (orc_law_pair_fn, orc_law_ode_fn)
   void* law_ode = nullptr;
   int law_sums = 0;
@@ -912,7 +918,7 @@ static void apply_rule(Orc& o, std::vector<int64_t>& holes, int64_t& nAdd) {
         if (o.growth) { o.vol[nw] = o.vol[i]; o.grNew[nw] = o.gr[i]; o.vbNew[nw] = o.vb[i]; o.gr[nw] = o.gr[i]; o.vb[nw] = o.vb[i]; }
       }
     } else if (o.m.rule == CBM_RULE_DEATH) {
-      U4 r = draw(o.m.seed, o.step, o.id[i], 0, 0x400u);
+      U4 r = draw(o.m.seed, o.step, o.id[i], o.sim, 0x400u);
       if (u53(r.v[0], r.v[1]) < o.m.rule_p[0] * o.m.dt) { holes.push_back(i); o.alive[i] = 0; }
     } else if (o.m.rule == CBM_RULE_GROWTH_DIVISION) {
       if (!(o.vol[i] >= o.m.rule_p[3] * o.vb[i])) continue;
@@ -920,8 +926,8 @@ static void apply_rule(Orc& o, std::vector<int64_t>& holes, int64_t& nAdd) {
       if (nw >= o.nMax) { o.err = CBM_ERR_CAPACITY; o.msg = "agent capacity exceeded"; continue; }
       int64_t newId = o.idMax + 1 + nAdd;
       nAdd++;
-      U4 ra = draw(o.m.seed, o.step, o.id[i], 0, 0x200u);
-      U4 rg = draw(o.m.seed, o.step, o.id[i], 0, 0x300u);
+      U4 ra = draw(o.m.seed, o.step, o.id[i], o.sim, 0x200u);
+      U4 rg = draw(o.m.seed, o.step, o.id[i], o.sim, 0x300u);
       double ax[3] = {0, 0, 0};
       if (o.dims == 1) ax[0] = (u53(ra.v[0], ra.v[1]) < 0.5) ? -1.0 : 1.0;
       else if (o.dims == 2) {
@@ -1051,9 +1057,9 @@ static void de_step(Orc& o, int64_t nA) {
     for (int a = 0; a < d; ++a) G1[a].assign(nA, gsig);
     for (int64_t i = 0; i < nA; ++i) {
       double z[4];
-      U4 r0 = draw(o.m.seed, o.step, o.id[i], 0, 0x100u);
+      U4 r0 = draw(o.m.seed, o.step, o.id[i], o.sim, 0x100u);
       normal2(r0, z[0], z[1]);
-      if (d > 2) { U4 r1 = draw(o.m.seed, o.step, o.id[i], 0, 0x101u); normal2(r1, z[2], z[3]); }
+      if (d > 2) { U4 r1 = draw(o.m.seed, o.step, o.id[i], o.sim, 0x101u); normal2(r1, z[2], z[3]); }
       for (int a = 0; a < d; ++a) {
         if (slope != 0.0) G1[a][i] = gsig + slope * U[a][i];
         H[a][i] = (U[a][i] + K1[a][i] * dt) + G1[a][i] * z[a] * sq;
@@ -1064,9 +1070,9 @@ static void de_step(Orc& o, int64_t nA) {
     eval(H, K2);
     for (int64_t i = 0; i < nA; ++i) {
       double z[4];
-      U4 r0 = draw(o.m.seed, o.step, o.id[i], 0, 0x500u);
+      U4 r0 = draw(o.m.seed, o.step, o.id[i], o.sim, 0x500u);
       normal2(r0, z[0], z[1]);
-      if (d > 2) { U4 r1 = draw(o.m.seed, o.step, o.id[i], 0, 0x501u); normal2(r1, z[2], z[3]); }
+      if (d > 2) { U4 r1 = draw(o.m.seed, o.step, o.id[i], o.sim, 0x501u); normal2(r1, z[2], z[3]); }
       for (int a = 0; a < d; ++a) {
         // (g(u) + g(h)) of the reference's corrector (src/integrators.jl:801-803)
         const double g2 = slope != 0.0 ? gsig + slope * H[a][i] : gsig;
@@ -1081,9 +1087,9 @@ static void de_step(Orc& o, int64_t nA) {
     const double slope = o.noise_slope;
     for (int64_t i = 0; i < nA; ++i) {
       double z[4];
-      U4 r0 = draw(o.m.seed, o.step, o.id[i], 0, 0x100u);
+      U4 r0 = draw(o.m.seed, o.step, o.id[i], o.sim, 0x100u);
       normal2(r0, z[0], z[1]);
-      if (d > 2) { U4 r1 = draw(o.m.seed, o.step, o.id[i], 0, 0x101u); normal2(r1, z[2], z[3]); }
+      if (d > 2) { U4 r1 = draw(o.m.seed, o.step, o.id[i], o.sim, 0x101u); normal2(r1, z[2], z[3]); }
       for (int a = 0; a < d; ++a) {
         const double g = slope != 0.0 ? gsig + slope * U[a][i] : gsig;   // g at u (src/integrators.jl:709-727)
         U[a][i] = (U[a][i] + K1[a][i] * dt) + g * z[a] * sq;
@@ -1117,6 +1123,49 @@ static void wrap_positions(Orc& o, int64_t n) {
   }
 }
 
+// ------------------------------------------------------------ model scope
+// modelStepRule! then modelStepDE! (step.jl:96-107): the rule writes new
+// values from the step-start ones, the DE parameters step by Euler or, with
+// a noise term, Euler-Maruyama (the default modelAlg, agentStructure.jl:
+// 276-280); the new values are committed at once, after every agent and
+// medium update of the step has read the old ones.
+struct OrcModelScope {
+  double t, dt;
+  int64_t N, step;
+  int32_t sim, pad;
+  double p[8];
+  double pn[8];
+};
+typedef void (*orc_model_rule_fn)(OrcModelScope* m);
+typedef void (*orc_model_ode_fn)(const OrcModelScope* m, double* dp);
+
+static void model_step(Orc& o) {
+  if (!o.model_rule && !o.model_ode) return;
+  OrcModelScope m;
+  m.t = o.t; m.dt = o.m.dt; m.N = o.N; m.step = o.step; m.sim = o.sim; m.pad = 0;
+  for (int k = 0; k < 8; ++k) { m.p[k] = o.s.p[k]; m.pn[k] = m.p[k]; }
+  if (o.model_rule) ((orc_model_rule_fn)o.model_rule)(&m);
+  if (o.model_ode) {
+    double dp[8] = {0, 0, 0, 0, 0, 0, 0, 0}, g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    ((orc_model_ode_fn)o.model_ode)(&m, dp);
+    if (o.model_noise) ((orc_model_ode_fn)o.model_noise)(&m, g);
+    const double sq = std::sqrt(m.dt);
+    for (int q = 0; q < 4; ++q) {
+      double z[2] = {0, 0};
+      if (o.model_noise && ((o.model_de >> (2 * q)) & 3u))
+        normal2(draw(o.m.seed, o.step, 0, o.sim, 0x600u + (uint32_t)q), z[0], z[1]);
+      for (int h = 0; h < 2; ++h) {
+        const int k = 2 * q + h;
+        if (!((o.model_de >> k) & 1u)) continue;
+        double v = m.p[k] + dp[k] * m.dt;
+        if (o.model_noise) v = v + g[k] * z[h] * sq;
+        m.pn[k] = v;
+      }
+    }
+  }
+  for (int k = 0; k < 8; ++k) o.s.p[k] = m.pn[k];
+}
+
 static void one_step(Orc& o) {
   std::vector<int64_t> holes; int64_t nAdd = 0;
   apply_rule(o, holes, nAdd);                 // agentStepRule!
@@ -1124,6 +1173,7 @@ static void one_step(Orc& o) {
   de_step(o, nA);                              // agentStepDE!
   wrap_positions(o, nA);                       // periodic wrap after the final stage: the medium sees it
   if (o.m.medium_n[0] > 0) medium_step(o, nA); // mediumStepDE!
+  model_step(o);                               // modelStepRule!, modelStepDE!
   // update!: commit the write-new buffers, then compact
   for (int a = 0; a < o.dims; ++a) for (int64_t i = 0; i < nA; ++i) o.F[a][i] = o.Fnew[a][i];
   if (o.growth) for (int64_t i = 0; i < o.N + nAdd; ++i) { o.gr[i] = o.grNew[i]; o.vb[i] = o.vbNew[i]; }
@@ -1417,6 +1467,22 @@ void orc_set_medium_assign(void* h, void* fn) { ((Orc*)h)->medium_assign = fn; }
 // per neighbor, ode (orc_law_ode_fn) gives dt of the positions and of the
 // model's user_de user variables (U0..).  Returns CBM_ERR_ARG for sums
 // outside 1..8.
+// A model-scope body: rule (orc_model_rule_fn), ode and noise
+// (orc_model_ode_fn: dp or g), any of them NULL; de = the DE parameters.
+void orc_set_model_body(void* h, void* rule, void* ode, void* noise, unsigned de) {
+  Orc& o = *(Orc*)h;
+  o.model_rule = rule;
+  o.model_ode = ode;
+  o.model_noise = noise;
+  o.model_de = de;
+}
+
+// The simulation's index among a batched context's simulations, which keys
+// every random draw (Philox counter word `sim`).
+void orc_set_sim(void* h, int32_t sim) { ((Orc*)h)->sim = sim; }
+
+double orc_param(void* h, int k) { return (k >= 0 && k < 8) ? ((Orc*)h)->s.p[k] : 0.0; }
+
 int orc_set_law_body(void* h, void* pair, void* ode, int sums) {
   if (sums < 1 || sums > 8) return CBM_ERR_ARG;
   Orc& o = *(Orc*)h;

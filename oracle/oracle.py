@@ -59,6 +59,9 @@ def lib():
             "orc_traj_grad_dual": (C.c_double, [P, P, C.c_int64, C.c_int32, P, P, P]),
             "orc_rod_intersection": (None, [P, P, P]),
             "orc_set_law_body": (C.c_int, [P, P, P, C.c_int]),
+            "orc_set_model_body": (None, [P, P, P, P, C.c_uint]),
+            "orc_param": (C.c_double, [P, C.c_int]),
+            "orc_set_sim": (None, [P, C.c_int32]),
             "orc_rod_forces": (C.c_int, [P, P, P, P]),
         }
         for name, (res, args) in sig.items():
@@ -89,6 +92,17 @@ LAW_PAIR_FN = C.CFUNCTYPE(None, C.POINTER(LawAgent), C.POINTER(LawAgent), C.POIN
                           C.POINTER(C.c_double))
 LAW_ODE_FN = C.CFUNCTYPE(None, C.POINTER(LawAgent), C.POINTER(C.c_double), C.POINTER(C.c_double),
                          C.POINTER(C.c_double))
+
+
+class ModelScope(C.Structure):
+    """What a model-scope body sees (OrcModelScope): t, dt, N, step, sim, the
+    step-start parameters p[8] and their new values pn[8]."""
+    _fields_ = [("t", C.c_double), ("dt", C.c_double), ("N", C.c_int64), ("step", C.c_int64), ("sim", C.c_int32),
+                ("pad", C.c_int32), ("p", C.c_double * 8), ("pn", C.c_double * 8)]
+
+
+MODEL_RULE_FN = C.CFUNCTYPE(None, C.POINTER(ModelScope))
+MODEL_ODE_FN = C.CFUNCTYPE(None, C.POINTER(ModelScope), C.POINTER(C.c_double))
 
 
 RULE_FN = C.CFUNCTYPE(None, C.c_int64, C.c_int64, C.POINTER(RuleRec), C.POINTER(RuleRec), C.POINTER(RuleRec),
@@ -257,6 +271,23 @@ class Oracle:
 
         self._rule_cb = RULE_FN(cb)
         self.lib.orc_set_rule(self.h, C.cast(self._rule_cb, C.c_void_p))
+
+    def set_model_body(self, rule=None, ode=None, noise=None, de=0):
+        """A model-scope body: rule(m) writes m.pn; ode(m, dp) and noise(m, g)
+        give dt and g of the parameters whose bit is set in `de`."""
+        none = C.c_void_p(None)
+        self._model_cb = (MODEL_RULE_FN(lambda m: rule(m.contents)) if rule else None,
+                          MODEL_ODE_FN(lambda m, dp: ode(m.contents, dp)) if ode else None,
+                          MODEL_ODE_FN(lambda m, g: noise(m.contents, g)) if noise else None)
+        ptrs = [C.cast(f, C.c_void_p) if f is not None else none for f in self._model_cb]
+        self.lib.orc_set_model_body(self.h, *ptrs, int(de))
+
+    def set_sim(self, sim):
+        """This simulation's index in a batched context (keys its random draws)."""
+        self.lib.orc_set_sim(self.h, int(sim))
+
+    def params(self):
+        return np.array([self.lib.orc_param(self.h, k) for k in range(8)])
 
     def set_law_body(self, pair, ode, sums):
         """A general agentODE body: pair(i, j, dx, d, s) adds neighbor j's terms

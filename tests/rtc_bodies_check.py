@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 for p in (HERE, ROOT, os.path.join(ROOT, "cellbasedmodels.jl_amd")):
     sys.path.insert(0, p)
 
-MODULES = ["law_bodies", "medium_bodies", "rule_bodies", "test_api_host", "test_gpu_custom_law", "test_gpu_law_self",
+MODULES = ["law_bodies", "medium_bodies", "model_bodies", "rule_bodies", "test_api_host", "test_gpu_custom_law", "test_gpu_law_self",
            "test_gpu_sde_body", "test_gpu_user_de", "test_gpu_user_de_rules", "test_gpu_user_rule"]
 
 
@@ -31,6 +31,12 @@ def check(A, label, body):
     from helpers import model
     log = C.create_string_buffer(1 << 16)
     args = [None, None, None, None]
+    if "struct UserModel" in body:
+        print(label, flush=True)
+        rc = A.lib().cbm_compile_model_scope_check(body.encode(), log, len(log))
+        if rc != A.OK:
+            print(f"  FAILED ({rc}): {log.value.decode(errors='replace')[:2000]}", flush=True)
+        return rc == A.OK
     if "struct UserMedium" in body:
         m = model(2, neighbors=A.CELL_LINKED, law=A.LAW_LINEAR, cell_edge=2.0)
         m.medium_n[0], m.medium_n[1] = 8, 8
