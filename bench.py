@@ -455,12 +455,14 @@ def run_adjoint(steps=200, every=20):
             "config": "C5 3-D overdamped Euler, N=65536 x 64 parameter sets, 200 steps, dt 0.01, FP64"}
 
 
-def run_colony(target_n=500_000, max_steps=6000):
+def run_colony(target_n=2_000_000, max_steps=8000):
     """C3: 1000 agents (r = 1, k = 1) at the centre of a 2048 x 2048 medium
     (spacing 1, zero flux, D = 1, uptake 0.1 c per agent, FTCS dt = 0.2, c0 = 1);
     volumes grow at 0.01 +-10 % per unit time and divide at twice the birth
-    volume.  Runs until target_n agents (500k keeps the colony, radius ~750,
-    inside the 2048 box)."""
+    volume.  Runs until target_n agents.  Division outpaces the weak
+    repulsion, so the colony compresses far past disc packing (a core of
+    ~7 agents per unit area at 623k agents, radius ~210): 2M agents stay
+    well inside the box."""
     import cbm._abi as A
     from cbm.api import Context
     m = A.ModelDesc()

@@ -25,6 +25,16 @@
 // agent past half the skin (the lead, the step that ends with the build and
 // one and a half more).
 constexpr int64_t CLV_HOST_LEAD = 3;
+// Builds that follow each other within CLV_DENSE_GAP steps twice in a row
+// mean agents cross the skin faster than the host's lead: the stages would
+// defer ever more agents to the exact walk.  (One such gap is common while
+// random initial overlaps relax.)  The context then builds after
+// every step for CLV_DENSE_STEPS steps before it schedules again.  The
+// closest two scheduled builds can be is the lead + 2 steps: the build
+// clears the cue, the next step's final stage raises it, the step after
+// records it, and the host reads that record CLV_HOST_LEAD steps later.
+constexpr int64_t CLV_DENSE_GAP = CLV_HOST_LEAD + 2;
+constexpr int64_t CLV_DENSE_STEPS = 64;
 constexpr double CLV_AHEAD = 2.5 + (double)CLV_HOST_LEAD;
 
 
@@ -83,6 +93,8 @@ struct cbm_ctx {
   ClvRecord* rec_host = nullptr;
   ClvRecord* rec_dev = nullptr;
   int64_t last_build_step = -1;             // step whose end saw the last build
+  int clv_short = 0;                        // consecutive builds CLV_DENSE_GAP or fewer steps apart
+  int64_t clv_dense_until = -1;             // build after every step before this one
   int64_t rec_first = 0;                    // first step whose record the device writes
   bool capturing = false;
   bool build_now = false;
@@ -355,7 +367,7 @@ static void morton_ranks(const int32_t* gn, int dims, std::vector<int32_t>& out)
 // its rows are one range).  m is the largest size whose expected population
 // (the densest simulation's mean density; capacity when rules change the
 // population) stays within ~1.25 blocks of 256 threads, lowered while the
-// LDS for 1.3x the expected halo population would exceed 64 KiB (a denser
+// LDS for 1.2x the expected halo population would exceed 64 KiB (a denser
 // brick is left to the row-parallel stage kernel).  Bricks
 // are listed per simulation in Morton order of their coordinates.
 static int setup_bricks(cbm_ctx* c) {
@@ -1433,7 +1445,14 @@ static int neighbor_update(cbm_ctx* c, int64_t s, bool disp) {
   } else if (c->clv && !c->capturing) {
     // skin list, eager steps: the build the host scheduled, unconditional
     // (the step's last fix kernel has advanced the device step)
-    if (c->build_now) {
+    if (c->build_now && s >= c->clv_dense_until) {
+      c->clv_short = s - c->last_build_step <= CLV_DENSE_GAP ? c->clv_short + 1 : 0;
+      if (c->clv_short >= 2) {
+        c->clv_dense_until = s + CLV_DENSE_STEPS;
+        c->clv_short = 0;
+      }
+    }
+    if (c->build_now || s < c->clv_dense_until) {
       if ((rc = rebuild_chain(c, nullptr, nullptr, false, true))) return rc;
       c->last_build_step = s;
       c->build_now = false;

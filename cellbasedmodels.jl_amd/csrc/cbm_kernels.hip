@@ -1178,10 +1178,10 @@ static hipError_t launch_stage_law(int law, const StageLaunch& L, const StageArg
 }
 
 // a stage of CellLinked on a skin list, then the exact walk of the agents it
-// deferred (a grid of at most 64 blocks; it exits at once when none were)
+// deferred (a grid of at most 256 blocks, one per CU; it exits at once when none were)
 template <int D, int PER, int NOISE>
 static hipError_t launch_clv_stage(int law, const StageLaunch& L, const StageArgs& a) {
-  const dim3 fg(L.grid.x < 64 ? L.grid.x : 64);
+  const dim3 fg(L.grid.x < 256 ? L.grid.x : 256);
   // the ordering stage keeps lists over 16 entries in LDS (16-bit keys)
   const size_t lds = a.clv_order ? (size_t)L.block.x * (size_t)a.lcap * 2 : 0;
   switch (law) {

@@ -191,3 +191,29 @@ def test_heun_steps_ten_times_longer(gpu_available):
         o.step(k)
         for a in range(3):
             assert np.array_equal(g.download(a, n), o.get(a)), (k, a)
+
+
+def test_fast_agents_switch_to_builds_every_step(gpu_available):
+    # four 2-D periodic simulations of random discs at packing 0.5 under
+    # REP_ADH: overlapping agents cross half the skin within a step or two of
+    # every build, so builds come 4 steps apart (the host's lead) and the
+    # context switches to a build after every step; each simulation stays
+    # its oracle's bit for bit
+    m = model(2, integrator=A.EULER, neighbors=A.CELL_LINKED, law=A.LAW_REP_ADH, dt=0.01, cell_edge=2.4,
+              periodic=1)
+    rng = np.random.default_rng(5)
+    sims, pos = [], []
+    for q in range(4):
+        n = 4000
+        L = float(np.sqrt(n * np.pi / 0.5))
+        sims.append(sim(n, [[0.0, L]] * 2, [2.0, 1.0, 0.1, 2.4]))
+        pos.append([rng.uniform(0.0, L, n), rng.uniform(0.0, L, n)])
+    print("skin-list CellLinked: 4 x 2-D periodic REP_ADH N=4000, packing 0.5, 40 steps", flush=True)
+    g = device(m, sims, pos, 0)
+    g.step(40)
+    for q in range(4):
+        o = oracle(m, sims[q], pos[q])
+        o.step(40)
+        for a in range(2):
+            assert np.array_equal(g.download(a, 4000, sim=q), o.get(a)), (q, a)
+    assert g.rebuilds() >= 20
