@@ -27,6 +27,15 @@ F_X, F_Y, F_Z, F_ID, F_FX, F_FY, F_FZ, F_VOL, F_GROWTH, F_VBIRTH, F_LF = range(1
 F_VX, F_VY, F_VZ, F_RADIUS, F_MASS, F_FATE = range(11, 17)
 F_U0, F_U1, F_U2, F_U3 = range(17, 21)   # user agent parameters (RULE_CUSTOM)
 F_THETA, F_OMEGA, F_DIAMETER, F_LENGTH, F_TORQUE, F_PRESSURE = range(21, 27)   # rods
+F_U4, F_U5, F_U6, F_U7 = range(27, 31)   # user agent parameters U4..U7
+USER_PARAMS = 8
+
+
+def user_field(k):
+    """Field id of user agent parameter U_k (0 <= k < 8)."""
+    if not 0 <= k < USER_PARAMS:
+        raise ValueError("user agent parameters are U0..U7")
+    return F_U0 + k if k < 4 else F_U4 + (k - 4)
 INT_FIELDS = (F_ID, F_LF, F_FATE)
 
 

@@ -206,7 +206,7 @@ class ABM:
                 compiled with hipRTC by loadToPlatform_
     agentRule   a built-in rule (CBMRules.*) or a HIP C++ body defining struct
                 UserRule (optionally as (body, rule_p)), compiled with hipRTC
-    agent       user agent parameters {name: float} (at most 4) for a UserRule
+    agent       user agent parameters {name: float} (at most 8) for a UserRule
                 body: Community attributes, u[agent::name] in the body
     deVariables names of user agent parameters that are DE variables of a custom
                 agentODE body (UserLaw::de; the dt(var) lines of functionDE.jl)
@@ -245,13 +245,15 @@ class ABM:
             self.rule_src = None
         self.agentRule = agentRule
         agent = dict(agent or {})
-        if len(agent) > 4:
-            raise ValueError("at most 4 user agent parameters")
+        if len(agent) > A.USER_PARAMS:
+            raise ValueError(f"at most {A.USER_PARAMS} user agent parameters")
         de = tuple(deVariables)
         if len(set(de)) != len(de):
             raise ValueError("deVariables names a parameter more than once")
         if any(k not in agent for k in de):
             raise ValueError("deVariables must name user agent parameters")
+        if len(de) > 4:
+            raise ValueError("at most 4 deVariables (U0..U3)")
         if de and self.agentODE.law != A.LAW_CUSTOM:
             raise ValueError("deVariables need a custom agentODE body defining UserLaw::de")
         if agent and self.rule_src is None and not de:
@@ -261,7 +263,7 @@ class ABM:
         for name, typ in agent.items():
             if typ is not float or name in _FIELDS or not name.isidentifier():
                 raise ValueError(f"user agent parameter {name!r} must be a new Float64 name")
-        self.user_fields = {name: A.F_U0 + k for k, name in enumerate(agent)}
+        self.user_fields = {name: A.user_field(k) for k, name in enumerate(agent)}
         self.drift = float(drift)
         self.neighbors = neighborsAlg if neighborsAlg is not None else CBMNeighbors.Full()
         self.agentAlg = agentAlg if agentAlg is not None else CBMIntegrators.Euler()
@@ -603,7 +605,7 @@ def loadToPlatform_(com, preallocateAgents=0):
     sim = sim_desc(abm, com.N, com.N + int(preallocateAgents), com.simBox.reshape(-1).tolist(), com.law_p)
     com.ctx = Context(model, [sim])
     # user parameter names as indices of RuleAgent u[] and of UserLaw::de's u/du
-    names = "".join(f"constexpr int {n} = {f - A.F_U0};\n" for n, f in abm.user_fields.items())
+    names = "".join(f"constexpr int {n} = {k};\n" for k, n in enumerate(abm.user_fields))
     names = "namespace agent {\n" + names + "}\n" if names else ""
     law_src = names + abm.agentODE.source if abm.agentODE.law == A.LAW_CUSTOM else None
     noise_src = abm.agentSDE if isinstance(abm.agentSDE, str) else None

@@ -56,7 +56,7 @@ enum ColId {
   C_VX, C_VY, C_VZ,                     // velocities (soft spheres), DE state too
   C_ID, C_SLOT, C_FX, C_FY, C_FZ, C_GR, C_VB, C_LF,
   C_RAD, C_MASS, C_FATE,                // per-agent soft-sphere properties
-  C_U0, C_U1, C_U2, C_U3,               // user agent parameters (CBM_RULE_CUSTOM)
+  C_U0, C_U1, C_U2, C_U3, C_U4, C_U5, C_U6, C_U7,   // user agent parameters U0..U7
   C_TH, C_OM,                           // rods: angle and its rate, DE state too
   C_DIAM, C_LEN, C_TQ, C_PR,            // rods: diameter, length, torque and pressure
   C_CCELL,                              // CellLinked on a skin list: packed cell of the step start (int32)
@@ -211,7 +211,7 @@ struct cbm_ctx {
   hipFunction_t user_brick[2] = {nullptr, nullptr};   // the same, cell-linked brick tiles
   hipFunction_t user_rule[2] = {nullptr, nullptr};    // agentRule body: decide, apply
   hipFunction_t user_medium = nullptr;                // mediumODE body: the FTCS step
-  double* rule_nb[8] = {};                // a neighbor-looping rule body's view: x, y, z, vol, u0..u3
+  double* rule_nb[4 + USER_PARAMS] = {};  // a neighbor-looping rule body's view: x, y, z, vol, u0..u7
   int user_nb = NB_FULL;                  // neighbor walk the module was built for
   bool user_noise = false;                // the module holds a user agentSDE body
   double* gacc[MAX_STATE] = {};           // EulerHeun with a user agentSDE body: g of stage 1
@@ -610,7 +610,7 @@ static void destroy_ctx(cbm_ctx* c) {
   F(c->gr_new); F(c->vb_new); F(c->dead);
   F(c->n_add_slot); F(c->dead_slot); F(c->off_add); F(c->off_dead); F(c->holes);
   F(c->med_c[0]); F(c->med_c[1]); F(c->med_count);
-  for (int k = 0; k < 8; ++k) F(c->rule_nb[k]);
+  for (int k = 0; k < 4 + USER_PARAMS; ++k) F(c->rule_nb[k]);
   F(c->med_init); F(c->med_tmp); F(c->adi_tab[0]); F(c->adi_tab[1]); F(c->adi_tab[2]);
   F(c->key); F(c->lin); F(c->gbucket); F(c->cell_range); F(c->count); F(c->cursor); F(c->start); F(c->tmp); F(c->cell_row);
   F(c->emax); F(c->wlist); F(c->wlist_n); F(c->fix_rows); F(c->fix_n);
@@ -710,7 +710,7 @@ static int init_ctx(cbm_ctx* c) {
   }
   // user agent parameters: read by a rule body, or DE variables of the law body
   if (m.rule == CBM_RULE_CUSTOM || m.user_de > 0)
-    for (int k = 0; k < 4; ++k) if ((rc = alloc_column(c, C_U0 + k, 8))) return rc;
+    for (int k = 0; k < USER_PARAMS; ++k) if ((rc = alloc_column(c, C_U0 + k, 8))) return rc;
   // every DE variable's column exists before any stage addresses it
   for (int v = 0; v < c->nvar; ++v)
     if (!c->has_col[statecol_k(c, v)]) return fail(c, CBM_ERR_CONFIG, "DE variable without a column");
@@ -1045,7 +1045,7 @@ static StageArgs stage_base(cbm_ctx* c) {
   a.write_fnew = c->lagged ? 1 : 0;
   if (c->growth) a.growth = (const double*)c->col[C_GR].cur();
   if (c->has_col[C_U0])
-    for (int k = 0; k < 4; ++k) a.upar[k] = (const double*)c->col[C_U0 + k].cur();
+    for (int k = 0; k < USER_PARAMS; ++k) a.upar[k] = (const double*)c->col[C_U0 + k].cur();
   for (int v = 0; v < MAX_STATE; ++v) a.gacc[v] = c->gacc[v];
   if (c->inertial) {
     a.rad = (const double*)c->col[C_RAD].cur();
@@ -1164,7 +1164,7 @@ static RuleArgs rule_args(cbm_ctx* c, int32_t* flag) {
   r.id = (int64_t*)c->col[C_ID].cur();
   r.lf = (int64_t*)c->col[C_LF].cur();
   if (c->has_col[C_U0])
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < USER_PARAMS; ++k) {
       r.u[k] = (double*)c->col[C_U0 + k].cur();
       r.u_alt[k] = (double*)c->col[C_U0 + k].alt();
     }
@@ -1173,7 +1173,7 @@ static RuleArgs rule_args(cbm_ctx* c, int32_t* flag) {
   r.nb_kind = c->nb_kind;
   for (int d = 0; d < 3; ++d) r.nb_x[d] = c->rule_nb[d];
   r.nb_vol = c->rule_nb[3];
-  for (int k = 0; k < 4; ++k) r.nb_u[k] = c->rule_nb[4 + k];
+  for (int k = 0; k < USER_PARAMS; ++k) r.nb_u[k] = c->rule_nb[4 + k];
   r.slot = (int32_t*)c->col[C_SLOT].cur();
   r.dead = c->dead;
   r.n_add_slot = c->n_add_slot; r.dead_slot = c->dead_slot;
@@ -1747,7 +1747,7 @@ int cbm_compile_model(cbm_ctx* c, const char* law_body, const char* noise_body, 
     // a body with a neighbor loop reads its neighbors from a copy taken
     // before the rule's passes
     if (std::strstr(rule_body, "forNeighbors"))
-      for (int k = 0; k < 8; ++k)
+      for (int k = 0; k < 4 + USER_PARAMS; ++k)
         if (!c->rule_nb[k]) HIPCHK(c, dalloc(&c->rule_nb[k], c->C));
     // a body adding to the medium sums its additions per node
     if (std::strstr(rule_body, "medium_add") && c->medium && !c->med_add) {
@@ -1807,6 +1807,9 @@ static int field_col(cbm_ctx* c, int32_t field, int& k) {
     case CBM_F_U0: case CBM_F_U1: case CBM_F_U2: case CBM_F_U3:
       if (!c->has_col[C_U0]) return CBM_ERR_ARG;
       k = C_U0 + (field - CBM_F_U0); return CBM_OK;
+    case CBM_F_U4: case CBM_F_U5: case CBM_F_U6: case CBM_F_U7:
+      if (!c->has_col[C_U0]) return CBM_ERR_ARG;
+      k = C_U4 + (field - CBM_F_U4); return CBM_OK;
     default: return CBM_ERR_ARG;
   }
 }

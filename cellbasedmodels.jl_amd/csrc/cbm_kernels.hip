@@ -624,7 +624,7 @@ struct RuleDecision { int32_t add; int32_t dead; };
 __device__ __forceinline__ void rule_copy_u(const RuleArgs& a, int64_t r, int64_t dst) {
   if (!a.u[0]) return;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) { const double v = a.u[q][r]; a.u[q][dst] = v; a.u_alt[q][dst] = v; }
+  for (int q = 0; q < USER_PARAMS; ++q) { const double v = a.u[q][r]; a.u[q][dst] = v; a.u_alt[q][dst] = v; }
 }
 
 __device__ __forceinline__ RuleDecision rule_decide(const RuleArgs& a, int sim, int64_t r, int64_t step) {
@@ -1361,7 +1361,7 @@ hipError_t launch_adi_border(hipStream_t s, double* dst, const double* init, con
   return hipGetLastError();
 }
 
-struct NbCopy { double* dst[8]; };
+struct NbCopy { double* dst[4 + USER_PARAMS]; };
 
 __global__ __launch_bounds__(256) void k_rule_nb_copy(RuleArgs a, NbCopy d) {
   const RowIdx ri = row_index(a.g);
@@ -1370,12 +1370,12 @@ __global__ __launch_bounds__(256) void k_rule_nb_copy(RuleArgs a, NbCopy d) {
   for (int c = 0; c < a.dims; ++c) d.dst[c][r] = a.pos[c][r];
   d.dst[3][r] = a.vol[r];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) d.dst[4 + k][r] = a.u[k][r];
+  for (int k = 0; k < USER_PARAMS; ++k) d.dst[4 + k][r] = a.u[k][r];
 }
 
 hipError_t launch_rule_nb_copy(int nblocks, int block, hipStream_t s, const RuleArgs& a, double* const* dst) {
   NbCopy d;
-  for (int k = 0; k < 8; ++k) d.dst[k] = dst[k];
+  for (int k = 0; k < 4 + USER_PARAMS; ++k) d.dst[k] = dst[k];
   hipLaunchKernelGGL(k_rule_nb_copy, dim3(nblocks), dim3(block), 0, s, a, d);
   return hipGetLastError();
 }

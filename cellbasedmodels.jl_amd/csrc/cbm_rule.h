@@ -44,7 +44,7 @@ __device__ __forceinline__ int64_t medium_node(const RuleArgs& a, const double* 
 struct RuleRec {
   double x[3];
   double vol, growth, vbirth;
-  double u[4];
+  double u[USER_PARAMS];
   int64_t lf;
 };
 
@@ -94,7 +94,7 @@ struct RuleAgent {
     t.growth = c ? s.growth : t.growth;
     t.vbirth = c ? s.vbirth : t.vbirth;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) t.u[i] = c ? s.u[i] : t.u[i];
+    for (int i = 0; i < USER_PARAMS; ++i) t.u[i] = c ? s.u[i] : t.u[i];
     t.lf = c ? s.lf : t.lf;
   }
   // @removeAgent
@@ -173,7 +173,7 @@ __device__ __forceinline__ void rule_load(const RuleArgs& a, int64_t r, RuleRec&
   v.growth = a.gr[r];
   v.vbirth = a.vb[r];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) v.u[k] = a.u[k][r];
+  for (int k = 0; k < USER_PARAMS; ++k) v.u[k] = a.u[k][r];
   v.lf = a.lf[r];
 }
 
@@ -242,7 +242,7 @@ __device__ void user_rule_apply(const RuleArgs& a) {
   a.gr_new[r] = ag.nw.growth;
   a.vb_new[r] = ag.nw.vbirth;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) a.u[k][r] = ag.nw.u[k];
+  for (int k = 0; k < USER_PARAMS; ++k) a.u[k][r] = ag.nw.u[k];
   a.lf[r] = ag.nw.lf;
   if (ag.n_add == 0) return;
   const int64_t o = a.off_add[a.slot[r]];
@@ -265,7 +265,7 @@ __device__ void user_rule_apply(const RuleArgs& a) {
     ((double*)a.gr)[dst] = d.growth; a.gr_new[dst] = d.growth;
     ((double*)a.vb)[dst] = d.vbirth; a.vb_new[dst] = d.vbirth;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) { a.u[q][dst] = d.u[q]; a.u_alt[q][dst] = d.u[q]; }
+    for (int q = 0; q < USER_PARAMS; ++q) { a.u[q][dst] = d.u[q]; a.u_alt[q][dst] = d.u[q]; }
     a.lf[dst] = d.lf;
     a.id[dst] = st.id_max + 1 + o + k;
     a.slot[dst] = (int32_t)nw;

@@ -147,7 +147,7 @@ template <class L> struct law_walls<L, decltype((void)L::walls)> {
 struct LawAgent {
   int64_t row, id;
   double x[3];              // position at the stage state
-  double uv[4];             // user parameters U0..U3: DE variables at the stage state, others as stored
+  double uv[USER_PARAMS];   // user parameters U0..U7: DE variables at the stage state, others as stored
   double volv;              // vol at the stage state (1 without the vol variable)
   __device__ double u(int k) const { return uv[k]; }
   __device__ double vol() const { return volv; }
@@ -161,7 +161,7 @@ __device__ __forceinline__ LawAgent law_agent(const Args& a, int64_t row, const 
 #pragma unroll
   for (int c = 0; c < 3; ++c) r.x[c] = c < D ? x[c] : 0.0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
+  for (int k = 0; k < USER_PARAMS; ++k)
     r.uv[k] = k < CBM_USER_DE ? a.S[CBM_DE_BASE + k][row] : (a.upar[k] ? a.upar[k][row] : 0.0);
   r.volv = a.growth ? a.S[D][row] : 1.0;
   return r;

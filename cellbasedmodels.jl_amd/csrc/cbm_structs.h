@@ -18,6 +18,7 @@ using __hip_internal::uint16_t;
 namespace cbm {
 
 constexpr int MAX_COLS = 20;      // permutable per-agent columns
+constexpr int USER_PARAMS = 8;    // user agent parameters U0..U7 (the first user_de of them DE variables)
 constexpr int MAX_STATE = 8;      // DE variables: 3 positions + vol + 4 user, or 3 positions + 3 velocities
 constexpr int BUILTIN_STATE = 6;  // without user DE variables
 
@@ -169,7 +170,7 @@ struct StageArgs {
   const double* rod_l;
   double* rod_out[4];           // rods: fx, fy, W, pressure of this stage
   int32_t* rod_err;             // rods: set when a contact's overlap d_i - r_ij is negative
-  const double* upar[4];        // user agent parameter columns U0..U3 (null when absent)
+  const double* upar[USER_PARAMS];   // user agent parameter columns U0..U7 (null when absent)
   const int64_t* id;
   const int32_t* list;          // ELL Verlet list, entry q of row i at q*ldl + i
   const int32_t* list_n;
@@ -290,8 +291,8 @@ struct RuleArgs {
   double* Fnew[3];
   int64_t* id;
   int64_t* lf;
-  double* u[4];                 // user agent parameters (user rules, user DE variables), written in place
-  double* u_alt[4];             // their other buffers: new agents are written to both (user DE
+  double* u[USER_PARAMS];       // user agent parameters (user rules, user DE variables), written in place
+  double* u_alt[USER_PARAMS];   // their other buffers: new agents are written to both (user DE
                                 // variables flip with the positions under Euler and EM)
   int32_t* slot;
   uint8_t* dead;                // per row
@@ -313,7 +314,7 @@ struct RuleArgs {
   int32_t nb_kind, pad2;
   const double* nb_x[3];
   const double* nb_vol;
-  const double* nb_u[4];
+  const double* nb_u[USER_PARAMS];
   // the medium at the agent's node (cellInMesh of its step-start position,
   // src/AgentStructure/auxiliar.jl:353-365): read from med_c (the field at
   // the step start, NULL without a medium); additions are summed per node

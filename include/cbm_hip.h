@@ -142,7 +142,7 @@ enum cbm_medium_border { CBM_BORDER_MIRROR = 0, CBM_BORDER_DIRICHLET = 1, CBM_BO
  *   CBM_RULE_CUSTOM  a user agentRule body compiled by cbm_compile_model
  *     (required before stepping).  The agents carry vol, growth, vbirth
  *     (dvol/dt = growth*vol, as for CBM_RULE_GROWTH_DIVISION), lf and the
- *     user parameters CBM_F_U0..U3 for the body to read and write.       */
+  *     user parameters CBM_F_U0..U7 for the body to read and write.       */
 enum cbm_rule {
   CBM_RULE_NONE = 0,
   CBM_RULE_GROWTH_DIVISION = 1,
@@ -166,7 +166,8 @@ enum cbm_field {
   CBM_F_THETA = 21, CBM_F_OMEGA = 22,          /* rods: angle and its rate (DE variables)  */
   CBM_F_DIAMETER = 23, CBM_F_LENGTH = 24,      /* rods: per-rod geometry                    */
   CBM_F_TORQUE = 25, CBM_F_PRESSURE = 26,      /* rods: W and pressure of the last stage    */
-  CBM_F_COUNT = 27
+  CBM_F_U4 = 27, CBM_F_U5 = 28, CBM_F_U6 = 29, CBM_F_U7 = 30,   /* user agent parameters U4..U7 */
+  CBM_F_COUNT = 31
 };
 
 /* --------------------------------------------------- model description */
@@ -322,7 +323,7 @@ int cbm_compile_check(const cbm_model_desc* model, const char* law_body, const c
  *   };
  * run once per agent and step before the DE stages.  RuleAgent (cbm_rule.h)
  * holds a.old, the agent's values at the start of the step (x[3], vol,
- * growth, vbirth, u[4], lf), and a.nw, its new values (initialised to old);
+ * growth, vbirth, u[8], lf), and a.nw, its new values (initialised to old);
  * a.id, a.step, a.dt and a.dims; a.child() is a new agent's record
  * (the parent's old values) and a.add(d) adds it (@addAgent; at most 2 per
  * agent and step); a.remove() removes the agent (@removeAgent);

@@ -155,10 +155,10 @@ struct Orc {
   std::vector<int64_t> fate;                  // cellFate, 1-based
   std::vector<double> vol, gr, vb, grNew, vbNew;
   std::vector<int64_t> id, lf;
-  std::vector<double> U[4];                   // user agent parameters (CBM_RULE_CUSTOM)
+  std::vector<double> U[8];                   // user agent parameters U0..U7
   std::vector<char> alive;
   void* rule_fn = nullptr;                    // CBM_RULE_CUSTOM body (orc_rule_fn)
-  std::vector<double> ruleX[3], ruleVol, ruleU[4];   // the rule's neighbor view: values at the step's start
+  std::vector<double> ruleX[3], ruleVol, ruleU[8];   // the rule's neighbor view: values at the step's start
   void* medium_fn = nullptr;                  // user mediumODE body (orc_medium_fn), else built-in
   void* medium_assign = nullptr;              // its assignment branches (orc_medium_assign_fn), optional
   void* model_rule = nullptr;                 // model scope (orc_model_rule_fn, orc_model_ode_fn,
@@ -346,7 +346,7 @@ static inline bool soft_s(const double* p, double d, double ri, double rj, int64
 struct OrcLawAgent {
   int64_t row, id;
   double x[3];
-  double u[4];
+  double u[8];
   double vol;
 };
 typedef void (*orc_law_pair_fn)(const OrcLawAgent* i, const OrcLawAgent* j, const double* dx, double d, double* s);
@@ -364,7 +364,7 @@ static void body_deriv(const Orc& o, int64_t i, const std::vector<double>* S, do
     r.row = j;
     r.id = o.id[j];
     for (int a = 0; a < 3; ++a) r.x[a] = a < d ? S[a][j] : 0.0;
-    for (int k = 0; k < 4; ++k) r.u[k] = k < o.m.user_de ? S[base + k][j] : o.U[k][j];
+    for (int k = 0; k < 8; ++k) r.u[k] = k < o.m.user_de ? S[base + k][j] : o.U[k][j];
     r.vol = o.growth ? S[d][j] : 1.0;
     return r;
   };
@@ -834,7 +834,7 @@ struct Births { std::vector<int64_t> parent; };
 // step (old), its new values (nw, initialised to old) and up to two added
 // agents (add, each initialised to old by the caller); sets *n_add, *dead.
 struct OrcRuleRec {
-  double x[3]; double vol, growth, vbirth; double u[4]; int64_t lf;
+  double x[3]; double vol, growth, vbirth; double u[8]; int64_t lf;
   double medium;   // old: the medium at the agent's node as the step started
   double madd;     // nw: the body's additions to that node, in call order
 };
@@ -854,13 +854,13 @@ static void apply_custom_rule(Orc& o, std::vector<int64_t>& holes, int64_t& nAdd
   // reads of x[i2] see the committed values; writes go to x__)
   for (int a = 0; a < 3; ++a) o.ruleX[a] = o.X[a];
   o.ruleVol = o.vol;
-  for (int k = 0; k < 4; ++k) o.ruleU[k] = o.U[k];
+  for (int k = 0; k < 8; ++k) o.ruleU[k] = o.U[k];
   const bool med = o.m.medium_n[0] > 0;
   for (int64_t i = 0; i < N; ++i) {
     OrcRuleRec old;
     for (int a = 0; a < 3; ++a) old.x[a] = a < o.dims ? o.X[a][i] : 0.0;
     old.vol = o.vol[i]; old.growth = o.gr[i]; old.vbirth = o.vb[i];
-    for (int k = 0; k < 4; ++k) old.u[k] = o.U[k][i];
+    for (int k = 0; k < 8; ++k) old.u[k] = o.U[k][i];
     old.lf = o.lf[i];
     const int64_t node = med ? medium_node(o, old.x) : 0;
     old.medium = med ? o.c[node] : 0.0;
@@ -877,7 +877,7 @@ static void apply_custom_rule(Orc& o, std::vector<int64_t>& holes, int64_t& nAdd
     }
     for (int a = 0; a < o.dims; ++a) o.X[a][i] = nw.x[a];
     o.vol[i] = nw.vol; o.grNew[i] = nw.growth; o.vbNew[i] = nw.vbirth;
-    for (int k = 0; k < 4; ++k) o.U[k][i] = nw.u[k];
+    for (int k = 0; k < 8; ++k) o.U[k][i] = nw.u[k];
     o.lf[i] = nw.lf;
     if (dead) { holes.push_back(i); o.alive[i] = 0; }
     for (int k = 0; k < na; ++k) {
@@ -886,7 +886,7 @@ static void apply_custom_rule(Orc& o, std::vector<int64_t>& holes, int64_t& nAdd
       const OrcRuleRec& d = add[k];
       for (int a = 0; a < o.dims; ++a) { o.X[a][j] = d.x[a]; o.F[a][j] = o.F[a][i]; o.Fnew[a][j] = o.F[a][i]; }
       o.vol[j] = d.vol; o.gr[j] = d.growth; o.grNew[j] = d.growth; o.vb[j] = d.vbirth; o.vbNew[j] = d.vbirth;
-      for (int q = 0; q < 4; ++q) o.U[q][j] = d.u[q];
+      for (int q = 0; q < 8; ++q) o.U[q][j] = d.u[q];
       o.lf[j] = d.lf;
       o.id[j] = o.idMax + 1 + nAdd; o.alive[j] = 1;
       nAdd++;
@@ -972,7 +972,7 @@ static void compact(Orc& o, const std::vector<int64_t>& holes, int64_t nAdd) {
       if (!(dst < src)) continue;
       for (int a = 0; a < o.dims; ++a) { o.X[a][dst] = o.X[a][src]; o.F[a][dst] = o.F[a][src]; o.Fnew[a][dst] = o.Fnew[a][src]; }
       o.id[dst] = o.id[src]; o.lf[dst] = o.lf[src];
-      for (int k = 0; k < 4; ++k) o.U[k][dst] = o.U[k][src];
+      for (int k = 0; k < 8; ++k) o.U[k][dst] = o.U[k][src];
       if (o.growth) { o.vol[dst] = o.vol[src]; o.gr[dst] = o.gr[src]; o.vb[dst] = o.vb[src]; o.grNew[dst] = o.grNew[src]; o.vbNew[dst] = o.vbNew[src]; }
     }
   }
@@ -1268,7 +1268,7 @@ void* orc_create(const cbm_model_desc* m, const cbm_sim_desc* s) {
   for (int64_t i = 0; i < o->nMax; ++i) o->id[i] = i + 1;
   o->vol.assign(o->nMax, 1.0); o->gr.assign(o->nMax, 0.0); o->vb.assign(o->nMax, 1.0);
   o->grNew.assign(o->nMax, 0.0); o->vbNew.assign(o->nMax, 1.0);
-  for (int k = 0; k < 4; ++k) o->U[k].assign(o->nMax, 0.0);
+  for (int k = 0; k < 8; ++k) o->U[k].assign(o->nMax, 0.0);
   double edge = m->cell_edge;
   if (m->neighbors == CBM_CELL_LINKED) {
     set_grid(*o, o->g, edge);
@@ -1323,6 +1323,7 @@ int orc_set(void* h, int field, const void* data, int64_t n) {
       case CBM_F_MASS: o.mass[i] = v; break;
       case CBM_F_FATE: if (iv < 1 || iv > 2) return CBM_ERR_ARG; o.fate[i] = iv; break;
       case CBM_F_U0: case CBM_F_U1: case CBM_F_U2: case CBM_F_U3: o.U[field - CBM_F_U0][i] = v; break;
+      case CBM_F_U4: case CBM_F_U5: case CBM_F_U6: case CBM_F_U7: o.U[4 + field - CBM_F_U4][i] = v; break;
       case CBM_F_THETA: o.TH[i] = v; break;
       case CBM_F_OMEGA: o.OM[i] = v; break;
       case CBM_F_DIAMETER: o.rodD[i] = v; break;
@@ -1354,6 +1355,7 @@ int orc_get(void* h, int field, void* out, int64_t n) {
       case CBM_F_MASS: dv[i] = o.mass[i]; break;
       case CBM_F_FATE: iv[i] = o.fate[i]; break;
       case CBM_F_U0: case CBM_F_U1: case CBM_F_U2: case CBM_F_U3: dv[i] = o.U[field - CBM_F_U0][i]; break;
+      case CBM_F_U4: case CBM_F_U5: case CBM_F_U6: case CBM_F_U7: dv[i] = o.U[4 + field - CBM_F_U4][i]; break;
       case CBM_F_THETA: dv[i] = o.TH[i]; break;
       case CBM_F_OMEGA: dv[i] = o.OM[i]; break;
       case CBM_F_DIAMETER: dv[i] = o.rodD[i]; break;
@@ -1443,7 +1445,7 @@ void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
 void orc_set_rule(void* h, void* fn) { ((Orc*)h)->rule_fn = fn; }
 // A custom rule's neighbor loop: the candidates of slot i in the structure's
 // walk order, and their values as the rule's loop started (field 0..2
-// positions, 3 vol, 4..7 the user parameters, 8 id).
+// positions, 3 vol, 4..11 the user parameters, 12 id).
 int64_t orc_rule_neighbors(void* h, int64_t i, int64_t* out, int64_t cap) {
   Orc& o = *(Orc*)h;
   int64_t k = 0;
@@ -1454,7 +1456,7 @@ double orc_rule_value(void* h, int field, int64_t j) {
   Orc& o = *(Orc*)h;
   if (field < 3) return o.ruleX[field][j];
   if (field == 3) return o.ruleVol[j];
-  if (field < 8) return o.ruleU[field - 4][j];
+  if (field < 12) return o.ruleU[field - 4][j];
   return (double)o.id[j];
 }
 

@@ -78,13 +78,13 @@ INT_FIELDS = (3, 10, 16)   # id, lf, cellFate (int64)
 class RuleRec(C.Structure):
     """One agent's values as a CBM_RULE_CUSTOM body sees them (OrcRuleRec)."""
     _fields_ = [("x", C.c_double * 3), ("vol", C.c_double), ("growth", C.c_double), ("vbirth", C.c_double),
-                ("u", C.c_double * 4), ("lf", C.c_int64), ("medium", C.c_double), ("madd", C.c_double)]
+                ("u", C.c_double * 8), ("lf", C.c_int64), ("medium", C.c_double), ("madd", C.c_double)]
 
 
 class LawAgent(C.Structure):
     """One agent as a general agentODE body sees it (OrcLawAgent): stage-state
-    position, user parameters u[0..4) (DE variables at the stage state), vol."""
-    _fields_ = [("row", C.c_int64), ("id", C.c_int64), ("x", C.c_double * 3), ("u", C.c_double * 4),
+    position, user parameters u[0..8) (DE variables at the stage state), vol."""
+    _fields_ = [("row", C.c_int64), ("id", C.c_int64), ("x", C.c_double * 3), ("u", C.c_double * 8),
                 ("vol", C.c_double)]
 
 
@@ -115,7 +115,7 @@ class RuleNeighbor:
     def __init__(self, oracle, ag, j):
         lib_, h = oracle.lib, oracle.h
         self.row = j
-        self.id = int(lib_.orc_rule_value(h, 8, j))
+        self.id = int(lib_.orc_rule_value(h, 12, j))
         self.x = [lib_.orc_rule_value(h, c, j) if c < ag.dims else 0.0 for c in range(3)]
         self.dx = [0.0, 0.0, 0.0]
         s2 = 0.0

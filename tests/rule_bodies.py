@@ -209,3 +209,41 @@ def secrete(s):
         a.nw.u[0] = a.medium()
         a.medium_add(s * a.dt)
     return body
+
+
+# eight user parameters: u[7] counts the agent's steps, u[5] doubles, u[6]
+# sums the neighbors' u[4] read before the rule; every third agent adds a
+# daughter carrying u[6] + 1 in its u[7]
+EIGHT = """
+struct UserRule {
+  __device__ UserRule(const double*, const double*) {}
+  __device__ void operator()(RuleAgent& a) const {
+    a.nw.u[7] = a.old.u[7] + 1.0;
+    a.nw.u[5] = a.old.u[5] * 2.0;
+    double s = 0.0;
+    a.forNeighbors([&](const RuleNeighbor& n) { if (n.d < 1.5) s = s + n.u(4); });
+    a.nw.u[6] = s;
+    if (a.id % 3 == 0 && a.step == 0) {
+      RuleRec d = a.child();
+      d.u[7] = s + 1.0;
+      d.x[0] = a.old.x[0] + 0.25;
+      a.add(d);
+    }
+  }
+};
+"""
+
+
+def eight(a):
+    a.nw.u[7] = a.old.u[7] + 1.0
+    a.nw.u[5] = a.old.u[5] * 2.0
+    s = 0.0
+    for n in a.neighbors():
+        if n.d < 1.5:
+            s = s + n.u(4)
+    a.nw.u[6] = s
+    if a.id % 3 == 0 and a.step == 0:
+        d = a.child()
+        d.u[7] = s + 1.0
+        d.x[0] = a.old.x[0] + 0.25
+        a.add(d)

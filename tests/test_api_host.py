@@ -56,3 +56,14 @@ def test_de_variables_come_first_and_set_user_de():
         ABM(2, agentODE=law, agent={"a": float}, deVariables=("b",))
     with pytest.raises(ValueError, match="more than once"):
         ABM(2, agentODE=law, agent={"a": float, "b": float}, deVariables=("a", "a"))
+
+
+def test_user_parameter_fields_u0_to_u7():
+    assert [A.user_field(k) for k in range(8)] == [17, 18, 19, 20, 27, 28, 29, 30]
+    with pytest.raises(ValueError):
+        A.user_field(8)
+    from cbm.api import ABM
+    with pytest.raises(ValueError, match="at most 8"):
+        ABM(2, agentRule="struct UserRule {};", agent={f"p{k}": float for k in range(9)})
+    abm = ABM(2, agentRule="struct UserRule {};", agent={f"p{k}": float for k in range(8)})
+    assert list(abm.user_fields.values()) == [17, 18, 19, 20, 27, 28, 29, 30]

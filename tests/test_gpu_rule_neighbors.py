@@ -66,3 +66,38 @@ def test_contact_inhibition_matches_oracle(gpu_available, nb):
         for f in (A.F_X, A.F_Y):
             assert rel_err(g.download(f, n), o.get(f)) <= 1e-12
     assert o.n > n0
+
+
+def test_eight_user_parameters(gpu_available):
+    # a rule reading and writing U4..U7 (neighbors' u(4) through the loop's
+    # view, daughters with their own u[7]) equals its oracle restatement bit
+    # for bit
+    n0, L = 200, 25.0
+    m = model(2, integrator=A.EULER, neighbors=A.CELL_LINKED, law=A.LAW_LINEAR, rule=A.RULE_CUSTOM, dt=0.02,
+              cell_edge=2.0)
+    s = sim(n0, [[0.0, L], [0.0, L]], [1.0, 2.0], n_max=400)
+    pos = uniform_positions(n0, [[0.0, L], [0.0, L]], 78)
+    rng = np.random.default_rng(2)
+    fields = [(A.F_X, pos[0]), (A.F_Y, pos[1]), (A.F_U4, rng.uniform(0.0, 1.0, n0)), (A.F_U5, np.ones(n0)),
+              (A.F_U2, rng.uniform(0.0, 1.0, n0))]
+    o = O.Oracle(m, s)
+    o.set_rule(RB.eight)
+    g = Context(m, [s])
+    g.compile_model(rule=RB.EIGHT)
+    for f, v in fields:
+        o.set(f, v)
+        g.upload(f, v)
+    assert o.init() == 0
+    g.neighbors()
+    for chunk in (1, 2):
+        o.step(chunk)
+        g.step(chunk)
+        n = o.n
+        assert g.n() == n
+        assert np.array_equal(g.download(A.F_ID, n), o.get(A.F_ID))
+        for k in range(8):
+            f = A.user_field(k)
+            assert np.array_equal(g.download(f, n), o.get(f)), k
+        for f in (A.F_X, A.F_Y):
+            assert rel_err(g.download(f, n), o.get(f)) <= 1e-12
+    assert o.n > n0 and np.all(o.get(A.F_U5)[:n0] == 8.0)
