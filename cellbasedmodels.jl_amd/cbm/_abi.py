@@ -19,7 +19,7 @@ ERRORS = {
 EULER, HEUN, RK4, EM, EULER_HEUN = 0, 1, 2, 3, 4
 FULL, CELL_LINKED, VERLET_DISPLACEMENT, VERLET_TIME, CLVD = 0, 1, 2, 3, 4
 LAW_NONE, LAW_LINEAR, LAW_REP_ADH, LAW_AGGREGATION, LAW_CUSTOM, LAW_SOFT_SPHERES, LAW_RODS = 0, 1, 2, 3, 4, 5, 6
-RULE_NONE, RULE_GROWTH_DIVISION, RULE_PARITY, RULE_DEATH, RULE_CUSTOM = 0, 1, 2, 3, 4
+RULE_NONE, RULE_GROWTH_DIVISION, RULE_PARITY, RULE_DEATH, RULE_CUSTOM, RULE_RODS_GROWTH = 0, 1, 2, 3, 4, 5
 MEDIUM_FTCS, MEDIUM_DGADI = 0, 1
 BORDER_MIRROR, BORDER_DIRICHLET, BORDER_COPY = 0, 1, 2
 
@@ -28,6 +28,7 @@ F_VX, F_VY, F_VZ, F_RADIUS, F_MASS, F_FATE = range(11, 17)
 F_U0, F_U1, F_U2, F_U3 = range(17, 21)   # user agent parameters (RULE_CUSTOM)
 F_THETA, F_OMEGA, F_DIAMETER, F_LENGTH, F_TORQUE, F_PRESSURE = range(21, 27)   # rods
 F_U4, F_U5, F_U6, F_U7 = range(27, 31)   # user agent parameters U4..U7
+F_LTARGET = 31                            # rods with RULE_RODS_GROWTH
 USER_PARAMS = 8
 
 

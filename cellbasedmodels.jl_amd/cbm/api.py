@@ -183,6 +183,13 @@ class CBMRules:
     """Built-in agentRule bodies (see enum cbm_rule in include/cbm_hip.h)."""
 
     @staticmethod
+    def rodsGrowth(growth, sigma_lTarget, lMax, alpha):
+        """rods2dGrowth (src/models/rods.jl:123-164) with CBMModels.rods2D: l
+        grows at growth/(1 + alpha pressure); past lTarget a rod becomes two
+        daughters of length (l - d)/2.  Per-agent field: lTarget."""
+        return (A.RULE_RODS_GROWTH, [growth, sigma_lTarget, lMax, alpha])
+
+    @staticmethod
     def growthDivision(g_mean=0.01, g_spread=0.1, split=0.5, ratio=2.0):
         return (A.RULE_GROWTH_DIVISION, [g_mean, g_spread, split, ratio])
 
@@ -332,7 +339,7 @@ _FIELDS = {"x": A.F_X, "y": A.F_Y, "z": A.F_Z, "id": A.F_ID, "fx": A.F_FX, "fy":
            "fz": A.F_FZ, "vol": A.F_VOL, "growth": A.F_GROWTH, "vbirth": A.F_VBIRTH, "lf": A.F_LF,
            "vx": A.F_VX, "vy": A.F_VY, "vz": A.F_VZ, "r": A.F_RADIUS, "m": A.F_MASS, "cellFate": A.F_FATE,
            "theta": A.F_THETA, "ω": A.F_OMEGA, "d": A.F_DIAMETER, "l": A.F_LENGTH, "W": A.F_TORQUE,
-           "pressure": A.F_PRESSURE}
+           "pressure": A.F_PRESSURE, "lTarget": A.F_LTARGET}
 
 
 class Context:
@@ -562,6 +569,8 @@ class Community:
                 self.params[k] = np.zeros(self.N)
             for k in ("d", "l", "m"):
                 self.params[k] = np.ones(self.N)
+            if abm.agentRule is not None and abm.agentRule[0] == A.RULE_RODS_GROWTH:
+                self.params["lTarget"] = np.zeros(self.N)
         if abm.rule_src is not None:
             self.params["vol"] = np.ones(self.N)
             self.params["growth"] = np.zeros(self.N)

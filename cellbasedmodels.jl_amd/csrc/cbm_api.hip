@@ -58,7 +58,8 @@ enum ColId {
   C_RAD, C_MASS, C_FATE,                // per-agent soft-sphere properties
   C_U0, C_U1, C_U2, C_U3, C_U4, C_U5, C_U6, C_U7,   // user agent parameters U0..U7
   C_TH, C_OM,                           // rods: angle and its rate, DE state too
-  C_DIAM, C_LEN, C_TQ, C_PR,            // rods: diameter, length, torque and pressure
+  C_DIAM, C_LEN, C_TQ, C_PR,            // rods: diameter, length (a DE variable with growth), torque, pressure
+  C_LT,                                 // rods with CBM_RULE_RODS_GROWTH: the division length lTarget
   C_CCELL,                              // CellLinked on a skin list: packed cell of the step start (int32)
   C_NCOLS
 };
@@ -272,7 +273,6 @@ static int validate(cbm_ctx* c) {
   if (m.law == CBM_LAW_RODS) {
     // rods2D (src/models/rods.jl:61-121): a 2-D agentODE without rules
     if (m.dims != 2) return fail(c, CBM_ERR_CONFIG, "the rod model is 2-D");
-    if (m.rule != CBM_RULE_NONE) return fail(c, CBM_ERR_CONFIG, "the rod model takes no agent rules");
     if (m.periodic) return fail(c, CBM_ERR_CONFIG, "the rod model has no periodic box (its contact geometry uses raw coordinates)");
     if (m.lagged) return fail(c, CBM_ERR_CONFIG, "the rod model sums its contacts at the stage state (no lagged forces)");
     if (m.integrator != CBM_EULER && m.integrator != CBM_HEUN && m.integrator != CBM_RK4)
@@ -290,7 +290,9 @@ static int validate(cbm_ctx* c) {
   if (verlet && !(m.skin > 0)) return fail(c, CBM_ERR_CONFIG, "Verlet algorithms need skin > 0");
   if (verlet && (m.max_neighbors < 1 || m.max_neighbors > 4096)) return fail(c, CBM_ERR_CONFIG, "nMaxNeighbors must be in [1, 4096]");
   if (m.neighbors == CBM_VERLET_TIME && !(m.dt_recompute > 0)) return fail(c, CBM_ERR_CONFIG, "VerletTime needs dtNeighborRecompute > 0");
-  if (m.rule < CBM_RULE_NONE || m.rule > CBM_RULE_CUSTOM) return fail(c, CBM_ERR_CONFIG, "unknown agent rule");
+  if (m.rule < CBM_RULE_NONE || m.rule > CBM_RULE_RODS_GROWTH) return fail(c, CBM_ERR_CONFIG, "unknown agent rule");
+  if ((m.rule == CBM_RULE_RODS_GROWTH) != (m.law == CBM_LAW_RODS && m.rule != CBM_RULE_NONE))
+    return fail(c, CBM_ERR_CONFIG, "the rod model takes CBM_RULE_RODS_GROWTH or no rule, and that rule needs it");
   if (m.rule != CBM_RULE_NONE && c->S != 1) return fail(c, CBM_ERR_CONFIG, "agent rules (division/death) need a single simulation per context");
   if (m.medium_n[0] > 0) {
     if (c->S != 1) return fail(c, CBM_ERR_CONFIG, "a medium needs a single simulation per context");
@@ -644,7 +646,8 @@ static int init_ctx(cbm_ctx* c) {
   c->rules = m.rule != CBM_RULE_NONE;
   c->inertial = m.law == CBM_LAW_SOFT_SPHERES;
   c->rods = m.law == CBM_LAW_RODS;
-  c->nvar = c->rods ? 6 : (c->inertial ? 2 * c->dims : c->dims + (c->growth ? 1 : 0) + c->m.user_de);
+  c->nvar = c->rods ? (m.rule == CBM_RULE_RODS_GROWTH ? 7 : 6)
+                    : (c->inertial ? 2 * c->dims : c->dims + (c->growth ? 1 : 0) + c->m.user_de);
   c->lagged = m.lagged != 0;
   c->full = m.neighbors == CBM_FULL;
   c->cl = m.neighbors == CBM_CELL_LINKED;
@@ -703,6 +706,7 @@ static int init_ctx(cbm_ctx* c) {
     // mass 1 until set; the last stage's fx, fy, W and pressure
     const int cols[] = {C_VX, C_VY, C_TH, C_OM, C_DIAM, C_LEN, C_MASS, C_FX, C_FY, C_TQ, C_PR};
     for (int k : cols) if ((rc = alloc_column(c, k, 8))) return rc;
+    if (m.rule == CBM_RULE_RODS_GROWTH && (rc = alloc_column(c, C_LT, 8))) return rc;
     std::vector<double> one((size_t)c->C, 1.0);
     for (int k : {C_DIAM, C_LEN, C_MASS})
       for (int b = 0; b < 2; ++b)
@@ -1059,6 +1063,8 @@ static StageArgs stage_base(cbm_ctx* c) {
     const int outs[4] = {C_FX, C_FY, C_TQ, C_PR};
     for (int k = 0; k < 4; ++k) a.rod_out[k] = (double*)c->col[outs[k]].cur();
     a.rod_err = c->flags + 7;
+    a.rod_growth = c->m.rule_p[0];
+    a.rod_alpha = c->m.rule_p[3];
   }
   return a;
 }
@@ -1129,7 +1135,7 @@ static int run_stage(cbm_ctx* c, StageArgs& a, int kind) {
 // Column of DE variable v: positions, then velocities (soft spheres) or vol.
 static int statecol_k(const cbm_ctx* c, int v) {
   if (v < c->dims) return C_X + v;
-  if (c->rods) return v < 4 ? C_VX + (v - 2) : C_TH + (v - 4);
+  if (c->rods) return v < 4 ? C_VX + (v - 2) : (v < 6 ? C_TH + (v - 4) : C_LEN);
   if (c->inertial) return C_VX + (v - c->dims);
   return c->growth && v == c->dims ? C_VOL : C_U0 + (v - c->dims - (c->growth ? 1 : 0));
 }
@@ -1153,6 +1159,13 @@ static RuleArgs rule_args(cbm_ctx* c, int32_t* flag) {
   r.periodic = c->m.periodic ? 1 : 0;
   for (int q = 0; q < 4; ++q) r.rp[q] = c->m.rule_p[q];
   r.dt = c->m.dt; r.seed = c->m.seed; r.step_ptr = c->counters + 1;
+  if (c->rods && c->has_col[C_LT]) {
+    const int rc[RC_N] = {C_VX, C_VY, C_TH, C_OM, C_LEN, C_DIAM, C_MASS, C_LT, C_FX, C_FY, C_TQ, C_PR};
+    for (int j = 0; j < RC_N; ++j) {
+      r.rcol[j] = (double*)c->col[rc[j]].cur();
+      r.rcol_alt[j] = (double*)c->col[rc[j]].alt();
+    }
+  }
   for (int d = 0; d < c->dims; ++d) { r.pos[d] = (double*)c->col[C_X + d].cur(); r.pos_alt[d] = (double*)c->col[C_X + d].alt(); }
   if (c->growth) {
     r.vol = (double*)c->col[C_VOL].cur();
@@ -1803,6 +1816,7 @@ static int field_col(cbm_ctx* c, int32_t field, int& k) {
     case CBM_F_LENGTH: if (!c->rods) return CBM_ERR_ARG; k = C_LEN; return CBM_OK;
     case CBM_F_TORQUE: if (!c->rods) return CBM_ERR_ARG; k = C_TQ; return CBM_OK;
     case CBM_F_PRESSURE: if (!c->rods) return CBM_ERR_ARG; k = C_PR; return CBM_OK;
+    case CBM_F_LTARGET: if (!c->has_col[C_LT]) return CBM_ERR_ARG; k = C_LT; return CBM_OK;
     case CBM_F_FATE: if (!c->inertial) return CBM_ERR_ARG; k = C_FATE; return CBM_OK;
     case CBM_F_U0: case CBM_F_U1: case CBM_F_U2: case CBM_F_U3:
       if (!c->has_col[C_U0]) return CBM_ERR_ARG;

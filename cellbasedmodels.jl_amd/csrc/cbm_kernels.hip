@@ -637,6 +637,10 @@ __device__ __forceinline__ RuleDecision rule_decide(const RuleArgs& a, int sim, 
     d.dead = u53(q.x, q.y) < a.rp[0] * a.dt ? 1 : 0;
   } else if (a.rule == 1) {                // growth and division
     d.add = (a.vol[r] >= a.rp[3] * a.vb[r]) ? 1 : 0;
+  } else if (a.rule == 5) {                // rods2dGrowth: two daughters replace a rod past lTarget
+    const bool div = a.rcol[RC_LEN][r] > a.rcol[RC_LT][r];
+    d.add = div ? 2 : 0;
+    d.dead = div ? 1 : 0;
   }
   return d;
 }
@@ -696,6 +700,42 @@ __global__ __launch_bounds__(256) void k_rule_apply(RuleArgs a) {
       a.id[dst] = st.id_max + 1 + o;
       a.lf[dst] = a.id[r];
       rule_copy_u(a, r, dst);
+      a.slot[dst] = (int32_t)nw;
+      a.dead[dst] = 0;
+    }
+  } else if (a.rule == 5 && d.add) {
+    // src/models/rods.jl:141-160: ww = uniform(-0.1, 0.1); each daughter at
+    // +-(l + d)/4 along the axis, length (l - d)/2, omega +- ww and its own
+    // lTarget = uniform(lMax - sigma, lMax + sigma); the rest copied
+    const int64_t o = a.off_add[s];
+    const int64_t id = a.id[r];
+    const u4 q0 = draw(a.seed, step, id, ri.sim, 0x700u);
+    const u4 q1 = draw(a.seed, step, id, ri.sim, 0x701u);
+    const double ww = (0.1 - (-0.1)) * u53(q0.x, q0.y) + (-0.1);
+    const double lo = a.rp[2] - a.rp[1], hi = a.rp[2] + a.rp[1];
+    const double lt[2] = {(hi - lo) * u53(q0.z, q0.w) + lo, (hi - lo) * u53(q1.x, q1.y) + lo};
+    const double x = a.pos[0][r], y = a.pos[1][r];
+    const double th = a.rcol[RC_TH][r], om = a.rcol[RC_OM][r];
+    const double l = a.rcol[RC_LEN][r], dd = a.rcol[RC_DIAM][r];
+    const double q = (l + dd) / 4;
+    const double cth = cos(th), sth = sin(th);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int64_t nw = N + o + k;
+      if (nw >= nmax) continue;
+      const int64_t dst = sd.row_off + nw;
+#pragma unroll
+      for (int j = 0; j < RC_N; ++j) { const double v = a.rcol[j][r]; a.rcol[j][dst] = v; a.rcol_alt[j][dst] = v; }
+      const double qs = k ? -q : q;
+      const double xn = qs * cth + x, yn = qs * sth + y;
+      a.pos[0][dst] = xn; a.pos_alt[0][dst] = xn;
+      a.pos[1][dst] = yn; a.pos_alt[1][dst] = yn;
+      const double ln = (l - dd) / 2, on = k ? om - ww : om + ww;
+      a.rcol[RC_LEN][dst] = ln; a.rcol_alt[RC_LEN][dst] = ln;
+      a.rcol[RC_OM][dst] = on; a.rcol_alt[RC_OM][dst] = on;
+      a.rcol[RC_LT][dst] = lt[k]; a.rcol_alt[RC_LT][dst] = lt[k];
+      a.id[dst] = st.id_max + 1 + o + k;
+      a.lf[dst] = a.lf[r];
       a.slot[dst] = (int32_t)nw;
       a.dead[dst] = 0;
     }

@@ -352,7 +352,7 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
   // derivative of each DE variable
   constexpr bool PAIRB = general_body<Law>::value;
   constexpr bool UDE = CBM_USER_DE > 0 && (has_de<Law>::value || PAIRB);
-  constexpr int NS = UDE ? MAX_STATE : BUILTIN_STATE;
+  constexpr int NS = (UDE || RODS) ? MAX_STATE : BUILTIN_STATE;
   double du[NS];
   double dudt[4] = {0.0, 0.0, 0.0, 0.0};
   if constexpr (PAIRB) {
@@ -371,7 +371,7 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
     // acc = {fx, fy, W, pressure}; the DE variables x, y, vx, vy, theta,
     // omega (src/models/rods.jl:111-116)
     const double beta = sd.p[5], bw = sd.p[6];
-    const double m = a.mass[i], d = a.rod_d[i], l = a.rod_l[i];
+    const double m = a.mass[i], d = a.rod_d[i], l = (a.nvar == 7 ? a.S[6] : a.rod_l)[i];
     const double vx = a.S[2][i], vy = a.S[3][i], om = a.S[5][i];
     du[0] = vx;
     du[1] = vy;
@@ -379,6 +379,9 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
     du[3] = -beta * vy + acc[1] / m;
     du[4] = om;
     du[5] = acc[2] / (m * ((d + l) * (d + l)) / 12 + m * (d * d)) - bw * om;
+    // rods2dGrowth (src/models/rods.jl:137-139): the length grows, slowed by
+    // this evaluation's pressure
+    if (a.nvar == 7) du[6] = a.rod_growth / (1 + a.rod_alpha * acc[3]);
 #pragma unroll
     for (int k = 0; k < 4; ++k) a.rod_out[k][i] = acc[k];
   } else if constexpr (SOFT) {
@@ -431,7 +434,7 @@ __device__ __forceinline__ void stage_finish(const StageArgs& a, const SimDev& s
   }
 
   const double dt = a.dt;
-  const int nv = UDE ? CBM_DE_BASE + CBM_USER_DE : (RODS ? 6 : (SOFT ? 2 * D : (grow ? D + 1 : D)));
+  const int nv = UDE ? CBM_DE_BASE + CBM_USER_DE : (RODS ? a.nvar : (SOFT ? 2 * D : (grow ? D + 1 : D)));
   double res[NS];
   if (NOISE && !same_type<Noise, ConstNoise>::value) {
     // a user agentSDE body: g of every DE variable at the stage state, one
@@ -629,6 +632,7 @@ __device__ __forceinline__ void rod_stage_body(const StageArgs& a) {
   const int64_t i = ri.row;
   if (ri.local >= N) return;
   const double kn = sd.p[0], gn = sd.p[1], gt = sd.p[2], mucc = sd.p[3];
+  const double* __restrict__ lcol = a.nvar == 7 ? a.S[6] : a.rod_l;   // l is a DE variable with growth
   auto rod_at = [&](int64_t j) {
     RodState r;
     r.a.x = a.S[0][j];
@@ -636,7 +640,7 @@ __device__ __forceinline__ void rod_stage_body(const StageArgs& a) {
     r.vx = a.S[2][j];
     r.vy = a.S[3][j];
     sincos(a.S[4][j], &r.a.s, &r.a.c);
-    r.a.l = a.rod_l[j];
+    r.a.l = lcol[j];
     r.d = a.rod_d[j];
     r.m = a.mass[j];
     return r;

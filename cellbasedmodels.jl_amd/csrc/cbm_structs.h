@@ -170,6 +170,7 @@ struct StageArgs {
   const double* rod_l;
   double* rod_out[4];           // rods: fx, fy, W, pressure of this stage
   int32_t* rod_err;             // rods: set when a contact's overlap d_i - r_ij is negative
+  double rod_growth, rod_alpha;  // rods with CBM_RULE_RODS_GROWTH: dt(l) = growth/(1 + alpha pressure)
   const double* upar[USER_PARAMS];   // user agent parameter columns U0..U7 (null when absent)
   const int64_t* id;
   const int32_t* list;          // ELL Verlet list, entry q of row i at q*ldl + i
@@ -271,6 +272,9 @@ __host__ __device__ inline int64_t flat_lds_bytes(int D, int chunk, int cap) {
 // additions in a step| < 2^31.
 constexpr double MEDIUM_ADD_SCALE = 4294967296.0;
 
+// The rod columns of RuleArgs::rcol (CBM_RULE_RODS_GROWTH).
+enum RodCol { RC_VX = 0, RC_VY, RC_TH, RC_OM, RC_LEN, RC_DIAM, RC_MASS, RC_LT, RC_FX, RC_FY, RC_TQ, RC_PR, RC_N };
+
 struct RuleArgs {
   RowGeom g;
   int32_t rule, dims, lagged, growth;
@@ -315,6 +319,10 @@ struct RuleArgs {
   const double* nb_x[3];
   const double* nb_vol;
   const double* nb_u[USER_PARAMS];
+  // rods (CBM_RULE_RODS_GROWTH): the rod columns a daughter takes from its
+  // parent (both buffers), RC_* order
+  double* rcol[RC_N];
+  double* rcol_alt[RC_N];
   // the medium at the agent's node (cellInMesh of its step-start position,
   // src/AgentStructure/auxiliar.jl:353-365): read from med_c (the field at
   // the step start, NULL without a medium); additions are summed per node

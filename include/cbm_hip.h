@@ -142,13 +142,22 @@ enum cbm_medium_border { CBM_BORDER_MIRROR = 0, CBM_BORDER_DIRICHLET = 1, CBM_BO
  *   CBM_RULE_CUSTOM  a user agentRule body compiled by cbm_compile_model
  *     (required before stepping).  The agents carry vol, growth, vbirth
  *     (dvol/dt = growth*vol, as for CBM_RULE_GROWTH_DIVISION), lf and the
-  *     user parameters CBM_F_U0..U7 for the body to read and write.       */
+  *     user parameters CBM_F_U0..U7 for the body to read and write.
+ *   CBM_RULE_RODS_GROWTH  rule_p = {growth, sigma_lTarget, lMax, alpha}:
+ *     rods2dGrowth (src/models/rods.jl:123-164) on CBM_LAW_RODS.  The length
+ *     l becomes a DE variable, dt(l) = growth/(1 + alpha*pressure) with the
+ *     stage's pressure; a rod with l > lTarget (CBM_F_LTARGET) is replaced by
+ *     two daughters at +-(l + d)/4 along its axis with length (l - d)/2,
+ *     omega +- ww, ww uniform in [-0.1, 0.1), and lTarget uniform in
+ *     [lMax - sigma, lMax + sigma) each (Philox keyed seed, step, id, sim,
+ *     0x700/0x701); every other field is the parent's.                    */
 enum cbm_rule {
   CBM_RULE_NONE = 0,
   CBM_RULE_GROWTH_DIVISION = 1,
   CBM_RULE_PARITY = 2,
   CBM_RULE_DEATH = 3,
-  CBM_RULE_CUSTOM = 4
+  CBM_RULE_CUSTOM = 4,
+  CBM_RULE_RODS_GROWTH = 5
 };
 
 /* ------------------------------------------------------ per-agent fields */
@@ -167,7 +176,8 @@ enum cbm_field {
   CBM_F_DIAMETER = 23, CBM_F_LENGTH = 24,      /* rods: per-rod geometry                    */
   CBM_F_TORQUE = 25, CBM_F_PRESSURE = 26,      /* rods: W and pressure of the last stage    */
   CBM_F_U4 = 27, CBM_F_U5 = 28, CBM_F_U6 = 29, CBM_F_U7 = 30,   /* user agent parameters U4..U7 */
-  CBM_F_COUNT = 31
+  CBM_F_LTARGET = 31,                          /* rods with CBM_RULE_RODS_GROWTH: division length */
+  CBM_F_COUNT = 32
 };
 
 /* --------------------------------------------------- model description */

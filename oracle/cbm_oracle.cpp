@@ -149,6 +149,8 @@ struct Orc {
   int32_t sim = 0;                            // its index in a batched context: the noise streams' key
   std::vector<double> TH, OM, rodD, rodL;     // rods: angle, its rate, diameter, length
   std::vector<double> rodW, rodP;             // rods: torque and pressure of the last evaluation
+  std::vector<double> LT;                     // rods with CBM_RULE_RODS_GROWTH: division length lTarget
+  bool rods_growth = false;                   // ... and l is the seventh DE variable
   double noise_slope = 0.0;                   // state-dependent agentSDE g_a(x) = noise + slope x_a (positions)
   std::vector<double> X[3], F[3], Fnew[3];
   std::vector<double> V[3], rad, mass;        // velocities, radius, mass (soft spheres)
@@ -491,9 +493,11 @@ static int rod_deriv(const Orc& o, int64_t i, const std::vector<double>* S, doub
   const double* p = o.s.p;
   double fx = 0, fy = 0, W = 0, pressure = 0;
   int bad = 0;
-  const double ri[8] = {S[0][i], S[1][i], o.rodD[i], o.rodL[i], S[4][i], S[2][i], S[3][i], o.mass[i]};
+  // with growth the length is the DE variable S[6] (its stage value)
+  const std::vector<double>& L = o.rods_growth ? S[6] : o.rodL;
+  const double ri[8] = {S[0][i], S[1][i], o.rodD[i], L[i], S[4][i], S[2][i], S[3][i], o.mass[i]};
   for_candidates(o, i, [&](int64_t j) {
-    const double rj[8] = {S[0][j], S[1][j], o.rodD[j], o.rodL[j], S[4][j], S[2][j], S[3][j], o.mass[j]};
+    const double rj[8] = {S[0][j], S[1][j], o.rodD[j], L[j], S[4][j], S[2][j], S[3][j], o.mass[j]};
     double f[3];
     if (rod_forces(ri, rj, p, f)) bad = -1;
     fx += f[0];
@@ -501,7 +505,7 @@ static int rod_deriv(const Orc& o, int64_t i, const std::vector<double>* S, doub
     W += f[2];
     pressure += std::sqrt(f[0] * f[0] + f[1] * f[1]);
   });
-  const double m = o.mass[i], d = o.rodD[i], l = o.rodL[i];
+  const double m = o.mass[i], d = o.rodD[i], l = L[i];
   const double vx = S[2][i], vy = S[3][i], w = S[5][i];
   du[0] = vx;
   du[1] = vy;
@@ -509,6 +513,8 @@ static int rod_deriv(const Orc& o, int64_t i, const std::vector<double>* S, doub
   du[3] = -p[5] * vy + fy / m;
   du[4] = w;
   du[5] = W / (m * ((d + l) * (d + l)) / 12 + m * (d * d)) - p[6] * w;
+  // rods2dGrowth: dt(l) = growth/(1+alpha*pressure) (src/models/rods.jl:137-139)
+  if (o.rods_growth) du[6] = o.m.rule_p[0] / (1 + o.m.rule_p[3] * pressure);
   acc[0] = fx; acc[1] = fy; acc[2] = W; acc[3] = pressure;
   return bad;
 }
@@ -920,6 +926,33 @@ static void apply_rule(Orc& o, std::vector<int64_t>& holes, int64_t& nAdd) {
     } else if (o.m.rule == CBM_RULE_DEATH) {
       U4 r = draw(o.m.seed, o.step, o.id[i], o.sim, 0x400u);
       if (u53(r.v[0], r.v[1]) < o.m.rule_p[0] * o.m.dt) { holes.push_back(i); o.alive[i] = 0; }
+    } else if (o.m.rule == CBM_RULE_RODS_GROWTH) {
+      // rods2dGrowth's agentRule (src/models/rods.jl:141-160): a rod past
+      // its lTarget is removed and two daughters are appended
+      if (!(o.rodL[i] > o.LT[i])) continue;
+      holes.push_back(i); o.alive[i] = 0;
+      const U4 q0 = draw(o.m.seed, o.step, o.id[i], o.sim, 0x700u);
+      const U4 q1 = draw(o.m.seed, o.step, o.id[i], o.sim, 0x701u);
+      const double ww = (0.1 - (-0.1)) * u53(q0.v[0], q0.v[1]) + (-0.1);      // uniform(-.1, .1)
+      const double l0 = o.m.rule_p[2] - o.m.rule_p[1], l1 = o.m.rule_p[2] + o.m.rule_p[1];
+      const double lt[2] = {(l1 - l0) * u53(q0.v[2], q0.v[3]) + l0, (l1 - l0) * u53(q1.v[0], q1.v[1]) + l0};
+      const double x = o.X[0][i], y = o.X[1][i], th = o.TH[i], om = o.OM[i], l = o.rodL[i], d = o.rodD[i];
+      for (int k = 0; k < 2; ++k) {
+        const int64_t nw = N + nAdd;
+        if (nw >= o.nMax) { o.err = CBM_ERR_CAPACITY; o.msg = "agent capacity exceeded"; continue; }
+        const int64_t newId = o.idMax + 1 + nAdd;
+        nAdd++;
+        for (int a = 0; a < 2; ++a) { o.V[a][nw] = o.V[a][i]; o.F[a][nw] = o.F[a][i]; o.Fnew[a][nw] = o.F[a][i]; }
+        o.TH[nw] = th; o.rodD[nw] = d; o.mass[nw] = o.mass[i]; o.rodW[nw] = o.rodW[i]; o.rodP[nw] = o.rodP[i];
+        // x = (l+d)/4*cos(theta)+x, the second daughter -(l+d)/4*cos(theta)+x
+        const double q = k ? -((l + d) / 4) : (l + d) / 4;
+        o.X[0][nw] = q * std::cos(th) + x;
+        o.X[1][nw] = q * std::sin(th) + y;
+        o.rodL[nw] = (l - d) / 2;
+        o.OM[nw] = k ? om - ww : om + ww;
+        o.LT[nw] = lt[k];
+        o.id[nw] = newId; o.lf[nw] = o.lf[i]; o.alive[nw] = 1;
+      }
     } else if (o.m.rule == CBM_RULE_GROWTH_DIVISION) {
       if (!(o.vol[i] >= o.m.rule_p[3] * o.vb[i])) continue;
       int64_t nw = N + nAdd;
@@ -974,6 +1007,11 @@ static void compact(Orc& o, const std::vector<int64_t>& holes, int64_t nAdd) {
       o.id[dst] = o.id[src]; o.lf[dst] = o.lf[src];
       for (int k = 0; k < 8; ++k) o.U[k][dst] = o.U[k][src];
       if (o.growth) { o.vol[dst] = o.vol[src]; o.gr[dst] = o.gr[src]; o.vb[dst] = o.vb[src]; o.grNew[dst] = o.grNew[src]; o.vbNew[dst] = o.vbNew[src]; }
+      if (o.rods) {
+        for (int a = 0; a < 2; ++a) o.V[a][dst] = o.V[a][src];
+        o.TH[dst] = o.TH[src]; o.OM[dst] = o.OM[src]; o.rodD[dst] = o.rodD[src]; o.rodL[dst] = o.rodL[src];
+        o.mass[dst] = o.mass[src]; o.LT[dst] = o.LT[src]; o.rodW[dst] = o.rodW[src]; o.rodP[dst] = o.rodP[src];
+      }
     }
   }
   o.N = top - R;
@@ -990,11 +1028,12 @@ static void de_step(Orc& o, int64_t nA) {
   const bool body = o.law_ode != nullptr;
   const int ude = body ? o.m.user_de : 0;
   const int base = d + (o.growth ? 1 : 0);
-  const int nv = o.rods ? 6 : (o.inertial ? 2 * d : base + ude);
+  const int nv = o.rods ? (o.rods_growth ? 7 : 6) : (o.inertial ? 2 * d : base + ude);
   std::vector<double> U[8];
   for (int a = 0; a < d; ++a) U[a] = o.X[a];
   if (o.inertial || o.rods) for (int a = 0; a < d; ++a) U[d + a] = o.V[a];
   if (o.rods) { U[4] = o.TH; U[5] = o.OM; }
+  if (o.rods_growth) U[6] = o.rodL;
   if (o.growth) U[d] = o.vol;
   for (int k = 0; k < ude; ++k) U[base + k] = o.U[k];
   std::vector<char> bad(o.rods ? nA : 0, 0);
@@ -1012,7 +1051,7 @@ static void de_step(Orc& o, int64_t nA) {
       if (o.rods) {
         // every evaluation leaves its fx, fy, W and pressure in the agent
         if (rod_deriv(o, i, S, du, acc)) bad[i] = 1;
-        for (int v = 0; v < 6; ++v) K[v][i] = du[v];
+        for (int v = 0; v < nv; ++v) K[v][i] = du[v];
         o.Fnew[0][i] = acc[0]; o.Fnew[1][i] = acc[1]; o.rodW[i] = acc[2]; o.rodP[i] = acc[3];
         continue;
       }
@@ -1102,6 +1141,7 @@ static void de_step(Orc& o, int64_t nA) {
   if (o.rods) {
     std::copy(U[4].begin(), U[4].begin() + nA, o.TH.begin());
     std::copy(U[5].begin(), U[5].begin() + nA, o.OM.begin());
+    if (o.rods_growth) std::copy(U[6].begin(), U[6].begin() + nA, o.rodL.begin());
     if (std::find(bad.begin(), bad.end(), 1) != bad.end()) {
       o.err = CBM_ERR_NONFINITE;
       o.msg = "rod contact with a negative overlap d_i - r_ij";
@@ -1252,7 +1292,9 @@ void* orc_create(const cbm_model_desc* m, const cbm_sim_desc* s) {
   o->inertial = (m->law == CBM_LAW_SOFT_SPHERES);
   if (o->inertial && m->rule != CBM_RULE_NONE) { o->err = CBM_ERR_CONFIG; o->msg = "soft spheres take no rules"; }
   o->rods = (m->law == CBM_LAW_RODS);
-  if (o->rods && (m->dims != 2 || m->rule != CBM_RULE_NONE || m->periodic || m->lagged || m->medium_n[0] > 0 ||
+  o->rods_growth = o->rods && m->rule == CBM_RULE_RODS_GROWTH;
+  if ((m->rule == CBM_RULE_RODS_GROWTH) != o->rods_growth) { o->err = CBM_ERR_CONFIG; o->msg = "CBM_RULE_RODS_GROWTH needs the rod model"; }
+  if (o->rods && (m->dims != 2 || (m->rule != CBM_RULE_NONE && !o->rods_growth) || m->periodic || m->lagged || m->medium_n[0] > 0 ||
                   m->drift != 0.0 || !(m->integrator == CBM_EULER || m->integrator == CBM_HEUN || m->integrator == CBM_RK4))) {
     o->err = CBM_ERR_CONFIG;
     o->msg = "the rod model is 2-D, without rules, periodic box, lagged forces, medium or drift; Euler, Heun or RK4";
@@ -1263,7 +1305,7 @@ void* orc_create(const cbm_model_desc* m, const cbm_sim_desc* s) {
   }
   o->rad.assign(o->nMax, 1.0); o->mass.assign(o->nMax, 1.0); o->fate.assign(o->nMax, 1);
   o->TH.assign(o->nMax, 0.0); o->OM.assign(o->nMax, 0.0); o->rodD.assign(o->nMax, 1.0); o->rodL.assign(o->nMax, 1.0);
-  o->rodW.assign(o->nMax, 0.0); o->rodP.assign(o->nMax, 0.0);
+  o->rodW.assign(o->nMax, 0.0); o->rodP.assign(o->nMax, 0.0); o->LT.assign(o->nMax, 0.0);
   o->id.resize(o->nMax); o->lf.assign(o->nMax, 0); o->alive.assign(o->nMax, 1);
   for (int64_t i = 0; i < o->nMax; ++i) o->id[i] = i + 1;
   o->vol.assign(o->nMax, 1.0); o->gr.assign(o->nMax, 0.0); o->vb.assign(o->nMax, 1.0);
@@ -1330,6 +1372,7 @@ int orc_set(void* h, int field, const void* data, int64_t n) {
       case CBM_F_LENGTH: o.rodL[i] = v; break;
       case CBM_F_TORQUE: o.rodW[i] = v; break;
       case CBM_F_PRESSURE: o.rodP[i] = v; break;
+      case CBM_F_LTARGET: o.LT[i] = v; break;
       default: return CBM_ERR_ARG;
     }
   }
@@ -1362,6 +1405,7 @@ int orc_get(void* h, int field, void* out, int64_t n) {
       case CBM_F_LENGTH: dv[i] = o.rodL[i]; break;
       case CBM_F_TORQUE: dv[i] = o.rodW[i]; break;
       case CBM_F_PRESSURE: dv[i] = o.rodP[i]; break;
+      case CBM_F_LTARGET: dv[i] = o.LT[i]; break;
       default: return CBM_ERR_ARG;
     }
   }

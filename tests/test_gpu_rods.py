@@ -178,3 +178,47 @@ def test_rods_python_mirror(gpu_available):
         e_dev = float(np.max(np.abs(com[name] - o.get(f))))
         e_nat = float(np.max(np.abs(o2.get(f) - o.get(f))))
         assert e_dev <= 100.0 * e_nat + 1e-13 * max(1.0, float(np.max(np.abs(o.get(f))))), (name, e_dev, e_nat)
+
+
+G0 = [0.5, 0.3, 3.0, 0.0]      # growth, sigma_lTarget, lMax, alpha = 0: division times independent of contacts
+G1 = [0.5, 0.3, 3.0, 0.8]
+
+
+def growth_pair(g, neigh, integ, n0=120, steps_seed=21):
+    L = 60.0
+    kw = dict(cell_edge=7.0) if neigh == A.CELL_LINKED else {}
+    m = model(2, integrator=integ, neighbors=neigh, law=A.LAW_RODS, dt=0.01, rule=A.RULE_RODS_GROWTH, rule_p=g,
+              seed=13, **kw)
+    s = sim(n0, [[0, L]] * 2, P, n_max=2000)
+    rng = np.random.default_rng(steps_seed)
+    ag = {A.F_X: rng.uniform(15, 45, n0), A.F_Y: rng.uniform(15, 45, n0), A.F_THETA: rng.uniform(-3, 3, n0),
+          A.F_LENGTH: rng.uniform(1.0, 2.9, n0), A.F_LTARGET: rng.uniform(2.7, 3.3, n0)}
+    return m, s, ag
+
+
+@pytest.mark.parametrize("neigh,integ", [(A.FULL, A.EULER), (A.CELL_LINKED, A.HEUN), (A.CELL_LINKED, A.RK4)])
+def test_rods_growth_colony(gpu_available, neigh, integ):
+    # alpha = 0: the lengths, the division steps, ids and lTargets are exact;
+    # positions and angles within the conditioning-scaled bound
+    m, s, ag = growth_pair(G0, neigh, integ)
+    o, g = both(m, s, ag)
+    for chunk in (50, 150):
+        o.step(chunk)
+        g.step(chunk)
+        n = o.n
+        assert g.n() == n and o.error() == 0
+        for f in (A.F_ID, A.F_LENGTH, A.F_LTARGET):
+            assert np.array_equal(g.download(f, n), o.get(f)), f
+    assert o.n > s.n + 40                       # about half of the rods divided, some daughters again
+    assert_close(o, g, nudged(m, s, ag, 200), "step 200", fields=(A.F_X, A.F_Y, A.F_THETA, A.F_OMEGA, A.F_VX))
+
+
+def test_rods_growth_with_pressure(gpu_available):
+    # alpha > 0: contacts slow growth; the lengths too follow the tolerance
+    m, s, ag = growth_pair(G1, A.CELL_LINKED, A.HEUN, steps_seed=22)
+    o, g = both(m, s, ag)
+    o.step(60)
+    g.step(60)
+    assert g.n() == o.n and np.array_equal(g.download(A.F_ID, o.n), o.get(A.F_ID))
+    assert o.n > s.n
+    assert_close(o, g, nudged(m, s, ag, 60), "step 60", fields=STATE + (A.F_LENGTH, A.F_PRESSURE))

@@ -273,3 +273,78 @@ def test_rod_model_refusals():
         dims = kw.pop("dims", 2)
         o = O.Oracle(model(dims, law=A.LAW_RODS, **kw), sim(2, [[0, 10]] * dims, P))
         assert o.error() != 0
+
+
+# ------------------------------------------------------------ rods2dGrowth
+G = [0.4, 0.25, 3.0, 0.8]     # growth, sigma_lTarget, lMax, alpha
+
+
+def growing(n, integ=A.EULER, neigh=A.FULL, dt=0.01, g=G, **kw):
+    m = model(2, integrator=integ, neighbors=neigh, law=A.LAW_RODS, dt=dt, rule=A.RULE_RODS_GROWTH, rule_p=g, **kw)
+    return O.Oracle(m, sim(n, [[0, 40]] * 2, P, n_max=4 * n + 8))
+
+
+def test_free_rod_grows_linearly_and_divides_past_its_target():
+    # no contacts, no pressure: dt(l) = growth = 0.5, dt = 1/8, so l gains
+    # exactly 1/16 a step; the rule divides only when l > lTarget
+    g = [0.5, 0.25, 3.0, 0.8]
+    o = growing(1, dt=0.125, g=g)
+    o.set(A.F_X, [20.0]); o.set(A.F_Y, [20.0]); o.set(A.F_LENGTH, [1.0]); o.set(A.F_LTARGET, [3.0])
+    assert o.init() == 0
+    o.step(32)
+    assert o.n == 1 and o.error() == 0
+    assert o.get(A.F_LENGTH)[0] == 3.0
+    o.step(1)                                  # l == lTarget: no division
+    assert o.n == 1 and o.get(A.F_LENGTH)[0] == 3.0625
+    o.step(1)                                  # l > lTarget
+    assert o.n == 2
+
+
+def test_division_places_two_daughters_along_the_axis():
+    th, l, d, om = 0.7, 3.2, 0.9, 0.3
+    o = growing(1)
+    o.set(A.F_X, [20.0]); o.set(A.F_Y, [10.0]); o.set(A.F_THETA, [th]); o.set(A.F_OMEGA, [om])
+    o.set(A.F_LENGTH, [l]); o.set(A.F_DIAMETER, [d]); o.set(A.F_LTARGET, [3.0]); o.set(A.F_VX, [0.2])
+    assert o.init() == 0
+    o.step(1)
+    assert o.n == 2 and sorted(o.get(A.F_ID, 2)) == [2, 3]
+    ids = list(o.get(A.F_ID, 2))
+    a, b = ids.index(2), ids.index(3)
+    q = (l + d) / 4
+    x, y = o.get(A.F_X), o.get(A.F_Y)
+    close([x[a], y[a]], [20.0 + q * math.cos(th), 10.0 + q * math.sin(th)], 1e-15)
+    close([x[b], y[b]], [20.0 - q * math.cos(th), 10.0 - q * math.sin(th)], 1e-15)
+    assert np.all(o.get(A.F_LENGTH) == (l - d) / 2)
+    w = o.get(A.F_OMEGA)
+    assert abs((w[a] - om) + (w[b] - om)) < 1e-16 and abs(w[a] - om) <= 0.1
+    lt = o.get(A.F_LTARGET)
+    assert np.all((lt >= G[2] - G[1]) & (lt < G[2] + G[1])) and lt[a] != lt[b]
+    assert np.all(o.get(A.F_VX) == 0.2) and np.all(o.get(A.F_THETA) == th)
+
+
+def test_division_draws_are_uniform():
+    # ww uniform on [-0.1, 0.1) and lTarget on [lMax - sigma, lMax + sigma)
+    from scipy import stats
+    n = 3000
+    o = growing(n, dt=0.01)
+    rng = np.random.default_rng(9)
+    o.set(A.F_X, rng.uniform(0, 40, n) * 1e3); o.set(A.F_Y, np.zeros(n))   # far apart: no contacts
+    o.set(A.F_LENGTH, np.full(n, 2.0)); o.set(A.F_LTARGET, np.full(n, 1.0)); o.set(A.F_OMEGA, np.zeros(n))
+    assert o.init() == 0
+    o.step(1)
+    assert o.n == 2 * n
+    w = o.get(A.F_OMEGA)
+    assert np.all(w[0::2] + w[1::2] == 0.0) or np.allclose(np.sort(w), -np.sort(w)[::-1])
+    assert stats.kstest(np.abs(w), stats.uniform(0, 0.1).cdf).pvalue > 1e-3
+    lt = o.get(A.F_LTARGET)
+    assert stats.kstest(lt, stats.uniform(G[2] - G[1], 2 * G[1]).cdf).pvalue > 1e-3
+
+
+def test_pressure_slows_growth():
+    # two overlapping parallel rods: dt(l) = growth/(1 + alpha p) with p = |F|
+    o = growing(2, dt=0.01)
+    o.set(A.F_X, [5.0, 5.0]); o.set(A.F_Y, [5.0, 5.8]); o.set(A.F_LTARGET, [9.0, 9.0])
+    assert o.init() == 0
+    o.step(1)
+    p = P[0] * 0.2 ** 1.5                      # kn delta^1.5 at the start of the step
+    close(o.get(A.F_LENGTH), [1.0 + 0.01 * G[0] / (1 + G[3] * p)] * 2, 1e-14)
