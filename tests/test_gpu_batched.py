@@ -102,3 +102,29 @@ def test_autotuner_choices_and_no_state_change(gpu_available, tmp_path):
     o.step(7)
     for a in range(2):
         assert np.array_equal(g.download(a, sims[1][0].n, sim=1), o.get(a))
+
+
+def test_c4_sweep_sample_equals_its_oracles(gpu_available):
+    # 128 simulations drawn as the C4 sweep draws its 512 (N log-uniform in
+    # [500, 50000], packing 0.2 to 0.5, k and adhesion per simulation, 2-D
+    # periodic, Euler, CellLinked 2.4): 30 steps, every simulation bit for bit
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    m, sims, states = bench.sweep_workload(n_sims=128, seed=6)
+    print(f"C4 sample: {len(sims)} simulations, {sum(s.n for s in sims)} agents, 30 steps", flush=True)
+    g = Context(m, sims)
+    for q, st in enumerate(states):
+        for a in range(2):
+            g.upload(a, st[a], sim=q)
+    g.neighbors()
+    g.step(30)
+    for q, (s, st) in enumerate(zip(sims, states)):
+        o = O.Oracle(m, s)
+        for a in range(2):
+            o.set(a, st[a])
+        assert o.init() == 0
+        o.step(30)
+        for a in range(2):
+            assert np.array_equal(g.download(a, s.n, sim=q), o.get(a)), (q, a)
