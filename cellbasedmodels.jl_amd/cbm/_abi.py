@@ -100,6 +100,7 @@ SIGNATURES = {
     "cbm_get_id_max": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64)]),
     "cbm_get_rebuilds": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "cbm_get_bricks": (C.c_int, [_P, _P, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "cbm_get_sub_bricks": (C.c_int, [_P, _P, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
     "cbm_get_neighbor_lists": (C.c_int, [_P, C.c_int32, C.c_double, _P, _P, C.c_int64]),
     "cbm_count_neighbors": (C.c_int, [_P, C.c_int32, C.c_double, _P]),
     "cbm_count_neighbors_noalloc": (C.c_int, [_P, C.c_int32, C.c_double, _P]),

@@ -428,6 +428,13 @@ class Context:
         self._chk(self.lib.cbm_get_bricks(self.h, bd.ctypes.data, C.byref(cap), C.byref(nb), C.byref(over)))
         return {"bdim": bd.tolist(), "cap": cap.value, "nbricks": nb.value, "overflows": over.value}
 
+    def sub_bricks(self):
+        """The sub-brick table of a CellLinked context with rules: {'bdim', 'cap', 'nbricks'}."""
+        bd = np.zeros(3, dtype=np.int32)
+        cap, nb = C.c_int32(), C.c_int64()
+        self._chk(self.lib.cbm_get_sub_bricks(self.h, bd.ctypes.data, C.byref(cap), C.byref(nb)))
+        return {"bdim": bd.tolist(), "cap": cap.value, "nbricks": nb.value}
+
     def neighbor_lists(self, radius, cap, sim=0):
         n = self.n(sim)
         counts = np.zeros(max(n, 1), dtype=np.int32)

@@ -155,6 +155,11 @@ struct cbm_ctx {
   int bdim[3] = {1, 1, 1};
   int brick_cap = 0;                      // agents staged per brick block
   size_t brick_lds = 0;                   // its LDS bytes
+  int32_t* bricks2 = nullptr;             // sub-bricks for dense regions (StageArgs::brick2)
+  int64_t nbricks2 = 0;
+  int bdim2[3] = {1, 1, 1};
+  int brick_cap2 = 0;
+  size_t brick_lds2 = 0;
 
   // adjoint work buffers, kept across calls and grown on demand
   struct DevBuf { void* p = nullptr; size_t bytes = 0; };
@@ -364,53 +369,11 @@ static void morton_ranks(const int32_t* gn, int dims, std::vector<int32_t>& out)
   for (int64_t r = 0; r < nc; ++r) out[base + (size_t)key[(size_t)r].second] = (int32_t)r;
 }
 
-// Bricks of the cell-linked stage: 2^m cells, Morton-aligned (x takes the
-// extra bits first, so a brick's cells have consecutive Morton ranks and
-// its rows are one range).  m is the largest size whose expected population
-// (the densest simulation's mean density; capacity when rules change the
-// population) stays within ~1.25 blocks of 256 threads, lowered while the
-// LDS for 1.2x the expected halo population would exceed 64 KiB (a denser
-// brick is left to the row-parallel stage kernel).  Bricks
-// are listed per simulation in Morton order of their coordinates.
-static int setup_bricks(cbm_ctx* c) {
+// The brick table for edges bd: per simulation, its bricks in Morton order
+// of their coordinates, BRICK_WORDS each (simulation, origin cell, bucket
+// range of its cells).
+static void brick_table(cbm_ctx* c, const int* bd, std::vector<int32_t>& tab) {
   const int D = c->dims;
-  const bool soft = c->inertial;
-  double rho = 0.0;
-  for (int s = 0; s < c->S; ++s) {
-    const double n = (double)std::max<int64_t>(c->rules ? c->sims[s].n_max : c->sims[s].n, 1);
-    rho = std::max(rho, n / (double)c->h_sims[s].ncells);
-  }
-  const int mmin = D, mmax = D == 3 ? 9 : 10;
-  int m = mmin;
-  while (m < mmax && rho * (double)(1 << (m + 1)) <= 320.0) ++m;
-  const int64_t budget = 64 * 1024;
-  // LDS bytes per staged agent and in all of a tile: the Verlet list build
-  // stages positions and slots, the cell-linked stage positions (and radius
-  // and fate for soft spheres); both add an owner and a home-row index
-  const bool build = c->verlet || c->clv;   // the bricks serve the list build
-  const int64_t per = build ? 8 * D + 4 + 4 : 8 * (D + (soft ? 1 : 0)) + (soft ? 4 : 0) + 4;
-  auto tile_bytes = [&](int cp, int H) {
-    return build ? vbrick_lds_bytes(D, cp, H) : brick_lds_bytes(D, soft, cp, H);
-  };
-  int bd[3] = {1, 1, 1};
-  int cap = 0;
-  for (;; --m) {
-    bd[0] = 1 << ((m + D - 1) / D);
-    bd[1] = D > 1 ? 1 << ((m + D - 2) / D) : 1;
-    bd[2] = D > 2 ? 1 << ((m + D - 3) / D) : 1;
-    const int H = brick_halo_cells(D, bd[0], bd[1], bd[2]);
-    const int64_t fixed = tile_bytes(0, H) + 16;
-    const int64_t capmax = (budget - fixed) / per;
-    const double want = 1.2 * rho * H + 32.0;
-    if (want <= (double)capmax || m == mmin) {
-      cap = (int)std::max<int64_t>(1, std::min<int64_t>(capmax, (int64_t)std::max(256.0, want)));
-      break;
-    }
-  }
-  for (int a = 0; a < 3; ++a) c->bdim[a] = bd[a];
-  c->brick_cap = cap;
-  c->brick_lds = (size_t)tile_bytes(cap, brick_halo_cells(D, bd[0], bd[1], bd[2]));
-  std::vector<int32_t> tab;
   for (int s = 0; s < c->S; ++s) {
     const SimDev& sd = c->h_sims[s];
     int64_t nb[3];
@@ -445,10 +408,89 @@ static int setup_bricks(cbm_ctx* c) {
       tab.push_back((int32_t)(sd.cell_off + lo + cnt));
     }
   }
+}
+
+// Bricks of the cell-linked stage: 2^m cells, Morton-aligned (x takes the
+// extra bits first, so a brick's cells have consecutive Morton ranks and
+// its rows are one range).  m is the largest size whose expected population
+// (the densest simulation's mean density; capacity when rules change the
+// population) stays within ~1.25 blocks of 256 threads, lowered while the
+// LDS for 1.2x the expected halo population would exceed 64 KiB (a denser
+// brick is left to the row-parallel stage kernel).  Bricks
+// are listed per simulation in Morton order of their coordinates.
+static int setup_bricks(cbm_ctx* c) {
+  const int D = c->dims;
+  const bool soft = c->inertial;
+  double rho = 0.0;
+  for (int s = 0; s < c->S; ++s) {
+    const double n = (double)std::max<int64_t>(c->rules ? c->sims[s].n_max : c->sims[s].n, 1);
+    rho = std::max(rho, n / (double)c->h_sims[s].ncells);
+  }
+  const int mmin = D, mmax = D == 3 ? 9 : 10;
+  int m = mmin;
+  while (m < mmax && rho * (double)(1 << (m + 1)) <= 320.0) ++m;
+  const int64_t budget = 64 * 1024;
+  // LDS bytes per staged agent and in all of a tile: the Verlet list build
+  // stages positions and slots, the cell-linked stage positions (and radius
+  // and fate for soft spheres); both add an owner and a home-row index
+  const bool build = c->verlet || c->clv;   // the bricks serve the list build
+  const int64_t per = build ? 8 * D + 4 + 4 : 8 * (D + (soft ? 1 : 0)) + (soft ? 4 : 0) + 4;
+  auto tile_bytes = [&](int cp, int H) {
+    return build ? vbrick_lds_bytes(D, cp, H) : brick_lds_bytes(D, soft, cp, H);
+  };
+  int bd[3] = {1, 1, 1};
+  int cap = 0;
+  auto edges = [&](int mm, int* e) {
+    e[0] = 1 << ((mm + D - 1) / D);
+    e[1] = D > 1 ? 1 << ((mm + D - 2) / D) : 1;
+    e[2] = D > 2 ? 1 << ((mm + D - 3) / D) : 1;
+  };
+  for (;; --m) {
+    edges(m, bd);
+    const int H = brick_halo_cells(D, bd[0], bd[1], bd[2]);
+    const int64_t fixed = tile_bytes(0, H) + 16;
+    const int64_t capmax = (budget - fixed) / per;
+    const double want = 1.2 * rho * H + 32.0;
+    if (want <= (double)capmax || m == mmin) {
+      cap = (int)std::max<int64_t>(1, std::min<int64_t>(capmax, (int64_t)std::max(256.0, want)));
+      break;
+    }
+  }
+  for (int a = 0; a < 3; ++a) c->bdim[a] = bd[a];
+  c->brick_cap = cap;
+  c->brick_lds = (size_t)tile_bytes(cap, brick_halo_cells(D, bd[0], bd[1], bd[2]));
+  std::vector<int32_t> tab;
+  brick_table(c, bd, tab);
   c->nbricks = (int64_t)tab.size() / BRICK_WORDS;
   if (c->nbricks > ((int64_t)1 << 31) - 1) return fail(c, CBM_ERR_CONFIG, "too many cell bricks; raise cellEdge");
   HIPCHK(c, dalloc(&c->bricks, (int64_t)tab.size()));
   HIPCHK(c, hipMemcpy(c->bricks, tab.data(), sizeof(int32_t) * tab.size(), hipMemcpyHostToDevice));
+
+  // Sub-bricks for a cell-linked stage whose population can concentrate
+  // (rules add agents where they are): the largest size below m, and at most
+  // 4^D cells, whose tile holds 32 agents per cell of its halo.  Their edges
+  // divide the main edges and both are Morton-aligned, so each lies inside
+  // one main brick.
+  if (!build && c->rules && m > mmin) {
+    for (int m2 = std::min(m - 1, 2 * D); m2 >= mmin; --m2) {
+      int b2[3];
+      edges(m2, b2);
+      const int H2 = brick_halo_cells(D, b2[0], b2[1], b2[2]);
+      const int64_t capmax = (budget - tile_bytes(0, H2) - 16) / per;
+      const double want = 1.2 * 32.0 * H2 + 32.0;
+      if (want > (double)capmax) continue;
+      for (int a = 0; a < 3; ++a) c->bdim2[a] = b2[a];
+      c->brick_cap2 = (int)std::min<int64_t>(capmax, (int64_t)std::max(256.0, want));
+      c->brick_lds2 = (size_t)tile_bytes(c->brick_cap2, H2);
+      std::vector<int32_t> tab2;
+      brick_table(c, b2, tab2);
+      c->nbricks2 = (int64_t)tab2.size() / BRICK_WORDS;
+      if (c->nbricks2 > ((int64_t)1 << 31) - 1) return fail(c, CBM_ERR_CONFIG, "too many cell bricks; raise cellEdge");
+      HIPCHK(c, dalloc(&c->bricks2, (int64_t)tab2.size()));
+      HIPCHK(c, hipMemcpy(c->bricks2, tab2.data(), sizeof(int32_t) * tab2.size(), hipMemcpyHostToDevice));
+      break;
+    }
+  }
   return CBM_OK;
 }
 
@@ -616,7 +658,7 @@ static void destroy_ctx(cbm_ctx* c) {
   F(c->med_init); F(c->med_tmp); F(c->adi_tab[0]); F(c->adi_tab[1]); F(c->adi_tab[2]);
   F(c->key); F(c->lin); F(c->gbucket); F(c->cell_range); F(c->count); F(c->cursor); F(c->start); F(c->tmp); F(c->cell_row);
   F(c->emax); F(c->wlist); F(c->wlist_n); F(c->fix_rows); F(c->fix_n);
-  F(c->scan_status); F(c->scan_ticket); F(c->list); F(c->list_n); F(c->bricks); F(c->brick_skip); F(c->blk_flag); F(c->flags); F(c->counters);
+  F(c->scan_status); F(c->scan_ticket); F(c->list); F(c->list_n); F(c->bricks); F(c->bricks2); F(c->brick_skip); F(c->blk_flag); F(c->flags); F(c->counters);
   for (cbm_ctx::DevBuf* b : {&c->adj_tgt, &c->adj_tbase, &c->adj_invn, &c->adj_ck, &c->adj_seg, &c->adj_lam,
                              &c->adj_gpart, &c->adj_lpart}) F(b->p);
   for (int32_t* p : c->snap_l) F(p);
@@ -1019,6 +1061,9 @@ static StageArgs stage_base(cbm_ctx* c) {
   a.brick_over = (unsigned long long*)(c->counters + 2);
   for (int q = 0; q < 3; ++q) a.bdim[q] = c->bdim[q];
   a.brick_cap = c->brick_cap;
+  a.brick2 = c->bricks2; a.nbricks2 = c->nbricks2; a.brick_lds2 = (int64_t)c->brick_lds2;
+  for (int q = 0; q < 3; ++q) a.bdim2[q] = c->bdim2[q];
+  a.brick_cap2 = c->brick_cap2;
   // flattened list stages: lists of at most FLAT_MAX_CAP entries
   {
     const int lcap = (c->verlet || c->clv) ? c->cap : (c->wl ? c->wcap : 0);
@@ -2030,6 +2075,15 @@ int cbm_get_bricks(cbm_ctx* c, int32_t* bdim, int32_t* cap, int64_t* nbricks, in
   *cap = tiled ? c->brick_cap : 0;
   *nbricks = tiled ? c->nbricks : 0;
   *overflows = v[2];
+  return CBM_OK;
+}
+
+int cbm_get_sub_bricks(cbm_ctx* c, int32_t* bdim, int32_t* cap, int64_t* nbricks) {
+  if (!c || !bdim || !cap || !nbricks) return CBM_ERR_ARG;
+  const bool sub = c->bricks2 != nullptr;
+  for (int a = 0; a < 3; ++a) bdim[a] = sub ? c->bdim2[a] : 0;
+  *cap = sub ? c->brick_cap2 : 0;
+  *nbricks = sub ? c->nbricks2 : 0;
   return CBM_OK;
 }
 

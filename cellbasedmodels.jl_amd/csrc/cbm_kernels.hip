@@ -1204,8 +1204,26 @@ static hipError_t launch_stage_law(int law, const StageLaunch& L, const StageArg
       case LAW_SOFT_SPHERES: hipLaunchKernelGGL((k_brick<D, LAW_SOFT_SPHERES, PER, NOISE>), L.bgrid, L.block, L.lds, L.s, a); break;
       default: return hipErrorInvalidValue;
     }
-    const hipError_t e = hipGetLastError();
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (a.brick2 && a.nbricks2 > 0) {
+      // the sub-bricks of the bricks that did not fit
+      StageArgs b = a;
+      b.brick = a.brick2;
+      for (int q = 0; q < 3; ++q) b.bdim[q] = a.bdim2[q];
+      b.brick_cap = a.brick_cap2;
+      b.brick_sub = 1;
+      const dim3 g2((unsigned)a.nbricks2);
+      const size_t lds2 = (size_t)a.brick_lds2;
+      switch (law) {
+        case LAW_LINEAR: hipLaunchKernelGGL((k_brick<D, LAW_LINEAR, PER, NOISE>), g2, L.block, lds2, L.s, b); break;
+        case LAW_REP_ADH: hipLaunchKernelGGL((k_brick<D, LAW_REP_ADH, PER, NOISE>), g2, L.block, lds2, L.s, b); break;
+        case LAW_AGGREGATION: hipLaunchKernelGGL((k_brick<D, LAW_AGGREGATION, PER, NOISE>), g2, L.block, lds2, L.s, b); break;
+        default: hipLaunchKernelGGL((k_brick<D, LAW_SOFT_SPHERES, PER, NOISE>), g2, L.block, lds2, L.s, b); break;
+      }
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
   }
   switch (law) {
     case LAW_LINEAR: hipLaunchKernelGGL((k_stage<D, NB, LAW_LINEAR, PER, NOISE>), L.grid, L.block, 0, L.s, a); break;

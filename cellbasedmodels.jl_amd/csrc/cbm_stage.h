@@ -845,6 +845,9 @@ __device__ __forceinline__ int brick_stage(const BrickSrc& src, int b, BrickTile
   t.r0 = src.bucket_start[bt[4]];
   t.r1 = src.bucket_start[bt[5]];
   if (t.r0 >= t.r1) return 0;
+  // a sub-brick whose main brick was staged: nothing left here (a main
+  // brick marks all its rows or none, and this brick lies inside one)
+  if (src.sub && !src.brick_skip[t.r0]) return 0;
   t.B[0] = src.bdim[0]; t.B[1] = D > 1 ? src.bdim[1] : 1; t.B[2] = D > 2 ? src.bdim[2] : 1;
   t.HX = t.B[0] + 2;
   t.HY = D > 1 ? t.B[1] + 2 : 1;
@@ -929,7 +932,7 @@ __device__ __forceinline__ int brick_stage(const BrickSrc& src, int b, BrickTile
   if (t.T > src.cap) {
     // too dense for LDS: the row-parallel kernel launched after this one
     // takes the brick's rows
-    if (tid == 0) atomicAdd(src.brick_over, 1ull);
+    if (tid == 0 && !src.sub) atomicAdd(src.brick_over, 1ull);
     for (int64_t i = (int64_t)r0 + tid; i < (int64_t)r1; i += nt) src.brick_skip[i] = 1;
     // wake the row-kernel blocks holding these rows (same block size)
     const int64_t q0 = src.block_off[t.sim] + ((int64_t)r0 - sd.row_off) / nt;
@@ -1053,6 +1056,7 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
   src.cell_range = a.cell_range; src.cap = cap;
   for (int e = 0; e < 3; ++e) src.bdim[e] = a.bdim[e];
   src.brick_skip = a.brick_skip; src.blk_flag = a.blk_flag; src.brick_over = a.brick_over;
+  src.sub = a.brick_sub;
   const int H = brick_halo_cells(D, a.bdim[0], D > 1 ? a.bdim[1] : 1, D > 2 ? a.bdim[2] : 1);
 
   BrickTile t;
@@ -1117,6 +1121,7 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
     const int32_t li = home_li[k];
     const int h = owner[li] & 0x7FFF;
     const int64_t i = (int64_t)hrow[h] + (li - hs[h]);
+    if (a.brick_sub) a.brick_skip[i] = 0;   // done here, not by the row kernel
     if (i - sd.row_off >= N) continue;
     double xi[3] = {0.0, 0.0, 0.0};
     double acc[3] = {0.0, 0.0, 0.0};
@@ -1138,7 +1143,9 @@ __device__ __forceinline__ void brick_body(const StageArgs& a, unsigned char* ld
         pair_into<D, Law, PER, SOFT>(law, xi, xj, L, hL, rad_i, SOFT ? rad[q] : 0.0, fate_i,
                                      SOFT ? (int64_t)fate[q] : 0, live, acc);
       };
-      if constexpr (FILTER) {
+      // a sub-brick walks every candidate once: its dense cells would
+      // overflow the per-lane queue of kept candidates and walk twice
+      if (FILTER && !a.brick_sub) {
         auto near = [&](int32_t q) {
           double dx[3];
 #pragma unroll

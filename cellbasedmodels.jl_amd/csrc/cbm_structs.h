@@ -123,6 +123,7 @@ struct BrickSrc {
   uint8_t* brick_skip;            // per row: left to the row-parallel kernel
   uint8_t* blk_flag;              // per row-kernel block: holds such rows
   unsigned long long* brick_over; // count of such bricks
+  int32_t sub;                    // sub-brick pass: only bricks whose rows an earlier pass left over
 };
 
 // LDS of one Verlet-build brick block: the header, D position arrays of cap
@@ -201,6 +202,18 @@ struct StageArgs {
   int32_t kind, periodic, lagged, walls, nvar, disp_check, write_fnew, pad;
   int32_t bdim[3];              // brick edge in cells per axis
   int32_t brick_cap;            // agents a brick block can stage in LDS
+  // Sub-bricks (cell-linked contexts whose population can concentrate): a
+  // finer brick table, each of whose bricks lies inside one brick of the
+  // main table.  After the main brick launch a second launch over it (its
+  // copy of these arguments has brick_sub = 1) takes the rows of the main
+  // bricks too dense for LDS; what it cannot stage either is left to the
+  // row-parallel kernel.  brick2 is NULL when there is no such table.
+  const int32_t* brick2;
+  int64_t nbricks2;
+  int64_t brick_lds2;
+  int32_t bdim2[3];
+  int32_t brick_cap2;
+  int32_t brick_sub;
   int32_t wcap;                 // entries per agent of wlist
   int32_t gate_run;
   int32_t fchunk;               // flattened list stage: pairs per LDS round

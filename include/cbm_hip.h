@@ -435,6 +435,12 @@ int cbm_get_rebuilds(cbm_ctx* ctx, int64_t* rebuilds);
  * results, slower).  Zeros for Full. */
 int cbm_get_bricks(cbm_ctx* ctx, int32_t* bdim, int32_t* cap, int64_t* nbricks,
                    int64_t* overflows);
+/* The sub-brick table of a CellLinked context with rules (no reference
+ * counterpart; diagnostics as cbm_get_bricks): a finer tiling whose bricks
+ * each lie inside one main brick, launched after the main bricks to stage
+ * the rows of those too dense for LDS, each sub-brick sized for 32 agents
+ * per cell.  Zeros when the context has none. */
+int cbm_get_sub_bricks(cbm_ctx* ctx, int32_t* bdim, int32_t* cap, int64_t* nbricks);
 /* Neighbor lists of simulation `sim` in logical slots: counts[n] and
  * lists[n*cap] (row i holds the slots of agent i's list, unordered).  For
  * CellLinked/Full the "list" is the candidate set within `radius` of the
