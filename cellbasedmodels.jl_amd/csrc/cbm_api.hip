@@ -369,6 +369,9 @@ static void morton_ranks(const int32_t* gn, int dims, std::vector<int32_t>& out)
   for (int64_t r = 0; r < nc; ++r) out[base + (size_t)key[(size_t)r].second] = (int32_t)r;
 }
 
+// Agents per halo cell a sub-brick tile is sized for.
+constexpr double SUB_BRICK_DENSITY = 48.0;
+
 // The brick table for edges bd: per simulation, its bricks in Morton order
 // of their coordinates, BRICK_WORDS each (simulation, origin cell, bucket
 // range of its cells).
@@ -468,7 +471,7 @@ static int setup_bricks(cbm_ctx* c) {
 
   // Sub-bricks for a cell-linked stage whose population can concentrate
   // (rules add agents where they are): the largest size below m, and at most
-  // 4^D cells, whose tile holds 32 agents per cell of its halo.  Their edges
+  // 4^D cells, whose tile holds SUB_BRICK_DENSITY agents per cell of its halo.  Their edges
   // divide the main edges and both are Morton-aligned, so each lies inside
   // one main brick.
   if (!build && c->rules && m > mmin) {
@@ -477,7 +480,7 @@ static int setup_bricks(cbm_ctx* c) {
       edges(m2, b2);
       const int H2 = brick_halo_cells(D, b2[0], b2[1], b2[2]);
       const int64_t capmax = (budget - tile_bytes(0, H2) - 16) / per;
-      const double want = 1.2 * 32.0 * H2 + 32.0;
+      const double want = 1.2 * SUB_BRICK_DENSITY * H2 + 32.0;
       if (want > (double)capmax) continue;
       for (int a = 0; a < 3; ++a) c->bdim2[a] = b2[a];
       c->brick_cap2 = (int)std::min<int64_t>(capmax, (int64_t)std::max(256.0, want));

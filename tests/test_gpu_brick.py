@@ -222,17 +222,17 @@ def test_cell_linked_two_stage_lists(gpu_available, integ, dt, walk):
 @pytest.mark.parametrize("law", [A.LAW_LINEAR, A.LAW_REP_ADH], ids=["linear", "rep_adh"])
 def test_dense_core_runs_on_sub_bricks_bit_exact(gpu_available, law):
     # With rules the main bricks are sized for the capacity's mean density; a
-    # core of ~30 agents per cell (edge 2; ~43 at edge 2.4) overflows them,
-    # and the sub-bricks (sized for 32 per cell) stage it, or at edge 2.4
-    # leave their own overflow to the row kernel.  A death rule at rate 0
-    # keeps the population.
+    # core of ~30 agents per cell (edge 2) overflows them and the sub-bricks
+    # (sized for 48 per cell) stage it; a core of ~64 per cell (edge 2.4)
+    # overflows some sub-bricks too, whose rows the row kernel takes.  A death
+    # rule at rate 0 keeps the population.
     rng = np.random.default_rng(5)
     n, L, core = 6000, 200.0, 4000
     box = [[0.0, L]] * 2
     pos = [rng.uniform(0.0, L, n) for _ in range(2)]
+    p, edge, half = ([1.0, 2.0], 2.0, 11.5) if law == A.LAW_LINEAR else (REP_ADH, 2.4, 9.5)
     for a in range(2):
-        pos[a][:core] = rng.uniform(88.5, 111.5, core)
-    p, edge = ([1.0, 2.0], 2.0) if law == A.LAW_LINEAR else (REP_ADH, 2.4)
+        pos[a][:core] = rng.uniform(100.0 - half, 100.0 + half, core)
     m = model(2, integrator=A.EULER, neighbors=A.CELL_LINKED, law=law, dt=1e-3, cell_edge=edge,
               rule=A.RULE_DEATH, rule_p=(0.0, 0, 0, 0), seed=3)
     m.walk = 1
